@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free heat Jacobian on one MI355X.
+
+Workload (the largest single-GPU configuration of the reference path): FE_Q3 on a 64^3 hex mesh
+of the unit cube (7,189,057 DoFs), nonlinear heat Jacobian with k(T) = 1 + 0.5 tanh((T-1700)/50)
+linearised at T = 300 + 1500 exp(-|x-x0|^2/(2*0.05^2)), x0 = (0.5,0.5,1), times (1 + 0.01 U),
+dt = 1e-4, homogeneous Dirichlet on all faces.  A "step" is one FP64 operator application
+dst = J src (dst zeroed first), inputs resident in HBM.  Also reported: Jacobi-CG time to a
+1e-10 relative residual in plain FP64 and FP32-inner/FP64-refined form, the roofline fraction of
+the cell kernel, and the CPU oracle's throughput on the host cores (a reported baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_case(mf, torch, degree=3, ncell=64):
+    s = mf.ScratchData(degree, ncell, stream=torch.cuda.current_stream())
+    mask = s.boundary_mask()
+    s.set_dirichlet_mask(mask)
+    X = s.support_points()
+    r2 = ((X - np.array([0.5, 0.5, 1.0])) ** 2).sum(axis=1)
+    T = 300.0 + 1500.0 * np.exp(-r2 / (2 * 0.05 ** 2))
+    T = T * (1.0 + 0.01 * s.random_field(2))
+    op = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                                tanh_Tc=1700.0, tanh_width=50.0, t_dependent=True, dt=1e-4)
+    T_dev = torch.tensor(T, dtype=torch.float64, device="cuda")
+    op.set_fields(T=T_dev)
+    b = s.random_field(3)
+    b[mask == 1] = 0.0
+    return s, op, T, T_dev, mask, b
+
+
+def flops_per_cell(K):
+    """Algorithmic FLOP per cell of the collocation kernel: evaluate u (3 interpolation + 3
+    derivative sweeps), evaluate T (same), integrate (6 sweeps); each sweep 2 K^4 flops; plus
+    ~40 flops per quadrature point for the k(T) law and the flux."""
+    return 18 * 2 * K ** 4 + 40 * K ** 3
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding "
+              "of this path is a design, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import meltpooldg_amd as mf
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    s, op, T, T_dev, mask, b = build_case(mf, torch)
+    n = s.n_dofs
+    src = torch.tensor(b, dtype=torch.float64, device="cuda")
+    dst = torch.zeros(n, dtype=torch.float64, device="cuda")
+    print(f"# workload: Q3 64^3, n_dofs={n}, n_cells={s.n_cells}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        op.vmult(dst, src)
+    torch.cuda.synchronize()
+    # timed region: K full vmults (zero + cell kernel), HIP events on the library's stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(args.steps):
+        op.vmult(dst, src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = e0.elapsed_time(e1) / args.steps
+    value = n / (ms_per_step * 1e-3)
+
+    # the cell kernel alone (vmult_add: no zeroing) for the roofline
+    k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    k0.record(stream)
+    for _ in range(args.steps):
+        op.vmult_add(dst, src)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kernel_ms = k0.elapsed_time(k1) / args.steps
+    alg_bytes = 3 * n * 8  # src read, dst write, T read: each DoF vector once
+    achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    K = 4
+    alg_flop = flops_per_cell(K) * s.n_cells
+    achieved_tflops = alg_flop / (kernel_ms * 1e-3) / 1e12
+    print(f"# vmult {ms_per_step*1e3:.1f} us/step, cell kernel {kernel_ms*1e3:.1f} us, "
+          f"{achieved_gbs:.0f} GB/s, {achieved_tflops:.2f} TFLOP/s", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_solve:
+        # Jacobi-CG to 1e-10: plain FP64 and FP32-inner / FP64-refined
+        x64 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        mf.LinearSolver.solve(op, x64, src, solver="CG", rel_tolerance=1e-10)  # warm-up
+        x64.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep64 = mf.LinearSolver.solve(op, x64, src, solver="CG", rel_tolerance=1e-10)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        x32 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        mf.LinearSolver.solve(op, x32, src, solver="CG", rel_tolerance=1e-10,
+                              precision="fp32_refined")
+        x32.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep32 = mf.LinearSolver.solve(op, x32, src, solver="CG", rel_tolerance=1e-10,
+                                      precision="fp32_refined", inner_rel_tolerance=1e-3)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t0
+        err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+        extra["solve"] = {
+            "cg_fp64_s": t64, "cg_fp64_iterations": rep64["iterations"],
+            "cg_fp64_rel_residual": rep64["final_residual"] / rep64["initial_residual"],
+            "cg_fp32_refined_s": t32, "cg_fp32_refined_inner_iterations": rep32["iterations"],
+            "cg_fp32_refined_outer": rep32["outer_iterations"],
+            "cg_fp32_refined_rel_residual": rep32["final_residual"] / rep32["initial_residual"],
+            "refined_vs_fp64_rel_error": err, "speedup_refined_over_fp64": t64 / t32}
+        print(f"# CG fp64 {t64*1e3:.1f} ms ({rep64['iterations']} it), refined {t32*1e3:.1f} ms "
+              f"({rep32['outer_iterations']} outer/{rep32['iterations']} inner), err {err:.2e}",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(T, mask, b, n, args.cpu_budget)
+
+    line = {
+        "metric": "DoFs/s per operator vmult; time-to-1e-10 solve FP64 vs FP32-refined; %HBM peak",
+        "value": value,
+        "unit": "DoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 lattice fields, analytic linearisation point)",
+        "config": {"workload": "FP64 heat Jacobian vmult, FE_Q3/QGauss(4), 64^3 cells of [0,1]^3 "
+                               "(7,189,057 DoFs), k(T)=1+0.5tanh((T-1700)/50), dt=1e-4, "
+                               "homogeneous Dirichlet",
+                   "n_dofs": n, "n_cells": s.n_cells, "degree": 3},
+        "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_us": kernel_ms * 1e3, "alg_bytes_per_launch": alg_bytes,
+                     "fp64_tflops": achieved_tflops,
+                     "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(T, mask, b, n, budget):
+    """The CPU oracle's sum-factorised vmult on the same 64^3 Q3 workload, OpenMP over the host
+    cores, timed for a bounded number of applications."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    ncores = len(os.sched_getaffinity(0))
+    nthreads = min(ncores, 16)
+    m = orc.make_mesh(3, 3, 64)
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=1700.0, tanh_width=50.0, dt=1e-4)
+    op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, mask=mask, T=T, T_old=T,
+                            nthreads=nthreads)
+    t0 = time.perf_counter()
+    op.vmult(b)
+    first = time.perf_counter() - t0
+    reps = max(1, min(20, int(budget / max(first, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.vmult(b)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": n / dt, "unit": "DoF/s", "cores": nthreads, "kind": "port",
+            "sample": f"{reps} FP64 vmults of the full 64^3 Q3 Jacobian (oracle, OpenMP, "
+                      f"{nthreads} threads), {dt*1e3:.0f} ms each"}
+
+
+if __name__ == "__main__":
+    main()

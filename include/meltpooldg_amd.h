@@ -1,0 +1,168 @@
+/*
+ * meltpooldg_amd.h -- C ABI of the MI355X (gfx950) matrix-free operator library.
+ *
+ * The library replaces the deal.II MatrixFree/FEEvaluation cell loop behind MeltPoolDG's
+ * heat-transfer Jacobian and level-set advection-diffusion operators, plus the Jacobi
+ * preconditioned CG/GMRES solves that call them.  Every entry point names the reference
+ * interface it stands in for.  All vector arguments are DEVICE pointers (hipMalloc'ed or torch
+ * device tensors) of n_dofs entries in the library's lexicographic node numbering; work is
+ * enqueued on the stream set with mf_set_stream (default: the null stream).  Functions return
+ * an MF_* status; mf_last_error() gives the message of the last failure on the calling thread.
+ * Contexts are not re-entrant (as the reference operators, which mutate caches in const
+ * methods): one thread and one stream per context.
+ */
+#ifndef MELTPOOLDG_AMD_H
+#define MELTPOOLDG_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+enum {
+  MF_OK = 0,
+  MF_ERR_ARG = 1,          /* invalid argument (reference: AssertThrow / ExcMessage) */
+  MF_ERR_HIP = 2,          /* HIP runtime error */
+  MF_NO_CONVERGENCE = 3,   /* reference: SolverControl::NoConvergence */
+  MF_BREAKDOWN = 4,        /* Krylov breakdown (zero or NaN denominator) */
+  MF_ERR_STATE = 5         /* call order violated (e.g. vmult before set_linearization) */
+};
+
+const char *mf_last_error(void);
+int mf_version(void);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Mesh / DoF / constraint setup.  Replaces ScratchData::build + MatrixFree::reinit and the     */
+/* DoF-index/constraint setup (source/core/scratch_data.cpp:194-293,                            */
+/* include/meltpooldg/utilities/constraints.hpp:254-269, include/meltpooldg/utilities/          */
+/* fe_util.hpp:21-81).  The mesh is a structured hex mesh of [lo,hi] with ncell cells per       */
+/* direction, FE_Q(degree) on Gauss-Lobatto points and QGauss(degree+1); DoFs are numbered      */
+/* lexicographically on the global node lattice (x fastest).                                    */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+  int dim;          /* must be 3 */
+  int degree;       /* 1, 2 or 3 */
+  int ncell[3];
+  double lo[3];
+  double hi[3];
+} mf_mesh_desc;
+
+typedef struct mf_scratch mf_scratch;
+
+int mf_scratch_create(const mf_mesh_desc *desc, mf_scratch **out);
+int mf_scratch_destroy(mf_scratch *s);
+int64_t mf_scratch_n_dofs(const mf_scratch *s);
+int64_t mf_scratch_n_cells(const mf_scratch *s);
+/* Stream on which all work of this scratch and its operators is enqueued (hipStream_t). */
+int mf_set_stream(mf_scratch *s, void *hip_stream);
+/* Dirichlet constraints (the heat_dof_idx / advec_diff_dof_idx constraint set): a host array of
+ * n_dofs bytes, 1 = constrained.  NULL clears them.  Replaces
+ * Constraints::make_DBC_and_HNC_plus_PBC_and_merge_HNC_plus_PBC_into_DBC (constraints.hpp:254-269)
+ * for meshes without hanging nodes. */
+int mf_scratch_set_dirichlet_mask(mf_scratch *s, const uint8_t *mask_host);
+/* 1D tables the kernels use (K = degree+1): S[K][K] = l_j(x_q), DS[K][K] = l_j'(x_q),
+ * Dq[K][K] collocation derivative on the Gauss points, w[K] Gauss weights, gl[K] support points */
+int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq, double *w,
+                          double *gl);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Operators.  Replaces OperatorMatrixFree<dim,double> (include/meltpooldg/core/              */
+/* operator_base.hpp:189-283) for HeatDiffuseMultiPhaseOperator (source/heat/                   */
+/* heat_diffuse_operator.cpp) and AdvectionDiffusionOperator (source/level_set/                 */
+/* advection_diffusion_operator.cpp).                                                           */
+/* ------------------------------------------------------------------------------------------ */
+enum { MF_LAW_CONSTANT = 0, MF_LAW_TANH = 1 };
+
+typedef struct {
+  int law;            /* MF_LAW_CONSTANT or MF_LAW_TANH */
+  int t_dependent;    /* include the dk/dT Jacobian term (reference do_solidification) */
+  double rho_cp;      /* volumetric heat capacity (constant) */
+  double k;           /* constant conductivity, or base value of the tanh law */
+  double tanh_amp;    /* k(T) = k + tanh_amp * tanh((T - tanh_Tc) / tanh_width) */
+  double tanh_Tc;
+  double tanh_width;
+} mf_heat_params;
+
+typedef struct {
+  double theta;        /* 1 implicit Euler, 0.5 Crank-Nicolson */
+  double diffusivity;
+  double mass_factor;  /* 1 for theta schemes, 1.5 for BDF2 */
+} mf_advec_params;
+
+typedef struct mf_operator mf_operator;
+
+int mf_heat_operator_create(mf_scratch *s, const mf_heat_params *p, mf_operator **out);
+int mf_advection_operator_create(mf_scratch *s, const mf_advec_params *p, mf_operator **out);
+int mf_operator_destroy(mf_operator *op);
+
+/* OperatorBase::reset_time_increment (operator_base.hpp:31-46) */
+int mf_operator_reset_time_increment(mf_operator *op, double dt);
+/* OperatorBase::reset_dof_index: 0 = constrained index (Dirichlet rows skipped, entries read as
+ * zero), 1 = no-BC index (used for the Dirichlet lifting, utilities_matrixfree.hpp:33-100) */
+int mf_operator_reset_dof_index(mf_operator *op, int dof_index);
+/* Linearisation / coefficient fields (device pointers, n_dofs doubles each, kept by reference
+ * like the reference operator keeps const references to its vectors; NULL = absent):
+ * T = temperature (heat) , vel = nodal velocity, component-major [3][n_dofs],
+ * k_cell = per-cell conductivity factor [n_cells].  Replaces the vectors wired in at
+ * heat_diffuse_operation.cpp:55-70 and the pre() ghost update (heat_diffuse_operator.cpp:137-157).
+ * Must be called again whenever the content of T changes (the FP32 copy is refreshed here). */
+int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
+                           const double *k_cell);
+
+/* OperatorMatrixFree::vmult (operator_base.hpp:239-243; heat_diffuse_operator.cpp:179-197,
+ * advection_diffusion_operator.cpp:236-272): dst = A src, dst zeroed first, constrained rows 0 */
+int mf_operator_vmult(mf_operator *op, double *dst, const double *src);
+/* dst += A src without zeroing dst first (deal.II MatrixFree::loop with zero_dst = false, the
+ * mode the reference uses for create_rhs, heat_diffuse_operator.cpp:801) */
+int mf_operator_vmult_add(mf_operator *op, double *dst, const double *src);
+/* FP32 instance of the same operator (no reference counterpart: new precision form) */
+int mf_operator_vmult_f32(mf_operator *op, float *dst, const float *src);
+/* OperatorMatrixFree::compute_inverse_diagonal_from_matrixfree (operator_base.hpp:268-272;
+ * heat_diffuse_operator.cpp:271-284, advection_diffusion_operator.cpp:399-446) */
+int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
+/* OperatorMatrixFree::create_rhs (operator_base.hpp:208-212; heat_diffuse_operator.cpp:783-806,
+ * advection_diffusion_operator.cpp:274-358): ADDS into dst.  Heat: src = T_old, q_s = heat
+ * source (may be NULL), result -R(T).  Advection: src = old field, q_s must be NULL. */
+int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Solvers.  Replaces LinearSolver::solve (include/meltpooldg/linear_algebra/                   */
+/* linear_solver.hpp:22-134) with JacobiPreconditioner (preconditioner_jacobi.hpp:29-89):       */
+/* ReductionControl(max_it, abs_tol, rel_tol), success when |r| <= max(abs_tol, rel_tol |r_0|). */
+/* ------------------------------------------------------------------------------------------ */
+enum { MF_SOLVER_CG = 0, MF_SOLVER_GMRES = 1 };
+enum { MF_PRECOND_IDENTITY = 0, MF_PRECOND_JACOBI = 1 };
+enum { MF_PRECISION_FP64 = 0, MF_PRECISION_FP32_REFINED = 1 };
+
+typedef struct {
+  int solver;         /* MF_SOLVER_CG or MF_SOLVER_GMRES (right preconditioned) */
+  int restart;        /* GMRES basis size (0 = 30) */
+  int max_iterations; /* 0 = 10000 */
+  double abs_tol;
+  double rel_tol;
+  int preconditioner; /* MF_PRECOND_* */
+  int precision;      /* MF_PRECISION_FP64 or MF_PRECISION_FP32_REFINED */
+  double inner_rel_tol; /* FP32-refined: relative tolerance of each inner FP32 solve (0 = 1e-3) */
+  int max_outer;      /* FP32-refined: maximum defect-correction passes (0 = 50) */
+} mf_solver_desc;
+
+typedef struct {
+  int iterations;        /* FP64: Krylov iterations; refined: total inner iterations */
+  int outer_iterations;  /* refined: defect-correction passes (FP64: 0) */
+  double initial_residual;
+  double final_residual; /* |b - A x| in FP64 */
+  int status;            /* MF_OK / MF_NO_CONVERGENCE / MF_BREAKDOWN */
+} mf_solve_report;
+
+/* Solve A x = b with x holding the initial guess (device pointers).  The Jacobi diagonal is
+ * recomputed from the operator (Preconditioner::update) when precond = JACOBI. */
+int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
+             mf_solve_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

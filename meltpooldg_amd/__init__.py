@@ -1,0 +1,336 @@
+"""MI355X matrix-free heat / level-set advection operators (host-side Python mirror).
+
+The compute path is ``libmeltpooldg_amd.so`` (hand-written HIP for gfx950) behind the C ABI in
+``include/meltpooldg_amd.h``.  This module binds that ABI with ctypes and mirrors the reference's
+operator interface (``OperatorMatrixFree``, include/meltpooldg/core/operator_base.hpp:189-283):
+``vmult``, ``compute_inverse_diagonal_from_matrixfree``, ``create_rhs``, ``reset_time_increment``,
+``reset_dof_index`` and ``LinearSolver.solve`` (include/meltpooldg/linear_algebra/
+linear_solver.hpp:22-134).  Vectors are torch device tensors (torch is used for device memory and
+streams only).  There is no CPU fallback: if the shared library is missing, importing the
+operators raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmeltpooldg_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "meltpooldg_amd.h")
+
+MF_OK, MF_ERR_ARG, MF_ERR_HIP, MF_NO_CONVERGENCE, MF_BREAKDOWN, MF_ERR_STATE = range(6)
+LAW_CONSTANT, LAW_TANH = 0, 1
+SOLVER_CG, SOLVER_GMRES = 0, 1
+PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
+PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
+
+
+class MFError(RuntimeError):
+    """Raised for a non-zero status of a C-ABI call (mirrors dealii::ExceptionBase)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class SolverNoConvergence(MFError):
+    """Mirrors dealii::SolverControl::NoConvergence."""
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("degree", C.c_int), ("ncell", C.c_int * 3),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3)]
+
+
+class HeatParams(C.Structure):
+    _fields_ = [("law", C.c_int), ("t_dependent", C.c_int), ("rho_cp", C.c_double),
+                ("k", C.c_double), ("tanh_amp", C.c_double), ("tanh_Tc", C.c_double),
+                ("tanh_width", C.c_double)]
+
+
+class AdvecParams(C.Structure):
+    _fields_ = [("theta", C.c_double), ("diffusivity", C.c_double), ("mass_factor", C.c_double)]
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [("solver", C.c_int), ("restart", C.c_int), ("max_iterations", C.c_int),
+                ("abs_tol", C.c_double), ("rel_tol", C.c_double), ("preconditioner", C.c_int),
+                ("precision", C.c_int), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int)]
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
+                ("initial_residual", C.c_double), ("final_residual", C.c_double),
+                ("status", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def build(force: bool = True) -> str:
+    """Compile libmeltpooldg_amd.so for gfx950 with hipcc (Makefile in csrc/)."""
+    args = ["make", "-s", "-j4", "-C", os.path.join(_HERE, "csrc")]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def header_functions():
+    """Names of the functions the C header declares (for the export check)."""
+    with open(HEADER_PATH) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mf_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load the HIP library (raises if it is not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run meltpooldg_amd.build() "
+                              "(the device path has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        vp, i64 = C.c_void_p, C.c_int64
+        L.mf_last_error.restype = C.c_char_p
+        L.mf_scratch_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
+        L.mf_scratch_destroy.argtypes = [vp]
+        L.mf_scratch_n_dofs.restype = i64
+        L.mf_scratch_n_dofs.argtypes = [vp]
+        L.mf_scratch_n_cells.restype = i64
+        L.mf_scratch_n_cells.argtypes = [vp]
+        L.mf_set_stream.argtypes = [vp, vp]
+        L.mf_scratch_set_dirichlet_mask.argtypes = [vp, vp]
+        L.mf_scratch_shape_info.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.mf_heat_operator_create.argtypes = [vp, C.POINTER(HeatParams), C.POINTER(vp)]
+        L.mf_advection_operator_create.argtypes = [vp, C.POINTER(AdvecParams), C.POINTER(vp)]
+        L.mf_operator_destroy.argtypes = [vp]
+        L.mf_operator_reset_time_increment.argtypes = [vp, C.c_double]
+        L.mf_operator_reset_dof_index.argtypes = [vp, C.c_int]
+        L.mf_operator_set_fields.argtypes = [vp, vp, vp, vp]
+        L.mf_operator_vmult.argtypes = [vp, vp, vp]
+        L.mf_operator_vmult_f32.argtypes = [vp, vp, vp]
+        L.mf_operator_vmult_add.argtypes = [vp, vp, vp]
+        L.mf_operator_compute_inverse_diagonal.argtypes = [vp, vp]
+        L.mf_operator_create_rhs.argtypes = [vp, vp, vp, vp]
+        L.mf_solve.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, C.POINTER(SolveReport)]
+        _lib = L
+    return _lib
+
+
+def splitmix64(x):
+    import numpy as np
+    with np.errstate(over="ignore"):
+        z = np.asarray(x, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def _check(code):
+    if code != MF_OK:
+        msg = lib().mf_last_error().decode()
+        if code == MF_NO_CONVERGENCE:
+            raise SolverNoConvergence(code, msg)
+        raise MFError(code, msg)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("vectors must be device tensors")
+    if not t.is_contiguous():
+        raise ValueError("vectors must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+class ScratchData:
+    """Mesh, DoF numbering, constraints and 1D tables (reference ScratchData + MatrixFree,
+    source/core/scratch_data.cpp:194-293) on a structured hex mesh of [lo, hi]."""
+
+    def __init__(self, degree, ncell, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), stream=None):
+        if isinstance(ncell, int):
+            ncell = (ncell,) * 3
+        self.degree = degree
+        self.ncell = tuple(ncell)
+        self.lo, self.hi = tuple(lo), tuple(hi)
+        d = MeshDesc(3, degree, (C.c_int * 3)(*ncell), (C.c_double * 3)(*lo),
+                     (C.c_double * 3)(*hi))
+        h = C.c_void_p()
+        _check(lib().mf_scratch_create(C.byref(d), C.byref(h)))
+        self._h = h
+        self.n_dofs = int(lib().mf_scratch_n_dofs(h))
+        self.n_cells = int(lib().mf_scratch_n_cells(h))
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream (its HIP handle is passed down) or a raw handle."""
+        handle = getattr(stream, "cuda_stream", stream)
+        _check(lib().mf_set_stream(self._h, C.c_void_p(handle)))
+
+    def set_dirichlet_mask(self, mask):
+        import numpy as np
+        if mask is None:
+            _check(lib().mf_scratch_set_dirichlet_mask(self._h, None))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        assert m.size == self.n_dofs
+        _check(lib().mf_scratch_set_dirichlet_mask(self._h, m.ctypes.data_as(C.c_void_p)))
+
+    def shape_info(self):
+        import numpy as np
+        K = self.degree + 1
+        S, DS, Dq, w, gl = (np.zeros((K, K)), np.zeros((K, K)), np.zeros((K, K)), np.zeros(K),
+                            np.zeros(K))
+        _check(lib().mf_scratch_shape_info(self._h, *(a.ctypes.data_as(C.c_void_p)
+                                                      for a in (S, DS, Dq, w, gl))))
+        return S, DS, Dq, w, gl
+
+    def support_points(self):
+        """(n_dofs, 3) coordinates of the DoF support points (Gauss-Lobatto lattice)."""
+        import numpy as np
+        gl = self.shape_info()[4]
+        p = self.degree
+        axes = []
+        for d in range(3):
+            h = (self.hi[d] - self.lo[d]) / self.ncell[d]
+            i = np.arange(self.ncell[d] * p + 1)
+            c = np.minimum(i // p, self.ncell[d] - 1)
+            axes.append(self.lo[d] + h * (c + gl[i - c * p]))
+        z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+        return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+
+    def random_field(self, seed):
+        """Seeded U(-1,1) per DoF keyed by the node's lattice coordinates (SplitMix64), the
+        synthetic-input convention shared with the test oracle."""
+        import numpy as np
+        nn = [self.ncell[d] * self.degree + 1 for d in range(3)]
+        i0, i1, i2 = np.meshgrid(np.arange(nn[0], dtype=np.uint64),
+                                 np.arange(nn[1], dtype=np.uint64),
+                                 np.arange(nn[2], dtype=np.uint64), indexing="xy")
+        # lexicographic x-fastest order: transpose meshgrid output to [z][y][x]
+        i0 = np.transpose(i0, (2, 0, 1)).ravel()
+        i1 = np.transpose(i1, (2, 0, 1)).ravel()
+        i2 = np.transpose(i2, (2, 0, 1)).ravel()
+        key = i0 | (i1 << np.uint64(21)) | (i2 << np.uint64(42))
+        z = splitmix64(key ^ splitmix64(np.uint64(seed)))
+        return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53) - 1.0
+
+    def boundary_mask(self):
+        import numpy as np
+        nn = [self.ncell[d] * self.degree + 1 for d in range(3)]
+        m = np.zeros((nn[2], nn[1], nn[0]), dtype=np.uint8)
+        m[0, :, :] = m[-1, :, :] = 1
+        m[:, 0, :] = m[:, -1, :] = 1
+        m[:, :, 0] = m[:, :, -1] = 1
+        return m.ravel()
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mf_scratch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class _OperatorBase:
+    """Common part of the OperatorMatrixFree mirror."""
+
+    def __init__(self, scratch, handle):
+        self.scratch = scratch
+        self._h = handle
+        self._fields = ()
+
+    def reset_time_increment(self, dt):
+        _check(lib().mf_operator_reset_time_increment(self._h, float(dt)))
+
+    def reset_dof_index(self, dof_index):
+        _check(lib().mf_operator_reset_dof_index(self._h, int(dof_index)))
+
+    def set_fields(self, T=None, velocity=None, k_cell=None):
+        # keep references alive: the library holds the pointers (as the reference holds const refs)
+        self._fields = (T, velocity, k_cell)
+        _check(lib().mf_operator_set_fields(self._h, _ptr(T), _ptr(velocity), _ptr(k_cell)))
+
+    def vmult(self, dst, src):
+        _check(lib().mf_operator_vmult(self._h, _ptr(dst), _ptr(src)))
+
+    def vmult_add(self, dst, src):
+        _check(lib().mf_operator_vmult_add(self._h, _ptr(dst), _ptr(src)))
+
+    def vmult_f32(self, dst, src):
+        _check(lib().mf_operator_vmult_f32(self._h, _ptr(dst), _ptr(src)))
+
+    def compute_inverse_diagonal_from_matrixfree(self, diag):
+        _check(lib().mf_operator_compute_inverse_diagonal(self._h, _ptr(diag)))
+
+    def create_rhs(self, dst, src, q_s=None):
+        _check(lib().mf_operator_create_rhs(self._h, _ptr(dst), _ptr(src), _ptr(q_s)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mf_operator_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class HeatDiffuseOperator(_OperatorBase):
+    """Heat Jacobian (reference HeatDiffuseMultiPhaseOperator, source/heat/
+    heat_diffuse_operator.cpp): tangent (rho cp/dt) M + K(k(T)) [+ dk/dT grad T u]."""
+
+    def __init__(self, scratch, law=LAW_CONSTANT, rho_cp=1.0, k=1.0, tanh_amp=0.0, tanh_Tc=0.0,
+                 tanh_width=1.0, t_dependent=False, dt=None):
+        p = HeatParams(law, int(bool(t_dependent)), rho_cp, k, tanh_amp, tanh_Tc, tanh_width)
+        h = C.c_void_p()
+        _check(lib().mf_heat_operator_create(scratch._h, C.byref(p), C.byref(h)))
+        super().__init__(scratch, h)
+        if dt is not None:
+            self.reset_time_increment(dt)
+
+
+class AdvectionDiffusionOperator(_OperatorBase):
+    """Level-set advection-diffusion tangent (reference AdvectionDiffusionOperator,
+    source/level_set/advection_diffusion_operator.cpp): (m/dt) M + theta (w . grad) + theta D K."""
+
+    def __init__(self, scratch, theta=1.0, diffusivity=0.0, mass_factor=1.0, dt=None):
+        p = AdvecParams(theta, diffusivity, mass_factor)
+        h = C.c_void_p()
+        _check(lib().mf_advection_operator_create(scratch._h, C.byref(p), C.byref(h)))
+        super().__init__(scratch, h)
+        if dt is not None:
+            self.reset_time_increment(dt)
+
+
+class LinearSolver:
+    """Mirror of MeltPoolDG::LinearSolver::solve (linear_solver.hpp:22-134)."""
+
+    @staticmethod
+    def solve(operator, solution, rhs, solver="CG", preconditioner="Diagonal",
+              max_iterations=10000, rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30,
+              precision="fp64", inner_rel_tolerance=1e-3, max_outer=50, raise_on_failure=True):
+        d = SolverDesc()
+        d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
+        d.restart = restart
+        d.max_iterations = max_iterations
+        d.abs_tol = abs_tolerance
+        d.rel_tol = rel_tolerance
+        d.preconditioner = {"Identity": PRECOND_IDENTITY, "Diagonal": PRECOND_JACOBI}[preconditioner]
+        d.precision = {"fp64": PRECISION_FP64, "fp32_refined": PRECISION_FP32_REFINED}[precision]
+        d.inner_rel_tol = inner_rel_tolerance
+        d.max_outer = max_outer
+        rep = SolveReport()
+        code = lib().mf_solve(operator._h, C.byref(d), _ptr(solution), _ptr(rhs), C.byref(rep))
+        if code != MF_OK and (raise_on_failure or code not in (MF_NO_CONVERGENCE, MF_BREAKDOWN)):
+            _check(code)
+        return rep.as_dict()

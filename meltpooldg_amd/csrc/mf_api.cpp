@@ -1,0 +1,702 @@
+// C ABI of libmeltpooldg_amd: scratch (mesh/DoF/constraint setup), operators, solvers.
+// Host-side control only; all arithmetic on vectors runs in the HIP kernels of mf_cell.hip and
+// mf_vec.hip.  See include/meltpooldg_amd.h for the reference interface of every entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mf_internal.hpp"
+#include "mf_vec.hpp"
+
+namespace mf {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------------------------------------
+// 1D tables: FE_Q(p) on Gauss-Lobatto points, QGauss(p+1)
+// (source/utilities/fe_util.cpp:18-72, source/core/finite_element_data.cpp:85-89)
+// ------------------------------------------------------------------------------------------
+static void legendre_pair(int n, double z, double &pn, double &pnm1)
+{
+  double a = 1.0, b = 0.0;  // P_j, P_{j-1}
+  for (int j = 1; j <= n; ++j) {
+    const double c = ((2.0 * j - 1.0) * z * a - (j - 1.0) * b) / j;
+    b = a;
+    a = c;
+  }
+  pn = a;
+  pnm1 = b;
+}
+
+static void gauss_points(int n, double *x, double *w)
+{
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pn, pm, dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      legendre_pair(n, z, pn, pm);
+      dp = n * (z * pn - pm) / (z * z - 1.0);
+      const double dz = pn / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    legendre_pair(n, z, pn, pm);
+    dp = n * (z * pn - pm) / (z * z - 1.0);
+    x[n - 1 - i] = 0.5 * (1.0 + z);
+    w[n - 1 - i] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+static void lobatto_points(int n, double *x)
+{
+  const int p = n - 1;
+  x[0] = 0.0;
+  x[p] = 1.0;
+  for (int i = 1; i < p; ++i) {
+    double z = -std::cos(M_PI * i / p);
+    for (int it = 0; it < 100; ++it) {
+      double pn, pm;
+      legendre_pair(p, z, pn, pm);
+      const double d1 = p * (z * pn - pm) / (z * z - 1.0);
+      const double d2 = (2.0 * z * d1 - p * (p + 1.0) * pn) / (1.0 - z * z);
+      const double dz = d1 / d2;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    x[i] = 0.5 * (1.0 + z);
+  }
+  for (int i = 0; i < n / 2; ++i) {  // exact symmetry of the point set
+    const double a = 0.5 * (x[i] + 1.0 - x[p - i]);
+    x[i] = a;
+    x[p - i] = 1.0 - a;
+  }
+  if (n % 2) x[n / 2] = 0.5;
+}
+
+// values and derivatives of the Lagrange polynomials on `nodes` at x
+static void lagrange_eval(int n, const double *nodes, double x, double *v, double *d)
+{
+  for (int j = 0; j < n; ++j) {
+    double val = 1.0, der = 0.0;
+    for (int m = 0; m < n; ++m) {
+      if (m == j) continue;
+      const double inv = 1.0 / (nodes[j] - nodes[m]);
+      der = der * (x - nodes[m]) * inv + val * inv;
+      val *= (x - nodes[m]) * inv;
+    }
+    v[j] = val;
+    d[j] = der;
+  }
+}
+
+void compute_tables(int degree, Tables &t)
+{
+  const int K = degree + 1;
+  std::memset(&t, 0, sizeof(t));
+  lobatto_points(K, t.gl);
+  gauss_points(K, t.xq, t.w);
+  double v[KMAX], d[KMAX];
+  for (int q = 0; q < K; ++q) {
+    lagrange_eval(K, t.gl, t.xq[q], v, d);
+    for (int j = 0; j < K; ++j) {
+      t.S[q * K + j] = v[j];
+      t.DS[q * K + j] = d[j];
+    }
+    lagrange_eval(K, t.xq, t.xq[q], v, d);
+    for (int j = 0; j < K; ++j) t.Dq[q * K + j] = d[j];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// helpers
+// ------------------------------------------------------------------------------------------
+template <typename Number>
+static CellArgs<Number> make_args(const mf_operator *op)
+{
+  const mf_scratch *s = op->s;
+  const MeshInfo &m = s->mesh;
+  CellArgs<Number> a;
+  std::memset(&a, 0, sizeof(a));
+  const int K = m.K;
+  for (int i = 0; i < K * K; ++i) {
+    a.S[i] = Number(s->tab.S[i]);
+    a.DS[i] = Number(s->tab.DS[i]);
+    a.Dq[i] = Number(s->tab.Dq[i]);
+  }
+  for (int i = 0; i < K; ++i) a.w[i] = Number(s->tab.w[i]);
+  for (int d = 0; d < 3; ++d) {
+    a.ncell[d] = m.ncell[d];
+    a.h[d] = Number(m.h[d]);
+    a.inv_h[d] = Number(1.0 / m.h[d]);
+  }
+  a.NX = uint32_t(m.NN[0]);
+  a.NY = uint32_t(m.NN[1]);
+  a.n_dofs = uint32_t(m.n_dofs);
+  a.n_cells = int(m.n_cells);
+  a.mask = s->d_mask;
+  a.use_mask = op->dof_index == 0 ? 1 : 0;
+  a.dt_inv = Number(op->dt > 0 ? 1.0 / op->dt : 0.0);
+  if (op->kind == 0) {
+    a.rho_cp = Number(op->heat.rho_cp);
+    a.k0 = Number(op->heat.k);
+    a.tanh_amp = Number(op->heat.tanh_amp);
+    a.tanh_Tc = Number(op->heat.tanh_Tc);
+    a.inv_tanh_w = Number(op->heat.tanh_width != 0 ? 1.0 / op->heat.tanh_width : 0.0);
+    a.t_dependent = op->heat.t_dependent;
+  } else {
+    a.theta = Number(op->advec.theta);
+    a.diffusivity = Number(op->advec.diffusivity);
+    a.mass_factor = Number(op->advec.mass_factor);
+  }
+  if constexpr (sizeof(Number) == 8) {
+    a.T = reinterpret_cast<const Number *>(op->T);
+    a.vel = reinterpret_cast<const Number *>(op->vel);
+    a.k_cell = reinterpret_cast<const Number *>(op->k_cell);
+  } else {
+    a.T = reinterpret_cast<const Number *>(op->T ? op->T32 : nullptr);
+    a.vel = reinterpret_cast<const Number *>(op->vel ? op->vel32 : nullptr);
+    a.k_cell = reinterpret_cast<const Number *>(op->k_cell ? op->kc32 : nullptr);
+  }
+  return a;
+}
+
+static bool needs_T(const mf_operator *op) { return op->kind == 0 && op->heat.law == MF_LAW_TANH; }
+
+static int check_ready(const mf_operator *op)
+{
+  MF_REQUIRE(op && op->s, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
+             "time increment not set (reference AssertThrowZeroTimeIncrement)");
+  if (needs_T(op)) MF_REQUIRE(op->T, MF_ERR_STATE, "temperature-dependent law needs T: call mf_operator_set_fields");
+  if (op->kind == 1) MF_REQUIRE(op->vel, MF_ERR_STATE, "advection operator needs a velocity field");
+  return MF_OK;
+}
+
+template <typename Number>
+static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double *dot_partial,
+                         const int *stop, bool zero)
+{
+  const MeshInfo &m = op->s->mesh;
+  hipStream_t st = op->s->stream;
+  if (zero) {
+    int rc = vec_fill<Number>(dst, Number(0), m.n_dofs, st, stop);
+    if (rc) return rc;
+  }
+  CellArgs<Number> a = make_args<Number>(op);
+  a.dot_partial = dot_partial;
+  a.stop = stop;
+  const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+  return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *mf_last_error(void) { return g_last_error.c_str(); }
+
+int mf_version(void) { return 100; }
+
+int mf_scratch_create(const mf_mesh_desc *desc, mf_scratch **out)
+{
+  MF_REQUIRE(desc && out, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(desc->dim == 3, MF_ERR_ARG, "only dim = 3 is supported on the device");
+  MF_REQUIRE(desc->degree >= 1 && desc->degree <= 3, MF_ERR_ARG, "degree must be 1, 2 or 3");
+  auto *s = new mf_scratch();
+  MeshInfo &m = s->mesh;
+  m.degree = desc->degree;
+  m.K = desc->degree + 1;
+  m.n_dofs = 1;
+  m.n_cells = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (desc->ncell[d] < 1 || !(desc->hi[d] > desc->lo[d])) {
+      delete s;
+      set_error("invalid mesh extent");
+      return MF_ERR_ARG;
+    }
+    m.ncell[d] = desc->ncell[d];
+    m.NN[d] = int64_t(desc->ncell[d]) * desc->degree + 1;
+    m.lo[d] = desc->lo[d];
+    m.hi[d] = desc->hi[d];
+    m.h[d] = (desc->hi[d] - desc->lo[d]) / desc->ncell[d];
+    m.n_dofs *= m.NN[d];
+    m.n_cells *= m.ncell[d];
+  }
+  if (m.n_dofs >= (int64_t(1) << 31)) {
+    delete s;
+    set_error("mesh too large for 32-bit DoF indices");
+    return MF_ERR_ARG;
+  }
+  compute_tables(m.degree, s->tab);
+  *out = s;
+  return MF_OK;
+}
+
+int mf_scratch_destroy(mf_scratch *s)
+{
+  if (!s) return MF_OK;
+  if (s->d_mask) (void)hipFree(s->d_mask);
+  delete s;
+  return MF_OK;
+}
+
+int64_t mf_scratch_n_dofs(const mf_scratch *s) { return s ? s->mesh.n_dofs : -1; }
+int64_t mf_scratch_n_cells(const mf_scratch *s) { return s ? s->mesh.n_cells : -1; }
+
+int mf_set_stream(mf_scratch *s, void *stream)
+{
+  MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
+  s->stream = reinterpret_cast<hipStream_t>(stream);
+  return MF_OK;
+}
+
+int mf_scratch_set_dirichlet_mask(mf_scratch *s, const uint8_t *mask_host)
+{
+  MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
+  if (!mask_host) {
+    if (s->d_mask) MF_HIP_TRY(hipFree(s->d_mask));
+    s->d_mask = nullptr;
+    s->h_mask.clear();
+    return MF_OK;
+  }
+  const size_t n = size_t(s->mesh.n_dofs);
+  s->h_mask.assign(mask_host, mask_host + n);
+  if (!s->d_mask) MF_HIP_TRY(hipMalloc(&s->d_mask, n));
+  MF_HIP_TRY(hipMemcpy(s->d_mask, mask_host, n, hipMemcpyHostToDevice));
+  return MF_OK;
+}
+
+int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq, double *w,
+                          double *gl)
+{
+  MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
+  const int K = s->mesh.K;
+  if (S) std::memcpy(S, s->tab.S, sizeof(double) * K * K);
+  if (DS) std::memcpy(DS, s->tab.DS, sizeof(double) * K * K);
+  if (Dq) std::memcpy(Dq, s->tab.Dq, sizeof(double) * K * K);
+  if (w) std::memcpy(w, s->tab.w, sizeof(double) * K);
+  if (gl) std::memcpy(gl, s->tab.gl, sizeof(double) * K);
+  return MF_OK;
+}
+
+int mf_heat_operator_create(mf_scratch *s, const mf_heat_params *p, mf_operator **out)
+{
+  MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(p->law == MF_LAW_CONSTANT || p->law == MF_LAW_TANH, MF_ERR_ARG, "unknown law");
+  MF_REQUIRE(p->law != MF_LAW_TANH || p->tanh_width > 0.0, MF_ERR_ARG, "tanh_width must be > 0");
+  auto *op = new mf_operator();
+  op->kind = 0;
+  op->s = s;
+  op->heat = *p;
+  *out = op;
+  return MF_OK;
+}
+
+int mf_advection_operator_create(mf_scratch *s, const mf_advec_params *p, mf_operator **out)
+{
+  MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(p->diffusivity >= 0.0, MF_ERR_ARG,
+             "Advection diffusion operator: diffusivity is smaller than zero!");
+  auto *op = new mf_operator();
+  op->kind = 1;
+  op->s = s;
+  op->advec = *p;
+  *out = op;
+  return MF_OK;
+}
+
+int mf_operator_destroy(mf_operator *op)
+{
+  if (!op) return MF_OK;
+  if (op->T32) (void)hipFree(op->T32);
+  if (op->vel32) (void)hipFree(op->vel32);
+  if (op->kc32) (void)hipFree(op->kc32);
+  delete op;
+  return MF_OK;
+}
+
+int mf_operator_reset_time_increment(mf_operator *op, double dt)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(dt > 0.0, MF_ERR_ARG, "time increment must be positive");
+  op->dt = dt;
+  return MF_OK;
+}
+
+int mf_operator_reset_dof_index(mf_operator *op, int dof_index)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(dof_index == 0 || dof_index == 1, MF_ERR_ARG, "dof_index must be 0 or 1");
+  op->dof_index = dof_index;
+  return MF_OK;
+}
+
+int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
+                           const double *k_cell)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  const MeshInfo &m = op->s->mesh;
+  hipStream_t st = op->s->stream;
+  op->T = T;
+  op->vel = vel;
+  op->k_cell = k_cell;
+  if (T) {
+    if (!op->T32) MF_HIP_TRY(hipMalloc(&op->T32, sizeof(float) * m.n_dofs));
+    int rc = vec_cast<double, float>(T, op->T32, 1.0, m.n_dofs, st);
+    if (rc) return rc;
+  }
+  if (vel) {
+    if (!op->vel32) MF_HIP_TRY(hipMalloc(&op->vel32, sizeof(float) * 3 * m.n_dofs));
+    int rc = vec_cast<double, float>(vel, op->vel32, 1.0, 3 * m.n_dofs, st);
+    if (rc) return rc;
+  }
+  if (k_cell) {
+    if (!op->kc32) MF_HIP_TRY(hipMalloc(&op->kc32, sizeof(float) * m.n_cells));
+    int rc = vec_cast<double, float>(k_cell, op->kc32, 1.0, m.n_cells, st);
+    if (rc) return rc;
+  }
+  return MF_OK;
+}
+
+int mf_operator_vmult(mf_operator *op, double *dst, const double *src)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
+  return apply_tangent<double>(op, dst, src, nullptr, nullptr, true);
+}
+
+int mf_operator_vmult_add(mf_operator *op, double *dst, const double *src)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
+  return apply_tangent<double>(op, dst, src, nullptr, nullptr, false);
+}
+
+int mf_operator_vmult_f32(mf_operator *op, float *dst, const float *src)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
+  return apply_tangent<float>(op, dst, src, nullptr, nullptr, true);
+}
+
+int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(inv_diag, MF_ERR_ARG, "null vector");
+  const MeshInfo &m = op->s->mesh;
+  hipStream_t st = op->s->stream;
+  rc = vec_fill<double>(inv_diag, 0.0, m.n_dofs, st);
+  if (rc) return rc;
+  CellArgs<double> a = make_args<double>(op);
+  const int code = op->kind == 0 ? HEAT_DIAG : ADVEC_DIAG;
+  rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
+  if (rc) return rc;
+  double *work = nullptr;
+  if (op->kind == 1) MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&work), sizeof(double) * RED_BLOCKS, st));
+  rc = vec_invert_diagonal(inv_diag, m.n_dofs, op->kind == 1, work, st);
+  if (work) MF_HIP_TRY(hipFreeAsync(work, st));
+  return rc;
+}
+
+int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
+  if (op->kind == 0) MF_REQUIRE(op->T, MF_ERR_STATE, "heat create_rhs needs T (mf_operator_set_fields)");
+  MF_REQUIRE(op->kind == 0 || !q_s, MF_ERR_ARG, "advection create_rhs takes no source term");
+  CellArgs<double> a = make_args<double>(op);
+  a.aux1 = q_s;
+  const int code = op->kind == 0 ? HEAT_RHS : ADVEC_RHS;
+  return launch_cell<double>(code, needs_T(op), op->vel != nullptr, op->s->mesh, a, src, dst,
+                             op->s->stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// solvers
+// ------------------------------------------------------------------------------------------
+namespace mf {
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  hipStream_t st = nullptr;
+  int alloc(size_t n, hipStream_t s)
+  {
+    st = s;
+    MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&p), sizeof(T) * std::max<size_t>(n, 1), s));
+    return MF_OK;
+  }
+  ~DevBuf()
+  {
+    if (p) (void)hipFreeAsync(p, st);
+  }
+};
+
+struct SolveOut {
+  int iterations = 0;
+  double res0 = 0, res = 0;
+  int status = MF_OK;
+};
+
+// Preconditioned CG on the device; T = double (FP64 operator) or float (FP32 operator).
+template <typename T>
+static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it, double abs_tol,
+                    double rel_tol, SolveOut &out)
+{
+  const MeshInfo &m = op->s->mesh;
+  const int64_t n = m.n_dofs;
+  hipStream_t st = op->s->stream;
+  const int ncb = cell_blocks(m);
+  DevBuf<T> r, p, q, prr, prz0, prz1;
+  DevBuf<double> pcell;
+  DevBuf<SolveState> sd;
+  int rc;
+  if ((rc = r.alloc(n, st)) || (rc = p.alloc(n, st)) || (rc = q.alloc(n, st)) ||
+      (rc = prr.alloc(2 * RED_BLOCKS, st)) || (rc = prz0.alloc(2 * RED_BLOCKS, st)) ||
+      (rc = prz1.alloc(2 * RED_BLOCKS, st)) || (rc = pcell.alloc(ncb, st)) ||
+      (rc = sd.alloc(1, st)))
+    return rc;
+  MF_HIP_TRY(hipMemsetAsync(sd.p, 0, sizeof(SolveState), st));
+  if ((rc = apply_tangent<T>(op, q.p, x, nullptr, nullptr, true))) return rc;
+  if ((rc = cg_init<T>(b, q.p, r.p, p.p, inv, n, prr.p, prz0.p, sd.p, abs_tol, rel_tol, st)))
+    return rc;
+  T *prz[2] = {prz0.p, prz1.p};
+  SolveState h{};
+  int it = 0;
+  const int chunk = 8;
+  for (;;) {
+    MF_HIP_TRY(hipMemcpyAsync(&h, sd.p, sizeof(SolveState), hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    if (h.stop) break;
+    for (int k = 0; k < chunk; ++k) {
+      ++it;
+      if ((rc = apply_tangent<T>(op, q.p, p.p, pcell.p, &sd.p->stop, false))) return rc;
+      if ((rc = cg_reduce_cell_dot(pcell.p, ncb, sd.p, st))) return rc;
+      if ((rc = cg_step<T>(x, r.p, p.p, q.p, inv, n, prz, prr.p, sd.p, it, max_it, st))) return rc;
+    }
+  }
+  out.iterations = h.iterations;
+  out.res0 = h.res0;
+  out.res = h.res;
+  out.status = h.stop == 1 ? MF_OK : (h.stop == 2 ? MF_NO_CONVERGENCE : MF_BREAKDOWN);
+  return MF_OK;
+}
+
+// Right-preconditioned GMRES(m) with two-pass classical Gram-Schmidt.  Convergence measure:
+// the Givens residual estimate |gamma_{j+1}| inside a cycle and the true residual at the start
+// of each cycle (deal.II SolverGMRES with right_preconditioning = true, linear_solver.hpp:112-119).
+template <typename T>
+static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int restart, int max_it,
+                       double abs_tol, double rel_tol, SolveOut &out)
+{
+  const MeshInfo &m = op->s->mesh;
+  const int64_t n = m.n_dofs;
+  hipStream_t st = op->s->stream;
+  const int mm = std::min(std::max(restart, 1), 30);
+  DevBuf<T> V, z, part;
+  DevBuf<double> dh;  // [0..31] h1, [32..63] h2, [64] norm, [65..96] y
+  int rc;
+  if ((rc = V.alloc(size_t(mm + 1) * n, st)) || (rc = z.alloc(n, st)) ||
+      (rc = part.alloc(size_t(32) * 2 * RED_BLOCKS, st)) || (rc = dh.alloc(128, st)))
+    return rc;
+  std::vector<double> H(size_t(mm + 1) * mm, 0.0), cs(mm), sn(mm), gam(mm + 1), y(mm);
+  double hbuf[128];
+  double tol = 0.0, res = 0.0, res0 = 0.0;
+  int acc = 0, state = 0;  // 0 iterate, 1 success, 2 failure, 3 breakdown
+  auto check = [&](int step, double v) {
+    if (step == 0 && acc == 0) tol = std::max(abs_tol, rel_tol * v);
+    if (v <= tol) return 1;
+    if (step >= max_it || std::isnan(v)) return 2;
+    return 0;
+  };
+  bool first = true;
+  for (;;) {
+    T *V0 = V.p;
+    if ((rc = apply_tangent<T>(op, z.p, x, nullptr, nullptr, true))) return rc;
+    if ((rc = vec_residual<T>(b, z.p, V0, n, part.p, dh.p + 64, st))) return rc;
+    MF_HIP_TRY(hipMemcpyAsync(hbuf + 64, dh.p + 64, sizeof(double), hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    double rho = hbuf[64];
+    if (first) {
+      res0 = rho;
+      first = false;
+    }
+    res = rho;
+    state = check(acc, rho);
+    if (state) break;
+    if ((rc = vec_scale_dev<T>(V0, V0, n, dh.p + 64, 1, st))) return rc;
+    std::fill(gam.begin(), gam.end(), 0.0);
+    gam[0] = rho;
+    int j = 0;
+    for (; j < mm && state == 0; ++j) {
+      ++acc;
+      T *vj = V.p + size_t(j) * n, *w = V.p + size_t(j + 1) * n;
+      if ((rc = vec_precond<T>(z.p, vj, inv, n, st))) return rc;
+      if ((rc = apply_tangent<T>(op, w, z.p, nullptr, nullptr, true))) return rc;
+      if ((rc = gmres_multidot<T>(V.p, n, j + 1, w, n, part.p, dh.p, st))) return rc;
+      if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p, w, n, part.p, dh.p + 64, st))) return rc;
+      if ((rc = gmres_multidot<T>(V.p, n, j + 1, w, n, part.p, dh.p + 32, st))) return rc;
+      if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p + 32, w, n, part.p, dh.p + 64, st))) return rc;
+      MF_HIP_TRY(hipMemcpyAsync(hbuf, dh.p, sizeof(double) * 65, hipMemcpyDeviceToHost, st));
+      MF_HIP_TRY(hipStreamSynchronize(st));
+      std::vector<double> hc(j + 2);
+      for (int i = 0; i <= j; ++i) hc[i] = hbuf[i] + hbuf[32 + i];
+      const double s = hbuf[64];
+      hc[j + 1] = s;
+      if (s != 0.0) {
+        if ((rc = vec_scale_dev<T>(w, w, n, dh.p + 64, 1, st))) return rc;
+      }
+      for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * hc[i] + sn[i] * hc[i + 1];
+        hc[i + 1] = -sn[i] * hc[i] + cs[i] * hc[i + 1];
+        hc[i] = t;
+      }
+      const double r = std::hypot(hc[j], hc[j + 1]);
+      if (!(r > 0.0) || std::isnan(r)) {
+        state = 3;
+        break;
+      }
+      cs[j] = hc[j] / r;
+      sn[j] = hc[j + 1] / r;
+      hc[j] = r;
+      gam[j + 1] = -sn[j] * gam[j];
+      gam[j] = cs[j] * gam[j];
+      for (int i = 0; i <= j; ++i) H[size_t(i) * mm + j] = hc[i];
+      rho = std::fabs(gam[j + 1]);
+      res = rho;
+      state = check(acc, rho);
+    }
+    if (state == 3) break;
+    for (int i = j - 1; i >= 0; --i) {
+      double sum = gam[i];
+      for (int l = i + 1; l < j; ++l) sum -= H[size_t(i) * mm + l] * y[l];
+      y[i] = sum / H[size_t(i) * mm + i];
+    }
+    MF_HIP_TRY(hipMemcpyAsync(dh.p + 65, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, st));
+    if ((rc = gmres_update<T>(x, V.p, n, j, dh.p + 65, inv, n, st))) return rc;
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    if (state) break;
+  }
+  out.iterations = acc;
+  out.res0 = res0;
+  out.res = res;
+  out.status = state == 1 ? MF_OK : (state == 2 ? MF_NO_CONVERGENCE : MF_BREAKDOWN);
+  return MF_OK;
+}
+
+template <typename T>
+static int krylov(mf_operator *op, const mf_solver_desc &d, T *x, const T *b, const T *inv,
+                  double abs_tol, double rel_tol, int max_it, SolveOut &out)
+{
+  if (d.solver == MF_SOLVER_CG) return solve_cg<T>(op, x, b, inv, max_it, abs_tol, rel_tol, out);
+  return solve_gmres<T>(op, x, b, inv, d.restart > 0 ? d.restart : 30, max_it, abs_tol, rel_tol,
+                        out);
+}
+
+}  // namespace mf
+
+extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
+                        mf_solve_report *report)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(desc && x && b, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(desc->solver == MF_SOLVER_CG || desc->solver == MF_SOLVER_GMRES, MF_ERR_ARG,
+             "unknown solver type");
+  const MeshInfo &m = op->s->mesh;
+  const int64_t n = m.n_dofs;
+  hipStream_t st = op->s->stream;
+  const int max_it = desc->max_iterations > 0 ? desc->max_iterations : 10000;
+  DevBuf<double> inv;
+  if (desc->preconditioner == MF_PRECOND_JACOBI) {
+    if ((rc = inv.alloc(n, st))) return rc;
+    if ((rc = mf_operator_compute_inverse_diagonal(op, inv.p))) return rc;
+  }
+  mf_solve_report rep{};
+  if (desc->precision == MF_PRECISION_FP64) {
+    SolveOut o;
+    if ((rc = krylov<double>(op, *desc, x, b, inv.p, desc->abs_tol, desc->rel_tol, max_it, o)))
+      return rc;
+    rep.iterations = o.iterations;
+    rep.initial_residual = o.res0;
+    rep.final_residual = o.res;
+    rep.status = o.status;
+  } else {
+    // FP32 inner Krylov inside FP64 defect correction (iterative refinement)
+    DevBuf<double> r, q, nrm;
+    DevBuf<double> part;
+    DevBuf<float> r32, e32, inv32;
+    if ((rc = r.alloc(n, st)) || (rc = q.alloc(n, st)) || (rc = nrm.alloc(1, st)) ||
+        (rc = part.alloc(2 * RED_BLOCKS, st)) || (rc = r32.alloc(n, st)) ||
+        (rc = e32.alloc(n, st)))
+      return rc;
+    if (inv.p) {
+      if ((rc = inv32.alloc(n, st))) return rc;
+      if ((rc = vec_cast<double, float>(inv.p, inv32.p, 1.0, n, st))) return rc;
+    }
+    const double inner_rel = desc->inner_rel_tol > 0 ? desc->inner_rel_tol : 1e-3;
+    const int max_outer = desc->max_outer > 0 ? desc->max_outer : 50;
+    auto residual = [&](double &res) -> int {
+      int c;
+      if ((c = apply_tangent<double>(op, q.p, x, nullptr, nullptr, true))) return c;
+      if ((c = vec_residual<double>(b, q.p, r.p, n, part.p, nrm.p, st))) return c;
+      MF_HIP_TRY(hipMemcpyAsync(&res, nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
+      MF_HIP_TRY(hipStreamSynchronize(st));
+      return MF_OK;
+    };
+    double res = 0.0;
+    if ((rc = residual(res))) return rc;
+    const double res0 = res, tol = std::max(desc->abs_tol, desc->rel_tol * res0);
+    int outer = 0, inner_total = 0, status = MF_OK;
+    while (!(res <= tol)) {
+      if (outer >= max_outer || std::isnan(res)) {
+        status = MF_NO_CONVERGENCE;
+        break;
+      }
+      ++outer;
+      if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
+      if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
+      SolveOut o;
+      if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32.p, 0.0, inner_rel, max_it, o)))
+        return rc;
+      inner_total += o.iterations;
+      if (o.status == MF_BREAKDOWN) {
+        status = MF_BREAKDOWN;
+        break;
+      }
+      if ((rc = vec_add_scaled_f32(x, e32.p, res, n, st))) return rc;
+      if ((rc = residual(res))) return rc;
+    }
+    rep.iterations = inner_total;
+    rep.outer_iterations = outer;
+    rep.initial_residual = res0;
+    rep.final_residual = res;
+    rep.status = status;
+  }
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  if (report) *report = rep;
+  if (rep.status != MF_OK) {
+    set_error(rep.status == MF_NO_CONVERGENCE ? "linear solver did not converge (NoConvergence)"
+                                              : "Krylov breakdown");
+    return rep.status;
+  }
+  return MF_OK;
+}

@@ -1,0 +1,541 @@
+// Matrix-free cell kernels for gfx950: gather -> sum-factorised evaluate -> quadrature-point
+// physics -> sum-factorised integrate -> scatter-add, for the heat Jacobian / residual /
+// diagonal and the advection-diffusion tangent / rhs / diagonal.
+//
+// Reference functions replaced (cell-level arithmetic of deal.II FEEvaluation driven by):
+//   HeatDiffuseMultiPhaseOperator::tangent_cell_loop + tangent_local_cell_operation
+//     (source/heat/heat_diffuse_operator.cpp:199-241, 955-1086)
+//   HeatDiffuseMultiPhaseOperator::rhs_cell_loop (source/heat/heat_diffuse_operator.cpp:393-529)
+//   internal_compute_diagonal_or_system_matrix (source/heat/heat_diffuse_operator.cpp:297-391)
+//   AdvectionDiffusionOperator::vmult / create_rhs / compute_inverse_diagonal_from_matrixfree /
+//     tangent_local_cell_operation (source/level_set/advection_diffusion_operator.cpp:236-495)
+//
+// Work decomposition.  A workgroup holds CPB cells; each cell is served by K*K threads, thread
+// (cell c, pencil p) with c fastest in threadIdx.x so that the 16-lane groups of a wave touch
+// 16 different cells (consecutive LDS words, consecutive cells in x in HBM).  Each 1D sweep is a
+// K x K matrix applied to K-long pencils: a thread reads its pencil from LDS into registers,
+// multiplies, and writes it back; the pencil direction changes between phases, separated by
+// barriers.  Cell tensors live in LDS as [K^3][CPB] arrays.  Gradients at the quadrature
+// points use the collocation derivative Dq on the Gauss points (grad = Dq (S u)), which equals
+// (DS u) exactly in exact arithmetic because K points determine the degree-p polynomial.
+// The scatter-add uses hardware FP64/FP32 atomics into a zeroed destination.
+#include "mf_internal.hpp"
+
+namespace mf {
+
+template <int K>
+constexpr int cells_per_block()
+{
+  return K == 2 ? 64 : 16;
+}
+
+template <int K>
+__device__ __forceinline__ int elem(int i, int j, int k)
+{
+  return i + K * j + K * K * k;
+}
+
+// element e of pencil p (p = a + K*b) along direction DIR
+template <int K, int DIR>
+__device__ __forceinline__ int pencil(int p, int e)
+{
+  const int a = p % K, b = p / K;
+  if constexpr (DIR == 0) return elem<K>(e, a, b);
+  if constexpr (DIR == 1) return elem<K>(a, e, b);
+  return elem<K>(a, b, e);
+}
+
+template <typename Number, int K, int CPB, int DIR>
+__device__ __forceinline__ void load_pencil(const Number *buf, int p, int c, Number (&v)[K])
+{
+#pragma unroll
+  for (int e = 0; e < K; ++e) v[e] = buf[pencil<K, DIR>(p, e) * CPB + c];
+}
+
+template <typename Number, int K, int CPB, int DIR>
+__device__ __forceinline__ void store_pencil(Number *buf, int p, int c, const Number (&v)[K])
+{
+#pragma unroll
+  for (int e = 0; e < K; ++e) buf[pencil<K, DIR>(p, e) * CPB + c] = v[e];
+}
+
+// out = M in (TRANSPOSE: out = M^T in), M row-major K x K
+template <typename Number, int K, bool TRANSPOSE>
+__device__ __forceinline__ void matvec(const Number *M, const Number (&in)[K], Number (&out)[K])
+{
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    Number acc = 0;
+#pragma unroll
+    for (int m = 0; m < K; ++m) acc = fma(TRANSPOSE ? M[m * K + i] : M[i * K + m], in[m], acc);
+    out[i] = acc;
+  }
+}
+
+template <typename Number, int K, int CPB, int DIR, bool TRANSPOSE>
+__device__ __forceinline__ void sweep_inplace(const Number *M, Number *buf, int p, int c)
+{
+  Number v[K], o[K];
+  load_pencil<Number, K, CPB, DIR>(buf, p, c, v);
+  matvec<Number, K, TRANSPOSE>(M, v, o);
+  store_pencil<Number, K, CPB, DIR>(buf, p, c, o);
+}
+
+template <typename Number, int K, int CPB, int DIR, bool TRANSPOSE>
+__device__ __forceinline__ void sweep(const Number *M, const Number *in, Number *out, int p, int c)
+{
+  Number v[K], o[K];
+  load_pencil<Number, K, CPB, DIR>(in, p, c, v);
+  matvec<Number, K, TRANSPOSE>(M, v, o);
+  store_pencil<Number, K, CPB, DIR>(out, p, c, o);
+}
+
+__device__ __forceinline__ void atomic_add(double *a, double v) { unsafeAtomicAdd(a, v); }
+__device__ __forceinline__ void atomic_add(float *a, float v) { unsafeAtomicAdd(a, v); }
+
+// tanh((T-Tc)/w) and sech^2 from one expm1: t = -em/(2+em), s2 = 4(1+em)/(2+em)^2 with
+// em = expm1(-2|x|).  Beyond |x| = 20 (double) / 10 (float) tanh is +-1 to the last bit and the
+// sech^2 term is below 1e-16 (1e-8) of the dk/dT scale, so it is taken as 0.
+template <typename Number>
+__device__ __forceinline__ void tanh_sech2(Number x, Number &t, Number &s2)
+{
+  const Number ax = fabs(x);
+  const Number cut = sizeof(Number) == 8 ? Number(20) : Number(10);
+  if (ax < cut) {
+    const Number em = expm1(Number(-2) * ax);
+    const Number inv = Number(1) / (Number(2) + em);
+    t = -em * inv;
+    s2 = Number(4) * (Number(1) + em) * inv * inv;
+    if (x < 0) t = -t;
+  } else {
+    t = x < 0 ? Number(-1) : Number(1);
+    s2 = 0;
+  }
+}
+
+template <typename Number>
+__device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
+{
+  // wave reduction with shuffles, then across waves through LDS
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  Number s = 0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) / 64;
+    for (int i = 0; i < nw; ++i) s += red[i];
+  }
+  return s;
+}
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+__global__ void __launch_bounds__(cells_per_block<K>() * K * K)
+    cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst)
+{
+  constexpr int CPB = cells_per_block<K>();
+  constexpr int NL = K * K * K;
+  constexpr int P = K - 1;
+  constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG);
+  constexpr bool HAS_U = !IS_DIAG;
+  constexpr bool USE_T = NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG);
+  constexpr bool RHS_HEAT = (OP == HEAT_RHS);
+  constexpr bool TANGENT = (OP == HEAT_TANGENT || OP == ADVEC_TANGENT);
+  constexpr bool NEED_A = RHS_HEAT || IS_DIAG;
+  constexpr bool NEED_GY = HAS_U || IS_DIAG;
+
+  __shared__ Number bU[NL * CPB];
+  __shared__ Number bGX[NL * CPB];
+  __shared__ Number bGY[NEED_GY ? NL * CPB : 1];
+  __shared__ Number bT[USE_T ? NL * CPB : 1];
+  __shared__ Number bTX[USE_T ? NL * CPB : 1];
+  __shared__ Number bTY[USE_T ? NL * CPB : 1];
+  __shared__ Number bW[VEL ? 3 * NL * CPB : 1];
+  __shared__ Number bA1[NEED_A ? NL * CPB : 1];
+  __shared__ Number bA2[NEED_A ? NL * CPB : 1];
+  __shared__ Number red[16];
+
+  if (A.stop && *A.stop) return;  // uniform over the grid
+
+  const int c = threadIdx.x % CPB;
+  const int p = threadIdx.x / CPB;
+  const int cell = blockIdx.x * CPB + c;
+  const bool active = cell < A.n_cells;
+  const int cx = cell % A.ncell[0];
+  const int cy = (cell / A.ncell[0]) % A.ncell[1];
+  const int cz = cell / (A.ncell[0] * A.ncell[1]);
+  const Number *S = A.S;
+  const Number *Dq = A.Dq;
+
+  // ---------------- E1: x pencils, gather from HBM ----------------
+  // pencil p -> (j, k) = (p % K, p / K); nodes (cx*P + i, cy*P + j, cz*P + k)
+  const uint32_t gbase = uint32_t(cx * P) +
+                         A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
+  Number u_loc[K];  // gathered values of the x pencil (kept for the fused dot)
+  {
+    Number o[K];
+    if constexpr (HAS_U) {
+      const Number *in = (OP == HEAT_RHS) ? A.T : src;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        Number v = 0;
+        if (active) {
+          v = in[gbase + i];
+          if (TANGENT && A.use_mask && A.mask && A.mask[gbase + i]) v = 0;
+        }
+        u_loc[i] = v;
+      }
+      matvec<Number, K, false>(S, u_loc, o);
+      store_pencil<Number, K, CPB, 0>(bU, p, c, o);
+    }
+    if constexpr (USE_T) {
+      Number v[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) v[i] = active ? A.T[gbase + i] : Number(0);
+      matvec<Number, K, false>(S, v, o);
+      store_pencil<Number, K, CPB, 0>(bT, p, c, o);
+    }
+    if constexpr (VEL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        Number v[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          v[i] = active ? A.vel[size_t(d) * A.n_dofs + gbase + i] : Number(0);
+        matvec<Number, K, false>(S, v, o);
+        store_pencil<Number, K, CPB, 0>(bW + d * NL * CPB, p, c, o);
+      }
+    }
+    if constexpr (RHS_HEAT) {
+      Number v[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) v[i] = active ? src[gbase + i] : Number(0);  // T_old
+      matvec<Number, K, false>(S, v, o);
+      store_pencil<Number, K, CPB, 0>(bA1, p, c, o);
+#pragma unroll
+      for (int i = 0; i < K; ++i) v[i] = (active && A.aux1) ? A.aux1[gbase + i] : Number(0);
+      matvec<Number, K, false>(S, v, o);
+      store_pencil<Number, K, CPB, 0>(bA2, p, c, o);
+    }
+  }
+  __syncthreads();
+
+  // ---------------- E2: y pencils ----------------
+  if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 1, false>(S, bU, p, c);
+  if constexpr (USE_T) sweep_inplace<Number, K, CPB, 1, false>(S, bT, p, c);
+  if constexpr (VEL) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sweep_inplace<Number, K, CPB, 1, false>(S, bW + d * NL * CPB, p, c);
+  }
+  if constexpr (RHS_HEAT) {
+    sweep_inplace<Number, K, CPB, 1, false>(S, bA1, p, c);
+    sweep_inplace<Number, K, CPB, 1, false>(S, bA2, p, c);
+  }
+  __syncthreads();
+
+  // ---------------- E3: z pencils (values at q and d/dz kept in registers) ----------------
+  Number uq[K], guz[K], Tq[K], gTz[K], wq[3][K], a1q[K], a2q[K];
+  if constexpr (HAS_U) {
+    Number v[K];
+    load_pencil<Number, K, CPB, 2>(bU, p, c, v);
+    matvec<Number, K, false>(S, v, uq);
+    matvec<Number, K, false>(Dq, uq, guz);
+    store_pencil<Number, K, CPB, 2>(bU, p, c, uq);
+  }
+  if constexpr (USE_T) {
+    Number v[K];
+    load_pencil<Number, K, CPB, 2>(bT, p, c, v);
+    matvec<Number, K, false>(S, v, Tq);
+    matvec<Number, K, false>(Dq, Tq, gTz);
+    store_pencil<Number, K, CPB, 2>(bT, p, c, Tq);
+  }
+  if constexpr (VEL) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      Number v[K];
+      load_pencil<Number, K, CPB, 2>(bW + d * NL * CPB, p, c, v);
+      matvec<Number, K, false>(S, v, wq[d]);
+    }
+  }
+  if constexpr (RHS_HEAT) {
+    Number v[K];
+    load_pencil<Number, K, CPB, 2>(bA1, p, c, v);
+    matvec<Number, K, false>(S, v, a1q);
+    load_pencil<Number, K, CPB, 2>(bA2, p, c, v);
+    matvec<Number, K, false>(S, v, a2q);
+  }
+  __syncthreads();
+
+  // ---------------- E4: d/dx and d/dy at the quadrature points ----------------
+  if constexpr (HAS_U) {
+    sweep<Number, K, CPB, 0, false>(Dq, bU, bGX, p, c);
+    sweep<Number, K, CPB, 1, false>(Dq, bU, bGY, p, c);
+  }
+  if constexpr (USE_T) {
+    sweep<Number, K, CPB, 0, false>(Dq, bT, bTX, p, c);
+    sweep<Number, K, CPB, 1, false>(Dq, bT, bTY, p, c);
+  }
+  __syncthreads();
+
+  // ---------------- Q: quadrature-point operation (z pencil (i,j), q = (i,j,z)) ----------------
+  const int qi = p % K, qj = p / K;
+  const Number kc = (A.k_cell && active) ? A.k_cell[cell] : Number(1);
+  const Number wij = A.w[qi] * A.w[qj] * A.h[0] * A.h[1] * A.h[2];
+  Number rq[K], fz[K];
+#pragma unroll
+  for (int z = 0; z < K; ++z) {
+    const int e = elem<K>(qi, qj, z) * CPB + c;
+    const Number jxw = wij * A.w[z];
+    Number g[3] = {0, 0, 0};
+    if constexpr (HAS_U) {
+      g[0] = bGX[e] * A.inv_h[0];
+      g[1] = bGY[e] * A.inv_h[1];
+      g[2] = guz[z] * A.inv_h[2];
+    }
+    Number w3[3] = {0, 0, 0};
+    if constexpr (VEL) {
+      w3[0] = wq[0][z];
+      w3[1] = wq[1][z];
+      w3[2] = wq[2][z];
+    }
+    Number val = 0, f[3] = {0, 0, 0};
+    Number cc = 0, kk = 0, bb[3] = {0, 0, 0};
+    if constexpr (OP == HEAT_TANGENT || OP == HEAT_DIAG) {
+      Number k = A.k0, dk = 0, gT[3] = {0, 0, 0};
+      if constexpr (USE_T) {
+        Number t, s2;
+        tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
+        k = A.k0 + A.tanh_amp * t;
+        dk = A.tanh_amp * A.inv_tanh_w * s2;
+        gT[0] = bTX[e] * A.inv_h[0];
+        gT[1] = bTY[e] * A.inv_h[1];
+        gT[2] = gTz[z] * A.inv_h[2];
+      }
+      k *= kc;
+      dk *= kc;
+      const bool td = USE_T && A.t_dependent;
+      if constexpr (OP == HEAT_TANGENT) {
+        const Number u = uq[z];
+        val = A.dt_inv * A.rho_cp * u;
+        if constexpr (VEL) val += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] = k * g[d];
+        if (td) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) f[d] += dk * gT[d] * u;
+        }
+      } else {
+        cc = A.dt_inv * A.rho_cp;
+        kk = k;
+        if constexpr (VEL) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) bb[d] = A.rho_cp * w3[d];
+        }
+        if (td) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) bb[d] += dk * gT[d];
+        }
+      }
+    } else if constexpr (OP == HEAT_RHS) {
+      // U holds T: values uq, gradient g
+      Number k = A.k0;
+      if constexpr (NEEDT) {
+        Number t, s2;
+        tanh_sech2((uq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
+        k = A.k0 + A.tanh_amp * t;
+      }
+      k *= kc;
+      Number v = A.dt_inv * A.rho_cp * (uq[z] - a1q[z]) - a2q[z];
+      if constexpr (VEL) v += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
+      val = -v;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] = -k * g[d];
+    } else if constexpr (OP == ADVEC_TANGENT) {
+      const Number vg = g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2];
+      val = A.mass_factor * A.dt_inv * uq[z] + A.theta * vg;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] = A.theta * A.diffusivity * g[d];
+    } else if constexpr (OP == ADVEC_RHS) {
+      const Number vg = g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2];
+      val = A.dt_inv * uq[z] - (Number(1) - A.theta) * vg;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[d] = -(Number(1) - A.theta) * A.diffusivity * g[d];
+    } else {  // ADVEC_DIAG
+      cc = A.mass_factor * A.dt_inv;
+      kk = A.theta * A.diffusivity;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) bb[d] = A.theta * w3[d];
+    }
+    if constexpr (IS_DIAG) {
+      bU[e] = jxw * cc;
+      bGX[e] = jxw * kk;
+      bGY[e] = jxw * bb[0];
+      bA1[e] = jxw * bb[1];
+      bA2[e] = jxw * bb[2];
+    } else {
+      rq[z] = jxw * val;
+      bGX[e] = jxw * f[0] * A.inv_h[0];
+      bGY[e] = jxw * f[1] * A.inv_h[1];
+      fz[z] = jxw * f[2] * A.inv_h[2];
+    }
+  }
+
+  if constexpr (IS_DIAG) {
+    __syncthreads();
+    // A_ii = sum_q [cc phi^2 + kk |grad phi|^2 + (bb . grad phi) phi] for the thread's x pencil
+    const int jb = p % K, kb = p / K;
+    Number diag[K];
+#pragma unroll
+    for (int ia = 0; ia < K; ++ia) diag[ia] = 0;
+    for (int q3 = 0; q3 < K; ++q3)
+      for (int q2 = 0; q2 < K; ++q2) {
+        const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * A.inv_h[1];
+        const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * A.inv_h[2];
+#pragma unroll
+        for (int q1 = 0; q1 < K; ++q1) {
+          const int e = elem<K>(q1, q2, q3) * CPB + c;
+          const Number cq = bU[e], kq = bGX[e], b0 = bGY[e], b1 = bA1[e], b2 = bA2[e];
+#pragma unroll
+          for (int ia = 0; ia < K; ++ia) {
+            const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * A.inv_h[0];
+            const Number phi = sx * sy * sz;
+            const Number gx = dx * sy * sz, gy = sx * dy * sz, gz = sx * sy * dz;
+            diag[ia] += cq * phi * phi + kq * (gx * gx + gy * gy + gz * gz) +
+                        (b0 * gx + b1 * gy + b2 * gz) * phi;
+          }
+        }
+      }
+    if (active) {
+#pragma unroll
+      for (int ia = 0; ia < K; ++ia) {
+        const uint32_t gi = gbase + ia;
+        if (A.use_mask && A.mask && A.mask[gi]) continue;
+        atomic_add(dst + gi, diag[ia]);
+      }
+    }
+    return;
+  } else {
+    // r = val + Dq_z^T fz (z direction in registers)
+    Number tz[K];
+    matvec<Number, K, true>(Dq, fz, tz);
+#pragma unroll
+    for (int z = 0; z < K; ++z) rq[z] += tz[z];
+    store_pencil<Number, K, CPB, 2>(bU, p, c, rq);
+    __syncthreads();
+
+    // ---------------- I1: Dq^T along x (on fx) and y (on fy) ----------------
+    sweep_inplace<Number, K, CPB, 0, true>(Dq, bGX, p, c);
+    sweep_inplace<Number, K, CPB, 1, true>(Dq, bGY, p, c);
+    __syncthreads();
+
+    // ---------------- I2: sum and S^T along z ----------------
+    {
+      Number a[K], b[K], d[K], o[K];
+      load_pencil<Number, K, CPB, 2>(bU, p, c, a);
+      load_pencil<Number, K, CPB, 2>(bGX, p, c, b);
+      load_pencil<Number, K, CPB, 2>(bGY, p, c, d);
+#pragma unroll
+      for (int e = 0; e < K; ++e) a[e] += b[e] + d[e];
+      matvec<Number, K, true>(S, a, o);
+      store_pencil<Number, K, CPB, 2>(bU, p, c, o);
+    }
+    __syncthreads();
+
+    // ---------------- I3: S^T along y ----------------
+    sweep_inplace<Number, K, CPB, 1, true>(S, bU, p, c);
+    __syncthreads();
+
+    // ---------------- I4: S^T along x and scatter-add ----------------
+    Number o[K];
+    {
+      Number v[K];
+      load_pencil<Number, K, CPB, 0>(bU, p, c, v);
+      matvec<Number, K, true>(S, v, o);
+    }
+    Number local_dot = 0;
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const uint32_t gi = gbase + i;
+        if (A.use_mask && A.mask && A.mask[gi]) continue;
+        atomic_add(dst + gi, o[i]);
+        if constexpr (TANGENT) local_dot += u_loc[i] * o[i];
+      }
+    }
+    if constexpr (TANGENT) {
+      if (A.dot_partial) {
+        const double s = block_reduce_sum<double>(double(local_dot), reinterpret_cast<double *>(red));
+        if (threadIdx.x == 0) A.dot_partial[blockIdx.x] = s;
+      }
+    }
+  }
+}
+
+int cell_blocks(const MeshInfo &m)
+{
+  const int cpb = m.K == 2 ? cells_per_block<2>() : cells_per_block<4>();
+  return int((m.n_cells + cpb - 1) / cpb);
+}
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src, Number *dst,
+                    hipStream_t st)
+{
+  constexpr int CPB = cells_per_block<K>();
+  const int blocks = int((m.n_cells + CPB - 1) / CPB);
+  hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL>), dim3(blocks), dim3(CPB * K * K), 0,
+                     st, a, src, dst);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template <typename Number, int K, int OP>
+static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                     const Number *src, Number *dst, hipStream_t st)
+{
+  if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
+    return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
+  } else {
+    if (needT) {
+      if (vel) return launch_k<Number, K, OP, true, true>(m, a, src, dst, st);
+      return launch_k<Number, K, OP, true, false>(m, a, src, dst, st);
+    }
+    if (vel) return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
+    return launch_k<Number, K, OP, false, false>(m, a, src, dst, st);
+  }
+}
+
+template <typename Number, int K>
+static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                      const Number *src, Number *dst, hipStream_t st)
+{
+  switch (op) {
+  case HEAT_TANGENT: return launch_op<Number, K, HEAT_TANGENT>(needT, vel, m, a, src, dst, st);
+  case HEAT_RHS: return launch_op<Number, K, HEAT_RHS>(needT, vel, m, a, src, dst, st);
+  case HEAT_DIAG: return launch_op<Number, K, HEAT_DIAG>(needT, vel, m, a, src, dst, st);
+  case ADVEC_TANGENT: return launch_op<Number, K, ADVEC_TANGENT>(needT, vel, m, a, src, dst, st);
+  case ADVEC_RHS: return launch_op<Number, K, ADVEC_RHS>(needT, vel, m, a, src, dst, st);
+  case ADVEC_DIAG: return launch_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st);
+  default: set_error("unknown cell operation"); return MF_ERR_ARG;
+  }
+}
+
+template <typename Number>
+int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                const Number *src, Number *dst, hipStream_t st)
+{
+  switch (m.K) {
+  case 2: return launch_deg<Number, 2>(op, needT, vel, m, a, src, dst, st);
+  case 3: return launch_deg<Number, 3>(op, needT, vel, m, a, src, dst, st);
+  case 4: return launch_deg<Number, 4>(op, needT, vel, m, a, src, dst, st);
+  default: set_error("degree must be 1, 2 or 3"); return MF_ERR_ARG;
+  }
+}
+
+template int launch_cell<double>(int, bool, bool, const MeshInfo &, const CellArgs<double> &,
+                                 const double *, double *, hipStream_t);
+template int launch_cell<float>(int, bool, bool, const MeshInfo &, const CellArgs<float> &,
+                                const float *, float *, hipStream_t);
+
+}  // namespace mf

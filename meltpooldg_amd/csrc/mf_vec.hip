@@ -1,0 +1,599 @@
+// Vector-layer and Krylov kernels: fills and casts, Jacobi inversion, the fused
+// preconditioned-CG update kernels with device-side convergence control, and the GMRES
+// orthogonalisation kernels.  Replaces the LA::distributed::Vector operations deal.II's
+// SolverCG / SolverGMRES perform inside LinearSolver::solve
+// (include/meltpooldg/linear_algebra/linear_solver.hpp:106-119) and JacobiPreconditioner::vmult
+// (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:51-56).
+//
+// Reductions are deterministic: every reduction kernel writes one partial per block
+// (RED_BLOCKS blocks, grid-stride), and consumers re-reduce the partials in a fixed order.
+// FP64 partials are plain sums; FP32 partials carry a Neumaier compensation term, combined with
+// two-sum at every level (wave, block, partials), so FP32 dot products keep ~FP64-like accuracy.
+#include "mf_vec.hpp"
+
+namespace mf {
+
+// ------------------------------------------------------------------------------------------
+// compensated accumulator
+// ------------------------------------------------------------------------------------------
+template <typename T>
+struct Acc;
+
+template <>
+struct Acc<double> {
+  double s = 0, c = 0;
+  __device__ void add(double x) { s += x; }
+  __device__ void merge(double os, double oc) { s += os; c += oc; }
+  __device__ double value() const { return s + c; }
+};
+
+template <>
+struct Acc<float> {
+  float s = 0, c = 0;
+  __device__ void add(float x)
+  {
+    const float t = s + x;
+    c += fabsf(s) >= fabsf(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ void merge(float os, float oc)
+  {
+    const float t = s + os;
+    c += (fabsf(s) >= fabsf(os) ? (s - t) + os : (os - t) + s) + oc;
+    s = t;
+  }
+  __device__ float value() const { return s + c; }
+};
+
+// block-wide reduction of an accumulator; result valid in thread 0
+template <typename T>
+__device__ Acc<T> block_reduce(Acc<T> a)
+{
+  __shared__ T rs[16], rc[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T os = __shfl_xor(a.s, off, 64);
+    const T oc = __shfl_xor(a.c, off, 64);
+    a.merge(os, oc);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    rs[wid] = a.s;
+    rc[wid] = a.c;
+  }
+  __syncthreads();
+  Acc<T> r;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) r.merge(rs[i], rc[i]);
+  }
+  return r;
+}
+
+// every block reduces the same partial array (2*n values: sums then compensations) and returns
+// the same value to all its threads
+template <typename T>
+__device__ T reduce_partials(const T *part, int n)
+{
+  __shared__ T result;
+  Acc<T> a;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a.merge(part[i], part[n + i]);
+  a = block_reduce(a);
+  if (threadIdx.x == 0) result = a.value();
+  __syncthreads();
+  return result;
+}
+
+template <typename T>
+__device__ void write_partial(T *part, int n, Acc<T> a)
+{
+  a = block_reduce(a);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a.s;
+    part[n + blockIdx.x] = a.c;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// elementwise kernels
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void fill_kernel(T *x, T v, int64_t n, const int *stop)
+{
+  if (stop && *stop) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+template <typename Ta, typename Tb>
+__global__ void cast_kernel(const Ta *a, Tb *b, double scale, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = Tb(double(a[i]) * scale);
+}
+
+// x += scale * e  (FP64 solution update of the defect correction)
+__global__ void add_scaled_f32_kernel(double *x, const float *e, double scale, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] += scale * double(e[i]);
+}
+
+// r = b - q, partials of r.r
+template <typename T>
+__global__ void residual_kernel(const T *b, const T *q, T *r, int64_t n, T *part)
+{
+  Acc<T> a;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T v = b[i] - q[i];
+    r[i] = v;
+    a.add(v * v);
+  }
+  write_partial(part, gridDim.x, a);
+}
+
+template <typename T>
+__global__ void dot_kernel(const T *a, const T *b, int64_t n, T *part)
+{
+  Acc<T> acc;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    acc.add(a[i] * b[i]);
+  write_partial(part, gridDim.x, acc);
+}
+
+// max |d| partials (no compensation needed)
+__global__ void absmax_kernel(const double *d, int64_t n, double *part)
+{
+  __shared__ double red[16];
+  double m = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    m = fmax(m, fabs(d[i]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// |d| > thr ? 1/d : 1; thr = abs_thr or rel_thr * max(1, max_i |d_i|) from the partials
+__global__ void invert_diag_kernel(double *d, int64_t n, double abs_thr, double rel_thr,
+                                   const double *max_part, int nparts)
+{
+  __shared__ double thr;
+  if (threadIdx.x == 0) {
+    double t = abs_thr;
+    if (max_part) {
+      double m = 1.0;
+      for (int i = 0; i < nparts; ++i) m = fmax(m, max_part[i]);
+      t = rel_thr * m;
+    }
+    thr = t;
+  }
+  __syncthreads();
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double v = d[i];
+    d[i] = fabs(v) > thr ? 1.0 / v : 1.0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// fused PCG (deal.II SolverCG semantics with ReductionControl)
+// ------------------------------------------------------------------------------------------
+// After q = A x:  r = b - q, z = D r, p = z, partial r.r and r.z (slot 0), then q = 0.
+template <typename T>
+__global__ void cg_init_kernel(const T *b, T *q, T *r, T *p, const T *inv, int64_t n,
+                               T *part_rr, T *part_rz0)
+{
+  Acc<T> arr, arz;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T v = b[i] - q[i];
+    const T z = inv ? inv[i] * v : v;
+    r[i] = v;
+    p[i] = z;
+    q[i] = 0;
+    arr.add(v * v);
+    arz.add(v * z);
+  }
+  write_partial(part_rr, gridDim.x, arr);
+  write_partial(part_rz0, gridDim.x, arz);
+}
+
+// step-0 check: tol = max(abs, rel*|r0|), success if |r0| <= tol
+template <typename T>
+__global__ void cg_check0_kernel(const T *part_rr, int nparts, SolveState *st, double abs_tol,
+                                 double rel_tol)
+{
+  const T rr = reduce_partials(part_rr, nparts);
+  if (threadIdx.x == 0) {
+    const double res = sqrt(double(rr));
+    st->res0 = res;
+    st->res = res;
+    st->tol = fmax(abs_tol, rel_tol * res);
+    st->iterations = 0;
+    st->stop = (res <= st->tol) ? 1 : (isnan(res) ? 3 : 0);
+  }
+}
+
+// 1-block reduction of the cell kernel's fused p.(Ap) partials into st->pAp
+__global__ void reduce_cell_dot_kernel(const double *part, int n, SolveState *st)
+{
+  if (st->stop) return;
+  Acc<double> a;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a.add(part[i]);
+  a = block_reduce(a);
+  if (threadIdx.x == 0) st->pAp = a.value();
+}
+
+// alpha = rz/pAp; x += alpha p; r -= alpha q; partial r.r, r.(D r) into slot it%2
+template <typename T>
+__global__ void cg_update_kernel(T *x, T *r, const T *p, const T *q, const T *inv, int64_t n,
+                                 const T *part_rz_old, T *part_rz_new, T *part_rr, int nparts,
+                                 SolveState *st)
+{
+  if (st->stop) return;
+  const T rz = reduce_partials(part_rz_old, nparts);
+  const double pAp = st->pAp;
+  if (!(pAp != 0.0) || isnan(pAp)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->breakdown = 1;
+    return;
+  }
+  const T alpha = T(double(rz) / pAp);
+  Acc<T> arr, arz;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    x[i] += alpha * p[i];
+    const T v = r[i] - alpha * q[i];
+    r[i] = v;
+    arr.add(v * v);
+    arz.add(v * (inv ? inv[i] * v : v));
+  }
+  write_partial(part_rr, nparts, arr);
+  write_partial(part_rz_new, nparts, arz);
+}
+
+// convergence check for step `it`, then p = D r + beta p, q = 0
+template <typename T>
+__global__ void cg_direction_kernel(T *p, T *q, const T *r, const T *inv, int64_t n,
+                                    const T *part_rz_new, const T *part_rz_old, const T *part_rr,
+                                    int nparts, SolveState *st, int it, int max_it)
+{
+  if (st->stop) return;
+  if (st->breakdown) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->stop = 3;
+      st->iterations = it;
+    }
+    return;
+  }
+  const T rr = reduce_partials(part_rr, nparts);
+  const double res = sqrt(double(rr));
+  const double tol = st->tol;
+  const bool done = res <= tol;
+  const bool fail = !done && (it >= max_it || isnan(res));
+  if (done || fail) {
+    // every block sees the same values; block 0 records, all blocks skip the update
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->res = res;
+      st->iterations = it;
+      st->stop = done ? 1 : 2;
+    }
+    return;
+  }
+  const T rz_new = reduce_partials(part_rz_new, nparts);
+  const T rz_old = reduce_partials(part_rz_old, nparts);
+  const T beta = T(double(rz_new) / double(rz_old));
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T z = inv ? inv[i] * r[i] : r[i];
+    p[i] = z + beta * p[i];
+    q[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->res = res;
+    st->iterations = it;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// GMRES helpers (classical Gram-Schmidt, two passes)
+// ------------------------------------------------------------------------------------------
+// partials of h_i = V_i . w for i < m (V row-major [m][n]); part layout [m][2*nparts]
+template <typename T>
+__global__ void multidot_kernel(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part)
+{
+  constexpr int MAXM = 32;
+  Acc<T> acc[MAXM];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const T wi = w[i];
+#pragma unroll 4
+    for (int k = 0; k < m; ++k) acc[k].add(V[k * ld + i] * wi);
+  }
+  for (int k = 0; k < m; ++k) write_partial(part + size_t(k) * 2 * gridDim.x, gridDim.x, acc[k]);
+}
+
+// reduce [m][2*nparts] partials into h[m] (1 block)
+template <typename T>
+__global__ void reduce_multi_kernel(const T *part, int nparts, int m, double *h)
+{
+  for (int k = 0; k < m; ++k) {
+    Acc<T> a;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+      a.merge(part[size_t(k) * 2 * nparts + i], part[size_t(k) * 2 * nparts + nparts + i]);
+    a = block_reduce(a);
+    if (threadIdx.x == 0) h[k] = double(a.value());
+    __syncthreads();
+  }
+}
+
+// w -= sum_k h_k V_k, then partial of w.w
+template <typename T>
+__global__ void multiaxpy_kernel(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n,
+                                 T *part_ww)
+{
+  __shared__ T hs[32];
+  if (threadIdx.x < m) hs[threadIdx.x] = T(h[threadIdx.x]);
+  __syncthreads();
+  Acc<T> a;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    T v = w[i];
+    for (int k = 0; k < m; ++k) v -= hs[k] * V[k * ld + i];
+    w[i] = v;
+    a.add(v * v);
+  }
+  write_partial(part_ww, gridDim.x, a);
+}
+
+// dst = scale * src, scale read on the device (1/s for normalisation)
+template <typename T>
+__global__ void scale_kernel(T *dst, const T *src, int64_t n, const double *s, int invert)
+{
+  const double v = *s;
+  const T f = T(invert ? 1.0 / v : v);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    dst[i] = f * src[i];
+}
+
+// x += D (sum_k y_k V_k)   (right-preconditioned update)
+template <typename T>
+__global__ void gmres_update_kernel(T *x, const T *V, int64_t ld, int m, const double *y,
+                                    const T *inv, int64_t n)
+{
+  __shared__ T ys[32];
+  if (threadIdx.x < m) ys[threadIdx.x] = T(y[threadIdx.x]);
+  __syncthreads();
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    T v = 0;
+    for (int k = 0; k < m; ++k) v += ys[k] * V[k * ld + i];
+    x[i] += inv ? inv[i] * v : v;
+  }
+}
+
+// z = D v
+template <typename T>
+__global__ void precond_kernel(T *z, const T *v, const T *inv, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    z[i] = inv ? inv[i] * v[i] : v[i];
+}
+
+// 1-block: s = sqrt(sum of partials)
+template <typename T>
+__global__ void norm_from_partials_kernel(const T *part, int nparts, double *out)
+{
+  const T s = reduce_partials(part, nparts);
+  if (threadIdx.x == 0) *out = sqrt(double(s));
+}
+
+// ------------------------------------------------------------------------------------------
+// host launchers
+// ------------------------------------------------------------------------------------------
+static inline dim3 grid_for(int64_t n)
+{
+  int64_t b = (n + VEC_THREADS - 1) / VEC_THREADS;
+  if (b > RED_BLOCKS) b = RED_BLOCKS;
+  if (b < 1) b = 1;
+  return dim3(unsigned(b));
+}
+
+template <typename T>
+int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop)
+{
+  hipLaunchKernelGGL(fill_kernel<T>, grid_for(n), dim3(VEC_THREADS), 0, st, x, v, n, stop);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_fill<double>(double *, double, int64_t, hipStream_t, const int *);
+template int vec_fill<float>(float *, float, int64_t, hipStream_t, const int *);
+
+template <typename Ta, typename Tb>
+int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL((cast_kernel<Ta, Tb>), grid_for(n), dim3(VEC_THREADS), 0, st, a, b, scale, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_cast<double, float>(const double *, float *, double, int64_t, hipStream_t);
+template int vec_cast<float, double>(const float *, double *, double, int64_t, hipStream_t);
+template int vec_cast<double, double>(const double *, double *, double, int64_t, hipStream_t);
+
+int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(add_scaled_f32_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, x, e, scale, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st)
+{
+  if (relative) {
+    hipLaunchKernelGGL(absmax_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, d, n, work);
+    MF_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(invert_diag_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, d, n, 0.0,
+                       1.0e-14, work, RED_BLOCKS);
+  } else {
+    hipLaunchKernelGGL(invert_diag_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, d, n, 1.0e-10,
+                       0.0, (const double *)nullptr, 0);
+  }
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template <typename T>
+int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out,
+                 hipStream_t st)
+{
+  hipLaunchKernelGGL(residual_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, b, q, r, n,
+                     part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(norm_from_partials_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, norm_out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_residual<double>(const double *, const double *, double *, int64_t, double *,
+                                  double *, hipStream_t);
+template int vec_residual<float>(const float *, const float *, float *, int64_t, float *, double *,
+                                 hipStream_t);
+
+template <typename T>
+int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st)
+{
+  hipLaunchKernelGGL(dot_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, a, a, n, part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(norm_from_partials_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, norm_out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_norm<double>(const double *, int64_t, double *, double *, hipStream_t);
+template int vec_norm<float>(const float *, int64_t, float *, double *, hipStream_t);
+
+template <typename T>
+int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,
+            SolveState *sd, double abs_tol, double rel_tol, hipStream_t st)
+{
+  hipLaunchKernelGGL(cg_init_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, b, q, r, p,
+                     inv, n, part_rr, part_rz0);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(cg_check0_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part_rr, RED_BLOCKS,
+                     sd, abs_tol, rel_tol);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int cg_init<double>(const double *, double *, double *, double *, const double *, int64_t,
+                             double *, double *, SolveState *, double, double, hipStream_t);
+template int cg_init<float>(const float *, float *, float *, float *, const float *, int64_t,
+                            float *, float *, SolveState *, double, double, hipStream_t);
+
+int cg_reduce_cell_dot(const double *part, int n, SolveState *sd, hipStream_t st)
+{
+  hipLaunchKernelGGL(reduce_cell_dot_kernel, dim3(1), dim3(1024), 0, st, part, n, sd);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template <typename T>
+int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *part_rr,
+            SolveState *sd, int it, int max_it, hipStream_t st)
+{
+  T *rz_new = part_rz[it & 1], *rz_old = part_rz[(it - 1) & 1];
+  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, x, r, p, q,
+                     inv, n, rz_old, rz_new, part_rr, RED_BLOCKS, sd);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, p, q, r,
+                     inv, n, rz_new, rz_old, part_rr, RED_BLOCKS, sd, it, max_it);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int cg_step<double>(double *, double *, double *, double *, const double *, int64_t,
+                             double *[2], double *, SolveState *, int, int, hipStream_t);
+template int cg_step<float>(float *, float *, float *, float *, const float *, int64_t, float *[2],
+                            float *, SolveState *, int, int, hipStream_t);
+
+template <typename T>
+int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h,
+                   hipStream_t st)
+{
+  hipLaunchKernelGGL(multidot_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, V, ld, m, w,
+                     n, part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(reduce_multi_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part, RED_BLOCKS,
+                     m, h);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int gmres_multidot<double>(const double *, int64_t, int, const double *, int64_t, double *,
+                                    double *, hipStream_t);
+template int gmres_multidot<float>(const float *, int64_t, int, const float *, int64_t, float *,
+                                   double *, hipStream_t);
+
+template <typename T>
+int gmres_multiaxpy(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n, T *part,
+                    double *norm_out, hipStream_t st)
+{
+  hipLaunchKernelGGL(multiaxpy_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, V, ld, m, h,
+                     w, n, part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(norm_from_partials_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, norm_out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int gmres_multiaxpy<double>(const double *, int64_t, int, const double *, double *,
+                                     int64_t, double *, double *, hipStream_t);
+template int gmres_multiaxpy<float>(const float *, int64_t, int, const double *, float *, int64_t,
+                                    float *, double *, hipStream_t);
+
+template <typename T>
+int vec_scale_dev(T *dst, const T *src, int64_t n, const double *s, int invert, hipStream_t st)
+{
+  hipLaunchKernelGGL(scale_kernel<T>, grid_for(n), dim3(VEC_THREADS), 0, st, dst, src, n, s, invert);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_scale_dev<double>(double *, const double *, int64_t, const double *, int,
+                                   hipStream_t);
+template int vec_scale_dev<float>(float *, const float *, int64_t, const double *, int, hipStream_t);
+
+template <typename T>
+int gmres_update(T *x, const T *V, int64_t ld, int m, const double *y, const T *inv, int64_t n,
+                 hipStream_t st)
+{
+  hipLaunchKernelGGL(gmres_update_kernel<T>, grid_for(n), dim3(VEC_THREADS), 0, st, x, V, ld, m, y,
+                     inv, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int gmres_update<double>(double *, const double *, int64_t, int, const double *,
+                                  const double *, int64_t, hipStream_t);
+template int gmres_update<float>(float *, const float *, int64_t, int, const double *,
+                                 const float *, int64_t, hipStream_t);
+
+template <typename T>
+int vec_precond(T *z, const T *v, const T *inv, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(precond_kernel<T>, grid_for(n), dim3(VEC_THREADS), 0, st, z, v, inv, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_precond<double>(double *, const double *, const double *, int64_t, hipStream_t);
+template int vec_precond<float>(float *, const float *, const float *, int64_t, hipStream_t);
+
+}  // namespace mf

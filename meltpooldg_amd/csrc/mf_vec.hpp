@@ -1,0 +1,45 @@
+// Vector-layer / Krylov kernel launchers (mf_vec.hip).
+#pragma once
+
+#include "mf_internal.hpp"
+
+namespace mf {
+
+constexpr int VEC_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;  // blocks of every reduction kernel (= partials per reduction)
+
+// Device-resident solver control block (ReductionControl state).
+struct SolveState {
+  int stop;        // 0 iterate, 1 success, 2 max iterations, 3 breakdown / NaN
+  int iterations;
+  int breakdown;
+  int pad;
+  double res0, tol, res;
+  double pAp;      // reduced p.(A p) of the current iteration
+};
+
+template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop = nullptr);
+template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
+int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipStream_t st);
+int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st);
+template <typename T> int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out, hipStream_t st);
+template <typename T> int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st);
+template <typename T> int vec_precond(T *z, const T *v, const T *inv, int64_t n, hipStream_t st);
+template <typename T> int vec_scale_dev(T *dst, const T *src, int64_t n, const double *s, int invert, hipStream_t st);
+
+template <typename T>
+int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,
+            SolveState *sd, double abs_tol, double rel_tol, hipStream_t st);
+int cg_reduce_cell_dot(const double *part, int n, SolveState *sd, hipStream_t st);
+template <typename T>
+int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *part_rr,
+            SolveState *sd, int it, int max_it, hipStream_t st);
+
+template <typename T>
+int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h, hipStream_t st);
+template <typename T>
+int gmres_multiaxpy(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n, T *part, double *norm_out, hipStream_t st);
+template <typename T>
+int gmres_update(T *x, const T *V, int64_t ld, int m, const double *y, const T *inv, int64_t n, hipStream_t st);
+
+}  // namespace mf

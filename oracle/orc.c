@@ -1,0 +1,1137 @@
+/*
+ * orc.c -- CPU oracle (test infrastructure only; see orc.h for the contract and citations).
+ *
+ * Structure: one generic cell routine per operator, dimension-generic tensor contractions on
+ * K^dim arrays (K = p+1 nodes and quadrature points per direction), a colour-by-parity OpenMP
+ * cell loop for the scatter, and the Krylov solvers of the reference's LinearSolver::solve.
+ * Gradients are evaluated directly, (DS (x) S (x) S) u etc., not by collocation, so that this
+ * path differs from the device kernels in its rounding.
+ */
+#include "orc.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define KMAX 8
+#define NLOC_MAX (KMAX * KMAX * KMAX)
+
+/* ------------------------------------------------------------------------------------------ */
+/* 1D basis                                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Legendre P_n(z) and P_{n-1}(z) by the three-term recurrence. */
+static void legendre(int n, double z, double *pn, double *pnm1)
+{
+  double p0 = 1.0, p1 = 0.0;
+  for (int j = 1; j <= n; ++j) {
+    const double p2 = p1;
+    p1 = p0;
+    p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p2) / j;
+  }
+  *pn = p0;
+  *pnm1 = p1;
+}
+
+/* QGauss(n) on [0,1]: Newton iteration on P_n (deal.II QGauss, used as QGauss(p+1) by the
+ * reference: source/core/finite_element_data.cpp:85-89). */
+void orc_gauss(int n, double *x, double *w)
+{
+  for (int i = 0; i < n; ++i) {
+    double z = cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pn, pm, dp;
+    for (int it = 0; it < 100; ++it) {
+      legendre(n, z, &pn, &pm);
+      dp = n * (z * pn - pm) / (z * z - 1.0);
+      const double dz = pn / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-17) break;
+    }
+    legendre(n, z, &pn, &pm);
+    dp = n * (z * pn - pm) / (z * z - 1.0);
+    /* z runs from +1 down to -1; store ascending on [0,1] */
+    x[i] = 0.5 * (1.0 - z);
+    w[i] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+}
+
+/* QGaussLobatto(n) on [0,1]: end points plus the roots of P'_{n-1} (FE_Q support points). */
+void orc_gauss_lobatto(int n, double *x)
+{
+  const int p = n - 1;
+  x[0] = 0.0;
+  x[n - 1] = 1.0;
+  for (int i = 1; i < p; ++i) {
+    double z = cos(M_PI * i / p);
+    for (int it = 0; it < 100; ++it) {
+      double pn, pm;
+      legendre(p, z, &pn, &pm);
+      const double d1 = p * (z * pn - pm) / (z * z - 1.0);
+      const double d2 = (2.0 * z * d1 - p * (p + 1.0) * pn) / (1.0 - z * z);
+      const double dz = d1 / d2;
+      z -= dz;
+      if (fabs(dz) < 1e-17) break;
+    }
+    x[i] = 0.5 * (1.0 - z);
+  }
+  /* symmetrise exactly like a symmetric point set */
+  for (int i = 0; i < n / 2; ++i) {
+    const double a = 0.5 * (x[i] + (1.0 - x[n - 1 - i]));
+    x[i] = a;
+    x[n - 1 - i] = 1.0 - a;
+  }
+  if (n % 2 == 1) x[n / 2] = 0.5;
+}
+
+void orc_lagrange(int n, const double *nodes, double x, double *val, double *der)
+{
+  for (int j = 0; j < n; ++j) {
+    double v = 1.0, d = 0.0;
+    for (int m = 0; m < n; ++m) {
+      if (m == j) continue;
+      const double inv = 1.0 / (nodes[j] - nodes[m]);
+      d = d * (x - nodes[m]) * inv + v * inv;
+      v *= (x - nodes[m]) * inv;
+    }
+    val[j] = v;
+    der[j] = d;
+  }
+}
+
+/* S[q][j] = l_j(x_q), DS[q][j] = l_j'(x_q), Dq[q][r] = L_r'(x_q) with L the Lagrange basis on the
+ * Gauss points (collocation derivative), w = Gauss weights.  K = degree + 1. */
+void orc_shape_matrices(int degree, double *S, double *DS, double *Dq, double *w)
+{
+  const int K = degree + 1;
+  double gl[KMAX], xq[KMAX], v[KMAX], d[KMAX];
+  orc_gauss_lobatto(K, gl);
+  orc_gauss(K, xq, w);
+  for (int q = 0; q < K; ++q) {
+    orc_lagrange(K, gl, xq[q], v, d);
+    for (int j = 0; j < K; ++j) {
+      S[q * K + j] = v[j];
+      DS[q * K + j] = d[j];
+    }
+    if (Dq) {
+      orc_lagrange(K, xq, xq[q], v, d);
+      for (int j = 0; j < K; ++j) Dq[q * K + j] = d[j];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Mesh                                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  int dim, K, p, nloc;
+  long NN[3], stride[3], ncell[3];
+  double h[3], lo[3];
+  double S[KMAX * KMAX], DS[KMAX * KMAX], wq[KMAX], xq[KMAX], gl[KMAX];
+} geo_t;
+
+static void geo_init(const orc_mesh *m, geo_t *g)
+{
+  g->dim = m->dim;
+  g->p = m->degree;
+  g->K = m->degree + 1;
+  g->nloc = 1;
+  long s = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < m->dim) {
+      g->ncell[d] = m->ncell[d];
+      g->NN[d] = (long)m->ncell[d] * m->degree + 1;
+      g->h[d] = (m->hi[d] - m->lo[d]) / m->ncell[d];
+      g->lo[d] = m->lo[d];
+      g->nloc *= g->K;
+    } else {
+      g->ncell[d] = 1;
+      g->NN[d] = 1;
+      g->h[d] = 1.0;
+      g->lo[d] = 0.0;
+    }
+    g->stride[d] = s;
+    s *= g->NN[d];
+  }
+  orc_shape_matrices(m->degree, g->S, g->DS, NULL, g->wq);
+  double w[KMAX];
+  orc_gauss(g->K, g->xq, w);
+  orc_gauss_lobatto(g->K, g->gl);
+}
+
+long orc_n_dofs(const orc_mesh *m)
+{
+  long n = 1;
+  for (int d = 0; d < m->dim; ++d) n *= (long)m->ncell[d] * m->degree + 1;
+  return n;
+}
+
+long orc_n_cells(const orc_mesh *m)
+{
+  long n = 1;
+  for (int d = 0; d < m->dim; ++d) n *= m->ncell[d];
+  return n;
+}
+
+static double node_coord(const geo_t *g, int d, long i)
+{
+  long c = i / g->p;
+  if (c >= g->ncell[d]) c = g->ncell[d] - 1;
+  const long a = i - c * g->p;
+  return g->lo[d] + g->h[d] * ((double)c + g->gl[a]);
+}
+
+void orc_node_coords(const orc_mesh *m, double *xyz)
+{
+  geo_t g;
+  geo_init(m, &g);
+  const long n = orc_n_dofs(m);
+  for (long idx = 0; idx < n; ++idx) {
+    long r = idx;
+    for (int d = 0; d < 3; ++d) {
+      const long i = r % g.NN[d];
+      r /= g.NN[d];
+      xyz[3 * idx + d] = d < m->dim ? node_coord(&g, d, i) : 0.0;
+    }
+  }
+}
+
+void orc_boundary_mask(const orc_mesh *m, uint8_t *mask)
+{
+  geo_t g;
+  geo_init(m, &g);
+  const long n = orc_n_dofs(m);
+  for (long idx = 0; idx < n; ++idx) {
+    long r = idx;
+    uint8_t b = 0;
+    for (int d = 0; d < m->dim; ++d) {
+      const long i = r % g.NN[d];
+      r /= g.NN[d];
+      if (i == 0 || i == g.NN[d] - 1) b = 1;
+    }
+    mask[idx] = b;
+  }
+}
+
+uint64_t orc_splitmix64(uint64_t x)
+{
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* U(-1,1) keyed by the node's lattice coordinates (i0, i1, i2) and the seed; identical for any
+ * DoF numbering. */
+void orc_random_field(const orc_mesh *m, uint64_t seed, double *out)
+{
+  geo_t g;
+  geo_init(m, &g);
+  const long n = orc_n_dofs(m);
+  const uint64_t s = orc_splitmix64(seed);
+  for (long idx = 0; idx < n; ++idx) {
+    long r = idx;
+    uint64_t key = 0;
+    for (int d = 0; d < 3; ++d) {
+      const uint64_t i = (uint64_t)(r % g.NN[d]);
+      r /= g.NN[d];
+      key |= i << (21 * d);
+    }
+    const uint64_t z = orc_splitmix64(key ^ s);
+    out[idx] = 2.0 * ((double)(z >> 11) * 0x1.0p-53) - 1.0;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Tensor contractions on K^dim arrays                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* out_i = sum_j M[i][j] in_j along direction dir (transpose: out_j = sum_i M[i][j] in_i). */
+static void contract(const geo_t *g, int dir, const double *M, int transpose, const double *in,
+                     double *out)
+{
+  const int K = g->K;
+  long stride = 1;
+  for (int d = 0; d < dir; ++d) stride *= K;
+  const long outer = g->nloc / (stride * K);
+  for (long o = 0; o < outer; ++o)
+    for (long s = 0; s < stride; ++s) {
+      const long base = o * stride * K + s;
+      for (int i = 0; i < K; ++i) {
+        double acc = 0.0;
+        for (int j = 0; j < K; ++j)
+          acc += (transpose ? M[j * K + i] : M[i * K + j]) * in[base + j * stride];
+        out[base + i * stride] = acc;
+      }
+    }
+}
+
+/* Apply one 1D matrix per direction: mats[d] along direction d. */
+static void tensor_apply(const geo_t *g, const double *const *mats, int transpose,
+                         const double *in, double *out, double *t1, double *t2)
+{
+  const double *src = in;
+  for (int d = 0; d < g->dim; ++d) {
+    double *dst = (d == g->dim - 1) ? out : ((d % 2 == 0) ? t1 : t2);
+    contract(g, d, mats[d], transpose, src, dst);
+    src = dst;
+  }
+}
+
+typedef struct {
+  double t1[NLOC_MAX], t2[NLOC_MAX], t3[NLOC_MAX];
+} work_t;
+
+/* values and physical gradients at the quadrature points */
+static void eval_vg(const geo_t *g, const double *u, double *val, double *grad, work_t *w)
+{
+  const double *mats[3];
+  for (int d = 0; d < 3; ++d) mats[d] = g->S;
+  tensor_apply(g, mats, 0, u, val, w->t1, w->t2);
+  for (int e = 0; e < g->dim; ++e) {
+    for (int d = 0; d < 3; ++d) mats[d] = (d == e) ? g->DS : g->S;
+    double *ge = grad + (long)e * g->nloc;
+    tensor_apply(g, mats, 0, u, ge, w->t1, w->t2);
+    const double s = 1.0 / g->h[e];
+    for (int q = 0; q < g->nloc; ++q) ge[q] *= s;
+  }
+}
+
+static void eval_v(const geo_t *g, const double *u, double *val, work_t *w)
+{
+  const double *mats[3] = {g->S, g->S, g->S};
+  tensor_apply(g, mats, 0, u, val, w->t1, w->t2);
+}
+
+/* out = sum_q val_q phi(q) + sum_q flux_q . grad phi(q); val and flux already carry JxW */
+static void integrate(const geo_t *g, const double *val, const double *flux, double *out,
+                      work_t *w)
+{
+  const double *mats[3] = {g->S, g->S, g->S};
+  tensor_apply(g, mats, 1, val, out, w->t1, w->t2);
+  if (!flux) return;
+  for (int e = 0; e < g->dim; ++e) {
+    for (int d = 0; d < 3; ++d) mats[d] = (d == e) ? g->DS : g->S;
+    tensor_apply(g, mats, 1, flux + (long)e * g->nloc, w->t3, w->t1, w->t2);
+    const double s = 1.0 / g->h[e];
+    for (int i = 0; i < g->nloc; ++i) out[i] += s * w->t3[i];
+  }
+}
+
+static void cell_coords(const geo_t *g, long cell, long *c)
+{
+  long r = cell;
+  for (int d = 0; d < 3; ++d) {
+    c[d] = r % g->ncell[d];
+    r /= g->ncell[d];
+  }
+}
+
+static void cell_dofs(const geo_t *g, long cell, long *idx)
+{
+  long c[3];
+  cell_coords(g, cell, c);
+  const int K = g->K;
+  for (int l = 0; l < g->nloc; ++l) {
+    int a[3] = {l % K, (l / K) % K, l / (K * K)};
+    long gi = 0;
+    for (int d = 0; d < g->dim; ++d) gi += (c[d] * g->p + a[d]) * g->stride[d];
+    idx[l] = gi;
+  }
+}
+
+static double cell_volume_weight(const geo_t *g, int q)
+{
+  const int K = g->K;
+  double w = 1.0;
+  int r = q;
+  for (int d = 0; d < g->dim; ++d) {
+    w *= g->wq[r % K] * g->h[d];
+    r /= K;
+  }
+  return w;
+}
+
+static void q_point(const geo_t *g, long cell, int q, double *x)
+{
+  long c[3];
+  cell_coords(g, cell, c);
+  const int K = g->K;
+  int r = q;
+  for (int d = 0; d < 3; ++d) {
+    x[d] = d < g->dim ? g->lo[d] + g->h[d] * ((double)c[d] + g->xq[r % K]) : 0.0;
+    r /= K;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Material law (reference slot: get_material_parameters_and_derivatives,                      */
+/* source/heat/heat_diffuse_operator.cpp:1167-1210)                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+static void material(const orc_heat_params *hp, double T, double *rc, double *k, double *drc,
+                     double *dk)
+{
+  *rc = hp->rho_cp;
+  *drc = 0.0;
+  if (hp->law == ORC_LAW_TANH) {
+    const double x = (T - hp->tanh_Tc) / hp->tanh_width;
+    const double t = tanh(x);
+    const double c = cosh(x);
+    *k = hp->k + hp->tanh_amp * t;
+    *dk = hp->tanh_amp / (hp->tanh_width * c * c);
+  } else {
+    *k = hp->k;
+    *dk = 0.0;
+  }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Cell operations                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  double u[NLOC_MAX], out[NLOC_MAX];
+  double val[NLOC_MAX], grad[3 * NLOC_MAX];
+  double Tv[NLOC_MAX], Tg[3 * NLOC_MAX], To[NLOC_MAX];
+  double wv[3][NLOC_MAX];
+  double fval[NLOC_MAX], flux[3 * NLOC_MAX];
+  double tmp[NLOC_MAX];
+  long idx[NLOC_MAX];
+  work_t w;
+} cell_ws_t;
+
+static int has_vel(const orc_operator *op) { return op->vel_kind != ORC_VEL_NONE; }
+
+/* velocity at the quadrature points of a cell */
+static void eval_velocity(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  const long n = orc_n_dofs(&op->mesh);
+  for (int e = 0; e < 3; ++e)
+    for (int q = 0; q < g->nloc; ++q) ws->wv[e][q] = 0.0;
+  if (op->vel_kind == ORC_VEL_NODAL) {
+    for (int e = 0; e < g->dim; ++e) {
+      for (int l = 0; l < g->nloc; ++l) ws->tmp[l] = op->vel[e * n + ws->idx[l]];
+      eval_v(g, ws->tmp, ws->wv[e], &ws->w);
+    }
+  } else if (op->vel_kind == ORC_VEL_VORTEX) {
+    /* advection_diffusion_vortex.hpp PrescribedVelocityField: w_0 = s*x_{dim-1},
+     * w_{dim-1} = -s*x_0 */
+    for (int q = 0; q < g->nloc; ++q) {
+      double x[3];
+      q_point(g, cell, q, x);
+      const double s = op->advec.vel_scale;
+      double w0 = s * x[g->dim - 1];
+      double wl = -s * x[0];
+      ws->wv[0][q] = w0;
+      ws->wv[g->dim - 1][q] = wl;
+    }
+  }
+}
+
+/* gather with constraints (read_dof_values: Dirichlet entries read as zero) */
+static void gather(const orc_operator *op, const geo_t *g, const double *src, cell_ws_t *ws,
+                   int use_mask)
+{
+  for (int l = 0; l < g->nloc; ++l) {
+    const long gi = ws->idx[l];
+    ws->u[l] = (use_mask && op->mask && op->mask[gi]) ? 0.0 : src[gi];
+  }
+}
+
+static void scatter(const orc_operator *op, const geo_t *g, double *dst, cell_ws_t *ws,
+                    int use_mask)
+{
+  for (int l = 0; l < g->nloc; ++l) {
+    const long gi = ws->idx[l];
+    if (use_mask && op->mask && op->mask[gi]) continue;
+    dst[gi] += ws->out[l];
+  }
+}
+
+/* heat Jacobian (reference tangent_local_cell_operation, heat_diffuse_operator.cpp:955-1086) */
+static void heat_tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  const orc_heat_params *hp = &op->heat;
+  const int nq = g->nloc, dim = g->dim;
+  const double kc = op->k_cell ? op->k_cell[cell] : 1.0;
+  const int td = hp->t_dependent;
+  const int hv = has_vel(op);
+  eval_vg(g, ws->u, ws->val, ws->grad, &ws->w);
+  const int needT = hp->law != ORC_LAW_CONSTANT || td;
+  if (needT) {
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = op->T[ws->idx[l]];
+    eval_vg(g, ws->tmp, ws->Tv, ws->Tg, &ws->w);
+  }
+  if (td) {
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = op->T_old[ws->idx[l]];
+    eval_v(g, ws->tmp, ws->To, &ws->w);
+  }
+  if (hv) eval_velocity(op, g, cell, ws);
+  const double dt_inv = 1.0 / hp->dt;
+  for (int q = 0; q < nq; ++q) {
+    double rc, k, drc, dk;
+    material(hp, needT ? ws->Tv[q] : 0.0, &rc, &k, &drc, &dk);
+    k *= kc;
+    dk *= kc;
+    const double u = ws->val[q];
+    double val = dt_inv * rc * u;
+    double gr[3];
+    for (int e = 0; e < dim; ++e) gr[e] = k * ws->grad[e * nq + q];
+    if (hv) {
+      double s = 0.0;
+      for (int e = 0; e < dim; ++e) s += ws->grad[e * nq + q] * ws->wv[e][q];
+      val += rc * s;
+    }
+    if (td) {
+      val += dt_inv * drc * (ws->Tv[q] - ws->To[q]) * u;
+      for (int e = 0; e < dim; ++e) gr[e] += dk * ws->Tg[e * nq + q] * u;
+    }
+    if (hv && td) {
+      double s = 0.0;
+      for (int e = 0; e < dim; ++e) s += ws->Tg[e * nq + q] * ws->wv[e][q];
+      val += drc * s * u;
+    }
+    const double jxw = cell_volume_weight(g, q);
+    ws->fval[q] = jxw * val;
+    for (int e = 0; e < dim; ++e) ws->flux[e * nq + q] = jxw * gr[e];
+  }
+  integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+}
+
+/* advection-diffusion tangent (advection_diffusion_operator.cpp:448-495, no SUPG) */
+static void advec_tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  const orc_advec_params *ap = &op->advec;
+  const int nq = g->nloc, dim = g->dim;
+  eval_vg(g, ws->u, ws->val, ws->grad, &ws->w);
+  eval_velocity(op, g, cell, ws);
+  const double m = ap->mass_factor / ap->dt;
+  for (int q = 0; q < nq; ++q) {
+    double vg = 0.0;
+    for (int e = 0; e < dim; ++e) vg += ws->wv[e][q] * ws->grad[e * nq + q];
+    const double jxw = cell_volume_weight(g, q);
+    ws->fval[q] = jxw * (m * ws->val[q] + ap->theta * vg);
+    for (int e = 0; e < dim; ++e)
+      ws->flux[e * nq + q] = jxw * ap->theta * ap->diffusivity * ws->grad[e * nq + q];
+  }
+  integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+}
+
+static void tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  if (op->kind == ORC_OP_HEAT)
+    heat_tangent_cell(op, g, cell, ws);
+  else
+    advec_tangent_cell(op, g, cell, ws);
+}
+
+static int nthreads_of(const orc_operator *op)
+{
+#ifdef _OPENMP
+  return op->nthreads > 0 ? op->nthreads : omp_get_max_threads();
+#else
+  (void)op;
+  return 1;
+#endif
+}
+
+/* Cell loop coloured by the parity of the cell coordinates: cells of one colour share no node,
+ * so the scatter needs no synchronisation. */
+typedef void (*cell_fn)(const orc_operator *, const geo_t *, long, cell_ws_t *, void *);
+
+static void cell_loop(const orc_operator *op, const geo_t *g, cell_fn fn, void *user)
+{
+  const int ncol = 1 << g->dim;
+  const long ncells = orc_n_cells(&op->mesh);
+  const int nt = nthreads_of(op);
+  for (int col = 0; col < ncol; ++col) {
+#pragma omp parallel num_threads(nt)
+    {
+      cell_ws_t *ws = (cell_ws_t *)malloc(sizeof(cell_ws_t));
+#pragma omp for schedule(static)
+      for (long cell = 0; cell < ncells; ++cell) {
+        long c[3];
+        cell_coords(g, cell, c);
+        int cc = 0;
+        for (int d = 0; d < g->dim; ++d) cc |= (int)(c[d] & 1) << d;
+        if (cc != col) continue;
+        fn(op, g, cell, ws, user);
+      }
+      free(ws);
+    }
+  }
+}
+
+typedef struct {
+  const double *src;
+  double *dst;
+  const double *q_s;
+} io_t;
+
+static void vmult_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+{
+  io_t *io = (io_t *)u;
+  cell_dofs(g, cell, ws->idx);
+  gather(op, g, io->src, ws, op->use_mask);
+  tangent_cell(op, g, cell, ws);
+  scatter(op, g, io->dst, ws, op->use_mask);
+}
+
+void orc_vmult(const orc_operator *op, const double *src, double *dst)
+{
+  geo_t g;
+  geo_init(&op->mesh, &g);
+  const long n = orc_n_dofs(&op->mesh);
+  memset(dst, 0, sizeof(double) * n);
+  io_t io = {src, dst, NULL};
+  cell_loop(op, &g, vmult_cell, &io);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Brute force: dense element matrices by textbook quadrature                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* phi_l(q) and its physical gradient by direct products of 1D functions */
+static void basis_at(const geo_t *g, int l, int q, double *phi, double *gphi)
+{
+  const int K = g->K;
+  int a[3] = {l % K, (l / K) % K, l / (K * K)};
+  int b[3] = {q % K, (q / K) % K, q / (K * K)};
+  double v[3], d[3];
+  for (int e = 0; e < 3; ++e) {
+    if (e < g->dim) {
+      v[e] = g->S[b[e] * K + a[e]];
+      d[e] = g->DS[b[e] * K + a[e]] / g->h[e];
+    } else {
+      v[e] = 1.0;
+      d[e] = 0.0;
+    }
+  }
+  *phi = v[0] * v[1] * v[2];
+  gphi[0] = d[0] * v[1] * v[2];
+  gphi[1] = v[0] * d[1] * v[2];
+  gphi[2] = v[0] * v[1] * d[2];
+}
+
+/* coefficients of the bilinear form at q:
+ *   a(u,v) = sum_q JxW [ c u v + (bu . grad u) v + kq grad u . grad v + u (bg . grad v) ] */
+typedef struct {
+  double c, k, bu[3], bg[3];
+} qcoef_t;
+
+static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, const long *idx,
+                           qcoef_t *qc)
+{
+  const int nq = g->nloc, dim = g->dim;
+  const long n = orc_n_dofs(&op->mesh);
+  for (int q = 0; q < nq; ++q) {
+    double phi[NLOC_MAX], gphi[NLOC_MAX][3];
+    for (int l = 0; l < nq; ++l) basis_at(g, l, q, &phi[l], gphi[l]);
+    double w[3] = {0, 0, 0};
+    if (op->vel_kind == ORC_VEL_NODAL) {
+      for (int e = 0; e < dim; ++e)
+        for (int l = 0; l < nq; ++l) w[e] += op->vel[e * n + idx[l]] * phi[l];
+    } else if (op->vel_kind == ORC_VEL_VORTEX) {
+      double x[3];
+      q_point(g, cell, q, x);
+      w[0] = op->advec.vel_scale * x[dim - 1];
+      w[dim - 1] = -op->advec.vel_scale * x[0];
+    }
+    qcoef_t *c = &qc[q];
+    memset(c, 0, sizeof(*c));
+    if (op->kind == ORC_OP_HEAT) {
+      const orc_heat_params *hp = &op->heat;
+      double T = 0, To = 0, Tg[3] = {0, 0, 0};
+      const int needT = hp->law != ORC_LAW_CONSTANT || hp->t_dependent;
+      if (needT)
+        for (int l = 0; l < nq; ++l) {
+          T += op->T[idx[l]] * phi[l];
+          for (int e = 0; e < 3; ++e) Tg[e] += op->T[idx[l]] * gphi[l][e];
+        }
+      if (hp->t_dependent)
+        for (int l = 0; l < nq; ++l) To += op->T_old[idx[l]] * phi[l];
+      double rc, k, drc, dk;
+      material(hp, T, &rc, &k, &drc, &dk);
+      const double kc = op->k_cell ? op->k_cell[cell] : 1.0;
+      k *= kc;
+      dk *= kc;
+      c->c = rc / hp->dt;
+      c->k = k;
+      if (op->vel_kind != ORC_VEL_NONE)
+        for (int e = 0; e < 3; ++e) c->bu[e] = rc * w[e];
+      if (hp->t_dependent) {
+        c->c += drc / hp->dt * (T - To);
+        for (int e = 0; e < 3; ++e) c->bg[e] = dk * Tg[e];
+        if (op->vel_kind != ORC_VEL_NONE)
+          c->c += drc * (Tg[0] * w[0] + Tg[1] * w[1] + Tg[2] * w[2]);
+      }
+    } else {
+      const orc_advec_params *ap = &op->advec;
+      c->c = ap->mass_factor / ap->dt;
+      c->k = ap->theta * ap->diffusivity;
+      for (int e = 0; e < 3; ++e) c->bu[e] = ap->theta * w[e];
+    }
+    const double jxw = cell_volume_weight(g, q);
+    c->c *= jxw;
+    c->k *= jxw;
+    for (int e = 0; e < 3; ++e) {
+      c->bu[e] *= jxw;
+      c->bg[e] *= jxw;
+    }
+  }
+}
+
+void orc_local_matrix(const orc_operator *op, long cell, double *A)
+{
+  geo_t g;
+  geo_init(&op->mesh, &g);
+  const int nq = g.nloc;
+  long idx[NLOC_MAX];
+  cell_dofs(&g, cell, idx);
+  qcoef_t *qc = (qcoef_t *)malloc(sizeof(qcoef_t) * nq);
+  q_coefficients(op, &g, cell, idx, qc);
+  memset(A, 0, sizeof(double) * nq * nq);
+  for (int q = 0; q < nq; ++q) {
+    double phi[NLOC_MAX], gphi[NLOC_MAX][3];
+    for (int l = 0; l < nq; ++l) basis_at(&g, l, q, &phi[l], gphi[l]);
+    const qcoef_t *c = &qc[q];
+    for (int i = 0; i < nq; ++i)
+      for (int j = 0; j < nq; ++j) {
+        const double gg = gphi[j][0] * gphi[i][0] + gphi[j][1] * gphi[i][1] + gphi[j][2] * gphi[i][2];
+        const double bu = c->bu[0] * gphi[j][0] + c->bu[1] * gphi[j][1] + c->bu[2] * gphi[j][2];
+        const double bg = c->bg[0] * gphi[i][0] + c->bg[1] * gphi[i][1] + c->bg[2] * gphi[i][2];
+        A[i * nq + j] += c->c * phi[j] * phi[i] + bu * phi[i] + c->k * gg + phi[j] * bg;
+      }
+  }
+  free(qc);
+}
+
+static void dense_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+{
+  io_t *io = (io_t *)u;
+  const int nq = g->nloc;
+  cell_dofs(g, cell, ws->idx);
+  gather(op, g, io->src, ws, op->use_mask);
+  double *A = (double *)malloc(sizeof(double) * nq * nq);
+  orc_local_matrix(op, cell, A);
+  for (int i = 0; i < nq; ++i) {
+    double s = 0.0;
+    for (int j = 0; j < nq; ++j) s += A[i * nq + j] * ws->u[j];
+    ws->out[i] = s;
+  }
+  free(A);
+  scatter(op, g, io->dst, ws, op->use_mask);
+}
+
+void orc_vmult_dense(const orc_operator *op, const double *src, double *dst)
+{
+  geo_t g;
+  geo_init(&op->mesh, &g);
+  const long n = orc_n_dofs(&op->mesh);
+  memset(dst, 0, sizeof(double) * n);
+  io_t io = {src, dst, NULL};
+  cell_loop(op, &g, dense_cell, &io);
+}
+
+/* diagonal of the constrained operator: sum over cells of the local diagonal entries of the
+ * bilinear form; constrained rows stay 0 (MatrixFreeTools::compute_diagonal semantics) */
+static void diag_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+{
+  io_t *io = (io_t *)u;
+  const int nq = g->nloc;
+  cell_dofs(g, cell, ws->idx);
+  qcoef_t *qc = (qcoef_t *)malloc(sizeof(qcoef_t) * nq);
+  q_coefficients(op, g, cell, ws->idx, qc);
+  for (int i = 0; i < nq; ++i) ws->out[i] = 0.0;
+  for (int q = 0; q < nq; ++q) {
+    const qcoef_t *c = &qc[q];
+    for (int i = 0; i < nq; ++i) {
+      double phi, gp[3];
+      basis_at(g, i, q, &phi, gp);
+      const double gg = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
+      const double bu = c->bu[0] * gp[0] + c->bu[1] * gp[1] + c->bu[2] * gp[2];
+      const double bg = c->bg[0] * gp[0] + c->bg[1] * gp[1] + c->bg[2] * gp[2];
+      ws->out[i] += c->c * phi * phi + bu * phi + c->k * gg + phi * bg;
+    }
+  }
+  free(qc);
+  scatter(op, g, io->dst, ws, op->use_mask);
+}
+
+void orc_diagonal(const orc_operator *op, double *diag)
+{
+  geo_t g;
+  geo_init(&op->mesh, &g);
+  const long n = orc_n_dofs(&op->mesh);
+  memset(diag, 0, sizeof(double) * n);
+  io_t io = {NULL, diag, NULL};
+  cell_loop(op, &g, diag_cell, &io);
+}
+
+/* heat: |d| > 1e-10 ? 1/d : 1  (heat_diffuse_operator.cpp:282-283)
+ * advection: |d| > 1e-14 max(1, |d|_inf) ? 1/d : 1  (advection_diffusion_operator.cpp:434-436) */
+void orc_invert_diagonal(int kind, long n, double *diag)
+{
+  double thr = 1.0e-10;
+  if (kind == ORC_OP_ADVEC) {
+    double mx = 1.0;
+    for (long i = 0; i < n; ++i) mx = fmax(mx, fabs(diag[i]));
+    thr = 1.0e-14 * mx;
+  }
+  for (long i = 0; i < n; ++i) diag[i] = fabs(diag[i]) > thr ? 1.0 / diag[i] : 1.0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Right-hand sides                                                                            */
+/* ------------------------------------------------------------------------------------------ */
+
+/* heat: -R(T) with values (rho cp/dt)(T - T_old) - q_s (+ rho cp grad T . w) and gradients k grad T
+ * (rhs_cell_loop, heat_diffuse_operator.cpp:393-529).  src = T_old.
+ * advection: values phi/dt - (1-theta) w.grad phi, gradients -(1-theta) D grad phi
+ * (create_rhs, advection_diffusion_operator.cpp:274-358).  src = phi_old. */
+static void rhs_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+{
+  io_t *io = (io_t *)u;
+  const int nq = g->nloc, dim = g->dim;
+  const long n = orc_n_dofs(&op->mesh);
+  (void)n;
+  cell_dofs(g, cell, ws->idx);
+  if (op->kind == ORC_OP_HEAT) {
+    const orc_heat_params *hp = &op->heat;
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = op->T[ws->idx[l]];
+    eval_vg(g, ws->tmp, ws->Tv, ws->Tg, &ws->w);
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = io->src[ws->idx[l]];
+    eval_v(g, ws->tmp, ws->To, &ws->w);
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = io->q_s ? io->q_s[ws->idx[l]] : 0.0;
+    eval_v(g, ws->tmp, ws->val, &ws->w);
+    if (has_vel(op)) eval_velocity(op, g, cell, ws);
+    const double kc = op->k_cell ? op->k_cell[cell] : 1.0;
+    for (int q = 0; q < nq; ++q) {
+      double rc, k, drc, dk;
+      material(hp, ws->Tv[q], &rc, &k, &drc, &dk);
+      k *= kc;
+      double val = rc / hp->dt * (ws->Tv[q] - ws->To[q]) - ws->val[q];
+      if (has_vel(op)) {
+        double s = 0.0;
+        for (int e = 0; e < dim; ++e) s += ws->Tg[e * nq + q] * ws->wv[e][q];
+        val += rc * s;
+      }
+      const double jxw = cell_volume_weight(g, q);
+      ws->fval[q] = -jxw * val;
+      for (int e = 0; e < dim; ++e) ws->flux[e * nq + q] = -jxw * k * ws->Tg[e * nq + q];
+    }
+  } else {
+    const orc_advec_params *ap = &op->advec;
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = io->src[ws->idx[l]];
+    eval_vg(g, ws->tmp, ws->val, ws->grad, &ws->w);
+    eval_velocity(op, g, cell, ws);
+    for (int q = 0; q < nq; ++q) {
+      double vg = 0.0;
+      for (int e = 0; e < dim; ++e) vg += ws->wv[e][q] * ws->grad[e * nq + q];
+      const double jxw = cell_volume_weight(g, q);
+      ws->fval[q] = jxw * (ws->val[q] / ap->dt - (1.0 - ap->theta) * vg);
+      for (int e = 0; e < dim; ++e)
+        ws->flux[e * nq + q] =
+            -jxw * (1.0 - ap->theta) * ap->diffusivity * ws->grad[e * nq + q];
+    }
+  }
+  integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+  scatter(op, g, io->dst, ws, op->use_mask);
+}
+
+void orc_create_rhs(const orc_operator *op, const double *src, const double *q_s, double *dst)
+{
+  geo_t g;
+  geo_init(&op->mesh, &g);
+  io_t io = {src, dst, q_s};
+  cell_loop(op, &g, rhs_cell, &io);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Krylov solvers (LinearSolver::solve, linear_solver.hpp:22-134; ReductionControl semantics:   */
+/* success when |r| <= max(abs_tol, rel_tol*|r_0|), failure when the step count reaches max_it) */
+/* ------------------------------------------------------------------------------------------ */
+
+static double dot(long n, const double *a, const double *b)
+{
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (long i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+typedef struct {
+  double tol;
+  int max_it;
+} control_t;
+
+/* 0 iterate, 1 success, 2 failure */
+static int check(control_t *c, int step, double res, double abs_tol, double rel_tol)
+{
+  if (step == 0) c->tol = fmax(abs_tol, rel_tol * res);
+  if (res <= c->tol) return 1;
+  if (step >= c->max_it || isnan(res)) return 2;
+  return 0;
+}
+
+static void precond(long n, const double *inv_diag, const double *r, double *z)
+{
+  if (inv_diag) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; ++i) z[i] = inv_diag[i] * r[i];
+  } else {
+    memcpy(z, r, sizeof(double) * n);
+  }
+}
+
+static int solve_cg(const orc_operator *op, long n, int max_it, double abs_tol, double rel_tol,
+                    const double *inv_diag, const double *b, double *x, int *iters, double *fres)
+{
+  double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *p = malloc(sizeof(double) * n), *q = malloc(sizeof(double) * n);
+  control_t c = {0.0, max_it};
+  orc_vmult(op, x, q);
+  for (long i = 0; i < n; ++i) r[i] = b[i] - q[i];
+  double res = sqrt(dot(n, r, r));
+  int state = check(&c, 0, res, abs_tol, rel_tol);
+  int it = 0;
+  double rz_old = 0.0;
+  while (state == 0) {
+    ++it;
+    precond(n, inv_diag, r, z);
+    const double rz = dot(n, r, z);
+    if (it == 1)
+      memcpy(p, z, sizeof(double) * n);
+    else {
+      const double beta = rz / rz_old;
+      for (long i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    rz_old = rz;
+    orc_vmult(op, p, q);
+    const double alpha = rz / dot(n, p, q);
+    for (long i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    res = sqrt(dot(n, r, r));
+    state = check(&c, it, res, abs_tol, rel_tol);
+  }
+  *iters = it;
+  *fres = res;
+  free(r);
+  free(z);
+  free(p);
+  free(q);
+  return state == 1 ? ORC_SUCCESS : ORC_NO_CONVERGENCE;
+}
+
+/* GMRES(m) with right preconditioning, classical Gram-Schmidt with one re-orthogonalisation
+ * pass, Givens rotations and the estimated residual |gamma_{j+1}| as convergence measure. */
+static int solve_gmres(const orc_operator *op, long n, int m, int max_it, double abs_tol,
+                       double rel_tol, const double *inv_diag, const double *b, double *x,
+                       int *iters, double *fres)
+{
+  double *V = malloc(sizeof(double) * n * (m + 1));
+  double *w = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n);
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+  double *gam = malloc(sizeof(double) * (m + 1)), *y = malloc(sizeof(double) * m);
+  double *hcol = malloc(sizeof(double) * (m + 1));
+  control_t c = {0.0, max_it};
+  int acc = 0, state = 0;
+  double res = 0.0;
+  for (;;) {
+    orc_vmult(op, x, w);
+    for (long i = 0; i < n; ++i) V[i] = b[i] - w[i];
+    double rho = sqrt(dot(n, V, V));
+    res = rho;
+    state = check(&c, acc, rho, abs_tol, rel_tol);
+    if (state != 0) break;
+    for (long i = 0; i < n; ++i) V[i] /= rho;
+    gam[0] = rho;
+    int j = 0;
+    for (; j < m && state == 0; ++j) {
+      ++acc;
+      double *vj = V + (long)j * n, *vn = V + (long)(j + 1) * n;
+      precond(n, inv_diag, vj, z);
+      orc_vmult(op, z, vn);
+      for (int i = 0; i <= j + 1; ++i) hcol[i] = 0.0;
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i <= j; ++i) {
+          const double hij = dot(n, V + (long)i * n, vn);
+          hcol[i] += hij;
+          for (long l = 0; l < n; ++l) vn[l] -= hij * V[(long)i * n + l];
+        }
+      const double s = sqrt(dot(n, vn, vn));
+      hcol[j + 1] = s;
+      if (s != 0.0)
+        for (long l = 0; l < n; ++l) vn[l] /= s;
+      for (int i = 0; i < j; ++i) {
+        const double t = cs[i] * hcol[i] + sn[i] * hcol[i + 1];
+        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1];
+        hcol[i] = t;
+      }
+      const double r = hypot(hcol[j], hcol[j + 1]);
+      cs[j] = hcol[j] / r;
+      sn[j] = hcol[j + 1] / r;
+      hcol[j] = r;
+      hcol[j + 1] = 0.0;
+      gam[j + 1] = -sn[j] * gam[j];
+      gam[j] = cs[j] * gam[j];
+      for (int i = 0; i <= j; ++i) H[i * m + j] = hcol[i];
+      rho = fabs(gam[j + 1]);
+      res = rho;
+      state = check(&c, acc, rho, abs_tol, rel_tol);
+    }
+    /* back substitution and update x += P (V y) */
+    for (int i = j - 1; i >= 0; --i) {
+      double s = gam[i];
+      for (int l = i + 1; l < j; ++l) s -= H[i * m + l] * y[l];
+      y[i] = s / H[i * m + i];
+    }
+    memset(w, 0, sizeof(double) * n);
+    for (int i = 0; i < j; ++i)
+      for (long l = 0; l < n; ++l) w[l] += y[i] * V[(long)i * n + l];
+    precond(n, inv_diag, w, z);
+    for (long l = 0; l < n; ++l) x[l] += z[l];
+    if (state != 0) break;
+  }
+  *iters = acc;
+  *fres = res;
+  free(V);
+  free(w);
+  free(z);
+  free(H);
+  free(cs);
+  free(sn);
+  free(gam);
+  free(y);
+  free(hcol);
+  return state == 1 ? ORC_SUCCESS : ORC_NO_CONVERGENCE;
+}
+
+int orc_solve(const orc_operator *op, int solver, int restart, int max_it, double abs_tol,
+              double rel_tol, const double *inv_diag, const double *b, double *x, int *iterations,
+              double *final_residual)
+{
+  const long n = orc_n_dofs(&op->mesh);
+  if (solver == ORC_CG)
+    return solve_cg(op, n, max_it, abs_tol, rel_tol, inv_diag, b, x, iterations, final_residual);
+  return solve_gmres(op, n, restart > 0 ? restart : 30, max_it, abs_tol, rel_tol, inv_diag, b, x,
+                     iterations, final_residual);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* FE L2 norm: VectorTools::integrate_difference into a Vector<float> of cell errors, then       */
+/* compute_global_error (source/utilities/vector_tools.cpp:137-201)                              */
+/* ------------------------------------------------------------------------------------------ */
+
+double orc_fe_l2_norm(const orc_mesh *m, const double *u)
+{
+  geo_t g;
+  geo_init(m, &g);
+  const long ncells = orc_n_cells(m);
+  double total = 0.0;
+  cell_ws_t *ws = (cell_ws_t *)malloc(sizeof(cell_ws_t));
+  for (long cell = 0; cell < ncells; ++cell) {
+    cell_dofs(&g, cell, ws->idx);
+    for (int l = 0; l < g.nloc; ++l) ws->tmp[l] = u[ws->idx[l]];
+    eval_v(&g, ws->tmp, ws->val, &ws->w);
+    double s = 0.0;
+    for (int q = 0; q < g.nloc; ++q) s += ws->val[q] * ws->val[q] * cell_volume_weight(&g, q);
+    const float e = (float)sqrt(s);
+    total += (double)e * (double)e;
+  }
+  free(ws);
+  return sqrt(total);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Whole-path drivers for the golden outputs                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Implicit Euler heat steps with the Newton loop of NewtonRaphsonSolver::solve
+ * (source/linear_algebra/newton_raphson_solver.cpp:22-80): r = -R(T); solve J d = r;
+ * converged when |r| < 5e-9 and |d| < 1e-10 (checked before d is added); otherwise T += d.
+ * The predictor is the old solution; the converged state does not depend on it. */
+int orc_heat_implicit_euler(const orc_operator *op_in, int n_steps, double *T, double *norms)
+{
+  orc_operator op = *op_in;
+  const long n = orc_n_dofs(&op.mesh);
+  double *T_old = malloc(sizeof(double) * n), *r = malloc(sizeof(double) * n);
+  double *d = malloc(sizeof(double) * n), *inv = malloc(sizeof(double) * n);
+  int status = 0;
+  for (int step = 0; step < n_steps; ++step) {
+    memcpy(T_old, T, sizeof(double) * n);
+    op.T = T;
+    op.T_old = T_old;
+    op.use_mask = 1;
+    int converged = 0;
+    for (int it = 0; it < 10 && !converged; ++it) {
+      memset(r, 0, sizeof(double) * n);
+      orc_create_rhs(&op, T_old, NULL, r);
+      orc_diagonal(&op, inv);
+      orc_invert_diagonal(ORC_OP_HEAT, n, inv);
+      memset(d, 0, sizeof(double) * n);
+      int iters;
+      double fres;
+      status |= orc_solve(&op, op.heat.t_dependent ? ORC_GMRES : ORC_CG, 30, 10000, 1e-20,
+                          1e-12, inv, r, d, &iters, &fres);
+      const double rn = sqrt(dot(n, r, r)), dn = sqrt(dot(n, d, d));
+      if (rn < 5e-9 && dn < 1e-10)
+        converged = 1;
+      else
+        for (long i = 0; i < n; ++i) T[i] += d[i];
+    }
+    if (!converged) status |= 2;
+    norms[step] = orc_fe_l2_norm(&op.mesh, T);
+  }
+  free(T_old);
+  free(r);
+  free(d);
+  free(inv);
+  return status;
+}
+
+/* theta-scheme advection steps as AdvectionDiffusionOperation::solve
+ * (source/level_set/advection_diffusion_operation.cpp:227-367) with the Dirichlet lifting of
+ * create_rhs_and_apply_dirichlet_matrixfree (utilities_matrixfree.hpp:33-100):
+ *   rhs = -A_noBC x_d + b(phi_old), constrained rows set to zero, GMRES + Jacobi,
+ *   constrained entries set to their Dirichlet values afterwards. */
+int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *dirichlet_values,
+                          double *phi, double *norms)
+{
+  orc_operator op = *op_in;
+  const long n = orc_n_dofs(&op.mesh);
+  double *old = malloc(sizeof(double) * n), *rhs = malloc(sizeof(double) * n);
+  double *xd = malloc(sizeof(double) * n), *inv = malloc(sizeof(double) * n);
+  int status = 0;
+  for (long i = 0; i < n; ++i) xd[i] = (op.mask && op.mask[i]) ? dirichlet_values[i] : 0.0;
+  for (int step = 0; step < n_steps; ++step) {
+    memcpy(old, phi, sizeof(double) * n);
+    op.use_mask = 0;
+    orc_vmult(&op, xd, rhs);
+    for (long i = 0; i < n; ++i) rhs[i] = -rhs[i];
+    orc_create_rhs(&op, old, NULL, rhs);
+    for (long i = 0; i < n; ++i)
+      if (op.mask && op.mask[i]) rhs[i] = 0.0;
+    op.use_mask = 1;
+    orc_diagonal(&op, inv);
+    orc_invert_diagonal(ORC_OP_ADVEC, n, inv);
+    int iters;
+    double fres;
+    status |= orc_solve(&op, ORC_GMRES, 30, 10000, 1e-20, 1e-12, inv, rhs, phi, &iters, &fres);
+    for (long i = 0; i < n; ++i)
+      if (op.mask && op.mask[i]) phi[i] = dirichlet_values[i];
+    norms[step] = orc_fe_l2_norm(&op.mesh, phi);
+  }
+  free(old);
+  free(rhs);
+  free(xd);
+  free(inv);
+  return status;
+}

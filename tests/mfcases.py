@@ -1,0 +1,100 @@
+"""Shared builders for parity tests: the same seeded case as an oracle operator (CPU, C) and as
+a device operator (libmeltpooldg_amd through the C ABI)."""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+import orc
+
+
+@dataclasses.dataclass
+class Case:
+    degree: int = 2
+    ncell: tuple = (3, 4, 5)
+    lo: tuple = (0.0, 0.0, 0.0)
+    hi: tuple = (1.0, 1.2, 0.9)
+    kind: str = "heat"          # heat | advec
+    law: str = "const"          # const | tanh
+    t_dependent: bool = False
+    vel: bool = False
+    k_cell: bool = False
+    mask: str = "boundary"      # none | boundary | random
+    dt: float = 1e-3
+    rho_cp: float = 1.3
+    k: float = 0.7
+    theta: float = 0.5
+    diffusivity: float = 0.05
+    mass_factor: float = 1.0
+    seed: int = 11
+
+    def mesh(self):
+        return orc.make_mesh(3, self.degree, self.ncell, self.lo, self.hi)
+
+
+def fields(case: Case):
+    m = case.mesh()
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(case.seed)
+    out = {}
+    out["T"] = 1700.0 + 50.0 * 3.0 * orc.random_field(m, case.seed + 1) if case.law == "tanh" \
+        else 300.0 + orc.random_field(m, case.seed + 1)
+    out["T_old"] = out["T"] - 5.0 * orc.random_field(m, case.seed + 2)
+    out["vel"] = rng.uniform(-1, 1, size=3 * n) if case.vel else None
+    out["k_cell"] = np.exp(0.3 * rng.standard_normal(orc.n_cells(m))) if case.k_cell else None
+    if case.mask == "none":
+        out["mask"] = np.zeros(n, dtype=np.uint8)
+    elif case.mask == "boundary":
+        out["mask"] = orc.boundary_mask(m)
+    else:
+        out["mask"] = (rng.uniform(size=n) < 0.2).astype(np.uint8)
+    out["q_s"] = orc.random_field(m, case.seed + 3)
+    return out
+
+
+def oracle_operator(case: Case, f, use_mask=1):
+    m = case.mesh()
+    if case.kind == "heat":
+        hp = orc.HeatParams(law=orc.LAW_TANH if case.law == "tanh" else orc.LAW_CONSTANT,
+                            t_dependent=int(case.t_dependent), rho_cp=case.rho_cp, k=case.k,
+                            tanh_amp=0.5, tanh_Tc=1700.0, tanh_width=50.0, dt=case.dt)
+        return orc.OracleOperator(orc.OP_HEAT, m, heat=hp,
+                                  vel_kind=orc.VEL_NODAL if case.vel else orc.VEL_NONE,
+                                  mask=f["mask"], T=f["T"], T_old=f["T_old"], vel=f["vel"],
+                                  k_cell=f["k_cell"], use_mask=use_mask)
+    ap = orc.AdvecParams(dt=case.dt, theta=case.theta, diffusivity=case.diffusivity,
+                         mass_factor=case.mass_factor, vel_scale=0.0)
+    return orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=f["mask"],
+                              vel=f["vel"], use_mask=use_mask)
+
+
+def device_operator(case: Case, f, dev="cuda"):
+    import torch
+
+    import meltpooldg_amd as mf
+    s = mf.ScratchData(case.degree, case.ncell, case.lo, case.hi,
+                       stream=torch.cuda.current_stream())
+    s.set_dirichlet_mask(f["mask"])
+    if case.kind == "heat":
+        op = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH if case.law == "tanh" else mf.LAW_CONSTANT,
+                                    rho_cp=case.rho_cp, k=case.k, tanh_amp=0.5, tanh_Tc=1700.0,
+                                    tanh_width=50.0, t_dependent=case.t_dependent, dt=case.dt)
+    else:
+        op = mf.AdvectionDiffusionOperator(s, theta=case.theta, diffusivity=case.diffusivity,
+                                           mass_factor=case.mass_factor, dt=case.dt)
+
+    def t(a):
+        return None if a is None else torch.tensor(a, dtype=torch.float64, device=dev)
+
+    vel = f["vel"] if (case.vel or case.kind == "advec") else None
+    op.set_fields(T=t(f["T"]) if case.kind == "heat" else None, velocity=t(vel),
+                  k_cell=t(f["k_cell"]))
+    return s, op
+
+
+def rel_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (scale if scale > 0 else 1.0))
