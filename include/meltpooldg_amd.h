@@ -33,6 +33,10 @@ enum {
 
 const char *mf_last_error(void);
 int mf_version(void);
+/* Tuning knob without reference counterpart: which device kernel applies the tangent
+ * (0 = pencil kernel, one LDS round trip per 1D sweep; 1 = slice-resident kernel, default).
+ * Both compute the same operator; process-wide setting. */
+int mf_set_tangent_kernel(int variant);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Mesh / DoF / constraint setup.  Replaces ScratchData::build + MatrixFree::reinit and the     */

@@ -1,0 +1,210 @@
+// meltpooldg_amd.hpp -- header-only C++ host layer over the C ABI (meltpooldg_amd.h).
+//
+// Mirrors the reference's operator / solver interfaces so that a C++ caller written against
+// MeltPoolDG reads the same:
+//   OperatorMatrixFree<dim,number>::{vmult, create_rhs, compute_inverse_diagonal_from_matrixfree}
+//   OperatorBase::{reset_time_increment, reset_dof_index}      (include/meltpooldg/core/operator_base.hpp:21-84,189-283)
+//   LinearSolver::solve                                        (include/meltpooldg/linear_algebra/linear_solver.hpp:22-134)
+//   JacobiPreconditioner::{update, vmult}                      (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)
+// Errors are thrown as mf::Exception (the reference throws dealii::ExceptionBase) and
+// mf::NoConvergence (dealii::SolverControl::NoConvergence).  DeviceVector is the subset of
+// LinearAlgebra::distributed::Vector the solver layer needs, resident in HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "meltpooldg_amd.h"
+
+namespace mf {
+
+class Exception : public std::runtime_error {
+ public:
+  Exception(int code, const std::string &what) : std::runtime_error(what), code(code) {}
+  int code;
+};
+
+class NoConvergence : public Exception {
+ public:
+  using Exception::Exception;
+};
+
+inline void check(int code)
+{
+  if (code == MF_OK) return;
+  if (code == MF_NO_CONVERGENCE) throw NoConvergence(code, mf_last_error());
+  throw Exception(code, mf_last_error());
+}
+
+inline void hip_check(hipError_t e)
+{
+  if (e != hipSuccess) throw Exception(MF_ERR_HIP, hipGetErrorString(e));
+}
+
+// Device-resident vector of n entries (HBM).
+template <typename Number>
+class DeviceVector {
+ public:
+  DeviceVector() = default;
+  explicit DeviceVector(std::size_t n) { reinit(n); }
+  DeviceVector(const DeviceVector &) = delete;
+  DeviceVector &operator=(const DeviceVector &) = delete;
+  DeviceVector(DeviceVector &&o) noexcept : ptr_(std::exchange(o.ptr_, nullptr)), n_(o.n_) {}
+  ~DeviceVector()
+  {
+    if (ptr_) (void)hipFree(ptr_);
+  }
+  void reinit(std::size_t n)
+  {
+    if (ptr_) hip_check(hipFree(ptr_));
+    n_ = n;
+    hip_check(hipMalloc(&ptr_, sizeof(Number) * (n ? n : 1)));
+    hip_check(hipMemset(ptr_, 0, sizeof(Number) * (n ? n : 1)));
+  }
+  std::size_t size() const { return n_; }
+  Number *data() { return ptr_; }
+  const Number *data() const { return ptr_; }
+  void copy_from_host(const Number *h) { hip_check(hipMemcpy(ptr_, h, sizeof(Number) * n_, hipMemcpyHostToDevice)); }
+  void copy_to_host(Number *h) const { hip_check(hipMemcpy(h, ptr_, sizeof(Number) * n_, hipMemcpyDeviceToHost)); }
+  std::vector<Number> to_host() const
+  {
+    std::vector<Number> v(n_);
+    copy_to_host(v.data());
+    return v;
+  }
+
+ private:
+  Number *ptr_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+// ScratchData + MatrixFree setup of a structured hex mesh (source/core/scratch_data.cpp:194-293).
+class ScratchData {
+ public:
+  ScratchData(int degree, int nx, int ny, int nz, const double lo[3] = nullptr,
+              const double hi[3] = nullptr, hipStream_t stream = nullptr)
+  {
+    mf_mesh_desc d{};
+    d.dim = 3;
+    d.degree = degree;
+    d.ncell[0] = nx;
+    d.ncell[1] = ny;
+    d.ncell[2] = nz;
+    for (int i = 0; i < 3; ++i) {
+      d.lo[i] = lo ? lo[i] : 0.0;
+      d.hi[i] = hi ? hi[i] : 1.0;
+    }
+    check(mf_scratch_create(&d, &h_));
+    check(mf_set_stream(h_, stream));
+  }
+  ScratchData(const ScratchData &) = delete;
+  ScratchData &operator=(const ScratchData &) = delete;
+  ~ScratchData() { mf_scratch_destroy(h_); }
+  std::size_t n_dofs() const { return std::size_t(mf_scratch_n_dofs(h_)); }
+  std::size_t n_cells() const { return std::size_t(mf_scratch_n_cells(h_)); }
+  void set_dirichlet_mask(const std::vector<std::uint8_t> &mask) { check(mf_scratch_set_dirichlet_mask(h_, mask.data())); }
+  template <typename Number>
+  void initialize_dof_vector(DeviceVector<Number> &v) const { v.reinit(n_dofs()); }
+  mf_scratch *handle() const { return h_; }
+
+ private:
+  mf_scratch *h_ = nullptr;
+};
+
+// OperatorMatrixFree<3,double> over device vectors.
+class OperatorMatrixFree {
+ public:
+  using VectorType = DeviceVector<double>;
+  OperatorMatrixFree(const OperatorMatrixFree &) = delete;
+  OperatorMatrixFree &operator=(const OperatorMatrixFree &) = delete;
+  virtual ~OperatorMatrixFree() { mf_operator_destroy(h_); }
+
+  void reset_time_increment(double dt) { check(mf_operator_reset_time_increment(h_, dt)); }
+  void reset_dof_index(unsigned int dof_idx) { check(mf_operator_reset_dof_index(h_, int(dof_idx))); }
+  void vmult(VectorType &dst, const VectorType &src) const { check(mf_operator_vmult(h_, dst.data(), src.data())); }
+  void vmult(DeviceVector<float> &dst, const DeviceVector<float> &src) const
+  {
+    check(mf_operator_vmult_f32(h_, dst.data(), src.data()));
+  }
+  void compute_inverse_diagonal_from_matrixfree(VectorType &diag) const
+  {
+    check(mf_operator_compute_inverse_diagonal(h_, diag.data()));
+  }
+  void create_rhs(VectorType &dst, const VectorType &src, const VectorType *q_s = nullptr) const
+  {
+    check(mf_operator_create_rhs(h_, dst.data(), src.data(), q_s ? q_s->data() : nullptr));
+  }
+  mf_operator *handle() const { return h_; }
+
+ protected:
+  OperatorMatrixFree() = default;
+  mf_operator *h_ = nullptr;
+};
+
+// HeatDiffuseMultiPhaseOperator (source/heat/heat_diffuse_operator.cpp) on the device.
+class HeatDiffuseOperator : public OperatorMatrixFree {
+ public:
+  HeatDiffuseOperator(ScratchData &s, const mf_heat_params &p) { check(mf_heat_operator_create(s.handle(), &p, &h_)); }
+  // the linearisation vectors the reference wires in at heat_diffuse_operation.cpp:55-70
+  void set_fields(const VectorType *T, const VectorType *velocity = nullptr,
+                  const VectorType *k_cell = nullptr)
+  {
+    check(mf_operator_set_fields(h_, T ? T->data() : nullptr, velocity ? velocity->data() : nullptr,
+                                 k_cell ? k_cell->data() : nullptr));
+  }
+};
+
+// AdvectionDiffusionOperator (source/level_set/advection_diffusion_operator.cpp) on the device.
+class AdvectionDiffusionOperator : public OperatorMatrixFree {
+ public:
+  AdvectionDiffusionOperator(ScratchData &s, const mf_advec_params &p)
+  {
+    check(mf_advection_operator_create(s.handle(), &p, &h_));
+  }
+  void set_advection_velocity(const VectorType &velocity)
+  {
+    check(mf_operator_set_fields(h_, nullptr, velocity.data(), nullptr));
+  }
+};
+
+// LinearSolverData (include/meltpooldg/linear_algebra/linear_solver_data.hpp:37-51)
+struct LinearSolverData {
+  int solver_type = MF_SOLVER_GMRES;
+  int preconditioner_type = MF_PRECOND_JACOBI;
+  unsigned int max_iterations = 10000;
+  double rel_tolerance = 1e-12;
+  double abs_tolerance = 1e-20;
+  int precision = MF_PRECISION_FP64;
+  double inner_rel_tolerance = 1e-3;
+};
+
+struct LinearSolver {
+  // returns the iteration count (SolverControl::last_step()); throws NoConvergence
+  static int solve(OperatorMatrixFree &op, DeviceVector<double> &solution,
+                   const DeviceVector<double> &rhs, const LinearSolverData &data,
+                   mf_solve_report *report = nullptr)
+  {
+    mf_solver_desc d{};
+    d.solver = data.solver_type;
+    d.restart = 30;
+    d.max_iterations = int(data.max_iterations);
+    d.abs_tol = data.abs_tolerance;
+    d.rel_tol = data.rel_tolerance;
+    d.preconditioner = data.preconditioner_type;
+    d.precision = data.precision;
+    d.inner_rel_tol = data.inner_rel_tolerance;
+    mf_solve_report rep{};
+    const int code = mf_solve(op.handle(), &d, solution.data(), rhs.data(), &rep);
+    if (report) *report = rep;
+    check(code);
+    return rep.iterations;
+  }
+};
+
+}  // namespace mf

@@ -120,6 +120,7 @@ def lib():
         L.mf_operator_compute_inverse_diagonal.argtypes = [vp, vp]
         L.mf_operator_create_rhs.argtypes = [vp, vp, vp, vp]
         L.mf_solve.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, C.POINTER(SolveReport)]
+        L.mf_set_tangent_kernel.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -131,6 +132,11 @@ def splitmix64(x):
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         return z ^ (z >> np.uint64(31))
+
+
+def set_tangent_kernel(variant):
+    """0 = pencil kernel, 1 = slice-resident kernel (default); same operator."""
+    _check(lib().mf_set_tangent_kernel(int(variant)))
 
 
 def _check(code):

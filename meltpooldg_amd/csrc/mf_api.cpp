@@ -192,8 +192,12 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+  if (g_tangent_variant == 1)
+    return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
 }
+
+int g_tangent_variant = 1;
 
 }  // namespace mf
 
@@ -207,6 +211,13 @@ extern "C" {
 const char *mf_last_error(void) { return g_last_error.c_str(); }
 
 int mf_version(void) { return 100; }
+
+int mf_set_tangent_kernel(int variant)
+{
+  MF_REQUIRE(variant == 0 || variant == 1, MF_ERR_ARG, "tangent kernel variant must be 0 or 1");
+  g_tangent_variant = variant;
+  return MF_OK;
+}
 
 int mf_scratch_create(const mf_mesh_desc *desc, mf_scratch **out)
 {
@@ -464,7 +475,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
-  const int ncb = cell_blocks(m);
+  const int ncb = tangent_blocks(m);
   DevBuf<T> r, p, q, prr, prz0, prz1;
   DevBuf<double> pcell;
   DevBuf<SolveState> sd;

@@ -93,6 +93,19 @@ int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<
 // number of blocks the cell kernel uses (size of dot_partial)
 int cell_blocks(const MeshInfo &m);
 
+// slice-resident tangent kernel (mf_slab.hip)
+template <typename Number>
+int launch_slab_tangent(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                        const Number *src, Number *dst, hipStream_t stream);
+int slab_blocks(const MeshInfo &m);
+
+// tangent kernel selection: 0 = pencil kernel (mf_cell.hip), 1 = slice-resident (mf_slab.hip)
+extern int g_tangent_variant;
+inline int tangent_blocks(const MeshInfo &m)
+{
+  return g_tangent_variant == 1 ? slab_blocks(m) : cell_blocks(m);
+}
+
 }  // namespace mf
 
 struct mf_scratch {

@@ -19,6 +19,7 @@ LAW_CONSTANT, LAW_TANH = 0, 1
 VEL_NONE, VEL_NODAL, VEL_VORTEX = 0, 1, 2
 OP_HEAT, OP_ADVEC = 0, 1
 CG, GMRES = 0, 1
+SUCCESS, NO_CONVERGENCE = 0, 1
 
 
 class Mesh(C.Structure):

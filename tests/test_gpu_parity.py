@@ -36,8 +36,16 @@ def _dev(a):
     return torch.tensor(a, dtype=torch.float64, device="cuda")
 
 
+@pytest.fixture(params=[0, 1], ids=["pencil", "slab"])
+def variant(request):
+    import meltpooldg_amd as mf
+    mf.set_tangent_kernel(request.param)
+    yield request.param
+    mf.set_tangent_kernel(1)
+
+
 @pytest.mark.parametrize("case", CASES, ids=ids)
-def test_vmult_fp64(case):
+def test_vmult_fp64(case, variant):
     f = fields(case)
     ref = oracle_operator(case, f)
     s, op = device_operator(case, f)
@@ -53,7 +61,7 @@ def test_vmult_fp64(case):
 
 
 @pytest.mark.parametrize("case", CASES[:6] + CASES[8:], ids=ids)
-def test_vmult_fp32(case):
+def test_vmult_fp32(case, variant):
     f = fields(case)
     ref = oracle_operator(case, f)
     s, op = device_operator(case, f)

@@ -1,0 +1,67 @@
+"""CPU-side checks of the device library: it loads, exports every entry point the C header
+declares, and its host logic (mesh setup, 1D tables, argument validation, synthetic inputs)
+agrees with the oracle.  No GPU compute is called here."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import orc
+
+mf = pytest.importorskip("meltpooldg_amd")
+
+
+def test_library_exports_every_header_function():
+    L = mf.lib()
+    names = mf.header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_tables_match_oracle(degree):
+    s = mf.ScratchData(degree, (2, 3, 4), hi=(1.0, 2.0, 0.5))
+    S, DS, Dq, w, gl = s.shape_info()
+    S2, DS2, Dq2, w2 = orc.shape_matrices(degree)
+    assert np.allclose(S, S2, atol=1e-14) and np.allclose(DS, DS2, atol=1e-13)
+    assert np.allclose(Dq, Dq2, atol=1e-13) and np.allclose(w, w2, atol=1e-15)
+    assert np.allclose(gl, orc.gauss_lobatto(degree + 1), atol=1e-15)
+    # collocation identity used by the kernels: Dq S = DS
+    assert np.allclose(Dq @ S, DS, atol=1e-12)
+    m = orc.make_mesh(3, degree, (2, 3, 4), (0, 0, 0), (1.0, 2.0, 0.5))
+    assert s.n_dofs == orc.n_dofs(m) and s.n_cells == orc.n_cells(m)
+    assert np.array_equal(s.support_points(), orc.node_coords(m))
+    assert np.array_equal(s.boundary_mask(), orc.boundary_mask(m))
+    assert np.array_equal(s.random_field(9), orc.random_field(m, 9))
+
+
+def test_argument_errors_are_reported():
+    with pytest.raises(mf.MFError) as e:
+        mf.ScratchData(4, 3)
+    assert e.value.code == mf.MF_ERR_ARG and "degree" in str(e.value)
+    with pytest.raises(mf.MFError):
+        mf.ScratchData(2, (0, 1, 1))
+    s = mf.ScratchData(2, 2)
+    with pytest.raises(mf.MFError) as e:
+        mf.AdvectionDiffusionOperator(s, diffusivity=-1.0)
+    assert "diffusivity" in str(e.value)
+    op = mf.HeatDiffuseOperator(s)
+    with pytest.raises(mf.MFError) as e:
+        op.reset_time_increment(0.0)
+    assert e.value.code == mf.MF_ERR_ARG
+    with pytest.raises(mf.MFError):
+        op.reset_dof_index(2)
+    # vmult before reset_time_increment is a state error (AssertThrowZeroTimeIncrement)
+    L = mf.lib()
+    code = L.mf_operator_vmult(op._h, C.c_void_p(8), C.c_void_p(8))
+    assert code == mf.MF_ERR_STATE
+    assert b"time increment" in L.mf_last_error()
+    # tanh law without T is a state error
+    op2 = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH, tanh_width=50.0, dt=1e-3)
+    assert L.mf_operator_vmult(op2._h, C.c_void_p(8), C.c_void_p(8)) == mf.MF_ERR_STATE
+
+
+def test_too_large_mesh_rejected():
+    with pytest.raises(mf.MFError):
+        mf.ScratchData(3, (1000, 1000, 1000))

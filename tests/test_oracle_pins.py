@@ -1,0 +1,280 @@
+"""Pins for every oracle function that the golden outputs leave open: closed forms of the 1D
+basis and quadrature, the dense (brute-force) element-matrix path against the sum-factorised
+path, invariants of the heat and advection operators, the diagonal against the assembled
+matrix, and the Krylov solvers against dense direct solves."""
+import itertools
+
+import numpy as np
+import pytest
+
+import orc
+
+
+# ---------------------------------------------------------------- 1D basis and quadrature
+@pytest.mark.parametrize("n", range(1, 9))
+def test_gauss_exactness(n):
+    x, w = orc.gauss(n)
+    for k in range(2 * n):
+        assert abs(np.dot(w, x ** k) - 1.0 / (k + 1)) < 1e-14
+    # not exact one degree higher (a wrong point set would pass the lower degrees only by luck)
+    assert abs(np.dot(w, x ** (2 * n)) - 1.0 / (2 * n + 1)) > 1e-13
+
+
+def test_gauss_lobatto_closed_forms():
+    assert np.allclose(orc.gauss_lobatto(2), [0, 1], atol=0, rtol=0)
+    assert np.allclose(orc.gauss_lobatto(3), [0, 0.5, 1], atol=1e-16)
+    r5 = 1 / np.sqrt(5)
+    assert np.allclose(orc.gauss_lobatto(4), [0, (1 - r5) / 2, (1 + r5) / 2, 1], atol=1e-15)
+    r37 = np.sqrt(3 / 7)
+    assert np.allclose(orc.gauss_lobatto(5), [0, (1 - r37) / 2, 0.5, (1 + r37) / 2, 1],
+                       atol=1e-15)
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3, 4, 5])
+def test_shape_matrices_reproduce_polynomials(degree):
+    K = degree + 1
+    S, DS, Dq, w = orc.shape_matrices(degree)
+    nodes = orc.gauss_lobatto(K)
+    xq, _ = orc.gauss(K)
+    for k in range(degree + 1):
+        f = nodes ** k
+        assert np.allclose(S @ f, xq ** k, atol=1e-13)
+        df = k * xq ** (k - 1) if k > 0 else np.zeros(K)
+        assert np.allclose(DS @ f, df, atol=1e-12)
+        assert np.allclose(Dq @ (xq ** k), df, atol=1e-12)
+    # partition of unity / Kronecker at the nodes
+    assert np.allclose(S.sum(axis=1), 1.0, atol=1e-14)
+    assert np.allclose(DS.sum(axis=1), 0.0, atol=1e-12)
+    v = np.zeros(K)
+    d = np.zeros(K)
+    for j, xj in enumerate(nodes):
+        orc.lib().orc_lagrange(K, nodes.ctypes.data_as(orc.C.POINTER(orc.C.c_double)),
+                               orc.C.c_double(xj), v.ctypes.data_as(orc.C.POINTER(orc.C.c_double)),
+                               d.ctypes.data_as(orc.C.POINTER(orc.C.c_double)))
+        assert np.allclose(v, np.eye(K)[j], atol=1e-14)
+
+
+# ---------------------------------------------------------------- operators
+def heat_op(m, law="const", td=0, vel=False, kcell=False, mask=None, seed=0, rho_cp=1.3, k=0.7,
+            dt=1e-2, use_mask=1):
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(seed)
+    T = 1700 + 100 * rng.uniform(-1, 1, n) if law == "tanh" else 300 + rng.uniform(-1, 1, n)
+    hp = orc.HeatParams(law=orc.LAW_TANH if law == "tanh" else orc.LAW_CONSTANT, t_dependent=td,
+                        rho_cp=rho_cp, k=k, tanh_amp=0.5, tanh_Tc=1700, tanh_width=50, dt=dt)
+    return orc.OracleOperator(orc.OP_HEAT, m, heat=hp,
+                              vel_kind=orc.VEL_NODAL if vel else orc.VEL_NONE, mask=mask, T=T,
+                              T_old=T - 1.0, vel=rng.uniform(-1, 1, 3 * n) if vel else None,
+                              k_cell=np.exp(rng.normal(0, .3, orc.n_cells(m))) if kcell else None,
+                              use_mask=use_mask)
+
+
+def advec_op(m, mask=None, seed=0, theta=0.5, D=0.1):
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(seed)
+    ap = orc.AdvecParams(dt=0.1, theta=theta, diffusivity=D, mass_factor=1.0, vel_scale=0)
+    return orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=mask,
+                              vel=rng.uniform(-1, 1, 3 * n))
+
+
+def assemble(op):
+    n = op.n
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = op.vmult(e)
+    return A
+
+
+MESHES = [(1, 1, (4,)), (1, 3, (3,)), (2, 1, (3, 2)), (2, 2, (2, 3)), (2, 3, (2, 2)),
+          (3, 1, (2, 3, 2)), (3, 2, (2, 2, 1)), (3, 3, (1, 2, 1))]
+
+
+@pytest.mark.parametrize("dim,deg,nc", MESHES)
+@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td", "heat-vel-kc", "advec"])
+def test_sum_factorisation_matches_dense(dim, deg, nc, variant):
+    hi = (1.0, 0.8, 1.3)[:dim]
+    m = orc.make_mesh(dim, deg, nc, (0.0,) * dim, hi)
+    mask = orc.boundary_mask(m)
+    if variant == "heat":
+        op = heat_op(m, mask=mask)
+    elif variant == "heat-tanh-td":
+        op = heat_op(m, law="tanh", td=1, mask=mask)
+    elif variant == "heat-vel-kc":
+        op = heat_op(m, vel=True, kcell=True, td=1, law="tanh")
+    else:
+        op = advec_op(m, mask=mask)
+    u = np.random.default_rng(1).uniform(-1, 1, op.n)
+    a, b = op.vmult(u), op.vmult_dense(u)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("dim,deg", [(1, 2), (2, 1), (2, 3), (3, 1), (3, 2), (3, 3)])
+def test_mass_sums_to_volume(dim, deg):
+    """rho_cp/dt * M applied to 1, summed, equals rho_cp * |Omega| / dt (k = 0)."""
+    hi = (1.0, 0.8, 1.3)[:dim]
+    m = orc.make_mesh(dim, deg, (3, 2, 2)[:dim], (0.0,) * dim, hi)
+    op = heat_op(m, k=0.0, rho_cp=2.5, dt=0.5)
+    vol = np.prod(hi)
+    assert abs(op.vmult(np.ones(op.n)).sum() - 2.5 * vol / 0.5) < 1e-12 * vol * 5
+
+
+@pytest.mark.parametrize("dim,deg", [(2, 2), (3, 1), (3, 3)])
+def test_laplace_annihilates_constants_and_interior_linears(dim, deg):
+    hi = (1.0, 0.8, 1.3)[:dim]
+    m = orc.make_mesh(dim, deg, (3, 3, 3)[:dim], (0.0,) * dim, hi)
+    op = heat_op(m, rho_cp=0.0, k=1.0)
+    assert np.max(np.abs(op.vmult(np.ones(op.n)))) < 1e-12
+    xyz = orc.node_coords(m)
+    lin = xyz[:, 0] - 2.0 * xyz[:, dim - 1]
+    r = op.vmult(lin)
+    interior = orc.boundary_mask(m) == 0
+    assert np.max(np.abs(r[interior])) < 1e-12
+    # and the boundary rows carry the Neumann flux: sum of all rows = boundary integral of dn(u)=0
+    assert abs(r.sum()) < 1e-12
+
+
+def test_symmetry_and_definiteness_constant_law():
+    m = orc.make_mesh(3, 2, (3, 2, 2), (0, 0, 0), (1, 0.7, 1.2))
+    op = heat_op(m, mask=orc.boundary_mask(m))
+    rng = np.random.default_rng(3)
+    u, v = rng.uniform(-1, 1, op.n), rng.uniform(-1, 1, op.n)
+    free = orc.boundary_mask(m) == 0
+    u[~free] = 0
+    v[~free] = 0
+    assert abs(v @ op.vmult(u) - u @ op.vmult(v)) < 1e-12 * abs(v @ op.vmult(u))
+    assert u @ op.vmult(u) > 0
+
+
+def test_advection_linear_field_and_skew_part():
+    """With D = 0, theta = 1 and phi = a.x, the convective part integrates w.a against the test
+    functions: A phi - M phi/dt = M_w a.  Check with the dense element matrices' skew part:
+    for divergence-free constant w and zero boundary values the convective matrix is skew."""
+    m = orc.make_mesh(3, 2, (3, 3, 3))
+    n = orc.n_dofs(m)
+    w = np.zeros(3 * n)
+    w[0:n] = 0.3
+    w[n:2 * n] = -0.2
+    w[2 * n:] = 0.5
+    mask = orc.boundary_mask(m)
+    ap = orc.AdvecParams(dt=1.0, theta=1.0, diffusivity=0.0, mass_factor=0.0, vel_scale=0)
+    op = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=mask, vel=w)
+    A = assemble(op)
+    free = mask == 0
+    C = A[np.ix_(free, free)]
+    assert np.max(np.abs(C + C.T)) < 1e-13
+    # linear field: row sums of (w . grad) phi_j for phi = a.x equal (w.a) * (M 1) in the interior
+    xyz = orc.node_coords(m)
+    a = np.array([1.0, 2.0, -1.0])
+    phi = xyz @ a
+    op_nomask = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, vel=w)
+    conv = op_nomask.vmult(phi)
+    hp = orc.HeatParams(law=0, t_dependent=0, rho_cp=1.0, k=0.0, dt=1.0)
+    mass1 = orc.OracleOperator(orc.OP_HEAT, m, heat=hp).vmult(np.ones(n))
+    assert np.allclose(conv, (0.3 * 1.0 - 0.2 * 2.0 + 0.5 * -1.0) * mass1, atol=1e-13)
+
+
+@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td-vel", "advec"])
+def test_diagonal_matches_assembled(variant):
+    m = orc.make_mesh(3, 2, (2, 2, 3), (0, 0, 0), (1, 1.1, 0.9))
+    mask = orc.boundary_mask(m)
+    mask[5] = 0
+    if variant == "heat":
+        op = heat_op(m, mask=mask)
+    elif variant == "heat-tanh-td-vel":
+        op = heat_op(m, law="tanh", td=1, vel=True, kcell=True, mask=mask)
+    else:
+        op = advec_op(m, mask=mask)
+    A = assemble(op)
+    d = op.diagonal()
+    assert np.max(np.abs(d - np.diag(A))) < 1e-13 * np.max(np.abs(d))
+    inv = op.inverse_diagonal()
+    free = mask == 0
+    assert np.allclose(inv[free], 1.0 / np.diag(A)[free], rtol=1e-14)
+    assert np.all(inv[~free] == 1.0)   # zero diagonal of constrained rows inverts to 1
+
+
+def test_inversion_thresholds():
+    d = np.array([1e-11, -2e-10, 0.0, 4.0, 1e-15, 3e-13])
+    h = d.copy()
+    orc.lib().orc_invert_diagonal(orc.OP_HEAT, orc.C.c_long(len(h)), orc._p(h))
+    assert list(h) == [1.0, 1 / -2e-10, 1.0, 0.25, 1.0, 1.0]
+    a = d.copy()
+    orc.lib().orc_invert_diagonal(orc.OP_ADVEC, orc.C.c_long(len(a)), orc._p(a))
+    # threshold 1e-14 * max(1, |d|_inf) = 4e-14
+    assert list(a) == [1 / 1e-11, 1 / -2e-10, 1.0, 0.25, 1.0, 1 / 3e-13]
+
+
+# ---------------------------------------------------------------- solvers
+def test_cg_and_gmres_match_dense_solve():
+    m = orc.make_mesh(3, 2, (3, 3, 2), (0, 0, 0), (1, 1, 0.7))
+    mask = orc.boundary_mask(m)
+    op = heat_op(m, mask=mask)
+    A = assemble(op)
+    free = mask == 0
+    rng = np.random.default_rng(5)
+    b = rng.uniform(-1, 1, op.n)
+    b[~free] = 0
+    x_dense = np.zeros(op.n)
+    x_dense[free] = np.linalg.solve(A[np.ix_(free, free)], b[free])
+    for solver in (orc.CG, orc.GMRES):
+        x, it, res, st = op.solve(b, solver=solver, rel_tol=1e-13, inv_diag=op.inverse_diagonal())
+        assert st == 0
+        assert np.max(np.abs(x - x_dense)) < 1e-11 * np.max(np.abs(x_dense))
+        assert res <= 1e-13 * np.linalg.norm(b)
+
+
+def test_gmres_nonsymmetric_matches_dense_and_finite_termination():
+    m = orc.make_mesh(3, 1, (3, 2, 2))
+    mask = orc.boundary_mask(m)
+    mask[:] = 0
+    op = advec_op(m, theta=1.0, D=0.02)
+    A = assemble(op)
+    b = np.random.default_rng(6).uniform(-1, 1, op.n)
+    x_dense = np.linalg.solve(A, b)
+    n = op.n
+    x, it, res, st = op.solve(b, solver=orc.GMRES, restart=n, rel_tol=1e-12)
+    assert st == 0 and it <= n
+    assert np.max(np.abs(x - x_dense)) < 1e-9 * np.max(np.abs(x_dense))
+    # unpreconditioned CG on a non-symmetric system must not be trusted: GMRES is the fallback
+    x, it, res, st = op.solve(b, solver=orc.GMRES, restart=5, rel_tol=1e-12,
+                              inv_diag=op.inverse_diagonal())
+    assert st == 0
+    assert np.max(np.abs(x - x_dense)) < 1e-9 * np.max(np.abs(x_dense))
+
+
+def test_reduction_control_semantics():
+    """Success when |r| <= max(abs_tol, rel_tol*|r0|); max_it reached -> NoConvergence;
+    zero rhs and zero guess -> converged at step 0."""
+    m = orc.make_mesh(3, 1, (3, 3, 3))
+    op = heat_op(m, mask=orc.boundary_mask(m), dt=1.0)
+    b = np.random.default_rng(7).uniform(-1, 1, op.n)
+    b[orc.boundary_mask(m) == 1] = 0
+    x, it, res, st = op.solve(b, solver=orc.CG, max_it=1)
+    assert st == orc.NO_CONVERGENCE and it == 1
+    x, it, res, st = op.solve(np.zeros(op.n), solver=orc.CG)
+    assert st == 0 and it == 0
+    x, it, res, st = op.solve(b, solver=orc.CG, abs_tol=1e3, rel_tol=0.0)
+    assert st == 0 and it == 0
+
+
+def test_fe_l2_norm_closed_form():
+    """||x||_L2 on [0,1]^3 = 1/sqrt(3), exact for the interpolant of a linear field."""
+    m = orc.make_mesh(3, 2, (2, 3, 2))
+    xyz = orc.node_coords(m)
+    v = orc.fe_l2_norm(m, xyz[:, 0])
+    assert abs(v - 1 / np.sqrt(3)) < 1e-7  # per-cell float storage (reference semantics)
+
+
+def test_random_field_is_numbering_independent_and_uniform():
+    m = orc.make_mesh(3, 3, (4, 4, 4))
+    a = orc.random_field(m, 1)
+    assert a.min() >= -1 and a.max() < 1 and abs(a.mean()) < 0.02
+    b = orc.random_field(m, 2)
+    assert np.corrcoef(a, b)[0, 1] < 0.05
+    # the value depends only on the lattice coordinates and the seed
+    m2 = orc.make_mesh(3, 3, (4, 4, 5))
+    c = orc.random_field(m2, 1)
+    n0 = 13 * 13 * 13
+    assert np.array_equal(a[:n0], c[:n0])
