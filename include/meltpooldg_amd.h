@@ -166,6 +166,102 @@ typedef struct {
 int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
              mf_solve_report *report);
 
+/* ------------------------------------------------------------------------------------------ */
+/* Callers on either side of the operator                                                      */
+/* ------------------------------------------------------------------------------------------ */
+/* NonlinearSolverData (include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:13-20):
+ * absolute tolerances; 0 selects the reference defaults (10, 1e-9, 1e-10). */
+typedef struct {
+  int max_iterations;
+  double residual_tolerance;
+  double correction_tolerance;
+} mf_newton_desc;
+
+typedef struct {
+  int iterations;          /* Newton iterations (linear solves) */
+  int linear_iterations;   /* accumulated Krylov iterations */
+  double residual_norm;    /* |r| of the last iteration */
+  double correction_norm;  /* |d| of the last iteration */
+  int status;
+} mf_newton_report;
+
+/* One implicit heat step: NewtonRaphsonSolver::solve (source/linear_algebra/
+ * newton_raphson_solver.cpp:22-80) with the residual / Jacobian wiring of HeatDiffuseOperation
+ * (source/heat/heat_diffuse_operation.cpp:86-116, 261-289).  T: predictor in, solution out (the
+ * operator's linearisation field is set to T); T_old: previous step; q_s: heat source (NULL =
+ * none); dirichlet_values: values of constrained DoFs (NULL = homogeneous).
+ * Failure: MF_NO_CONVERGENCE (reference ExcNewtonDidNotConverge). */
+int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const double *q_s,
+                        const double *dirichlet_values, const mf_solver_desc *desc,
+                        const mf_newton_desc *newton, mf_newton_report *report);
+/* One advection-diffusion step: AdvectionDiffusionOperation::solve (source/level_set/
+ * advection_diffusion_operation.cpp:227-367) with the Dirichlet lifting of
+ * create_rhs_and_apply_dirichlet_matrixfree (include/meltpooldg/linear_algebra/
+ * utilities_matrixfree.hpp:33-100).  phi: initial guess in, solution out; phi_old: field the
+ * right-hand side is built from (the old solution, or the BDF2 combination 2 phi_n - phi_{n-1}/2
+ * with mass_factor 1.5 and theta 1); dirichlet_values: NULL = homogeneous. */
+int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
+                      const double *dirichlet_values, const mf_solver_desc *desc,
+                      mf_solve_report *report);
+/* Predictor::vmult with linear extrapolation (include/meltpooldg/linear_algebra/
+ * predictor.hpp:88-161, source/linear_algebra/predictor_linear.cpp:74-96):
+ * dst = (1 + tau) x_n - tau x_{n-1}, tau = dt_n / dt_{n-1}. */
+int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *x_n,
+                                      const double *x_nm1, double tau);
+/* LaserHeatSourceVolumetric::compute_volumetric_heat_source with the Gaussian volumetric
+ * profile (source/heat/laser_heat_source_volumetric.cpp:20-58,
+ * include/meltpooldg/heat/laser_intensity_profiles.hpp:175-218): nodal values of
+ * P / (r sqrt(pi/2))^3 exp(-2 |x - x_L|^2 / r^2). */
+int mf_laser_heat_source(mf_scratch *s, double power, double radius, const double position[3],
+                         double *q_s);
+/* VectorTools::compute_norm, FE L2 norm (source/utilities/vector_tools.cpp:137-201) with the
+ * per-cell values rounded to single precision as in the reference; result on the host. */
+int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result);
+
+/* Derivative of the Gaussian volumetric source with respect to the radius (adjoint). */
+int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double radius,
+                                           const double position[3], double *dq_dr);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Differentiation of the implicit heat transient (no reference counterpart: new).             */
+/* ------------------------------------------------------------------------------------------ */
+/* vmult / solves apply A^T instead of A when transpose = 1 (a distinct device code path for
+ * the non-symmetric convection and dk/dT terms). The diagonal is the same for both. */
+int mf_operator_set_transpose(mf_operator *op, int transpose);
+/* dst = M src (mass matrix, Dirichlet rows zero). */
+int mf_mass_vmult(mf_scratch *s, double *dst, const double *src);
+/* result = a . b (deterministic device reduction; result on the host). */
+int mf_vector_dot(mf_scratch *s, const double *a, const double *b, double *result);
+/* grad_k_cell[c] += scale * sum_q JxW grad(lambda) . grad(T) over cell c (unit conductivity):
+ * the parameter-gradient kernel for per-cell conductivity factors. */
+int mf_heat_conductivity_gradient(mf_operator *op, const double *lambda, const double *T,
+                                  double scale, double *grad_k_cell);
+
+/* Moving-laser implicit-Euler transient, T_n = A^{-1}((rho cp/dt) M T_{n-1} + M q_n), laser at
+ * start + scan_velocity * n dt, with checkpointed adjoint (states stored every
+ * checkpoint_interval steps, recomputed segment-wise in the backward sweep). */
+typedef struct {
+  int n_steps;
+  int checkpoint_interval;
+  double dt;
+  double power;
+  double radius;
+  double start[3];
+  double scan_velocity[3];
+  mf_solver_desc solver;   /* for the forward and the adjoint solves */
+} mf_transient_desc;
+
+typedef struct mf_transient mf_transient;
+
+int mf_transient_create(mf_operator *op, const mf_transient_desc *d, mf_transient **out);
+int mf_transient_destroy(mf_transient *tr);
+/* T_final = T_N from T_0 (device vectors) */
+int mf_transient_forward(mf_transient *tr, const double *T0, double *T_final);
+/* Given dJ/dT_N (dual vector, e.g. M (T_N - T_target)): accumulates dJ/dk_cell into
+ * grad_k_cell[n_cells] (may be NULL) and returns dJ/dP, dJ/dr on the host. */
+int mf_transient_backward(mf_transient *tr, const double *dJ_dTN, double *grad_k_cell,
+                          double *grad_power, double *grad_radius);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,26 @@ class SolverDesc(C.Structure):
                 ("precision", C.c_int), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int)]
 
 
+class TransientDesc(C.Structure):
+    _fields_ = [("n_steps", C.c_int), ("checkpoint_interval", C.c_int), ("dt", C.c_double),
+                ("power", C.c_double), ("radius", C.c_double), ("start", C.c_double * 3),
+                ("scan_velocity", C.c_double * 3), ("solver", SolverDesc)]
+
+
+class NewtonDesc(C.Structure):
+    _fields_ = [("max_iterations", C.c_int), ("residual_tolerance", C.c_double),
+                ("correction_tolerance", C.c_double)]
+
+
+class NewtonReport(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("linear_iterations", C.c_int),
+                ("residual_norm", C.c_double), ("correction_norm", C.c_double),
+                ("status", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SolveReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("outer_iterations", C.c_int),
                 ("initial_residual", C.c_double), ("final_residual", C.c_double),
@@ -121,6 +141,25 @@ def lib():
         L.mf_operator_create_rhs.argtypes = [vp, vp, vp, vp]
         L.mf_solve.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, C.POINTER(SolveReport)]
         L.mf_set_tangent_kernel.argtypes = [C.c_int]
+        L.mf_heat_newton_step.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolverDesc),
+                                          C.POINTER(NewtonDesc), C.POINTER(NewtonReport)]
+        L.mf_advection_step.argtypes = [vp, vp, vp, vp, C.POINTER(SolverDesc),
+                                        C.POINTER(SolveReport)]
+        L.mf_predictor_linear_extrapolation.argtypes = [vp, vp, vp, vp, C.c_double]
+        L.mf_laser_heat_source.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                           vp]
+        L.mf_compute_norm_l2.argtypes = [vp, vp, C.POINTER(C.c_double)]
+        L.mf_operator_set_transpose.argtypes = [vp, C.c_int]
+        L.mf_mass_vmult.argtypes = [vp, vp, vp]
+        L.mf_vector_dot.argtypes = [vp, vp, vp, C.POINTER(C.c_double)]
+        L.mf_heat_conductivity_gradient.argtypes = [vp, vp, vp, C.c_double, vp]
+        L.mf_laser_heat_source_radius_derivative.argtypes = [vp, C.c_double, C.c_double,
+                                                             C.POINTER(C.c_double), vp]
+        L.mf_transient_create.argtypes = [vp, C.POINTER(TransientDesc), C.POINTER(vp)]
+        L.mf_transient_destroy.argtypes = [vp]
+        L.mf_transient_forward.argtypes = [vp, vp, vp]
+        L.mf_transient_backward.argtypes = [vp, vp, vp, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -282,6 +321,10 @@ class _OperatorBase:
     def create_rhs(self, dst, src, q_s=None):
         _check(lib().mf_operator_create_rhs(self._h, _ptr(dst), _ptr(src), _ptr(q_s)))
 
+    def set_transpose(self, transpose):
+        """Apply A^T in vmult / solves (adjoint); the diagonal is unchanged."""
+        _check(lib().mf_operator_set_transpose(self._h, int(bool(transpose))))
+
     def __del__(self):
         try:
             if getattr(self, "_h", None):
@@ -316,6 +359,120 @@ class AdvectionDiffusionOperator(_OperatorBase):
         super().__init__(scratch, h)
         if dt is not None:
             self.reset_time_increment(dt)
+
+
+def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
+                 rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30, precision="fp64",
+                 inner_rel_tolerance=1e-3, max_outer=50):
+    d = SolverDesc()
+    d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
+    d.restart = restart
+    d.max_iterations = max_iterations
+    d.abs_tol = abs_tolerance
+    d.rel_tol = rel_tolerance
+    d.preconditioner = {"Identity": PRECOND_IDENTITY, "Diagonal": PRECOND_JACOBI}[preconditioner]
+    d.precision = {"fp64": PRECISION_FP64, "fp32_refined": PRECISION_FP32_REFINED}[precision]
+    d.inner_rel_tol = inner_rel_tolerance
+    d.max_outer = max_outer
+    return d
+
+
+def compute_norm_l2(scratch, u):
+    """FE L2 norm of a device vector (VectorTools::compute_norm semantics)."""
+    r = C.c_double()
+    _check(lib().mf_compute_norm_l2(scratch._h, _ptr(u), C.byref(r)))
+    return r.value
+
+
+def laser_heat_source(scratch, q_s, power, radius, position):
+    """Nodal Gaussian volumetric laser source (LaserHeatSourceVolumetric)."""
+    pos = (C.c_double * 3)(*position)
+    _check(lib().mf_laser_heat_source(scratch._h, power, radius, pos, _ptr(q_s)))
+
+
+def predictor_linear_extrapolation(scratch, dst, x_n, x_nm1, tau=1.0):
+    _check(lib().mf_predictor_linear_extrapolation(scratch._h, _ptr(dst), _ptr(x_n),
+                                                   _ptr(x_nm1), float(tau)))
+
+
+def heat_newton_step(operator, T, T_old, q_s=None, dirichlet_values=None, newton=None,
+                     **solver):
+    """One implicit heat step solved by Newton (HeatDiffuseOperation::solve)."""
+    nd = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
+    rep = NewtonReport()
+    code = lib().mf_heat_newton_step(operator._h, _ptr(T), _ptr(T_old), _ptr(q_s),
+                                     _ptr(dirichlet_values), C.byref(_solver_desc(**solver)),
+                                     C.byref(nd), C.byref(rep))
+    _check(code)
+    return rep.as_dict()
+
+
+def advection_step(operator, phi, phi_old, dirichlet_values=None, **solver):
+    """One advection-diffusion step with Dirichlet lifting (AdvectionDiffusionOperation::solve)."""
+    rep = SolveReport()
+    code = lib().mf_advection_step(operator._h, _ptr(phi), _ptr(phi_old), _ptr(dirichlet_values),
+                                   C.byref(_solver_desc(**solver)), C.byref(rep))
+    _check(code)
+    return rep.as_dict()
+
+
+def mass_vmult(scratch, dst, src):
+    _check(lib().mf_mass_vmult(scratch._h, _ptr(dst), _ptr(src)))
+
+
+def vector_dot(scratch, a, b):
+    r = C.c_double()
+    _check(lib().mf_vector_dot(scratch._h, _ptr(a), _ptr(b), C.byref(r)))
+    return r.value
+
+
+def conductivity_gradient(operator, lam, T, grad_k_cell, scale=1.0):
+    _check(lib().mf_heat_conductivity_gradient(operator._h, _ptr(lam), _ptr(T), float(scale),
+                                               _ptr(grad_k_cell)))
+
+
+def laser_heat_source_radius_derivative(scratch, dq, power, radius, position):
+    pos = (C.c_double * 3)(*position)
+    _check(lib().mf_laser_heat_source_radius_derivative(scratch._h, power, radius, pos, _ptr(dq)))
+
+
+class HeatTransient:
+    """Moving-laser implicit heat transient with a checkpointed hand-written adjoint
+    (C++ forward/backward behind the C ABI; no reference counterpart)."""
+
+    def __init__(self, operator, n_steps, dt, power, radius, start, scan_velocity,
+                 checkpoint_interval=5, **solver):
+        self.operator = operator
+        d = TransientDesc()
+        d.n_steps = n_steps
+        d.checkpoint_interval = checkpoint_interval
+        d.dt = dt
+        d.power = power
+        d.radius = radius
+        d.start = (C.c_double * 3)(*start)
+        d.scan_velocity = (C.c_double * 3)(*scan_velocity)
+        d.solver = _solver_desc(**solver)
+        self.desc = d
+        h = C.c_void_p()
+        _check(lib().mf_transient_create(operator._h, C.byref(d), C.byref(h)))
+        self._h = h
+
+    def forward(self, T0, T_final):
+        _check(lib().mf_transient_forward(self._h, _ptr(T0), _ptr(T_final)))
+
+    def backward(self, dJ_dTN, grad_k_cell=None):
+        gp, gr = C.c_double(), C.c_double()
+        _check(lib().mf_transient_backward(self._h, _ptr(dJ_dTN), _ptr(grad_k_cell),
+                                           C.byref(gp), C.byref(gr)))
+        return gp.value, gr.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mf_transient_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
 
 
 class LinearSolver:

@@ -141,6 +141,7 @@ static CellArgs<Number> make_args(const mf_operator *op)
   a.n_cells = int(m.n_cells);
   a.mask = s->d_mask;
   a.use_mask = op->dof_index == 0 ? 1 : 0;
+  a.transpose = op->transpose;
   a.dt_inv = Number(op->dt > 0 ? 1.0 / op->dt : 0.0);
   if (op->kind == 0) {
     a.rho_cp = Number(op->heat.rho_cp);
@@ -192,7 +193,8 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
-  if (g_tangent_variant == 1)
+  // the transposed operator (adjoint) runs on the pencil kernel
+  if (g_tangent_variant == 1 && !op->transpose)
     return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
 }
@@ -475,7 +477,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
-  const int ncb = tangent_blocks(m);
+  const int ncb = tangent_blocks(m, op->transpose != 0);
   DevBuf<T> r, p, q, prr, prz0, prz1;
   DevBuf<double> pcell;
   DevBuf<SolveState> sd;

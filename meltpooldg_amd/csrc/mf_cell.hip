@@ -139,7 +139,8 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   constexpr int P = K - 1;
   constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG);
   constexpr bool HAS_U = !IS_DIAG;
-  constexpr bool USE_T = NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG);
+  constexpr bool PGRAD = (OP == HEAT_PGRAD);
+  constexpr bool USE_T = (NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG)) || PGRAD;
   constexpr bool RHS_HEAT = (OP == HEAT_RHS);
   constexpr bool TANGENT = (OP == HEAT_TANGENT || OP == ADVEC_TANGENT);
   constexpr bool NEED_A = RHS_HEAT || IS_DIAG;
@@ -301,7 +302,10 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     }
     Number val = 0, f[3] = {0, 0, 0};
     Number cc = 0, kk = 0, bb[3] = {0, 0, 0};
-    if constexpr (OP == HEAT_TANGENT || OP == HEAT_DIAG) {
+    if constexpr (PGRAD) {
+      const Number gb0 = bTX[e] * A.inv_h[0], gb1 = bTY[e] * A.inv_h[1], gb2 = gTz[z] * A.inv_h[2];
+      val = g[0] * gb0 + g[1] * gb1 + g[2] * gb2;  // JxW applied below
+    } else if constexpr (OP == HEAT_TANGENT || OP == HEAT_DIAG) {
       Number k = A.k0, dk = 0, gT[3] = {0, 0, 0};
       if constexpr (USE_T) {
         Number t, s2;
@@ -318,12 +322,22 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       if constexpr (OP == HEAT_TANGENT) {
         const Number u = uq[z];
         val = A.dt_inv * A.rho_cp * u;
-        if constexpr (VEL) val += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[d] = k * g[d];
-        if (td) {
+        if (!A.transpose) {
+          // (rho cp w . grad u) v + dk/dT u (grad T . grad v)
+          if constexpr (VEL) val += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
+          if (td) {
 #pragma unroll
-          for (int d = 0; d < 3; ++d) f[d] += dk * gT[d] * u;
+            for (int d = 0; d < 3; ++d) f[d] += dk * gT[d] * u;
+          }
+        } else {
+          // transposed: (rho cp w u) . grad v + dk/dT (grad T . grad u) v
+          if constexpr (VEL) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] += A.rho_cp * w3[d] * u;
+          }
+          if (td) val += dk * (gT[0] * g[0] + gT[1] * g[1] + gT[2] * g[2]);
         }
       } else {
         cc = A.dt_inv * A.rho_cp;
@@ -353,9 +367,15 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       for (int d = 0; d < 3; ++d) f[d] = -k * g[d];
     } else if constexpr (OP == ADVEC_TANGENT) {
       const Number vg = g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2];
-      val = A.mass_factor * A.dt_inv * uq[z] + A.theta * vg;
+      val = A.mass_factor * A.dt_inv * uq[z];
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] = A.theta * A.diffusivity * g[d];
+      if (!A.transpose) {
+        val += A.theta * vg;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] += A.theta * w3[d] * uq[z];
+      }
     } else if constexpr (OP == ADVEC_RHS) {
       const Number vg = g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2];
       val = A.dt_inv * uq[z] - (Number(1) - A.theta) * vg;
@@ -379,6 +399,21 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       bGY[e] = jxw * f[1] * A.inv_h[1];
       fz[z] = jxw * f[2] * A.inv_h[2];
     }
+  }
+
+  if constexpr (PGRAD) {
+    // per-cell sum of the K^2 z-pencil partials (bU is free after E4)
+    Number part = 0;
+#pragma unroll
+    for (int z = 0; z < K; ++z) part += rq[z];
+    bU[p * CPB + c] = part;
+    __syncthreads();
+    if (p == 0 && active) {
+      double sum = 0;
+      for (int i = 0; i < K * K; ++i) sum += double(bU[i * CPB + c]);
+      dst[cell] += Number(A.pgrad_scale * sum);
+    }
+    return;
   }
 
   if constexpr (IS_DIAG) {
@@ -517,6 +552,7 @@ static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const Cel
   case ADVEC_TANGENT: return launch_op<Number, K, ADVEC_TANGENT>(needT, vel, m, a, src, dst, st);
   case ADVEC_RHS: return launch_op<Number, K, ADVEC_RHS>(needT, vel, m, a, src, dst, st);
   case ADVEC_DIAG: return launch_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st);
+  case HEAT_PGRAD: return launch_k<Number, K, HEAT_PGRAD, false, false>(m, a, src, dst, st);
   default: set_error("unknown cell operation"); return MF_ERR_ARG;
   }
 }

@@ -39,7 +39,8 @@ enum CellOp : int {
   HEAT_DIAG = 2,     // heat_diffuse_operator.cpp:297-391 (diagonal of the tangent)
   ADVEC_TANGENT = 3, // advection_diffusion_operator.cpp:448-495
   ADVEC_RHS = 4,     // advection_diffusion_operator.cpp:274-358
-  ADVEC_DIAG = 5     // advection_diffusion_operator.cpp:399-446
+  ADVEC_DIAG = 5,    // advection_diffusion_operator.cpp:399-446
+  HEAT_PGRAD = 6     // per-cell sum_q JxW grad(src) . grad(T): conductivity gradient (adjoint)
 };
 
 // Everything a cell kernel needs besides src/dst; passed by value (kernel argument segment).
@@ -67,6 +68,8 @@ struct CellArgs {
   const Number *vel;       // nodal velocity [3][n_dofs]
   const Number *k_cell;    // per-cell conductivity factor
   const Number *aux1;      // heat rhs: q_s (nullable)
+  int transpose;           // tangent ops: apply the transposed operator (adjoint)
+  double pgrad_scale;      // HEAT_PGRAD: dst[cell] += pgrad_scale * sum_q JxW grad src . grad T
   double *dot_partial;     // fused src . (A src) per block (nullable)
   const int *stop;         // early-exit flag set by the solver (nullable)
 };
@@ -101,9 +104,9 @@ int slab_blocks(const MeshInfo &m);
 
 // tangent kernel selection: 0 = pencil kernel (mf_cell.hip), 1 = slice-resident (mf_slab.hip)
 extern int g_tangent_variant;
-inline int tangent_blocks(const MeshInfo &m)
+inline int tangent_blocks(const MeshInfo &m, bool transpose = false)
 {
-  return g_tangent_variant == 1 ? slab_blocks(m) : cell_blocks(m);
+  return (g_tangent_variant == 1 && !transpose) ? slab_blocks(m) : cell_blocks(m);
 }
 
 }  // namespace mf
@@ -126,4 +129,5 @@ struct mf_operator {
   const double *T = nullptr, *vel = nullptr, *k_cell = nullptr;
   float *T32 = nullptr, *vel32 = nullptr, *kc32 = nullptr;  // FP32 copies of the fields
   bool fields32_valid = false;
+  int transpose = 0;  // vmult applies A^T (adjoint solves)
 };

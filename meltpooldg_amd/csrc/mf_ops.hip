@@ -1,0 +1,382 @@
+// Device kernels and C-ABI entry points for the callers on either side of the operator:
+// one advection-diffusion time step with Dirichlet lifting, one Newton-solved implicit heat step,
+// the linear-extrapolation predictor, the volumetric Gaussian laser source and the FE L2 norm.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mf_internal.hpp"
+#include "mf_vec.hpp"
+
+namespace mf {
+
+// ------------------------------------------------------------------------------------------
+// small vector kernels
+// ------------------------------------------------------------------------------------------
+// y = a x + b y
+__global__ void axpby_kernel(double a, const double *x, double b, double *y, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+// dst[i] = mask[i] ? (vals ? vals[i] : 0) : keep (when keep_free) or 0
+__global__ void masked_assign_kernel(double *dst, const uint8_t *mask, const double *vals,
+                                     int keep_free, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (mask && mask[i])
+      dst[i] = vals ? vals[i] : 0.0;
+    else if (!keep_free)
+      dst[i] = 0.0;
+  }
+}
+
+static dim3 vgrid(int64_t n)
+{
+  int64_t b = (n + VEC_THREADS - 1) / VEC_THREADS;
+  return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>(b, RED_BLOCKS))));
+}
+
+int vec_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(axpby_kernel, vgrid(n), dim3(VEC_THREADS), 0, st, a, x, b, y, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+int vec_masked_assign(double *dst, const uint8_t *mask, const double *vals, bool keep_free,
+                      int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(masked_assign_kernel, vgrid(n), dim3(VEC_THREADS), 0, st, dst, mask, vals,
+                     keep_free ? 1 : 0, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// laser heat source: nodal interpolation of P/(r sqrt(pi/2))^3 exp(-2 (d/r)^2)
+// (GaussVolumetricIntensityProfile, include/meltpooldg/heat/laser_intensity_profiles.hpp:175-218;
+//  LaserHeatSourceVolumetric::compute_volumetric_heat_source,
+//  source/heat/laser_heat_source_volumetric.cpp:20-58)
+// ------------------------------------------------------------------------------------------
+struct NodeGeo {
+  double lo[3], h[3];
+  int64_t NN[3];
+  int ncell[3];
+  int p;
+  double gl[KMAX];
+};
+
+__device__ __forceinline__ double node_coord(const NodeGeo &g, int d, int64_t i)
+{
+  int64_t c = i / g.p;
+  if (c >= g.ncell[d]) c = g.ncell[d] - 1;
+  return g.lo[d] + g.h[d] * (double(c) + g.gl[i - c * g.p]);
+}
+
+__global__ void laser_kernel(NodeGeo g, double peak, double inv_r2, double x0, double y0,
+                             double z0, double *q, int64_t n, double radius, int d_dr)
+{
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t ix = idx % g.NN[0], iy = (idx / g.NN[0]) % g.NN[1], iz = idx / (g.NN[0] * g.NN[1]);
+    const double dx = node_coord(g, 0, ix) - x0, dy = node_coord(g, 1, iy) - y0,
+                 dz = node_coord(g, 2, iz) - z0;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double v = peak * exp(-2.0 * d2 * inv_r2);
+    // d/dr of P/(r sqrt(pi/2))^3 exp(-2 d^2/r^2) = q (-3/r + 4 d^2/r^3)
+    q[idx] = d_dr ? v * (-3.0 / radius + 4.0 * d2 * inv_r2 / radius) : v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// FE L2 norm: per cell sqrt(sum_q u_q^2 JxW) stored in single precision (the reference's
+// integrate_difference writes a Vector<float>, source/utilities/vector_tools.cpp:137-168), then
+// the square root of the sum of squares.  One thread per cell, tensor in registers.
+// ------------------------------------------------------------------------------------------
+template <int K>
+__global__ void l2_norm_kernel(CellArgs<double> A, const double *u, double *part)
+{
+  constexpr int P = K - 1;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  double e2 = 0.0;
+  if (cell < A.n_cells) {
+    const int cx = cell % A.ncell[0], cy = (cell / A.ncell[0]) % A.ncell[1],
+              cz = cell / (A.ncell[0] * A.ncell[1]);
+    double v[K][K][K], w1[K][K][K];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          v[k][j][i] = u[uint32_t(cx * P + i) +
+                         A.NX * (uint32_t(cy * P + j) + A.NY * uint32_t(cz * P + k))];
+    // x, y, z interpolation
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+          double s = 0;
+#pragma unroll
+          for (int b = 0; b < K; ++b) s += A.S[a * K + b] * v[k][j][b];
+          w1[k][j][a] = s;
+        }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          double s = 0;
+#pragma unroll
+          for (int b = 0; b < K; ++b) s += A.S[a * K + b] * w1[k][b][i];
+          v[k][a][i] = s;
+        }
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          double s = 0;
+#pragma unroll
+          for (int b = 0; b < K; ++b) s += A.S[a * K + b] * v[b][j][i];
+          acc += s * s * (A.w[i] * A.w[j] * A.w[a]);
+        }
+    acc *= A.h[0] * A.h[1] * A.h[2];
+    const float ef = float(sqrt(acc));
+    e2 = double(ef) * double(ef);
+  }
+  // deterministic block partial
+  __shared__ double red[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e2;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void sum_sqrt_kernel(const double *part, int n, double *out)
+{
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += part[i];
+    *out = sqrt(s);
+  }
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int mf_operator_vmult(mf_operator *op, double *dst, const double *src);
+int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
+int mf_operator_reset_dof_index(mf_operator *op, int dof_index);
+int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
+                           const double *k_cell);
+int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
+             mf_solve_report *report);
+
+int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *x_n,
+                                      const double *x_nm1, double tau)
+{
+  MF_REQUIRE(s && dst && x_n && x_nm1, MF_ERR_ARG, "null argument");
+  const int64_t n = s->mesh.n_dofs;
+  MF_HIP_TRY(hipMemcpyAsync(dst, x_n, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  // dst = (1 + tau) x_n - tau x_{n-1}
+  int rc = vec_axpby(-tau, x_nm1, 1.0 + tau, dst, n, s->stream);
+  return rc;
+}
+
+static int laser_impl(mf_scratch *s, double power, double radius, const double position[3],
+                      double *q_s, int d_dr)
+{
+  MF_REQUIRE(s && position && q_s, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(radius > 0.0, MF_ERR_ARG, "The laser beam radius must be greater than zero! Abort..");
+  const MeshInfo &m = s->mesh;
+  NodeGeo g{};
+  for (int d = 0; d < 3; ++d) {
+    g.lo[d] = m.lo[d];
+    g.h[d] = m.h[d];
+    g.NN[d] = m.NN[d];
+    g.ncell[d] = m.ncell[d];
+  }
+  g.p = m.degree;
+  for (int i = 0; i < m.K; ++i) g.gl[i] = s->tab.gl[i];
+  const double a = radius * std::sqrt(0.5 * M_PI);
+  const double peak = power / (a * a * a);
+  hipLaunchKernelGGL(laser_kernel, vgrid(m.n_dofs), dim3(VEC_THREADS), 0, s->stream, g, peak,
+                     1.0 / (radius * radius), position[0], position[1], position[2], q_s,
+                     m.n_dofs, radius, d_dr);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+int mf_laser_heat_source(mf_scratch *s, double power, double radius, const double position[3],
+                         double *q_s)
+{
+  return laser_impl(s, power, radius, position, q_s, 0);
+}
+
+int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double radius,
+                                           const double position[3], double *dq_dr)
+{
+  return laser_impl(s, power, radius, position, dq_dr, 1);
+}
+
+int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result)
+{
+  MF_REQUIRE(s && u && result, MF_ERR_ARG, "null argument");
+  const MeshInfo &m = s->mesh;
+  CellArgs<double> a;
+  std::memset(&a, 0, sizeof(a));
+  for (int i = 0; i < m.K * m.K; ++i) a.S[i] = s->tab.S[i];
+  for (int i = 0; i < m.K; ++i) a.w[i] = s->tab.w[i];
+  for (int d = 0; d < 3; ++d) {
+    a.ncell[d] = m.ncell[d];
+    a.h[d] = m.h[d];
+  }
+  a.NX = uint32_t(m.NN[0]);
+  a.NY = uint32_t(m.NN[1]);
+  a.n_cells = int(m.n_cells);
+  const int threads = 256;
+  const int blocks = int((m.n_cells + threads - 1) / threads);
+  double *part = nullptr, *out = nullptr;
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * (blocks + 1), s->stream));
+  out = part + blocks;
+  switch (m.K) {
+  case 2: hipLaunchKernelGGL(l2_norm_kernel<2>, dim3(blocks), dim3(threads), 0, s->stream, a, u, part); break;
+  case 3: hipLaunchKernelGGL(l2_norm_kernel<3>, dim3(blocks), dim3(threads), 0, s->stream, a, u, part); break;
+  default: hipLaunchKernelGGL(l2_norm_kernel<4>, dim3(blocks), dim3(threads), 0, s->stream, a, u, part); break;
+  }
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(64), 0, s->stream, part, blocks, out);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipMemcpyAsync(result, out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  MF_HIP_TRY(hipFreeAsync(part, s->stream));
+  MF_HIP_TRY(hipStreamSynchronize(s->stream));
+  return MF_OK;
+}
+
+// One time step of AdvectionDiffusionOperation::solve
+// (source/level_set/advection_diffusion_operation.cpp:227-367) with the Dirichlet lifting of
+// create_rhs_and_apply_dirichlet_matrixfree (include/meltpooldg/linear_algebra/
+// utilities_matrixfree.hpp:33-100): rhs = b(old) - A_noBC x_d, constrained rows zeroed; solve;
+// constrained entries set to their values (AffineConstraints::distribute).
+int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
+                      const double *dirichlet_values, const mf_solver_desc *desc,
+                      mf_solve_report *report)
+{
+  MF_REQUIRE(op && phi && phi_old && desc, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(op->kind == 1, MF_ERR_ARG, "mf_advection_step needs an advection operator");
+  mf_scratch *s = op->s;
+  const int64_t n = s->mesh.n_dofs;
+  hipStream_t st = s->stream;
+  double *rhs = nullptr, *xd = nullptr;
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&rhs), sizeof(double) * n, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&xd), sizeof(double) * n, st));
+  int rc = MF_OK;
+  const int saved = op->dof_index;
+  do {
+    // x_d: Dirichlet values on constrained entries, zero elsewhere
+    if ((rc = vec_masked_assign(xd, s->d_mask, dirichlet_values, false, n, st))) break;
+    op->dof_index = 1;
+    if ((rc = mf_operator_vmult(op, rhs, xd))) break;
+    if ((rc = vec_axpby(0.0, rhs, -1.0, rhs, n, st))) break;
+    if ((rc = mf_operator_create_rhs(op, rhs, phi_old, nullptr))) break;
+    op->dof_index = 0;
+    if ((rc = vec_masked_assign(rhs, s->d_mask, nullptr, true, n, st))) break;
+    rc = mf_solve(op, desc, phi, rhs, report);
+    if (rc != MF_OK && rc != MF_NO_CONVERGENCE && rc != MF_BREAKDOWN) break;
+    const int solve_rc = rc;
+    if ((rc = vec_masked_assign(phi, s->d_mask, dirichlet_values, true, n, st))) break;
+    rc = solve_rc;
+  } while (false);
+  op->dof_index = saved;
+  (void)hipFreeAsync(rhs, st);
+  (void)hipFreeAsync(xd, st);
+  return rc;
+}
+
+// One implicit-Euler heat step solved with NewtonRaphsonSolver::solve
+// (source/linear_algebra/newton_raphson_solver.cpp:22-80, wired as in
+// source/heat/heat_diffuse_operation.cpp:86-116,261-289): the preconditioner is updated once,
+// then r = -R(T) (create_rhs), solve J d = r, converged when |r| < residual_tol and
+// |d| < correction_tol (checked before d is added), else T += d and the constraints are
+// distributed.  T holds the predictor on entry and the converged temperature on exit; the
+// operator's linearisation field is T itself.
+int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const double *q_s,
+                        const double *dirichlet_values, const mf_solver_desc *desc,
+                        const mf_newton_desc *newton, mf_newton_report *report)
+{
+  MF_REQUIRE(op && T && T_old && desc && newton, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "mf_heat_newton_step needs a heat operator");
+  mf_scratch *s = op->s;
+  const int64_t n = s->mesh.n_dofs;
+  hipStream_t st = s->stream;
+  double *r = nullptr, *d = nullptr, *nrm = nullptr, *part = nullptr;
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&r), sizeof(double) * n, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&d), sizeof(double) * n, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&nrm), sizeof(double) * 2, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * 2 * RED_BLOCKS, st));
+  mf_newton_report rep{};
+  int rc = MF_OK;
+  const int max_it = newton->max_iterations > 0 ? newton->max_iterations : 10;
+  const double tol_r = newton->residual_tolerance > 0 ? newton->residual_tolerance : 1e-9;
+  const double tol_d = newton->correction_tolerance > 0 ? newton->correction_tolerance : 1e-10;
+  // constraints of the predictor (AffineConstraints::distribute)
+  if (dirichlet_values) rc = vec_masked_assign(T, s->d_mask, dirichlet_values, true, n, st);
+  const double *vel = op->vel, *kc = op->k_cell;
+  bool converged = false;
+  for (int it = 0; rc == MF_OK && it < max_it; ++it) {
+    if ((rc = mf_operator_set_fields(op, T, vel, kc))) break;
+    if ((rc = vec_fill<double>(r, 0.0, n, st))) break;
+    if ((rc = mf_operator_create_rhs(op, r, T_old, q_s))) break;
+    if ((rc = vec_fill<double>(d, 0.0, n, st))) break;
+    mf_solve_report lin{};
+    rc = mf_solve(op, desc, d, r, &lin);
+    rep.linear_iterations += lin.iterations;
+    if (rc != MF_OK) break;
+    double h[2];
+    if ((rc = vec_norm<double>(r, n, part, nrm, st))) break;
+    if ((rc = vec_norm<double>(d, n, part, nrm + 1, st))) break;
+    MF_HIP_TRY(hipMemcpyAsync(h, nrm, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    rep.iterations = it + 1;
+    rep.residual_norm = h[0];
+    rep.correction_norm = h[1];
+    if (h[0] < tol_r && h[1] < tol_d) {
+      converged = true;
+      break;
+    }
+    if ((rc = vec_axpby(1.0, d, 1.0, T, n, st))) break;
+  }
+  (void)hipFreeAsync(r, st);
+  (void)hipFreeAsync(d, st);
+  (void)hipFreeAsync(nrm, st);
+  (void)hipFreeAsync(part, st);
+  if (rc == MF_OK) rc = mf_operator_set_fields(op, T, vel, kc);
+  if (rc == MF_OK) MF_HIP_TRY(hipStreamSynchronize(st));
+  rep.status = rc != MF_OK ? rc : (converged ? MF_OK : MF_NO_CONVERGENCE);
+  if (report) *report = rep;
+  if (rc == MF_OK && !converged) {
+    set_error("Newton-Raphson solver did not converge (ExcNewtonDidNotConverge)");
+    return MF_NO_CONVERGENCE;
+  }
+  return rc;
+}
+
+}  // extern "C"

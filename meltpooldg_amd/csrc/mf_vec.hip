@@ -393,6 +393,14 @@ __global__ void precond_kernel(T *z, const T *v, const T *inv, int64_t n)
     z[i] = inv ? inv[i] * v[i] : v[i];
 }
 
+// 1-block: s = sum of partials
+template <typename T>
+__global__ void sum_from_partials_kernel(const T *part, int nparts, double *out)
+{
+  const T s = reduce_partials(part, nparts);
+  if (threadIdx.x == 0) *out = double(s);
+}
+
 // 1-block: s = sqrt(sum of partials)
 template <typename T>
 __global__ void norm_from_partials_kernel(const T *part, int nparts, double *out)
@@ -484,6 +492,20 @@ int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st)
 }
 template int vec_norm<double>(const double *, int64_t, double *, double *, hipStream_t);
 template int vec_norm<float>(const float *, int64_t, float *, double *, hipStream_t);
+
+template <typename T>
+int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(dot_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, a, b, n, part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sum_from_partials_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int vec_dot<double>(const double *, const double *, int64_t, double *, double *,
+                             hipStream_t);
+template int vec_dot<float>(const float *, const float *, int64_t, float *, double *, hipStream_t);
 
 template <typename T>
 int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,

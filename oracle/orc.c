@@ -412,7 +412,7 @@ static void eval_velocity(const orc_operator *op, const geo_t *g, long cell, cel
   const long n = orc_n_dofs(&op->mesh);
   for (int e = 0; e < 3; ++e)
     for (int q = 0; q < g->nloc; ++q) ws->wv[e][q] = 0.0;
-  if (op->vel_kind == ORC_VEL_NODAL) {
+  if (op->vel_kind == ORC_VEL_NODAL && op->vel) {
     for (int e = 0; e < g->dim; ++e) {
       for (int l = 0; l < g->nloc; ++l) ws->tmp[l] = op->vel[e * n + ws->idx[l]];
       eval_v(g, ws->tmp, ws->wv[e], &ws->w);
@@ -632,7 +632,7 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
     double phi[NLOC_MAX], gphi[NLOC_MAX][3];
     for (int l = 0; l < nq; ++l) basis_at(g, l, q, &phi[l], gphi[l]);
     double w[3] = {0, 0, 0};
-    if (op->vel_kind == ORC_VEL_NODAL) {
+    if (op->vel_kind == ORC_VEL_NODAL && op->vel) {
       for (int e = 0; e < dim; ++e)
         for (int l = 0; l < nq; ++l) w[e] += op->vel[e * n + idx[l]] * phi[l];
     } else if (op->vel_kind == ORC_VEL_VORTEX) {

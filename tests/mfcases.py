@@ -41,7 +41,7 @@ def fields(case: Case):
     out["T"] = 1700.0 + 50.0 * 3.0 * orc.random_field(m, case.seed + 1) if case.law == "tanh" \
         else 300.0 + orc.random_field(m, case.seed + 1)
     out["T_old"] = out["T"] - 5.0 * orc.random_field(m, case.seed + 2)
-    out["vel"] = rng.uniform(-1, 1, size=3 * n) if case.vel else None
+    out["vel"] = rng.uniform(-1, 1, size=3 * n) if (case.vel or case.kind == "advec") else None
     out["k_cell"] = np.exp(0.3 * rng.standard_normal(orc.n_cells(m))) if case.k_cell else None
     if case.mask == "none":
         out["mask"] = np.zeros(n, dtype=np.uint8)

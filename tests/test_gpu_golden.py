@@ -1,0 +1,119 @@
+"""The device path against the reference's own golden outputs, and the device-side callers
+(Newton heat step, advection step with Dirichlet lifting, laser source, FE norm) against the
+oracle.  Golden fixtures: tests/golden/reference_norms.json (see tests/test_oracle_golden.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import orc
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_norms.json")
+
+
+def _dev(a):
+    return torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def test_advection_vortex_3d_golden():
+    """advection_diffusion_vortex_mf_3d: Q1 8^3 on [-1,1]^3, Crank-Nicolson, dt 0.01, inflow
+    Dirichlet -1, velocity (4z, 0, -4x) (linear, so its nodal Q1 interpolant is exact)."""
+    import meltpooldg_amd as mf
+    g = json.load(open(GOLDEN))["advection_diffusion_vortex_mf_3d"]["norms"]
+    s = mf.ScratchData(1, 8, (-1.0,) * 3, (1.0,) * 3, stream=torch.cuda.current_stream())
+    X = s.support_points()
+    x, z = X[:, 0], X[:, 2]
+    tol = 1e-12
+    mask = (((np.abs(x + 1) < tol) & (z >= 0)) | ((np.abs(x - 1) < tol) & (z <= 0)) |
+            ((np.abs(z - 1) < tol) & (x >= 0)) | ((np.abs(z + 1) < tol) & (x <= 0)))
+    mask = mask.astype(np.uint8)
+    s.set_dirichlet_mask(mask)
+    eps = 0.5 * (0.25 * np.sqrt(3)) / np.sqrt(3)
+    sd = -(np.linalg.norm(X - np.array([0.0, 0.0, 0.5]), axis=1) - 0.25)
+    phi0 = np.tanh(sd / (2 * eps))
+    phi0[mask == 1] = -1.0
+    vel = np.concatenate([4.0 * z, np.zeros_like(z), -4.0 * x])
+    op = mf.AdvectionDiffusionOperator(s, theta=0.5, diffusivity=0.0, dt=0.01)
+    vel_d = _dev(vel)
+    op.set_fields(velocity=vel_d)
+    dv = _dev(-np.ones(s.n_dofs))
+    phi = _dev(phi0)
+    norms = []
+    for _ in range(5):
+        old = phi.clone()
+        rep = mf.advection_step(op, phi, old, dv, solver="GMRES", rel_tolerance=1e-12)
+        assert rep["status"] == 0
+        norms.append(mf.compute_norm_l2(s, phi))
+    rel = np.abs(np.array(norms) / np.array(g, dtype=float) - 1)
+    assert rel.max() < 5e-8, (norms, rel)
+
+
+def test_heat_unidirectional_golden_extruded():
+    """unidirectional_heat_transfer (2D Q1 32^2 on [0,0.1]^2, T0 = x, adiabatic, dt 1e-4) as a
+    one-cell-thick 3D slab: the solution is independent of z, so the 3D L2 norm is the 2D norm
+    times sqrt(thickness)."""
+    import meltpooldg_amd as mf
+    g = json.load(open(GOLDEN))["unidirectional_heat_transfer"]["norms"]
+    Lz = 0.1 / 32
+    s = mf.ScratchData(1, (32, 32, 1), (0, 0, 0), (0.1, 0.1, Lz),
+                       stream=torch.cuda.current_stream())
+    X = s.support_points()
+    op = mf.HeatDiffuseOperator(s, rho_cp=1.0, k=1.0, dt=1e-4)
+    T = _dev(X[:, 0].copy())
+    norms = []
+    for _ in range(5):
+        T_old = T.clone()
+        rep = mf.heat_newton_step(op, T, T_old, newton=(10, 5e-9 * Lz, 1e-10), solver="GMRES",
+                                  rel_tolerance=1e-12)
+        assert rep["status"] == 0 and rep["iterations"] >= 1
+        norms.append(mf.compute_norm_l2(s, T) / np.sqrt(Lz))
+    assert [f"{v:.5e}" for v in norms] == g
+
+
+def test_newton_step_tanh_matches_oracle():
+    import meltpooldg_amd as mf
+    nc = (5, 4, 4)
+    s = mf.ScratchData(2, nc, stream=torch.cuda.current_stream())
+    mask = s.boundary_mask()
+    s.set_dirichlet_mask(mask)
+    X = s.support_points()
+    T0 = 1650.0 + 100.0 * X[:, 0] + 20 * s.random_field(3)
+    T0[mask == 1] = 1650.0 + 100.0 * X[mask == 1, 0]
+    op = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                                tanh_Tc=1700.0, tanh_width=50.0, t_dependent=True, dt=1e-2)
+    T = _dev(T0)
+    T_old = _dev(T0)
+    rep = mf.heat_newton_step(op, T, T_old, dirichlet_values=_dev(T0), solver="GMRES",
+                              rel_tolerance=1e-12)
+    assert rep["status"] == 0
+    m = orc.make_mesh(3, 2, nc)
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=1700.0, tanh_width=50.0, dt=1e-2)
+    ref = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, mask=mask)
+    T_ref, _, st = ref.heat_implicit_euler(T0, 1)
+    assert st == 0
+    got = T.cpu().numpy()
+    assert np.max(np.abs(got - T_ref)) / np.max(np.abs(T_ref - T0)) < 1e-8
+
+
+def test_laser_source_and_norm():
+    import meltpooldg_amd as mf
+    s = mf.ScratchData(3, (6, 5, 4), stream=torch.cuda.current_stream())
+    q = torch.zeros(s.n_dofs, dtype=torch.float64, device="cuda")
+    mf.laser_heat_source(s, q, power=2.0, radius=0.2, position=(0.4, 0.5, 1.0))
+    X = s.support_points()
+    d2 = ((X - np.array([0.4, 0.5, 1.0])) ** 2).sum(axis=1)
+    expect = 2.0 / (0.2 * np.sqrt(np.pi / 2)) ** 3 * np.exp(-2 * d2 / 0.2 ** 2)
+    assert np.max(np.abs(q.cpu().numpy() - expect)) <= 1e-13 * expect.max()
+    m = orc.make_mesh(3, 3, (6, 5, 4))
+    assert abs(mf.compute_norm_l2(s, q) / orc.fe_l2_norm(m, expect) - 1) < 1e-7
+    # predictor: (1 + tau) x_n - tau x_{n-1}
+    a, b = _dev(s.random_field(1)), _dev(s.random_field(2))
+    out = torch.zeros_like(a)
+    mf.predictor_linear_extrapolation(s, out, a, b, tau=0.5)
+    torch.cuda.synchronize()
+    assert torch.allclose(out, 1.5 * a - 0.5 * b, rtol=0, atol=1e-15)
