@@ -34,7 +34,7 @@ enum {
 const char *mf_last_error(void);
 int mf_version(void);
 /* Tuning knob without reference counterpart: which device kernel applies the tangent
- * (0 = pencil kernel, one LDS round trip per 1D sweep; 1 = slice-resident kernel, default).
+ * (0 = pencil kernel, one LDS round trip per 1D sweep, default; 1 = slice-resident kernel).
  * Both compute the same operator; process-wide setting. */
 int mf_set_tangent_kernel(int variant);
 
@@ -228,7 +228,8 @@ int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double r
 /* vmult / solves apply A^T instead of A when transpose = 1 (a distinct device code path for
  * the non-symmetric convection and dk/dT terms). The diagonal is the same for both. */
 int mf_operator_set_transpose(mf_operator *op, int transpose);
-/* dst = M src (mass matrix, Dirichlet rows zero). */
+/* dst = M src (mass matrix; src read without constraints, as the heat residual reads its
+ * source term; Dirichlet rows of dst zero). */
 int mf_mass_vmult(mf_scratch *s, double *dst, const double *src);
 /* result = a . b (deterministic device reduction; result on the host). */
 int mf_vector_dot(mf_scratch *s, const double *a, const double *b, double *result);

@@ -174,7 +174,7 @@ def splitmix64(x):
 
 
 def set_tangent_kernel(variant):
-    """0 = pencil kernel, 1 = slice-resident kernel (default); same operator."""
+    """0 = pencil kernel (default), 1 = slice-resident kernel; same operator."""
     _check(lib().mf_set_tangent_kernel(int(variant)))
 
 

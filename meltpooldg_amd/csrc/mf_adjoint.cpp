@@ -62,6 +62,7 @@ int mf_mass_vmult(mf_scratch *s, double *dst, const double *src)
   m.heat.rho_cp = 1.0;
   m.heat.k = 0.0;
   m.dt = 1.0;
+  m.gather_plain = 1;  // sources are read plain, as the heat residual reads q_s
   return mf_operator_vmult(&m, dst, src);
 }
 

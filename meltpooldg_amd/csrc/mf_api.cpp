@@ -142,6 +142,7 @@ static CellArgs<Number> make_args(const mf_operator *op)
   a.mask = s->d_mask;
   a.use_mask = op->dof_index == 0 ? 1 : 0;
   a.transpose = op->transpose;
+  a.gather_plain = op->gather_plain;
   a.dt_inv = Number(op->dt > 0 ? 1.0 / op->dt : 0.0);
   if (op->kind == 0) {
     a.rho_cp = Number(op->heat.rho_cp);
@@ -199,7 +200,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
 }
 
-int g_tangent_variant = 1;
+int g_tangent_variant = 0;
 
 }  // namespace mf
 

@@ -29,6 +29,13 @@ constexpr int cells_per_block()
   return K == 2 ? 64 : 16;
 }
 
+// Brick of cells served by one workgroup (BX x BY x BZ = cells_per_block).  Cell outputs are
+// summed into an LDS node brick; nodes no other brick touches are stored directly, nodes on a
+// face shared with a neighbouring brick are added with global atomics.
+template <int K> constexpr int brick_x() { return 4; }
+template <int K> constexpr int brick_y() { return K == 2 ? 4 : 2; }
+template <int K> constexpr int brick_z() { return K == 2 ? 4 : 2; }
+
 template <int K>
 __device__ __forceinline__ int elem(int i, int j, int k)
 {
@@ -92,6 +99,12 @@ __device__ __forceinline__ void sweep(const Number *M, const Number *in, Number 
 
 __device__ __forceinline__ void atomic_add(double *a, double v) { unsafeAtomicAdd(a, v); }
 __device__ __forceinline__ void atomic_add(float *a, float v) { unsafeAtomicAdd(a, v); }
+// LDS accumulation (ds_add_f64 / ds_add_f32)
+template <typename Number>
+__device__ __forceinline__ void lds_add(Number *a, Number v)
+{
+  atomicAdd(a, v);
+}
 
 // tanh((T-Tc)/w) and sech^2 from one expm1: t = -em/(2+em), s2 = 4(1+em)/(2+em)^2 with
 // em = expm1(-2|x|).  Beyond |x| = 20 (double) / 10 (float) tanh is +-1 to the last bit and the
@@ -130,6 +143,36 @@ __device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
   return s;
 }
 
+// After all cells of the brick have added into the LDS node brick: write it out.  Constrained
+// rows are skipped (distribute_local_to_global semantics).  A node on a brick face that is not
+// a domain face is shared with a neighbouring brick and is added atomically; every other node
+// belongs to this brick alone and is added with a plain read-modify-write.
+template <typename Number, int K, int BX, int BY, int BZ>
+__device__ __forceinline__ void brick_scatter(const CellArgs<Number> &A, const Number *nodeb,
+                                              int bx, int by, int bz, Number *dst)
+{
+  constexpr int P = K - 1;
+  constexpr int NBX = BX * P + 1, NBY = BY * P + 1, NBZ = BZ * P + 1;
+  const int NX = int(A.NX), NY = int(A.NY), NZ = A.ncell[2] * P + 1;
+  const int x0 = bx * BX * P, y0 = by * BY * P, z0 = bz * BZ * P;
+  __syncthreads();
+  for (int e = threadIdx.x; e < NBX * NBY * NBZ; e += blockDim.x) {
+    const int ix = e % NBX, iy = (e / NBX) % NBY, iz = e / (NBX * NBY);
+    const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
+    if (gx >= NX || gy >= NY || gz >= NZ) continue;
+    const uint32_t gi = uint32_t(gx) + A.NX * (uint32_t(gy) + A.NY * uint32_t(gz));
+    if (A.use_mask && A.mask && A.mask[gi]) continue;
+    const bool shared = (ix == 0 && gx > 0) || (ix == NBX - 1 && gx < NX - 1) ||
+                        (iy == 0 && gy > 0) || (iy == NBY - 1 && gy < NY - 1) ||
+                        (iz == 0 && gz > 0) || (iz == NBZ - 1 && gz < NZ - 1);
+    const Number v = nodeb[e];
+    if (shared)
+      atomic_add(dst + gi, v);
+    else
+      dst[gi] += v;
+  }
+}
+
 template <typename Number, int K, int OP, bool NEEDT, bool VEL>
 __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst)
@@ -159,15 +202,24 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 
   if (A.stop && *A.stop) return;  // uniform over the grid
 
+  constexpr int BX = brick_x<K>(), BY = brick_y<K>(), BZ = brick_z<K>();
+  constexpr int NBX = BX * P + 1, NBY = BY * P + 1, NBZ = BZ * P + 1;
+  constexpr int NB = NBX * NBY * NBZ;
+  __shared__ Number nodeb[(PGRAD) ? 1 : NB];
+
   const int c = threadIdx.x % CPB;
   const int p = threadIdx.x / CPB;
-  const int cell = blockIdx.x * CPB + c;
-  const bool active = cell < A.n_cells;
-  const int cx = cell % A.ncell[0];
-  const int cy = (cell / A.ncell[0]) % A.ncell[1];
-  const int cz = cell / (A.ncell[0] * A.ncell[1]);
+  const int nbx = (A.ncell[0] + BX - 1) / BX, nby = (A.ncell[1] + BY - 1) / BY;
+  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
+  const int lx = c % BX, ly = (c / BX) % BY, lz = c / (BX * BY);
+  const int cx = bx * BX + lx, cy = by * BY + ly, cz = bz * BZ + lz;
+  const bool active = cx < A.ncell[0] && cy < A.ncell[1] && cz < A.ncell[2];
+  const int cell = cx + A.ncell[0] * (cy + A.ncell[1] * cz);
   const Number *S = A.S;
   const Number *Dq = A.Dq;
+  if constexpr (!PGRAD) {
+    for (int e = threadIdx.x; e < NB; e += blockDim.x) nodeb[e] = Number(0);
+  }
 
   // ---------------- E1: x pencils, gather from HBM ----------------
   // pencil p -> (j, k) = (p % K, p / K); nodes (cx*P + i, cy*P + j, cz*P + k)
@@ -183,7 +235,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
         Number v = 0;
         if (active) {
           v = in[gbase + i];
-          if (TANGENT && A.use_mask && A.mask && A.mask[gbase + i]) v = 0;
+          if (TANGENT && A.use_mask && !A.gather_plain && A.mask && A.mask[gbase + i]) v = 0;
         }
         u_loc[i] = v;
       }
@@ -443,12 +495,10 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       }
     if (active) {
 #pragma unroll
-      for (int ia = 0; ia < K; ++ia) {
-        const uint32_t gi = gbase + ia;
-        if (A.use_mask && A.mask && A.mask[gi]) continue;
-        atomic_add(dst + gi, diag[ia]);
-      }
+      for (int ia = 0; ia < K; ++ia)
+        lds_add(&nodeb[(lx * P + ia) + NBX * ((ly * P + jb) + NBY * (lz * P + kb))], diag[ia]);
     }
+    brick_scatter<Number, K, BX, BY, BZ>(A, nodeb, bx, by, bz, dst);
     return;
   } else {
     // r = val + Dq_z^T fz (z direction in registers)
@@ -490,14 +540,15 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     }
     Number local_dot = 0;
     if (active) {
+      const int jb = p % K, kb = p / K;
 #pragma unroll
       for (int i = 0; i < K; ++i) {
-        const uint32_t gi = gbase + i;
-        if (A.use_mask && A.mask && A.mask[gi]) continue;
-        atomic_add(dst + gi, o[i]);
+        lds_add(&nodeb[(lx * P + i) + NBX * ((ly * P + jb) + NBY * (lz * P + kb))], o[i]);
+        // u_loc is zero on constrained entries, so they do not contribute to the dot
         if constexpr (TANGENT) local_dot += u_loc[i] * o[i];
       }
     }
+    brick_scatter<Number, K, BX, BY, BZ>(A, nodeb, bx, by, bz, dst);
     if constexpr (TANGENT) {
       if (A.dot_partial) {
         const double s = block_reduce_sum<double>(double(local_dot), reinterpret_cast<double *>(red));
@@ -507,10 +558,17 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   }
 }
 
+template <int K>
+static int brick_blocks(const MeshInfo &m)
+{
+  const int bx = brick_x<K>(), by = brick_y<K>(), bz = brick_z<K>();
+  return ((m.ncell[0] + bx - 1) / bx) * ((m.ncell[1] + by - 1) / by) *
+         ((m.ncell[2] + bz - 1) / bz);
+}
+
 int cell_blocks(const MeshInfo &m)
 {
-  const int cpb = m.K == 2 ? cells_per_block<2>() : cells_per_block<4>();
-  return int((m.n_cells + cpb - 1) / cpb);
+  return m.K == 2 ? brick_blocks<2>(m) : (m.K == 3 ? brick_blocks<3>(m) : brick_blocks<4>(m));
 }
 
 template <typename Number, int K, int OP, bool NEEDT, bool VEL>
@@ -518,7 +576,7 @@ static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *
                     hipStream_t st)
 {
   constexpr int CPB = cells_per_block<K>();
-  const int blocks = int((m.n_cells + CPB - 1) / CPB);
+  const int blocks = brick_blocks<K>(m);
   hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL>), dim3(blocks), dim3(CPB * K * K), 0,
                      st, a, src, dst);
   MF_HIP_TRY(hipGetLastError());

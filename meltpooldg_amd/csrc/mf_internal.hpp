@@ -69,6 +69,7 @@ struct CellArgs {
   const Number *k_cell;    // per-cell conductivity factor
   const Number *aux1;      // heat rhs: q_s (nullable)
   int transpose;           // tangent ops: apply the transposed operator (adjoint)
+  int gather_plain;        // tangent ops: read src without constraints (rows still skipped)
   double pgrad_scale;      // HEAT_PGRAD: dst[cell] += pgrad_scale * sum_q JxW grad src . grad T
   double *dot_partial;     // fused src . (A src) per block (nullable)
   const int *stop;         // early-exit flag set by the solver (nullable)
@@ -130,4 +131,5 @@ struct mf_operator {
   float *T32 = nullptr, *vel32 = nullptr, *kc32 = nullptr;  // FP32 copies of the fields
   bool fields32_valid = false;
   int transpose = 0;  // vmult applies A^T (adjoint solves)
+  int gather_plain = 0;  // vmult reads src without constraints (source mass products)
 };

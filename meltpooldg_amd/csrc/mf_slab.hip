@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(slab_cells_per_block<K>() * K)
         if (active) {
           const uint32_t gi = zbase + j * NX + i;
           x = src[gi];
-          if (A.use_mask && A.mask && A.mask[gi]) x = 0;
+          if (A.use_mask && !A.gather_plain && A.mask && A.mask[gi]) x = 0;
         }
         v[j][i] = x;
       }

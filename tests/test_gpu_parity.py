@@ -41,7 +41,7 @@ def variant(request):
     import meltpooldg_amd as mf
     mf.set_tangent_kernel(request.param)
     yield request.param
-    mf.set_tangent_kernel(1)
+    mf.set_tangent_kernel(0)
 
 
 @pytest.mark.parametrize("case", CASES, ids=ids)
