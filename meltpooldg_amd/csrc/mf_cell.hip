@@ -18,7 +18,11 @@
 // barriers.  Cell tensors live in LDS as [K^3][CPB] arrays.  Gradients at the quadrature
 // points use the collocation derivative Dq on the Gauss points (grad = Dq (S u)), which equals
 // (DS u) exactly in exact arithmetic because K points determine the degree-p polynomial.
-// The scatter-add uses hardware FP64/FP32 atomics into a zeroed destination.
+// Scatter without atomics: a workgroup is a strip of CPB cells consecutive in x, and the strips
+// are launched in 8 colours by the parity of (strip x index, cell y, cell z).  Strips of one
+// colour share no node, so each adds its nodes with a plain read-modify-write; the node shared
+// by x-neighbours inside a strip is combined with a lane shuffle and written once.  The 8
+// launches run in stream order.
 #include "mf_internal.hpp"
 
 namespace mf {
@@ -28,13 +32,6 @@ constexpr int cells_per_block()
 {
   return K == 2 ? 64 : 16;
 }
-
-// Brick of cells served by one workgroup (BX x BY x BZ = cells_per_block).  Cell outputs are
-// summed into an LDS node brick; nodes no other brick touches are stored directly, nodes on a
-// face shared with a neighbouring brick are added with global atomics.
-template <int K> constexpr int brick_x() { return 4; }
-template <int K> constexpr int brick_y() { return K == 2 ? 4 : 2; }
-template <int K> constexpr int brick_z() { return K == 2 ? 4 : 2; }
 
 template <int K>
 __device__ __forceinline__ int elem(int i, int j, int k)
@@ -97,15 +94,6 @@ __device__ __forceinline__ void sweep(const Number *M, const Number *in, Number 
   store_pencil<Number, K, CPB, DIR>(out, p, c, o);
 }
 
-__device__ __forceinline__ void atomic_add(double *a, double v) { unsafeAtomicAdd(a, v); }
-__device__ __forceinline__ void atomic_add(float *a, float v) { unsafeAtomicAdd(a, v); }
-// LDS accumulation (ds_add_f64 / ds_add_f32)
-template <typename Number>
-__device__ __forceinline__ void lds_add(Number *a, Number v)
-{
-  atomicAdd(a, v);
-}
-
 // tanh((T-Tc)/w) and sech^2 from one expm1: t = -em/(2+em), s2 = 4(1+em)/(2+em)^2 with
 // em = expm1(-2|x|).  Beyond |x| = 20 (double) / 10 (float) tanh is +-1 to the last bit and the
 // sech^2 term is below 1e-16 (1e-8) of the dk/dT scale, so it is taken as 0.
@@ -143,39 +131,58 @@ __device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
   return s;
 }
 
-// After all cells of the brick have added into the LDS node brick: write it out.  Constrained
-// rows are skipped (distribute_local_to_global semantics).  A node on a brick face that is not
-// a domain face is shared with a neighbouring brick and is added atomically; every other node
-// belongs to this brick alone and is added with a plain read-modify-write.
-template <typename Number, int K, int BX, int BY, int BZ>
-__device__ __forceinline__ void brick_scatter(const CellArgs<Number> &A, const Number *nodeb,
-                                              int bx, int by, int bz, Number *dst)
+// Strip decomposition of the mesh for colour `colour` (0..7): bit 0 = parity of the strip index
+// in x, bit 1 = parity of the cell row y, bit 2 = parity of the cell layer z.
+struct StripColour {
+  int sb, yb, zb;  // parities
+  int hx, hy, hz;  // number of strips / rows / layers of this colour
+};
+
+template <int CPB>
+__host__ __device__ inline StripColour strip_colour(const int ncell[3], int colour)
 {
-  constexpr int P = K - 1;
-  constexpr int NBX = BX * P + 1, NBY = BY * P + 1, NBZ = BZ * P + 1;
-  const int NX = int(A.NX), NY = int(A.NY), NZ = A.ncell[2] * P + 1;
-  const int x0 = bx * BX * P, y0 = by * BY * P, z0 = bz * BZ * P;
-  __syncthreads();
-  for (int e = threadIdx.x; e < NBX * NBY * NBZ; e += blockDim.x) {
-    const int ix = e % NBX, iy = (e / NBX) % NBY, iz = e / (NBX * NBY);
-    const int gx = x0 + ix, gy = y0 + iy, gz = z0 + iz;
-    if (gx >= NX || gy >= NY || gz >= NZ) continue;
-    const uint32_t gi = uint32_t(gx) + A.NX * (uint32_t(gy) + A.NY * uint32_t(gz));
+  StripColour s;
+  s.sb = colour & 1;
+  s.yb = (colour >> 1) & 1;
+  s.zb = (colour >> 2) & 1;
+  const int nsx = (ncell[0] + CPB - 1) / CPB;
+  s.hx = (nsx - s.sb + 1) / 2;
+  s.hy = (ncell[1] - s.yb + 1) / 2;
+  s.hz = (ncell[2] - s.zb + 1) / 2;
+  return s;
+}
+
+// Add the K outputs of a cell's x pencil into dst.  Thread (c, p) holds o[i] for the nodes
+// (cx*P + i, row, layer); cell c-1 of the strip sits in the previous lane, so its last node
+// (shared with this cell's first) is fetched with a shuffle and added here.  A cell writes
+// nodes 0..K-2 and, when it is the last cell of the strip or of the mesh row, node K-1.
+// Constrained rows are skipped (distribute_local_to_global semantics).
+template <typename Number, int K, int CPB>
+__device__ __forceinline__ void strip_scatter(const CellArgs<Number> &A, Number (&o)[K],
+                                              uint32_t gbase, int c, int cx, bool active,
+                                              Number *dst)
+{
+  if (!active) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) o[i] = Number(0);
+  }
+  const Number prev = __shfl_up(o[K - 1], 1, CPB);
+  if (c > 0) o[0] += prev;
+  if (!active) return;
+  const bool last = (c == CPB - 1) || (cx == A.ncell[0] - 1);
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i == K - 1 && !last) break;
+    const uint32_t gi = gbase + i;
     if (A.use_mask && A.mask && A.mask[gi]) continue;
-    const bool shared = (ix == 0 && gx > 0) || (ix == NBX - 1 && gx < NX - 1) ||
-                        (iy == 0 && gy > 0) || (iy == NBY - 1 && gy < NY - 1) ||
-                        (iz == 0 && gz > 0) || (iz == NBZ - 1 && gz < NZ - 1);
-    const Number v = nodeb[e];
-    if (shared)
-      atomic_add(dst + gi, v);
-    else
-      dst[gi] += v;
+    dst[gi] += o[i];
   }
 }
 
 template <typename Number, int K, int OP, bool NEEDT, bool VEL>
 __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
-    cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst)
+    cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst,
+                int colour, int part_off)
 {
   constexpr int CPB = cells_per_block<K>();
   constexpr int NL = K * K * K;
@@ -202,24 +209,17 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 
   if (A.stop && *A.stop) return;  // uniform over the grid
 
-  constexpr int BX = brick_x<K>(), BY = brick_y<K>(), BZ = brick_z<K>();
-  constexpr int NBX = BX * P + 1, NBY = BY * P + 1, NBZ = BZ * P + 1;
-  constexpr int NB = NBX * NBY * NBZ;
-  __shared__ Number nodeb[(PGRAD) ? 1 : NB];
-
   const int c = threadIdx.x % CPB;
   const int p = threadIdx.x / CPB;
-  const int nbx = (A.ncell[0] + BX - 1) / BX, nby = (A.ncell[1] + BY - 1) / BY;
-  const int bx = blockIdx.x % nbx, by = (blockIdx.x / nbx) % nby, bz = blockIdx.x / (nbx * nby);
-  const int lx = c % BX, ly = (c / BX) % BY, lz = c / (BX * BY);
-  const int cx = bx * BX + lx, cy = by * BY + ly, cz = bz * BZ + lz;
-  const bool active = cx < A.ncell[0] && cy < A.ncell[1] && cz < A.ncell[2];
+  const StripColour sc = strip_colour<CPB>(A.ncell, colour);
+  const int sx = 2 * (blockIdx.x % sc.hx) + sc.sb;
+  const int cy = 2 * ((blockIdx.x / sc.hx) % sc.hy) + sc.yb;
+  const int cz = 2 * (blockIdx.x / (sc.hx * sc.hy)) + sc.zb;
+  const int cx = sx * CPB + c;
+  const bool active = cx < A.ncell[0];
   const int cell = cx + A.ncell[0] * (cy + A.ncell[1] * cz);
   const Number *S = A.S;
   const Number *Dq = A.Dq;
-  if constexpr (!PGRAD) {
-    for (int e = threadIdx.x; e < NB; e += blockDim.x) nodeb[e] = Number(0);
-  }
 
   // ---------------- E1: x pencils, gather from HBM ----------------
   // pencil p -> (j, k) = (p % K, p / K); nodes (cx*P + i, cy*P + j, cz*P + k)
@@ -493,12 +493,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
           }
         }
       }
-    if (active) {
-#pragma unroll
-      for (int ia = 0; ia < K; ++ia)
-        lds_add(&nodeb[(lx * P + ia) + NBX * ((ly * P + jb) + NBY * (lz * P + kb))], diag[ia]);
-    }
-    brick_scatter<Number, K, BX, BY, BZ>(A, nodeb, bx, by, bz, dst);
+    strip_scatter<Number, K, CPB>(A, diag, gbase, c, cx, active, dst);
     return;
   } else {
     // r = val + Dq_z^T fz (z direction in registers)
@@ -538,37 +533,39 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       load_pencil<Number, K, CPB, 0>(bU, p, c, v);
       matvec<Number, K, true>(S, v, o);
     }
+    // src . (A src) of this cell (u_loc is zero on constrained entries), before the x-shared
+    // node is combined across cells
     Number local_dot = 0;
-    if (active) {
-      const int jb = p % K, kb = p / K;
+    if constexpr (TANGENT) {
+      if (active) {
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        lds_add(&nodeb[(lx * P + i) + NBX * ((ly * P + jb) + NBY * (lz * P + kb))], o[i]);
-        // u_loc is zero on constrained entries, so they do not contribute to the dot
-        if constexpr (TANGENT) local_dot += u_loc[i] * o[i];
+        for (int i = 0; i < K; ++i) local_dot += u_loc[i] * o[i];
       }
     }
-    brick_scatter<Number, K, BX, BY, BZ>(A, nodeb, bx, by, bz, dst);
+    strip_scatter<Number, K, CPB>(A, o, gbase, c, cx, active, dst);
     if constexpr (TANGENT) {
       if (A.dot_partial) {
         const double s = block_reduce_sum<double>(double(local_dot), reinterpret_cast<double *>(red));
-        if (threadIdx.x == 0) A.dot_partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) A.dot_partial[part_off + blockIdx.x] = s;
       }
     }
   }
 }
 
 template <int K>
-static int brick_blocks(const MeshInfo &m)
+static int strip_blocks(const MeshInfo &m, int colour)
 {
-  const int bx = brick_x<K>(), by = brick_y<K>(), bz = brick_z<K>();
-  return ((m.ncell[0] + bx - 1) / bx) * ((m.ncell[1] + by - 1) / by) *
-         ((m.ncell[2] + bz - 1) / bz);
+  const StripColour s = strip_colour<cells_per_block<K>()>(m.ncell, colour);
+  return s.hx * s.hy * s.hz;
 }
 
 int cell_blocks(const MeshInfo &m)
 {
-  return m.K == 2 ? brick_blocks<2>(m) : (m.K == 3 ? brick_blocks<3>(m) : brick_blocks<4>(m));
+  int total = 0;
+  for (int colour = 0; colour < 8; ++colour)
+    total += m.K == 2 ? strip_blocks<2>(m, colour)
+                      : (m.K == 3 ? strip_blocks<3>(m, colour) : strip_blocks<4>(m, colour));
+  return total;
 }
 
 template <typename Number, int K, int OP, bool NEEDT, bool VEL>
@@ -576,10 +573,15 @@ static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *
                     hipStream_t st)
 {
   constexpr int CPB = cells_per_block<K>();
-  const int blocks = brick_blocks<K>(m);
-  hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL>), dim3(blocks), dim3(CPB * K * K), 0,
-                     st, a, src, dst);
-  MF_HIP_TRY(hipGetLastError());
+  int part_off = 0;
+  for (int colour = 0; colour < 8; ++colour) {
+    const int blocks = strip_blocks<K>(m, colour);
+    if (blocks == 0) continue;
+    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL>), dim3(blocks), dim3(CPB * K * K),
+                       0, st, a, src, dst, colour, part_off);
+    MF_HIP_TRY(hipGetLastError());
+    part_off += blocks;
+  }
   return MF_OK;
 }
 
