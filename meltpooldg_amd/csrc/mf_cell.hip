@@ -152,138 +152,177 @@ __host__ __device__ inline StripColour strip_colour(const int ncell[3], int colo
   return s;
 }
 
-// Add the K outputs of a cell's x pencil into dst.  Thread (c, p) holds o[i] for the nodes
-// (cx*P + i, row, layer); cell c-1 of the strip sits in the previous lane, so its last node
-// (shared with this cell's first) is fetched with a shuffle and added here.  A cell writes
-// nodes 0..K-2 and, when it is the last cell of the strip or of the mesh row, node K-1.
-// Constrained rows are skipped (distribute_local_to_global semantics).
-template <typename Number, int K, int CPB>
-__device__ __forceinline__ void strip_scatter(const CellArgs<Number> &A, Number (&o)[K],
-                                              uint32_t gbase, int c, int cx, bool active,
-                                              Number *dst)
-{
-  if (!active) {
-#pragma unroll
-    for (int i = 0; i < K; ++i) o[i] = Number(0);
-  }
-  const Number prev = __shfl_up(o[K - 1], 1, CPB);
-  if (c > 0) o[0] += prev;
-  if (!active) return;
-  const bool last = (c == CPB - 1) || (cx == A.ncell[0] - 1);
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    if (i == K - 1 && !last) break;
-    const uint32_t gi = gbase + i;
-    if (A.use_mask && A.mask && A.mask[gi]) continue;
-    dst[gi] += o[i];
-  }
-}
+// One colour launch of the cell kernel.
+struct LaunchPart {
+  int colour;    // structured meshes: strip colour 0..7
+  int part_off;  // offset of this launch's blocks in dot_partial
+};
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+// Node access of the structured mesh.  A workgroup is a strip of CPB cells consecutive in x;
+// thread (c, p) serves the x pencil p = j + K*k of cell c, i.e. the nodes
+// (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
+template <typename Number, int K, int CPB>
+struct GridAccess {
+  static constexpr int P = K - 1;
+  uint32_t gbase;
+  int cx, cell;
+  bool active;
+  Number h[3], ih[3];
+
+  __device__ GridAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c, int p)
+  {
+    const StripColour sc = strip_colour<CPB>(A.ncell, lp.colour);
+    const int sx = 2 * (blockIdx.x % sc.hx) + sc.sb;
+    const int cy = 2 * ((blockIdx.x / sc.hx) % sc.hy) + sc.yb;
+    const int cz = 2 * (blockIdx.x / (sc.hx * sc.hy)) + sc.zb;
+    cx = sx * CPB + c;
+    active = cx < A.ncell[0];
+    cell = cx + A.ncell[0] * (cy + A.ncell[1] * cz);
+    gbase = uint32_t(cx * P) + A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      h[d] = A.h[d];
+      ih[d] = A.inv_h[d];
+    }
+  }
+  __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
+  {
+    return A.use_mask && A.mask && A.mask[n];
+  }
+  // entry i of field f as stored (read_dof_values_plain)
+  __device__ Number plain(const Number *f, int i) const { return active ? f[gbase + i] : Number(0); }
+  // entry i with the constraints applied (read_dof_values: Dirichlet entries read as zero)
+  __device__ Number value(const CellArgs<Number> &A, const Number *f, int i) const
+  {
+    return (active && !masked(A, gbase + i)) ? f[gbase + i] : Number(0);
+  }
+  // current dst entries of the pencil; strips of one colour own their nodes during the launch,
+  // so these can be read before the cell is computed
+  __device__ void prefetch(const Number *dst, Number (&d)[K]) const
+  {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = active ? dst[gbase + i] : Number(0);
+  }
+  // Add the K outputs of the x pencil into dst.  Cell c-1 of the strip sits in the previous
+  // lane, so its last node (shared with this cell's first) is fetched with a shuffle and added
+  // here.  A cell writes nodes 0..K-2 and, when it is the last cell of the strip or of the mesh
+  // row, node K-1.  Constrained rows are skipped (distribute_local_to_global semantics).
+  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int c,
+                          Number *dst) const
+  {
+    if (!active) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) o[i] = Number(0);
+    }
+    const Number prev = __shfl_up(o[K - 1], 1, CPB);
+    if (c > 0) o[0] += prev;
+    if (!active) return;
+    const bool last = (c == CPB - 1) || (cx == A.ncell[0] - 1);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if (i == K - 1 && !last) break;
+      if (masked(A, gbase + i)) continue;
+      dst[gbase + i] = d[i] + o[i];
+    }
+  }
+};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL, class Access>
 __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst,
-                int colour, int part_off)
+                const LaunchPart lp)
 {
   constexpr int CPB = cells_per_block<K>();
   constexpr int NL = K * K * K;
-  constexpr int P = K - 1;
+  constexpr int BUF = NL * CPB;
   constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG);
   constexpr bool HAS_U = !IS_DIAG;
   constexpr bool PGRAD = (OP == HEAT_PGRAD);
   constexpr bool USE_T = (NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG)) || PGRAD;
   constexpr bool RHS_HEAT = (OP == HEAT_RHS);
   constexpr bool TANGENT = (OP == HEAT_TANGENT || OP == ADVEC_TANGENT);
-  constexpr bool NEED_A = RHS_HEAT || IS_DIAG;
-  constexpr bool NEED_GY = HAS_U || IS_DIAG;
 
-  __shared__ Number bU[NL * CPB];
-  __shared__ Number bGX[NL * CPB];
-  __shared__ Number bGY[NEED_GY ? NL * CPB : 1];
-  __shared__ Number bT[USE_T ? NL * CPB : 1];
-  __shared__ Number bTX[USE_T ? NL * CPB : 1];
-  __shared__ Number bTY[USE_T ? NL * CPB : 1];
-  __shared__ Number bW[VEL ? 3 * NL * CPB : 1];
-  __shared__ Number bA1[NEED_A ? NL * CPB : 1];
-  __shared__ Number bA2[NEED_A ? NL * CPB : 1];
-  __shared__ Number red[16];
+  // LDS cell tensors [K^3][CPB].  While values are interpolated: u in L0, T in L1, the velocity
+  // components and the rhs fields after it.  Then the T gradients use L2, L3 and are reduced to
+  // per-quadrature-point coefficients in registers, after which the u gradients take L1, L2.
+  // The diagonal needs five coefficient tensors.
+  constexpr int I_T = 1;
+  constexpr int I_W = 1 + (USE_T ? 1 : 0);
+  constexpr int I_A = I_W + (VEL ? 3 : 0);
+  constexpr int N_EVAL = I_A + (RHS_HEAT ? 2 : 0);
+  constexpr int N_GRAD = USE_T ? 4 : 3;
+  constexpr int NBUF = cmax(cmax(N_EVAL, N_GRAD), IS_DIAG ? 5 : 0);
+  __shared__ Number lds[NBUF * BUF];
+  __shared__ double red[16];
+  Number *const L0 = lds;
+  Number *const L1 = lds + BUF;
+  Number *const L2 = lds + 2 * BUF;
 
   if (A.stop && *A.stop) return;  // uniform over the grid
 
   const int c = threadIdx.x % CPB;
   const int p = threadIdx.x / CPB;
-  const StripColour sc = strip_colour<CPB>(A.ncell, colour);
-  const int sx = 2 * (blockIdx.x % sc.hx) + sc.sb;
-  const int cy = 2 * ((blockIdx.x / sc.hx) % sc.hy) + sc.yb;
-  const int cz = 2 * (blockIdx.x / (sc.hx * sc.hy)) + sc.zb;
-  const int cx = sx * CPB + c;
-  const bool active = cx < A.ncell[0];
-  const int cell = cx + A.ncell[0] * (cy + A.ncell[1] * cz);
+  const Access ac(A, lp, c, p);
   const Number *S = A.S;
   const Number *Dq = A.Dq;
 
   // ---------------- E1: x pencils, gather from HBM ----------------
-  // pencil p -> (j, k) = (p % K, p / K); nodes (cx*P + i, cy*P + j, cz*P + k)
-  const uint32_t gbase = uint32_t(cx * P) +
-                         A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
   Number u_loc[K];  // gathered values of the x pencil (kept for the fused dot)
+  Number dpre[K];   // dst entries of the pencil, read early
   {
     Number o[K];
     if constexpr (HAS_U) {
-      const Number *in = (OP == HEAT_RHS) ? A.T : src;
+      const Number *in = RHS_HEAT ? A.T : src;
+      const bool cons = TANGENT && !A.gather_plain;
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        Number v = 0;
-        if (active) {
-          v = in[gbase + i];
-          if (TANGENT && A.use_mask && !A.gather_plain && A.mask && A.mask[gbase + i]) v = 0;
-        }
-        u_loc[i] = v;
-      }
+      for (int i = 0; i < K; ++i) u_loc[i] = cons ? ac.value(A, in, i) : ac.plain(in, i);
       matvec<Number, K, false>(S, u_loc, o);
-      store_pencil<Number, K, CPB, 0>(bU, p, c, o);
+      store_pencil<Number, K, CPB, 0>(L0, p, c, o);
     }
     if constexpr (USE_T) {
       Number v[K];
 #pragma unroll
-      for (int i = 0; i < K; ++i) v[i] = active ? A.T[gbase + i] : Number(0);
+      for (int i = 0; i < K; ++i) v[i] = ac.plain(A.T, i);
       matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(bT, p, c, o);
+      store_pencil<Number, K, CPB, 0>(lds + I_T * BUF, p, c, o);
     }
     if constexpr (VEL) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         Number v[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i)
-          v[i] = active ? A.vel[size_t(d) * A.n_dofs + gbase + i] : Number(0);
+        for (int i = 0; i < K; ++i) v[i] = ac.plain(A.vel + size_t(d) * A.n_dofs, i);
         matvec<Number, K, false>(S, v, o);
-        store_pencil<Number, K, CPB, 0>(bW + d * NL * CPB, p, c, o);
+        store_pencil<Number, K, CPB, 0>(lds + (I_W + d) * BUF, p, c, o);
       }
     }
     if constexpr (RHS_HEAT) {
       Number v[K];
 #pragma unroll
-      for (int i = 0; i < K; ++i) v[i] = active ? src[gbase + i] : Number(0);  // T_old
+      for (int i = 0; i < K; ++i) v[i] = ac.plain(src, i);  // T_old
       matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(bA1, p, c, o);
+      store_pencil<Number, K, CPB, 0>(lds + I_A * BUF, p, c, o);
 #pragma unroll
-      for (int i = 0; i < K; ++i) v[i] = (active && A.aux1) ? A.aux1[gbase + i] : Number(0);
+      for (int i = 0; i < K; ++i) v[i] = A.aux1 ? ac.plain(A.aux1, i) : Number(0);
       matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(bA2, p, c, o);
+      store_pencil<Number, K, CPB, 0>(lds + (I_A + 1) * BUF, p, c, o);
     }
+    if constexpr (!PGRAD) ac.prefetch(dst, dpre);
   }
   __syncthreads();
 
   // ---------------- E2: y pencils ----------------
-  if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 1, false>(S, bU, p, c);
-  if constexpr (USE_T) sweep_inplace<Number, K, CPB, 1, false>(S, bT, p, c);
+  if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 1, false>(S, L0, p, c);
+  if constexpr (USE_T) sweep_inplace<Number, K, CPB, 1, false>(S, lds + I_T * BUF, p, c);
   if constexpr (VEL) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) sweep_inplace<Number, K, CPB, 1, false>(S, bW + d * NL * CPB, p, c);
+    for (int d = 0; d < 3; ++d) sweep_inplace<Number, K, CPB, 1, false>(S, lds + (I_W + d) * BUF, p, c);
   }
   if constexpr (RHS_HEAT) {
-    sweep_inplace<Number, K, CPB, 1, false>(S, bA1, p, c);
-    sweep_inplace<Number, K, CPB, 1, false>(S, bA2, p, c);
+    sweep_inplace<Number, K, CPB, 1, false>(S, lds + I_A * BUF, p, c);
+    sweep_inplace<Number, K, CPB, 1, false>(S, lds + (I_A + 1) * BUF, p, c);
   }
   __syncthreads();
 
@@ -291,50 +330,75 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   Number uq[K], guz[K], Tq[K], gTz[K], wq[3][K], a1q[K], a2q[K];
   if constexpr (HAS_U) {
     Number v[K];
-    load_pencil<Number, K, CPB, 2>(bU, p, c, v);
+    load_pencil<Number, K, CPB, 2>(L0, p, c, v);
     matvec<Number, K, false>(S, v, uq);
     matvec<Number, K, false>(Dq, uq, guz);
-    store_pencil<Number, K, CPB, 2>(bU, p, c, uq);
+    store_pencil<Number, K, CPB, 2>(L0, p, c, uq);
   }
   if constexpr (USE_T) {
     Number v[K];
-    load_pencil<Number, K, CPB, 2>(bT, p, c, v);
+    load_pencil<Number, K, CPB, 2>(lds + I_T * BUF, p, c, v);
     matvec<Number, K, false>(S, v, Tq);
     matvec<Number, K, false>(Dq, Tq, gTz);
-    store_pencil<Number, K, CPB, 2>(bT, p, c, Tq);
+    store_pencil<Number, K, CPB, 2>(lds + I_T * BUF, p, c, Tq);
   }
   if constexpr (VEL) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       Number v[K];
-      load_pencil<Number, K, CPB, 2>(bW + d * NL * CPB, p, c, v);
+      load_pencil<Number, K, CPB, 2>(lds + (I_W + d) * BUF, p, c, v);
       matvec<Number, K, false>(S, v, wq[d]);
     }
   }
   if constexpr (RHS_HEAT) {
     Number v[K];
-    load_pencil<Number, K, CPB, 2>(bA1, p, c, v);
+    load_pencil<Number, K, CPB, 2>(lds + I_A * BUF, p, c, v);
     matvec<Number, K, false>(S, v, a1q);
-    load_pencil<Number, K, CPB, 2>(bA2, p, c, v);
+    load_pencil<Number, K, CPB, 2>(lds + (I_A + 1) * BUF, p, c, v);
     matvec<Number, K, false>(S, v, a2q);
   }
   __syncthreads();
 
-  // ---------------- E4: d/dx and d/dy at the quadrature points ----------------
-  if constexpr (HAS_U) {
-    sweep<Number, K, CPB, 0, false>(Dq, bU, bGX, p, c);
-    sweep<Number, K, CPB, 1, false>(Dq, bU, bGY, p, c);
-  }
+  const int qi = p % K, qj = p / K;
+  const Number kc = (A.k_cell && ac.active) ? A.k_cell[ac.cell] : Number(1);
+
+  // ---------------- T gradients -> per-quadrature-point coefficients ----------------
+  // heat tangent / diagonal: kq = k(T) kc and dkg = dk/dT kc grad T;  PGRAD: dkg = grad T
+  Number kq[K], dkg[3][K];
   if constexpr (USE_T) {
-    sweep<Number, K, CPB, 0, false>(Dq, bT, bTX, p, c);
-    sweep<Number, K, CPB, 1, false>(Dq, bT, bTY, p, c);
+    sweep<Number, K, CPB, 0, false>(Dq, lds + I_T * BUF, L2, p, c);
+    sweep<Number, K, CPB, 1, false>(Dq, lds + I_T * BUF, lds + 3 * BUF, p, c);
+    __syncthreads();
+#pragma unroll
+    for (int z = 0; z < K; ++z) {
+      const int e = elem<K>(qi, qj, z) * CPB + c;
+      const Number g0 = L2[e] * ac.ih[0], g1 = lds[3 * BUF + e] * ac.ih[1], g2 = gTz[z] * ac.ih[2];
+      if constexpr (PGRAD) {
+        dkg[0][z] = g0;
+        dkg[1][z] = g1;
+        dkg[2][z] = g2;
+      } else {
+        Number t, s2;
+        tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
+        kq[z] = (A.k0 + A.tanh_amp * t) * kc;
+        const Number dk = A.tanh_amp * A.inv_tanh_w * s2 * kc;
+        dkg[0][z] = dk * g0;
+        dkg[1][z] = dk * g1;
+        dkg[2][z] = dk * g2;
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
+
+  // ---------------- E4: d/dx and d/dy of u at the quadrature points ----------------
+  if constexpr (HAS_U) {
+    sweep<Number, K, CPB, 0, false>(Dq, L0, L1, p, c);
+    sweep<Number, K, CPB, 1, false>(Dq, L0, L2, p, c);
+    __syncthreads();
+  }
 
   // ---------------- Q: quadrature-point operation (z pencil (i,j), q = (i,j,z)) ----------------
-  const int qi = p % K, qj = p / K;
-  const Number kc = (A.k_cell && active) ? A.k_cell[cell] : Number(1);
-  const Number wij = A.w[qi] * A.w[qj] * A.h[0] * A.h[1] * A.h[2];
+  const Number wij = A.w[qi] * A.w[qj] * ac.h[0] * ac.h[1] * ac.h[2];
   Number rq[K], fz[K];
 #pragma unroll
   for (int z = 0; z < K; ++z) {
@@ -342,9 +406,9 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     const Number jxw = wij * A.w[z];
     Number g[3] = {0, 0, 0};
     if constexpr (HAS_U) {
-      g[0] = bGX[e] * A.inv_h[0];
-      g[1] = bGY[e] * A.inv_h[1];
-      g[2] = guz[z] * A.inv_h[2];
+      g[0] = L1[e] * ac.ih[0];
+      g[1] = L2[e] * ac.ih[1];
+      g[2] = guz[z] * ac.ih[2];
     }
     Number w3[3] = {0, 0, 0};
     if constexpr (VEL) {
@@ -355,21 +419,10 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     Number val = 0, f[3] = {0, 0, 0};
     Number cc = 0, kk = 0, bb[3] = {0, 0, 0};
     if constexpr (PGRAD) {
-      const Number gb0 = bTX[e] * A.inv_h[0], gb1 = bTY[e] * A.inv_h[1], gb2 = gTz[z] * A.inv_h[2];
-      val = g[0] * gb0 + g[1] * gb1 + g[2] * gb2;  // JxW applied below
+      val = g[0] * dkg[0][z] + g[1] * dkg[1][z] + g[2] * dkg[2][z];  // JxW applied below
     } else if constexpr (OP == HEAT_TANGENT || OP == HEAT_DIAG) {
-      Number k = A.k0, dk = 0, gT[3] = {0, 0, 0};
-      if constexpr (USE_T) {
-        Number t, s2;
-        tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
-        k = A.k0 + A.tanh_amp * t;
-        dk = A.tanh_amp * A.inv_tanh_w * s2;
-        gT[0] = bTX[e] * A.inv_h[0];
-        gT[1] = bTY[e] * A.inv_h[1];
-        gT[2] = gTz[z] * A.inv_h[2];
-      }
-      k *= kc;
-      dk *= kc;
+      Number k = A.k0 * kc;
+      if constexpr (USE_T) k = kq[z];
       const bool td = USE_T && A.t_dependent;
       if constexpr (OP == HEAT_TANGENT) {
         const Number u = uq[z];
@@ -379,9 +432,11 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
         if (!A.transpose) {
           // (rho cp w . grad u) v + dk/dT u (grad T . grad v)
           if constexpr (VEL) val += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
-          if (td) {
+          if constexpr (USE_T) {
+            if (td) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) f[d] += dk * gT[d] * u;
+              for (int d = 0; d < 3; ++d) f[d] += dkg[d][z] * u;
+            }
           }
         } else {
           // transposed: (rho cp w u) . grad v + dk/dT (grad T . grad u) v
@@ -389,7 +444,9 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 #pragma unroll
             for (int d = 0; d < 3; ++d) f[d] += A.rho_cp * w3[d] * u;
           }
-          if (td) val += dk * (gT[0] * g[0] + gT[1] * g[1] + gT[2] * g[2]);
+          if constexpr (USE_T) {
+            if (td) val += dkg[0][z] * g[0] + dkg[1][z] * g[1] + dkg[2][z] * g[2];
+          }
         }
       } else {
         cc = A.dt_inv * A.rho_cp;
@@ -398,9 +455,11 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 #pragma unroll
           for (int d = 0; d < 3; ++d) bb[d] = A.rho_cp * w3[d];
         }
-        if (td) {
+        if constexpr (USE_T) {
+          if (td) {
 #pragma unroll
-          for (int d = 0; d < 3; ++d) bb[d] += dk * gT[d];
+            for (int d = 0; d < 3; ++d) bb[d] += dkg[d][z];
+          }
         }
       }
     } else if constexpr (OP == HEAT_RHS) {
@@ -440,30 +499,30 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       for (int d = 0; d < 3; ++d) bb[d] = A.theta * w3[d];
     }
     if constexpr (IS_DIAG) {
-      bU[e] = jxw * cc;
-      bGX[e] = jxw * kk;
-      bGY[e] = jxw * bb[0];
-      bA1[e] = jxw * bb[1];
-      bA2[e] = jxw * bb[2];
+      L0[e] = jxw * cc;
+      L1[e] = jxw * kk;
+      L2[e] = jxw * bb[0];
+      lds[3 * BUF + e] = jxw * bb[1];
+      lds[4 * BUF + e] = jxw * bb[2];
     } else {
       rq[z] = jxw * val;
-      bGX[e] = jxw * f[0] * A.inv_h[0];
-      bGY[e] = jxw * f[1] * A.inv_h[1];
-      fz[z] = jxw * f[2] * A.inv_h[2];
+      L1[e] = jxw * f[0] * ac.ih[0];
+      L2[e] = jxw * f[1] * ac.ih[1];
+      fz[z] = jxw * f[2] * ac.ih[2];
     }
   }
 
   if constexpr (PGRAD) {
-    // per-cell sum of the K^2 z-pencil partials (bU is free after E4)
+    // per-cell sum of the K^2 z-pencil partials (L0 is free after E4)
     Number part = 0;
 #pragma unroll
     for (int z = 0; z < K; ++z) part += rq[z];
-    bU[p * CPB + c] = part;
+    L0[p * CPB + c] = part;
     __syncthreads();
-    if (p == 0 && active) {
+    if (p == 0 && ac.active) {
       double sum = 0;
-      for (int i = 0; i < K * K; ++i) sum += double(bU[i * CPB + c]);
-      dst[cell] += Number(A.pgrad_scale * sum);
+      for (int i = 0; i < K * K; ++i) sum += double(L0[i * CPB + c]);
+      dst[ac.cell] += Number(A.pgrad_scale * sum);
     }
     return;
   }
@@ -477,23 +536,24 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     for (int ia = 0; ia < K; ++ia) diag[ia] = 0;
     for (int q3 = 0; q3 < K; ++q3)
       for (int q2 = 0; q2 < K; ++q2) {
-        const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * A.inv_h[1];
-        const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * A.inv_h[2];
+        const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * ac.ih[1];
+        const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * ac.ih[2];
 #pragma unroll
         for (int q1 = 0; q1 < K; ++q1) {
           const int e = elem<K>(q1, q2, q3) * CPB + c;
-          const Number cq = bU[e], kq = bGX[e], b0 = bGY[e], b1 = bA1[e], b2 = bA2[e];
+          const Number cq = L0[e], kq2 = L1[e], b0 = L2[e], b1 = lds[3 * BUF + e],
+                       b2 = lds[4 * BUF + e];
 #pragma unroll
           for (int ia = 0; ia < K; ++ia) {
-            const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * A.inv_h[0];
+            const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * ac.ih[0];
             const Number phi = sx * sy * sz;
             const Number gx = dx * sy * sz, gy = sx * dy * sz, gz = sx * sy * dz;
-            diag[ia] += cq * phi * phi + kq * (gx * gx + gy * gy + gz * gz) +
+            diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) +
                         (b0 * gx + b1 * gy + b2 * gz) * phi;
           }
         }
       }
-    strip_scatter<Number, K, CPB>(A, diag, gbase, c, cx, active, dst);
+    ac.scatter(A, diag, dpre, c, dst);
     return;
   } else {
     // r = val + Dq_z^T fz (z direction in registers)
@@ -501,52 +561,52 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     matvec<Number, K, true>(Dq, fz, tz);
 #pragma unroll
     for (int z = 0; z < K; ++z) rq[z] += tz[z];
-    store_pencil<Number, K, CPB, 2>(bU, p, c, rq);
+    store_pencil<Number, K, CPB, 2>(L0, p, c, rq);
     __syncthreads();
 
     // ---------------- I1: Dq^T along x (on fx) and y (on fy) ----------------
-    sweep_inplace<Number, K, CPB, 0, true>(Dq, bGX, p, c);
-    sweep_inplace<Number, K, CPB, 1, true>(Dq, bGY, p, c);
+    sweep_inplace<Number, K, CPB, 0, true>(Dq, L1, p, c);
+    sweep_inplace<Number, K, CPB, 1, true>(Dq, L2, p, c);
     __syncthreads();
 
     // ---------------- I2: sum and S^T along z ----------------
     {
       Number a[K], b[K], d[K], o[K];
-      load_pencil<Number, K, CPB, 2>(bU, p, c, a);
-      load_pencil<Number, K, CPB, 2>(bGX, p, c, b);
-      load_pencil<Number, K, CPB, 2>(bGY, p, c, d);
+      load_pencil<Number, K, CPB, 2>(L0, p, c, a);
+      load_pencil<Number, K, CPB, 2>(L1, p, c, b);
+      load_pencil<Number, K, CPB, 2>(L2, p, c, d);
 #pragma unroll
       for (int e = 0; e < K; ++e) a[e] += b[e] + d[e];
       matvec<Number, K, true>(S, a, o);
-      store_pencil<Number, K, CPB, 2>(bU, p, c, o);
+      store_pencil<Number, K, CPB, 2>(L0, p, c, o);
     }
     __syncthreads();
 
     // ---------------- I3: S^T along y ----------------
-    sweep_inplace<Number, K, CPB, 1, true>(S, bU, p, c);
+    sweep_inplace<Number, K, CPB, 1, true>(S, L0, p, c);
     __syncthreads();
 
     // ---------------- I4: S^T along x and scatter-add ----------------
     Number o[K];
     {
       Number v[K];
-      load_pencil<Number, K, CPB, 0>(bU, p, c, v);
+      load_pencil<Number, K, CPB, 0>(L0, p, c, v);
       matvec<Number, K, true>(S, v, o);
     }
     // src . (A src) of this cell (u_loc is zero on constrained entries), before the x-shared
     // node is combined across cells
     Number local_dot = 0;
     if constexpr (TANGENT) {
-      if (active) {
+      if (ac.active) {
 #pragma unroll
         for (int i = 0; i < K; ++i) local_dot += u_loc[i] * o[i];
       }
     }
-    strip_scatter<Number, K, CPB>(A, o, gbase, c, cx, active, dst);
+    ac.scatter(A, o, dpre, c, dst);
     if constexpr (TANGENT) {
       if (A.dot_partial) {
-        const double s = block_reduce_sum<double>(double(local_dot), reinterpret_cast<double *>(red));
-        if (threadIdx.x == 0) A.dot_partial[part_off + blockIdx.x] = s;
+        const double s = block_reduce_sum<double>(double(local_dot), red);
+        if (threadIdx.x == 0) A.dot_partial[lp.part_off + blockIdx.x] = s;
       }
     }
   }
@@ -577,8 +637,9 @@ static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *
   for (int colour = 0; colour < 8; ++colour) {
     const int blocks = strip_blocks<K>(m, colour);
     if (blocks == 0) continue;
-    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL>), dim3(blocks), dim3(CPB * K * K),
-                       0, st, a, src, dst, colour, part_off);
+    const LaunchPart lp{colour, part_off};
+    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL, GridAccess<Number, K, CPB>>),
+                       dim3(blocks), dim3(CPB * K * K), 0, st, a, src, dst, lp);
     MF_HIP_TRY(hipGetLastError());
     part_off += blocks;
   }
