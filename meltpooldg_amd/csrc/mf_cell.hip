@@ -18,11 +18,11 @@
 // barriers.  Cell tensors live in LDS as [K^3][CPB] arrays.  Gradients at the quadrature
 // points use the collocation derivative Dq on the Gauss points (grad = Dq (S u)), which equals
 // (DS u) exactly in exact arithmetic because K points determine the degree-p polynomial.
-// Scatter without atomics: a workgroup is a strip of CPB cells consecutive in x, and the strips
-// are launched in 8 colours by the parity of (strip x index, cell y, cell z).  Strips of one
-// colour share no node, so each adds its nodes with a plain read-modify-write; the node shared
-// by x-neighbours inside a strip is combined with a lane shuffle and written once.  The 8
-// launches run in stream order.
+// Scatter without atomics: a workgroup walks one row of cells along x, CPB cells per step, and
+// the rows are launched in 4 colours by the parity of (cell y, cell z).  Rows of one colour share
+// no node, so each adds its nodes with a plain read-modify-write; the node shared by
+// x-neighbours is combined with a lane shuffle (or carried to the next step) and written once.
+// The 4 launches run in stream order.
 #include "mf_internal.hpp"
 
 namespace mf {
@@ -131,22 +131,18 @@ __device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
   return s;
 }
 
-// Strip decomposition of the mesh for colour `colour` (0..7): bit 0 = parity of the strip index
-// in x, bit 1 = parity of the cell row y, bit 2 = parity of the cell layer z.
-struct StripColour {
-  int sb, yb, zb;  // parities
-  int hx, hy, hz;  // number of strips / rows / layers of this colour
+// Row decomposition of the mesh for colour `colour` (0..3): bit 0 = parity of the cell row y,
+// bit 1 = parity of the cell layer z.  Rows of one colour share no node.
+struct RowColour {
+  int yb, zb;  // parities
+  int hy, hz;  // number of rows / layers of this colour
 };
 
-template <int CPB>
-__host__ __device__ inline StripColour strip_colour(const int ncell[3], int colour)
+__host__ __device__ inline RowColour row_colour(const int ncell[3], int colour)
 {
-  StripColour s;
-  s.sb = colour & 1;
-  s.yb = (colour >> 1) & 1;
-  s.zb = (colour >> 2) & 1;
-  const int nsx = (ncell[0] + CPB - 1) / CPB;
-  s.hx = (nsx - s.sb + 1) / 2;
+  RowColour s;
+  s.yb = colour & 1;
+  s.zb = (colour >> 1) & 1;
   s.hy = (ncell[1] - s.yb + 1) / 2;
   s.hz = (ncell[2] - s.zb + 1) / 2;
   return s;
@@ -154,36 +150,44 @@ __host__ __device__ inline StripColour strip_colour(const int ncell[3], int colo
 
 // One colour launch of the cell kernel.
 struct LaunchPart {
-  int colour;    // structured meshes: strip colour 0..7
+  int colour;    // structured meshes: row colour 0..3
   int part_off;  // offset of this launch's blocks in dot_partial
 };
 
-// Node access of the structured mesh.  A workgroup is a strip of CPB cells consecutive in x;
-// thread (c, p) serves the x pencil p = j + K*k of cell c, i.e. the nodes
-// (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
+// Node access of the structured mesh.  A workgroup walks one row of cells along x in steps of
+// CPB cells; thread (c, p) serves the x pencil p = j + K*k of cell c of the step, i.e. the
+// nodes (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
 template <typename Number, int K, int CPB>
 struct GridAccess {
   static constexpr int P = K - 1;
-  uint32_t gbase;
+  uint32_t rowbase, gbase;
+  int c, ncx, cellrow, nsub;
   int cx, cell;
   bool active;
-  Number h[3], ih[3];
+  Number carry;  // output of the previous step's last cell at the node it shares with cell 0
 
-  __device__ GridAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c, int p)
+  __device__ GridAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p)
   {
-    const StripColour sc = strip_colour<CPB>(A.ncell, lp.colour);
-    const int sx = 2 * (blockIdx.x % sc.hx) + sc.sb;
-    const int cy = 2 * ((blockIdx.x / sc.hx) % sc.hy) + sc.yb;
-    const int cz = 2 * (blockIdx.x / (sc.hx * sc.hy)) + sc.zb;
-    cx = sx * CPB + c;
-    active = cx < A.ncell[0];
-    cell = cx + A.ncell[0] * (cy + A.ncell[1] * cz);
-    gbase = uint32_t(cx * P) + A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      h[d] = A.h[d];
-      ih[d] = A.inv_h[d];
-    }
+    const RowColour rc = row_colour(A.ncell, lp.colour);
+    const int cy = 2 * (blockIdx.x % rc.hy) + rc.yb;
+    const int cz = 2 * (blockIdx.x / rc.hy) + rc.zb;
+    c = c_;
+    ncx = A.ncell[0];
+    nsub = (ncx + CPB - 1) / CPB;
+    rowbase = A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
+    cellrow = ncx * (cy + A.ncell[1] * cz);
+    carry = Number(0);
+    at(0);
+  }
+  __device__ int iterations() const { return nsub; }
+  // cell size relative to the mesh's base cell
+  __device__ Number hscale() const { return Number(1); }
+  __device__ void at(int it)
+  {
+    cx = it * CPB + c;
+    active = cx < ncx;
+    cell = cellrow + cx;
+    gbase = rowbase + uint32_t(cx * P);
   }
   __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
   {
@@ -196,28 +200,31 @@ struct GridAccess {
   {
     return (active && !masked(A, gbase + i)) ? f[gbase + i] : Number(0);
   }
-  // current dst entries of the pencil; strips of one colour own their nodes during the launch,
+  // current dst entries of the pencil; rows of one colour own their nodes during the launch,
   // so these can be read before the cell is computed
   __device__ void prefetch(const Number *dst, Number (&d)[K]) const
   {
 #pragma unroll
     for (int i = 0; i < K; ++i) d[i] = active ? dst[gbase + i] : Number(0);
   }
-  // Add the K outputs of the x pencil into dst.  Cell c-1 of the strip sits in the previous
+  // Add the K outputs of the x pencil into dst.  Cell c-1 of the step sits in the previous
   // lane, so its last node (shared with this cell's first) is fetched with a shuffle and added
-  // here.  A cell writes nodes 0..K-2 and, when it is the last cell of the strip or of the mesh
-  // row, node K-1.  Constrained rows are skipped (distribute_local_to_global semantics).
-  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int c,
-                          Number *dst) const
+  // here; cell 0 takes it from the previous step's last cell.  A cell writes nodes 0..K-2 and,
+  // at the end of the row, node K-1.  Constrained rows are skipped
+  // (distribute_local_to_global semantics).
+  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
+                          Number *dst)
   {
     if (!active) {
 #pragma unroll
       for (int i = 0; i < K; ++i) o[i] = Number(0);
     }
     const Number prev = __shfl_up(o[K - 1], 1, CPB);
-    if (c > 0) o[0] += prev;
+    const Number wrap = __shfl(o[K - 1], CPB - 1, CPB);
+    o[0] += (c > 0) ? prev : carry;
+    carry = wrap;
     if (!active) return;
-    const bool last = (c == CPB - 1) || (cx == A.ncell[0] - 1);
+    const bool last = (cx == ncx - 1);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       if (i == K - 1 && !last) break;
@@ -229,9 +236,9 @@ struct GridAccess {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL, class Access>
+template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT, class Access>
 __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
-    cell_kernel(const CellArgs<Number> A, const Number *__restrict__ src, Number *__restrict__ dst,
+    cell_kernel(const CellArgs<Number> A0, const Number *__restrict__ src, Number *__restrict__ dst,
                 const LaunchPart lp)
 {
   constexpr int CPB = cells_per_block<K>();
@@ -253,33 +260,53 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   constexpr int I_A = I_W + (VEL ? 3 : 0);
   constexpr int N_EVAL = I_A + (RHS_HEAT ? 2 : 0);
   constexpr int N_GRAD = USE_T ? 4 : 3;
-  constexpr int NBUF = cmax(cmax(N_EVAL, N_GRAD), IS_DIAG ? 5 : 0);
+  // with the fused dot, the gathered source is kept in one more tensor
+  constexpr bool FDOT = DOT && TANGENT;
+  constexpr int I_SRC = cmax(cmax(N_EVAL, N_GRAD), IS_DIAG ? 5 : 0);
+  constexpr int NBUF = I_SRC + (FDOT ? 1 : 0);
   __shared__ Number lds[NBUF * BUF];
-  __shared__ double red[16];
   Number *const L0 = lds;
   Number *const L1 = lds + BUF;
   Number *const L2 = lds + 2 * BUF;
 
-  if (A.stop && *A.stop) return;  // uniform over the grid
+  if (A0.stop && *A0.stop) return;  // uniform over the grid
 
   const int c = threadIdx.x % CPB;
   const int p = threadIdx.x / CPB;
-  const Access ac(A, lp, c, p);
+  const int qi = p % K, qj = p / K;
+  Access ac(A0, lp, c, p);
+  double dot_acc = 0;
+  // The arguments are re-read from the kernel-argument segment in every step (the empty asm
+  // hides the pointer from loop-invariant hoisting), so the 1D matrices and coefficients live
+  // in scalar registers only while a step uses them.
+  typedef __attribute__((address_space(4))) const CellArgs<Number> KernArgs;
+  KernArgs *kargs = (KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+
+  for (int it = 0; it < ac.iterations(); ++it) {
+  if (it > 0) {
+    __syncthreads();  // LDS of the previous step fully consumed
+    ac.at(it);
+  }
+  asm volatile("" : "+s"(kargs));
+  const CellArgs<Number> &A = *(const CellArgs<Number> *)kargs;
   const Number *S = A.S;
   const Number *Dq = A.Dq;
+  const Number hs = ac.hscale();
+  const Number ih[3] = {A.inv_h[0] / hs, A.inv_h[1] / hs, A.inv_h[2] / hs};
 
   // ---------------- E1: x pencils, gather from HBM ----------------
-  Number u_loc[K];  // gathered values of the x pencil (kept for the fused dot)
-  Number dpre[K];   // dst entries of the pencil, read early
+  Number dpre[K];  // dst entries of the pencil, read while the integration sweeps run
   {
     Number o[K];
     if constexpr (HAS_U) {
       const Number *in = RHS_HEAT ? A.T : src;
       const bool cons = TANGENT && !A.gather_plain;
+      Number u_loc[K];
 #pragma unroll
       for (int i = 0; i < K; ++i) u_loc[i] = cons ? ac.value(A, in, i) : ac.plain(in, i);
       matvec<Number, K, false>(S, u_loc, o);
       store_pencil<Number, K, CPB, 0>(L0, p, c, o);
+      if constexpr (FDOT) store_pencil<Number, K, CPB, 0>(lds + I_SRC * BUF, p, c, u_loc);
     }
     if constexpr (USE_T) {
       Number v[K];
@@ -309,7 +336,6 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       matvec<Number, K, false>(S, v, o);
       store_pencil<Number, K, CPB, 0>(lds + (I_A + 1) * BUF, p, c, o);
     }
-    if constexpr (!PGRAD) ac.prefetch(dst, dpre);
   }
   __syncthreads();
 
@@ -359,7 +385,6 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   }
   __syncthreads();
 
-  const int qi = p % K, qj = p / K;
   const Number kc = (A.k_cell && ac.active) ? A.k_cell[ac.cell] : Number(1);
 
   // ---------------- T gradients -> per-quadrature-point coefficients ----------------
@@ -372,7 +397,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 #pragma unroll
     for (int z = 0; z < K; ++z) {
       const int e = elem<K>(qi, qj, z) * CPB + c;
-      const Number g0 = L2[e] * ac.ih[0], g1 = lds[3 * BUF + e] * ac.ih[1], g2 = gTz[z] * ac.ih[2];
+      const Number g0 = L2[e] * ih[0], g1 = lds[3 * BUF + e] * ih[1], g2 = gTz[z] * ih[2];
       if constexpr (PGRAD) {
         dkg[0][z] = g0;
         dkg[1][z] = g1;
@@ -398,7 +423,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   }
 
   // ---------------- Q: quadrature-point operation (z pencil (i,j), q = (i,j,z)) ----------------
-  const Number wij = A.w[qi] * A.w[qj] * ac.h[0] * ac.h[1] * ac.h[2];
+  const Number wij = A.w[qi] * A.w[qj] * (A.h[0] * hs) * (A.h[1] * hs) * (A.h[2] * hs);
   Number rq[K], fz[K];
 #pragma unroll
   for (int z = 0; z < K; ++z) {
@@ -406,9 +431,9 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     const Number jxw = wij * A.w[z];
     Number g[3] = {0, 0, 0};
     if constexpr (HAS_U) {
-      g[0] = L1[e] * ac.ih[0];
-      g[1] = L2[e] * ac.ih[1];
-      g[2] = guz[z] * ac.ih[2];
+      g[0] = L1[e] * ih[0];
+      g[1] = L2[e] * ih[1];
+      g[2] = guz[z] * ih[2];
     }
     Number w3[3] = {0, 0, 0};
     if constexpr (VEL) {
@@ -506,9 +531,9 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       lds[4 * BUF + e] = jxw * bb[2];
     } else {
       rq[z] = jxw * val;
-      L1[e] = jxw * f[0] * ac.ih[0];
-      L2[e] = jxw * f[1] * ac.ih[1];
-      fz[z] = jxw * f[2] * ac.ih[2];
+      L1[e] = jxw * f[0] * ih[0];
+      L2[e] = jxw * f[1] * ih[1];
+      fz[z] = jxw * f[2] * ih[2];
     }
   }
 
@@ -524,8 +549,10 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       for (int i = 0; i < K * K; ++i) sum += double(L0[i * CPB + c]);
       dst[ac.cell] += Number(A.pgrad_scale * sum);
     }
-    return;
+    continue;
   }
+
+  if constexpr (!PGRAD) ac.prefetch(dst, dpre);
 
   if constexpr (IS_DIAG) {
     __syncthreads();
@@ -536,8 +563,8 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     for (int ia = 0; ia < K; ++ia) diag[ia] = 0;
     for (int q3 = 0; q3 < K; ++q3)
       for (int q2 = 0; q2 < K; ++q2) {
-        const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * ac.ih[1];
-        const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * ac.ih[2];
+        const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * ih[1];
+        const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * ih[2];
 #pragma unroll
         for (int q1 = 0; q1 < K; ++q1) {
           const int e = elem<K>(q1, q2, q3) * CPB + c;
@@ -545,7 +572,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
                        b2 = lds[4 * BUF + e];
 #pragma unroll
           for (int ia = 0; ia < K; ++ia) {
-            const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * ac.ih[0];
+            const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * ih[0];
             const Number phi = sx * sy * sz;
             const Number gx = dx * sy * sz, gy = sx * dy * sz, gz = sx * sy * dz;
             diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) +
@@ -554,7 +581,6 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
         }
       }
     ac.scatter(A, diag, dpre, c, dst);
-    return;
   } else {
     // r = val + Dq_z^T fz (z direction in registers)
     Number tz[K];
@@ -593,57 +619,66 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       load_pencil<Number, K, CPB, 0>(L0, p, c, v);
       matvec<Number, K, true>(S, v, o);
     }
-    // src . (A src) of this cell (u_loc is zero on constrained entries), before the x-shared
-    // node is combined across cells
-    Number local_dot = 0;
-    if constexpr (TANGENT) {
-      if (ac.active) {
+    if constexpr (FDOT) {
+      // src . (A src) over the cell's nodes (constrained entries of the gathered source are 0),
+      // before the x-shared node is combined across cells
+      Number u[K];
+      load_pencil<Number, K, CPB, 0>(lds + I_SRC * BUF, p, c, u);
+      Number local_dot = 0;
 #pragma unroll
-        for (int i = 0; i < K; ++i) local_dot += u_loc[i] * o[i];
-      }
+      for (int i = 0; i < K; ++i) local_dot += u[i] * o[i];
+      dot_acc += double(local_dot);
     }
     ac.scatter(A, o, dpre, c, dst);
-    if constexpr (TANGENT) {
-      if (A.dot_partial) {
-        const double s = block_reduce_sum<double>(double(local_dot), red);
-        if (threadIdx.x == 0) A.dot_partial[lp.part_off + blockIdx.x] = s;
-      }
-    }
+  }
+  }  // steps along the row
+
+  if constexpr (FDOT) {
+    __syncthreads();  // L0 is reused for the wave partials
+    const double s = block_reduce_sum<double>(dot_acc, reinterpret_cast<double *>(lds));
+    if (threadIdx.x == 0) A0.dot_partial[lp.part_off + blockIdx.x] = s;
   }
 }
 
-template <int K>
-static int strip_blocks(const MeshInfo &m, int colour)
+static int row_blocks(const MeshInfo &m, int colour)
 {
-  const StripColour s = strip_colour<cells_per_block<K>()>(m.ncell, colour);
-  return s.hx * s.hy * s.hz;
+  const RowColour s = row_colour(m.ncell, colour);
+  return s.hy * s.hz;
 }
 
 int cell_blocks(const MeshInfo &m)
 {
   int total = 0;
-  for (int colour = 0; colour < 8; ++colour)
-    total += m.K == 2 ? strip_blocks<2>(m, colour)
-                      : (m.K == 3 ? strip_blocks<3>(m, colour) : strip_blocks<4>(m, colour));
+  for (int colour = 0; colour < 4; ++colour) total += row_blocks(m, colour);
   return total;
+}
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT>
+static int launch_colours(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
+                          Number *dst, hipStream_t st)
+{
+  constexpr int CPB = cells_per_block<K>();
+  int part_off = 0;
+  for (int colour = 0; colour < 4; ++colour) {
+    const int blocks = row_blocks(m, colour);
+    if (blocks == 0) continue;
+    const LaunchPart lp{colour, part_off};
+    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL, DOT, GridAccess<Number, K, CPB>>),
+                       dim3(blocks), dim3(CPB * K * K), 0, st, a, src, dst, lp);
+    MF_HIP_TRY(hipGetLastError());
+    part_off += blocks;
+  }
+  return MF_OK;
 }
 
 template <typename Number, int K, int OP, bool NEEDT, bool VEL>
 static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src, Number *dst,
                     hipStream_t st)
 {
-  constexpr int CPB = cells_per_block<K>();
-  int part_off = 0;
-  for (int colour = 0; colour < 8; ++colour) {
-    const int blocks = strip_blocks<K>(m, colour);
-    if (blocks == 0) continue;
-    const LaunchPart lp{colour, part_off};
-    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL, GridAccess<Number, K, CPB>>),
-                       dim3(blocks), dim3(CPB * K * K), 0, st, a, src, dst, lp);
-    MF_HIP_TRY(hipGetLastError());
-    part_off += blocks;
+  if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT) {
+    if (a.dot_partial) return launch_colours<Number, K, OP, NEEDT, VEL, true>(m, a, src, dst, st);
   }
-  return MF_OK;
+  return launch_colours<Number, K, OP, NEEDT, VEL, false>(m, a, src, dst, st);
 }
 
 template <typename Number, int K, int OP>
