@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="only run 5 operator applications (vmult_add) and 5 plain copies of one "
+                         "DoF vector (calibration), for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU sharding "
@@ -75,6 +78,15 @@ def main():
     src = torch.tensor(b, dtype=torch.float64, device="cuda")
     dst = torch.zeros(n, dtype=torch.float64, device="cuda")
     print(f"# workload: Q3 64^3, n_dofs={n}, n_cells={s.n_cells}", file=sys.stderr, flush=True)
+    if args.pmc_workload:
+        for _ in range(5):
+            op.vmult_add(dst, src)
+        other = torch.empty_like(src)
+        for _ in range(5):
+            other.copy_(src)
+        torch.cuda.synchronize()
+        print("# pmc workload done", file=sys.stderr, flush=True)
+        return
 
     def timed(fn, steps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

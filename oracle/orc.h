@@ -15,7 +15,10 @@
  *   - Jacobi, CG, GMRES(right preconditioned), ReductionControl   (include/meltpooldg/linear_algebra/linear_solver.hpp:22-134,
  *                                                                   include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:51-69)
  *   - FE L2 norm with per-cell float storage                      (source/utilities/vector_tools.cpp:137-201)
- * on structured axis-aligned box meshes in 1, 2 or 3 dimensions with lexicographic DoF numbering.
+ *   - hanging-node constraints of adaptive meshes                 (include/meltpooldg/utilities/constraints.hpp:117-151,
+ *                                                                   dealii::DoFTools::make_hanging_node_constraints)
+ * on structured axis-aligned box meshes in 1, 2 or 3 dimensions with lexicographic DoF numbering,
+ * and on 2:1-balanced adaptive octree meshes in 3D.
  *
  * Parity pins for every function are listed in tests/test_oracle_*.py (closed forms, brute-force
  * dense element matrices, dense direct solves, and the reference's own golden outputs).
@@ -62,6 +65,29 @@ typedef struct {
 
 enum { ORC_OP_HEAT = 0, ORC_OP_ADVEC = 1 };
 
+/* Adaptive hexahedral mesh: leaves of an octree over a base box mesh (2:1 balanced by the
+ * caller).  Cell c is the cube of level cells[4c], index (cells[4c+1..4c+3]) on the lattice of
+ * that level (base cells per direction * 2^level). */
+typedef struct {
+  int degree;
+  int ncell[3];            /* base (level 0) cells per direction */
+  double lo[3];
+  double hi[3];
+  long n_cells;
+  const int32_t *cells;    /* [n_cells][4]: level, ix, iy, iz */
+} orc_amesh;
+
+/* Nodes and hanging-node constraints of an adaptive mesh (orc_amesh_build). */
+typedef struct {
+  long n_nodes;
+  long *cell_nodes;        /* [n_cells][K^3], local nodes in lexicographic order */
+  double *xyz;             /* [n_nodes][3] support points */
+  uint8_t *boundary;       /* [n_nodes] 1 = on the domain boundary */
+  long *line_ptr;          /* [n_nodes + 1]: constraint line of each node (empty = free) */
+  long *line_col;          /* master nodes (free nodes only: chains resolved) */
+  double *line_val;        /* weights */
+} orc_adata;
+
 typedef struct {
   int kind;                 /* ORC_OP_HEAT or ORC_OP_ADVEC */
   orc_mesh mesh;
@@ -75,6 +101,8 @@ typedef struct {
   const double *vel;        /* nodal velocity, component-major [3][n_dofs] */
   const double *k_cell;     /* per-cell conductivity factor, NULL = 1 */
   int nthreads;             /* OpenMP threads, <= 0: library default */
+  const orc_amesh *amesh;   /* adaptive mesh (NULL: the structured mesh above) */
+  const orc_adata *adata;   /* its nodes and constraints */
 } orc_operator;
 
 /* --- basis / mesh ----------------------------------------------------------------------- */
@@ -96,6 +124,11 @@ void orc_diagonal(const orc_operator *op, double *diag);       /* raw diagonal, 
 void orc_invert_diagonal(int kind, long n, double *diag);      /* reference inversion thresholds */
 void orc_create_rhs(const orc_operator *op, const double *src, const double *q_s, double *dst);
 void orc_local_matrix(const orc_operator *op, long cell, double *A); /* dense element matrix */
+
+/* --- adaptive meshes with hanging nodes (vmult, diagonal and solve dispatch on op->amesh) ---- */
+int orc_amesh_build(const orc_amesh *am, orc_adata *ad);  /* 0 = success */
+void orc_adata_free(orc_adata *ad);
+void orc_distribute_constraints(const orc_adata *ad, double *x); /* x_h = sum_m w x_m */
 
 /* --- solvers -------------------------------------------------------------------------------- */
 enum { ORC_CG = 0, ORC_GMRES = 1 };
