@@ -406,10 +406,16 @@ typedef struct {
 
 static int has_vel(const orc_operator *op) { return op->vel_kind != ORC_VEL_NONE; }
 
+/* number of DoF entries of the operator's vectors (structured or adaptive mesh) */
+static long op_ndofs(const orc_operator *op)
+{
+  return op->amesh ? op->adata->n_nodes : orc_n_dofs(&op->mesh);
+}
+
 /* velocity at the quadrature points of a cell */
 static void eval_velocity(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
 {
-  const long n = orc_n_dofs(&op->mesh);
+  const long n = op_ndofs(op);
   for (int e = 0; e < 3; ++e)
     for (int q = 0; q < g->nloc; ++q) ws->wv[e][q] = 0.0;
   if (op->vel_kind == ORC_VEL_NODAL && op->vel) {
@@ -581,8 +587,15 @@ static void vmult_cell(const orc_operator *op, const geo_t *g, long cell, cell_w
   scatter(op, g, io->dst, ws, op->use_mask);
 }
 
+static void avmult(const orc_operator *op, const double *src, double *dst);
+static void adiagonal(const orc_operator *op, double *diag);
+
 void orc_vmult(const orc_operator *op, const double *src, double *dst)
 {
+  if (op->amesh) {
+    avmult(op, src, dst);
+    return;
+  }
   geo_t g;
   geo_init(&op->mesh, &g);
   const long n = orc_n_dofs(&op->mesh);
@@ -764,6 +777,10 @@ static void diag_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws
 
 void orc_diagonal(const orc_operator *op, double *diag)
 {
+  if (op->amesh) {
+    adiagonal(op, diag);
+    return;
+  }
   geo_t g;
   geo_init(&op->mesh, &g);
   const long n = orc_n_dofs(&op->mesh);
@@ -1018,7 +1035,7 @@ int orc_solve(const orc_operator *op, int solver, int restart, int max_it, doubl
               double rel_tol, const double *inv_diag, const double *b, double *x, int *iterations,
               double *final_residual)
 {
-  const long n = orc_n_dofs(&op->mesh);
+  const long n = op_ndofs(op);
   if (solver == ORC_CG)
     return solve_cg(op, n, max_it, abs_tol, rel_tol, inv_diag, b, x, iterations, final_residual);
   return solve_gmres(op, n, restart > 0 ? restart : 30, max_it, abs_tol, rel_tol, inv_diag, b, x,
@@ -1134,4 +1151,407 @@ int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *
   free(xd);
   free(inv);
   return status;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Adaptive meshes: node identification and hanging-node constraints                          */
+/* (the constraints deal.II's DoFTools::make_hanging_node_constraints produces for FE_Q on a     */
+/* 2:1-balanced mesh, as used by Constraints::make_HNC, constraints.hpp:117-136).                */
+/*                                                                                              */
+/* Nodes are identified by their coordinates, quantised to 1/1024 of the finest cell: distinct   */
+/* Gauss-Lobatto nodes are at least 0.17 of the finest cell apart.  A node is constrained when  */
+/* it is not a node of the coarsest cell whose closure contains it; its line is the             */
+/* interpolation of that cell's Lagrange basis at the node, with chains resolved.               */
+/* ------------------------------------------------------------------------------------------ */
+
+typedef struct {
+  long k[3];
+  long id;
+} hslot_t;
+
+static uint64_t key_hash(const long *k)
+{
+  uint64_t h = orc_splitmix64((uint64_t)k[0]);
+  h = orc_splitmix64(h ^ (uint64_t)k[1]);
+  return orc_splitmix64(h ^ (uint64_t)k[2]);
+}
+
+static int amesh_levels(const orc_amesh *am)
+{
+  int L = 0;
+  for (long c = 0; c < am->n_cells; ++c)
+    if (am->cells[4 * c] > L) L = am->cells[4 * c];
+  return L;
+}
+
+/* cell geometry as a one-cell structured mesh */
+static void acell_geo(const orc_amesh *am, long cell, geo_t *g)
+{
+  const int32_t *cl = am->cells + 4 * cell;
+  orc_mesh m;
+  m.dim = 3;
+  m.degree = am->degree;
+  for (int d = 0; d < 3; ++d) {
+    const double h0 = (am->hi[d] - am->lo[d]) / am->ncell[d];
+    const double h = h0 / (double)(1L << cl[0]);
+    m.ncell[d] = 1;
+    m.lo[d] = am->lo[d] + h * cl[1 + d];
+    m.hi[d] = m.lo[d] + h;
+  }
+  geo_init(&m, g);
+}
+
+int orc_amesh_build(const orc_amesh *am, orc_adata *ad)
+{
+  memset(ad, 0, sizeof(*ad));
+  const int K = am->degree + 1, nloc = K * K * K;
+  const int L = amesh_levels(am);
+  const long nc = am->n_cells;
+  double gl[KMAX];
+  orc_gauss_lobatto(K, gl);
+  /* positions in finest-cell units: u = (ix + gl[a]) * 2^(L - level) */
+  long cap = 1;
+  while (cap < 2 * nc * nloc + 16) cap <<= 1;
+  hslot_t *tab = (hslot_t *)malloc(sizeof(hslot_t) * cap);
+  for (long i = 0; i < cap; ++i) tab[i].id = -1;
+  ad->cell_nodes = (long *)malloc(sizeof(long) * nc * nloc);
+  double *upos = (double *)malloc(sizeof(double) * 3 * nc * nloc);
+  long nn = 0;
+  for (long c = 0; c < nc; ++c) {
+    const int32_t *cl = am->cells + 4 * c;
+    const double sc = (double)(1L << (L - cl[0]));
+    for (int l = 0; l < nloc; ++l) {
+      const int a[3] = {l % K, (l / K) % K, l / (K * K)};
+      double u[3];
+      long k[3];
+      for (int d = 0; d < 3; ++d) {
+        u[d] = ((double)cl[1 + d] + gl[a[d]]) * sc;
+        k[d] = lround(u[d] * 1024.0);
+      }
+      uint64_t h = key_hash(k) & (uint64_t)(cap - 1);
+      for (;;) {
+        if (tab[h].id < 0) {
+          tab[h].k[0] = k[0];
+          tab[h].k[1] = k[1];
+          tab[h].k[2] = k[2];
+          tab[h].id = nn;
+          for (int d = 0; d < 3; ++d) upos[3 * nn + d] = u[d];
+          ++nn;
+          break;
+        }
+        if (tab[h].k[0] == k[0] && tab[h].k[1] == k[1] && tab[h].k[2] == k[2]) break;
+        h = (h + 1) & (uint64_t)(cap - 1);
+      }
+      ad->cell_nodes[c * nloc + l] = tab[h].id;
+    }
+  }
+  free(tab);
+  ad->n_nodes = nn;
+  ad->xyz = (double *)malloc(sizeof(double) * 3 * nn);
+  ad->boundary = (uint8_t *)calloc(nn, 1);
+  for (long i = 0; i < nn; ++i)
+    for (int d = 0; d < 3; ++d) {
+      const double ext = (double)am->ncell[d] * (double)(1L << L);
+      const double h0 = (am->hi[d] - am->lo[d]) / am->ncell[d];
+      ad->xyz[3 * i + d] = am->lo[d] + h0 * upos[3 * i + d] / (double)(1L << L);
+      if (fabs(upos[3 * i + d]) < 1e-9 || fabs(upos[3 * i + d] - ext) < 1e-9) ad->boundary[i] = 1;
+    }
+  /* leaves per base cell for point location */
+  const long nb = (long)am->ncell[0] * am->ncell[1] * am->ncell[2];
+  if (nb <= 0 || nb > (1L << 40)) return 1;
+  long *bptr = (long *)calloc(nb + 1, sizeof(long)), *blist = (long *)malloc(sizeof(long) * nc);
+  for (long c = 0; c < nc; ++c) {
+    const int32_t *cl = am->cells + 4 * c;
+    const long b = (cl[1] >> cl[0]) + am->ncell[0] * ((cl[2] >> cl[0]) + (long)am->ncell[1] * (cl[3] >> cl[0]));
+    bptr[b + 1]++;
+  }
+  for (long b = 0; b < nb; ++b) bptr[b + 1] += bptr[b];
+  long *fill = (long *)calloc(nb, sizeof(long));
+  for (long c = 0; c < nc; ++c) {
+    const int32_t *cl = am->cells + 4 * c;
+    const long b = (cl[1] >> cl[0]) + am->ncell[0] * ((cl[2] >> cl[0]) + (long)am->ncell[1] * (cl[3] >> cl[0]));
+    blist[bptr[b] + fill[b]++] = c;
+  }
+  free(fill);
+  /* direct constraints */
+  const double tol = 1e-9;
+  long *dptr = (long *)calloc(nn + 1, sizeof(long));
+  long dcap = 1024, dn = 0;
+  long *dcol = (long *)malloc(sizeof(long) * dcap);
+  double *dval = (double *)malloc(sizeof(double) * dcap);
+  const long B = 1L << L;
+  for (long i = 0; i < nn; ++i) {
+    const double *u = upos + 3 * i;
+    long best = -1;
+    int blev = 1 << 30;
+    long lo_b[3], hi_b[3];
+    for (int d = 0; d < 3; ++d) {
+      lo_b[d] = (long)floor((u[d] - tol) / (double)B);
+      hi_b[d] = (long)floor((u[d] + tol) / (double)B);
+      if (lo_b[d] < 0) lo_b[d] = 0;
+      if (hi_b[d] > am->ncell[d] - 1) hi_b[d] = am->ncell[d] - 1;
+    }
+    for (long bz = lo_b[2]; bz <= hi_b[2]; ++bz)
+      for (long by = lo_b[1]; by <= hi_b[1]; ++by)
+        for (long bx = lo_b[0]; bx <= hi_b[0]; ++bx) {
+          const long b = bx + am->ncell[0] * (by + (long)am->ncell[1] * bz);
+          for (long t = bptr[b]; t < bptr[b + 1]; ++t) {
+            const long c = blist[t];
+            const int32_t *cl = am->cells + 4 * c;
+            const double sc = (double)(1L << (L - cl[0]));
+            int in = 1;
+            for (int d = 0; d < 3 && in; ++d)
+              in = u[d] >= cl[1 + d] * sc - tol && u[d] <= (cl[1 + d] + 1) * sc + tol;
+            if (in && cl[0] < blev) {
+              blev = cl[0];
+              best = c;
+            }
+          }
+        }
+    dptr[i + 1] = dptr[i];
+    if (best < 0) continue;
+    const int32_t *cl = am->cells + 4 * best;
+    const double sc = (double)(1L << (L - cl[0]));
+    double wv[3][KMAX];
+    int node_of_cell = 1;
+    for (int d = 0; d < 3; ++d) {
+      const double t = u[d] / sc - cl[1 + d];
+      int match = -1;
+      for (int j = 0; j < K; ++j)
+        if (fabs(t - gl[j]) < 1e-12) match = j;
+      if (match >= 0) {
+        for (int j = 0; j < K; ++j) wv[d][j] = (j == match) ? 1.0 : 0.0;
+      } else {
+        double der[KMAX];
+        orc_lagrange(K, gl, t, wv[d], der);
+        node_of_cell = 0;
+      }
+    }
+    if (node_of_cell) continue;
+    for (int l = 0; l < nloc; ++l) {
+      const double w = wv[0][l % K] * wv[1][(l / K) % K] * wv[2][l / (K * K)];
+      if (w == 0.0) continue;
+      if (dn == dcap) {
+        dcap *= 2;
+        dcol = (long *)realloc(dcol, sizeof(long) * dcap);
+        dval = (double *)realloc(dval, sizeof(double) * dcap);
+      }
+      dcol[dn] = ad->cell_nodes[best * nloc + l];
+      dval[dn] = w;
+      ++dn;
+    }
+    dptr[i + 1] = dn;
+  }
+  free(bptr);
+  free(blist);
+  free(upos);
+  /* resolve chains: expand constrained masters until every master is free */
+  long *ptr = dptr, *col = dcol;
+  double *val = dval;
+  for (int pass = 0; pass < 16; ++pass) {
+    int changed = 0;
+    long ncap = ptr[nn] + 16, nnz = 0;
+    long *nptr = (long *)calloc(nn + 1, sizeof(long));
+    long *ncol = (long *)malloc(sizeof(long) * ncap);
+    double *nval = (double *)malloc(sizeof(double) * ncap);
+    for (long i = 0; i < nn; ++i) {
+      const long start = nnz;
+      for (long t = ptr[i]; t < ptr[i + 1]; ++t) {
+        const long m = col[t];
+        if (ptr[m + 1] > ptr[m]) {
+          changed = 1;
+          for (long s2 = ptr[m]; s2 < ptr[m + 1]; ++s2) {
+            if (nnz == ncap) {
+              ncap *= 2;
+              ncol = (long *)realloc(ncol, sizeof(long) * ncap);
+              nval = (double *)realloc(nval, sizeof(double) * ncap);
+            }
+            ncol[nnz] = col[s2];
+            nval[nnz] = val[t] * val[s2];
+            ++nnz;
+          }
+        } else {
+          if (nnz == ncap) {
+            ncap *= 2;
+            ncol = (long *)realloc(ncol, sizeof(long) * ncap);
+            nval = (double *)realloc(nval, sizeof(double) * ncap);
+          }
+          ncol[nnz] = m;
+          nval[nnz] = val[t];
+          ++nnz;
+        }
+      }
+      /* merge duplicate masters */
+      long w = start;
+      for (long a = start; a < nnz; ++a) {
+        long b = start;
+        for (; b < w; ++b)
+          if (ncol[b] == ncol[a]) break;
+        if (b < w) {
+          nval[b] += nval[a];
+        } else {
+          ncol[w] = ncol[a];
+          nval[w] = nval[a];
+          ++w;
+        }
+      }
+      nnz = w;
+      nptr[i + 1] = nnz;
+    }
+    free(ptr);
+    free(col);
+    free(val);
+    ptr = nptr;
+    col = ncol;
+    val = nval;
+    if (!changed) break;
+  }
+  ad->line_ptr = ptr;
+  ad->line_col = col;
+  ad->line_val = val;
+  return 0;
+}
+
+void orc_adata_free(orc_adata *ad)
+{
+  free(ad->cell_nodes);
+  free(ad->xyz);
+  free(ad->boundary);
+  free(ad->line_ptr);
+  free(ad->line_col);
+  free(ad->line_val);
+  memset(ad, 0, sizeof(*ad));
+}
+
+void orc_distribute_constraints(const orc_adata *ad, double *x)
+{
+  for (long i = 0; i < ad->n_nodes; ++i) {
+    if (ad->line_ptr[i + 1] == ad->line_ptr[i]) continue;
+    double s = 0.0;
+    for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t) s += ad->line_val[t] * x[ad->line_col[t]];
+    x[i] = s;
+  }
+}
+
+static int amasked(const orc_operator *op, long n)
+{
+  return op->use_mask && op->mask && op->mask[n];
+}
+
+/* read_dof_values with hanging-node and Dirichlet constraints */
+static void agather(const orc_operator *op, const double *src, int nloc, cell_ws_t *ws)
+{
+  const orc_adata *ad = op->adata;
+  for (int l = 0; l < nloc; ++l) {
+    const long i = ws->idx[l];
+    if (ad->line_ptr[i + 1] > ad->line_ptr[i]) {
+      double s = 0.0;
+      for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t)
+        if (!amasked(op, ad->line_col[t])) s += ad->line_val[t] * src[ad->line_col[t]];
+      ws->u[l] = s;
+    } else {
+      ws->u[l] = amasked(op, i) ? 0.0 : src[i];
+    }
+  }
+}
+
+/* distribute_local_to_global: hanging rows go to their masters, constrained rows skipped */
+static void ascatter(const orc_operator *op, const double *out, int nloc, const long *idx,
+                     double *dst)
+{
+  const orc_adata *ad = op->adata;
+  for (int l = 0; l < nloc; ++l) {
+    const long i = idx[l];
+    if (ad->line_ptr[i + 1] > ad->line_ptr[i]) {
+      for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t)
+        if (!amasked(op, ad->line_col[t])) dst[ad->line_col[t]] += ad->line_val[t] * out[l];
+    } else if (!amasked(op, i)) {
+      dst[i] += out[l];
+    }
+  }
+}
+
+static void avmult(const orc_operator *op, const double *src, double *dst)
+{
+  const orc_amesh *am = op->amesh;
+  const int K = am->degree + 1, nloc = K * K * K;
+  memset(dst, 0, sizeof(double) * op->adata->n_nodes);
+  cell_ws_t *ws = (cell_ws_t *)malloc(sizeof(cell_ws_t));
+  for (long c = 0; c < am->n_cells; ++c) {
+    geo_t g;
+    acell_geo(am, c, &g);
+    for (int l = 0; l < nloc; ++l) ws->idx[l] = op->adata->cell_nodes[c * nloc + l];
+    agather(op, src, nloc, ws);
+    tangent_cell(op, &g, c, ws);
+    ascatter(op, ws->out, nloc, ws->idx, dst);
+  }
+  free(ws);
+}
+
+/* exact diagonal of the condensed operator: for every target DoF D of a cell, the local vector
+ * c_D (c_D[l] = 1 if local node l is D, the weight of D if l is a hanging node with master D)
+ * gives diag[D] += c_D^T A_cell c_D (MatrixFreeTools::compute_diagonal with constraints). */
+static void adiagonal(const orc_operator *op, double *diag)
+{
+  const orc_amesh *am = op->amesh;
+  const orc_adata *ad = op->adata;
+  const int K = am->degree + 1, nloc = K * K * K;
+  memset(diag, 0, sizeof(double) * ad->n_nodes);
+  cell_ws_t *ws = (cell_ws_t *)malloc(sizeof(cell_ws_t));
+  double *A = (double *)malloc(sizeof(double) * nloc * nloc);
+  long *targets = (long *)malloc(sizeof(long) * nloc * 64);
+  double *cv = (double *)malloc(sizeof(double) * nloc);
+  for (long c = 0; c < am->n_cells; ++c) {
+    geo_t g;
+    acell_geo(am, c, &g);
+    for (int l = 0; l < nloc; ++l) ws->idx[l] = ad->cell_nodes[c * nloc + l];
+    /* local matrix, column by column */
+    for (int j = 0; j < nloc; ++j) {
+      for (int l = 0; l < nloc; ++l) ws->u[l] = (l == j) ? 1.0 : 0.0;
+      tangent_cell(op, &g, c, ws);
+      for (int l = 0; l < nloc; ++l) A[l * nloc + j] = ws->out[l];
+    }
+    int nt = 0;
+    for (int l = 0; l < nloc; ++l) {
+      const long i = ws->idx[l];
+      if (ad->line_ptr[i + 1] > ad->line_ptr[i]) {
+        for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t) {
+          int found = 0;
+          for (int q = 0; q < nt; ++q) found |= targets[q] == ad->line_col[t];
+          if (!found) targets[nt++] = ad->line_col[t];
+        }
+      } else {
+        int found = 0;
+        for (int q = 0; q < nt; ++q) found |= targets[q] == i;
+        if (!found) targets[nt++] = i;
+      }
+    }
+    for (int q = 0; q < nt; ++q) {
+      const long D = targets[q];
+      if (amasked(op, D)) continue;
+      for (int l = 0; l < nloc; ++l) {
+        const long i = ws->idx[l];
+        double w = 0.0;
+        if (ad->line_ptr[i + 1] > ad->line_ptr[i]) {
+          for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t)
+            if (ad->line_col[t] == D) w += ad->line_val[t];
+        } else if (i == D) {
+          w = 1.0;
+        }
+        cv[l] = w;
+      }
+      double s = 0.0;
+      for (int a = 0; a < nloc; ++a) {
+        if (cv[a] == 0.0) continue;
+        double r = 0.0;
+        for (int b = 0; b < nloc; ++b) r += A[a * nloc + b] * cv[b];
+        s += cv[a] * r;
+      }
+      diag[D] += s;
+    }
+  }
+  free(ws);
+  free(A);
+  free(targets);
+  free(cv);
 }

@@ -38,11 +38,24 @@ class AdvecParams(C.Structure):
                 ("mass_factor", C.c_double), ("vel_scale", C.c_double)]
 
 
+class AMeshC(C.Structure):
+    _fields_ = [("degree", C.c_int), ("ncell", C.c_int * 3), ("lo", C.c_double * 3),
+                ("hi", C.c_double * 3), ("n_cells", C.c_long), ("cells", C.c_void_p)]
+
+
+class ADataC(C.Structure):
+    _fields_ = [("n_nodes", C.c_long), ("cell_nodes", C.POINTER(C.c_long)),
+                ("xyz", C.POINTER(C.c_double)), ("boundary", C.POINTER(C.c_uint8)),
+                ("line_ptr", C.POINTER(C.c_long)), ("line_col", C.POINTER(C.c_long)),
+                ("line_val", C.POINTER(C.c_double))]
+
+
 class Operator(C.Structure):
     _fields_ = [("kind", C.c_int), ("mesh", Mesh), ("heat", HeatParams), ("advec", AdvecParams),
                 ("vel_kind", C.c_int), ("mask", C.c_void_p), ("use_mask", C.c_int),
                 ("T", C.c_void_p), ("T_old", C.c_void_p), ("vel", C.c_void_p),
-                ("k_cell", C.c_void_p), ("nthreads", C.c_int)]
+                ("k_cell", C.c_void_p), ("nthreads", C.c_int),
+                ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC))]
 
 
 def build(force: bool = False) -> str:
@@ -73,6 +86,9 @@ def lib():
                                 C.c_double, C.c_void_p, dp, dp, C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]
         L.orc_local_matrix.argtypes = [C.POINTER(Operator), C.c_long, dp]
+        L.orc_amesh_build.argtypes = [C.POINTER(AMeshC), C.POINTER(ADataC)]
+        L.orc_adata_free.argtypes = [C.POINTER(ADataC)]
+        L.orc_distribute_constraints.argtypes = [C.POINTER(ADataC), dp]
         _lib = L
     return _lib
 
@@ -135,14 +151,62 @@ def random_field(mesh, seed):
     return out
 
 
+class AdaptiveMesh:
+    """An adaptive octree mesh (cells = int array [n_cells, 4]: level, ix, iy, iz) with the
+    oracle's own node identification and hanging-node constraints."""
+
+    def __init__(self, degree, ncell, cells, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0)):
+        if isinstance(ncell, int):
+            ncell = (ncell,) * 3
+        self.degree = degree
+        self.K = degree + 1
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        self.n_cells = len(self.cells)
+        self.c = AMeshC(degree, (C.c_int * 3)(*ncell), (C.c_double * 3)(*lo),
+                        (C.c_double * 3)(*hi), self.n_cells, self.cells.ctypes.data)
+        self.d = ADataC()
+        if lib().orc_amesh_build(C.byref(self.c), C.byref(self.d)) != 0:
+            raise ValueError("orc_amesh_build failed")
+        n = self.n_nodes = int(self.d.n_nodes)
+        nloc = self.K ** 3
+        self.cell_nodes = np.ctypeslib.as_array(self.d.cell_nodes, (self.n_cells * nloc,)).reshape(
+            self.n_cells, nloc).copy()
+        self.xyz = np.ctypeslib.as_array(self.d.xyz, (n * 3,)).reshape(n, 3).copy()
+        self.boundary = np.ctypeslib.as_array(self.d.boundary, (n,)).copy()
+        self.line_ptr = np.ctypeslib.as_array(self.d.line_ptr, (n + 1,)).copy()
+        nnz = int(self.line_ptr[-1])
+        self.line_col = (np.ctypeslib.as_array(self.d.line_col, (nnz,)).copy() if nnz
+                         else np.zeros(0, dtype=np.int64))
+        self.line_val = (np.ctypeslib.as_array(self.d.line_val, (nnz,)).copy() if nnz
+                         else np.zeros(0))
+        self.hanging = (self.line_ptr[1:] > self.line_ptr[:-1]).astype(np.uint8)
+        # Dirichlet on the boundary; a hanging node keeps its hanging constraint
+        # (AffineConstraints::merge with right_object_wins, constraints.hpp:142-151)
+        self.dirichlet = (self.boundary.astype(bool) & ~self.hanging.astype(bool)).astype(np.uint8)
+
+    def distribute(self, x):
+        x = np.array(x, dtype=np.float64)
+        lib().orc_distribute_constraints(C.byref(self.d), _p(x))
+        return x
+
+    def __del__(self):
+        try:
+            lib().orc_adata_free(C.byref(self.d))
+        except Exception:
+            pass
+
+
 class OracleOperator:
     """Holds the numpy arrays an ``orc_operator`` points to, so they stay alive."""
 
     def __init__(self, kind, mesh, heat=None, advec=None, vel_kind=VEL_NONE, mask=None,
-                 T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1):
+                 T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None):
         self._keep = []
+        self.amesh = amesh
+        if amesh is not None:
+            mesh = make_mesh(3, amesh.degree, tuple(amesh.c.ncell))
         self.mesh = mesh
-        self.n = n_dofs(mesh)
+        self.n = amesh.n_nodes if amesh is not None else n_dofs(mesh)
 
         def ptr(a, dtype=np.float64):
             if a is None:
@@ -164,6 +228,9 @@ class OracleOperator:
         self.op.vel = ptr(vel)
         self.op.k_cell = ptr(k_cell)
         self.op.nthreads = nthreads
+        if amesh is not None:
+            self.op.amesh = C.pointer(amesh.c)
+            self.op.adata = C.pointer(amesh.d)
 
     def vmult(self, src):
         dst = np.zeros(self.n)
@@ -234,3 +301,71 @@ class OracleOperator:
 def fe_l2_norm(mesh, u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     return float(lib().orc_fe_l2_norm(C.byref(mesh), _p(u)))
+
+
+def refine_band_cells(ncell, levels, center, radius, half_width, lo=(0.0, 0.0, 0.0),
+                      hi=(1.0, 1.0, 1.0)):
+    """Reference generator of the band-refined test meshes (pure Python, small meshes only):
+    cells whose box meets the shell | |x - center| - radius | < half_width are refined, level by
+    level, `levels` times; then the mesh is 2:1 balanced across faces, edges and vertices by
+    refining the coarser cell.  Returns int array [n_cells, 4] (level, ix, iy, iz), sorted."""
+    if isinstance(ncell, int):
+        ncell = (ncell,) * 3
+    lo, hi, center = np.asarray(lo, float), np.asarray(hi, float), np.asarray(center, float)
+    h0 = (hi - lo) / np.asarray(ncell)
+
+    def meets(l, ix):
+        a = lo + h0 * np.asarray(ix) / 2 ** l
+        b = a + h0 / 2 ** l
+        near = np.clip(center, a, b)
+        dmin = np.linalg.norm(near - center)
+        far = np.where(np.abs(a - center) > np.abs(b - center), a, b)
+        dmax = np.linalg.norm(far - center)
+        return dmax > radius - half_width and dmin < radius + half_width
+
+    leaves = {(0, i, j, k) for i in range(ncell[0]) for j in range(ncell[1])
+              for k in range(ncell[2])}
+    for lev in range(levels):
+        new = set()
+        for c in leaves:
+            if c[0] == lev and meets(c[0], c[1:]):
+                for d in range(8):
+                    new.add((lev + 1, 2 * c[1] + (d & 1), 2 * c[2] + ((d >> 1) & 1),
+                             2 * c[3] + (d >> 2)))
+            else:
+                new.add(c)
+        leaves = new
+
+    def covering(l, ix):
+        for lc in range(l, -1, -1):
+            s = l - lc
+            key = (lc, ix[0] >> s, ix[1] >> s, ix[2] >> s)
+            if key in leaves:
+                return key
+        return None
+
+    changed = True
+    while changed:
+        changed = False
+        refine = set()
+        for c in leaves:
+            l = c[0]
+            if l < 2:
+                continue
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        nb = (c[1] + dx, c[2] + dy, c[3] + dz)
+                        if any(nb[d] < 0 or nb[d] >= ncell[d] << l for d in range(3)):
+                            continue
+                        cov = covering(l, nb)
+                        if cov is not None and cov[0] < l - 1:
+                            refine.add(cov)
+        if refine:
+            changed = True
+            for c in refine:
+                leaves.discard(c)
+                for d in range(8):
+                    leaves.add((c[0] + 1, 2 * c[1] + (d & 1), 2 * c[2] + ((d >> 1) & 1),
+                                2 * c[3] + (d >> 2)))
+    return np.array(sorted(leaves), dtype=np.int32)
