@@ -72,6 +72,41 @@ int mf_scratch_set_dirichlet_mask(mf_scratch *s, const uint8_t *mask_host);
 int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq, double *w,
                           double *gl);
 
+/* --- adaptive meshes with hanging nodes --------------------------------------------------- */
+/* Band-refined octree mesh over a base box mesh: cells meeting the spherical shell
+ * | |x - center| - radius | < half_width are refined `levels` times, level by level, then the
+ * mesh is 2:1 balanced across faces, edges and vertices.  Output: cells[n][4] = (level, ix, iy,
+ * iz) in Morton order.  With cells == NULL only *n_cells is set.  Replaces the refinement loop
+ * of the mp-heat-transfer application (applications/mp-heat-transfer/
+ * heat_transfer_application.cpp:484-527, source/utilities/amr.cpp) for the benchmark mesh. */
+int mf_mesh_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
+                        const double center[3], double radius, double half_width, int32_t *cells,
+                        int64_t capacity, int64_t *n_cells);
+/* Scratch on an adaptive mesh (cells as above, 2:1 balanced): DoF numbering in order of first
+ * appearance over the cells, hanging-node constraints with chains resolved, and the default
+ * Dirichlet set = boundary nodes that are not hanging (HNC merged into DBC with the hanging line
+ * winning).  Replaces FiniteElementUtils::distribute_dofs (fe_util.hpp:21-81) and
+ * Constraints::make_DBC_and_HNC_plus_PBC_and_merge_HNC_plus_PBC_into_DBC (constraints.hpp:
+ * 117-151, 254-269).  k_cell fields are indexed in the order of `cells`. */
+int mf_scratch_create_adaptive(int degree, const int ncell[3], const double lo[3],
+                               const double hi[3], const int32_t *cells, int64_t n_cells,
+                               mf_scratch **out);
+int mf_scratch_is_adaptive(const mf_scratch *s);
+/* support points [n_dofs][3] (host); DoFHandler support points (fe_util.hpp) */
+int mf_scratch_support_points(const mf_scratch *s, double *xyz);
+/* default Dirichlet rows (host, n_dofs bytes): all boundary nodes of a structured mesh,
+ * non-hanging boundary nodes of an adaptive mesh */
+int mf_scratch_default_dirichlet_mask(const mf_scratch *s, uint8_t *mask);
+/* number of hanging nodes, constraint entries and cell colours of the scratch */
+int mf_scratch_constraint_info(const mf_scratch *s, int64_t *n_hanging, int64_t *n_entries,
+                               int *n_colours);
+/* hanging-node constraint lines (host copies): node_line[n_dofs] (-1 = free), line_ptr
+ * [n_hanging+1], line_col / line_val [n_entries] (AffineConstraints lines after close()) */
+int mf_scratch_constraint_lines(const mf_scratch *s, int32_t *node_line, uint32_t *line_ptr,
+                                uint32_t *line_col, double *line_val);
+/* AffineConstraints::distribute on a device vector: x_h = sum_m w_hm x_m on hanging nodes */
+int mf_distribute_constraints(mf_scratch *s, double *x);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Operators.  Replaces OperatorMatrixFree<dim,double> (include/meltpooldg/core/              */
 /* operator_base.hpp:189-283) for HeatDiffuseMultiPhaseOperator (source/heat/                   */

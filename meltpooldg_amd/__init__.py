@@ -128,6 +128,18 @@ def lib():
         L.mf_set_stream.argtypes = [vp, vp]
         L.mf_scratch_set_dirichlet_mask.argtypes = [vp, vp]
         L.mf_scratch_shape_info.argtypes = [vp, vp, vp, vp, vp, vp]
+        dp3 = C.POINTER(C.c_double)
+        L.mf_mesh_refine_band.argtypes = [C.POINTER(C.c_int), dp3, dp3, C.c_int, dp3, C.c_double,
+                                          C.c_double, vp, i64, C.POINTER(i64)]
+        L.mf_scratch_create_adaptive.argtypes = [C.c_int, C.POINTER(C.c_int), dp3, dp3, vp, i64,
+                                                 C.POINTER(vp)]
+        L.mf_scratch_is_adaptive.argtypes = [vp]
+        L.mf_scratch_support_points.argtypes = [vp, vp]
+        L.mf_scratch_default_dirichlet_mask.argtypes = [vp, vp]
+        L.mf_scratch_constraint_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64),
+                                                 C.POINTER(C.c_int)]
+        L.mf_scratch_constraint_lines.argtypes = [vp, vp, vp, vp, vp]
+        L.mf_distribute_constraints.argtypes = [vp, vp]
         L.mf_heat_operator_create.argtypes = [vp, C.POINTER(HeatParams), C.POINTER(vp)]
         L.mf_advection_operator_create.argtypes = [vp, C.POINTER(AdvecParams), C.POINTER(vp)]
         L.mf_operator_destroy.argtypes = [vp]
@@ -253,6 +265,12 @@ class ScratchData:
         z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
         return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
 
+    def default_dirichlet_mask(self):
+        import numpy as np
+        m = np.zeros(self.n_dofs, dtype=np.uint8)
+        _check(lib().mf_scratch_default_dirichlet_mask(self._h, m.ctypes.data_as(C.c_void_p)))
+        return m
+
     def random_field(self, seed):
         """Seeded U(-1,1) per DoF keyed by the node's lattice coordinates (SplitMix64), the
         synthetic-input convention shared with the test oracle."""
@@ -271,6 +289,8 @@ class ScratchData:
 
     def boundary_mask(self):
         import numpy as np
+        if getattr(self, "adaptive", False):
+            return self.default_dirichlet_mask()
         nn = [self.ncell[d] * self.degree + 1 for d in range(3)]
         m = np.zeros((nn[2], nn[1], nn[0]), dtype=np.uint8)
         m[0, :, :] = m[-1, :, :] = 1
@@ -285,6 +305,79 @@ class ScratchData:
                 self._h = None
         except Exception:
             pass
+
+
+def refine_band_cells(ncell, levels, center, radius, half_width, lo=(0.0, 0.0, 0.0),
+                      hi=(1.0, 1.0, 1.0)):
+    """Band-refined, 2:1-balanced octree mesh (mf_mesh_refine_band): int32 array [n, 4] of
+    (level, ix, iy, iz) in Morton order."""
+    import numpy as np
+    if isinstance(ncell, int):
+        ncell = (ncell,) * 3
+    nc = (C.c_int * 3)(*ncell)
+    lo3, hi3, ce3 = ((C.c_double * 3)(*v) for v in (lo, hi, center))
+    n = C.c_int64(0)
+    _check(lib().mf_mesh_refine_band(nc, lo3, hi3, levels, ce3, radius, half_width, None, 0,
+                                     C.byref(n)))
+    cells = np.zeros((n.value, 4), dtype=np.int32)
+    _check(lib().mf_mesh_refine_band(nc, lo3, hi3, levels, ce3, radius, half_width,
+                                     cells.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+    return cells
+
+
+class AdaptiveScratchData(ScratchData):
+    """ScratchData on an adaptive octree mesh with hanging nodes (mf_scratch_create_adaptive).
+    The default Dirichlet rows (non-hanging boundary nodes) are set at creation."""
+
+    def __init__(self, degree, ncell, cells, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), stream=None):
+        import numpy as np
+        if isinstance(ncell, int):
+            ncell = (ncell,) * 3
+        self.adaptive = True
+        self.degree = degree
+        self.ncell = tuple(ncell)
+        self.lo, self.hi = tuple(lo), tuple(hi)
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        h = C.c_void_p()
+        _check(lib().mf_scratch_create_adaptive(degree, (C.c_int * 3)(*ncell),
+                                                (C.c_double * 3)(*lo), (C.c_double * 3)(*hi),
+                                                self.cells.ctypes.data_as(C.c_void_p),
+                                                len(self.cells), C.byref(h)))
+        self._h = h
+        self.n_dofs = int(lib().mf_scratch_n_dofs(h))
+        self.n_cells = int(lib().mf_scratch_n_cells(h))
+        if stream is not None:
+            self.set_stream(stream)
+
+    def support_points(self):
+        import numpy as np
+        xyz = np.zeros((self.n_dofs, 3))
+        _check(lib().mf_scratch_support_points(self._h, xyz.ctypes.data_as(C.c_void_p)))
+        return xyz
+
+    def constraint_info(self):
+        nh, ne, nc = C.c_int64(0), C.c_int64(0), C.c_int(0)
+        _check(lib().mf_scratch_constraint_info(self._h, C.byref(nh), C.byref(ne), C.byref(nc)))
+        return {"n_hanging": nh.value, "n_entries": ne.value, "n_colours": nc.value}
+
+    def constraint_lines(self):
+        """(node_line[n_dofs], line_ptr, line_col, line_val) as numpy arrays."""
+        import numpy as np
+        info = self.constraint_info()
+        node_line = np.zeros(self.n_dofs, dtype=np.int32)
+        ptr = np.zeros(info["n_hanging"] + 1, dtype=np.uint32)
+        col = np.zeros(max(info["n_entries"], 1), dtype=np.uint32)
+        val = np.zeros(max(info["n_entries"], 1))
+        _check(lib().mf_scratch_constraint_lines(self._h, *(a.ctypes.data_as(C.c_void_p)
+                                                           for a in (node_line, ptr, col, val))))
+        return node_line, ptr, col[:info["n_entries"]], val[:info["n_entries"]]
+
+    def distribute_constraints(self, x):
+        """x_h = sum_m w x_m on the hanging nodes of the device vector x (in place)."""
+        _check(lib().mf_distribute_constraints(self._h, _ptr(x)))
+
+    def random_field(self, seed):
+        raise NotImplementedError("lattice-keyed random fields exist on structured meshes only")
 
 
 class _OperatorBase:

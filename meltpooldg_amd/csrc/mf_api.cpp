@@ -95,6 +95,11 @@ static void lagrange_eval(int n, const double *nodes, double x, double *v, doubl
   }
 }
 
+void lagrange_basis(int n, const double *nodes, double x, double *v, double *d)
+{
+  lagrange_eval(n, nodes, x, v, d);
+}
+
 void compute_tables(int degree, Tables &t)
 {
   const int K = degree + 1;
@@ -156,6 +161,29 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.diffusivity = Number(op->advec.diffusivity);
     a.mass_factor = Number(op->advec.mass_factor);
   }
+  if (const AdaptiveData *ad = s->ad) {
+    a.cell_nodes = ad->d.cell_nodes;
+    a.cell_hmask = ad->d.cell_hmask;
+    a.cell_level = ad->d.cell_level;
+    a.cell_hidx = ad->d.cell_hidx;
+    a.node_line = ad->d.node_line;
+    a.line_ptr = ad->d.line_ptr;
+    a.line_col = ad->d.line_col;
+    a.list = ad->d.list;
+    a.ht_ptr = ad->d.ht_ptr;
+    a.tgt_node = ad->d.tgt_node;
+    a.tgt_ptr = ad->d.tgt_ptr;
+    a.ent_local = ad->d.ent_local;
+    a.slot_cell = ad->d.slot_cell;
+    a.slot_tgt = ad->d.slot_tgt;
+    if constexpr (sizeof(Number) == 8) {
+      a.line_val = reinterpret_cast<const Number *>(ad->d.line_val);
+      a.ent_w = reinterpret_cast<const Number *>(ad->d.ent_w);
+    } else {
+      a.line_val = reinterpret_cast<const Number *>(ad->d.line_val32);
+      a.ent_w = reinterpret_cast<const Number *>(ad->d.ent_w32);
+    }
+  }
   if constexpr (sizeof(Number) == 8) {
     a.T = reinterpret_cast<const Number *>(op->T);
     a.vel = reinterpret_cast<const Number *>(op->vel);
@@ -194,6 +222,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+  if (m.ad) return launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
   // the transposed operator (adjoint) runs on the pencil kernel
   if (g_tangent_variant == 1 && !op->transpose)
     return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
@@ -260,6 +289,7 @@ int mf_scratch_create(const mf_mesh_desc *desc, mf_scratch **out)
 int mf_scratch_destroy(mf_scratch *s)
 {
   if (!s) return MF_OK;
+  free_adaptive(s);
   if (s->d_mask) (void)hipFree(s->d_mask);
   delete s;
   return MF_OK;
@@ -309,6 +339,7 @@ int mf_heat_operator_create(mf_scratch *s, const mf_heat_params *p, mf_operator 
   MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
   MF_REQUIRE(p->law == MF_LAW_CONSTANT || p->law == MF_LAW_TANH, MF_ERR_ARG, "unknown law");
   MF_REQUIRE(p->law != MF_LAW_TANH || p->tanh_width > 0.0, MF_ERR_ARG, "tanh_width must be > 0");
+  if (int rc = ensure_adaptive_on_device(s)) return rc;
   auto *op = new mf_operator();
   op->kind = 0;
   op->s = s;
@@ -322,6 +353,7 @@ int mf_advection_operator_create(mf_scratch *s, const mf_advec_params *p, mf_ope
   MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
   MF_REQUIRE(p->diffusivity >= 0.0, MF_ERR_ARG,
              "Advection diffusion operator: diffusivity is smaller than zero!");
+  if (int rc = ensure_adaptive_on_device(s)) return rc;
   auto *op = new mf_operator();
   op->kind = 1;
   op->s = s;
@@ -418,7 +450,16 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   if (rc) return rc;
   CellArgs<double> a = make_args<double>(op);
   const int code = op->kind == 0 ? HEAT_DIAG : ADVEC_DIAG;
-  rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
+  if (m.ad) {
+    // cells without hanging nodes: local diagonal; cells with hanging nodes: condensed
+    // diagonal c_D . (A_cell c_D) per target node D
+    rc = launch_cell_list<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
+    if (rc) return rc;
+    const int tcode = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+    rc = launch_cell_list<double>(tcode, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
+  } else {
+    rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
+  }
   if (rc) return rc;
   double *work = nullptr;
   if (op->kind == 1) MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&work), sizeof(double) * RED_BLOCKS, st));

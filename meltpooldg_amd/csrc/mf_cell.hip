@@ -150,8 +150,10 @@ __host__ __device__ inline RowColour row_colour(const int ncell[3], int colour)
 
 // One colour launch of the cell kernel.
 struct LaunchPart {
-  int colour;    // structured meshes: row colour 0..3
-  int part_off;  // offset of this launch's blocks in dot_partial
+  int colour;     // structured meshes: row colour 0..3
+  int part_off;   // offset of this launch's blocks in dot_partial
+  int64_t start;  // adaptive meshes: first list entry (cell or slot) of the launch
+  int64_t count;  // number of list entries
 };
 
 // Node access of the structured mesh.  A workgroup walks one row of cells along x in steps of
@@ -213,7 +215,7 @@ struct GridAccess {
   // at the end of the row, node K-1.  Constrained rows are skipped
   // (distribute_local_to_global semantics).
   __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
-                          Number *dst)
+                          Number *dst, Number *)
   {
     if (!active) {
 #pragma unroll
@@ -230,6 +232,154 @@ struct GridAccess {
       if (i == K - 1 && !last) break;
       if (masked(A, gbase + i)) continue;
       dst[gbase + i] = d[i] + o[i];
+    }
+  }
+};
+
+// Node access of an adaptive mesh: a workgroup serves CPB cells of one colour from the cell
+// list (cells of one colour share no written node).  Local node e = i + K*p of a cell reads
+// its global node from the cell's node table; a hanging node reads the interpolation of its
+// masters (read_dof_values with hanging-node constraints).
+template <typename Number, int K, int CPB>
+struct ListAccess {
+  static constexpr int NLOC = K * K * K;
+  const uint32_t *nodes;  // the K nodes of the thread's x pencil
+  uint32_t hbits;         // hanging flags of these K nodes
+  int c, p, cell, level;
+  bool active, hanging_cell;
+
+  __device__ ListAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p_)
+  {
+    c = c_;
+    p = p_;
+    const int64_t slot = int64_t(blockIdx.x) * CPB + c;
+    active = slot < lp.count;
+    cell = active ? int(A.list[lp.start + slot]) : 0;
+    nodes = A.cell_nodes + size_t(cell) * NLOC + K * p;
+    const uint64_t hm = active ? A.cell_hmask[cell] : 0ull;
+    hanging_cell = hm != 0ull;
+    hbits = uint32_t(hm >> (K * p)) & ((1u << K) - 1u);
+    level = active ? int(A.cell_level[cell]) : 0;
+  }
+  __device__ int iterations() const { return 1; }
+  __device__ void at(int) {}
+  __device__ Number hscale() const { return ldexp(Number(1), -level); }
+  __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
+  {
+    return A.use_mask && A.mask && A.mask[n];
+  }
+  __device__ Number plain(const Number *f, int i) const { return active ? f[nodes[i]] : Number(0); }
+  __device__ Number value(const CellArgs<Number> &A, const Number *f, int i) const
+  {
+    if (!active) return Number(0);
+    const uint32_t n = nodes[i];
+    if ((hbits >> i) & 1u) {
+      const int32_t l = A.node_line[n];
+      Number v = 0;
+      for (uint32_t t = A.line_ptr[l]; t < A.line_ptr[l + 1]; ++t) {
+        const uint32_t mn = A.line_col[t];
+        if (!masked(A, mn)) v += A.line_val[t] * f[mn];
+      }
+      return v;
+    }
+    return masked(A, n) ? Number(0) : f[n];
+  }
+  __device__ void prefetch(const Number *dst, Number (&d)[K]) const
+  {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = (active && !hanging_cell) ? dst[nodes[i]] : Number(0);
+  }
+  // Cells without hanging nodes store their nodes directly.  A cell with hanging nodes stages
+  // its outputs in LDS; each of its threads then sums the entries of some of the cell's
+  // targets (the distinct nodes it writes: free local nodes and masters), so every target is
+  // written by one thread (distribute_local_to_global with constraints).
+  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
+                          Number *dst, Number *stage)
+  {
+    __syncthreads();  // the stage may still be read by other cells' threads
+    store_pencil<Number, K, CPB, 0>(stage, p, c, o);
+    __syncthreads();
+    if (!active) return;
+    if (!hanging_cell) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const uint32_t n = nodes[i];
+        if (!masked(A, n)) dst[n] = d[i] + o[i];
+      }
+      return;
+    }
+    const int32_t hi = A.cell_hidx[cell];
+    for (uint32_t t = A.ht_ptr[hi] + uint32_t(p); t < A.ht_ptr[hi + 1]; t += uint32_t(K * K)) {
+      const uint32_t n = A.tgt_node[t];
+      if (masked(A, n)) continue;
+      Number v = 0;
+      for (uint32_t q = A.tgt_ptr[t]; q < A.tgt_ptr[t + 1]; ++q)
+        v += A.ent_w[q] * stage[int(A.ent_local[q]) * CPB + c];
+      dst[n] += v;
+    }
+  }
+};
+
+// Condensed diagonal on cells with hanging nodes: a slot is (cell, target node D).  The local
+// source is the coefficient vector c_D of D in the cell's constrained basis (1 on the free
+// local node D, the weight of D on each hanging local node with master D); the slot adds
+// c_D . (A_cell c_D) to diag[D] (MatrixFreeTools::compute_diagonal with constraints).
+template <typename Number, int K, int CPB>
+struct SlotAccess {
+  static constexpr int NLOC = K * K * K;
+  const uint32_t *nodes;
+  uint32_t tgt;
+  int c, p, cell, level;
+  bool active;
+  Number cv[K];
+
+  __device__ SlotAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p_)
+  {
+    c = c_;
+    p = p_;
+    const int64_t slot = int64_t(blockIdx.x) * CPB + c;
+    active = slot < lp.count;
+    cell = active ? int(A.slot_cell[lp.start + slot]) : 0;
+    tgt = active ? A.slot_tgt[lp.start + slot] : 0u;
+    nodes = A.cell_nodes + size_t(cell) * NLOC + K * p;
+    level = active ? int(A.cell_level[cell]) : 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) cv[i] = Number(0);
+    if (active) {
+      for (uint32_t q = A.tgt_ptr[tgt]; q < A.tgt_ptr[tgt + 1]; ++q) {
+        const int e = int(A.ent_local[q]);
+        if (e / K == p) {
+#pragma unroll
+          for (int i = 0; i < K; ++i)
+            if (e % K == i) cv[i] += A.ent_w[q];
+        }
+      }
+    }
+  }
+  __device__ int iterations() const { return 1; }
+  __device__ void at(int) {}
+  __device__ Number hscale() const { return ldexp(Number(1), -level); }
+  __device__ Number plain(const Number *f, int i) const { return active ? f[nodes[i]] : Number(0); }
+  __device__ Number value(const CellArgs<Number> &, const Number *, int i) const { return cv[i]; }
+  __device__ void prefetch(const Number *, Number (&d)[K]) const
+  {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = Number(0);
+  }
+  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&)[K], int,
+                          Number *dst, Number *stage)
+  {
+    Number part = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) part += cv[i] * o[i];
+    __syncthreads();
+    stage[p * CPB + c] = part;
+    __syncthreads();
+    if (p == 0 && active) {
+      Number sum = 0;
+      for (int j = 0; j < K * K; ++j) sum += stage[j * CPB + c];
+      const uint32_t n = A.tgt_node[tgt];
+      if (!(A.use_mask && A.mask && A.mask[n])) dst[n] += sum;
     }
   }
 };
@@ -580,7 +730,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
           }
         }
       }
-    ac.scatter(A, diag, dpre, c, dst);
+    ac.scatter(A, diag, dpre, c, dst, L0);
   } else {
     // r = val + Dq_z^T fz (z direction in registers)
     Number tz[K];
@@ -629,7 +779,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       for (int i = 0; i < K; ++i) local_dot += u[i] * o[i];
       dot_acc += double(local_dot);
     }
-    ac.scatter(A, o, dpre, c, dst);
+    ac.scatter(A, o, dpre, c, dst, L0);
   }
   }  // steps along the row
 
@@ -648,6 +798,13 @@ static int row_blocks(const MeshInfo &m, int colour)
 
 int cell_blocks(const MeshInfo &m)
 {
+  if (m.ad) {
+    constexpr int CPBS[5] = {0, 0, cells_per_block<2>(), cells_per_block<3>(), cells_per_block<4>()};
+    int total = 0;
+    for (int col = 0; col < m.ad->n_colours; ++col)
+      total += int((m.ad->colour_start[col + 1] - m.ad->colour_start[col] + CPBS[m.K] - 1) / CPBS[m.K]);
+    return total;
+  }
   int total = 0;
   for (int colour = 0; colour < 4; ++colour) total += row_blocks(m, colour);
   return total;
@@ -662,7 +819,7 @@ static int launch_colours(const MeshInfo &m, const CellArgs<Number> &a, const Nu
   for (int colour = 0; colour < 4; ++colour) {
     const int blocks = row_blocks(m, colour);
     if (blocks == 0) continue;
-    const LaunchPart lp{colour, part_off};
+    const LaunchPart lp{colour, part_off, 0, 0};
     hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL, DOT, GridAccess<Number, K, CPB>>),
                        dim3(blocks), dim3(CPB * K * K), 0, st, a, src, dst, lp);
     MF_HIP_TRY(hipGetLastError());
@@ -680,6 +837,129 @@ static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *
   }
   return launch_colours<Number, K, OP, NEEDT, VEL, false>(m, a, src, dst, st);
 }
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT, class Access>
+static int launch_parts(const std::vector<int64_t> &start, const std::vector<int64_t> &end,
+                        const CellArgs<Number> &a, const Number *src, Number *dst,
+                        hipStream_t st)
+{
+  constexpr int CPB = cells_per_block<K>();
+  int part_off = 0;
+  for (size_t col = 0; col < start.size(); ++col) {
+    const int64_t count = end[col] - start[col];
+    if (count <= 0) continue;
+    const int blocks = int((count + CPB - 1) / CPB);
+    const LaunchPart lp{int(col), part_off, start[col], count};
+    hipLaunchKernelGGL((cell_kernel<Number, K, OP, NEEDT, VEL, DOT, Access>), dim3(blocks),
+                       dim3(CPB * K * K), 0, st, a, src, dst, lp);
+    MF_HIP_TRY(hipGetLastError());
+    part_off += blocks;
+  }
+  return MF_OK;
+}
+
+// colour ranges of an adaptive launch: mode 0 all cells, 1 cells without hanging nodes,
+// 2 condensed-diagonal slots
+static void list_ranges(const AdaptiveData &ad, int mode, std::vector<int64_t> &start,
+                        std::vector<int64_t> &end)
+{
+  const int nc = ad.n_colours;
+  start.resize(nc);
+  end.resize(nc);
+  for (int col = 0; col < nc; ++col) {
+    if (mode == 2) {
+      start[col] = ad.slot_start[col];
+      end[col] = ad.slot_start[col + 1];
+    } else {
+      start[col] = ad.colour_start[col];
+      end[col] = mode == 1 ? ad.colour_hang[col] : ad.colour_start[col + 1];
+    }
+  }
+}
+
+template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+static int launch_list_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
+                         Number *dst, hipStream_t st, int mode)
+{
+  constexpr int CPB = cells_per_block<K>();
+  std::vector<int64_t> start, end;
+  list_ranges(*m.ad, mode, start, end);
+  if (mode == 2) {
+    if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT)
+      return launch_parts<Number, K, OP, NEEDT, VEL, false, SlotAccess<Number, K, CPB>>(
+          start, end, a, src, dst, st);
+    return MF_ERR_ARG;
+  }
+  if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT) {
+    if (a.dot_partial)
+      return launch_parts<Number, K, OP, NEEDT, VEL, true, ListAccess<Number, K, CPB>>(
+          start, end, a, src, dst, st);
+  }
+  return launch_parts<Number, K, OP, NEEDT, VEL, false, ListAccess<Number, K, CPB>>(
+      start, end, a, src, dst, st);
+}
+
+template <typename Number, int K, int OP>
+static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                          const Number *src, Number *dst, hipStream_t st, int mode)
+{
+  if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_DIAG) {
+    return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
+  } else {
+    if (needT) {
+      if (vel) return launch_list_k<Number, K, OP, true, true>(m, a, src, dst, st, mode);
+      return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
+    }
+    if (vel) return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
+    return launch_list_k<Number, K, OP, false, false>(m, a, src, dst, st, mode);
+  }
+}
+
+template <typename Number, int K>
+static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
+                           const CellArgs<Number> &a, const Number *src, Number *dst,
+                           hipStream_t st, int mode)
+{
+  switch (op) {
+  case HEAT_TANGENT:
+    return launch_list_op<Number, K, HEAT_TANGENT>(needT, vel, m, a, src, dst, st, mode);
+  case ADVEC_TANGENT:
+    return launch_list_op<Number, K, ADVEC_TANGENT>(needT, vel, m, a, src, dst, st, mode);
+  case HEAT_DIAG:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, HEAT_DIAG>(needT, vel, m, a, src, dst, st, mode);
+    break;
+  case ADVEC_DIAG:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st, mode);
+    break;
+  default:
+    break;
+  }
+  set_error("operation not available on adaptive meshes");
+  return MF_ERR_ARG;
+}
+
+template <typename Number>
+int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                     const Number *src, Number *dst, hipStream_t st, int mode)
+{
+  if (!m.ad) {
+    set_error("not an adaptive mesh");
+    return MF_ERR_ARG;
+  }
+  switch (m.K) {
+  case 2: return launch_list_deg<Number, 2>(op, needT, vel, m, a, src, dst, st, mode);
+  case 3: return launch_list_deg<Number, 3>(op, needT, vel, m, a, src, dst, st, mode);
+  case 4: return launch_list_deg<Number, 4>(op, needT, vel, m, a, src, dst, st, mode);
+  default: set_error("degree must be 1, 2 or 3"); return MF_ERR_ARG;
+  }
+}
+
+template int launch_cell_list<double>(int, bool, bool, const MeshInfo &, const CellArgs<double> &,
+                                      const double *, double *, hipStream_t, int);
+template int launch_cell_list<float>(int, bool, bool, const MeshInfo &, const CellArgs<float> &,
+                                     const float *, float *, hipStream_t, int);
 
 template <typename Number, int K, int OP>
 static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
@@ -717,6 +997,10 @@ template <typename Number>
 int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t st)
 {
+  if (m.ad) {
+    set_error("operation not available on adaptive meshes (structured-mesh kernel)");
+    return MF_ERR_ARG;
+  }
   switch (m.K) {
   case 2: return launch_deg<Number, 2>(op, needT, vel, m, a, src, dst, st);
   case 3: return launch_deg<Number, 3>(op, needT, vel, m, a, src, dst, st);

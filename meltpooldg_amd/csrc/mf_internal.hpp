@@ -9,6 +9,8 @@
 
 #include "../../include/meltpooldg_amd.h"
 
+struct mf_scratch;
+
 namespace mf {
 
 void set_error(const std::string &msg);
@@ -73,9 +75,29 @@ struct CellArgs {
   double pgrad_scale;      // HEAT_PGRAD: dst[cell] += pgrad_scale * sum_q JxW grad src . grad T
   double *dot_partial;     // fused src . (A src) per block (nullable)
   const int *stop;         // early-exit flag set by the solver (nullable)
+  // adaptive meshes (cell lists, hanging-node constraints); null on structured meshes
+  const uint32_t *cell_nodes;  // [n_cells][K^3] global node of each local node
+  const uint64_t *cell_hmask;  // [n_cells] bit e: local node e is a hanging node
+  const uint8_t *cell_level;   // [n_cells] refinement level: h = h_base / 2^level
+  const int32_t *cell_hidx;    // [n_cells] index into the hanging-cell target table (-1: none)
+  const int32_t *node_line;    // [n_dofs] constraint line of a hanging node (-1: free)
+  const uint32_t *line_ptr;    // [n_lines + 1]
+  const uint32_t *line_col;    // masters (free nodes)
+  const Number *line_val;      // weights
+  const uint32_t *list;        // cells in colour order
+  const uint32_t *ht_ptr;      // [n_hanging_cells + 1] targets of each hanging cell
+  const uint32_t *tgt_node;    // node of each target
+  const uint32_t *tgt_ptr;     // [n_targets + 1] entries of each target
+  const uint8_t *ent_local;    // local node of each entry
+  const Number *ent_w;         // weight of each entry
+  const uint32_t *slot_cell;   // condensed diagonal: (cell, target) per slot
+  const uint32_t *slot_tgt;
 };
 
+struct AdaptiveData;
+
 struct MeshInfo {
+  const AdaptiveData *ad = nullptr;  // adaptive mesh tables (null: structured mesh)
   int degree = 0, K = 0;
   int ncell[3] = {0, 0, 0};
   int64_t NN[3] = {0, 0, 0};
@@ -88,11 +110,65 @@ struct Tables {
 };
 
 void compute_tables(int degree, Tables &t);
+// Lagrange polynomials on `nodes` (n of them): values v and derivatives d at x
+void lagrange_basis(int n, const double *nodes, double x, double *v, double *d);
+
+// Adaptive mesh: host tables and their device copies (mf_adaptive.cpp)
+struct AdaptiveData {
+  int levels = 0, n_colours = 0;
+  int64_t n_hanging = 0;
+  bool uploaded = false;
+  std::vector<int32_t> cells;          // [n_cells][4] level, ix, iy, iz
+  std::vector<double> xyz;             // [n_dofs][3]
+  std::vector<uint8_t> boundary;       // [n_dofs]
+  std::vector<uint32_t> cell_nodes;    // [n_cells][K^3]
+  std::vector<uint64_t> cell_hmask;
+  std::vector<uint8_t> cell_level;
+  std::vector<int32_t> cell_hidx;
+  std::vector<int32_t> node_line;
+  std::vector<uint32_t> line_ptr, line_col;
+  std::vector<double> line_val;
+  std::vector<uint32_t> list;          // cells by colour, non-hanging cells first
+  std::vector<int64_t> colour_start;   // [n_colours + 1] offsets into list
+  std::vector<int64_t> colour_hang;    // [n_colours] first hanging cell of each colour
+  std::vector<uint32_t> ht_ptr, tgt_node, tgt_ptr;
+  std::vector<uint8_t> ent_local;
+  std::vector<double> ent_w;
+  std::vector<uint32_t> slot_cell, slot_tgt;
+  std::vector<int64_t> slot_start;     // [n_colours + 1]
+  struct {
+    uint32_t *cell_nodes = nullptr;
+    uint64_t *cell_hmask = nullptr;
+    uint8_t *cell_level = nullptr;
+    int32_t *cell_hidx = nullptr, *node_line = nullptr;
+    uint32_t *line_ptr = nullptr, *line_col = nullptr;
+    double *line_val = nullptr;
+    float *line_val32 = nullptr;
+    uint32_t *list = nullptr, *ht_ptr = nullptr, *tgt_node = nullptr, *tgt_ptr = nullptr;
+    uint8_t *ent_local = nullptr;
+    double *ent_w = nullptr;
+    float *ent_w32 = nullptr;
+    uint32_t *slot_cell = nullptr, *slot_tgt = nullptr;
+  } d;
+};
+
+int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
+int upload_adaptive(::mf_scratch *s);
+int ensure_adaptive_on_device(::mf_scratch *s);
+// x[h] = sum_m w x[m] on every hanging node h (AffineConstraints::distribute), mf_ops.hip
+int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
+                      const double *line_val, double *x, int64_t n, hipStream_t st);
+void free_adaptive(::mf_scratch *s);
 
 // launches (mf_cell.hip)
 template <typename Number>
 int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t stream);
+// adaptive meshes: mode 0 = all cells (tangent ops), 1 = cells without hanging nodes (diagonal
+// ops), 2 = condensed diagonal of the cells with hanging nodes (op = a tangent op)
+template <typename Number>
+int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+                     const Number *src, Number *dst, hipStream_t stream, int mode);
 
 // number of blocks the cell kernel uses (size of dot_partial)
 int cell_blocks(const MeshInfo &m);
@@ -107,7 +183,7 @@ int slab_blocks(const MeshInfo &m);
 extern int g_tangent_variant;
 inline int tangent_blocks(const MeshInfo &m, bool transpose = false)
 {
-  return (g_tangent_variant == 1 && !transpose) ? slab_blocks(m) : cell_blocks(m);
+  return (g_tangent_variant == 1 && !transpose && !m.ad) ? slab_blocks(m) : cell_blocks(m);
 }
 
 }  // namespace mf
@@ -118,6 +194,7 @@ struct mf_scratch {
   hipStream_t stream = nullptr;
   uint8_t *d_mask = nullptr;
   std::vector<uint8_t> h_mask;
+  mf::AdaptiveData *ad = nullptr;  // adaptive mesh (null: structured)
 };
 
 struct mf_operator {

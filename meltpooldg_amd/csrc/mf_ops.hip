@@ -180,6 +180,37 @@ using namespace mf;
 // ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
+namespace mf {
+
+// x[h] = sum_m w x[m] for every hanging node h; masters are free nodes, so the order of the
+// updates does not matter
+__global__ void distribute_kernel(const int32_t *__restrict__ node_line,
+                                  const uint32_t *__restrict__ line_ptr,
+                                  const uint32_t *__restrict__ line_col,
+                                  const double *__restrict__ line_val, double *x, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t l = node_line[i];
+    if (l < 0) continue;
+    double v = 0.0;
+    for (uint32_t t = line_ptr[l]; t < line_ptr[l + 1]; ++t) v += line_val[t] * x[line_col[t]];
+    x[i] = v;
+  }
+}
+
+int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
+                      const double *line_val, double *x, int64_t n, hipStream_t st)
+{
+  const int blocks = int(std::min<int64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(distribute_kernel, dim3(blocks), dim3(256), 0, st, node_line, line_ptr,
+                     line_col, line_val, x, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+}  // namespace mf
+
 extern "C" {
 
 int mf_operator_vmult(mf_operator *op, double *dst, const double *src);

@@ -1,0 +1,164 @@
+"""Device operators on adaptive meshes with hanging nodes against the oracle (same cells, nodes
+mapped by coordinates): FP64 vmult within 1e-12, FP32 vmult within FP32 rounding, the condensed
+inverse diagonal, constraint distribution, CG iteration counts within +-1 and solutions within
+1e-10, and the FP32-inner/FP64-refined solve reaching the FP64 tolerance."""
+import numpy as np
+import pytest
+
+import orc
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "q1": (1, 2, 2, (0.1, 0.1, 0.1), 0.2, 0.03),
+    "q2": (2, 3, 2, (0.2, 0.3, 0.25), 0.3, 0.04),
+    "q3": (3, 2, 2, (0.1, 0.1, 0.1), 0.2, 0.03),
+    "q3b": (3, 4, 1, (0.5, 0.5, 0.5), 0.3, 0.05),
+}
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+class Pair:
+    """The same adaptive case on the device and in the oracle."""
+
+    def __init__(self, name, seed=3):
+        import meltpooldg_amd as mf
+        degree, nc, lev, ce, r, hw = CASES[name]
+        self.mf = mf
+        self.cells = mf.refine_band_cells(nc, lev, ce, r, hw)
+        self.s = mf.AdaptiveScratchData(degree, nc, self.cells,
+                                        stream=torch.cuda.current_stream())
+        self.am = orc.AdaptiveMesh(degree, nc, self.cells)
+        xs = self.s.support_points()
+        pos = {tuple(np.round(x * 2 ** 24).astype(np.int64)): i for i, x in enumerate(self.am.xyz)}
+        self.perm = np.array([pos[tuple(np.round(x * 2 ** 24).astype(np.int64))] for x in xs])
+        self.n = self.s.n_dofs
+        self.rng = np.random.default_rng(seed)
+
+    def to_dev_order(self, a_oracle):
+        return np.asarray(a_oracle)[self.perm]
+
+    def field(self, scale=1.0, offset=0.0):
+        return offset + scale * self.rng.uniform(-1, 1, self.n)   # oracle order
+
+
+def _heat_pair(P, law="tanh", vel=False, kcell=False, dt=1e-3):
+    mf = P.mf
+    T = P.field(150.0, 1700.0) if law == "tanh" else None
+    velo = P.rng.uniform(-1, 1, (3, P.n)) if vel else None
+    kc = np.exp(0.3 * P.rng.standard_normal(len(P.cells))) if kcell else None
+    hp = orc.HeatParams(law=orc.LAW_TANH if law == "tanh" else orc.LAW_CONSTANT,
+                        t_dependent=int(law == "tanh"), rho_cp=1.3, k=0.7, tanh_amp=0.5,
+                        tanh_Tc=1700.0, tanh_width=50.0, dt=dt)
+    ref = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, T=T, T_old=T, k_cell=kc,
+                             vel_kind=orc.VEL_NODAL if vel else orc.VEL_NONE,
+                             vel=velo.ravel() if vel else None, mask=P.am.dirichlet, amesh=P.am)
+    op = mf.HeatDiffuseOperator(P.s, law=mf.LAW_TANH if law == "tanh" else mf.LAW_CONSTANT,
+                                rho_cp=1.3, k=0.7, tanh_amp=0.5, tanh_Tc=1700.0, tanh_width=50.0,
+                                t_dependent=(law == "tanh"), dt=dt)
+    keep = {}
+    keep["T"] = _dev(P.to_dev_order(T)) if T is not None else None
+    keep["vel"] = _dev(np.stack([P.to_dev_order(v) for v in velo]).ravel()) if vel else None
+    keep["kc"] = _dev(kc) if kcell else None
+    op.set_fields(T=keep["T"], velocity=keep["vel"], k_cell=keep["kc"])
+    op._keep = keep
+    return ref, op
+
+
+@pytest.mark.parametrize("name", ["q1", "q2", "q3", "q3b"])
+@pytest.mark.parametrize("law,vel,kcell", [("const", False, False), ("tanh", False, True),
+                                           ("tanh", True, False)])
+def test_vmult_fp64_and_fp32(name, law, vel, kcell):
+    P = Pair(name)
+    ref, op = _heat_pair(P, law, vel, kcell)
+    u = P.field()
+    y_ref = ref.vmult(u)
+    x = _dev(P.to_dev_order(u))
+    y = torch.zeros_like(x)
+    op.vmult(y, x)
+    torch.cuda.synchronize()
+    yd = y.cpu().numpy()
+    scale = np.abs(y_ref).max()
+    assert np.abs(yd - P.to_dev_order(y_ref)).max() <= 1e-12 * scale
+    # hanging and Dirichlet rows stay zero
+    node_line = P.s.constraint_lines()[0]
+    assert np.all(yd[node_line >= 0] == 0.0)
+    x32, y32 = x.float(), torch.zeros(P.n, dtype=torch.float32, device="cuda")
+    op.vmult_f32(y32, x32)
+    torch.cuda.synchronize()
+    assert np.abs(y32.double().cpu().numpy() - P.to_dev_order(y_ref)).max() <= 2e-5 * scale
+
+
+@pytest.mark.parametrize("name", ["q1", "q2", "q3"])
+@pytest.mark.parametrize("law,vel", [("const", False), ("tanh", True)])
+def test_inverse_diagonal(name, law, vel):
+    P = Pair(name)
+    ref, op = _heat_pair(P, law, vel, kcell=True)
+    d_ref = ref.inverse_diagonal()
+    d = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    op.compute_inverse_diagonal_from_matrixfree(d)
+    torch.cuda.synchronize()
+    assert np.abs(d.cpu().numpy() - P.to_dev_order(d_ref)).max() <= 1e-12 * np.abs(d_ref).max()
+
+
+def test_advection_operator_on_adaptive_mesh():
+    P = Pair("q2")
+    mf = P.mf
+    velo = P.rng.uniform(-1, 1, (3, P.n))
+    ap = orc.AdvecParams(dt=0.05, theta=0.5, diffusivity=0.02, mass_factor=1.5, vel_scale=0.0)
+    ref = orc.OracleOperator(orc.OP_ADVEC, None, advec=ap, vel_kind=orc.VEL_NODAL,
+                             vel=velo.ravel(), mask=P.am.dirichlet, amesh=P.am)
+    op = mf.AdvectionDiffusionOperator(P.s, theta=0.5, diffusivity=0.02, mass_factor=1.5,
+                                       dt=0.05)
+    vdev = _dev(np.stack([P.to_dev_order(v) for v in velo]).ravel())
+    op.set_fields(velocity=vdev)
+    u = P.field()
+    y = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    op.vmult(y, _dev(P.to_dev_order(u)))
+    d = torch.zeros_like(y)
+    op.compute_inverse_diagonal_from_matrixfree(d)
+    torch.cuda.synchronize()
+    y_ref = ref.vmult(u)
+    assert np.abs(y.cpu().numpy() - P.to_dev_order(y_ref)).max() <= 1e-12 * np.abs(y_ref).max()
+    d_ref = ref.inverse_diagonal()
+    assert np.abs(d.cpu().numpy() - P.to_dev_order(d_ref)).max() <= 1e-12 * np.abs(d_ref).max()
+
+
+def test_distribute_constraints():
+    P = Pair("q3")
+    u = P.field()
+    x = _dev(P.to_dev_order(u))
+    P.s.distribute_constraints(x)
+    torch.cuda.synchronize()
+    assert np.abs(x.cpu().numpy() - P.to_dev_order(P.am.distribute(u))).max() < 1e-14
+
+
+@pytest.mark.parametrize("name", ["q2", "q3"])
+@pytest.mark.parametrize("law,solver", [("const", "CG"), ("tanh", "GMRES")])
+def test_solve_and_refined_solve(name, law, solver):
+    """CG on the symmetric (constant-law) Jacobian, GMRES on the non-symmetric tanh Jacobian."""
+    P = Pair(name)
+    mf = P.mf
+    ref, op = _heat_pair(P, law, False, kcell=True, dt=1e-3)
+    b = P.field()
+    b[(P.am.dirichlet == 1) | (P.am.hanging == 1)] = 0.0
+    x_ref, it_ref, _, st = ref.solve(b, solver=orc.CG if solver == "CG" else orc.GMRES,
+                                     rel_tol=1e-10, inv_diag=ref.inverse_diagonal())
+    assert st == orc.SUCCESS
+    bd = _dev(P.to_dev_order(b))
+    x = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    rep = mf.LinearSolver.solve(op, x, bd, solver=solver, rel_tolerance=1e-10)
+    assert abs(rep["iterations"] - it_ref) <= 1
+    xr = P.to_dev_order(x_ref)
+    free = P.to_dev_order((P.am.dirichlet == 0) & (P.am.hanging == 0))
+    assert np.abs(x.cpu().numpy() - xr)[free].max() <= 1e-9 * np.abs(xr).max()
+    x32 = torch.zeros_like(x)
+    rep32 = mf.LinearSolver.solve(op, x32, bd, solver=solver, rel_tolerance=1e-10,
+                                  precision="fp32_refined", inner_rel_tolerance=1e-3)
+    assert rep32["status"] == 0 and rep32["outer_iterations"] >= 2
+    assert rep32["final_residual"] <= 1e-10 * rep32["initial_residual"]
+    assert np.abs(x32.cpu().numpy() - xr)[free].max() <= 1e-8 * np.abs(xr).max()
