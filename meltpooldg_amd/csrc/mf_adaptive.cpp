@@ -351,28 +351,23 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   // default Dirichlet rows: boundary nodes that are not hanging
   s->h_mask.assign(size_t(nn), 0);
   for (int64_t i = 0; i < nn; ++i) s->h_mask[i] = (A->boundary[i] && line_of[i] < 0) ? 1 : 0;
-  // --- per-cell hanging masks, written sets, colouring -------------------------------------
+  // --- per-cell hanging masks and colouring ------------------------------------------------
+  // A cell writes its own K^3 nodes (hanging ones included); the hanging rows are moved to
+  // their masters afterwards by a separate gather over the masters.  So cells conflict only
+  // when they share a node, as on a conforming mesh.
   A->cell_hmask.assign(size_t(n_cells), 0);
   A->cell_level.resize(size_t(n_cells));
   std::vector<uint64_t> used(static_cast<size_t>(nn), 0ull);
   std::vector<int> colour(static_cast<size_t>(n_cells));
   int n_col = 0;
-  std::vector<uint32_t> written;
   for (int64_t c = 0; c < n_cells; ++c) {
     A->cell_level[c] = uint8_t(cells[4 * c]);
-    written.clear();
+    uint64_t forbidden = 0;
     for (int e = 0; e < nloc; ++e) {
       const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
-      if (line_of[id] >= 0) {
-        A->cell_hmask[c] |= 1ull << e;
-        for (uint32_t t = A->line_ptr[line_of[id]]; t < A->line_ptr[line_of[id] + 1]; ++t)
-          written.push_back(A->line_col[t]);
-      } else {
-        written.push_back(id);
-      }
+      if (line_of[id] >= 0) A->cell_hmask[c] |= 1ull << e;
+      forbidden |= used[id];
     }
-    uint64_t forbidden = 0;
-    for (uint32_t id : written) forbidden |= used[id];
     if (~forbidden == 0) {
       set_error("adaptive mesh: more than 64 cell colours");
       return MF_ERR_ARG;
@@ -381,7 +376,37 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
     while ((forbidden >> col) & 1ull) ++col;
     colour[c] = col;
     n_col = std::max(n_col, col + 1);
-    for (uint32_t id : written) used[id] |= 1ull << col;
+    for (int e = 0; e < nloc; ++e) used[A->cell_nodes[size_t(c) * nloc + e]] |= 1ull << col;
+  }
+  // transposed constraints: for every master, the hanging nodes (and weights) it receives from
+  A->hang_nodes.clear();
+  for (int64_t i = 0; i < nn; ++i)
+    if (line_of[i] >= 0) A->hang_nodes.push_back(uint32_t(i));
+  {
+    std::vector<std::pair<uint32_t, std::pair<uint32_t, double>>> tr;  // master, (hanging, w)
+    tr.reserve(A->line_col.size());
+    for (uint32_t h : A->hang_nodes) {
+      const int32_t l = line_of[h];
+      for (uint32_t t = A->line_ptr[l]; t < A->line_ptr[l + 1]; ++t)
+        tr.push_back({A->line_col[t], {h, A->line_val[t]}});
+    }
+    std::stable_sort(tr.begin(), tr.end(),
+                     [](const auto &a, const auto &b) { return a.first < b.first; });
+    A->cm_node.clear();
+    A->cm_ptr.assign(1, 0u);
+    A->cm_h.clear();
+    A->cm_w.clear();
+    for (size_t a = 0; a < tr.size();) {
+      size_t b = a;
+      A->cm_node.push_back(tr[a].first);
+      while (b < tr.size() && tr[b].first == tr[a].first) {
+        A->cm_h.push_back(tr[b].second.first);
+        A->cm_w.push_back(tr[b].second.second);
+        ++b;
+      }
+      A->cm_ptr.push_back(uint32_t(A->cm_h.size()));
+      a = b;
+    }
   }
   // cell lists: by colour, non-hanging cells first
   A->colour_start.assign(n_col + 1, 0);
@@ -430,20 +455,17 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
     }
     A->ht_ptr.push_back(uint32_t(A->tgt_node.size()));
   }
-  // slots of the condensed diagonal: (hanging cell, target), grouped by colour
-  A->slot_start.assign(n_col + 1, 0);
-  for (int col = 0; col < n_col; ++col) {
-    A->slot_start[col] = int64_t(A->slot_cell.size());
-    for (int64_t i = A->colour_hang[col]; i < A->colour_start[col + 1]; ++i) {
-      const uint32_t c = A->list[i];
-      const int32_t hi = A->cell_hidx[c];
-      for (uint32_t t = A->ht_ptr[hi]; t < A->ht_ptr[hi + 1]; ++t) {
-        A->slot_cell.push_back(c);
-        A->slot_tgt.push_back(t);
-      }
+  // slots of the condensed diagonal: (hanging cell, target); added with atomics
+  A->slot_start.assign(2, 0);
+  for (int64_t c = 0; c < n_cells; ++c) {
+    const int32_t hi = A->cell_hidx[c];
+    if (hi < 0) continue;
+    for (uint32_t t = A->ht_ptr[hi]; t < A->ht_ptr[hi + 1]; ++t) {
+      A->slot_cell.push_back(uint32_t(c));
+      A->slot_tgt.push_back(t);
     }
   }
-  A->slot_start[n_col] = int64_t(A->slot_cell.size());
+  A->slot_start[1] = int64_t(A->slot_cell.size());
   m.ad = A;
   return MF_OK;  // device tables are uploaded when the first operator is created
 }
@@ -474,6 +496,7 @@ int upload_adaptive(mf_scratch *s)
   int rc;
   std::vector<float> lv32(A->line_val.begin(), A->line_val.end());
   std::vector<float> ew32(A->ent_w.begin(), A->ent_w.end());
+  std::vector<float> cw32(A->cm_w.begin(), A->cm_w.end());
   if ((rc = upload(&A->d.cell_nodes, A->cell_nodes)) || (rc = upload(&A->d.cell_hmask, A->cell_hmask)) ||
       (rc = upload(&A->d.cell_level, A->cell_level)) || (rc = upload(&A->d.cell_hidx, A->cell_hidx)) ||
       (rc = upload(&A->d.node_line, A->node_line)) || (rc = upload(&A->d.line_ptr, A->line_ptr)) ||
@@ -482,7 +505,10 @@ int upload_adaptive(mf_scratch *s)
       (rc = upload(&A->d.ht_ptr, A->ht_ptr)) || (rc = upload(&A->d.tgt_node, A->tgt_node)) ||
       (rc = upload(&A->d.tgt_ptr, A->tgt_ptr)) || (rc = upload(&A->d.ent_local, A->ent_local)) ||
       (rc = upload(&A->d.ent_w, A->ent_w)) || (rc = upload(&A->d.ent_w32, ew32)) ||
-      (rc = upload(&A->d.slot_cell, A->slot_cell)) || (rc = upload(&A->d.slot_tgt, A->slot_tgt)))
+      (rc = upload(&A->d.slot_cell, A->slot_cell)) || (rc = upload(&A->d.slot_tgt, A->slot_tgt)) ||
+      (rc = upload(&A->d.hang_nodes, A->hang_nodes)) || (rc = upload(&A->d.cm_node, A->cm_node)) ||
+      (rc = upload(&A->d.cm_ptr, A->cm_ptr)) || (rc = upload(&A->d.cm_h, A->cm_h)) ||
+      (rc = upload(&A->d.cm_w, A->cm_w)) || (rc = upload(&A->d.cm_w32, cw32)))
     return rc;
   const size_t n = size_t(s->mesh.n_dofs);
   if (!s->d_mask && !s->h_mask.empty()) {
@@ -500,7 +526,8 @@ void free_adaptive(mf_scratch *s)
                   A->d.node_line,  A->d.line_ptr,   A->d.line_col,   A->d.line_val,
                   A->d.line_val32, A->d.list,       A->d.ht_ptr,     A->d.tgt_node,
                   A->d.tgt_ptr,    A->d.ent_local,  A->d.ent_w,      A->d.ent_w32,
-                  A->d.slot_cell,  A->d.slot_tgt};
+                  A->d.slot_cell,  A->d.slot_tgt,   A->d.hang_nodes, A->d.cm_node,
+                  A->d.cm_ptr,     A->d.cm_h,       A->d.cm_w,       A->d.cm_w32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete A;

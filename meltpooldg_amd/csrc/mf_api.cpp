@@ -222,7 +222,12 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
-  if (m.ad) return launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
+  if (m.ad) {
+    int rc = launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
+    if (rc) return rc;
+    // hanging rows hold the unconstrained cell sums: move them to their masters
+    return launch_condense<Number>(op->s, dst, stop);
+  }
   // the transposed operator (adjoint) runs on the pencil kernel
   if (g_tangent_variant == 1 && !op->transpose)
     return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);

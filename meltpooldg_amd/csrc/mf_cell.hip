@@ -237,7 +237,7 @@ struct GridAccess {
 };
 
 // Node access of an adaptive mesh: a workgroup serves CPB cells of one colour from the cell
-// list (cells of one colour share no written node).  Local node e = i + K*p of a cell reads
+// list (cells of one colour share no node).  Local node e = i + K*p of a cell reads
 // its global node from the cell's node table; a hanging node reads the interpolation of its
 // masters (read_dof_values with hanging-node constraints).
 template <typename Number, int K, int CPB>
@@ -287,35 +287,19 @@ struct ListAccess {
   __device__ void prefetch(const Number *dst, Number (&d)[K]) const
   {
 #pragma unroll
-    for (int i = 0; i < K; ++i) d[i] = (active && !hanging_cell) ? dst[nodes[i]] : Number(0);
+    for (int i = 0; i < K; ++i) d[i] = active ? dst[nodes[i]] : Number(0);
   }
-  // Cells without hanging nodes store their nodes directly.  A cell with hanging nodes stages
-  // its outputs in LDS; each of its threads then sums the entries of some of the cell's
-  // targets (the distinct nodes it writes: free local nodes and masters), so every target is
-  // written by one thread (distribute_local_to_global with constraints).
+  // Every cell adds into its own nodes, hanging ones included (cells of one colour share no
+  // node); the hanging rows are moved to their masters by the condensation that follows the
+  // colour launches.  Dirichlet rows are skipped.
   __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
-                          Number *dst, Number *stage)
+                          Number *dst, Number *)
   {
-    __syncthreads();  // the stage may still be read by other cells' threads
-    store_pencil<Number, K, CPB, 0>(stage, p, c, o);
-    __syncthreads();
     if (!active) return;
-    if (!hanging_cell) {
 #pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const uint32_t n = nodes[i];
-        if (!masked(A, n)) dst[n] = d[i] + o[i];
-      }
-      return;
-    }
-    const int32_t hi = A.cell_hidx[cell];
-    for (uint32_t t = A.ht_ptr[hi] + uint32_t(p); t < A.ht_ptr[hi + 1]; t += uint32_t(K * K)) {
-      const uint32_t n = A.tgt_node[t];
-      if (masked(A, n)) continue;
-      Number v = 0;
-      for (uint32_t q = A.tgt_ptr[t]; q < A.tgt_ptr[t + 1]; ++q)
-        v += A.ent_w[q] * stage[int(A.ent_local[q]) * CPB + c];
-      dst[n] += v;
+    for (int i = 0; i < K; ++i) {
+      const uint32_t n = nodes[i];
+      if (!masked(A, n)) dst[n] = d[i] + o[i];
     }
   }
 };
@@ -323,7 +307,8 @@ struct ListAccess {
 // Condensed diagonal on cells with hanging nodes: a slot is (cell, target node D).  The local
 // source is the coefficient vector c_D of D in the cell's constrained basis (1 on the free
 // local node D, the weight of D on each hanging local node with master D); the slot adds
-// c_D . (A_cell c_D) to diag[D] (MatrixFreeTools::compute_diagonal with constraints).
+// c_D . (A_cell c_D) to diag[D] (MatrixFreeTools::compute_diagonal with constraints).  Slots of
+// different cells may share D, so the addition is atomic (this runs once per diagonal).
 template <typename Number, int K, int CPB>
 struct SlotAccess {
   static constexpr int NLOC = K * K * K;
@@ -379,7 +364,7 @@ struct SlotAccess {
       Number sum = 0;
       for (int j = 0; j < K * K; ++j) sum += stage[j * CPB + c];
       const uint32_t n = A.tgt_node[tgt];
-      if (!(A.use_mask && A.mask && A.mask[n])) dst[n] += sum;
+      if (!(A.use_mask && A.mask && A.mask[n])) atomicAdd(dst + n, sum);
     }
   }
 };
@@ -863,17 +848,17 @@ static int launch_parts(const std::vector<int64_t> &start, const std::vector<int
 static void list_ranges(const AdaptiveData &ad, int mode, std::vector<int64_t> &start,
                         std::vector<int64_t> &end)
 {
+  if (mode == 2) {
+    start.assign(1, ad.slot_start[0]);
+    end.assign(1, ad.slot_start[1]);
+    return;
+  }
   const int nc = ad.n_colours;
   start.resize(nc);
   end.resize(nc);
   for (int col = 0; col < nc; ++col) {
-    if (mode == 2) {
-      start[col] = ad.slot_start[col];
-      end[col] = ad.slot_start[col + 1];
-    } else {
-      start[col] = ad.colour_start[col];
-      end[col] = mode == 1 ? ad.colour_hang[col] : ad.colour_start[col + 1];
-    }
+    start[col] = ad.colour_start[col];
+    end[col] = mode == 1 ? ad.colour_hang[col] : ad.colour_start[col + 1];
   }
 }
 

@@ -135,7 +135,10 @@ struct AdaptiveData {
   std::vector<uint8_t> ent_local;
   std::vector<double> ent_w;
   std::vector<uint32_t> slot_cell, slot_tgt;
-  std::vector<int64_t> slot_start;     // [n_colours + 1]
+  std::vector<int64_t> slot_start;     // [2]: all slots in one launch
+  std::vector<uint32_t> hang_nodes;    // hanging nodes
+  std::vector<uint32_t> cm_node, cm_ptr, cm_h;  // per master: the hanging nodes it receives from
+  std::vector<double> cm_w;
   struct {
     uint32_t *cell_nodes = nullptr;
     uint64_t *cell_hmask = nullptr;
@@ -149,12 +152,19 @@ struct AdaptiveData {
     double *ent_w = nullptr;
     float *ent_w32 = nullptr;
     uint32_t *slot_cell = nullptr, *slot_tgt = nullptr;
+    uint32_t *hang_nodes = nullptr, *cm_node = nullptr, *cm_ptr = nullptr, *cm_h = nullptr;
+    double *cm_w = nullptr;
+    float *cm_w32 = nullptr;
   } d;
 };
 
 int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
 int upload_adaptive(::mf_scratch *s);
 int ensure_adaptive_on_device(::mf_scratch *s);
+// y[m] += sum_h w_hm y[h] over the hanging nodes h with master m, then y[h] = 0 (the transpose
+// of the constraints applied to the unconstrained cell sums), mf_ops.hip
+template <typename Number>
+int launch_condense(const ::mf_scratch *s, Number *y, const int *stop);
 // x[h] = sum_m w x[m] on every hanging node h (AffineConstraints::distribute), mf_ops.hip
 int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
                       const double *line_val, double *x, int64_t n, hipStream_t st);

@@ -199,6 +199,61 @@ __global__ void distribute_kernel(const int32_t *__restrict__ node_line,
   }
 }
 
+// y[m] += sum_h w_hm y[h] for every master m (one thread per master, no conflicts)
+template <typename Number>
+__global__ void condense_masters_kernel(const uint32_t *__restrict__ cm_node,
+                                        const uint32_t *__restrict__ cm_ptr,
+                                        const uint32_t *__restrict__ cm_h,
+                                        const Number *__restrict__ cm_w,
+                                        const uint8_t *__restrict__ mask, int64_t n_masters,
+                                        Number *y, const int *stop)
+{
+  if (stop && *stop) return;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_masters;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t m = cm_node[i];
+    if (mask && mask[m]) continue;
+    Number v = 0;
+    for (uint32_t t = cm_ptr[i]; t < cm_ptr[i + 1]; ++t) v += cm_w[t] * y[cm_h[t]];
+    y[m] += v;
+  }
+}
+
+template <typename Number>
+__global__ void zero_hanging_kernel(const uint32_t *__restrict__ hang, int64_t n_hang, Number *y,
+                                    const int *stop)
+{
+  if (stop && *stop) return;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_hang;
+       i += int64_t(gridDim.x) * blockDim.x)
+    y[hang[i]] = Number(0);
+}
+
+template <typename Number>
+int launch_condense(const mf_scratch *s, Number *y, const int *stop)
+{
+  const AdaptiveData *A = s->ad;
+  if (!A || A->hang_nodes.empty()) return MF_OK;
+  const int64_t nm = int64_t(A->cm_node.size()), nh = int64_t(A->hang_nodes.size());
+  const Number *w;
+  if constexpr (sizeof(Number) == 8)
+    w = reinterpret_cast<const Number *>(A->d.cm_w);
+  else
+    w = reinterpret_cast<const Number *>(A->d.cm_w32);
+  const int bm = int(std::min<int64_t>((nm + 255) / 256, 4096));
+  hipLaunchKernelGGL(condense_masters_kernel<Number>, dim3(bm), dim3(256), 0, s->stream,
+                     A->d.cm_node, A->d.cm_ptr, A->d.cm_h, w, s->d_mask, nm, y, stop);
+  MF_HIP_TRY(hipGetLastError());
+  const int bh = int(std::min<int64_t>((nh + 255) / 256, 4096));
+  hipLaunchKernelGGL(zero_hanging_kernel<Number>, dim3(bh), dim3(256), 0, s->stream,
+                     A->d.hang_nodes, nh, y, stop);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template int launch_condense<double>(const mf_scratch *, double *, const int *);
+template int launch_condense<float>(const mf_scratch *, float *, const int *);
+
 int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
                       const double *line_val, double *x, int64_t n, hipStream_t st)
 {
