@@ -58,6 +58,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-adaptive", action="store_true",
+                    help="skip the adaptive-mesh (hanging-node) solve section")
+    ap.add_argument("--no-levelset", action="store_true",
+                    help="skip the level-set advection solve section")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only run 5 operator applications (vmult_add) and 5 plain copies of one "
                          "DoF vector (calibration), for hardware-counter runs")
@@ -157,6 +161,11 @@ def main():
               f"({rep32['outer_iterations']} outer/{rep32['iterations']} inner), err {err:.2e}",
               file=sys.stderr, flush=True)
 
+    if not args.no_adaptive:
+        extra["adaptive"] = adaptive_case(mf, torch, stream)
+    if not args.no_levelset:
+        extra["levelset"] = levelset_case(mf, torch, stream)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(T, mask, b, n, args.cpu_budget)
@@ -189,6 +198,121 @@ def main():
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def _timed_solve(mf, torch, op, b, **kw):
+    x = torch.zeros_like(b)
+    mf.LinearSolver.solve(op, x, b, **kw)          # warm-up (allocations, code paths)
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = mf.LinearSolver.solve(op, x, b, **kw)
+    torch.cuda.synchronize()
+    return x, rep, time.perf_counter() - t0
+
+
+def _time_vmult(torch, op, x, y, reps):
+    op.vmult(y, x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        op.vmult(y, x)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
+def adaptive_case(mf, torch, stream):
+    """Mixed-precision heat solve on an adaptive mesh: 32^3 base cells refined twice in the band
+    | |x-(0.5,0.5,0.5)| - 0.3 | < 0.05 (2:1 balanced, hanging nodes), FE_Q3, the tanh Jacobian at
+    the same linearisation point as the main workload, seeded U(-1,1) RHS (numpy seed 4, zero on
+    constrained rows).  FP64 Jacobi-CG and FP32-inner (rtol 1e-3) / FP64-refined CG to 1e-10."""
+    t0 = time.perf_counter()
+    cells = mf.refine_band_cells(32, 2, (0.5, 0.5, 0.5), 0.3, 0.05)
+    s = mf.AdaptiveScratchData(3, 32, cells, stream=stream)
+    setup_s = time.perf_counter() - t0
+    X = s.support_points()
+    r2 = ((X - np.array([0.5, 0.5, 1.0])) ** 2).sum(axis=1)
+    rng = np.random.default_rng(2)
+    T = (300.0 + 1500.0 * np.exp(-r2 / (2 * 0.05 ** 2))) * (1.0 + 0.01 * rng.uniform(-1, 1, s.n_dofs))
+    op = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                                tanh_Tc=1700.0, tanh_width=50.0, t_dependent=True, dt=1e-4)
+    T_dev = torch.tensor(T, dtype=torch.float64, device="cuda")
+    op.set_fields(T=T_dev)
+    info = s.constraint_info()
+    node_line = s.constraint_lines()[0]
+    b = np.random.default_rng(4).uniform(-1, 1, s.n_dofs)
+    b[(s.default_dirichlet_mask() == 1) | (node_line >= 0)] = 0.0
+    bd = torch.tensor(b, dtype=torch.float64, device="cuda")
+    y = torch.zeros_like(bd)
+    t_vm = _time_vmult(torch, op, bd, y, 20)
+    x64, r64, t64 = _timed_solve(mf, torch, op, bd, solver="CG", rel_tolerance=1e-10)
+    x32, r32, t32 = _timed_solve(mf, torch, op, bd, solver="CG", rel_tolerance=1e-10,
+                                 precision="fp32_refined", inner_rel_tolerance=1e-3)
+    err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    out = {"workload": "FE_Q3 tanh heat Jacobian on a band-refined 32^3 mesh (+2 levels, "
+                       "hanging nodes)",
+           "n_cells": int(len(cells)), "n_dofs": int(s.n_dofs),
+           "n_hanging": info["n_hanging"], "n_colours": info["n_colours"],
+           "setup_s": setup_s, "vmult_us": t_vm * 1e6, "vmult_dofs_per_s": s.n_dofs / t_vm,
+           "cg_fp64_s": t64, "cg_fp64_iterations": r64["iterations"],
+           "cg_fp32_refined_s": t32, "cg_fp32_refined_inner_iterations": r32["iterations"],
+           "cg_fp32_refined_outer": r32["outer_iterations"],
+           "cg_fp32_refined_rel_residual": r32["final_residual"] / r32["initial_residual"],
+           "refined_vs_fp64_rel_error": err, "speedup_refined_over_fp64": t64 / t32}
+    print(f"# adaptive: {len(cells)} cells, {s.n_dofs} DoFs, vmult {t_vm*1e6:.0f} us, CG fp64 "
+          f"{t64*1e3:.1f} ms ({r64['iterations']} it), refined {t32*1e3:.1f} ms "
+          f"({r32['outer_iterations']}/{r32['iterations']}), err {err:.2e}", file=sys.stderr,
+          flush=True)
+    return out
+
+
+def levelset_case(mf, torch, stream):
+    """Conservative level-set advection, mixed precision: FE_Q2 on 96^3 cells, phi0 =
+    tanh(d/(2 eps)) for the sphere R=0.25 at (0.35,0.35,0.35), eps = 1.5h, the LeVeque 3D
+    deformation field as a nodal velocity, artificial diffusion 0.5h, BDF2 tangent (3/(2dt)) M +
+    A with dt from CFL 0.5, right-hand side (phi0/dt) M-weighted (create_rhs with theta = 1).
+    Jacobi-GMRES(30) FP64 and FP32-inner / FP64-refined to 1e-10."""
+    ncell, deg = 96, 2
+    s = mf.ScratchData(deg, ncell, stream=stream)
+    mask = s.boundary_mask()
+    s.set_dirichlet_mask(mask)
+    X = s.support_points()
+    h = 1.0 / ncell
+    eps = 1.5 * h
+    d = 0.25 - np.linalg.norm(X - 0.35, axis=1)
+    phi0 = np.tanh(d / (2 * eps))
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    pi = np.pi
+    u = np.stack([2 * np.sin(pi * x) ** 2 * np.sin(2 * pi * y) * np.sin(2 * pi * z),
+                  -np.sin(2 * pi * x) * np.sin(pi * y) ** 2 * np.sin(2 * pi * z),
+                  -np.sin(2 * pi * x) * np.sin(2 * pi * y) * np.sin(pi * z) ** 2])
+    umax = float(np.abs(u).max())
+    dt = 0.5 * h / umax
+    op = mf.AdvectionDiffusionOperator(s, theta=1.0, diffusivity=0.5 * h, mass_factor=1.5, dt=dt)
+    vel = torch.tensor(u.ravel(), dtype=torch.float64, device="cuda")
+    op.set_fields(velocity=vel)
+    phi_d = torch.tensor(phi0, dtype=torch.float64, device="cuda")
+    rhs = torch.zeros_like(phi_d)
+    op.create_rhs(rhs, phi_d)
+    y_ = torch.zeros_like(phi_d)
+    t_vm = _time_vmult(torch, op, phi_d, y_, 20)
+    x64, r64, t64 = _timed_solve(mf, torch, op, rhs, solver="GMRES", rel_tolerance=1e-10)
+    x32, r32, t32 = _timed_solve(mf, torch, op, rhs, solver="GMRES", rel_tolerance=1e-10,
+                                 precision="fp32_refined", inner_rel_tolerance=1e-3)
+    err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    out = {"workload": "FE_Q2 level-set advection-diffusion, 96^3 cells, BDF2 tangent, "
+                       "deformation velocity, GMRES(30)",
+           "n_dofs": int(s.n_dofs), "dt": dt, "vmult_us": t_vm * 1e6,
+           "vmult_dofs_per_s": s.n_dofs / t_vm,
+           "gmres_fp64_s": t64, "gmres_fp64_iterations": r64["iterations"],
+           "gmres_fp32_refined_s": t32, "gmres_fp32_refined_inner_iterations": r32["iterations"],
+           "gmres_fp32_refined_outer": r32["outer_iterations"],
+           "gmres_fp32_refined_rel_residual": r32["final_residual"] / r32["initial_residual"],
+           "refined_vs_fp64_rel_error": err, "speedup_refined_over_fp64": t64 / t32}
+    print(f"# levelset: {s.n_dofs} DoFs, vmult {t_vm*1e6:.0f} us, GMRES fp64 {t64*1e3:.1f} ms "
+          f"({r64['iterations']} it), refined {t32*1e3:.1f} ms ({r32['outer_iterations']}/"
+          f"{r32['iterations']}), err {err:.2e}", file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(T, mask, b, n, budget):

@@ -224,3 +224,64 @@ def test_autograd_gradient_vs_central_fd():
         fd = (Jp - Jm) / (2 * h)
         ad = g @ d
         assert abs(fd - ad) <= 1e-5 * max(abs(ad), 1e-3 * np.abs(g).max()), (fd, ad)
+
+
+def test_moving_laser_transient_full_size():
+    """The full moving-laser case: FE_Q2 on 48^3 cells (912,673 DoFs), per-cell k = exp(g) with g
+    ~ N(0, 0.3^2) (seed 5), melt velocity (0.1,0,0) in the top 20 %, Gaussian laser P=1, r=0.05
+    moving along x at 0.5 for 20 implicit-Euler steps of dt=1e-3, checkpoints every 5 steps;
+    J = 1/2 ||T_20 - T*||^2_L2 with T* the forward solution for k (1 + 0.1 U) (seed 6).
+    Adjoint gradient (k_cell, P, r) against central differences (h = 1e-6) along 20 seeded
+    directions (seed 7)."""
+    import meltpooldg_amd as mf
+    nc, steps, interval, dt = 48, 20, 5, 1e-3
+    s = mf.ScratchData(2, nc, stream=torch.cuda.current_stream())
+    mask = s.boundary_mask()
+    s.set_dirichlet_mask(mask)
+    X = s.support_points()
+    vel = np.zeros((3, s.n_dofs))
+    vel[0, X[:, 2] > 0.8] = 0.1
+    veld = _dev(vel.ravel())
+    k0 = np.exp(0.3 * np.random.default_rng(5).standard_normal(s.n_cells))
+    op = mf.HeatDiffuseOperator(s, rho_cp=1.0, k=1.0, dt=dt)
+    T0 = torch.zeros(s.n_dofs, dtype=torch.float64, device="cuda")
+    start, scan = (0.25, 0.5, 1.0), (0.5, 0.0, 0.0)
+
+    def forward(kcell, P, r):
+        op.set_fields(T=T0, velocity=veld, k_cell=_dev(kcell))
+        tr = mf.HeatTransient(op, steps, dt, P, r, start, scan, checkpoint_interval=interval,
+                              solver="GMRES", rel_tolerance=1e-13)
+        TN = torch.zeros_like(T0)
+        tr.forward(T0, TN)
+        return tr, TN
+
+    _, target = forward(k0 * (1 + 0.1 * np.random.default_rng(6).uniform(-1, 1, s.n_cells)),
+                        1.0, 0.05)
+    target = target.clone()
+
+    def objective(kcell, P, r, grad=False):
+        tr, TN = forward(kcell, P, r)
+        e = TN - target
+        Me = torch.zeros_like(e)
+        mf.mass_vmult(s, Me, e)
+        J = 0.5 * float(e @ Me)
+        if not grad:
+            return J
+        gk = torch.zeros(s.n_cells, dtype=torch.float64, device="cuda")
+        gP, gR = tr.backward(Me, gk)
+        return J, np.concatenate([gk.cpu().numpy(), [gP, gR]])
+
+    J, g = objective(k0, 1.0, 0.05, grad=True)
+    assert J > 0 and np.all(np.isfinite(g))
+    theta = np.concatenate([k0, [1.0, 0.05]])
+    rng = np.random.default_rng(7)
+    h = 1e-6
+    worst = 0.0
+    for _ in range(20):
+        d = rng.standard_normal(len(theta))
+        d /= np.linalg.norm(d)
+        tp, tm = theta + h * d, theta - h * d
+        fd = (objective(tp[:-2], tp[-2], tp[-1]) - objective(tm[:-2], tm[-2], tm[-1])) / (2 * h)
+        ad = g @ d
+        worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-3 * np.abs(g).max()))
+    assert worst < 1e-5, worst
