@@ -360,7 +360,19 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   std::vector<uint64_t> used(static_cast<size_t>(nn), 0ull);
   std::vector<int> colour(static_cast<size_t>(n_cells));
   int n_col = 0;
-  for (int64_t c = 0; c < n_cells; ++c) {
+  // greedy colouring visiting the cells class by class, class = (level parity, index parity):
+  // cells of one class never touch (same level: two apart; levels two apart never touch on a
+  // 2:1-balanced mesh), so the first 8 colours come out as balanced parity classes and the
+  // coarse/fine interface cells fill the rest
+  std::vector<int64_t> visit(static_cast<size_t>(n_cells));
+  for (int64_t c = 0; c < n_cells; ++c) visit[c] = c;
+  auto cls = [&](int64_t c) {
+    const int32_t *cl = cells + 4 * c;
+    return (cl[0] & 1) * 8 + (cl[1] & 1) + 2 * (cl[2] & 1) + 4 * (cl[3] & 1);
+  };
+  std::stable_sort(visit.begin(), visit.end(),
+                   [&](int64_t a, int64_t b) { return cls(a) < cls(b); });
+  for (int64_t c : visit) {
     A->cell_level[c] = uint8_t(cells[4 * c]);
     uint64_t forbidden = 0;
     for (int e = 0; e < nloc; ++e) {
@@ -510,6 +522,10 @@ int upload_adaptive(mf_scratch *s)
       (rc = upload(&A->d.cm_ptr, A->cm_ptr)) || (rc = upload(&A->d.cm_h, A->cm_h)) ||
       (rc = upload(&A->d.cm_w, A->cm_w)) || (rc = upload(&A->d.cm_w32, cw32)))
     return rc;
+  if (A->n_hanging > 0) {
+    MF_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&A->d.hang_val), sizeof(double) * A->n_hanging));
+    MF_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&A->d.hang_val32), sizeof(float) * A->n_hanging));
+  }
   const size_t n = size_t(s->mesh.n_dofs);
   if (!s->d_mask && !s->h_mask.empty()) {
     MF_HIP_TRY(hipMalloc(&s->d_mask, n));
@@ -527,7 +543,8 @@ void free_adaptive(mf_scratch *s)
                   A->d.line_val32, A->d.list,       A->d.ht_ptr,     A->d.tgt_node,
                   A->d.tgt_ptr,    A->d.ent_local,  A->d.ent_w,      A->d.ent_w32,
                   A->d.slot_cell,  A->d.slot_tgt,   A->d.hang_nodes, A->d.cm_node,
-                  A->d.cm_ptr,     A->d.cm_h,       A->d.cm_w,       A->d.cm_w32};
+                  A->d.cm_ptr,     A->d.cm_h,       A->d.cm_w,       A->d.cm_w32,
+                  A->d.hang_val,   A->d.hang_val32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete A;

@@ -223,6 +223,16 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.stop = stop;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
   if (m.ad) {
+    if (!op->gather_plain && m.ad->n_hanging > 0) {
+      Number *hv;
+      if constexpr (sizeof(Number) == 8)
+        hv = reinterpret_cast<Number *>(m.ad->d.hang_val);
+      else
+        hv = reinterpret_cast<Number *>(m.ad->d.hang_val32);
+      int rc = launch_hanging_values<Number>(op->s, src, a.use_mask, hv, stop);
+      if (rc) return rc;
+      a.hang_val = hv;
+    }
     int rc = launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
     if (rc) return rc;
     // hanging rows hold the unconstrained cell sums: move them to their masters

@@ -161,6 +161,7 @@ struct LaunchPart {
 // nodes (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
 template <typename Number, int K, int CPB>
 struct GridAccess {
+  static constexpr bool COOP = false;
   static constexpr int P = K - 1;
   uint32_t rowbase, gbase;
   int c, ncx, cellrow, nsub;
@@ -237,69 +238,88 @@ struct GridAccess {
 };
 
 // Node access of an adaptive mesh: a workgroup serves CPB cells of one colour from the cell
-// list (cells of one colour share no node).  Local node e = i + K*p of a cell reads
-// its global node from the cell's node table; a hanging node reads the interpolation of its
-// masters (read_dof_values with hanging-node constraints).
+// list (cells of one colour share no node).  Node values are gathered cooperatively: the
+// block's CPB*K^3 (cell, local node) slots are spread over its threads so that consecutive
+// lanes read the nodes of one cell (whose numbers are close: nodes are numbered in order of
+// first appearance over the Morton-ordered cells), and staged in LDS in the x-pencil layout.
+// A hanging node reads the interpolation of its masters (read_dof_values with hanging-node
+// constraints), evaluated for all hanging nodes by a pass before the cell launches.
 template <typename Number, int K, int CPB>
 struct ListAccess {
+  static constexpr bool COOP = true;
   static constexpr int NLOC = K * K * K;
-  const uint32_t *nodes;  // the K nodes of the thread's x pencil
-  uint32_t hbits;         // hanging flags of these K nodes
-  int c, p, cell, level;
-  bool active, hanging_cell;
+  int cell, level;     // the cell of the thread's pencils
+  bool active;
+  uint32_t node[K];    // global node of each cooperative slot of the thread
+  uint32_t hbits = 0;  // slot i is a hanging node
+  uint32_t abits = 0;  // slot i belongs to an existing cell
 
-  __device__ ListAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p_)
+  __device__ ListAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c, int)
   {
-    c = c_;
-    p = p_;
     const int64_t slot = int64_t(blockIdx.x) * CPB + c;
     active = slot < lp.count;
     cell = active ? int(A.list[lp.start + slot]) : 0;
-    nodes = A.cell_nodes + size_t(cell) * NLOC + K * p;
-    const uint64_t hm = active ? A.cell_hmask[cell] : 0ull;
-    hanging_cell = hm != 0ull;
-    hbits = uint32_t(hm >> (K * p)) & ((1u << K) - 1u);
     level = active ? int(A.cell_level[cell]) : 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int j = int(threadIdx.x) + i * int(blockDim.x);
+      const int64_t cs = int64_t(blockIdx.x) * CPB + j / NLOC;
+      const int e = j % NLOC;
+      node[i] = 0u;
+      if (cs < lp.count) {
+        const uint32_t g = A.list[lp.start + cs];
+        node[i] = A.cell_nodes[size_t(g) * NLOC + e];
+        abits |= 1u << i;
+        if ((A.cell_hmask[g] >> e) & 1ull) hbits |= 1u << i;
+      }
+    }
   }
   __device__ int iterations() const { return 1; }
   __device__ void at(int) {}
   __device__ Number hscale() const { return ldexp(Number(1), -level); }
+  // LDS position ([local node][cell] layout) of cooperative slot i
+  __device__ int coop_pos(int i) const
+  {
+    const int j = int(threadIdx.x) + i * int(blockDim.x);
+    return (j % NLOC) * CPB + j / NLOC;
+  }
   __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
   {
     return A.use_mask && A.mask && A.mask[n];
   }
-  __device__ Number plain(const Number *f, int i) const { return active ? f[nodes[i]] : Number(0); }
-  __device__ Number value(const CellArgs<Number> &A, const Number *f, int i) const
+  __device__ Number coop_plain(const Number *f, int i) const
   {
-    if (!active) return Number(0);
-    const uint32_t n = nodes[i];
-    if ((hbits >> i) & 1u) {
-      const int32_t l = A.node_line[n];
-      Number v = 0;
-      for (uint32_t t = A.line_ptr[l]; t < A.line_ptr[l + 1]; ++t) {
-        const uint32_t mn = A.line_col[t];
-        if (!masked(A, mn)) v += A.line_val[t] * f[mn];
-      }
-      return v;
-    }
+    return ((abits >> i) & 1u) ? f[node[i]] : Number(0);
+  }
+  __device__ Number coop_value(const CellArgs<Number> &A, const Number *f, int i) const
+  {
+    if (!((abits >> i) & 1u)) return Number(0);
+    const uint32_t n = node[i];
+    // hanging node: its interpolated value, computed for the whole source beforehand
+    if ((hbits >> i) & 1u) return A.hang_val[A.node_line[n]];
     return masked(A, n) ? Number(0) : f[n];
   }
   __device__ void prefetch(const Number *dst, Number (&d)[K]) const
   {
 #pragma unroll
-    for (int i = 0; i < K; ++i) d[i] = active ? dst[nodes[i]] : Number(0);
+    for (int i = 0; i < K; ++i) d[i] = ((abits >> i) & 1u) ? dst[node[i]] : Number(0);
   }
   // Every cell adds into its own nodes, hanging ones included (cells of one colour share no
   // node); the hanging rows are moved to their masters by the condensation that follows the
-  // colour launches.  Dirichlet rows are skipped.
-  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
-                          Number *dst, Number *)
+  // colour launches.  Dirichlet rows are skipped.  The outputs go through LDS back to the
+  // cooperative slots.
+  __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int c,
+                          Number *dst, Number *stage)
   {
-    if (!active) return;
+    const int p = int(threadIdx.x) / CPB;
+    __syncthreads();  // the stage may still be read by other threads
+    store_pencil<Number, K, CPB, 0>(stage, p, c, o);
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < K; ++i) {
-      const uint32_t n = nodes[i];
-      if (!masked(A, n)) dst[n] = d[i] + o[i];
+      if (!((abits >> i) & 1u)) continue;
+      const uint32_t n = node[i];
+      if (!masked(A, n)) dst[n] = d[i] + stage[coop_pos(i)];
     }
   }
 };
@@ -311,6 +331,7 @@ struct ListAccess {
 // different cells may share D, so the addition is atomic (this runs once per diagonal).
 template <typename Number, int K, int CPB>
 struct SlotAccess {
+  static constexpr bool COOP = false;
   static constexpr int NLOC = K * K * K;
   const uint32_t *nodes;
   uint32_t tgt;
@@ -431,6 +452,49 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 
   // ---------------- E1: x pencils, gather from HBM ----------------
   Number dpre[K];  // dst entries of the pencil, read while the integration sweeps run
+  if constexpr (Access::COOP) {
+    // cooperative gather into the LDS tensors, then the x interpolation in place
+    if constexpr (HAS_U) {
+      const Number *in = RHS_HEAT ? A.T : src;
+      const bool cons = TANGENT && !A.gather_plain;
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const Number v = cons ? ac.coop_value(A, in, i) : ac.coop_plain(in, i);
+        L0[ac.coop_pos(i)] = v;
+        if constexpr (FDOT) lds[I_SRC * BUF + ac.coop_pos(i)] = v;
+      }
+    }
+    if constexpr (USE_T) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) lds[I_T * BUF + ac.coop_pos(i)] = ac.coop_plain(A.T, i);
+    }
+    if constexpr (VEL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+          lds[(I_W + d) * BUF + ac.coop_pos(i)] = ac.coop_plain(A.vel + size_t(d) * A.n_dofs, i);
+    }
+    if constexpr (RHS_HEAT) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        lds[I_A * BUF + ac.coop_pos(i)] = ac.coop_plain(src, i);
+        lds[(I_A + 1) * BUF + ac.coop_pos(i)] = A.aux1 ? ac.coop_plain(A.aux1, i) : Number(0);
+      }
+    }
+    __syncthreads();
+    if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 0, false>(S, L0, p, c);
+    if constexpr (USE_T) sweep_inplace<Number, K, CPB, 0, false>(S, lds + I_T * BUF, p, c);
+    if constexpr (VEL) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        sweep_inplace<Number, K, CPB, 0, false>(S, lds + (I_W + d) * BUF, p, c);
+    }
+    if constexpr (RHS_HEAT) {
+      sweep_inplace<Number, K, CPB, 0, false>(S, lds + I_A * BUF, p, c);
+      sweep_inplace<Number, K, CPB, 0, false>(S, lds + (I_A + 1) * BUF, p, c);
+    }
+  } else {
   {
     Number o[K];
     if constexpr (HAS_U) {
@@ -471,6 +535,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
       matvec<Number, K, false>(S, v, o);
       store_pencil<Number, K, CPB, 0>(lds + (I_A + 1) * BUF, p, c, o);
     }
+  }
   }
   __syncthreads();
 

@@ -92,6 +92,7 @@ struct CellArgs {
   const Number *ent_w;         // weight of each entry
   const uint32_t *slot_cell;   // condensed diagonal: (cell, target) per slot
   const uint32_t *slot_tgt;
+  const Number *hang_val;      // [n_hanging] interpolated source values of the hanging nodes
 };
 
 struct AdaptiveData;
@@ -155,6 +156,8 @@ struct AdaptiveData {
     uint32_t *hang_nodes = nullptr, *cm_node = nullptr, *cm_ptr = nullptr, *cm_h = nullptr;
     double *cm_w = nullptr;
     float *cm_w32 = nullptr;
+    double *hang_val = nullptr;   // work: hanging-node values of the current source (FP64)
+    float *hang_val32 = nullptr;  // and FP32
   } d;
 };
 
@@ -165,6 +168,11 @@ int ensure_adaptive_on_device(::mf_scratch *s);
 // of the constraints applied to the unconstrained cell sums), mf_ops.hip
 template <typename Number>
 int launch_condense(const ::mf_scratch *s, Number *y, const int *stop);
+// hv[l] = sum_t w_t src[m_t] over the line of every hanging node (constrained masters read as 0
+// when use_mask), the source values hanging nodes take in read_dof_values, mf_ops.hip
+template <typename Number>
+int launch_hanging_values(const ::mf_scratch *s, const Number *src, int use_mask, Number *hv,
+                          const int *stop);
 // x[h] = sum_m w x[m] on every hanging node h (AffineConstraints::distribute), mf_ops.hip
 int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
                       const double *line_val, double *x, int64_t n, hipStream_t st);

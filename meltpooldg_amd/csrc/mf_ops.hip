@@ -199,7 +199,8 @@ __global__ void distribute_kernel(const int32_t *__restrict__ node_line,
   }
 }
 
-// y[m] += sum_h w_hm y[h] for every master m (one thread per master, no conflicts)
+// y[m] += sum_h w_hm y[h] for every master m: 16 lanes per master share its entries and
+// combine them with shuffles (one group per master, no conflicts)
 template <typename Number>
 __global__ void condense_masters_kernel(const uint32_t *__restrict__ cm_node,
                                         const uint32_t *__restrict__ cm_ptr,
@@ -209,15 +210,68 @@ __global__ void condense_masters_kernel(const uint32_t *__restrict__ cm_node,
                                         Number *y, const int *stop)
 {
   if (stop && *stop) return;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n_masters;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  const int lane = threadIdx.x & 15;
+  const int64_t groups = int64_t(gridDim.x) * (blockDim.x / 16);
+  for (int64_t i = int64_t(blockIdx.x) * (blockDim.x / 16) + (threadIdx.x >> 4); i < n_masters;
+       i += groups) {
     const uint32_t m = cm_node[i];
-    if (mask && mask[m]) continue;
     Number v = 0;
-    for (uint32_t t = cm_ptr[i]; t < cm_ptr[i + 1]; ++t) v += cm_w[t] * y[cm_h[t]];
-    y[m] += v;
+    for (uint32_t t = cm_ptr[i] + uint32_t(lane); t < cm_ptr[i + 1]; t += 16u)
+      v += cm_w[t] * y[cm_h[t]];
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+    if (lane == 0 && !(mask && mask[m])) y[m] += v;
   }
 }
+
+// hv[l] = sum_t w_t src[m_t], 16 lanes per line
+template <typename Number>
+__global__ void hanging_values_kernel(const uint32_t *__restrict__ line_ptr,
+                                      const uint32_t *__restrict__ line_col,
+                                      const Number *__restrict__ line_val,
+                                      const uint8_t *__restrict__ mask, const Number *__restrict__ src,
+                                      Number *__restrict__ hv, int64_t n_lines, const int *stop)
+{
+  if (stop && *stop) return;
+  const int lane = threadIdx.x & 15;
+  const int64_t groups = int64_t(gridDim.x) * (blockDim.x / 16);
+  for (int64_t l = int64_t(blockIdx.x) * (blockDim.x / 16) + (threadIdx.x >> 4); l < n_lines;
+       l += groups) {
+    Number v = 0;
+    for (uint32_t t = line_ptr[l] + uint32_t(lane); t < line_ptr[l + 1]; t += 16u) {
+      const uint32_t m = line_col[t];
+      if (!(mask && mask[m])) v += line_val[t] * src[m];
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+    if (lane == 0) hv[l] = v;
+  }
+}
+
+template <typename Number>
+int launch_hanging_values(const mf_scratch *s, const Number *src, int use_mask, Number *hv,
+                          const int *stop)
+{
+  const AdaptiveData *A = s->ad;
+  if (!A || A->n_hanging == 0) return MF_OK;
+  const Number *w;
+  if constexpr (sizeof(Number) == 8)
+    w = reinterpret_cast<const Number *>(A->d.line_val);
+  else
+    w = reinterpret_cast<const Number *>(A->d.line_val32);
+  const int64_t nl = A->n_hanging;
+  const int b = int(std::min<int64_t>((nl + 15) / 16, 16384));
+  hipLaunchKernelGGL(hanging_values_kernel<Number>, dim3(b), dim3(256), 0, s->stream,
+                     A->d.line_ptr, A->d.line_col, w, use_mask ? s->d_mask : nullptr, src, hv, nl,
+                     stop);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template int launch_hanging_values<double>(const mf_scratch *, const double *, int, double *,
+                                           const int *);
+template int launch_hanging_values<float>(const mf_scratch *, const float *, int, float *,
+                                          const int *);
 
 template <typename Number>
 __global__ void zero_hanging_kernel(const uint32_t *__restrict__ hang, int64_t n_hang, Number *y,
@@ -240,7 +294,7 @@ int launch_condense(const mf_scratch *s, Number *y, const int *stop)
     w = reinterpret_cast<const Number *>(A->d.cm_w);
   else
     w = reinterpret_cast<const Number *>(A->d.cm_w32);
-  const int bm = int(std::min<int64_t>((nm + 255) / 256, 4096));
+  const int bm = int(std::min<int64_t>((nm + 15) / 16, 16384));
   hipLaunchKernelGGL(condense_masters_kernel<Number>, dim3(bm), dim3(256), 0, s->stream,
                      A->d.cm_node, A->d.cm_ptr, A->d.cm_h, w, s->d_mask, nm, y, stop);
   MF_HIP_TRY(hipGetLastError());
