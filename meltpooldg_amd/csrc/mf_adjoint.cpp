@@ -115,6 +115,7 @@ int mf_transient_create(mf_operator *op, const mf_transient_desc *d, mf_transien
 {
   MF_REQUIRE(op && d && out, MF_ERR_ARG, "null argument");
   MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "the transient needs a heat operator");
+  MF_REQUIRE(!op->s->ad, MF_ERR_ARG, "the transient is available on structured meshes only");
   MF_REQUIRE(op->heat.law == MF_LAW_CONSTANT, MF_ERR_ARG,
              "the adjoint transient supports temperature-independent conductivity");
   MF_REQUIRE(d->n_steps >= 1 && d->dt > 0 && d->radius > 0, MF_ERR_ARG, "invalid transient");

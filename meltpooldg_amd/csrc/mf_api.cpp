@@ -208,19 +208,58 @@ static int check_ready(const mf_operator *op)
   return MF_OK;
 }
 
+int arena_reserve(Arena &a, size_t bytes)
+{
+  if (a.bytes >= bytes) return MF_OK;
+  if (a.p) MF_HIP_TRY(hipFree(a.p));  // synchronises: only when a larger problem comes along
+  a.p = nullptr;
+  a.bytes = 0;
+  MF_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&a.p), bytes));
+  a.bytes = bytes;
+  return MF_OK;
+}
+
+// Sub-allocation of an arena: reserve(sizes...) then take<T>(count) in the same order.
+struct Carve {
+  std::vector<size_t> sizes;
+  char *cur = nullptr;
+  template <typename T>
+  void plan(size_t count) { sizes.push_back((sizeof(T) * std::max<size_t>(count, 1) + 255) & ~size_t(255)); }
+  int reserve(Arena &a)
+  {
+    size_t total = 0;
+    for (size_t b : sizes) total += b;
+    int rc = arena_reserve(a, total);
+    cur = a.p;
+    return rc;
+  }
+  template <typename T>
+  T *take(size_t count)
+  {
+    T *r = reinterpret_cast<T *>(cur);
+    cur += (sizeof(T) * std::max<size_t>(count, 1) + 255) & ~size_t(255);
+    return r;
+  }
+};
+
+// dst = A src (overwrite = true, the reference's vmult with zeroed dst) or dst += A src.
+// The structured-mesh kernel overwrites in place (the first colour that writes a node stores
+// it); the other kernels accumulate into a zeroed dst.
 template <typename Number>
 static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double *dot_partial,
-                         const int *stop, bool zero)
+                         const int *stop, bool overwrite)
 {
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
-  if (zero) {
+  const bool slab = g_tangent_variant == 1 && !op->transpose && !m.ad;
+  if (overwrite && (m.ad || slab)) {
     int rc = vec_fill<Number>(dst, Number(0), m.n_dofs, st, stop);
     if (rc) return rc;
   }
   CellArgs<Number> a = make_args<Number>(op);
   a.dot_partial = dot_partial;
   a.stop = stop;
+  a.overwrite = (overwrite && !m.ad && !slab) ? 1 : 0;
   const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
   if (m.ad) {
     if (!op->gather_plain && m.ad->n_hanging > 0) {
@@ -239,7 +278,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     return launch_condense<Number>(op->s, dst, stop);
   }
   // the transposed operator (adjoint) runs on the pencil kernel
-  if (g_tangent_variant == 1 && !op->transpose)
+  if (slab)
     return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
 }
@@ -383,6 +422,8 @@ int mf_operator_destroy(mf_operator *op)
   if (op->T32) (void)hipFree(op->T32);
   if (op->vel32) (void)hipFree(op->vel32);
   if (op->kc32) (void)hipFree(op->kc32);
+  for (auto &ar : op->arena)
+    if (ar.p) (void)hipFree(ar.p);
   delete op;
   return MF_OK;
 }
@@ -461,11 +502,11 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   MF_REQUIRE(inv_diag, MF_ERR_ARG, "null vector");
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
-  rc = vec_fill<double>(inv_diag, 0.0, m.n_dofs, st);
-  if (rc) return rc;
   CellArgs<double> a = make_args<double>(op);
   const int code = op->kind == 0 ? HEAT_DIAG : ADVEC_DIAG;
   if (m.ad) {
+    rc = vec_fill<double>(inv_diag, 0.0, m.n_dofs, st);
+    if (rc) return rc;
     // cells without hanging nodes: local diagonal; cells with hanging nodes: condensed
     // diagonal c_D . (A_cell c_D) per target node D
     rc = launch_cell_list<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
@@ -473,14 +514,16 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
     const int tcode = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
     rc = launch_cell_list<double>(tcode, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
   } else {
+    a.overwrite = 1;
     rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
   }
   if (rc) return rc;
   double *work = nullptr;
-  if (op->kind == 1) MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&work), sizeof(double) * RED_BLOCKS, st));
-  rc = vec_invert_diagonal(inv_diag, m.n_dofs, op->kind == 1, work, st);
-  if (work) MF_HIP_TRY(hipFreeAsync(work, st));
-  return rc;
+  if (op->kind == 1) {
+    if ((rc = arena_reserve(op->arena[3], sizeof(double) * RED_BLOCKS))) return rc;
+    work = reinterpret_cast<double *>(op->arena[3].p);
+  }
+  return vec_invert_diagonal(inv_diag, m.n_dofs, op->kind == 1, work, st);
 }
 
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)
@@ -504,21 +547,6 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
 // ------------------------------------------------------------------------------------------
 namespace mf {
 
-template <typename T>
-struct DevBuf {
-  T *p = nullptr;
-  hipStream_t st = nullptr;
-  int alloc(size_t n, hipStream_t s)
-  {
-    st = s;
-    MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&p), sizeof(T) * std::max<size_t>(n, 1), s));
-    return MF_OK;
-  }
-  ~DevBuf()
-  {
-    if (p) (void)hipFreeAsync(p, st);
-  }
-};
 
 struct SolveOut {
   int iterations = 0;
@@ -535,32 +563,43 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
   const int ncb = tangent_blocks(m, op->transpose != 0);
-  DevBuf<T> r, p, q, prr, prz0, prz1;
-  DevBuf<double> pcell;
-  DevBuf<SolveState> sd;
-  int rc;
-  if ((rc = r.alloc(n, st)) || (rc = p.alloc(n, st)) || (rc = q.alloc(n, st)) ||
-      (rc = prr.alloc(2 * RED_BLOCKS, st)) || (rc = prz0.alloc(2 * RED_BLOCKS, st)) ||
-      (rc = prz1.alloc(2 * RED_BLOCKS, st)) || (rc = pcell.alloc(ncb, st)) ||
-      (rc = sd.alloc(1, st)))
-    return rc;
-  MF_HIP_TRY(hipMemsetAsync(sd.p, 0, sizeof(SolveState), st));
-  if ((rc = apply_tangent<T>(op, q.p, x, nullptr, nullptr, true))) return rc;
-  if ((rc = cg_init<T>(b, q.p, r.p, p.p, inv, n, prr.p, prz0.p, sd.p, abs_tol, rel_tol, st)))
-    return rc;
-  T *prz[2] = {prz0.p, prz1.p};
+  Carve cv;
+  cv.plan<T>(n);
+  cv.plan<T>(n);
+  cv.plan<T>(n);
+  for (int i = 0; i < 3; ++i) cv.plan<T>(2 * RED_BLOCKS);
+  cv.plan<double>(ncb);
+  cv.plan<SolveState>(1);
+  int rc = cv.reserve(op->arena[1]);
+  if (rc) return rc;
+  T *r = cv.take<T>(n), *p = cv.take<T>(n), *q = cv.take<T>(n);
+  T *prr = cv.take<T>(2 * RED_BLOCKS), *prz0 = cv.take<T>(2 * RED_BLOCKS),
+    *prz1 = cv.take<T>(2 * RED_BLOCKS);
+  double *pcell = cv.take<double>(ncb);
+  SolveState *sd = cv.take<SolveState>(1);
+  if ((rc = apply_tangent<T>(op, q, x, nullptr, nullptr, true))) return rc;
+  // cg_check0 initialises every field of the state block
+  if ((rc = cg_init<T>(b, q, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st))) return rc;
+  T *prz[2] = {prz0, prz1};
   SolveState h{};
   int it = 0;
-  const int chunk = 8;
+  int chunk = 8;
   for (;;) {
-    MF_HIP_TRY(hipMemcpyAsync(&h, sd.p, sizeof(SolveState), hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipMemcpyAsync(&h, sd, sizeof(SolveState), hipMemcpyDeviceToHost, st));
     MF_HIP_TRY(hipStreamSynchronize(st));
     if (h.stop) break;
+    if (h.iterations > 0 && h.res > 0 && h.res < h.res0 && h.tol > 0) {
+      // iterations left at the observed average rate (plus one), so that few launches run
+      // after convergence and few host round trips are needed
+      const double rate = std::log(h.res / h.res0) / h.iterations;
+      const double left = std::log(h.tol / h.res) / rate;
+      chunk = int(std::min(64.0, std::max(2.0, std::ceil(left) + 1.0)));
+    }
     for (int k = 0; k < chunk; ++k) {
       ++it;
-      if ((rc = apply_tangent<T>(op, q.p, p.p, pcell.p, &sd.p->stop, false))) return rc;
-      if ((rc = cg_reduce_cell_dot(pcell.p, ncb, sd.p, st))) return rc;
-      if ((rc = cg_step<T>(x, r.p, p.p, q.p, inv, n, prz, prr.p, sd.p, it, max_it, st))) return rc;
+      if ((rc = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return rc;
+      if ((rc = cg_reduce_cell_dot(pcell, ncb, sd, st))) return rc;
+      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, sd, it, max_it, st))) return rc;
     }
   }
   out.iterations = h.iterations;
@@ -581,12 +620,19 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
   const int mm = std::min(std::max(restart, 1), 30);
-  DevBuf<T> V, z, part;
-  DevBuf<double> dh;  // [0..31] h1, [32..63] h2, [64] norm, [65..96] y
-  int rc;
-  if ((rc = V.alloc(size_t(mm + 1) * n, st)) || (rc = z.alloc(n, st)) ||
-      (rc = part.alloc(size_t(32) * 2 * RED_BLOCKS, st)) || (rc = dh.alloc(128, st)))
-    return rc;
+  Carve cv;
+  cv.plan<T>(size_t(mm + 1) * n);
+  cv.plan<T>(n);
+  cv.plan<T>(size_t(32) * 2 * RED_BLOCKS);
+  cv.plan<double>(128);
+  int rc = cv.reserve(op->arena[1]);
+  if (rc) return rc;
+  struct {
+    T *p;
+  } V{cv.take<T>(size_t(mm + 1) * n)}, z{cv.take<T>(n)}, part{cv.take<T>(size_t(32) * 2 * RED_BLOCKS)};
+  struct {
+    double *p;
+  } dh{cv.take<double>(128)};  // [0..31] h1, [32..63] h2, [64] norm, [65..96] y
   std::vector<double> H(size_t(mm + 1) * mm, 0.0), cs(mm), sn(mm), gam(mm + 1), y(mm);
   double hbuf[128];
   double tol = 0.0, res = 0.0, res0 = 0.0;
@@ -695,9 +741,13 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
   const int max_it = desc->max_iterations > 0 ? desc->max_iterations : 10000;
-  DevBuf<double> inv;
+  struct {
+    double *p = nullptr;
+  } inv;
   if (desc->preconditioner == MF_PRECOND_JACOBI) {
-    if ((rc = inv.alloc(n, st))) return rc;
+    if ((rc = arena_reserve(op->arena[2], sizeof(double) * std::max<int64_t>(n, RED_BLOCKS))))
+      return rc;
+    inv.p = reinterpret_cast<double *>(op->arena[2].p);
     if ((rc = mf_operator_compute_inverse_diagonal(op, inv.p))) return rc;
   }
   mf_solve_report rep{};
@@ -711,16 +761,28 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
     rep.status = o.status;
   } else {
     // FP32 inner Krylov inside FP64 defect correction (iterative refinement)
-    DevBuf<double> r, q, nrm;
-    DevBuf<double> part;
-    DevBuf<float> r32, e32, inv32;
-    if ((rc = r.alloc(n, st)) || (rc = q.alloc(n, st)) || (rc = nrm.alloc(1, st)) ||
-        (rc = part.alloc(2 * RED_BLOCKS, st)) || (rc = r32.alloc(n, st)) ||
-        (rc = e32.alloc(n, st)))
-      return rc;
+    Carve cv;
+    cv.plan<double>(n);
+    cv.plan<double>(n);
+    cv.plan<double>(1);
+    cv.plan<double>(2 * RED_BLOCKS);
+    cv.plan<float>(n);
+    cv.plan<float>(n);
+    cv.plan<float>(n);
+    if ((rc = cv.reserve(op->arena[0]))) return rc;
+    struct D {
+      double *p;
+    };
+    struct F {
+      float *p;
+    };
+    D r{cv.take<double>(n)}, q{cv.take<double>(n)}, nrm{cv.take<double>(1)},
+        part{cv.take<double>(2 * RED_BLOCKS)};
+    F r32{cv.take<float>(n)}, e32{cv.take<float>(n)}, inv32{cv.take<float>(n)};
     if (inv.p) {
-      if ((rc = inv32.alloc(n, st))) return rc;
       if ((rc = vec_cast<double, float>(inv.p, inv32.p, 1.0, n, st))) return rc;
+    } else {
+      inv32.p = nullptr;
     }
     const double inner_rel = desc->inner_rel_tol > 0 ? desc->inner_rel_tol : 1e-3;
     const int max_outer = desc->max_outer > 0 ? desc->max_outer : 50;

@@ -167,6 +167,7 @@ struct GridAccess {
   int c, ncx, cellrow, nsub;
   int cx, cell;
   bool active;
+  bool first;    // this colour is the first to write the pencil's nodes
   Number carry;  // output of the previous step's last cell at the node it shares with cell 0
 
   __device__ GridAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p)
@@ -179,6 +180,13 @@ struct GridAccess {
     nsub = (ncx + CPB - 1) / CPB;
     rowbase = A.NX * (uint32_t(cy * P + p % K) + A.NY * uint32_t(cz * P + p / K));
     cellrow = ncx * (cy + A.ncell[1] * cz);
+    // A node on the face between rows cy and cy+1 (or layers cz, cz+1) is written by both
+    // parities; the launch order is colour 0..3 = (y parity) + 2 (z parity), so the first
+    // writer of a shared node is the even one.
+    const int j = p % K, k = p / K;
+    const bool shared_y = (j == 0 && cy > 0) || (j == P && cy < A.ncell[1] - 1);
+    const bool shared_z = (k == 0 && cz > 0) || (k == P && cz < A.ncell[2] - 1);
+    first = (!shared_y || rc.yb == 0) && (!shared_z || rc.zb == 0);
     carry = Number(0);
     at(0);
   }
@@ -203,12 +211,14 @@ struct GridAccess {
   {
     return (active && !masked(A, gbase + i)) ? f[gbase + i] : Number(0);
   }
-  // current dst entries of the pencil; rows of one colour own their nodes during the launch,
-  // so these can be read before the cell is computed
-  __device__ void prefetch(const Number *dst, Number (&d)[K]) const
+  // current dst entries of the pencil (not needed where this launch is the first writer of an
+  // overwriting application); rows of one colour own their nodes during the launch, so these
+  // can be read before the cell is computed
+  __device__ void prefetch(const CellArgs<Number> &A, const Number *dst, Number (&d)[K]) const
   {
+    const bool need = active && !(A.overwrite && first);
 #pragma unroll
-    for (int i = 0; i < K; ++i) d[i] = active ? dst[gbase + i] : Number(0);
+    for (int i = 0; i < K; ++i) d[i] = need ? dst[gbase + i] : Number(0);
   }
   // Add the K outputs of the x pencil into dst.  Cell c-1 of the step sits in the previous
   // lane, so its last node (shared with this cell's first) is fetched with a shuffle and added
@@ -228,11 +238,15 @@ struct GridAccess {
     carry = wrap;
     if (!active) return;
     const bool last = (cx == ncx - 1);
+    const bool store = A.overwrite && first;  // first writer of an overwriting application
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       if (i == K - 1 && !last) break;
-      if (masked(A, gbase + i)) continue;
-      dst[gbase + i] = d[i] + o[i];
+      if (masked(A, gbase + i)) {
+        if (store) dst[gbase + i] = Number(0);  // constrained rows of dst = A src are zero
+        continue;
+      }
+      dst[gbase + i] = store ? o[i] : d[i] + o[i];
     }
   }
 };
@@ -299,7 +313,7 @@ struct ListAccess {
     if ((hbits >> i) & 1u) return A.hang_val[A.node_line[n]];
     return masked(A, n) ? Number(0) : f[n];
   }
-  __device__ void prefetch(const Number *dst, Number (&d)[K]) const
+  __device__ void prefetch(const CellArgs<Number> &, const Number *dst, Number (&d)[K]) const
   {
 #pragma unroll
     for (int i = 0; i < K; ++i) d[i] = ((abits >> i) & 1u) ? dst[node[i]] : Number(0);
@@ -367,7 +381,7 @@ struct SlotAccess {
   __device__ Number hscale() const { return ldexp(Number(1), -level); }
   __device__ Number plain(const Number *f, int i) const { return active ? f[nodes[i]] : Number(0); }
   __device__ Number value(const CellArgs<Number> &, const Number *, int i) const { return cv[i]; }
-  __device__ void prefetch(const Number *, Number (&d)[K]) const
+  __device__ void prefetch(const CellArgs<Number> &, const Number *, Number (&d)[K]) const
   {
 #pragma unroll
     for (int i = 0; i < K; ++i) d[i] = Number(0);
@@ -752,7 +766,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     continue;
   }
 
-  if constexpr (!PGRAD) ac.prefetch(dst, dpre);
+  if constexpr (!PGRAD) ac.prefetch(A, dst, dpre);
 
   if constexpr (IS_DIAG) {
     __syncthreads();

@@ -93,6 +93,7 @@ struct CellArgs {
   const uint32_t *slot_cell;   // condensed diagonal: (cell, target) per slot
   const uint32_t *slot_tgt;
   const Number *hang_val;      // [n_hanging] interpolated source values of the hanging nodes
+  int overwrite;               // structured meshes: dst = A src (first writer of a node stores)
 };
 
 struct AdaptiveData;
@@ -215,7 +216,16 @@ struct mf_scratch {
   mf::AdaptiveData *ad = nullptr;  // adaptive mesh (null: structured)
 };
 
+namespace mf {
+// device memory an operator keeps between solves (grown on demand, freed with the operator)
+struct Arena {
+  char *p = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace mf
+
 struct mf_operator {
+  mf::Arena arena[4];  // 0: refinement vectors, 1: Krylov vectors, 2: inverse diagonal, 3: small work
   int kind = 0;  // 0 heat, 1 advection
   mf_scratch *s = nullptr;
   mf_heat_params heat{};

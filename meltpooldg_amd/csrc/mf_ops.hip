@@ -344,6 +344,8 @@ int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *
 static int laser_impl(mf_scratch *s, double power, double radius, const double position[3],
                       double *q_s, int d_dr)
 {
+  MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
+  MF_REQUIRE(!s->ad, MF_ERR_ARG, "the laser source is available on structured meshes only");
   MF_REQUIRE(s && position && q_s, MF_ERR_ARG, "null argument");
   MF_REQUIRE(radius > 0.0, MF_ERR_ARG, "The laser beam radius must be greater than zero! Abort..");
   const MeshInfo &m = s->mesh;
@@ -380,6 +382,7 @@ int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double r
 int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result)
 {
   MF_REQUIRE(s && u && result, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(!s->ad, MF_ERR_ARG, "the FE norm is available on structured meshes only");
   const MeshInfo &m = s->mesh;
   CellArgs<double> a;
   std::memset(&a, 0, sizeof(a));

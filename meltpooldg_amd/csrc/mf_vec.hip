@@ -190,7 +190,7 @@ __global__ void invert_diag_kernel(double *d, int64_t n, double abs_thr, double 
 // ------------------------------------------------------------------------------------------
 // fused PCG (deal.II SolverCG semantics with ReductionControl)
 // ------------------------------------------------------------------------------------------
-// After q = A x:  r = b - q, z = D r, p = z, partial r.r and r.z (slot 0), then q = 0.
+// After q = A x:  r = b - q, z = D r, p = z, partial r.r and r.z (slot 0).
 template <typename T>
 __global__ void cg_init_kernel(const T *b, T *q, T *r, T *p, const T *inv, int64_t n,
                                T *part_rr, T *part_rz0)
@@ -202,7 +202,6 @@ __global__ void cg_init_kernel(const T *b, T *q, T *r, T *p, const T *inv, int64
     const T z = inv ? inv[i] * v : v;
     r[i] = v;
     p[i] = z;
-    q[i] = 0;
     arr.add(v * v);
     arz.add(v * z);
   }
@@ -222,6 +221,9 @@ __global__ void cg_check0_kernel(const T *part_rr, int nparts, SolveState *st, d
     st->res = res;
     st->tol = fmax(abs_tol, rel_tol * res);
     st->iterations = 0;
+    st->breakdown = 0;
+    st->pAp = 0.0;
+    st->alpha = 0.0;
     st->stop = (res <= st->tol) ? 1 : (isnan(res) ? 3 : 0);
   }
 }
@@ -236,9 +238,10 @@ __global__ void reduce_cell_dot_kernel(const double *part, int n, SolveState *st
   if (threadIdx.x == 0) st->pAp = a.value();
 }
 
-// alpha = rz/pAp; x += alpha p; r -= alpha q; partial r.r, r.(D r) into slot it%2
+// alpha = rz/pAp; r -= alpha q; partial r.r, r.(D r) into slot it%2.  x is updated by the
+// direction kernel, which reads p anyway.
 template <typename T>
-__global__ void cg_update_kernel(T *x, T *r, const T *p, const T *q, const T *inv, int64_t n,
+__global__ void cg_update_kernel(T *r, const T *q, const T *inv, int64_t n,
                                  const T *part_rz_old, T *part_rz_new, T *part_rr, int nparts,
                                  SolveState *st)
 {
@@ -250,10 +253,10 @@ __global__ void cg_update_kernel(T *x, T *r, const T *p, const T *q, const T *in
     return;
   }
   const T alpha = T(double(rz) / pAp);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = double(alpha);
   Acc<T> arr, arz;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    x[i] += alpha * p[i];
     const T v = r[i] - alpha * q[i];
     r[i] = v;
     arr.add(v * v);
@@ -263,9 +266,10 @@ __global__ void cg_update_kernel(T *x, T *r, const T *p, const T *q, const T *in
   write_partial(part_rz_new, nparts, arz);
 }
 
-// convergence check for step `it`, then p = D r + beta p, q = 0
+// x += alpha p (this iteration's step), convergence check for step `it`, then
+// p = D r + beta p
 template <typename T>
-__global__ void cg_direction_kernel(T *p, T *q, const T *r, const T *inv, int64_t n,
+__global__ void cg_direction_kernel(T *x, T *p, const T *r, const T *inv, int64_t n,
                                     const T *part_rz_new, const T *part_rz_old, const T *part_rr,
                                     int nparts, SolveState *st, int it, int max_it)
 {
@@ -277,13 +281,17 @@ __global__ void cg_direction_kernel(T *p, T *q, const T *r, const T *inv, int64_
     }
     return;
   }
+  const T alpha = T(st->alpha);
   const T rr = reduce_partials(part_rr, nparts);
   const double res = sqrt(double(rr));
   const double tol = st->tol;
   const bool done = res <= tol;
   const bool fail = !done && (it >= max_it || isnan(res));
   if (done || fail) {
-    // every block sees the same values; block 0 records, all blocks skip the update
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x)
+      x[i] += alpha * p[i];
+    // every block sees the same values; block 0 records
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->res = res;
       st->iterations = it;
@@ -296,9 +304,10 @@ __global__ void cg_direction_kernel(T *p, T *q, const T *r, const T *inv, int64_
   const T beta = T(double(rz_new) / double(rz_old));
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
+    const T pi = p[i];
+    x[i] += alpha * pi;
     const T z = inv ? inv[i] * r[i] : r[i];
-    p[i] = z + beta * p[i];
-    q[i] = 0;
+    p[i] = z + beta * pi;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->res = res;
@@ -536,10 +545,10 @@ int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *p
             SolveState *sd, int it, int max_it, hipStream_t st)
 {
   T *rz_new = part_rz[it & 1], *rz_old = part_rz[(it - 1) & 1];
-  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, x, r, p, q,
-                     inv, n, rz_old, rz_new, part_rr, RED_BLOCKS, sd);
+  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, r, q, inv,
+                     n, rz_old, rz_new, part_rr, RED_BLOCKS, sd);
   MF_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, p, q, r,
+  hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, x, p, r,
                      inv, n, rz_new, rz_old, part_rr, RED_BLOCKS, sd, it, max_it);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;

@@ -16,6 +16,7 @@ struct SolveState {
   int pad;
   double res0, tol, res;
   double pAp;      // reduced p.(A p) of the current iteration
+  double alpha;    // step length of the current iteration (x is updated one kernel later)
 };
 
 template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop = nullptr);
