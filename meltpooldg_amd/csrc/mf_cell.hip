@@ -94,19 +94,39 @@ __device__ __forceinline__ void sweep(const Number *M, const Number *in, Number 
   store_pencil<Number, K, CPB, DIR>(out, p, c, o);
 }
 
-// tanh((T-Tc)/w) and sech^2 from one expm1: t = -em/(2+em), s2 = 4(1+em)/(2+em)^2 with
-// em = expm1(-2|x|).  Beyond |x| = 20 (double) / 10 (float) tanh is +-1 to the last bit and the
-// sech^2 term is below 1e-16 (1e-8) of the dk/dT scale, so it is taken as 0.
+// tanh(x) and sech^2(x) from e = exp(-2|x|): tanh|x| = (1 - e)/(1 + e), sech^2 x = 4e/(1 + e)^2.
+// 1/(1 + e) lies in [1/2, 1], so the hardware reciprocal estimate refined by Newton steps (two
+// in double, one in float) is accurate to the last bit or two without special cases.  For
+// small |x| the (1 - e) cancellation costs relative accuracy of tanh but not absolute accuracy,
+// which is what k = k0 + a tanh needs.  Beyond |x| = 20 (double) / 10 (float) tanh is +-1 to the
+// last bit and the sech^2 term is below 1e-16 (1e-8) of the dk/dT scale, so it is taken as 0.
+__device__ __forceinline__ double fast_rcp(double d)
+{
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(fma(-d, r, 1.0), r, r);
+  return fma(fma(-d, r, 1.0), r, r);
+}
+__device__ __forceinline__ float fast_rcp(float d)
+{
+  const float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+__device__ __forceinline__ double exp_neg2(double ax) { return exp(-2.0 * ax); }
+__device__ __forceinline__ float exp_neg2(float ax)
+{
+  return __builtin_amdgcn_exp2f(-2.8853900817779268f * ax);  // 2 log2(e)
+}
+
 template <typename Number>
 __device__ __forceinline__ void tanh_sech2(Number x, Number &t, Number &s2)
 {
   const Number ax = fabs(x);
   const Number cut = sizeof(Number) == 8 ? Number(20) : Number(10);
   if (ax < cut) {
-    const Number em = expm1(Number(-2) * ax);
-    const Number inv = Number(1) / (Number(2) + em);
-    t = -em * inv;
-    s2 = Number(4) * (Number(1) + em) * inv * inv;
+    const Number e = exp_neg2(ax);
+    const Number inv = fast_rcp(Number(1) + e);
+    t = (Number(1) - e) * inv;
+    s2 = Number(4) * e * inv * inv;
     if (x < 0) t = -t;
   } else {
     t = x < 0 ? Number(-1) : Number(1);
