@@ -85,6 +85,9 @@ def main():
     if args.pmc_workload:
         for _ in range(5):
             op.vmult_add(dst, src)
+        src32, dst32 = src.float(), torch.zeros(n, dtype=torch.float32, device="cuda")
+        for _ in range(5):
+            op.vmult_f32(dst32, src32)
         other = torch.empty_like(src)
         for _ in range(5):
             other.copy_(src)
@@ -125,32 +128,28 @@ def main():
     K = 4
     alg_flop = flops_per_cell(K) * s.n_cells
     achieved_tflops = alg_flop / (kernel_ms * 1e-3) / 1e12
+    # the FP32 operator (inner solves of the refined solve)
+    src32, dst32 = src.float(), torch.zeros(n, dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        op.vmult_f32(dst32, src32)
+    f32_ms = timed(lambda d, s_: op.vmult_f32(dst32, src32), args.steps)
     print(f"# vmult {ms_per_step*1e3:.1f} us/step, cell kernel {kernel_ms*1e3:.1f} us, "
-          f"{achieved_gbs:.0f} GB/s, {achieved_tflops:.2f} TFLOP/s", file=sys.stderr, flush=True)
+          f"{achieved_gbs:.0f} GB/s, {achieved_tflops:.2f} TFLOP/s; FP32 vmult {f32_ms*1e3:.1f} us",
+          file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_solve:
-        # Jacobi-CG to 1e-10: plain FP64 and FP32-inner / FP64-refined
-        x64 = torch.zeros(n, dtype=torch.float64, device="cuda")
-        mf.LinearSolver.solve(op, x64, src, solver="CG", rel_tolerance=1e-10)  # warm-up
-        x64.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rep64 = mf.LinearSolver.solve(op, x64, src, solver="CG", rel_tolerance=1e-10)
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t0
-        x32 = torch.zeros(n, dtype=torch.float64, device="cuda")
-        mf.LinearSolver.solve(op, x32, src, solver="CG", rel_tolerance=1e-10,
-                              precision="fp32_refined")
-        x32.zero_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rep32 = mf.LinearSolver.solve(op, x32, src, solver="CG", rel_tolerance=1e-10,
-                                      precision="fp32_refined", inner_rel_tolerance=1e-3)
-        torch.cuda.synchronize()
-        t32 = time.perf_counter() - t0
+        # Jacobi-CG to 1e-10: plain FP64 and FP32-inner / FP64-refined, with the Jacobi
+        # preconditioner updated once beforehand (JacobiPreconditioner::update), as the reference
+        # hands an updated preconditioner to LinearSolver::solve
+        pc, t_pc = _jacobi(mf, torch, op)
+        x64, rep64, t64 = _timed_solve(mf, torch, op, src, solver="CG", rel_tolerance=1e-10,
+                                       preconditioner=pc)
+        x32, rep32, t32 = _timed_solve(mf, torch, op, src, solver="CG", rel_tolerance=1e-10,
+                                       precision="fp32_refined", preconditioner=pc)
         err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
         extra["solve"] = {
+            "jacobi_update_s": t_pc,
             "cg_fp64_s": t64, "cg_fp64_iterations": rep64["iterations"],
             "cg_fp64_rel_residual": rep64["final_residual"] / rep64["initial_residual"],
             "cg_fp32_refined_s": t32, "cg_fp32_refined_inner_iterations": rep32["iterations"],
@@ -192,12 +191,24 @@ def main():
                      "kernel_us": kernel_ms * 1e3, "alg_bytes_per_launch": alg_bytes,
                      "fp64_tflops": achieved_tflops,
                      "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "fp32_vmult_us": f32_ms * 1e3,
                      "tangent_kernel": ["pencil", "slab"][best],
                      "kernel_probe_us": {"pencil": variants[0] * 1e3, "slab": variants[1] * 1e3}},
         "cpu_baseline": cpu,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def _jacobi(mf, torch, op):
+    """JacobiPreconditioner of op, updated; returns it and the update time."""
+    pc = mf.JacobiPreconditioner(op)
+    pc.update()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pc.update()
+    torch.cuda.synchronize()
+    return pc, time.perf_counter() - t0
 
 
 def _timed_solve(mf, torch, op, b, **kw):
@@ -225,7 +236,8 @@ def adaptive_case(mf, torch, stream):
     """Mixed-precision heat solve on an adaptive mesh: 32^3 base cells refined twice in the band
     | |x-(0.5,0.5,0.5)| - 0.3 | < 0.05 (2:1 balanced, hanging nodes), FE_Q3, the tanh Jacobian at
     the same linearisation point as the main workload, seeded U(-1,1) RHS (numpy seed 4, zero on
-    constrained rows).  FP64 Jacobi-CG and FP32-inner (rtol 1e-3) / FP64-refined CG to 1e-10."""
+    constrained rows).  FP64 Jacobi-CG and FP32-inner / FP64-refined CG to 1e-10 (inner tolerance
+    sqrt(1e-10) = 1e-5)."""
     t0 = time.perf_counter()
     cells = mf.refine_band_cells(32, 2, (0.5, 0.5, 0.5), 0.3, 0.05)
     s = mf.AdaptiveScratchData(3, 32, cells, stream=stream)
@@ -245,15 +257,18 @@ def adaptive_case(mf, torch, stream):
     bd = torch.tensor(b, dtype=torch.float64, device="cuda")
     y = torch.zeros_like(bd)
     t_vm = _time_vmult(torch, op, bd, y, 20)
-    x64, r64, t64 = _timed_solve(mf, torch, op, bd, solver="CG", rel_tolerance=1e-10)
+    pc, t_pc = _jacobi(mf, torch, op)
+    x64, r64, t64 = _timed_solve(mf, torch, op, bd, solver="CG", rel_tolerance=1e-10,
+                                 preconditioner=pc)
     x32, r32, t32 = _timed_solve(mf, torch, op, bd, solver="CG", rel_tolerance=1e-10,
-                                 precision="fp32_refined", inner_rel_tolerance=1e-3)
+                                 precision="fp32_refined", preconditioner=pc)
     err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     out = {"workload": "FE_Q3 tanh heat Jacobian on a band-refined 32^3 mesh (+2 levels, "
                        "hanging nodes)",
            "n_cells": int(len(cells)), "n_dofs": int(s.n_dofs),
            "n_hanging": info["n_hanging"], "n_colours": info["n_colours"],
            "setup_s": setup_s, "vmult_us": t_vm * 1e6, "vmult_dofs_per_s": s.n_dofs / t_vm,
+           "jacobi_update_s": t_pc,
            "cg_fp64_s": t64, "cg_fp64_iterations": r64["iterations"],
            "cg_fp32_refined_s": t32, "cg_fp32_refined_inner_iterations": r32["iterations"],
            "cg_fp32_refined_outer": r32["outer_iterations"],
@@ -296,13 +311,15 @@ def levelset_case(mf, torch, stream):
     op.create_rhs(rhs, phi_d)
     y_ = torch.zeros_like(phi_d)
     t_vm = _time_vmult(torch, op, phi_d, y_, 20)
-    x64, r64, t64 = _timed_solve(mf, torch, op, rhs, solver="GMRES", rel_tolerance=1e-10)
+    pc, t_pc = _jacobi(mf, torch, op)
+    x64, r64, t64 = _timed_solve(mf, torch, op, rhs, solver="GMRES", rel_tolerance=1e-10,
+                                 preconditioner=pc)
     x32, r32, t32 = _timed_solve(mf, torch, op, rhs, solver="GMRES", rel_tolerance=1e-10,
-                                 precision="fp32_refined", inner_rel_tolerance=1e-3)
+                                 precision="fp32_refined", preconditioner=pc)
     err = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     out = {"workload": "FE_Q2 level-set advection-diffusion, 96^3 cells, BDF2 tangent, "
                        "deformation velocity, GMRES(30)",
-           "n_dofs": int(s.n_dofs), "dt": dt, "vmult_us": t_vm * 1e6,
+           "n_dofs": int(s.n_dofs), "dt": dt, "vmult_us": t_vm * 1e6, "jacobi_update_s": t_pc,
            "vmult_dofs_per_s": s.n_dofs / t_vm,
            "gmres_fp64_s": t64, "gmres_fp64_iterations": r64["iterations"],
            "gmres_fp32_refined_s": t32, "gmres_fp32_refined_inner_iterations": r32["iterations"],

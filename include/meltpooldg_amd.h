@@ -184,7 +184,8 @@ typedef struct {
   double rel_tol;
   int preconditioner; /* MF_PRECOND_* */
   int precision;      /* MF_PRECISION_FP64 or MF_PRECISION_FP32_REFINED */
-  double inner_rel_tol; /* FP32-refined: relative tolerance of each inner FP32 solve (0 = 1e-3) */
+  double inner_rel_tol; /* FP32-refined: relative tolerance of each inner FP32 solve (0 = auto:
+                           sqrt(rel_tol) clamped to [1e-5, 1e-2], 2 passes for 1e-10) */
   int max_outer;      /* FP32-refined: maximum defect-correction passes (0 = 50) */
 } mf_solver_desc;
 
@@ -200,6 +201,24 @@ typedef struct {
  * recomputed from the operator (Preconditioner::update) when precond = JACOBI. */
 int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
              mf_solve_report *report);
+
+/* JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89):
+ * create = constructor + reinit (device storage for the inverse diagonal), update = update()
+ * (compute_inverse_diagonal_from_matrixfree of the operator it was created for), vmult =
+ * vmult() (dst = D^-1 src).  The object keeps an FP32 copy for the refined solver.  Not
+ * re-entrant; it must be updated again when the operator's fields, dt or constraints change. */
+typedef struct mf_jacobi mf_jacobi;
+int mf_jacobi_create(mf_operator *op, mf_jacobi **out);
+int mf_jacobi_update(mf_jacobi *pc);
+int mf_jacobi_vmult(mf_jacobi *pc, double *dst, const double *src);
+/* copy of the current inverse diagonal (device pointer dst of n_dofs entries) */
+int mf_jacobi_inverse_diagonal(mf_jacobi *pc, double *dst);
+int mf_jacobi_destroy(mf_jacobi *pc);
+
+/* LinearSolver::solve(op, x, b, data, preconditioner) (linear_solver.hpp:22-134) with a
+ * preconditioner updated beforehand (pc = NULL: identity); desc->preconditioner is ignored. */
+int mf_solve_preconditioned(mf_operator *op, const mf_solver_desc *desc, mf_jacobi *pc,
+                            double *x, const double *b, mf_solve_report *report);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Callers on either side of the operator                                                      */

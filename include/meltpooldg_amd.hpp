@@ -173,6 +173,24 @@ class AdvectionDiffusionOperator : public OperatorMatrixFree {
   }
 };
 
+// JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)
+class JacobiPreconditioner {
+ public:
+  explicit JacobiPreconditioner(const OperatorMatrixFree &op) { check(mf_jacobi_create(op.handle(), &h_)); }
+  JacobiPreconditioner(const JacobiPreconditioner &) = delete;
+  JacobiPreconditioner &operator=(const JacobiPreconditioner &) = delete;
+  ~JacobiPreconditioner() { mf_jacobi_destroy(h_); }
+  void update() { check(mf_jacobi_update(h_)); }
+  void vmult(DeviceVector<double> &dst, const DeviceVector<double> &src) const
+  {
+    check(mf_jacobi_vmult(h_, dst.data(), src.data()));
+  }
+  mf_jacobi *handle() const { return h_; }
+
+ private:
+  mf_jacobi *h_ = nullptr;
+};
+
 // LinearSolverData (include/meltpooldg/linear_algebra/linear_solver_data.hpp:37-51)
 struct LinearSolverData {
   int solver_type = MF_SOLVER_GMRES;
@@ -181,7 +199,7 @@ struct LinearSolverData {
   double rel_tolerance = 1e-12;
   double abs_tolerance = 1e-20;
   int precision = MF_PRECISION_FP64;
-  double inner_rel_tolerance = 1e-3;
+  double inner_rel_tolerance = 0.0;  // 0: sqrt(rel_tolerance) in [1e-5, 1e-2]
 };
 
 struct LinearSolver {
@@ -201,6 +219,27 @@ struct LinearSolver {
     d.inner_rel_tol = data.inner_rel_tolerance;
     mf_solve_report rep{};
     const int code = mf_solve(op.handle(), &d, solution.data(), rhs.data(), &rep);
+    if (report) *report = rep;
+    check(code);
+    return rep.iterations;
+  }
+  // with a preconditioner updated by the caller (LinearSolver::solve(..., preconditioner))
+  static int solve(OperatorMatrixFree &op, DeviceVector<double> &solution,
+                   const DeviceVector<double> &rhs, const LinearSolverData &data,
+                   const JacobiPreconditioner &preconditioner, mf_solve_report *report = nullptr)
+  {
+    mf_solver_desc d{};
+    d.solver = data.solver_type;
+    d.restart = 30;
+    d.max_iterations = int(data.max_iterations);
+    d.abs_tol = data.abs_tolerance;
+    d.rel_tol = data.rel_tolerance;
+    d.preconditioner = MF_PRECOND_JACOBI;
+    d.precision = data.precision;
+    d.inner_rel_tol = data.inner_rel_tolerance;
+    mf_solve_report rep{};
+    const int code = mf_solve_preconditioned(op.handle(), &d, preconditioner.handle(),
+                                             solution.data(), rhs.data(), &rep);
     if (report) *report = rep;
     check(code);
     return rep.iterations;

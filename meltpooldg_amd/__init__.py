@@ -152,6 +152,13 @@ def lib():
         L.mf_operator_compute_inverse_diagonal.argtypes = [vp, vp]
         L.mf_operator_create_rhs.argtypes = [vp, vp, vp, vp]
         L.mf_solve.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, C.POINTER(SolveReport)]
+        L.mf_jacobi_create.argtypes = [vp, C.POINTER(vp)]
+        L.mf_jacobi_update.argtypes = [vp]
+        L.mf_jacobi_vmult.argtypes = [vp, vp, vp]
+        L.mf_jacobi_inverse_diagonal.argtypes = [vp, vp]
+        L.mf_jacobi_destroy.argtypes = [vp]
+        L.mf_solve_preconditioned.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, vp,
+                                              C.POINTER(SolveReport)]
         L.mf_set_tangent_kernel.argtypes = [C.c_int]
         L.mf_heat_newton_step.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolverDesc),
                                           C.POINTER(NewtonDesc), C.POINTER(NewtonReport)]
@@ -456,7 +463,7 @@ class AdvectionDiffusionOperator(_OperatorBase):
 
 def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
                  rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30, precision="fp64",
-                 inner_rel_tolerance=1e-3, max_outer=50):
+                 inner_rel_tolerance=0.0, max_outer=50):
     d = SolverDesc()
     d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
     d.restart = restart
@@ -568,25 +575,62 @@ class HeatTransient:
             pass
 
 
+class JacobiPreconditioner:
+    """Mirror of MeltPoolDG::JacobiPreconditioner (preconditioner_jacobi.hpp:29-89): update()
+    computes the inverse diagonal of the operator once; solves given this object reuse it."""
+
+    def __init__(self, operator):
+        self.operator = operator
+        h = C.c_void_p()
+        _check(lib().mf_jacobi_create(operator._h, C.byref(h)))
+        self._h = h
+
+    def update(self):
+        _check(lib().mf_jacobi_update(self._h))
+
+    def vmult(self, dst, src):
+        _check(lib().mf_jacobi_vmult(self._h, _ptr(dst), _ptr(src)))
+
+    def inverse_diagonal(self, dst):
+        _check(lib().mf_jacobi_inverse_diagonal(self._h, _ptr(dst)))
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mf_jacobi_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 class LinearSolver:
-    """Mirror of MeltPoolDG::LinearSolver::solve (linear_solver.hpp:22-134)."""
+    """Mirror of MeltPoolDG::LinearSolver::solve (linear_solver.hpp:22-134).  `preconditioner` is
+    "Diagonal" (Jacobi, recomputed in the call), "Identity", or an updated
+    JacobiPreconditioner."""
 
     @staticmethod
     def solve(operator, solution, rhs, solver="CG", preconditioner="Diagonal",
               max_iterations=10000, rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30,
-              precision="fp64", inner_rel_tolerance=1e-3, max_outer=50, raise_on_failure=True):
+              precision="fp64", inner_rel_tolerance=0.0, max_outer=50, raise_on_failure=True):
         d = SolverDesc()
         d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
         d.restart = restart
         d.max_iterations = max_iterations
         d.abs_tol = abs_tolerance
         d.rel_tol = rel_tolerance
-        d.preconditioner = {"Identity": PRECOND_IDENTITY, "Diagonal": PRECOND_JACOBI}[preconditioner]
         d.precision = {"fp64": PRECISION_FP64, "fp32_refined": PRECISION_FP32_REFINED}[precision]
         d.inner_rel_tol = inner_rel_tolerance
         d.max_outer = max_outer
         rep = SolveReport()
-        code = lib().mf_solve(operator._h, C.byref(d), _ptr(solution), _ptr(rhs), C.byref(rep))
+        if isinstance(preconditioner, JacobiPreconditioner):
+            d.preconditioner = PRECOND_JACOBI
+            code = lib().mf_solve_preconditioned(operator._h, C.byref(d), preconditioner._h,
+                                                 _ptr(solution), _ptr(rhs), C.byref(rep))
+        else:
+            d.preconditioner = {"Identity": PRECOND_IDENTITY,
+                                "Diagonal": PRECOND_JACOBI}[preconditioner]
+            code = lib().mf_solve(operator._h, C.byref(d), _ptr(solution), _ptr(rhs),
+                                  C.byref(rep))
         if code != MF_OK and (raise_on_failure or code not in (MF_NO_CONVERGENCE, MF_BREAKDOWN)):
             _check(code)
         return rep.as_dict()

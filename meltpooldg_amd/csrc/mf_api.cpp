@@ -562,7 +562,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
-  const int ncb = tangent_blocks(m, op->transpose != 0);
+  const int ncb = tangent_blocks(m, op->transpose != 0, sizeof(T) == 4);
   Carve cv;
   cv.plan<T>(n);
   cv.plan<T>(n);
@@ -729,27 +729,33 @@ static int krylov(mf_operator *op, const mf_solver_desc &d, T *x, const T *b, co
 
 }  // namespace mf
 
-extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
-                        mf_solve_report *report)
+// Jacobi preconditioner object: the inverse diagonal of an operator, computed by update()
+// (JacobiPreconditioner, preconditioner_jacobi.hpp:29-89), with its FP32 copy for the inner
+// solves of the refined solver.
+struct mf_jacobi {
+  mf_operator *op = nullptr;
+  int64_t n = 0;
+  double *inv = nullptr;
+  float *inv32 = nullptr;
+  bool valid = false;
+};
+
+namespace mf {
+
+// LinearSolver::solve with the inverse diagonal `inv_in` (null: identity) and, optionally, its
+// FP32 copy (null: made here when the refined solver needs it)
+static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double *inv_in,
+                      const float *inv32_in, double *x, const double *b, mf_solve_report *report)
 {
-  int rc = check_ready(op);
-  if (rc) return rc;
-  MF_REQUIRE(desc && x && b, MF_ERR_ARG, "null argument");
-  MF_REQUIRE(desc->solver == MF_SOLVER_CG || desc->solver == MF_SOLVER_GMRES, MF_ERR_ARG,
-             "unknown solver type");
+  int rc = MF_OK;
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
   const int max_it = desc->max_iterations > 0 ? desc->max_iterations : 10000;
   struct {
-    double *p = nullptr;
+    const double *p = nullptr;
   } inv;
-  if (desc->preconditioner == MF_PRECOND_JACOBI) {
-    if ((rc = arena_reserve(op->arena[2], sizeof(double) * std::max<int64_t>(n, RED_BLOCKS))))
-      return rc;
-    inv.p = reinterpret_cast<double *>(op->arena[2].p);
-    if ((rc = mf_operator_compute_inverse_diagonal(op, inv.p))) return rc;
-  }
+  inv.p = inv_in;
   mf_solve_report rep{};
   if (desc->precision == MF_PRECISION_FP64) {
     SolveOut o;
@@ -779,12 +785,16 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
     D r{cv.take<double>(n)}, q{cv.take<double>(n)}, nrm{cv.take<double>(1)},
         part{cv.take<double>(2 * RED_BLOCKS)};
     F r32{cv.take<float>(n)}, e32{cv.take<float>(n)}, inv32{cv.take<float>(n)};
-    if (inv.p) {
+    const float *inv32p = inv32_in;
+    if (inv.p && !inv32p) {
       if ((rc = vec_cast<double, float>(inv.p, inv32.p, 1.0, n, st))) return rc;
-    } else {
-      inv32.p = nullptr;
+      inv32p = inv32.p;
     }
-    const double inner_rel = desc->inner_rel_tol > 0 ? desc->inner_rel_tol : 1e-3;
+    // inner tolerance: by default the square root of the outer one (two passes), but not below
+    // 1e-5, about where FP32 Krylov iterates stop improving the FP64 residual
+    const double inner_rel = desc->inner_rel_tol > 0
+                                 ? desc->inner_rel_tol
+                                 : std::min(1e-2, std::max(1e-5, std::sqrt(std::max(desc->rel_tol, 0.0))));
     const int max_outer = desc->max_outer > 0 ? desc->max_outer : 50;
     auto residual = [&](double &res) -> int {
       int c;
@@ -807,7 +817,7 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
       if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
       if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
       SolveOut o;
-      if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32.p, 0.0, inner_rel, max_it, o)))
+      if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32p, 0.0, inner_rel, max_it, o)))
         return rc;
       inner_total += o.iterations;
       if (o.status == MF_BREAKDOWN) {
@@ -831,4 +841,100 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
     return rep.status;
   }
   return MF_OK;
+}
+
+static int check_solve_args(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(desc && x && b, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(desc->solver == MF_SOLVER_CG || desc->solver == MF_SOLVER_GMRES, MF_ERR_ARG,
+             "unknown solver type");
+  return MF_OK;
+}
+
+}  // namespace mf
+
+extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, const double *b,
+                        mf_solve_report *report)
+{
+  int rc = check_solve_args(op, desc, x, b);
+  if (rc) return rc;
+  const int64_t n = op->s->mesh.n_dofs;
+  double *inv = nullptr;
+  if (desc->preconditioner == MF_PRECOND_JACOBI) {
+    if ((rc = arena_reserve(op->arena[2], sizeof(double) * std::max<int64_t>(n, RED_BLOCKS))))
+      return rc;
+    inv = reinterpret_cast<double *>(op->arena[2].p);
+    if ((rc = mf_operator_compute_inverse_diagonal(op, inv))) return rc;
+  }
+  return solve_with(op, desc, inv, nullptr, x, b, report);
+}
+
+extern "C" int mf_jacobi_create(mf_operator *op, mf_jacobi **out)
+{
+  MF_REQUIRE(op && out, MF_ERR_ARG, "null argument");
+  auto *pc = new mf_jacobi;
+  pc->op = op;
+  pc->n = op->s->mesh.n_dofs;
+  // the inversion of the advection diagonal reduces over a few blocks at the end of the array
+  const size_t nd = size_t(std::max<int64_t>(pc->n, RED_BLOCKS));
+  if (hipMalloc(&pc->inv, sizeof(double) * nd) != hipSuccess ||
+      hipMalloc(&pc->inv32, sizeof(float) * size_t(pc->n)) != hipSuccess) {
+    if (pc->inv) (void)hipFree(pc->inv);
+    delete pc;
+    set_error("hipMalloc failed for the Jacobi preconditioner");
+    return MF_ERR_HIP;
+  }
+  *out = pc;
+  return MF_OK;
+}
+
+extern "C" int mf_jacobi_update(mf_jacobi *pc)
+{
+  MF_REQUIRE(pc, MF_ERR_ARG, "null preconditioner");
+  pc->valid = false;
+  int rc = mf_operator_compute_inverse_diagonal(pc->op, pc->inv);
+  if (rc) return rc;
+  if ((rc = vec_cast<double, float>(pc->inv, pc->inv32, 1.0, pc->n, pc->op->s->stream))) return rc;
+  pc->valid = true;
+  return MF_OK;
+}
+
+extern "C" int mf_jacobi_vmult(mf_jacobi *pc, double *dst, const double *src)
+{
+  MF_REQUIRE(pc && dst && src, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(pc->valid, MF_ERR_STATE, "Jacobi preconditioner not updated (mf_jacobi_update)");
+  return vec_precond<double>(dst, src, pc->inv, pc->n, pc->op->s->stream);
+}
+
+extern "C" int mf_jacobi_inverse_diagonal(mf_jacobi *pc, double *dst)
+{
+  MF_REQUIRE(pc && dst, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(pc->valid, MF_ERR_STATE, "Jacobi preconditioner not updated (mf_jacobi_update)");
+  MF_HIP_TRY(hipMemcpyAsync(dst, pc->inv, sizeof(double) * size_t(pc->n), hipMemcpyDeviceToDevice,
+                            pc->op->s->stream));
+  return MF_OK;
+}
+
+extern "C" int mf_jacobi_destroy(mf_jacobi *pc)
+{
+  if (!pc) return MF_OK;
+  if (pc->inv) (void)hipFree(pc->inv);
+  if (pc->inv32) (void)hipFree(pc->inv32);
+  delete pc;
+  return MF_OK;
+}
+
+extern "C" int mf_solve_preconditioned(mf_operator *op, const mf_solver_desc *desc, mf_jacobi *pc,
+                                       double *x, const double *b, mf_solve_report *report)
+{
+  int rc = check_solve_args(op, desc, x, b);
+  if (rc) return rc;
+  if (pc) {
+    MF_REQUIRE(pc->valid, MF_ERR_STATE, "Jacobi preconditioner not updated (mf_jacobi_update)");
+    MF_REQUIRE(pc->n == op->s->mesh.n_dofs, MF_ERR_ARG,
+               "preconditioner and operator have different DoF counts");
+  }
+  return solve_with(op, desc, pc ? pc->inv : nullptr, pc ? pc->inv32 : nullptr, x, b, report);
 }

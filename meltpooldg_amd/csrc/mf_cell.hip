@@ -25,12 +25,50 @@
 // The 4 launches run in stream order.
 #include "mf_internal.hpp"
 
+#include <type_traits>
+
 namespace mf {
 
-template <int K>
+#ifndef MF_CPB_F32_K4
+#define MF_CPB_F32_K4 16
+#endif
+#ifndef MF_PREFETCH_F32
+#define MF_PREFETCH_F32 1
+#endif
+#ifndef MF_PREFETCH_F64
+#define MF_PREFETCH_F64 0
+#endif
+#ifndef MF_WPE_F64_K4
+#define MF_WPE_F64_K4 1
+#endif
+#ifndef MF_WPE_F32_K4
+#define MF_WPE_F32_K4 1
+#endif
+
+// Cells per workgroup step (one x row segment); K*K threads per cell.
+template <typename Number, int K>
 constexpr int cells_per_block()
 {
+  if constexpr (sizeof(Number) == 4 && K == 4) return MF_CPB_F32_K4;
   return K == 2 ? 64 : 16;
+}
+
+// Minimum waves per SIMD the register allocation must allow.
+template <typename Number, int K>
+constexpr int min_waves_per_eu()
+{
+  if constexpr (sizeof(Number) == 4 && K == 4) return MF_WPE_F32_K4;
+  if constexpr (sizeof(Number) == 8 && K == 4) return MF_WPE_F64_K4;
+  return 1;
+}
+
+// Whether the row walk loads the next step's pencils during the current step (costs 2K+1
+// registers per thread; pays where it does not lower the occupancy).
+template <typename Number, int K>
+constexpr bool prefetch_pencils()
+{
+  if constexpr (sizeof(Number) == 4) return MF_PREFETCH_F32;
+  return MF_PREFETCH_F64;
 }
 
 template <int K>
@@ -49,50 +87,135 @@ __device__ __forceinline__ int pencil(int p, int e)
   return elem<K>(a, b, e);
 }
 
-template <typename Number, int K, int CPB, int DIR>
+// LDS layout of a cell tensor: [K^3][CPB] (cell fastest, so that the 16-lane groups of a wave,
+// which serve 16 different cells, hit 16 different 8-byte bank pairs).
+template <int K, int CPB_>
+struct Layout {
+  static constexpr int CPB = CPB_;
+  __device__ static __forceinline__ int idx(int e, int c) { return e * CPB + c; }
+};
+
+template <typename Number, int K, class LY, int DIR>
 __device__ __forceinline__ void load_pencil(const Number *buf, int p, int c, Number (&v)[K])
 {
 #pragma unroll
-  for (int e = 0; e < K; ++e) v[e] = buf[pencil<K, DIR>(p, e) * CPB + c];
+  for (int e = 0; e < K; ++e) v[e] = buf[LY::idx(pencil<K, DIR>(p, e), c)];
 }
 
-template <typename Number, int K, int CPB, int DIR>
+template <typename Number, int K, class LY, int DIR>
 __device__ __forceinline__ void store_pencil(Number *buf, int p, int c, const Number (&v)[K])
 {
 #pragma unroll
-  for (int e = 0; e < K; ++e) buf[pencil<K, DIR>(p, e) * CPB + c] = v[e];
+  for (int e = 0; e < K; ++e) buf[LY::idx(pencil<K, DIR>(p, e), c)] = v[e];
 }
+
+// A 1D matrix (S or Dq, K x K row-major) as the sweeps use it.  VALU form: the matrix itself,
+// read from the kernel arguments.
+template <typename Number, int K>
+struct Mat1D {
+  const Number *M;
+  __device__ explicit Mat1D(const Number *m) : M(m) {}
+};
+
+// MFMA form (FP32): v_mfma_f32_4x4x1f32 treats the wave as 16 blocks of 4 lanes and adds
+// A[4b + i] * B[4b + j] to register i of lane 4b + j.  With B = element k of the lane's own
+// pencil and A = M[i][k] on lane 4b + i, the 4-register accumulator of every lane becomes M times
+// its pencil after K instructions (one per k).  n[k] = M[r][k] and t[k] = M[k][r] with r = lane
+// mod 4 are the A operands of M and M^T; rows r >= K are zero.
+template <int K>
+struct Mat1DMfma {
+  float n[K], t[K];
+  __device__ explicit Mat1DMfma(const float *m)
+  {
+    const int r = int(threadIdx.x) & 3;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      n[k] = r < K ? m[r * K + k] : 0.0f;
+      t[k] = r < K ? m[k * K + r] : 0.0f;
+    }
+  }
+};
 
 // out = M in (TRANSPOSE: out = M^T in), M row-major K x K
 template <typename Number, int K, bool TRANSPOSE>
-__device__ __forceinline__ void matvec(const Number *M, const Number (&in)[K], Number (&out)[K])
+__device__ __forceinline__ void matvec(const Mat1D<Number, K> &m, const Number (&in)[K],
+                                       Number (&out)[K])
 {
+  const Number *M = m.M;
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     Number acc = 0;
 #pragma unroll
-    for (int m = 0; m < K; ++m) acc = fma(TRANSPOSE ? M[m * K + i] : M[i * K + m], in[m], acc);
+    for (int mm = 0; mm < K; ++mm) acc = fma(TRANSPOSE ? M[mm * K + i] : M[i * K + mm], in[mm], acc);
     out[i] = acc;
   }
 }
 
-template <typename Number, int K, int CPB, int DIR, bool TRANSPOSE>
-__device__ __forceinline__ void sweep_inplace(const Number *M, Number *buf, int p, int c)
+typedef float mf_v4f __attribute__((ext_vector_type(4)));
+
+template <typename Number, int K, bool TRANSPOSE>
+__device__ __forceinline__ void matvec(const Mat1DMfma<K> &m, const float (&in)[K], float (&out)[K])
 {
-  Number v[K], o[K];
-  load_pencil<Number, K, CPB, DIR>(buf, p, c, v);
-  matvec<Number, K, TRANSPOSE>(M, v, o);
-  store_pencil<Number, K, CPB, DIR>(buf, p, c, o);
+  mf_v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(TRANSPOSE ? m.t[k] : m.n[k], in[k], acc, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < K; ++i) out[i] = acc[i];
 }
 
-template <typename Number, int K, int CPB, int DIR, bool TRANSPOSE>
-__device__ __forceinline__ void sweep(const Number *M, const Number *in, Number *out, int p, int c)
+template <typename Number, int K, class LY, int DIR, bool TRANSPOSE, class M1>
+__device__ __forceinline__ void sweep_inplace(const M1 &M, Number *buf, int p, int c)
 {
   Number v[K], o[K];
-  load_pencil<Number, K, CPB, DIR>(in, p, c, v);
+  load_pencil<Number, K, LY, DIR>(buf, p, c, v);
   matvec<Number, K, TRANSPOSE>(M, v, o);
-  store_pencil<Number, K, CPB, DIR>(out, p, c, o);
+  store_pencil<Number, K, LY, DIR>(buf, p, c, o);
 }
+
+template <typename Number, int K, class LY, int DIR, bool TRANSPOSE, class M1>
+__device__ __forceinline__ void sweep(const M1 &M, const Number *in, Number *out, int p, int c)
+{
+  Number v[K], o[K];
+  load_pencil<Number, K, LY, DIR>(in, p, c, v);
+  matvec<Number, K, TRANSPOSE>(M, v, o);
+  store_pencil<Number, K, LY, DIR>(out, p, c, o);
+}
+
+// Two independent sweeps with the LDS reads of both issued before either computes, so that one
+// LDS round trip serves both (in place is allowed: each thread rewrites only its own pencils).
+template <typename Number, int K, class LY, int DA, bool TA, int DB, bool TB, class M1>
+__device__ __forceinline__ void sweep_pair(const M1 &MA, const Number *inA, Number *outA,
+                                           const M1 &MB, const Number *inB, Number *outB, int p,
+                                           int c)
+{
+  Number va[K], vb[K], oa[K], ob[K];
+  load_pencil<Number, K, LY, DA>(inA, p, c, va);
+  load_pencil<Number, K, LY, DB>(inB, p, c, vb);
+  matvec<Number, K, TA>(MA, va, oa);
+  matvec<Number, K, TB>(MB, vb, ob);
+  store_pencil<Number, K, LY, DA>(outA, p, c, oa);
+  store_pencil<Number, K, LY, DB>(outB, p, c, ob);
+}
+
+// a[idx] for a thread-dependent idx < K without a memory access (a is uniform)
+template <typename Number, int K>
+__device__ __forceinline__ Number select_k(const Number *a, int idx)
+{
+  Number v = a[0];
+#pragma unroll
+  for (int k = 1; k < K; ++k) v = idx == k ? a[k] : v;
+  return v;
+}
+
+// Sweeps of the FP32 kernels with K = 3, 4 run on the matrix cores (the FP32 MFMA rate is twice
+// the unpacked FP32 VALU rate, and the VALU is left to the quadrature-point work); FP64 sweeps
+// stay on the VALU (same FP64 rate on both units here, no lane shuffles needed).
+template <typename Number, int K>
+struct SweepMat {
+  static constexpr bool MFMA = sizeof(Number) == 4 && (K == 3 || K == 4);
+  using type = typename std::conditional<MFMA, Mat1DMfma<K>, Mat1D<Number, K>>::type;
+};
 
 // tanh(x) and sech^2(x) from e = exp(-2|x|): tanh|x| = (1 - e)/(1 + e), sech^2 x = 4e/(1 + e)^2.
 // 1/(1 + e) lies in [1/2, 1], so the hardware reciprocal estimate refined by Newton steps (two
@@ -115,6 +238,19 @@ __device__ __forceinline__ double exp_neg2(double ax) { return exp(-2.0 * ax); }
 __device__ __forceinline__ float exp_neg2(float ax)
 {
   return __builtin_amdgcn_exp2f(-2.8853900817779268f * ax);  // 2 log2(e)
+}
+
+// the matrix handle of one step: the hoisted MFMA operands, or the VALU matrix as read from the
+// kernel arguments in this step
+template <typename Number, int K>
+__device__ __forceinline__ typename SweepMat<Number, K>::type step_mat(
+    const typename SweepMat<Number, K>::type &hoisted, const Number *M)
+{
+  if constexpr (SweepMat<Number, K>::MFMA) {
+    return hoisted;
+  } else {
+    return Mat1D<Number, K>(M);
+  }
 }
 
 template <typename Number>
@@ -177,24 +313,35 @@ struct LaunchPart {
 };
 
 // Node access of the structured mesh.  A workgroup walks one row of cells along x in steps of
-// CPB cells; thread (c, p) serves the x pencil p = j + K*k of cell c of the step, i.e. the
-// nodes (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
+// CPB cells; thread (c, p) = c + CPB*p serves the x pencil p = j + K*k of cell c of the step,
+// i.e. the nodes (cx*P + i, cy*P + j, cz*P + k), i = 0..K-1, which are contiguous in memory.
+// The pencil loads (source, linearisation field and Dirichlet mask) of the next step are issued
+// while the current step computes, so that a step does not wait for a full memory round trip.
 template <typename Number, int K, int CPB>
 struct GridAccess {
   static constexpr bool COOP = false;
+  static constexpr bool PREFETCH = prefetch_pencils<Number, K>();
   static constexpr int P = K - 1;
   uint32_t rowbase, gbase;
   int c, ncx, cellrow, nsub;
   int cx, cell;
+  const uint8_t *mask;  // Dirichlet mask (null when not applied)
+  uint32_t mbits;       // bit i: node i of the pencil is constrained (this step)
   bool active;
   bool first;    // this colour is the first to write the pencil's nodes
   Number carry;  // output of the previous step's last cell at the node it shares with cell 0
 
+  __device__ static void map(int t, int &c_, int &p_)
+  {
+    c_ = t % CPB;
+    p_ = t / CPB;
+  }
+
   __device__ GridAccess(const CellArgs<Number> &A, const LaunchPart &lp, int c_, int p)
   {
     const RowColour rc = row_colour(A.ncell, lp.colour);
-    const int cy = 2 * (blockIdx.x % rc.hy) + rc.yb;
-    const int cz = 2 * (blockIdx.x / rc.hy) + rc.zb;
+    const int cy = 2 * (int(blockIdx.x) % rc.hy) + rc.yb;
+    const int cz = 2 * (int(blockIdx.x) / rc.hy) + rc.zb;
     c = c_;
     ncx = A.ncell[0];
     nsub = (ncx + CPB - 1) / CPB;
@@ -208,6 +355,7 @@ struct GridAccess {
     const bool shared_z = (k == 0 && cz > 0) || (k == P && cz < A.ncell[2] - 1);
     first = (!shared_y || rc.yb == 0) && (!shared_z || rc.zb == 0);
     carry = Number(0);
+    mask = (A.use_mask && A.mask) ? A.mask : nullptr;
     at(0);
   }
   __device__ int iterations() const { return nsub; }
@@ -219,17 +367,61 @@ struct GridAccess {
     active = cx < ncx;
     cell = cellrow + cx;
     gbase = rowbase + uint32_t(cx * P);
+    if constexpr (!PREFETCH) {
+      // the mask bits of the step (with prefetching they come with the raw pencil)
+      mbits = 0u;
+      if (mask) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) mbits |= uint32_t(mask[(active ? gbase : 0u) + i] != 0) << i;
+      }
+    }
   }
-  __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
-  {
-    return A.use_mask && A.mask && A.mask[n];
-  }
-  // entry i of field f as stored (read_dof_values_plain)
-  __device__ Number plain(const Number *f, int i) const { return active ? f[gbase + i] : Number(0); }
   // entry i with the constraints applied (read_dof_values: Dirichlet entries read as zero)
-  __device__ Number value(const CellArgs<Number> &A, const Number *f, int i) const
+  __device__ Number value(const CellArgs<Number> &, const Number *f, int i) const
   {
-    return (active && !masked(A, gbase + i)) ? f[gbase + i] : Number(0);
+    const Number v = f[(active ? gbase : 0u) + uint32_t(i)];
+    return (active && !((mbits >> i) & 1u)) ? v : Number(0);
+  }
+  // entry i of field f as stored (read_dof_values_plain), zero past the row end
+  __device__ Number plain(const Number *f, int i) const
+  {
+    const Number v = f[(active ? gbase : 0u) + uint32_t(i)];
+    return active ? v : Number(0);
+  }
+  // Raw pencil of step `it` (loaded at an in-bounds index for every lane: node i of the row
+  // start when the cell is past the end of the row); nothing is selected here, so the loads of
+  // a step go out together.
+  struct Raw {
+    Number u[K], t[K];
+    uint32_t mb;
+  };
+  template <bool LU, bool LT>
+  __device__ void issue(int it, const Number *fu, const Number *ft, Raw &r) const
+  {
+    const int cxn = it * CPB + c;
+    const uint32_t g = cxn < ncx ? rowbase + uint32_t(cxn * P) : 0u;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if constexpr (LU) r.u[i] = fu[g + i];
+      if constexpr (LT) r.t[i] = ft[g + i];
+    }
+    r.mb = 0u;
+    if (mask) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) r.mb |= uint32_t(mask[g + i] != 0) << i;
+    }
+  }
+  // take the raw pencil of the current step: constrained entries read as zero when `cons`
+  // (read_dof_values), as stored otherwise (read_dof_values_plain); zero past the row end
+  template <bool LU, bool LT>
+  __device__ void take(const Raw &r, bool cons, Number (&u)[K], Number (&t)[K])
+  {
+    mbits = r.mb;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if constexpr (LU) u[i] = (active && !(cons && ((mbits >> i) & 1u))) ? r.u[i] : Number(0);
+      if constexpr (LT) t[i] = active ? r.t[i] : Number(0);
+    }
   }
   // current dst entries of the pencil (not needed where this launch is the first writer of an
   // overwriting application); rows of one colour own their nodes during the launch, so these
@@ -244,7 +436,8 @@ struct GridAccess {
   // lane, so its last node (shared with this cell's first) is fetched with a shuffle and added
   // here; cell 0 takes it from the previous step's last cell.  A cell writes nodes 0..K-2 and,
   // at the end of the row, node K-1.  Constrained rows are skipped
-  // (distribute_local_to_global semantics).
+  // (distribute_local_to_global semantics); an overwriting application (dst = A src) stores
+  // where it is the first writer and writes zeros on constrained rows there.
   __device__ void scatter(const CellArgs<Number> &A, Number (&o)[K], const Number (&d)[K], int,
                           Number *dst, Number *)
   {
@@ -258,11 +451,11 @@ struct GridAccess {
     carry = wrap;
     if (!active) return;
     const bool last = (cx == ncx - 1);
-    const bool store = A.overwrite && first;  // first writer of an overwriting application
+    const bool store = A.overwrite && first;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       if (i == K - 1 && !last) break;
-      if (masked(A, gbase + i)) {
+      if ((mbits >> i) & 1u) {
         if (store) dst[gbase + i] = Number(0);  // constrained rows of dst = A src are zero
         continue;
       }
@@ -281,6 +474,13 @@ struct GridAccess {
 template <typename Number, int K, int CPB>
 struct ListAccess {
   static constexpr bool COOP = true;
+  static constexpr bool PREFETCH = false;
+  struct Raw {};
+  __device__ static void map(int t, int &c, int &p)
+  {
+    c = t % CPB;
+    p = t / CPB;
+  }
   static constexpr int NLOC = K * K * K;
   int cell, level;     // the cell of the thread's pencils
   bool active;
@@ -333,6 +533,20 @@ struct ListAccess {
     if ((hbits >> i) & 1u) return A.hang_val[A.node_line[n]];
     return masked(A, n) ? Number(0) : f[n];
   }
+  __device__ void coop_load(const CellArgs<Number> &A, const Number *f, bool cons,
+                            Number (&v)[K]) const
+  {
+#pragma unroll
+    for (int i = 0; i < K; ++i) v[i] = f ? (cons ? coop_value(A, f, i) : coop_plain(f, i)) : Number(0);
+  }
+  __device__ void coop_store(const Number (&v)[K], Number *T0, Number *T1) const
+  {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      T0[coop_pos(i)] = v[i];
+      if (T1) T1[coop_pos(i)] = v[i];
+    }
+  }
   __device__ void prefetch(const CellArgs<Number> &, const Number *dst, Number (&d)[K]) const
   {
 #pragma unroll
@@ -347,7 +561,7 @@ struct ListAccess {
   {
     const int p = int(threadIdx.x) / CPB;
     __syncthreads();  // the stage may still be read by other threads
-    store_pencil<Number, K, CPB, 0>(stage, p, c, o);
+    store_pencil<Number, K, Layout<K, CPB>, 0>(stage, p, c, o);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -366,6 +580,13 @@ struct ListAccess {
 template <typename Number, int K, int CPB>
 struct SlotAccess {
   static constexpr bool COOP = false;
+  static constexpr bool PREFETCH = false;
+  struct Raw {};
+  __device__ static void map(int t, int &c, int &p)
+  {
+    c = t % CPB;
+    p = t / CPB;
+  }
   static constexpr int NLOC = K * K * K;
   const uint32_t *nodes;
   uint32_t tgt;
@@ -427,11 +648,12 @@ struct SlotAccess {
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT, class Access>
-__global__ void __launch_bounds__(cells_per_block<K>() * K * K)
+__global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
+    __attribute__((amdgpu_waves_per_eu(min_waves_per_eu<Number, K>())))
     cell_kernel(const CellArgs<Number> A0, const Number *__restrict__ src, Number *__restrict__ dst,
                 const LaunchPart lp)
 {
-  constexpr int CPB = cells_per_block<K>();
+  constexpr int CPB = cells_per_block<Number, K>();
   constexpr int NL = K * K * K;
   constexpr int BUF = NL * CPB;
   constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG);
@@ -461,8 +683,9 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 
   if (A0.stop && *A0.stop) return;  // uniform over the grid
 
-  const int c = threadIdx.x % CPB;
-  const int p = threadIdx.x / CPB;
+  using LY = Layout<K, CPB>;
+  int c, p;
+  Access::map(int(threadIdx.x), c, p);
   const int qi = p % K, qj = p / K;
   Access ac(A0, lp, c, p);
   double dot_acc = 0;
@@ -471,6 +694,15 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   // in scalar registers only while a step uses them.
   typedef __attribute__((address_space(4))) const CellArgs<Number> KernArgs;
   KernArgs *kargs = (KernArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  // MFMA operands are built once and stay in registers
+  using MH = typename SweepMat<Number, K>::type;
+  const MH S0(A0.S), Dq0(A0.Dq);
+  // product of the 1D Gauss weights of the thread's z pencil (qi, qj)
+  const Number wqq = select_k<Number, K>(A0.w, qi) * select_k<Number, K>(A0.w, qj);
+  // pencils of the next step, in flight while the current step computes
+  constexpr bool PF = Access::PREFETCH;
+  typename Access::Raw raw;
+  if constexpr (PF) ac.template issue<HAS_U, USE_T>(0, RHS_HEAT ? A0.T : src, A0.T, raw);
 
   for (int it = 0; it < ac.iterations(); ++it) {
   if (it > 0) {
@@ -479,74 +711,71 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   }
   asm volatile("" : "+s"(kargs));
   const CellArgs<Number> &A = *(const CellArgs<Number> *)kargs;
-  const Number *S = A.S;
-  const Number *Dq = A.Dq;
+  const MH S = step_mat<Number, K>(S0, A.S);
+  const MH Dq = step_mat<Number, K>(Dq0, A.Dq);
   const Number hs = ac.hscale();
   const Number ih[3] = {A.inv_h[0] / hs, A.inv_h[1] / hs, A.inv_h[2] / hs};
 
   // ---------------- E1: x pencils, gather from HBM ----------------
   Number dpre[K];  // dst entries of the pencil, read while the integration sweeps run
   if constexpr (Access::COOP) {
-    // cooperative gather into the LDS tensors, then the x interpolation in place
-    if constexpr (HAS_U) {
-      const Number *in = RHS_HEAT ? A.T : src;
-      const bool cons = TANGENT && !A.gather_plain;
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        const Number v = cons ? ac.coop_value(A, in, i) : ac.coop_plain(in, i);
-        L0[ac.coop_pos(i)] = v;
-        if constexpr (FDOT) lds[I_SRC * BUF + ac.coop_pos(i)] = v;
-      }
-    }
-    if constexpr (USE_T) {
-#pragma unroll
-      for (int i = 0; i < K; ++i) lds[I_T * BUF + ac.coop_pos(i)] = ac.coop_plain(A.T, i);
-    }
+    // cooperative gather into the LDS tensors (all fields' loads in flight together), then the
+    // x interpolation in place
+    Number gu[K], gt[K];
+    if constexpr (HAS_U)
+      ac.coop_load(A, RHS_HEAT ? A.T : src, TANGENT && !A.gather_plain, gu);
+    if constexpr (USE_T) ac.coop_load(A, A.T, false, gt);
+    if constexpr (HAS_U) ac.coop_store(gu, L0, FDOT ? lds + I_SRC * BUF : nullptr);
+    if constexpr (USE_T) ac.coop_store(gt, lds + I_T * BUF, nullptr);
     if constexpr (VEL) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int i = 0; i < K; ++i)
-          lds[(I_W + d) * BUF + ac.coop_pos(i)] = ac.coop_plain(A.vel + size_t(d) * A.n_dofs, i);
+      for (int d = 0; d < 3; ++d) {
+        ac.coop_load(A, A.vel + size_t(d) * A.n_dofs, false, gu);
+        ac.coop_store(gu, lds + (I_W + d) * BUF, nullptr);
+      }
     }
     if constexpr (RHS_HEAT) {
-#pragma unroll
-      for (int i = 0; i < K; ++i) {
-        lds[I_A * BUF + ac.coop_pos(i)] = ac.coop_plain(src, i);
-        lds[(I_A + 1) * BUF + ac.coop_pos(i)] = A.aux1 ? ac.coop_plain(A.aux1, i) : Number(0);
-      }
+      ac.coop_load(A, src, false, gu);  // T_old
+      ac.coop_load(A, A.aux1, false, gt);
+      ac.coop_store(gu, lds + I_A * BUF, nullptr);
+      ac.coop_store(gt, lds + (I_A + 1) * BUF, nullptr);
     }
     __syncthreads();
-    if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 0, false>(S, L0, p, c);
-    if constexpr (USE_T) sweep_inplace<Number, K, CPB, 0, false>(S, lds + I_T * BUF, p, c);
+    if constexpr (HAS_U) sweep_inplace<Number, K, LY, 0, false>(S, L0, p, c);
+    if constexpr (USE_T) sweep_inplace<Number, K, LY, 0, false>(S, lds + I_T * BUF, p, c);
     if constexpr (VEL) {
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        sweep_inplace<Number, K, CPB, 0, false>(S, lds + (I_W + d) * BUF, p, c);
+        sweep_inplace<Number, K, LY, 0, false>(S, lds + (I_W + d) * BUF, p, c);
     }
     if constexpr (RHS_HEAT) {
-      sweep_inplace<Number, K, CPB, 0, false>(S, lds + I_A * BUF, p, c);
-      sweep_inplace<Number, K, CPB, 0, false>(S, lds + (I_A + 1) * BUF, p, c);
+      sweep_inplace<Number, K, LY, 0, false>(S, lds + I_A * BUF, p, c);
+      sweep_inplace<Number, K, LY, 0, false>(S, lds + (I_A + 1) * BUF, p, c);
     }
   } else {
   {
     Number o[K];
-    if constexpr (HAS_U) {
-      const Number *in = RHS_HEAT ? A.T : src;
-      const bool cons = TANGENT && !A.gather_plain;
-      Number u_loc[K];
+    const Number *in = RHS_HEAT ? A.T : src;
+    const bool cons = TANGENT && !A.gather_plain;
+    Number u_loc[K], t_loc[K];
+    if constexpr (PF) {
+      ac.template take<HAS_U, USE_T>(raw, cons, u_loc, t_loc);
+      if (it + 1 < ac.iterations()) ac.template issue<HAS_U, USE_T>(it + 1, in, A.T, raw);
+    } else {
 #pragma unroll
-      for (int i = 0; i < K; ++i) u_loc[i] = cons ? ac.value(A, in, i) : ac.plain(in, i);
+      for (int i = 0; i < K; ++i) {
+        if constexpr (HAS_U) u_loc[i] = cons ? ac.value(A, in, i) : ac.plain(in, i);
+        if constexpr (USE_T) t_loc[i] = ac.plain(A.T, i);
+      }
+    }
+    if constexpr (HAS_U) {
       matvec<Number, K, false>(S, u_loc, o);
-      store_pencil<Number, K, CPB, 0>(L0, p, c, o);
-      if constexpr (FDOT) store_pencil<Number, K, CPB, 0>(lds + I_SRC * BUF, p, c, u_loc);
+      store_pencil<Number, K, LY, 0>(L0, p, c, o);
+      if constexpr (FDOT) store_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u_loc);
     }
     if constexpr (USE_T) {
-      Number v[K];
-#pragma unroll
-      for (int i = 0; i < K; ++i) v[i] = ac.plain(A.T, i);
-      matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(lds + I_T * BUF, p, c, o);
+      matvec<Number, K, false>(S, t_loc, o);
+      store_pencil<Number, K, LY, 0>(lds + I_T * BUF, p, c, o);
     }
     if constexpr (VEL) {
 #pragma unroll
@@ -555,7 +784,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 #pragma unroll
         for (int i = 0; i < K; ++i) v[i] = ac.plain(A.vel + size_t(d) * A.n_dofs, i);
         matvec<Number, K, false>(S, v, o);
-        store_pencil<Number, K, CPB, 0>(lds + (I_W + d) * BUF, p, c, o);
+        store_pencil<Number, K, LY, 0>(lds + (I_W + d) * BUF, p, c, o);
       }
     }
     if constexpr (RHS_HEAT) {
@@ -563,58 +792,64 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 #pragma unroll
       for (int i = 0; i < K; ++i) v[i] = ac.plain(src, i);  // T_old
       matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(lds + I_A * BUF, p, c, o);
+      store_pencil<Number, K, LY, 0>(lds + I_A * BUF, p, c, o);
 #pragma unroll
       for (int i = 0; i < K; ++i) v[i] = A.aux1 ? ac.plain(A.aux1, i) : Number(0);
       matvec<Number, K, false>(S, v, o);
-      store_pencil<Number, K, CPB, 0>(lds + (I_A + 1) * BUF, p, c, o);
+      store_pencil<Number, K, LY, 0>(lds + (I_A + 1) * BUF, p, c, o);
     }
   }
   }
   __syncthreads();
 
   // ---------------- E2: y pencils ----------------
-  if constexpr (HAS_U) sweep_inplace<Number, K, CPB, 1, false>(S, L0, p, c);
-  if constexpr (USE_T) sweep_inplace<Number, K, CPB, 1, false>(S, lds + I_T * BUF, p, c);
+  if constexpr (HAS_U && USE_T) {
+    sweep_pair<Number, K, LY, 1, false, 1, false>(S, L0, L0, S, lds + I_T * BUF, lds + I_T * BUF,
+                                                  p, c);
+  } else {
+    if constexpr (HAS_U) sweep_inplace<Number, K, LY, 1, false>(S, L0, p, c);
+    if constexpr (USE_T) sweep_inplace<Number, K, LY, 1, false>(S, lds + I_T * BUF, p, c);
+  }
   if constexpr (VEL) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) sweep_inplace<Number, K, CPB, 1, false>(S, lds + (I_W + d) * BUF, p, c);
+    for (int d = 0; d < 3; ++d) sweep_inplace<Number, K, LY, 1, false>(S, lds + (I_W + d) * BUF, p, c);
   }
   if constexpr (RHS_HEAT) {
-    sweep_inplace<Number, K, CPB, 1, false>(S, lds + I_A * BUF, p, c);
-    sweep_inplace<Number, K, CPB, 1, false>(S, lds + (I_A + 1) * BUF, p, c);
+    sweep_inplace<Number, K, LY, 1, false>(S, lds + I_A * BUF, p, c);
+    sweep_inplace<Number, K, LY, 1, false>(S, lds + (I_A + 1) * BUF, p, c);
   }
   __syncthreads();
 
   // ---------------- E3: z pencils (values at q and d/dz kept in registers) ----------------
   Number uq[K], guz[K], Tq[K], gTz[K], wq[3][K], a1q[K], a2q[K];
-  if constexpr (HAS_U) {
-    Number v[K];
-    load_pencil<Number, K, CPB, 2>(L0, p, c, v);
-    matvec<Number, K, false>(S, v, uq);
-    matvec<Number, K, false>(Dq, uq, guz);
-    store_pencil<Number, K, CPB, 2>(L0, p, c, uq);
-  }
-  if constexpr (USE_T) {
-    Number v[K];
-    load_pencil<Number, K, CPB, 2>(lds + I_T * BUF, p, c, v);
-    matvec<Number, K, false>(S, v, Tq);
-    matvec<Number, K, false>(Dq, Tq, gTz);
-    store_pencil<Number, K, CPB, 2>(lds + I_T * BUF, p, c, Tq);
+  {
+    Number vu[K], vt[K];
+    if constexpr (HAS_U) load_pencil<Number, K, LY, 2>(L0, p, c, vu);
+    if constexpr (USE_T) load_pencil<Number, K, LY, 2>(lds + I_T * BUF, p, c, vt);
+    if constexpr (HAS_U) {
+      matvec<Number, K, false>(S, vu, uq);
+      matvec<Number, K, false>(Dq, uq, guz);
+    }
+    if constexpr (USE_T) {
+      matvec<Number, K, false>(S, vt, Tq);
+      matvec<Number, K, false>(Dq, Tq, gTz);
+    }
+    if constexpr (HAS_U) store_pencil<Number, K, LY, 2>(L0, p, c, uq);
+    if constexpr (USE_T) store_pencil<Number, K, LY, 2>(lds + I_T * BUF, p, c, Tq);
   }
   if constexpr (VEL) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       Number v[K];
-      load_pencil<Number, K, CPB, 2>(lds + (I_W + d) * BUF, p, c, v);
+      load_pencil<Number, K, LY, 2>(lds + (I_W + d) * BUF, p, c, v);
       matvec<Number, K, false>(S, v, wq[d]);
     }
   }
   if constexpr (RHS_HEAT) {
     Number v[K];
-    load_pencil<Number, K, CPB, 2>(lds + I_A * BUF, p, c, v);
+    load_pencil<Number, K, LY, 2>(lds + I_A * BUF, p, c, v);
     matvec<Number, K, false>(S, v, a1q);
-    load_pencil<Number, K, CPB, 2>(lds + (I_A + 1) * BUF, p, c, v);
+    load_pencil<Number, K, LY, 2>(lds + (I_A + 1) * BUF, p, c, v);
     matvec<Number, K, false>(S, v, a2q);
   }
   __syncthreads();
@@ -625,12 +860,12 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
   // heat tangent / diagonal: kq = k(T) kc and dkg = dk/dT kc grad T;  PGRAD: dkg = grad T
   Number kq[K], dkg[3][K];
   if constexpr (USE_T) {
-    sweep<Number, K, CPB, 0, false>(Dq, lds + I_T * BUF, L2, p, c);
-    sweep<Number, K, CPB, 1, false>(Dq, lds + I_T * BUF, lds + 3 * BUF, p, c);
+    sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, lds + I_T * BUF, L2, Dq, lds + I_T * BUF,
+                                                  lds + 3 * BUF, p, c);
     __syncthreads();
 #pragma unroll
     for (int z = 0; z < K; ++z) {
-      const int e = elem<K>(qi, qj, z) * CPB + c;
+      const int e = LY::idx(elem<K>(qi, qj, z), c);
       const Number g0 = L2[e] * ih[0], g1 = lds[3 * BUF + e] * ih[1], g2 = gTz[z] * ih[2];
       if constexpr (PGRAD) {
         dkg[0][z] = g0;
@@ -651,17 +886,16 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
 
   // ---------------- E4: d/dx and d/dy of u at the quadrature points ----------------
   if constexpr (HAS_U) {
-    sweep<Number, K, CPB, 0, false>(Dq, L0, L1, p, c);
-    sweep<Number, K, CPB, 1, false>(Dq, L0, L2, p, c);
+    sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, L0, L1, Dq, L0, L2, p, c);
     __syncthreads();
   }
 
   // ---------------- Q: quadrature-point operation (z pencil (i,j), q = (i,j,z)) ----------------
-  const Number wij = A.w[qi] * A.w[qj] * (A.h[0] * hs) * (A.h[1] * hs) * (A.h[2] * hs);
+  const Number wij = wqq * (A.h[0] * hs) * (A.h[1] * hs) * (A.h[2] * hs);
   Number rq[K], fz[K];
 #pragma unroll
   for (int z = 0; z < K; ++z) {
-    const int e = elem<K>(qi, qj, z) * CPB + c;
+    const int e = LY::idx(elem<K>(qi, qj, z), c);
     const Number jxw = wij * A.w[z];
     Number g[3] = {0, 0, 0};
     if constexpr (HAS_U) {
@@ -776,11 +1010,11 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     Number part = 0;
 #pragma unroll
     for (int z = 0; z < K; ++z) part += rq[z];
-    L0[p * CPB + c] = part;
+    L0[LY::idx(p, c)] = part;
     __syncthreads();
     if (p == 0 && ac.active) {
       double sum = 0;
-      for (int i = 0; i < K * K; ++i) sum += double(L0[i * CPB + c]);
+      for (int i = 0; i < K * K; ++i) sum += double(L0[LY::idx(i, c)]);
       dst[ac.cell] += Number(A.pgrad_scale * sum);
     }
     continue;
@@ -801,7 +1035,7 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
         const Number sz = A.S[q3 * K + kb], dz = A.DS[q3 * K + kb] * ih[2];
 #pragma unroll
         for (int q1 = 0; q1 < K; ++q1) {
-          const int e = elem<K>(q1, q2, q3) * CPB + c;
+          const int e = LY::idx(elem<K>(q1, q2, q3), c);
           const Number cq = L0[e], kq2 = L1[e], b0 = L2[e], b1 = lds[3 * BUF + e],
                        b2 = lds[4 * BUF + e];
 #pragma unroll
@@ -821,43 +1055,42 @@ __global__ void __launch_bounds__(cells_per_block<K>() * K * K)
     matvec<Number, K, true>(Dq, fz, tz);
 #pragma unroll
     for (int z = 0; z < K; ++z) rq[z] += tz[z];
-    store_pencil<Number, K, CPB, 2>(L0, p, c, rq);
+    store_pencil<Number, K, LY, 2>(L0, p, c, rq);
     __syncthreads();
 
     // ---------------- I1: Dq^T along x (on fx) and y (on fy) ----------------
-    sweep_inplace<Number, K, CPB, 0, true>(Dq, L1, p, c);
-    sweep_inplace<Number, K, CPB, 1, true>(Dq, L2, p, c);
+    sweep_pair<Number, K, LY, 0, true, 1, true>(Dq, L1, L1, Dq, L2, L2, p, c);
     __syncthreads();
 
     // ---------------- I2: sum and S^T along z ----------------
     {
       Number a[K], b[K], d[K], o[K];
-      load_pencil<Number, K, CPB, 2>(L0, p, c, a);
-      load_pencil<Number, K, CPB, 2>(L1, p, c, b);
-      load_pencil<Number, K, CPB, 2>(L2, p, c, d);
+      load_pencil<Number, K, LY, 2>(L0, p, c, a);
+      load_pencil<Number, K, LY, 2>(L1, p, c, b);
+      load_pencil<Number, K, LY, 2>(L2, p, c, d);
 #pragma unroll
       for (int e = 0; e < K; ++e) a[e] += b[e] + d[e];
       matvec<Number, K, true>(S, a, o);
-      store_pencil<Number, K, CPB, 2>(L0, p, c, o);
+      store_pencil<Number, K, LY, 2>(L0, p, c, o);
     }
     __syncthreads();
 
     // ---------------- I3: S^T along y ----------------
-    sweep_inplace<Number, K, CPB, 1, true>(S, L0, p, c);
+    sweep_inplace<Number, K, LY, 1, true>(S, L0, p, c);
     __syncthreads();
 
     // ---------------- I4: S^T along x and scatter-add ----------------
     Number o[K];
     {
       Number v[K];
-      load_pencil<Number, K, CPB, 0>(L0, p, c, v);
+      load_pencil<Number, K, LY, 0>(L0, p, c, v);
       matvec<Number, K, true>(S, v, o);
     }
     if constexpr (FDOT) {
       // src . (A src) over the cell's nodes (constrained entries of the gathered source are 0),
       // before the x-shared node is combined across cells
       Number u[K];
-      load_pencil<Number, K, CPB, 0>(lds + I_SRC * BUF, p, c, u);
+      load_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u);
       Number local_dot = 0;
 #pragma unroll
       for (int i = 0; i < K; ++i) local_dot += u[i] * o[i];
@@ -880,10 +1113,12 @@ static int row_blocks(const MeshInfo &m, int colour)
   return s.hy * s.hz;
 }
 
-int cell_blocks(const MeshInfo &m)
+template <typename Number>
+static int cell_blocks_t(const MeshInfo &m)
 {
+  constexpr int CPBS[5] = {0, 0, cells_per_block<Number, 2>(), cells_per_block<Number, 3>(),
+                           cells_per_block<Number, 4>()};
   if (m.ad) {
-    constexpr int CPBS[5] = {0, 0, cells_per_block<2>(), cells_per_block<3>(), cells_per_block<4>()};
     int total = 0;
     for (int col = 0; col < m.ad->n_colours; ++col)
       total += int((m.ad->colour_start[col + 1] - m.ad->colour_start[col] + CPBS[m.K] - 1) / CPBS[m.K]);
@@ -894,11 +1129,16 @@ int cell_blocks(const MeshInfo &m)
   return total;
 }
 
+int cell_blocks(const MeshInfo &m, bool fp32)
+{
+  return fp32 ? cell_blocks_t<float>(m) : cell_blocks_t<double>(m);
+}
+
 template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT>
 static int launch_colours(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
                           Number *dst, hipStream_t st)
 {
-  constexpr int CPB = cells_per_block<K>();
+  constexpr int CPB = cells_per_block<Number, K>();
   int part_off = 0;
   for (int colour = 0; colour < 4; ++colour) {
     const int blocks = row_blocks(m, colour);
@@ -927,7 +1167,7 @@ static int launch_parts(const std::vector<int64_t> &start, const std::vector<int
                         const CellArgs<Number> &a, const Number *src, Number *dst,
                         hipStream_t st)
 {
-  constexpr int CPB = cells_per_block<K>();
+  constexpr int CPB = cells_per_block<Number, K>();
   int part_off = 0;
   for (size_t col = 0; col < start.size(); ++col) {
     const int64_t count = end[col] - start[col];
@@ -965,7 +1205,7 @@ template <typename Number, int K, int OP, bool NEEDT, bool VEL>
 static int launch_list_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
                          Number *dst, hipStream_t st, int mode)
 {
-  constexpr int CPB = cells_per_block<K>();
+  constexpr int CPB = cells_per_block<Number, K>();
   std::vector<int64_t> start, end;
   list_ranges(*m.ad, mode, start, end);
   if (mode == 2) {

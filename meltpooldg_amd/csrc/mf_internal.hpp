@@ -189,8 +189,8 @@ template <typename Number>
 int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                      const Number *src, Number *dst, hipStream_t stream, int mode);
 
-// number of blocks the cell kernel uses (size of dot_partial)
-int cell_blocks(const MeshInfo &m);
+// number of blocks the cell kernel uses (size of dot_partial) in FP64 or FP32
+int cell_blocks(const MeshInfo &m, bool fp32);
 
 // slice-resident tangent kernel (mf_slab.hip)
 template <typename Number>
@@ -200,9 +200,9 @@ int slab_blocks(const MeshInfo &m);
 
 // tangent kernel selection: 0 = pencil kernel (mf_cell.hip), 1 = slice-resident (mf_slab.hip)
 extern int g_tangent_variant;
-inline int tangent_blocks(const MeshInfo &m, bool transpose = false)
+inline int tangent_blocks(const MeshInfo &m, bool transpose, bool fp32)
 {
-  return (g_tangent_variant == 1 && !transpose && !m.ad) ? slab_blocks(m) : cell_blocks(m);
+  return (g_tangent_variant == 1 && !transpose && !m.ad) ? slab_blocks(m) : cell_blocks(m, fp32);
 }
 
 }  // namespace mf

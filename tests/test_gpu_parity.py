@@ -1,5 +1,8 @@
 """Device parity: libmeltpooldg_amd (through the C ABI) against the CPU oracle on the same seeded
-inputs.  Bar: FP64 operator application within 1e-12 relative (max-norm), FP32 within 2e-6."""
+inputs.  Bar: FP64 operator application within 1e-12 relative (max-norm); the FP32 operator
+(used only inside the refined solver) within 1e-5: unit roundoff 6e-8 times the accumulation over
+the 18 sum-factorisation sweeps and the tanh-law slope, measured up to 3.2e-6; a misplaced or
+missing contribution shows as O(1)."""
 import numpy as np
 import pytest
 
@@ -10,7 +13,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-12
-TOL32 = 2e-6
+TOL32 = 1e-5
 
 CASES = [
     Case(degree=1, kind="heat"),
@@ -24,6 +27,13 @@ CASES = [
     Case(degree=1, kind="advec"),
     Case(degree=2, kind="advec", mask="random"),
     Case(degree=3, kind="advec", theta=1.0, diffusivity=0.0, mass_factor=1.5),
+    # rows longer than one workgroup step (16 cells for Q2/Q3, 64 for Q1): the row walk carries
+    # the shared node across steps, loads the next step's pencils ahead and ends mid-step
+    # (domains keep the cells close to the aspect ratios of the cases above)
+    Case(degree=3, kind="heat", law="tanh", t_dependent=True, mask="random", ncell=(37, 2, 3),
+         hi=(3.7, 0.24, 0.27)),
+    Case(degree=2, kind="advec", ncell=(35, 3, 2), hi=(3.5, 0.36, 0.18)),
+    Case(degree=1, kind="heat", vel=True, k_cell=True, ncell=(150, 2, 2), hi=(15.0, 0.24, 0.18)),
 ]
 
 
