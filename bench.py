@@ -62,6 +62,8 @@ def main():
                     help="skip the adaptive-mesh (hanging-node) solve section")
     ap.add_argument("--no-levelset", action="store_true",
                     help="skip the level-set advection solve section")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the HBM traffic measurement (two rocprofv3 --pmc child runs)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only run 5 operator applications (vmult_add) and 5 plain copies of one "
                          "DoF vector (calibration), for hardware-counter runs")
@@ -121,7 +123,10 @@ def main():
     # timed region: K full vmults (zero + cell kernel), HIP events on the library's stream
     ms_per_step = timed(op.vmult, args.steps)
     value = n / (ms_per_step * 1e-3)
-    # the cell kernel alone (vmult_add: no zeroing) for the roofline
+    # the cell kernel alone (vmult_add: no zeroing) for the roofline.  One application is 4
+    # launches (one per (y, z) row-parity colour, a quarter of the cells each); the achieved
+    # rate is the algorithmic bytes of an application over its 4 launches' time, which equals the
+    # per-launch bytes (a quarter) over the average launch duration.
     kernel_ms = timed(op.vmult_add, args.steps)
     alg_bytes = 3 * n * 8  # src read, dst write, T read: each DoF vector once
     achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
@@ -165,6 +170,12 @@ def main():
     if not args.no_levelset:
         extra["levelset"] = levelset_case(mf, torch, stream)
 
+    traffic = None if args.no_traffic else measure_traffic()
+    if traffic:
+        print(f"# traffic per cell-kernel launch: {traffic['bytes_per_launch']/1e6:.1f} MB "
+              f"(read {traffic['read_bytes']/1e6:.1f}, write {traffic['write_bytes']/1e6:.1f}; "
+              f"algorithmic {alg_bytes/4/1e6:.1f})", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(T, mask, b, n, args.cpu_budget)
@@ -187,8 +198,13 @@ def main():
                                "homogeneous Dirichlet",
                    "n_dofs": n, "n_cells": s.n_cells, "degree": 3},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel_us": kernel_ms * 1e3, "alg_bytes_per_launch": alg_bytes,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_detail": traffic,
+                     "launches_per_vmult": 4,
+                     "kernel_us_per_launch": kernel_ms * 1e3 / 4,
+                     "alg_bytes_per_launch": alg_bytes / 4,
+                     "kernel_us_per_vmult": kernel_ms * 1e3, "alg_bytes_per_vmult": alg_bytes,
                      "fp64_tflops": achieved_tflops,
                      "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "fp32_vmult_us": f32_ms * 1e3,
@@ -330,6 +346,62 @@ def levelset_case(mf, torch, stream):
           f"({r64['iterations']} it), refined {t32*1e3:.1f} ms ({r32['outer_iterations']}/"
           f"{r32['iterations']}), err {err:.2e}", file=sys.stderr, flush=True)
     return out
+
+
+TRAFFIC_KERNEL = "cell_kernel<double, 4, 0, true, false, false"  # FP64 heat tangent, no fused dot
+COPY_KERNEL = "copyBuffer"
+
+
+def measure_traffic(timeout=180):
+    """HBM bytes per launch of the FP64 cell kernel from two rocprofv3 --pmc child runs of this
+    script's --pmc-workload (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what
+    the same counter reads for a device copy of one DoF vector (known bytes) in the same run.
+    Returns None when rocprofv3 is missing or a run fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None
+    env = dict(os.environ, TMPDIR="/tmp")
+    vals = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--pmc-workload"]
+            try:
+                subprocess.run(cmd, cwd="/tmp", env=env, timeout=timeout, check=True,
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+            except (subprocess.SubprocessError, OSError):
+                return None
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            if not files:
+                return None
+            kern, copy = [], []
+            for f in files:
+                for r in csv.DictReader(open(f)):
+                    if r.get("Counter_Name") != ctr:
+                        continue
+                    name, v = r["Kernel_Name"], float(r["Counter_Value"])
+                    if TRAFFIC_KERNEL in name:
+                        kern.append(v)
+                    elif COPY_KERNEL in name:
+                        copy.append(v)
+            if not kern or not copy:
+                return None
+            vals[ctr] = (sum(kern) / len(kern), sum(copy) / len(copy))
+    n_bytes = 7189057 * 8  # one FP64 DoF vector: the copy reads and writes this much
+    f_read = n_bytes / 1024.0 / vals["FETCH_SIZE"][1]
+    f_write = n_bytes / 1024.0 / vals["WRITE_SIZE"][1]
+    rd = vals["FETCH_SIZE"][0] * 1024.0 * f_read
+    wr = vals["WRITE_SIZE"][0] * 1024.0 * f_write
+    return {"bytes_per_launch": rd + wr, "read_bytes": rd, "write_bytes": wr,
+            "fetch_kib": vals["FETCH_SIZE"][0], "write_kib": vals["WRITE_SIZE"][0],
+            "fetch_scale": f_read, "write_scale": f_write,
+            "calibration": "device copy of one 7,189,057-entry FP64 vector (57.5 MB read + "
+                           "57.5 MB written) in the same run"}
 
 
 def cpu_baseline(T, mask, b, n, budget):

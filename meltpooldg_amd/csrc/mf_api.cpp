@@ -557,7 +557,7 @@ struct SolveOut {
 // Preconditioned CG on the device; T = double (FP64 operator) or float (FP32 operator).
 template <typename T>
 static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it, double abs_tol,
-                    double rel_tol, SolveOut &out)
+                    double rel_tol, SolveOut &out, bool x_zero)
 {
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
@@ -577,9 +577,11 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
     *prz1 = cv.take<T>(2 * RED_BLOCKS);
   double *pcell = cv.take<double>(ncb);
   SolveState *sd = cv.take<SolveState>(1);
-  if ((rc = apply_tangent<T>(op, q, x, nullptr, nullptr, true))) return rc;
+  // r0 = b - A x (x known to be zero: r0 = b, no operator application)
+  if (!x_zero && (rc = apply_tangent<T>(op, q, x, nullptr, nullptr, true))) return rc;
   // cg_check0 initialises every field of the state block
-  if ((rc = cg_init<T>(b, q, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st))) return rc;
+  if ((rc = cg_init<T>(b, x_zero ? nullptr : q, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st)))
+    return rc;
   T *prz[2] = {prz0, prz1};
   SolveState h{};
   int it = 0;
@@ -614,7 +616,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
 // of each cycle (deal.II SolverGMRES with right_preconditioning = true, linear_solver.hpp:112-119).
 template <typename T>
 static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int restart, int max_it,
-                       double abs_tol, double rel_tol, SolveOut &out)
+                       double abs_tol, double rel_tol, SolveOut &out, bool x_zero)
 {
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
@@ -646,8 +648,9 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
   bool first = true;
   for (;;) {
     T *V0 = V.p;
-    if ((rc = apply_tangent<T>(op, z.p, x, nullptr, nullptr, true))) return rc;
-    if ((rc = vec_residual<T>(b, z.p, V0, n, part.p, dh.p + 64, st))) return rc;
+    const bool r_is_b = first && x_zero;  // zero initial guess: r0 = b
+    if (!r_is_b && (rc = apply_tangent<T>(op, z.p, x, nullptr, nullptr, true))) return rc;
+    if ((rc = vec_residual<T>(b, r_is_b ? nullptr : z.p, V0, n, part.p, dh.p + 64, st))) return rc;
     MF_HIP_TRY(hipMemcpyAsync(hbuf + 64, dh.p + 64, sizeof(double), hipMemcpyDeviceToHost, st));
     MF_HIP_TRY(hipStreamSynchronize(st));
     double rho = hbuf[64];
@@ -720,11 +723,12 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
 
 template <typename T>
 static int krylov(mf_operator *op, const mf_solver_desc &d, T *x, const T *b, const T *inv,
-                  double abs_tol, double rel_tol, int max_it, SolveOut &out)
+                  double abs_tol, double rel_tol, int max_it, SolveOut &out, bool x_zero = false)
 {
-  if (d.solver == MF_SOLVER_CG) return solve_cg<T>(op, x, b, inv, max_it, abs_tol, rel_tol, out);
+  if (d.solver == MF_SOLVER_CG)
+    return solve_cg<T>(op, x, b, inv, max_it, abs_tol, rel_tol, out, x_zero);
   return solve_gmres<T>(op, x, b, inv, d.restart > 0 ? d.restart : 30, max_it, abs_tol, rel_tol,
-                        out);
+                        out, x_zero);
 }
 
 }  // namespace mf
@@ -817,7 +821,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
       if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
       if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
       SolveOut o;
-      if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32p, 0.0, inner_rel, max_it, o)))
+      if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32p, 0.0, inner_rel, max_it, o, true)))
         return rc;
       inner_total += o.iterations;
       if (o.status == MF_BREAKDOWN) {

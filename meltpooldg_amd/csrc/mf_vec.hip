@@ -125,12 +125,14 @@ __global__ void add_scaled_f32_kernel(double *x, const float *e, double scale, i
 
 // r = b - q, partials of r.r
 template <typename T>
-__global__ void residual_kernel(const T *b, const T *q, T *r, int64_t n, T *part)
+__global__ void residual_kernel(const T *__restrict__ b, const T *__restrict__ q,
+                                T *__restrict__ r, int64_t n, T *part)
 {
   Acc<T> a;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    const T v = b[i] - q[i];
+    const T v = q ? b[i] - q[i] : b[i];  // q = null: A x = 0 (zero initial guess)
     r[i] = v;
     a.add(v * v);
   }
@@ -138,9 +140,10 @@ __global__ void residual_kernel(const T *b, const T *q, T *r, int64_t n, T *part
 }
 
 template <typename T>
-__global__ void dot_kernel(const T *a, const T *b, int64_t n, T *part)
+__global__ void dot_kernel(const T *__restrict__ a, const T *__restrict__ b, int64_t n, T *part)
 {
   Acc<T> acc;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
     acc.add(a[i] * b[i]);
@@ -192,13 +195,15 @@ __global__ void invert_diag_kernel(double *d, int64_t n, double abs_thr, double 
 // ------------------------------------------------------------------------------------------
 // After q = A x:  r = b - q, z = D r, p = z, partial r.r and r.z (slot 0).
 template <typename T>
-__global__ void cg_init_kernel(const T *b, T *q, T *r, T *p, const T *inv, int64_t n,
+__global__ void cg_init_kernel(const T *__restrict__ b, const T *__restrict__ q, T *__restrict__ r,
+                               T *__restrict__ p, const T *__restrict__ inv, int64_t n,
                                T *part_rr, T *part_rz0)
 {
   Acc<T> arr, arz;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
-    const T v = b[i] - q[i];
+    const T v = q ? b[i] - q[i] : b[i];  // q = null: A x = 0 (zero initial guess)
     const T z = inv ? inv[i] * v : v;
     r[i] = v;
     p[i] = z;
@@ -241,7 +246,8 @@ __global__ void reduce_cell_dot_kernel(const double *part, int n, SolveState *st
 // alpha = rz/pAp; r -= alpha q; partial r.r, r.(D r) into slot it%2.  x is updated by the
 // direction kernel, which reads p anyway.
 template <typename T>
-__global__ void cg_update_kernel(T *r, const T *q, const T *inv, int64_t n,
+__global__ void cg_update_kernel(T *__restrict__ r, const T *__restrict__ q,
+                                 const T *__restrict__ inv, int64_t n,
                                  const T *part_rz_old, T *part_rz_new, T *part_rr, int nparts,
                                  SolveState *st)
 {
@@ -255,6 +261,7 @@ __global__ void cg_update_kernel(T *r, const T *q, const T *inv, int64_t n,
   const T alpha = T(double(rz) / pAp);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = double(alpha);
   Acc<T> arr, arz;
+#pragma unroll 4
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const T v = r[i] - alpha * q[i];
@@ -269,7 +276,8 @@ __global__ void cg_update_kernel(T *r, const T *q, const T *inv, int64_t n,
 // x += alpha p (this iteration's step), convergence check for step `it`, then
 // p = D r + beta p
 template <typename T>
-__global__ void cg_direction_kernel(T *x, T *p, const T *r, const T *inv, int64_t n,
+__global__ void cg_direction_kernel(T *__restrict__ x, T *__restrict__ p, const T *__restrict__ r,
+                                    const T *__restrict__ inv, int64_t n,
                                     const T *part_rz_new, const T *part_rz_old, const T *part_rr,
                                     int nparts, SolveState *st, int it, int max_it)
 {
@@ -288,7 +296,8 @@ __global__ void cg_direction_kernel(T *x, T *p, const T *r, const T *inv, int64_
   const bool done = res <= tol;
   const bool fail = !done && (it >= max_it || isnan(res));
   if (done || fail) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+  #pragma unroll 4
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
          i += int64_t(gridDim.x) * blockDim.x)
       x[i] += alpha * p[i];
     // every block sees the same values; block 0 records
@@ -302,6 +311,7 @@ __global__ void cg_direction_kernel(T *x, T *p, const T *r, const T *inv, int64_
   const T rz_new = reduce_partials(part_rz_new, nparts);
   const T rz_old = reduce_partials(part_rz_old, nparts);
   const T beta = T(double(rz_new) / double(rz_old));
+#pragma unroll 4
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const T pi = p[i];
