@@ -521,23 +521,40 @@ struct ListAccess {
   {
     return A.use_mask && A.mask && A.mask[n];
   }
-  __device__ Number coop_plain(const Number *f, int i) const
-  {
-    return ((abits >> i) & 1u) ? f[node[i]] : Number(0);
-  }
-  __device__ Number coop_value(const CellArgs<Number> &A, const Number *f, int i) const
-  {
-    if (!((abits >> i) & 1u)) return Number(0);
-    const uint32_t n = node[i];
-    // hanging node: its interpolated value, computed for the whole source beforehand
-    if ((hbits >> i) & 1u) return A.hang_val[A.node_line[n]];
-    return masked(A, n) ? Number(0) : f[n];
-  }
+  // Values of field f at the thread's slots.  The value, mask and constraint-line loads of all
+  // slots are issued before any is used (unused slots read node 0); only a hanging node's
+  // interpolated value waits for its line index.
   __device__ void coop_load(const CellArgs<Number> &A, const Number *f, bool cons,
                             Number (&v)[K]) const
   {
+    if (!f) {
 #pragma unroll
-    for (int i = 0; i < K; ++i) v[i] = f ? (cons ? coop_value(A, f, i) : coop_plain(f, i)) : Number(0);
+      for (int i = 0; i < K; ++i) v[i] = Number(0);
+      return;
+    }
+    Number raw[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) raw[i] = f[node[i]];
+    if (!cons) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) v[i] = ((abits >> i) & 1u) ? raw[i] : Number(0);
+      return;
+    }
+    const bool um = A.use_mask && A.mask;
+    uint32_t mb = 0u;
+    int32_t line[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      if (um) mb |= uint32_t(A.mask[node[i]] != 0) << i;
+      line[i] = ((hbits >> i) & 1u) ? A.node_line[node[i]] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      // hanging node: its interpolated value, computed for the whole source beforehand
+      const Number h = line[i] >= 0 ? A.hang_val[line[i]] : Number(0);
+      const bool keep = ((abits >> i) & 1u) && !((mb >> i) & 1u);
+      v[i] = !((abits >> i) & 1u) ? Number(0) : (line[i] >= 0 ? h : (keep ? raw[i] : Number(0)));
+    }
   }
   __device__ void coop_store(const Number (&v)[K], Number *T0, Number *T1) const
   {
@@ -672,10 +689,13 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr int I_A = I_W + (VEL ? 3 : 0);
   constexpr int N_EVAL = I_A + (RHS_HEAT ? 2 : 0);
   constexpr int N_GRAD = USE_T ? 4 : 3;
-  // with the fused dot, the gathered source is kept in one more tensor
+  // with the fused dot, the gathered source is kept: in registers (FP32 x pencils), or in one
+  // more tensor when it was gathered cooperatively or when K more registers would cost the FP64
+  // kernel a wave per SIMD (it sits just under 128)
   constexpr bool FDOT = DOT && TANGENT;
+  constexpr bool FDOT_LDS = FDOT && (Access::COOP || sizeof(Number) == 8);
   constexpr int I_SRC = cmax(cmax(N_EVAL, N_GRAD), IS_DIAG ? 5 : 0);
-  constexpr int NBUF = I_SRC + (FDOT ? 1 : 0);
+  constexpr int NBUF = I_SRC + (FDOT_LDS ? 1 : 0);
   __shared__ Number lds[NBUF * BUF];
   Number *const L0 = lds;
   Number *const L1 = lds + BUF;
@@ -718,6 +738,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
 
   // ---------------- E1: x pencils, gather from HBM ----------------
   Number dpre[K];  // dst entries of the pencil, read while the integration sweeps run
+  [[maybe_unused]] Number usrc[K];  // fused dot: the gathered source x pencil
   if constexpr (Access::COOP) {
     // cooperative gather into the LDS tensors (all fields' loads in flight together), then the
     // x interpolation in place
@@ -725,7 +746,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     if constexpr (HAS_U)
       ac.coop_load(A, RHS_HEAT ? A.T : src, TANGENT && !A.gather_plain, gu);
     if constexpr (USE_T) ac.coop_load(A, A.T, false, gt);
-    if constexpr (HAS_U) ac.coop_store(gu, L0, FDOT ? lds + I_SRC * BUF : nullptr);
+    if constexpr (HAS_U) ac.coop_store(gu, L0, FDOT_LDS ? lds + I_SRC * BUF : nullptr);
     if constexpr (USE_T) ac.coop_store(gt, lds + I_T * BUF, nullptr);
     if constexpr (VEL) {
 #pragma unroll
@@ -771,7 +792,12 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     if constexpr (HAS_U) {
       matvec<Number, K, false>(S, u_loc, o);
       store_pencil<Number, K, LY, 0>(L0, p, c, o);
-      if constexpr (FDOT) store_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u_loc);
+      if constexpr (FDOT_LDS) {
+        store_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u_loc);
+      } else if constexpr (FDOT) {
+#pragma unroll
+        for (int i = 0; i < K; ++i) usrc[i] = u_loc[i];
+      }
     }
     if constexpr (USE_T) {
       matvec<Number, K, false>(S, t_loc, o);
@@ -1090,7 +1116,12 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       // src . (A src) over the cell's nodes (constrained entries of the gathered source are 0),
       // before the x-shared node is combined across cells
       Number u[K];
-      load_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u);
+      if constexpr (FDOT_LDS) {
+        load_pencil<Number, K, LY, 0>(lds + I_SRC * BUF, p, c, u);
+      } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i) u[i] = usrc[i];
+      }
       Number local_dot = 0;
 #pragma unroll
       for (int i = 0; i < K; ++i) local_dot += u[i] * o[i];

@@ -151,13 +151,14 @@ __global__ void dot_kernel(const T *__restrict__ a, const T *__restrict__ b, int
 }
 
 // max |d| partials (no compensation needed)
-__global__ void absmax_kernel(const double *d, int64_t n, double *part)
+template <typename T>
+__global__ void absmax_kernel(const T *d, int64_t n, double *part)
 {
   __shared__ double red[16];
   double m = 0;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
-    m = fmax(m, fabs(d[i]));
+    m = fmax(m, fabs(double(d[i])));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -470,7 +471,7 @@ int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipSt
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st)
 {
   if (relative) {
-    hipLaunchKernelGGL(absmax_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, d, n, work);
+    hipLaunchKernelGGL(absmax_kernel<double>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, d, n, work);
     MF_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(invert_diag_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, d, n, 0.0,
                        1.0e-14, work, RED_BLOCKS);
@@ -498,6 +499,40 @@ template int vec_residual<double>(const double *, const double *, double *, int6
                                   double *, hipStream_t);
 template int vec_residual<float>(const float *, const float *, float *, int64_t, float *, double *,
                                  hipStream_t);
+
+// max over the partials of absmax_kernel
+__global__ void max_from_partials_kernel(const double *part, int nparts, double *out)
+{
+  __shared__ double red[16];
+  double m = 0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) m = fmax(m, part[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+    *out = m;
+  }
+}
+
+// whether x is zero everywhere (Vector::all_zero); work holds RED_BLOCKS + 1 doubles
+template <typename T>
+int vec_all_zero(const T *x, int64_t n, double *work, bool &zero, hipStream_t st)
+{
+  hipLaunchKernelGGL(absmax_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, x, n, work);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(max_from_partials_kernel, dim3(1), dim3(VEC_THREADS), 0, st, work,
+                     RED_BLOCKS, work + RED_BLOCKS);
+  MF_HIP_TRY(hipGetLastError());
+  double m = 1.0;
+  MF_HIP_TRY(hipMemcpyAsync(&m, work + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  zero = (m == 0.0);
+  return MF_OK;
+}
+template int vec_all_zero<double>(const double *, int64_t, double *, bool &, hipStream_t);
+template int vec_all_zero<float>(const float *, int64_t, double *, bool &, hipStream_t);
 
 template <typename T>
 int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st)

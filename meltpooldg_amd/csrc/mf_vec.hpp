@@ -26,6 +26,7 @@ int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipSt
 template <typename T> int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out, hipStream_t st);
 template <typename T> int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st);
 template <typename T> int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st);
+template <typename T> int vec_all_zero(const T *x, int64_t n, double *work, bool &zero, hipStream_t st);
 template <typename T> int vec_precond(T *z, const T *v, const T *inv, int64_t n, hipStream_t st);
 template <typename T> int vec_scale_dev(T *dst, const T *src, int64_t n, const double *s, int invert, hipStream_t st);
 
