@@ -549,6 +549,7 @@ namespace mf {
 
 
 struct SolveOut {
+  int first_chunk = 8;  // in: iterations enqueued before the first host check (CG)
   int iterations = 0;
   double res0 = 0, res = 0;
   int status = MF_OK;
@@ -585,23 +586,27 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   T *prz[2] = {prz0, prz1};
   SolveState h{};
   int it = 0;
-  int chunk = 8;
+  // Iterations are enqueued in chunks and the host reads the state after each chunk; kernels
+  // launched after convergence (or after the step-0 check succeeded) exit at once.
+  int chunk = std::max(1, out.first_chunk);
   for (;;) {
+    for (int k = 0; k < chunk && it < max_it; ++k) {
+      ++it;
+      if ((rc = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return rc;
+      if ((rc = cg_reduce_cell_dot(pcell, ncb, sd, st))) return rc;
+      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, sd, it, max_it, st))) return rc;
+    }
     MF_HIP_TRY(hipMemcpyAsync(&h, sd, sizeof(SolveState), hipMemcpyDeviceToHost, st));
     MF_HIP_TRY(hipStreamSynchronize(st));
     if (h.stop) break;
+    MF_REQUIRE(it < max_it, MF_ERR_STATE, "CG state not stopped at max_it");
+    chunk = 8;
     if (h.iterations > 0 && h.res > 0 && h.res < h.res0 && h.tol > 0) {
       // iterations left at the observed average rate (plus one), so that few launches run
       // after convergence and few host round trips are needed
       const double rate = std::log(h.res / h.res0) / h.iterations;
       const double left = std::log(h.tol / h.res) / rate;
       chunk = int(std::min(64.0, std::max(2.0, std::ceil(left) + 1.0)));
-    }
-    for (int k = 0; k < chunk; ++k) {
-      ++it;
-      if ((rc = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return rc;
-      if ((rc = cg_reduce_cell_dot(pcell, ncb, sd, st))) return rc;
-      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, sd, it, max_it, st))) return rc;
     }
   }
   out.iterations = h.iterations;
@@ -818,7 +823,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
     double res = 0.0;
     if ((rc = residual(res, x_zero))) return rc;
     const double res0 = res, tol = std::max(desc->abs_tol, desc->rel_tol * res0);
-    int outer = 0, inner_total = 0, status = MF_OK;
+    int outer = 0, inner_total = 0, last_inner = 0, status = MF_OK;
     while (!(res <= tol)) {
       if (outer >= max_outer || std::isnan(res)) {
         status = MF_NO_CONVERGENCE;
@@ -828,9 +833,13 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
       if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
       if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
       SolveOut o;
+      // the passes reduce the residual by about the same factor: enqueue the previous pass's
+      // iteration count before the first check
+      o.first_chunk = inner_total > 0 ? last_inner + 1 : 8;
       if ((rc = krylov<float>(op, *desc, e32.p, r32.p, inv32p, 0.0, inner_rel, max_it, o, true)))
         return rc;
       inner_total += o.iterations;
+      last_inner = o.iterations;
       if (o.status == MF_BREAKDOWN) {
         status = MF_BREAKDOWN;
         break;
