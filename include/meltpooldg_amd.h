@@ -151,6 +151,28 @@ int mf_operator_reset_dof_index(mf_operator *op, int dof_index);
 int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
                            const double *k_cell);
 
+/* Robin / Neumann boundary conditions of the heat operator on the sides of a structured box mesh:
+ * side s = 2*direction + (0 lower, 1 upper), the boundary ids of a colourised
+ * hyper_rectangle.  Convection alpha (T - T_inf) and radiation eps sigma (T^4 - T_inf^4)
+ * (HeatData::convection / ::radiation, include/meltpooldg/heat/heat_data.hpp:46-56) enter the
+ * tangent (alpha u + 4 eps sigma T^3 u, tangent_boundary_loop, heat_diffuse_operator.cpp:243-269,
+ * 1088-1129), its diagonal and create_rhs (rhs_boundary_loop, :719-781); a Neumann side adds
+ * -(q.n) to create_rhs, here with a constant normal flux per side.  A side may not carry Neumann
+ * together with radiation or convection (the reference's AssertThrow at :743-746). */
+typedef struct {
+  unsigned convection_sides; /* bit s: side s has convection */
+  unsigned radiation_sides;
+  unsigned neumann_sides;
+  double convection_coefficient;
+  double convection_temperature_infinity;
+  double emissivity;
+  double radiation_temperature_infinity;
+  double neumann_flux[6]; /* q.n on each Neumann side */
+} mf_heat_boundary;
+
+/* NULL removes all boundary terms.  Heat operators on structured meshes only. */
+int mf_operator_set_boundary(mf_operator *op, const mf_heat_boundary *b);
+
 /* OperatorMatrixFree::vmult (operator_base.hpp:239-243; heat_diffuse_operator.cpp:179-197,
  * advection_diffusion_operator.cpp:236-272): dst = A src, dst zeroed first, constrained rows 0 */
 int mf_operator_vmult(mf_operator *op, double *dst, const double *src);

@@ -158,6 +158,9 @@ class HeatDiffuseOperator : public OperatorMatrixFree {
     check(mf_operator_set_fields(h_, T ? T->data() : nullptr, velocity ? velocity->data() : nullptr,
                                  k_cell ? k_cell->data() : nullptr));
   }
+  // convection / radiation / Neumann sides (HeatData::convection, ::radiation and the boundary
+  // id lists of the reference); nullptr removes them
+  void set_boundary(const mf_heat_boundary *b) { check(mf_operator_set_boundary(h_, b)); }
 };
 
 // AdvectionDiffusionOperator (source/level_set/advection_diffusion_operator.cpp) on the device.

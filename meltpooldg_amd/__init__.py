@@ -89,6 +89,15 @@ class SolveReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class HeatBoundary(C.Structure):
+    """mf_heat_boundary: Robin / Neumann sides of the heat operator, side s = 2*direction +
+    (0 lower, 1 upper) (colourised hyper_rectangle boundary ids)."""
+    _fields_ = [("convection_sides", C.c_uint), ("radiation_sides", C.c_uint),
+                ("neumann_sides", C.c_uint), ("convection_coefficient", C.c_double),
+                ("convection_temperature_infinity", C.c_double), ("emissivity", C.c_double),
+                ("radiation_temperature_infinity", C.c_double), ("neumann_flux", C.c_double * 6)]
+
+
 def build(force: bool = True) -> str:
     """Compile libmeltpooldg_amd.so for gfx950 with hipcc (Makefile in csrc/)."""
     args = ["make", "-s", "-j4", "-C", os.path.join(_HERE, "csrc")]
@@ -146,6 +155,7 @@ def lib():
         L.mf_operator_reset_time_increment.argtypes = [vp, C.c_double]
         L.mf_operator_reset_dof_index.argtypes = [vp, C.c_int]
         L.mf_operator_set_fields.argtypes = [vp, vp, vp, vp]
+        L.mf_operator_set_boundary.argtypes = [vp, C.POINTER(HeatBoundary)]
         L.mf_operator_vmult.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_f32.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_add.argtypes = [vp, vp, vp]
@@ -446,6 +456,27 @@ class HeatDiffuseOperator(_OperatorBase):
         super().__init__(scratch, h)
         if dt is not None:
             self.reset_time_increment(dt)
+
+    def set_boundary(self, convection_sides=(), radiation_sides=(), neumann_sides=None,
+                     convection_coefficient=0.0, convection_temperature_infinity=0.0,
+                     emissivity=0.0, radiation_temperature_infinity=0.0):
+        """Robin (convection, radiation) and Neumann sides; neumann_sides maps side -> q.n.
+        Sides are 0..5 = 2*direction + (0 lower, 1 upper)."""
+        b = HeatBoundary()
+        b.convection_sides = sum(1 << int(x) for x in convection_sides)
+        b.radiation_sides = sum(1 << int(x) for x in radiation_sides)
+        for side, g in (neumann_sides or {}).items():
+            b.neumann_sides |= 1 << int(side)
+            b.neumann_flux[int(side)] = float(g)
+        b.convection_coefficient = convection_coefficient
+        b.convection_temperature_infinity = convection_temperature_infinity
+        b.emissivity = emissivity
+        b.radiation_temperature_infinity = radiation_temperature_infinity
+        _check(lib().mf_operator_set_boundary(self._h, C.byref(b)))
+        return b
+
+    def clear_boundary(self):
+        _check(lib().mf_operator_set_boundary(self._h, None))
 
 
 class AdvectionDiffusionOperator(_OperatorBase):

@@ -278,9 +278,15 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     return launch_condense<Number>(op->s, dst, stop);
   }
   // the transposed operator (adjoint) runs on the pencil kernel
-  if (slab)
-    return launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
-  return launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
+  int rc = slab ? launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st)
+                : launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
+  if (rc) return rc;
+  // Robin boundary faces (symmetric: the same for the transposed operator); their part of the
+  // fused dot follows the cell kernel's partials
+  return launch_heat_faces<Number>(
+      op, src, dst, 0, stop,
+      dot_partial ? dot_partial + tangent_blocks(m, op->transpose != 0, sizeof(Number) == 4)
+                  : nullptr);
 }
 
 int g_tangent_variant = 0;
@@ -516,6 +522,8 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   } else {
     a.overwrite = 1;
     rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
+    if (rc) return rc;
+    rc = launch_heat_faces<double>(op, nullptr, inv_diag, 1, nullptr);
   }
   if (rc) return rc;
   double *work = nullptr;
@@ -536,8 +544,31 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
   CellArgs<double> a = make_args<double>(op);
   a.aux1 = q_s;
   const int code = op->kind == 0 ? HEAT_RHS : ADVEC_RHS;
-  return launch_cell<double>(code, needs_T(op), op->vel != nullptr, op->s->mesh, a, src, dst,
-                             op->s->stream);
+  int rc2 = launch_cell<double>(code, needs_T(op), op->vel != nullptr, op->s->mesh, a, src, dst,
+                                op->s->stream);
+  if (rc2) return rc2;
+  return launch_heat_faces<double>(op, nullptr, dst, 2, nullptr);
+}
+
+int mf_operator_set_boundary(mf_operator *op, const mf_heat_boundary *b)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  if (!b) {
+    op->bnd_set = false;
+    return MF_OK;
+  }
+  MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "boundary terms belong to the heat operator");
+  MF_REQUIRE(!op->s->mesh.ad, MF_ERR_ARG, "boundary faces are not available on adaptive meshes");
+  const unsigned all = b->convection_sides | b->radiation_sides | b->neumann_sides;
+  MF_REQUIRE((all & ~0x3Fu) == 0, MF_ERR_ARG, "side bits beyond the 6 sides of the box");
+  MF_REQUIRE((b->neumann_sides & (b->convection_sides | b->radiation_sides)) == 0, MF_ERR_ARG,
+             "It is not allowed to specify both Neumann and radiation and/or convection boundary "
+             "conditions at the same face.");
+  MF_REQUIRE(b->convection_coefficient >= 0.0 && b->emissivity >= 0.0, MF_ERR_ARG,
+             "convection coefficient and emissivity must be non-negative");
+  op->bnd = *b;
+  op->bnd_set = all != 0;
+  return MF_OK;
 }
 
 }  // extern "C"
@@ -563,7 +594,8 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   const MeshInfo &m = op->s->mesh;
   const int64_t n = m.n_dofs;
   hipStream_t st = op->s->stream;
-  const int ncb = tangent_blocks(m, op->transpose != 0, sizeof(T) == 4);
+  // per-block partials of the fused p.(Ap): cell kernel blocks, then boundary-face blocks
+  const int ncb = tangent_blocks(m, op->transpose != 0, sizeof(T) == 4) + heat_face_blocks(op);
   Carve cv;
   cv.plan<T>(n);
   cv.plan<T>(n);

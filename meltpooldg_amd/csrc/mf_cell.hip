@@ -88,11 +88,19 @@ __device__ __forceinline__ int pencil(int p, int e)
 }
 
 // LDS layout of a cell tensor: [K^3][CPB] (cell fastest, so that the 16-lane groups of a wave,
-// which serve 16 different cells, hit 16 different 8-byte bank pairs).
-template <int K, int CPB_>
+// which serve 16 different cells, hit 16 different 8-byte bank pairs).  FP32 accesses are served
+// in 32-lane groups, two pencils of 16 cells; for x pencils those sit 4 element rows apart, in
+// the same banks, so the FP32 layout swaps element rows e and e ^ 1 where bit 2 of e is set
+// (a bijection that puts such pairs into opposite bank halves and keeps y and z pencils apart).
+template <typename Number, int K, int CPB_>
 struct Layout {
   static constexpr int CPB = CPB_;
-  __device__ static __forceinline__ int idx(int e, int c) { return e * CPB + c; }
+  static constexpr bool SWZ = sizeof(Number) == 4 && K == 4;
+  __device__ static __forceinline__ int idx(int e, int c)
+  {
+    if constexpr (SWZ) return (e ^ ((e >> 2) & 1)) * CPB + c;
+    return e * CPB + c;
+  }
 };
 
 template <typename Number, int K, class LY, int DIR>
@@ -515,7 +523,7 @@ struct ListAccess {
   __device__ int coop_pos(int i) const
   {
     const int j = int(threadIdx.x) + i * int(blockDim.x);
-    return (j % NLOC) * CPB + j / NLOC;
+    return Layout<Number, K, CPB>::idx(j % NLOC, j / NLOC);
   }
   __device__ bool masked(const CellArgs<Number> &A, uint32_t n) const
   {
@@ -578,7 +586,7 @@ struct ListAccess {
   {
     const int p = int(threadIdx.x) / CPB;
     __syncthreads();  // the stage may still be read by other threads
-    store_pencil<Number, K, Layout<K, CPB>, 0>(stage, p, c, o);
+    store_pencil<Number, K, Layout<Number, K, CPB>, 0>(stage, p, c, o);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -703,7 +711,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
 
   if (A0.stop && *A0.stop) return;  // uniform over the grid
 
-  using LY = Layout<K, CPB>;
+  using LY = Layout<Number, K, CPB>;
   int c, p;
   Access::map(int(threadIdx.x), c, p);
   const int qi = p % K, qj = p / K;

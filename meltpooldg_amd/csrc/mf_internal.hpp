@@ -4,12 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/meltpooldg_amd.h"
 
 struct mf_scratch;
+struct mf_operator;
 
 namespace mf {
 
@@ -189,6 +191,17 @@ template <typename Number>
 int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                      const Number *src, Number *dst, hipStream_t stream, int mode);
 
+// Stefan-Boltzmann constant in W/(m^2 K^4) (include/meltpooldg/utilities/physical_constants.hpp:6)
+#define MF_STEFAN_BOLTZMANN 5.670374419e-8
+
+// boundary faces of the heat operator (mf_face.hip): mode 0 dst += B src, 1 dst += diag(B),
+// 2 dst += boundary part of -R(T); nothing when no side is active.  Mode 0 writes
+// heat_face_blocks(op) partials of src . (B src) when dot_partial is given.
+template <typename Number>
+int launch_heat_faces(const mf_operator *op, const Number *src, Number *dst, int mode,
+                      const int *stop, double *dot_partial = nullptr);
+int heat_face_blocks(const mf_operator *op);
+
 // number of blocks the cell kernel uses (size of dot_partial) in FP64 or FP32
 int cell_blocks(const MeshInfo &m, bool fp32);
 
@@ -236,5 +249,7 @@ struct mf_operator {
   float *T32 = nullptr, *vel32 = nullptr, *kc32 = nullptr;  // FP32 copies of the fields
   bool fields32_valid = false;
   int transpose = 0;  // vmult applies A^T (adjoint solves)
+  mf_heat_boundary bnd{};  // Robin / Neumann sides (mf_operator_set_boundary)
+  bool bnd_set = false;
   int gather_plain = 0;  // vmult reads src without constraints (source mass products)
 };

@@ -590,6 +590,115 @@ static void vmult_cell(const orc_operator *op, const geo_t *g, long cell, cell_w
 static void avmult(const orc_operator *op, const double *src, double *dst);
 static void adiagonal(const orc_operator *op, double *diag);
 
+/* Boundary faces of the heat operator by textbook face quadrature (QGauss(p+1) in the
+ * tangential directions, basis = products of 1D Lagrange values; on a face of the box the
+ * normal-direction basis is 1 on the face nodes and 0 elsewhere).
+ * mode 0: dst += B src, the Robin tangent alpha u + 4 eps sigma T^3 u
+ *         (tangent_boundary_loop / tangent_local_boundary_operation,
+ *          heat_diffuse_operator.cpp:243-269, 1088-1129);
+ * mode 1: dst += diag(B) (the boundary operation of compute_diagonal, :297-391);
+ * mode 2: dst += the boundary part of -R(T): -(q.n) on Neumann sides,
+ *         -eps sigma (T^4 - T_inf^4) and -alpha (T - T_inf) (rhs_boundary_loop, :719-781).
+ * T is read plain, src with the Dirichlet constraints; constrained rows are skipped. */
+#define ORC_STEFAN_BOLTZMANN 5.670374419e-8 /* include/meltpooldg/utilities/physical_constants.hpp:6 */
+
+static void heat_faces(const orc_operator *op, const geo_t *g, const double *src, double *dst,
+                       int mode)
+{
+  const orc_heat_boundary *b = op->bnd;
+  if (!b || op->kind != ORC_OP_HEAT) return;
+  const int K = g->K, dim = g->dim, P = g->p;
+  for (int side = 0; side < 2 * dim; ++side) {
+    const unsigned bit = 1u << side;
+    const int conv = (b->convection_sides & bit) != 0;
+    const int rad = (b->radiation_sides & bit) != 0;
+    const int neu = (b->neumann_sides & bit) != 0;
+    if (!conv && !rad && !(mode == 2 && neu)) continue;
+    const int nd = side / 2, upper = side % 2;
+    int td[2], ntd = 0;
+    for (int d = 0; d < dim; ++d)
+      if (d != nd) td[ntd++] = d;
+    long nf = 1;
+    for (int t = 0; t < ntd; ++t) nf *= g->ncell[td[t]];
+    int nfn = 1;
+    for (int t = 0; t < ntd; ++t) nfn *= K;
+    for (long f = 0; f < nf; ++f) {
+      long c[3] = {0, 0, 0};
+      long r = f;
+      for (int t = 0; t < ntd; ++t) {
+        c[td[t]] = r % g->ncell[td[t]];
+        r /= g->ncell[td[t]];
+      }
+      c[nd] = upper ? g->ncell[nd] - 1 : 0;
+      /* face nodes (tangential local indices a[t]) and their global numbers */
+      long gi[KMAX * KMAX];
+      double uu[KMAX * KMAX], TT[KMAX * KMAX], out[KMAX * KMAX];
+      for (int l = 0; l < nfn; ++l) {
+        int a[3] = {0, 0, 0};
+        int rr = l;
+        for (int t = 0; t < ntd; ++t) {
+          a[td[t]] = rr % K;
+          rr /= K;
+        }
+        a[nd] = upper ? P : 0;
+        long idx = 0;
+        for (int d = 0; d < dim; ++d) idx += (c[d] * P + a[d]) * g->stride[d];
+        gi[l] = idx;
+        const int cons = op->use_mask && op->mask && op->mask[idx];
+        uu[l] = (mode == 0 && !cons) ? src[idx] : 0.0;
+        TT[l] = op->T ? op->T[idx] : 0.0;
+        out[l] = 0.0;
+      }
+      for (int q = 0; q < nfn; ++q) {
+        int qa[2] = {0, 0};
+        int rq = q;
+        double wq = 1.0;
+        for (int t = 0; t < ntd; ++t) {
+          qa[t] = rq % K;
+          rq /= K;
+          wq *= g->wq[qa[t]] * g->h[td[t]];
+        }
+        double phi[KMAX * KMAX];
+        double Tq = 0.0, uq = 0.0;
+        for (int l = 0; l < nfn; ++l) {
+          double ph = 1.0;
+          int rr = l;
+          for (int t = 0; t < ntd; ++t) {
+            double v[KMAX], dv[KMAX];
+            orc_lagrange(K, g->gl, g->xq[qa[t]], v, dv);
+            ph *= v[rr % K];
+            rr /= K;
+          }
+          phi[l] = ph;
+          Tq += ph * TT[l];
+          uq += ph * uu[l];
+        }
+        const double sb = ORC_STEFAN_BOLTZMANN;
+        const double coef = (conv ? b->convection_coefficient : 0.0) +
+                            (rad ? 4.0 * b->emissivity * sb * Tq * Tq * Tq : 0.0);
+        double val = 0.0;
+        if (mode == 2) {
+          if (neu) val -= b->neumann_flux[side];
+          if (rad) {
+            const double Ti = b->radiation_temperature_infinity;
+            val -= b->emissivity * sb * (Tq * Tq * Tq * Tq - Ti * Ti * Ti * Ti);
+          }
+          if (conv) val -= b->convection_coefficient * (Tq - b->convection_temperature_infinity);
+        }
+        for (int l = 0; l < nfn; ++l) {
+          if (mode == 0) out[l] += wq * coef * uq * phi[l];
+          else if (mode == 1) out[l] += wq * coef * phi[l] * phi[l];
+          else out[l] += wq * val * phi[l];
+        }
+      }
+      for (int l = 0; l < nfn; ++l) {
+        if (op->use_mask && op->mask && op->mask[gi[l]]) continue;
+        dst[gi[l]] += out[l];
+      }
+    }
+  }
+}
+
 void orc_vmult(const orc_operator *op, const double *src, double *dst)
 {
   if (op->amesh) {
@@ -602,6 +711,7 @@ void orc_vmult(const orc_operator *op, const double *src, double *dst)
   memset(dst, 0, sizeof(double) * n);
   io_t io = {src, dst, NULL};
   cell_loop(op, &g, vmult_cell, &io);
+  heat_faces(op, &g, src, dst, 0);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -748,6 +858,7 @@ void orc_vmult_dense(const orc_operator *op, const double *src, double *dst)
   memset(dst, 0, sizeof(double) * n);
   io_t io = {src, dst, NULL};
   cell_loop(op, &g, dense_cell, &io);
+  heat_faces(op, &g, src, dst, 0);
 }
 
 /* diagonal of the constrained operator: sum over cells of the local diagonal entries of the
@@ -787,6 +898,7 @@ void orc_diagonal(const orc_operator *op, double *diag)
   memset(diag, 0, sizeof(double) * n);
   io_t io = {NULL, diag, NULL};
   cell_loop(op, &g, diag_cell, &io);
+  heat_faces(op, &g, NULL, diag, 1);
 }
 
 /* heat: |d| > 1e-10 ? 1/d : 1  (heat_diffuse_operator.cpp:282-283)
@@ -866,6 +978,7 @@ void orc_create_rhs(const orc_operator *op, const double *src, const double *q_s
   geo_init(&op->mesh, &g);
   io_t io = {src, dst, q_s};
   cell_loop(op, &g, rhs_cell, &io);
+  heat_faces(op, &g, NULL, dst, 2);
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -88,6 +88,21 @@ typedef struct {
   double *line_val;        /* weights */
 } orc_adata;
 
+/* Robin / Neumann boundary conditions of the heat operator on the sides of the box: side
+ * s = 2*direction + (0 lower, 1 upper), the boundary ids of a colourised hyper_rectangle.
+ * HeatData::convection / ::radiation (include/meltpooldg/heat/heat_data.hpp:46-56) and the
+ * Neumann flux function, here a constant normal flux q.n per side. */
+typedef struct {
+  unsigned convection_sides;
+  unsigned radiation_sides;
+  unsigned neumann_sides;
+  double convection_coefficient;
+  double convection_temperature_infinity;
+  double emissivity;
+  double radiation_temperature_infinity;
+  double neumann_flux[6];
+} orc_heat_boundary;
+
 typedef struct {
   int kind;                 /* ORC_OP_HEAT or ORC_OP_ADVEC */
   orc_mesh mesh;
@@ -103,6 +118,7 @@ typedef struct {
   int nthreads;             /* OpenMP threads, <= 0: library default */
   const orc_amesh *amesh;   /* adaptive mesh (NULL: the structured mesh above) */
   const orc_adata *adata;   /* its nodes and constraints */
+  const orc_heat_boundary *bnd; /* heat: Robin / Neumann sides (NULL: none) */
 } orc_operator;
 
 /* --- basis / mesh ----------------------------------------------------------------------- */

@@ -50,12 +50,25 @@ class ADataC(C.Structure):
                 ("line_val", C.POINTER(C.c_double))]
 
 
+class HeatBoundary(C.Structure):
+    """Robin / Neumann sides of the heat operator (orc_heat_boundary): side s = 2*direction +
+    (0 lower, 1 upper)."""
+    _fields_ = [("convection_sides", C.c_uint), ("radiation_sides", C.c_uint),
+                ("neumann_sides", C.c_uint), ("convection_coefficient", C.c_double),
+                ("convection_temperature_infinity", C.c_double), ("emissivity", C.c_double),
+                ("radiation_temperature_infinity", C.c_double), ("neumann_flux", C.c_double * 6)]
+
+
+STEFAN_BOLTZMANN = 5.670374419e-8
+
+
 class Operator(C.Structure):
     _fields_ = [("kind", C.c_int), ("mesh", Mesh), ("heat", HeatParams), ("advec", AdvecParams),
                 ("vel_kind", C.c_int), ("mask", C.c_void_p), ("use_mask", C.c_int),
                 ("T", C.c_void_p), ("T_old", C.c_void_p), ("vel", C.c_void_p),
                 ("k_cell", C.c_void_p), ("nthreads", C.c_int),
-                ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC))]
+                ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC)),
+                ("bnd", C.POINTER(HeatBoundary))]
 
 
 def build(force: bool = False) -> str:
@@ -200,7 +213,8 @@ class OracleOperator:
     """Holds the numpy arrays an ``orc_operator`` points to, so they stay alive."""
 
     def __init__(self, kind, mesh, heat=None, advec=None, vel_kind=VEL_NONE, mask=None,
-                 T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None):
+                 T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None,
+                 boundary=None):
         self._keep = []
         self.amesh = amesh
         if amesh is not None:
@@ -231,6 +245,9 @@ class OracleOperator:
         if amesh is not None:
             self.op.amesh = C.pointer(amesh.c)
             self.op.adata = C.pointer(amesh.d)
+        if boundary is not None:
+            self._bnd = boundary
+            self.op.bnd = C.pointer(boundary)
 
     def vmult(self, src):
         dst = np.zeros(self.n)

@@ -53,7 +53,21 @@ def fields(case: Case):
     return out
 
 
-def oracle_operator(case: Case, f, use_mask=1):
+def oracle_boundary(spec):
+    """orc.HeatBoundary from a dict of HeatDiffuseOperator.set_boundary keyword arguments."""
+    b = orc.HeatBoundary()
+    b.convection_sides = sum(1 << s_ for s_ in spec.get("convection_sides", ()))
+    b.radiation_sides = sum(1 << s_ for s_ in spec.get("radiation_sides", ()))
+    for side, g in spec.get("neumann_sides", {}).items():
+        b.neumann_sides |= 1 << side
+        b.neumann_flux[side] = g
+    for key in ("convection_coefficient", "convection_temperature_infinity", "emissivity",
+                "radiation_temperature_infinity"):
+        setattr(b, key, spec.get(key, 0.0))
+    return b
+
+
+def oracle_operator(case: Case, f, use_mask=1, boundary=None):
     m = case.mesh()
     if case.kind == "heat":
         hp = orc.HeatParams(law=orc.LAW_TANH if case.law == "tanh" else orc.LAW_CONSTANT,
@@ -62,7 +76,8 @@ def oracle_operator(case: Case, f, use_mask=1):
         return orc.OracleOperator(orc.OP_HEAT, m, heat=hp,
                                   vel_kind=orc.VEL_NODAL if case.vel else orc.VEL_NONE,
                                   mask=f["mask"], T=f["T"], T_old=f["T_old"], vel=f["vel"],
-                                  k_cell=f["k_cell"], use_mask=use_mask)
+                                  k_cell=f["k_cell"], use_mask=use_mask,
+                                  boundary=oracle_boundary(boundary) if boundary else None)
     ap = orc.AdvecParams(dt=case.dt, theta=case.theta, diffusivity=case.diffusivity,
                          mass_factor=case.mass_factor, vel_scale=0.0)
     return orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=f["mask"],

@@ -65,3 +65,23 @@ def test_argument_errors_are_reported():
 def test_too_large_mesh_rejected():
     with pytest.raises(mf.MFError):
         mf.ScratchData(3, (1000, 1000, 1000))
+
+
+def test_boundary_argument_errors():
+    """mf_operator_set_boundary validation (host side only)."""
+    s = mf.ScratchData(2, (2, 2, 2))
+    op = mf.HeatDiffuseOperator(s, dt=1e-3)
+    with pytest.raises(mf.MFError) as e:
+        op.set_boundary(convection_sides=[1], neumann_sides={1: 2.0})
+    assert "Neumann" in str(e.value)
+    with pytest.raises(mf.MFError):
+        op.set_boundary(convection_sides=[6])
+    with pytest.raises(mf.MFError):
+        op.set_boundary(convection_sides=[0], convection_coefficient=-1.0)
+    op.set_boundary(convection_sides=[0, 5], convection_coefficient=1.0)
+    op.clear_boundary()
+    adv = mf.AdvectionDiffusionOperator(s, dt=1e-3)
+    L = mf.lib()
+    b = mf.HeatBoundary()
+    b.convection_sides = 1
+    assert L.mf_operator_set_boundary(adv._h, C.byref(b)) == mf.MF_ERR_ARG

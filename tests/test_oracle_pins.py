@@ -278,3 +278,106 @@ def test_random_field_is_numbering_independent_and_uniform():
     c = orc.random_field(m2, 1)
     n0 = 13 * 13 * 13
     assert np.array_equal(a[:n0], c[:n0])
+
+
+# --- boundary faces of the heat operator (Robin / Neumann), textbook face quadrature ---------
+
+def _face_ops(degree=2, ncell=(2, 3, 2), hi=(1.0, 1.5, 0.7), T=None, **bnd):
+    m = orc.make_mesh(3, degree, ncell, (0, 0, 0), hi)
+    n = orc.n_dofs(m)
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, t_dependent=0, rho_cp=1.0, k=1.0, dt=1e-3)
+    b = orc.HeatBoundary()
+    for key, val in bnd.items():
+        if key == "neumann":
+            for side, g in val.items():
+                b.neumann_sides |= 1 << side
+                b.neumann_flux[side] = g
+        elif key.endswith("_sides"):
+            setattr(b, key, sum(1 << s_ for s_ in val))
+        else:
+            setattr(b, key, val)
+    T = np.zeros(n) if T is None else T
+    kw = dict(heat=hp, mask=np.zeros(n, np.uint8), T=T, T_old=T)
+    return m, n, orc.OracleOperator(orc.OP_HEAT, m, **kw), \
+        orc.OracleOperator(orc.OP_HEAT, m, boundary=b, **kw)
+
+
+def _side_area(hi, side):
+    d = side // 2
+    return np.prod([hi[e] for e in range(3) if e != d])
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_convection_face_term_integrates_constants(degree):
+    """sum_i (B 1)_i = alpha |faces|: the Robin term of u = 1 integrates alpha over the sides."""
+    hi = (1.0, 1.5, 0.7)
+    m, n, op0, opb = _face_ops(degree, hi=hi, convection_sides=[0, 3, 5],
+                               convection_coefficient=2.5)
+    one = np.ones(n)
+    face = opb.vmult(one) - op0.vmult(one)
+    area = sum(_side_area(hi, s_) for s_ in (0, 3, 5))
+    assert abs(face.sum() - 2.5 * area) < 1e-12 * 2.5 * area
+
+
+def test_radiation_face_term_uses_T_cubed():
+    """Radiation tangent 4 eps sigma T^3 u: with T constant it is a mass term on the side."""
+    hi = (1.0, 1.5, 0.7)
+    m = orc.make_mesh(3, 2, (2, 3, 2), (0, 0, 0), hi)
+    Tc = 1700.0
+    _, n, op0, opb = _face_ops(2, hi=hi, T=np.full(orc.n_dofs(m), Tc), radiation_sides=[2],
+                               emissivity=0.4)
+    one = np.ones(n)
+    face = opb.vmult(one) - op0.vmult(one)
+    expect = 4 * 0.4 * orc.STEFAN_BOLTZMANN * Tc ** 3 * _side_area(hi, 2)
+    assert abs(face.sum() - expect) < 1e-12 * expect
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_face_term_linear_moment(degree):
+    """A linear field u = y on the side x = 1: sum_i (B u)_i = alpha * int y ds = alpha A y_c."""
+    hi = (1.0, 1.5, 0.7)
+    m, n, op0, opb = _face_ops(degree, hi=hi, convection_sides=[1], convection_coefficient=1.7)
+    X = orc.node_coords(m)
+    u = X[:, 1]
+    face = opb.vmult(u) - op0.vmult(u)
+    expect = 1.7 * _side_area(hi, 1) * hi[1] / 2
+    assert abs(face.sum() - expect) < 1e-12 * expect
+
+
+def test_face_term_symmetric_and_diagonal_consistent():
+    """B is symmetric (v.Bu = u.Bv) and its diagonal (mode of compute_diagonal) equals the
+    diagonal of B assembled from unit vectors."""
+    m, n, op0, opb = _face_ops(2, ncell=(2, 2, 1), hi=(1.0, 1.0, 0.5),
+                               T=1500.0 + 100.0 * orc.random_field(orc.make_mesh(3, 2, (2, 2, 1), (0, 0, 0), (1.0, 1.0, 0.5)), 3),
+                               convection_sides=[0, 4], radiation_sides=[3, 5],
+                               convection_coefficient=3.0, emissivity=0.8)
+    B = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        B[:, i] = opb.vmult(e) - op0.vmult(e)
+    assert np.abs(B - B.T).max() < 1e-12 * np.abs(B).max()
+    dB = opb.diagonal() - op0.diagonal()
+    assert np.abs(dB - np.diag(B)).max() < 1e-12 * np.abs(B).max()
+
+
+def test_face_rhs_terms():
+    """Boundary part of -R(T): zero for T = T_inf (convection and radiation), and
+    sum_i = -g |side| on a Neumann side."""
+    hi = (1.0, 1.5, 0.7)
+    m = orc.make_mesh(3, 2, (2, 3, 2), (0, 0, 0), hi)
+    n = orc.n_dofs(m)
+    Tinf = 900.0
+    T = np.full(n, Tinf)
+    _, _, op0, opb = _face_ops(2, hi=hi, T=T, convection_sides=[0, 1], radiation_sides=[2, 3],
+                               convection_coefficient=5.0, convection_temperature_infinity=Tinf,
+                               emissivity=0.7, radiation_temperature_infinity=Tinf,
+                               neumann={5: 2.0})
+    r = opb.create_rhs(T) - op0.create_rhs(T)
+    # T at the quadrature points is 900 (1 + O(1e-16)); T^4 - T_inf^4 then carries 4 T^4 times
+    # that, i.e. ~1e-11 after the eps sigma factor
+    assert abs(r.sum() + 2.0 * _side_area(hi, 5)) < 1e-9
+    # only the Neumann side contributes
+    X = orc.node_coords(m)
+    off = np.abs(X[:, 2] - hi[2]) > 1e-12
+    assert np.abs(r[off]).max() < 1e-10
