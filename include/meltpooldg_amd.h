@@ -190,6 +190,46 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Level-set normal vector and curvature projections.  Replaces NormalVectorOperator           */
+/* (source/level_set/normal_vector_operator.cpp:117-366) and CurvatureOperator                  */
+/* (source/level_set/curvature_operator.cpp:160-371) in their matrix-free form.  Tangent:       */
+/* a(u,v) = sum_q m_q (u v + eta grad u . grad v) JxW with eta = filter_parameter *             */
+/* (cell diameter / sqrt(3))^2 per cell (compute_cell_size_dependent_filter_parameter_mf,       */
+/* normal_vector_operator.hpp:95-112) and the narrow-band indicator m_q = |phi_q| < threshold   */
+/* (Tools::compute_mask_narrow_band, level_set_tools.hpp:359-368), else 1.  The normal vector  */
+/* operator acts on block vectors [3][n_dofs] (BlockVectorType), the same scalar operator on    */
+/* every block; the curvature operator on [n_dofs].  No Dirichlet rows (the normal and          */
+/* curvature DoF indices carry hanging-node constraints only).  vmult, vmult_f32,               */
+/* compute_inverse_diagonal (|d| > 1e-14 max(1, |d|_inf) ? 1/d : 1), create_rhs, mf_solve and   */
+/* the Jacobi object take these operators like the others; no time increment is needed.        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+  double filter_parameter;        /* NormalVectorData / CurvatureData::filter_parameter (2.0) */
+  int narrow_band;                /* narrow_band.enable (0) */
+  double narrow_band_threshold;   /* narrow_band.level_set_threshold (0.9999999) */
+  int compute_normalized_vector;  /* normal vector rhs submits grad phi / |grad phi| (0) */
+} mf_levelset_projection_params;
+
+/* NormalVectorOperator: create_rhs(dst [3][n_dofs], src = level set [n_dofs]) adds
+ * sum_q v m_q grad phi JxW (normalised when asked, normalize<dim>(.., 1e-16),
+ * normal_vector_operator.cpp:158-237) */
+int mf_normal_vector_operator_create(mf_scratch *s, const mf_levelset_projection_params *p,
+                                     mf_operator **out);
+/* CurvatureOperator: create_rhs(dst [n_dofs], src = normal vector [3][n_dofs]) adds
+ * -sum_q v m_q div(n_hat) JxW, n_hat = the nodal normal normalised with the tolerance of
+ * compute_numerical_zero_of_norm of the domain volume (curvature_operator.cpp:185-255) */
+int mf_curvature_operator_create(mf_scratch *s, const mf_levelset_projection_params *p,
+                                 mf_operator **out);
+/* the level set the narrow band reads (device pointer [n_dofs], kept by reference; NULL =
+ * none; required when narrow_band is set).  Call again when its content changes. */
+int mf_operator_set_level_set(mf_operator *op, const double *level_set);
+/* entries of the operator's vectors: 3 n_dofs for the normal vector operator, else n_dofs */
+int64_t mf_operator_n_rows(const mf_operator *op);
+/* the curvature operator's nodal normalisation tolerance (UtilityFunctions::
+ * compute_numerical_zero_of_norm, utility_functions.hpp:133-143) */
+double mf_operator_normal_tolerance(const mf_operator *op);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Solvers.  Replaces LinearSolver::solve (include/meltpooldg/linear_algebra/                   */
 /* linear_solver.hpp:22-134) with JacobiPreconditioner (preconditioner_jacobi.hpp:29-89):       */
 /* ReductionControl(max_it, abs_tol, rel_tol), success when |r| <= max(abs_tol, rel_tol |r_0|). */

@@ -89,6 +89,13 @@ class SolveReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LevelSetProjectionParams(C.Structure):
+    """mf_levelset_projection_params: NormalVectorData / CurvatureData (filter_parameter,
+    narrow_band.enable, narrow_band.level_set_threshold, compute_normalized_vector)."""
+    _fields_ = [("filter_parameter", C.c_double), ("narrow_band", C.c_int),
+                ("narrow_band_threshold", C.c_double), ("compute_normalized_vector", C.c_int)]
+
+
 class HeatBoundary(C.Structure):
     """mf_heat_boundary: Robin / Neumann sides of the heat operator, side s = 2*direction +
     (0 lower, 1 upper) (colourised hyper_rectangle boundary ids)."""
@@ -156,6 +163,15 @@ def lib():
         L.mf_operator_reset_dof_index.argtypes = [vp, C.c_int]
         L.mf_operator_set_fields.argtypes = [vp, vp, vp, vp]
         L.mf_operator_set_boundary.argtypes = [vp, C.POINTER(HeatBoundary)]
+        L.mf_normal_vector_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
+                                                       C.POINTER(vp)]
+        L.mf_curvature_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
+                                                   C.POINTER(vp)]
+        L.mf_operator_set_level_set.argtypes = [vp, vp]
+        L.mf_operator_n_rows.argtypes = [vp]
+        L.mf_operator_n_rows.restype = i64
+        L.mf_operator_normal_tolerance.argtypes = [vp]
+        L.mf_operator_normal_tolerance.restype = C.c_double
         L.mf_operator_vmult.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_f32.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_add.argtypes = [vp, vp, vp]
@@ -492,6 +508,51 @@ class AdvectionDiffusionOperator(_OperatorBase):
             self.reset_time_increment(dt)
 
 
+class _LevelSetProjection(_OperatorBase):
+    """Tangent sum_q m_q (u v + eta grad u . grad v) JxW, eta = filter_parameter *
+    (cell diameter / sqrt(3))^2, m_q the narrow-band indicator |phi_q| < threshold."""
+
+    _create = None
+
+    def __init__(self, scratch, filter_parameter=2.0, narrow_band=False,
+                 narrow_band_threshold=0.9999999, compute_normalized_vector=False,
+                 level_set=None):
+        p = LevelSetProjectionParams(float(filter_parameter), int(bool(narrow_band)),
+                                     float(narrow_band_threshold),
+                                     int(bool(compute_normalized_vector)))
+        h = C.c_void_p()
+        _check(getattr(lib(), self._create)(scratch._h, C.byref(p), C.byref(h)))
+        super().__init__(scratch, h)
+        self._level_set = None
+        if level_set is not None:
+            self.set_level_set(level_set)
+
+    def set_level_set(self, level_set):
+        """The level set of the narrow band (kept by reference; call again after it changes)."""
+        self._level_set = level_set
+        _check(lib().mf_operator_set_level_set(self._h, _ptr(level_set)))
+
+    @property
+    def n_rows(self):
+        return int(lib().mf_operator_n_rows(self._h))
+
+
+class NormalVectorOperator(_LevelSetProjection):
+    """NormalVectorOperator (source/level_set/normal_vector_operator.cpp): a block operator on
+    [3][n_dofs] vectors; create_rhs(dst [3][n_dofs], level_set) adds the projected gradient."""
+    _create = "mf_normal_vector_operator_create"
+
+
+class CurvatureOperator(_LevelSetProjection):
+    """CurvatureOperator (source/level_set/curvature_operator.cpp); create_rhs(dst, normal
+    [3][n_dofs]) adds -div of the nodally normalised normal vector."""
+    _create = "mf_curvature_operator_create"
+
+    @property
+    def normal_tolerance(self):
+        return float(lib().mf_operator_normal_tolerance(self._h))
+
+
 def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
                  rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30, precision="fp64",
                  inner_rel_tolerance=0.0, max_outer=50):
@@ -665,3 +726,100 @@ class LinearSolver:
         if code != MF_OK and (raise_on_failure or code not in (MF_NO_CONVERGENCE, MF_BREAKDOWN)):
             _check(code)
         return rep.as_dict()
+
+
+class NormalVectorOperation:
+    """NormalVectorOperation (source/level_set/normal_vector_operation.cpp:36-150) on the device:
+    reinit() updates the Jacobi preconditioner (as the reference does at reinit), solve() builds
+    the right-hand side from the level set and solves the block system with CG from the current
+    solution (PredictorType::none).  Linear solver defaults of NormalVectorData (CG, Diagonal,
+    rel 1e-12, abs 1e-20)."""
+
+    def __init__(self, scratch, level_set, filter_parameter=2.0, narrow_band=False,
+                 narrow_band_threshold=0.9999999, compute_normalized_vector=False,
+                 solver="CG", rel_tolerance=1e-12, abs_tolerance=1e-20, max_iterations=10000,
+                 precision="fp64"):
+        import torch
+        self.scratch = scratch
+        self.level_set = level_set
+        self.operator = NormalVectorOperator(scratch, filter_parameter, narrow_band,
+                                             narrow_band_threshold, compute_normalized_vector,
+                                             level_set=level_set)
+        n = self.operator.n_rows
+        self.solution = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.rhs = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.preconditioner = JacobiPreconditioner(self.operator)
+        self.solver = dict(solver=solver, rel_tolerance=rel_tolerance,
+                           abs_tolerance=abs_tolerance, max_iterations=max_iterations,
+                           precision=precision)
+        self.last_report = None
+        self.reinit()
+
+    def reinit(self):
+        self.operator.set_level_set(self.level_set)
+        self.preconditioner.update()
+
+    def solve(self):
+        # the level set's content may have changed: refresh the narrow band's FP32 copy
+        self.operator.set_level_set(self.level_set)
+        self.rhs.zero_()
+        self.operator.create_rhs(self.rhs, self.level_set)
+        self.last_report = LinearSolver.solve(self.operator, self.solution, self.rhs,
+                                              preconditioner=self.preconditioner, **self.solver)
+        return self.last_report["iterations"]
+
+    def get_solution_normal_vector(self):
+        return self.solution
+
+
+class CurvatureOperation:
+    """CurvatureOperation (source/level_set/curvature_operation.cpp:13-115, 198-224) on the
+    device: solve() computes the normal vector, then the curvature
+    kappa = -div(n / |n|) projected with the filter."""
+
+    def __init__(self, scratch, level_set, filter_parameter=2.0, narrow_band=False,
+                 narrow_band_threshold=0.9999999, normal_filter_parameter=2.0,
+                 normal_narrow_band=False, normal_narrow_band_threshold=0.9999999,
+                 compute_normalized_vector=False, solver="CG", rel_tolerance=1e-12,
+                 abs_tolerance=1e-20, max_iterations=10000, precision="fp64"):
+        import torch
+        self.scratch = scratch
+        self.level_set = level_set
+        self.normal_vector_operation = NormalVectorOperation(
+            scratch, level_set, normal_filter_parameter, normal_narrow_band,
+            normal_narrow_band_threshold, compute_normalized_vector, solver, rel_tolerance,
+            abs_tolerance, max_iterations, precision)
+        self.operator = CurvatureOperator(scratch, filter_parameter, narrow_band,
+                                          narrow_band_threshold, level_set=level_set)
+        n = self.operator.n_rows
+        self.solution = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.rhs = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.preconditioner = JacobiPreconditioner(self.operator)
+        self.solver = dict(solver=solver, rel_tolerance=rel_tolerance,
+                           abs_tolerance=abs_tolerance, max_iterations=max_iterations,
+                           precision=precision)
+        self.last_report = None
+        self.reinit()
+
+    def reinit(self):
+        self.operator.set_level_set(self.level_set)
+        self.preconditioner.update()
+        self.normal_vector_operation.reinit()
+
+    def update_normal_vector(self):
+        return self.normal_vector_operation.solve()
+
+    def solve(self):
+        self.normal_vector_operation.solve()
+        self.operator.set_level_set(self.level_set)
+        self.rhs.zero_()
+        self.operator.create_rhs(self.rhs, self.normal_vector_operation.get_solution_normal_vector())
+        self.last_report = LinearSolver.solve(self.operator, self.solution, self.rhs,
+                                              preconditioner=self.preconditioner, **self.solver)
+        return self.last_report["iterations"]
+
+    def get_curvature(self):
+        return self.solution
+
+    def get_normal_vector(self):
+        return self.normal_vector_operation.get_solution_normal_vector()

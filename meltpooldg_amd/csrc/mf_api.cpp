@@ -145,7 +145,8 @@ static CellArgs<Number> make_args(const mf_operator *op)
   a.n_dofs = uint32_t(m.n_dofs);
   a.n_cells = int(m.n_cells);
   a.mask = s->d_mask;
-  a.use_mask = op->dof_index == 0 ? 1 : 0;
+  // the level-set projections have no Dirichlet rows
+  a.use_mask = (op->kind < 2 && op->dof_index == 0) ? 1 : 0;
   a.transpose = op->transpose;
   a.gather_plain = op->gather_plain;
   a.dt_inv = Number(op->dt > 0 ? 1.0 / op->dt : 0.0);
@@ -156,10 +157,15 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.tanh_Tc = Number(op->heat.tanh_Tc);
     a.inv_tanh_w = Number(op->heat.tanh_width != 0 ? 1.0 / op->heat.tanh_width : 0.0);
     a.t_dependent = op->heat.t_dependent;
-  } else {
+  } else if (op->kind == 1) {
     a.theta = Number(op->advec.theta);
     a.diffusivity = Number(op->advec.diffusivity);
     a.mass_factor = Number(op->advec.mass_factor);
+  } else {
+    // eta = filter_parameter (diameter / sqrt(3))^2 of a level-0 cell
+    const double d2 = m.h[0] * m.h[0] + m.h[1] * m.h[1] + m.h[2] * m.h[2];
+    a.ls_eta = Number(op->ls.filter_parameter * d2 / 3.0);
+    a.ls_threshold = Number(op->ls.narrow_band_threshold);
   }
   if (const AdaptiveData *ad = s->ad) {
     a.cell_nodes = ad->d.cell_nodes;
@@ -196,11 +202,27 @@ static CellArgs<Number> make_args(const mf_operator *op)
   return a;
 }
 
-static bool needs_T(const mf_operator *op) { return op->kind == 0 && op->heat.law == MF_LAW_TANH; }
+// T: the temperature of the tanh law, or the level set of a narrow band
+static bool needs_T(const mf_operator *op)
+{
+  if (op->kind >= 2) return op->ls.narrow_band != 0;
+  return op->kind == 0 && op->heat.law == MF_LAW_TANH;
+}
+
+// vector blocks of the operator (BlockVectorType of the normal vector operator)
+static int n_blocks(const mf_operator *op) { return op->kind == 2 ? 3 : 1; }
+static int64_t n_rows(const mf_operator *op) { return n_blocks(op) * op->s->mesh.n_dofs; }
 
 static int check_ready(const mf_operator *op)
 {
   MF_REQUIRE(op && op->s, MF_ERR_ARG, "null operator");
+  if (op->kind >= 2) {
+    if (op->ls.narrow_band)
+      MF_REQUIRE(op->T, MF_ERR_STATE,
+                 "Level set solution vector must not be nullptr for a narrow band computation "
+                 "(mf_operator_set_level_set)");
+    return MF_OK;
+  }
   MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
              "time increment not set (reference AssertThrowZeroTimeIncrement)");
   if (needs_T(op)) MF_REQUIRE(op->T, MF_ERR_STATE, "temperature-dependent law needs T: call mf_operator_set_fields");
@@ -242,6 +264,16 @@ struct Carve {
   }
 };
 
+// blocks that write fused-dot partials in one operator application: the cell kernel (per
+// vector block) and the boundary faces
+static int op_tangent_blocks(const mf_operator *op, bool fp32)
+{
+  const MeshInfo &m = op->s->mesh;
+  // the level-set projections use the pencil kernel like the transposed operator
+  const int cells = tangent_blocks(m, op->transpose != 0 || op->kind >= 2, fp32);
+  return n_blocks(op) * cells + heat_face_blocks(op);
+}
+
 // dst = A src (overwrite = true, the reference's vmult with zeroed dst) or dst += A src.
 // The structured-mesh kernel overwrites in place (the first colour that writes a node stores
 // it); the other kernels accumulate into a zeroed dst.
@@ -251,7 +283,20 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
 {
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
-  const bool slab = g_tangent_variant == 1 && !op->transpose && !m.ad;
+  const bool slab = g_tangent_variant == 1 && !op->transpose && !m.ad && op->kind < 2;
+  if (n_blocks(op) > 1) {
+    // the same scalar operator on every block; the fused dot's partials follow block by block
+    const int per = op_tangent_blocks(op, sizeof(Number) == 4) / n_blocks(op);
+    const int64_t nd = m.n_dofs;
+    mf_operator scalar_view = *op;  // shallow: reads the same fields, owns nothing
+    scalar_view.kind = 3;
+    for (int b = 0; b < n_blocks(op); ++b) {
+      int rc = apply_tangent<Number>(&scalar_view, dst + b * nd, src + b * nd,
+                                     dot_partial ? dot_partial + b * per : nullptr, stop, overwrite);
+      if (rc) return rc;
+    }
+    return MF_OK;
+  }
   if (overwrite && (m.ad || slab)) {
     int rc = vec_fill<Number>(dst, Number(0), m.n_dofs, st, stop);
     if (rc) return rc;
@@ -260,7 +305,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   a.overwrite = (overwrite && !m.ad && !slab) ? 1 : 0;
-  const int code = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+  const int code = op->kind == 0 ? HEAT_TANGENT : (op->kind == 1 ? ADVEC_TANGENT : LS_TANGENT);
   if (m.ad) {
     if (!op->gather_plain && m.ad->n_hanging > 0) {
       Number *hv;
@@ -275,9 +320,9 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     int rc = launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
     if (rc) return rc;
     // hanging rows hold the unconstrained cell sums: move them to their masters
-    return launch_condense<Number>(op->s, dst, stop);
+    return launch_condense<Number>(op->s, dst, stop, a.use_mask != 0);
   }
-  // the transposed operator (adjoint) runs on the pencil kernel
+  // the transposed operator (adjoint) and the level-set projections run on the pencil kernel
   int rc = slab ? launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st)
                 : launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   if (rc) return rc;
@@ -285,7 +330,8 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   // fused dot follows the cell kernel's partials
   return launch_heat_faces<Number>(
       op, src, dst, 0, stop,
-      dot_partial ? dot_partial + tangent_blocks(m, op->transpose != 0, sizeof(Number) == 4)
+      dot_partial ? dot_partial + tangent_blocks(m, op->transpose != 0 || op->kind >= 2,
+                                                 sizeof(Number) == 4)
                   : nullptr);
 }
 
@@ -422,6 +468,58 @@ int mf_advection_operator_create(mf_scratch *s, const mf_advec_params *p, mf_ope
   return MF_OK;
 }
 
+static int levelset_projection_create(mf_scratch *s, const mf_levelset_projection_params *p,
+                                      int kind, mf_operator **out)
+{
+  MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(p->filter_parameter >= 0.0, MF_ERR_ARG, "filter_parameter must be >= 0");
+  // NormalVectorData::check_input_parameters / CurvatureData (normal_vector_data.cpp:83)
+  MF_REQUIRE(!p->narrow_band || p->narrow_band_threshold > 0.0, MF_ERR_ARG,
+             "The narrow band threshold must be larger than zero.");
+  if (int rc = ensure_adaptive_on_device(s)) return rc;
+  auto *op = new mf_operator();
+  op->kind = kind;
+  op->s = s;
+  op->ls = *p;
+  // UtilityFunctions::compute_numerical_zero_of_norm of the domain volume
+  // (utility_functions.hpp:104-143): 10^floor(log10(V^(1/3))) * 1e-3, clamped to [1e-12, 1e-2]
+  const MeshInfo &m = s->mesh;
+  const double vol = (m.hi[0] - m.lo[0]) * (m.hi[1] - m.lo[1]) * (m.hi[2] - m.lo[2]);
+  const double e = std::floor(std::log10(std::fabs(std::pow(vol, 1.0 / 3.0))));
+  op->ls_tol = std::min(1e-2, std::max(std::pow(10.0, e) * 1e-3, 1e-12));
+  *out = op;
+  return MF_OK;
+}
+
+int mf_normal_vector_operator_create(mf_scratch *s, const mf_levelset_projection_params *p,
+                                     mf_operator **out)
+{
+  return levelset_projection_create(s, p, 2, out);
+}
+
+int mf_curvature_operator_create(mf_scratch *s, const mf_levelset_projection_params *p,
+                                 mf_operator **out)
+{
+  return levelset_projection_create(s, p, 3, out);
+}
+
+int mf_operator_set_level_set(mf_operator *op, const double *level_set)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->kind >= 2, MF_ERR_ARG, "only the level-set projections take a level set");
+  const MeshInfo &m = op->s->mesh;
+  op->T = level_set;
+  if (level_set) {
+    if (!op->T32) MF_HIP_TRY(hipMalloc(&op->T32, sizeof(float) * m.n_dofs));
+    return vec_cast<double, float>(level_set, op->T32, 1.0, m.n_dofs, op->s->stream);
+  }
+  return MF_OK;
+}
+
+int64_t mf_operator_n_rows(const mf_operator *op) { return op && op->s ? n_rows(op) : -1; }
+
+double mf_operator_normal_tolerance(const mf_operator *op) { return op ? op->ls_tol : 0.0; }
+
 int mf_operator_destroy(mf_operator *op)
 {
   if (!op) return MF_OK;
@@ -454,6 +552,8 @@ int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
                            const double *k_cell)
 {
   MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->kind < 2, MF_ERR_ARG,
+             "the level-set projections take their level set from mf_operator_set_level_set");
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
   op->T = T;
@@ -509,7 +609,7 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
   CellArgs<double> a = make_args<double>(op);
-  const int code = op->kind == 0 ? HEAT_DIAG : ADVEC_DIAG;
+  const int code = op->kind == 0 ? HEAT_DIAG : (op->kind == 1 ? ADVEC_DIAG : LS_DIAG);
   if (m.ad) {
     rc = vec_fill<double>(inv_diag, 0.0, m.n_dofs, st);
     if (rc) return rc;
@@ -517,7 +617,7 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
     // diagonal c_D . (A_cell c_D) per target node D
     rc = launch_cell_list<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
     if (rc) return rc;
-    const int tcode = op->kind == 0 ? HEAT_TANGENT : ADVEC_TANGENT;
+    const int tcode = op->kind == 0 ? HEAT_TANGENT : (op->kind == 1 ? ADVEC_TANGENT : LS_TANGENT);
     rc = launch_cell_list<double>(tcode, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
   } else {
     a.overwrite = 1;
@@ -526,12 +626,18 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
     rc = launch_heat_faces<double>(op, nullptr, inv_diag, 1, nullptr);
   }
   if (rc) return rc;
+  // every block of the normal vector operator has the same diagonal
+  for (int b = 1; b < n_blocks(op); ++b)
+    MF_HIP_TRY(hipMemcpyAsync(inv_diag + b * m.n_dofs, inv_diag, sizeof(double) * m.n_dofs,
+                              hipMemcpyDeviceToDevice, st));
+  // heat: |d| > 1e-10; advection and the level-set projections: |d| > 1e-14 max(1, |d|_inf)
+  const bool relative = op->kind != 0;
   double *work = nullptr;
-  if (op->kind == 1) {
+  if (relative) {
     if ((rc = arena_reserve(op->arena[3], sizeof(double) * RED_BLOCKS))) return rc;
     work = reinterpret_cast<double *>(op->arena[3].p);
   }
-  return vec_invert_diagonal(inv_diag, m.n_dofs, op->kind == 1, work, st);
+  return vec_invert_diagonal(inv_diag, n_rows(op), relative, work, st);
 }
 
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)
@@ -540,7 +646,12 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
   if (rc) return rc;
   MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
   if (op->kind == 0) MF_REQUIRE(op->T, MF_ERR_STATE, "heat create_rhs needs T (mf_operator_set_fields)");
-  MF_REQUIRE(op->kind == 0 || !q_s, MF_ERR_ARG, "advection create_rhs takes no source term");
+  MF_REQUIRE(op->kind == 0 || !q_s, MF_ERR_ARG, "only the heat create_rhs takes a source term");
+  if (op->kind >= 2) {
+    // normal vector: src = level set, dst [3][n_dofs]; curvature: src = normal vector
+    // [3][n_dofs], dst [n_dofs]
+    return launch_levelset_rhs(op, op->kind == 2 ? 0 : 1, src, dst);
+  }
   CellArgs<double> a = make_args<double>(op);
   a.aux1 = q_s;
   const int code = op->kind == 0 ? HEAT_RHS : ADVEC_RHS;
@@ -591,11 +702,10 @@ template <typename T>
 static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it, double abs_tol,
                     double rel_tol, SolveOut &out, bool x_zero)
 {
-  const MeshInfo &m = op->s->mesh;
-  const int64_t n = m.n_dofs;
+  const int64_t n = n_rows(op);
   hipStream_t st = op->s->stream;
   // per-block partials of the fused p.(Ap): cell kernel blocks, then boundary-face blocks
-  const int ncb = tangent_blocks(m, op->transpose != 0, sizeof(T) == 4) + heat_face_blocks(op);
+  const int ncb = op_tangent_blocks(op, sizeof(T) == 4);
   Carve cv;
   cv.plan<T>(n);
   cv.plan<T>(n);
@@ -655,8 +765,7 @@ template <typename T>
 static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int restart, int max_it,
                        double abs_tol, double rel_tol, SolveOut &out, bool x_zero)
 {
-  const MeshInfo &m = op->s->mesh;
-  const int64_t n = m.n_dofs;
+  const int64_t n = n_rows(op);
   hipStream_t st = op->s->stream;
   const int mm = std::min(std::max(restart, 1), 30);
   Carve cv;
@@ -789,8 +898,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
                       const float *inv32_in, double *x, const double *b, mf_solve_report *report)
 {
   int rc = MF_OK;
-  const MeshInfo &m = op->s->mesh;
-  const int64_t n = m.n_dofs;
+  const int64_t n = n_rows(op);
   hipStream_t st = op->s->stream;
   const int max_it = desc->max_iterations > 0 ? desc->max_iterations : 10000;
   struct {
@@ -912,7 +1020,7 @@ extern "C" int mf_solve(mf_operator *op, const mf_solver_desc *desc, double *x, 
 {
   int rc = check_solve_args(op, desc, x, b);
   if (rc) return rc;
-  const int64_t n = op->s->mesh.n_dofs;
+  const int64_t n = n_rows(op);
   double *inv = nullptr;
   if (desc->preconditioner == MF_PRECOND_JACOBI) {
     if ((rc = arena_reserve(op->arena[2], sizeof(double) * std::max<int64_t>(n, RED_BLOCKS))))
@@ -928,7 +1036,7 @@ extern "C" int mf_jacobi_create(mf_operator *op, mf_jacobi **out)
   MF_REQUIRE(op && out, MF_ERR_ARG, "null argument");
   auto *pc = new mf_jacobi;
   pc->op = op;
-  pc->n = op->s->mesh.n_dofs;
+  pc->n = n_rows(op);
   // the inversion of the advection diagonal reduces over a few blocks at the end of the array
   const size_t nd = size_t(std::max<int64_t>(pc->n, RED_BLOCKS));
   if (hipMalloc(&pc->inv, sizeof(double) * nd) != hipSuccess ||
@@ -985,7 +1093,7 @@ extern "C" int mf_solve_preconditioned(mf_operator *op, const mf_solver_desc *de
   if (rc) return rc;
   if (pc) {
     MF_REQUIRE(pc->valid, MF_ERR_STATE, "Jacobi preconditioner not updated (mf_jacobi_update)");
-    MF_REQUIRE(pc->n == op->s->mesh.n_dofs, MF_ERR_ARG,
+    MF_REQUIRE(pc->n == n_rows(op), MF_ERR_ARG,
                "preconditioner and operator have different DoF counts");
   }
   return solve_with(op, desc, pc ? pc->inv : nullptr, pc ? pc->inv32 : nullptr, x, b, report);

@@ -681,12 +681,16 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr int CPB = cells_per_block<Number, K>();
   constexpr int NL = K * K * K;
   constexpr int BUF = NL * CPB;
-  constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG);
+  constexpr bool LS = (OP == LS_TANGENT || OP == LS_DIAG);
+  constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG || OP == LS_DIAG);
   constexpr bool HAS_U = !IS_DIAG;
   constexpr bool PGRAD = (OP == HEAT_PGRAD);
-  constexpr bool USE_T = (NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG)) || PGRAD;
+  // T: the linearisation temperature, or for the level-set projections the level set whose
+  // values give the narrow band (NEEDT = band enabled); only the heat operator needs grad T
+  constexpr bool USE_T = (NEEDT && (OP == HEAT_TANGENT || OP == HEAT_DIAG || LS)) || PGRAD;
+  constexpr bool TGRAD = USE_T && !LS;
   constexpr bool RHS_HEAT = (OP == HEAT_RHS);
-  constexpr bool TANGENT = (OP == HEAT_TANGENT || OP == ADVEC_TANGENT);
+  constexpr bool TANGENT = is_tangent_op(OP);
 
   // LDS cell tensors [K^3][CPB].  While values are interpolated: u in L0, T in L1, the velocity
   // components and the rhs fields after it.  Then the T gradients use L2, L3 and are reduced to
@@ -696,7 +700,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr int I_W = 1 + (USE_T ? 1 : 0);
   constexpr int I_A = I_W + (VEL ? 3 : 0);
   constexpr int N_EVAL = I_A + (RHS_HEAT ? 2 : 0);
-  constexpr int N_GRAD = USE_T ? 4 : 3;
+  constexpr int N_GRAD = TGRAD ? 4 : 3;
   // with the fused dot, the gathered source is kept: in registers (FP32 x pencils), or in one
   // more tensor when it was gathered cooperatively or when K more registers would cost the FP64
   // kernel a wave per SIMD (it sits just under 128)
@@ -893,7 +897,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   // ---------------- T gradients -> per-quadrature-point coefficients ----------------
   // heat tangent / diagonal: kq = k(T) kc and dkg = dk/dT kc grad T;  PGRAD: dkg = grad T
   Number kq[K], dkg[3][K];
-  if constexpr (USE_T) {
+  if constexpr (TGRAD) {
     sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, lds + I_T * BUF, L2, Dq, lds + I_T * BUF,
                                                   lds + 3 * BUF, p, c);
     __syncthreads();
@@ -1019,6 +1023,23 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       val = A.dt_inv * uq[z] - (Number(1) - A.theta) * vg;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] = -(Number(1) - A.theta) * A.diffusivity * g[d];
+    } else if constexpr (LS) {
+      // m (u v + eta grad u . grad v); m = 0 where |level set| >= threshold (a NaN keeps 1,
+      // as the reference's test does)
+      Number msk = 1;
+      if constexpr (USE_T) {
+        const Number at = Tq[z] < Number(0) ? -Tq[z] : Tq[z];
+        msk = (at >= A.ls_threshold) ? Number(0) : Number(1);
+      }
+      const Number eta = A.ls_eta * hs * hs;
+      if constexpr (OP == LS_TANGENT) {
+        val = msk * uq[z];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] = msk * eta * g[d];
+      } else {
+        cc = msk;
+        kk = msk * eta;
+      }
     } else {  // ADVEC_DIAG
       cc = A.mass_factor * A.dt_inv;
       kk = A.theta * A.diffusivity;
@@ -1195,7 +1216,7 @@ template <typename Number, int K, int OP, bool NEEDT, bool VEL>
 static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src, Number *dst,
                     hipStream_t st)
 {
-  if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT) {
+  if constexpr (is_tangent_op(OP)) {
     if (a.dot_partial) return launch_colours<Number, K, OP, NEEDT, VEL, true>(m, a, src, dst, st);
   }
   return launch_colours<Number, K, OP, NEEDT, VEL, false>(m, a, src, dst, st);
@@ -1248,12 +1269,12 @@ static int launch_list_k(const MeshInfo &m, const CellArgs<Number> &a, const Num
   std::vector<int64_t> start, end;
   list_ranges(*m.ad, mode, start, end);
   if (mode == 2) {
-    if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT)
+    if constexpr (is_tangent_op(OP))
       return launch_parts<Number, K, OP, NEEDT, VEL, false, SlotAccess<Number, K, CPB>>(
           start, end, a, src, dst, st);
     return MF_ERR_ARG;
   }
-  if constexpr (OP == HEAT_TANGENT || OP == ADVEC_TANGENT) {
+  if constexpr (is_tangent_op(OP)) {
     if (a.dot_partial)
       return launch_parts<Number, K, OP, NEEDT, VEL, true, ListAccess<Number, K, CPB>>(
           start, end, a, src, dst, st);
@@ -1268,6 +1289,9 @@ static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArg
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_DIAG) {
     return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
+  } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
+    if (needT) return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
+    return launch_list_k<Number, K, OP, false, false>(m, a, src, dst, st, mode);
   } else {
     if (needT) {
       if (vel) return launch_list_k<Number, K, OP, true, true>(m, a, src, dst, st, mode);
@@ -1295,6 +1319,12 @@ static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
   case ADVEC_DIAG:
     if constexpr (sizeof(Number) == 8)
       return launch_list_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st, mode);
+    break;
+  case LS_TANGENT:
+    return launch_list_op<Number, K, LS_TANGENT>(needT, vel, m, a, src, dst, st, mode);
+  case LS_DIAG:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, LS_DIAG>(needT, vel, m, a, src, dst, st, mode);
     break;
   default:
     break;
@@ -1330,6 +1360,9 @@ static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Num
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
     return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
+  } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
+    if (needT) return launch_k<Number, K, OP, true, false>(m, a, src, dst, st);
+    return launch_k<Number, K, OP, false, false>(m, a, src, dst, st);
   } else {
     if (needT) {
       if (vel) return launch_k<Number, K, OP, true, true>(m, a, src, dst, st);
@@ -1352,8 +1385,14 @@ static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const Cel
   case ADVEC_RHS: return launch_op<Number, K, ADVEC_RHS>(needT, vel, m, a, src, dst, st);
   case ADVEC_DIAG: return launch_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st);
   case HEAT_PGRAD: return launch_k<Number, K, HEAT_PGRAD, false, false>(m, a, src, dst, st);
-  default: set_error("unknown cell operation"); return MF_ERR_ARG;
+  case LS_TANGENT: return launch_op<Number, K, LS_TANGENT>(needT, vel, m, a, src, dst, st);
+  case LS_DIAG:
+    if constexpr (sizeof(Number) == 8) return launch_op<Number, K, LS_DIAG>(needT, vel, m, a, src, dst, st);
+    break;
+  default: break;
   }
+  set_error("unknown cell operation");
+  return MF_ERR_ARG;
 }
 
 template <typename Number>

@@ -44,8 +44,12 @@ enum CellOp : int {
   ADVEC_TANGENT = 3, // advection_diffusion_operator.cpp:448-495
   ADVEC_RHS = 4,     // advection_diffusion_operator.cpp:274-358
   ADVEC_DIAG = 5,    // advection_diffusion_operator.cpp:399-446
-  HEAT_PGRAD = 6     // per-cell sum_q JxW grad(src) . grad(T): conductivity gradient (adjoint)
+  HEAT_PGRAD = 6,    // per-cell sum_q JxW grad(src) . grad(T): conductivity gradient (adjoint)
+  LS_TANGENT = 7,    // normal_vector_operator.cpp:316-347, curvature_operator.cpp:339-371
+  LS_DIAG = 8        // their diagonal (normal_vector_operator.cpp:256-311)
 };
+
+constexpr bool is_tangent_op(int op) { return op == HEAT_TANGENT || op == ADVEC_TANGENT || op == LS_TANGENT; }
 
 // Everything a cell kernel needs besides src/dst; passed by value (kernel argument segment).
 template <typename Number>
@@ -67,6 +71,9 @@ struct CellArgs {
   // advection
   Number theta, diffusivity, mass_factor;
   Number dt_inv;
+  // level-set projections: eta of a level-0 cell (scaled by h^2 on refined cells) and the
+  // narrow-band threshold on |T| (T holds the level set)
+  Number ls_eta, ls_threshold;
   // fields
   const Number *T;         // linearisation temperature (heat tangent / diagonal)
   const Number *vel;       // nodal velocity [3][n_dofs]
@@ -168,9 +175,10 @@ int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
 int upload_adaptive(::mf_scratch *s);
 int ensure_adaptive_on_device(::mf_scratch *s);
 // y[m] += sum_h w_hm y[h] over the hanging nodes h with master m, then y[h] = 0 (the transpose
-// of the constraints applied to the unconstrained cell sums), mf_ops.hip
+// of the constraints applied to the unconstrained cell sums); with use_mask, Dirichlet masters
+// keep their (zero) rows.  mf_ops.hip
 template <typename Number>
-int launch_condense(const ::mf_scratch *s, Number *y, const int *stop);
+int launch_condense(const ::mf_scratch *s, Number *y, const int *stop, bool use_mask);
 // hv[l] = sum_t w_t src[m_t] over the line of every hanging node (constrained masters read as 0
 // when use_mask), the source values hanging nodes take in read_dof_values, mf_ops.hip
 template <typename Number>
@@ -204,6 +212,10 @@ int heat_face_blocks(const mf_operator *op);
 
 // number of blocks the cell kernel uses (size of dot_partial) in FP64 or FP32
 int cell_blocks(const MeshInfo &m, bool fp32);
+
+// level-set right-hand sides (mf_levelset.hip): mode 0 normal vector (src = level set, dst
+// [3][n_dofs]), 1 curvature (src = normal vector [3][n_dofs], dst [n_dofs]); dst += ...
+int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, double *dst);
 
 // slice-resident tangent kernel (mf_slab.hip)
 template <typename Number>
@@ -239,7 +251,9 @@ struct Arena {
 
 struct mf_operator {
   mf::Arena arena[4];  // 0: refinement vectors, 1: Krylov vectors, 2: inverse diagonal, 3: small work
-  int kind = 0;  // 0 heat, 1 advection
+  int kind = 0;  // 0 heat, 1 advection, 2 normal vector (3 blocks), 3 curvature
+  mf_levelset_projection_params ls{};  // kinds 2, 3
+  double ls_tol = 0.0;                 // curvature: nodal normalisation tolerance
   mf_scratch *s = nullptr;
   mf_heat_params heat{};
   mf_advec_params advec{};

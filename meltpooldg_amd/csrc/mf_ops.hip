@@ -284,7 +284,7 @@ __global__ void zero_hanging_kernel(const uint32_t *__restrict__ hang, int64_t n
 }
 
 template <typename Number>
-int launch_condense(const mf_scratch *s, Number *y, const int *stop)
+int launch_condense(const mf_scratch *s, Number *y, const int *stop, bool use_mask)
 {
   const AdaptiveData *A = s->ad;
   if (!A || A->hang_nodes.empty()) return MF_OK;
@@ -296,7 +296,8 @@ int launch_condense(const mf_scratch *s, Number *y, const int *stop)
     w = reinterpret_cast<const Number *>(A->d.cm_w32);
   const int bm = int(std::min<int64_t>((nm + 15) / 16, 16384));
   hipLaunchKernelGGL(condense_masters_kernel<Number>, dim3(bm), dim3(256), 0, s->stream,
-                     A->d.cm_node, A->d.cm_ptr, A->d.cm_h, w, s->d_mask, nm, y, stop);
+                     A->d.cm_node, A->d.cm_ptr, A->d.cm_h, w, use_mask ? s->d_mask : nullptr,
+                     nm, y, stop);
   MF_HIP_TRY(hipGetLastError());
   const int bh = int(std::min<int64_t>((nh + 255) / 256, 4096));
   hipLaunchKernelGGL(zero_hanging_kernel<Number>, dim3(bh), dim3(256), 0, s->stream,
@@ -305,8 +306,8 @@ int launch_condense(const mf_scratch *s, Number *y, const int *stop)
   return MF_OK;
 }
 
-template int launch_condense<double>(const mf_scratch *, double *, const int *);
-template int launch_condense<float>(const mf_scratch *, float *, const int *);
+template int launch_condense<double>(const mf_scratch *, double *, const int *, bool);
+template int launch_condense<float>(const mf_scratch *, float *, const int *, bool);
 
 int launch_distribute(const int32_t *node_line, const uint32_t *line_ptr, const uint32_t *line_col,
                       const double *line_val, double *x, int64_t n, hipStream_t st)

@@ -412,6 +412,30 @@ static long op_ndofs(const orc_operator *op)
   return op->amesh ? op->adata->n_nodes : orc_n_dofs(&op->mesh);
 }
 
+static int op_dim(const orc_operator *op) { return op->amesh ? 3 : op->mesh.dim; }
+
+long orc_n_rows(const orc_operator *op)
+{
+  return (op->kind == ORC_OP_NORMAL ? op_dim(op) : 1) * op_ndofs(op);
+}
+
+/* filter parameter eta = scale * (cell diameter / sqrt(dim))^2 of the cell with geometry g
+ * (compute_cell_size_dependent_filter_parameter_mf, normal_vector_operator.hpp:95-112: the
+ * reference takes max(min cell size, cell size), which is the cell size of every cell) */
+static double ls_damping(const orc_operator *op, const geo_t *g)
+{
+  double d2 = 0.0;
+  for (int d = 0; d < g->dim; ++d) d2 += g->h[d] * g->h[d];
+  return op->ls.filter_parameter * d2 / g->dim;
+}
+
+/* narrow-band indicator (Tools::compute_mask_narrow_band, level_set_tools.hpp:359-368) */
+static double ls_mask(const orc_operator *op, double phi)
+{
+  if (!op->ls.narrow_band) return 1.0;
+  return fabs(phi) >= op->ls.narrow_band_threshold ? 0.0 : 1.0;
+}
+
 /* velocity at the quadrature points of a cell */
 static void eval_velocity(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
 {
@@ -527,12 +551,46 @@ static void advec_tangent_cell(const orc_operator *op, const geo_t *g, long cell
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
 }
 
+/* normal vector / curvature projection tangent: values m u, gradients m eta grad u
+ * (tangent_local_cell_operation, normal_vector_operator.cpp:316-347,
+ *  curvature_operator.cpp:339-371); ws->idx holds the cell's nodes */
+static void ls_tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  (void)cell;
+  const int nq = g->nloc, dim = g->dim;
+  const double eta = ls_damping(op, g);
+  eval_vg(g, ws->u, ws->val, ws->grad, &ws->w);
+  if (op->ls.narrow_band) {
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = op->level_set[ws->idx[l]];
+    eval_v(g, ws->tmp, ws->Tv, &ws->w);
+  }
+  for (int q = 0; q < nq; ++q) {
+    const double m = op->ls.narrow_band ? ls_mask(op, ws->Tv[q]) : 1.0;
+    const double jxw = cell_volume_weight(g, q);
+    ws->fval[q] = jxw * m * ws->val[q];
+    for (int e = 0; e < dim; ++e) ws->flux[e * nq + q] = jxw * m * eta * ws->grad[e * nq + q];
+  }
+  integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+}
+
 static void tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
 {
   if (op->kind == ORC_OP_HEAT)
     heat_tangent_cell(op, g, cell, ws);
-  else
+  else if (op->kind == ORC_OP_ADVEC)
     advec_tangent_cell(op, g, cell, ws);
+  else
+    ls_tangent_cell(op, g, cell, ws);
+}
+
+/* the normal vector operator applies the scalar projection to every block */
+static int for_blocks(const orc_operator *op, orc_operator *scalar, long *n)
+{
+  if (op->kind != ORC_OP_NORMAL) return 0;
+  *scalar = *op;
+  scalar->kind = ORC_OP_CURVATURE;
+  *n = op_ndofs(op);
+  return op_dim(op);
 }
 
 static int nthreads_of(const orc_operator *op)
@@ -701,6 +759,12 @@ static void heat_faces(const orc_operator *op, const geo_t *g, const double *src
 
 void orc_vmult(const orc_operator *op, const double *src, double *dst)
 {
+  orc_operator sc;
+  long nb;
+  if (for_blocks(op, &sc, &nb)) {
+    for (int d = 0; d < op_dim(op); ++d) orc_vmult(&sc, src + d * nb, dst + d * nb);
+    return;
+  }
   if (op->amesh) {
     avmult(op, src, dst);
     return;
@@ -792,11 +856,18 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
         if (op->vel_kind != ORC_VEL_NONE)
           c->c += drc * (Tg[0] * w[0] + Tg[1] * w[1] + Tg[2] * w[2]);
       }
-    } else {
+    } else if (op->kind == ORC_OP_ADVEC) {
       const orc_advec_params *ap = &op->advec;
       c->c = ap->mass_factor / ap->dt;
       c->k = ap->theta * ap->diffusivity;
       for (int e = 0; e < 3; ++e) c->bu[e] = ap->theta * w[e];
+    } else {
+      double ph = 0.0;
+      if (op->ls.narrow_band)
+        for (int l = 0; l < nq; ++l) ph += op->level_set[idx[l]] * phi[l];
+      const double m = ls_mask(op, ph);
+      c->c = m;
+      c->k = m * ls_damping(op, g);
     }
     const double jxw = cell_volume_weight(g, q);
     c->c *= jxw;
@@ -852,6 +923,12 @@ static void dense_cell(const orc_operator *op, const geo_t *g, long cell, cell_w
 
 void orc_vmult_dense(const orc_operator *op, const double *src, double *dst)
 {
+  orc_operator sc;
+  long nb;
+  if (for_blocks(op, &sc, &nb)) {
+    for (int d = 0; d < op_dim(op); ++d) orc_vmult_dense(&sc, src + d * nb, dst + d * nb);
+    return;
+  }
   geo_t g;
   geo_init(&op->mesh, &g);
   const long n = orc_n_dofs(&op->mesh);
@@ -888,6 +965,12 @@ static void diag_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws
 
 void orc_diagonal(const orc_operator *op, double *diag)
 {
+  orc_operator sc;
+  long nb;
+  if (for_blocks(op, &sc, &nb)) {
+    for (int d = 0; d < op_dim(op); ++d) orc_diagonal(&sc, diag + d * nb);
+    return;
+  }
   if (op->amesh) {
     adiagonal(op, diag);
     return;
@@ -902,11 +985,13 @@ void orc_diagonal(const orc_operator *op, double *diag)
 }
 
 /* heat: |d| > 1e-10 ? 1/d : 1  (heat_diffuse_operator.cpp:282-283)
- * advection: |d| > 1e-14 max(1, |d|_inf) ? 1/d : 1  (advection_diffusion_operator.cpp:434-436) */
+ * advection, normal vector, curvature: |d| > 1e-14 max(1, |d|_inf) ? 1/d : 1
+ * (advection_diffusion_operator.cpp:434-436, normal_vector_operator.cpp:306-310,
+ *  curvature_operator.cpp:331-334) */
 void orc_invert_diagonal(int kind, long n, double *diag)
 {
   double thr = 1.0e-10;
-  if (kind == ORC_OP_ADVEC) {
+  if (kind != ORC_OP_HEAT) {
     double mx = 1.0;
     for (long i = 0; i < n; ++i) mx = fmax(mx, fabs(diag[i]));
     thr = 1.0e-14 * mx;
@@ -1148,7 +1233,7 @@ int orc_solve(const orc_operator *op, int solver, int restart, int max_it, doubl
               double rel_tol, const double *inv_diag, const double *b, double *x, int *iterations,
               double *final_residual)
 {
-  const long n = op_ndofs(op);
+  const long n = orc_n_rows(op);
   if (solver == ORC_CG)
     return solve_cg(op, n, max_it, abs_tol, rel_tol, inv_diag, b, x, iterations, final_residual);
   return solve_gmres(op, n, restart > 0 ? restart : 30, max_it, abs_tol, rel_tol, inv_diag, b, x,
@@ -1667,4 +1752,132 @@ static void adiagonal(const orc_operator *op, double *diag)
   free(A);
   free(targets);
   free(cv);
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Level-set normal vector and curvature right-hand sides (textbook quadrature: basis values   */
+/* and gradients as products of 1D Lagrange functions at every quadrature point)               */
+/* ------------------------------------------------------------------------------------------ */
+
+double orc_numerical_zero_of_norm(int dim, double volume)
+{
+  const double x = pow(volume, 1.0 / dim);
+  const double e = floor(log10(fabs(x)));
+  return fmin(1e-2, fmax(pow(10.0, e) * 1e-3, 1e-12));
+}
+
+/* dst += out over the cell's nodes: structured meshes directly, adaptive meshes through the
+ * transposed hanging-node constraints (distribute_local_to_global) */
+static void ls_add(const orc_operator *op, const long *idx, int nloc, const double *out,
+                   double *dst)
+{
+  if (op->amesh) {
+    const orc_adata *ad = op->adata;
+    for (int l = 0; l < nloc; ++l) {
+      const long i = idx[l];
+      if (ad->line_ptr[i + 1] > ad->line_ptr[i]) {
+        for (long t = ad->line_ptr[i]; t < ad->line_ptr[i + 1]; ++t)
+          dst[ad->line_col[t]] += ad->line_val[t] * out[l];
+      } else {
+        dst[i] += out[l];
+      }
+    }
+  } else {
+    for (int l = 0; l < nloc; ++l) dst[idx[l]] += out[l];
+  }
+}
+
+static long ls_cells(const orc_operator *op)
+{
+  return op->amesh ? op->amesh->n_cells : orc_n_cells(&op->mesh);
+}
+
+static void ls_cell_geo(const orc_operator *op, long cell, geo_t *g, long *idx)
+{
+  if (op->amesh) {
+    acell_geo(op->amesh, cell, g);
+    for (int l = 0; l < g->nloc; ++l) idx[l] = op->adata->cell_nodes[cell * g->nloc + l];
+  } else {
+    geo_init(&op->mesh, g);
+    cell_dofs(g, cell, idx);
+  }
+}
+
+void orc_ls_normal_rhs(const orc_operator *op, const double *phi, double *dst)
+{
+  const long n = op_ndofs(op), ncells = ls_cells(op);
+  const int dim = op_dim(op);
+  for (long cell = 0; cell < ncells; ++cell) {
+    geo_t g;
+    long idx[NLOC_MAX];
+    ls_cell_geo(op, cell, &g, idx);
+    const int nq = g.nloc;
+    double out[3][NLOC_MAX];
+    memset(out, 0, sizeof(out));
+    for (int q = 0; q < nq; ++q) {
+      double bv[NLOC_MAX], bg[NLOC_MAX][3];
+      double pq = 0.0, gq[3] = {0, 0, 0};
+      for (int l = 0; l < nq; ++l) {
+        basis_at(&g, l, q, &bv[l], bg[l]);
+        const double u = phi[idx[l]];
+        pq += u * bv[l];
+        for (int e = 0; e < 3; ++e) gq[e] += u * bg[l][e];
+      }
+      const double m = ls_mask(op, pq);
+      double v[3];
+      for (int e = 0; e < 3; ++e) v[e] = m * gq[e];
+      if (op->ls.compute_normalized_vector) {
+        /* normalize<dim>(.., 1e-16), include/meltpooldg/utilities/dealii_tensor.hpp:173-191 */
+        const double zero = 1e-16;
+        const double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int e = 0; e < 3; ++e) v[e] = nrm > zero ? v[e] / nrm : v[e] / zero;
+      }
+      const double jxw = cell_volume_weight(&g, q);
+      for (int d = 0; d < dim; ++d)
+        for (int l = 0; l < nq; ++l) out[d][l] += jxw * v[d] * bv[l];
+    }
+    for (int d = 0; d < dim; ++d) ls_add(op, idx, nq, out[d], dst + d * n);
+  }
+}
+
+void orc_ls_curvature_rhs(const orc_operator *op, const double *normal, double tol, double *dst)
+{
+  const long n = op_ndofs(op), ncells = ls_cells(op);
+  const int dim = op_dim(op);
+  for (long cell = 0; cell < ncells; ++cell) {
+    geo_t g;
+    long idx[NLOC_MAX];
+    ls_cell_geo(op, cell, &g, idx);
+    const int nq = g.nloc;
+    /* nodal unit normal: n/|n| where |n| > tol, else n/tol (dim 1: the sign, +1 where n > tol) */
+    double nh[3][NLOC_MAX];
+    for (int l = 0; l < nq; ++l) {
+      double v[3] = {0, 0, 0}, nrm = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        v[d] = normal[d * n + idx[l]];
+        nrm += v[d] * v[d];
+      }
+      nrm = sqrt(nrm);
+      for (int d = 0; d < 3; ++d) {
+        if (dim == 1)
+          nh[d][l] = d == 0 ? (v[0] > tol ? 1.0 : -1.0) : 0.0;
+        else
+          nh[d][l] = nrm > tol ? v[d] / nrm : v[d] / tol;
+      }
+    }
+    double out[NLOC_MAX];
+    memset(out, 0, sizeof(out));
+    for (int q = 0; q < nq; ++q) {
+      double bv[NLOC_MAX], bg[NLOC_MAX][3];
+      double pq = 0.0, div = 0.0;
+      for (int l = 0; l < nq; ++l) {
+        basis_at(&g, l, q, &bv[l], bg[l]);
+        if (op->ls.narrow_band) pq += op->level_set[idx[l]] * bv[l];
+        for (int d = 0; d < dim; ++d) div += nh[d][l] * bg[l][d];
+      }
+      const double val = -ls_mask(op, pq) * div * cell_volume_weight(&g, q);
+      for (int l = 0; l < nq; ++l) out[l] += val * bv[l];
+    }
+    ls_add(op, idx, nq, out, dst);
+  }
 }

@@ -17,6 +17,8 @@
  *   - FE L2 norm with per-cell float storage                      (source/utilities/vector_tools.cpp:137-201)
  *   - hanging-node constraints of adaptive meshes                 (include/meltpooldg/utilities/constraints.hpp:117-151,
  *                                                                   dealii::DoFTools::make_hanging_node_constraints)
+ *   - level-set normal vector and curvature operators             (source/level_set/normal_vector_operator.cpp:117-366,
+ *                                                                   source/level_set/curvature_operator.cpp:160-371)
  * on structured axis-aligned box meshes in 1, 2 or 3 dimensions with lexicographic DoF numbering,
  * and on 2:1-balanced adaptive octree meshes in 3D.
  *
@@ -63,7 +65,20 @@ typedef struct {
   double vel_scale;    /* ORC_VEL_VORTEX: w = s*(x_last, 0.., -x_0) */
 } orc_advec_params;
 
-enum { ORC_OP_HEAT = 0, ORC_OP_ADVEC = 1 };
+/* ORC_OP_NORMAL: the normal vector projection, a block operator on [dim][n] vectors (the same
+ * scalar operator on every component); ORC_OP_CURVATURE: the curvature projection (scalar). */
+enum { ORC_OP_HEAT = 0, ORC_OP_ADVEC = 1, ORC_OP_NORMAL = 2, ORC_OP_CURVATURE = 3 };
+
+/* NormalVectorData / CurvatureData (include/meltpooldg/level_set/normal_vector_data.hpp:13-46,
+ * curvature_data.hpp:13-46): tangent a(u,v) = sum_q m_q (u v + eta grad u . grad v) JxW with
+ * eta = filter_parameter * (cell diameter / sqrt(dim))^2 and, when narrow_band is set,
+ * m_q = |phi_q| < narrow_band_threshold ? 1 : 0 (phi = the level set), else m_q = 1. */
+typedef struct {
+  double filter_parameter;
+  int narrow_band;
+  double narrow_band_threshold;
+  int compute_normalized_vector; /* normal rhs: submit grad phi / |grad phi| */
+} orc_ls_params;
 
 /* Adaptive hexahedral mesh: leaves of an octree over a base box mesh (2:1 balanced by the
  * caller).  Cell c is the cube of level cells[4c], index (cells[4c+1..4c+3]) on the lattice of
@@ -119,6 +134,8 @@ typedef struct {
   const orc_amesh *amesh;   /* adaptive mesh (NULL: the structured mesh above) */
   const orc_adata *adata;   /* its nodes and constraints */
   const orc_heat_boundary *bnd; /* heat: Robin / Neumann sides (NULL: none) */
+  orc_ls_params ls;         /* ORC_OP_NORMAL / ORC_OP_CURVATURE */
+  const double *level_set;  /* their narrow-band field (read plain) */
 } orc_operator;
 
 /* --- basis / mesh ----------------------------------------------------------------------- */
@@ -140,6 +157,19 @@ void orc_diagonal(const orc_operator *op, double *diag);       /* raw diagonal, 
 void orc_invert_diagonal(int kind, long n, double *diag);      /* reference inversion thresholds */
 void orc_create_rhs(const orc_operator *op, const double *src, const double *q_s, double *dst);
 void orc_local_matrix(const orc_operator *op, long cell, double *A); /* dense element matrix */
+
+/* --- level-set normal vector and curvature right-hand sides (dst += ..., read plain) -------- */
+/* dst [dim][n] += sum_q v m_q grad phi_q JxW, or of the normalised gradient
+ * (NormalVectorOperator::create_rhs, normal_vector_operator.cpp:158-237) */
+void orc_ls_normal_rhs(const orc_operator *op, const double *phi, double *dst);
+/* dst [n] += -sum_q v m_q div(n_hat)_q JxW with n_hat the nodal normal vector normalised with
+ * tolerance tol (CurvatureOperator::create_rhs, curvature_operator.cpp:185-255) */
+void orc_ls_curvature_rhs(const orc_operator *op, const double *normal, double tol, double *dst);
+/* UtilityFunctions::compute_numerical_zero_of_norm (include/meltpooldg/utilities/
+ * utility_functions.hpp:104-143) for a domain of the given volume */
+double orc_numerical_zero_of_norm(int dim, double volume);
+/* number of entries of the operator's vectors (dim * n for ORC_OP_NORMAL) */
+long orc_n_rows(const orc_operator *op);
 
 /* --- adaptive meshes with hanging nodes (vmult, diagonal and solve dispatch on op->amesh) ---- */
 int orc_amesh_build(const orc_amesh *am, orc_adata *ad);  /* 0 = success */

@@ -17,7 +17,7 @@ _LIB_PATH = os.path.join(_HERE, "liborc.so")
 
 LAW_CONSTANT, LAW_TANH = 0, 1
 VEL_NONE, VEL_NODAL, VEL_VORTEX = 0, 1, 2
-OP_HEAT, OP_ADVEC = 0, 1
+OP_HEAT, OP_ADVEC, OP_NORMAL, OP_CURVATURE = 0, 1, 2, 3
 CG, GMRES = 0, 1
 SUCCESS, NO_CONVERGENCE = 0, 1
 
@@ -62,13 +62,20 @@ class HeatBoundary(C.Structure):
 STEFAN_BOLTZMANN = 5.670374419e-8
 
 
+class LsParams(C.Structure):
+    """Normal vector / curvature projection parameters (orc_ls_params)."""
+    _fields_ = [("filter_parameter", C.c_double), ("narrow_band", C.c_int),
+                ("narrow_band_threshold", C.c_double), ("compute_normalized_vector", C.c_int)]
+
+
 class Operator(C.Structure):
     _fields_ = [("kind", C.c_int), ("mesh", Mesh), ("heat", HeatParams), ("advec", AdvecParams),
                 ("vel_kind", C.c_int), ("mask", C.c_void_p), ("use_mask", C.c_int),
                 ("T", C.c_void_p), ("T_old", C.c_void_p), ("vel", C.c_void_p),
                 ("k_cell", C.c_void_p), ("nthreads", C.c_int),
                 ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC)),
-                ("bnd", C.POINTER(HeatBoundary))]
+                ("bnd", C.POINTER(HeatBoundary)), ("ls", LsParams),
+                ("level_set", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -102,6 +109,12 @@ def lib():
         L.orc_amesh_build.argtypes = [C.POINTER(AMeshC), C.POINTER(ADataC)]
         L.orc_adata_free.argtypes = [C.POINTER(ADataC)]
         L.orc_distribute_constraints.argtypes = [C.POINTER(ADataC), dp]
+        L.orc_ls_normal_rhs.argtypes = [C.POINTER(Operator), dp, dp]
+        L.orc_ls_curvature_rhs.argtypes = [C.POINTER(Operator), dp, C.c_double, dp]
+        L.orc_numerical_zero_of_norm.argtypes = [C.c_int, C.c_double]
+        L.orc_numerical_zero_of_norm.restype = C.c_double
+        L.orc_n_rows.argtypes = [C.POINTER(Operator)]
+        L.orc_n_rows.restype = C.c_long
         _lib = L
     return _lib
 
@@ -214,13 +227,15 @@ class OracleOperator:
 
     def __init__(self, kind, mesh, heat=None, advec=None, vel_kind=VEL_NONE, mask=None,
                  T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None,
-                 boundary=None):
+                 boundary=None, ls=None, level_set=None):
         self._keep = []
         self.amesh = amesh
         if amesh is not None:
-            mesh = make_mesh(3, amesh.degree, tuple(amesh.c.ncell))
+            mesh = make_mesh(3, amesh.degree, tuple(amesh.c.ncell), tuple(amesh.c.lo),
+                             tuple(amesh.c.hi))
         self.mesh = mesh
-        self.n = amesh.n_nodes if amesh is not None else n_dofs(mesh)
+        self.n_dofs = amesh.n_nodes if amesh is not None else n_dofs(mesh)
+        self.n = self.n_dofs * (mesh.dim if kind == OP_NORMAL else 1)
 
         def ptr(a, dtype=np.float64):
             if a is None:
@@ -248,6 +263,9 @@ class OracleOperator:
         if boundary is not None:
             self._bnd = boundary
             self.op.bnd = C.pointer(boundary)
+        if ls is not None:
+            self.op.ls = ls
+        self.op.level_set = ptr(level_set)
 
     def vmult(self, src):
         dst = np.zeros(self.n)
@@ -279,6 +297,22 @@ class OracleOperator:
             qs = np.ascontiguousarray(q_s, dtype=np.float64)
         lib().orc_create_rhs(C.byref(self.op), _p(src), _p(qs) if qs is not None else None,
                              _p(dst))
+        return dst
+
+    def normal_rhs(self, phi, dst=None):
+        """NormalVectorOperator::create_rhs: dst [dim][n] += the projected level-set gradient."""
+        dst = np.zeros(self.n) if dst is None else np.array(dst, dtype=np.float64)
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        lib().orc_ls_normal_rhs(C.byref(self.op), _p(phi), _p(dst))
+        return dst
+
+    def curvature_rhs(self, normal, tol=None, dst=None):
+        """CurvatureOperator::create_rhs: dst [n] += -div of the normalised nodal normal."""
+        if tol is None:
+            tol = numerical_zero_of_norm(self.mesh)
+        dst = np.zeros(self.n) if dst is None else np.array(dst, dtype=np.float64)
+        normal = np.ascontiguousarray(normal, dtype=np.float64)
+        lib().orc_ls_curvature_rhs(C.byref(self.op), _p(normal), tol, _p(dst))
         return dst
 
     def local_matrix(self, cell):
@@ -313,6 +347,14 @@ class OracleOperator:
         dv = np.ascontiguousarray(dirichlet_values, dtype=np.float64)
         st = lib().orc_advec_theta_steps(C.byref(self.op), n_steps, _p(dv), _p(phi), _p(norms))
         return phi, norms, st
+
+
+def numerical_zero_of_norm(mesh):
+    """compute_numerical_zero_of_norm of the box mesh's volume."""
+    vol = 1.0
+    for d in range(mesh.dim):
+        vol *= mesh.hi[d] - mesh.lo[d]
+    return float(lib().orc_numerical_zero_of_norm(mesh.dim, vol))
 
 
 def fe_l2_norm(mesh, u):

@@ -85,3 +85,34 @@ def test_boundary_argument_errors():
     b = mf.HeatBoundary()
     b.convection_sides = 1
     assert L.mf_operator_set_boundary(adv._h, C.byref(b)) == mf.MF_ERR_ARG
+
+
+def test_levelset_projection_host_logic():
+    """Level-set projections: argument validation, block sizes and the normalisation tolerance
+    against the oracle's compute_numerical_zero_of_norm (host side only)."""
+    s = mf.ScratchData(2, (3, 2, 2), lo=(0.0, 0.0, 0.0), hi=(1e-4, 2e-4, 5e-5))
+    nop = mf.NormalVectorOperator(s)
+    cop = mf.CurvatureOperator(s)
+    n = s.n_dofs
+    assert nop.n_rows == 3 * n and cop.n_rows == n
+    vol = 1e-4 * 2e-4 * 5e-5
+    assert cop.normal_tolerance == pytest.approx(orc.lib().orc_numerical_zero_of_norm(3, vol),
+                                                 rel=1e-15)
+    assert cop.normal_tolerance == pytest.approx(1e-7)
+    assert mf.CurvatureOperator(mf.ScratchData(1, (2, 2, 2))).normal_tolerance == pytest.approx(1e-3)
+    with pytest.raises(mf.MFError) as e:
+        mf.NormalVectorOperator(s, narrow_band=True, narrow_band_threshold=0.0)
+    assert "narrow band threshold" in str(e.value)
+    with pytest.raises(mf.MFError):
+        mf.CurvatureOperator(s, filter_parameter=-1.0)
+    # a narrow band needs its level set before any application (no device work is started)
+    band = mf.NormalVectorOperator(s, narrow_band=True)
+    L = mf.lib()
+    fake = C.c_void_p(256)
+    assert L.mf_operator_vmult(band._h, fake, fake) == mf.MF_ERR_STATE
+    assert "nullptr" in L.mf_last_error().decode()
+    # the projections take their field from set_level_set only, and no heat source term
+    assert L.mf_operator_set_fields(nop._h, None, None, None) == mf.MF_ERR_ARG
+    assert L.mf_operator_create_rhs(nop._h, fake, fake, fake) == mf.MF_ERR_ARG
+    heat = mf.HeatDiffuseOperator(s, dt=1.0)
+    assert L.mf_operator_set_level_set(heat._h, None) == mf.MF_ERR_ARG
