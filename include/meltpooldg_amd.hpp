@@ -6,6 +6,8 @@
 //   OperatorBase::{reset_time_increment, reset_dof_index}      (include/meltpooldg/core/operator_base.hpp:21-84,189-283)
 //   LinearSolver::solve                                        (include/meltpooldg/linear_algebra/linear_solver.hpp:22-134)
 //   JacobiPreconditioner::{update, vmult}                      (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)
+//   NormalVectorOperator / CurvatureOperator                   (source/level_set/normal_vector_operator.cpp,
+//                                                               source/level_set/curvature_operator.cpp)
 // Errors are thrown as mf::Exception (the reference throws dealii::ExceptionBase) and
 // mf::NoConvergence (dealii::SolverControl::NoConvergence).  DeviceVector is the subset of
 // LinearAlgebra::distributed::Vector the solver layer needs, resident in HBM.
@@ -174,6 +176,41 @@ class AdvectionDiffusionOperator : public OperatorMatrixFree {
   {
     check(mf_operator_set_fields(h_, nullptr, velocity.data(), nullptr));
   }
+};
+
+// NormalVectorOperator / CurvatureOperator (source/level_set/normal_vector_operator.cpp,
+// curvature_operator.cpp): the filtered projections of the level-set gradient and of the
+// curvature.  The normal vector operator works on block vectors [3][n_dofs] (n_rows()).
+class LevelSetProjectionOperator : public OperatorMatrixFree {
+ public:
+  void set_level_set(const VectorType *level_set)
+  {
+    check(mf_operator_set_level_set(h_, level_set ? level_set->data() : nullptr));
+  }
+  std::size_t n_rows() const { return std::size_t(mf_operator_n_rows(h_)); }
+  // create_rhs(dst, src): normal vector: src = level set, dst [3][n_dofs];
+  // curvature: src = normal vector [3][n_dofs], dst [n_dofs]
+  void create_rhs(VectorType &dst, const VectorType &src) const
+  {
+    check(mf_operator_create_rhs(h_, dst.data(), src.data(), nullptr));
+  }
+};
+
+class NormalVectorOperator : public LevelSetProjectionOperator {
+ public:
+  NormalVectorOperator(ScratchData &s, const mf_levelset_projection_params &p)
+  {
+    check(mf_normal_vector_operator_create(s.handle(), &p, &h_));
+  }
+};
+
+class CurvatureOperator : public LevelSetProjectionOperator {
+ public:
+  CurvatureOperator(ScratchData &s, const mf_levelset_projection_params &p)
+  {
+    check(mf_curvature_operator_create(s.handle(), &p, &h_));
+  }
+  double normal_tolerance() const { return mf_operator_normal_tolerance(h_); }
 };
 
 // JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)
