@@ -738,8 +738,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
       if ((rc = cg_reduce_cell_dot(pcell, ncb, sd, st))) return rc;
       if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, sd, it, max_it, st))) return rc;
     }
-    MF_HIP_TRY(hipMemcpyAsync(&h, sd, sizeof(SolveState), hipMemcpyDeviceToHost, st));
-    MF_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = read_to_host(&h, sd, sizeof(SolveState), st))) return rc;
     if (h.stop) break;
     MF_REQUIRE(it < max_it, MF_ERR_STATE, "CG state not stopped at max_it");
     chunk = 8;
@@ -797,8 +796,7 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
     const bool r_is_b = first && x_zero;  // zero initial guess: r0 = b
     if (!r_is_b && (rc = apply_tangent<T>(op, z.p, x, nullptr, nullptr, true))) return rc;
     if ((rc = vec_residual<T>(b, r_is_b ? nullptr : z.p, V0, n, part.p, dh.p + 64, st))) return rc;
-    MF_HIP_TRY(hipMemcpyAsync(hbuf + 64, dh.p + 64, sizeof(double), hipMemcpyDeviceToHost, st));
-    MF_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = read_to_host(hbuf + 64, dh.p + 64, sizeof(double), st))) return rc;
     double rho = hbuf[64];
     if (first) {
       res0 = rho;
@@ -820,8 +818,7 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
       if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p, w, n, part.p, dh.p + 64, st))) return rc;
       if ((rc = gmres_multidot<T>(V.p, n, j + 1, w, n, part.p, dh.p + 32, st))) return rc;
       if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p + 32, w, n, part.p, dh.p + 64, st))) return rc;
-      MF_HIP_TRY(hipMemcpyAsync(hbuf, dh.p, sizeof(double) * 65, hipMemcpyDeviceToHost, st));
-      MF_HIP_TRY(hipStreamSynchronize(st));
+      if ((rc = read_to_host(hbuf, dh.p, sizeof(double) * 65, st))) return rc;
       std::vector<double> hc(j + 2);
       for (int i = 0; i <= j; ++i) hc[i] = hbuf[i] + hbuf[32 + i];
       const double s = hbuf[64];
@@ -956,8 +953,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
       int c;
       if (!xz && (c = apply_tangent<double>(op, q.p, x, nullptr, nullptr, true))) return c;
       if ((c = vec_residual<double>(b, xz ? nullptr : q.p, r.p, n, part.p, nrm.p, st))) return c;
-      MF_HIP_TRY(hipMemcpyAsync(&res, nrm.p, sizeof(double), hipMemcpyDeviceToHost, st));
-      MF_HIP_TRY(hipStreamSynchronize(st));
+      if ((c = read_to_host(&res, nrm.p, sizeof(double), st))) return c;
       return MF_OK;
     };
     double res = 0.0;

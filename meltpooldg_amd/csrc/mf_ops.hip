@@ -497,8 +497,7 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
     double h[2];
     if ((rc = vec_norm<double>(r, n, part, nrm, st))) break;
     if ((rc = vec_norm<double>(d, n, part, nrm + 1, st))) break;
-    MF_HIP_TRY(hipMemcpyAsync(h, nrm, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
-    MF_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = read_to_host(h, nrm, sizeof(double) * 2, st))) break;
     rep.iterations = it + 1;
     rep.residual_norm = h[0];
     rep.correction_norm = h[1];

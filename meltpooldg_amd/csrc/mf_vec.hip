@@ -468,6 +468,37 @@ int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipSt
   return MF_OK;
 }
 
+int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+  // staged through pinned memory and awaited by polling an event: the host sees the result as
+  // soon as the copy ends instead of waking from a blocking stream synchronisation
+  struct Stage {
+    int dev = -1;
+    void *buf = nullptr;
+    hipEvent_t ev = nullptr;
+  };
+  static thread_local Stage stage;
+  MF_REQUIRE(bytes <= 4096, MF_ERR_ARG, "read_to_host: at most 4 KiB");
+  int dev = 0;
+  MF_HIP_TRY(hipGetDevice(&dev));
+  if (stage.dev != dev) {
+    if (stage.buf) (void)hipHostFree(stage.buf);
+    if (stage.ev) (void)hipEventDestroy(stage.ev);
+    stage = Stage{};
+    MF_HIP_TRY(hipHostMalloc(&stage.buf, 4096));
+    MF_HIP_TRY(hipEventCreateWithFlags(&stage.ev, hipEventDisableTiming));
+    stage.dev = dev;
+  }
+  MF_HIP_TRY(hipMemcpyAsync(stage.buf, src, bytes, hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipEventRecord(stage.ev, st));
+  hipError_t e;
+  while ((e = hipEventQuery(stage.ev)) == hipErrorNotReady) {
+  }
+  MF_HIP_TRY(e);
+  std::memcpy(dst, stage.buf, bytes);
+  return MF_OK;
+}
+
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st)
 {
   if (relative) {
@@ -526,8 +557,7 @@ int vec_all_zero(const T *x, int64_t n, double *work, bool &zero, hipStream_t st
                      RED_BLOCKS, work + RED_BLOCKS);
   MF_HIP_TRY(hipGetLastError());
   double m = 1.0;
-  MF_HIP_TRY(hipMemcpyAsync(&m, work + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, st));
-  MF_HIP_TRY(hipStreamSynchronize(st));
+  if (int rc = read_to_host(&m, work + RED_BLOCKS, sizeof(double), st)) return rc;
   zero = (m == 0.0);
   return MF_OK;
 }

@@ -19,6 +19,9 @@ struct SolveState {
   double alpha;    // step length of the current iteration (x is updated one kernel later)
 };
 
+// Device -> host copy of a few bytes that the caller waits for (<= 4 KiB)
+int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st);
+
 template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop = nullptr);
 template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
 int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipStream_t st);
