@@ -230,6 +230,32 @@ int64_t mf_operator_n_rows(const mf_operator *op);
 double mf_operator_normal_tolerance(const mf_operator *op);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Olsson (hyperbolic, continuous Galerkin) reinitialization.  Replaces OlssonOperator          */
+/* (source/level_set/olsson_CG_operator.cpp) in its matrix-free form without interface         */
+/* constraints and wetting faces: tangent a(u,v) = sum_q [u v / dt + (D grad u) . grad v] JxW,  */
+/* D = eps_t I + (eps_n - eps_t) n n^T, n = the interpolated normal vector normalised with the  */
+/* tolerance of compute_numerical_zero_of_norm; create_rhs(dst, psi) adds                       */
+/* sum_q JxW [(1 - psi^2)/2 n - eps_n (grad psi . n) n - eps_t (grad psi - (grad psi . n) n)] . */
+/* grad v (the reference zeroes dst first).  No Dirichlet rows.                                 */
+/* ------------------------------------------------------------------------------------------ */
+enum { MF_THICKNESS_PROPORTIONAL = 0, MF_THICKNESS_ABSOLUTE = 1, MF_THICKNESS_CELLS = 2 };
+
+typedef struct {
+  int thickness_type;                 /* interface_thickness_parameter.type: eps_n = value * h,
+                                         value, or value * h / 6 (h = cell diameter / sqrt(3)) */
+  double thickness_value;             /* interface_thickness_parameter.value (0.5) */
+  double tangential_diffusion_factor; /* hyperbolic.cg.tangential_diffusion_factor (0): eps_t */
+} mf_reinit_params;
+
+int mf_reinit_operator_create(mf_scratch *s, const mf_reinit_params *p, mf_operator **out);
+/* the normal vector [3][n_dofs] (device pointer, kept by reference); call again after it
+ * changes (its FP32 copy is refreshed here) */
+int mf_operator_set_normal_vector(mf_operator *op, const double *normal);
+/* eps_n of the smallest cell: the pseudo time step of the reference is
+ * pseudo_time_step_factor * this value (level_set_operation.cpp:528-532) */
+double mf_reinit_min_epsilon(const mf_operator *op);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Solvers.  Replaces LinearSolver::solve (include/meltpooldg/linear_algebra/                   */
 /* linear_solver.hpp:22-134) with JacobiPreconditioner (preconditioner_jacobi.hpp:29-89):       */
 /* ReductionControl(max_it, abs_tol, rel_tol), success when |r| <= max(abs_tol, rel_tol |r_0|). */

@@ -8,6 +8,7 @@
 //   JacobiPreconditioner::{update, vmult}                      (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)
 //   NormalVectorOperator / CurvatureOperator                   (source/level_set/normal_vector_operator.cpp,
 //                                                               source/level_set/curvature_operator.cpp)
+//   OlssonOperator (reinitialization)                          (source/level_set/olsson_CG_operator.cpp)
 // Errors are thrown as mf::Exception (the reference throws dealii::ExceptionBase) and
 // mf::NoConvergence (dealii::SolverControl::NoConvergence).  DeviceVector is the subset of
 // LinearAlgebra::distributed::Vector the solver layer needs, resident in HBM.
@@ -211,6 +212,26 @@ class CurvatureOperator : public LevelSetProjectionOperator {
     check(mf_curvature_operator_create(s.handle(), &p, &h_));
   }
   double normal_tolerance() const { return mf_operator_normal_tolerance(h_); }
+};
+
+// OlssonOperator (source/level_set/olsson_CG_operator.cpp): the hyperbolic CG reinitialization
+// step (1/dt M + D) delta = rhs with the normal vector of a NormalVectorOperator solve.
+class ReinitializationOperator : public OperatorMatrixFree {
+ public:
+  ReinitializationOperator(ScratchData &s, const mf_reinit_params &p)
+  {
+    check(mf_reinit_operator_create(s.handle(), &p, &h_));
+  }
+  void set_normal_vector(const VectorType *normal)
+  {
+    check(mf_operator_set_normal_vector(h_, normal ? normal->data() : nullptr));
+  }
+  // create_rhs(dst, level set): adds the compressive flux and diffusion terms
+  void create_rhs(VectorType &dst, const VectorType &level_set) const
+  {
+    check(mf_operator_create_rhs(h_, dst.data(), level_set.data(), nullptr));
+  }
+  double min_epsilon() const { return mf_reinit_min_epsilon(h_); }
 };
 
 // JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)

@@ -96,6 +96,16 @@ class LevelSetProjectionParams(C.Structure):
                 ("narrow_band_threshold", C.c_double), ("compute_normalized_vector", C.c_int)]
 
 
+class ReinitParams(C.Structure):
+    """mf_reinit_params: interface_thickness_parameter (type, value) and
+    hyperbolic.cg.tangential_diffusion_factor of ReinitializationData."""
+    _fields_ = [("thickness_type", C.c_int), ("thickness_value", C.c_double),
+                ("tangential_diffusion_factor", C.c_double)]
+
+
+THICKNESS_PROPORTIONAL, THICKNESS_ABSOLUTE, THICKNESS_CELLS = 0, 1, 2
+
+
 class HeatBoundary(C.Structure):
     """mf_heat_boundary: Robin / Neumann sides of the heat operator, side s = 2*direction +
     (0 lower, 1 upper) (colourised hyper_rectangle boundary ids)."""
@@ -172,6 +182,10 @@ def lib():
         L.mf_operator_n_rows.restype = i64
         L.mf_operator_normal_tolerance.argtypes = [vp]
         L.mf_operator_normal_tolerance.restype = C.c_double
+        L.mf_reinit_operator_create.argtypes = [vp, C.POINTER(ReinitParams), C.POINTER(vp)]
+        L.mf_operator_set_normal_vector.argtypes = [vp, vp]
+        L.mf_reinit_min_epsilon.argtypes = [vp]
+        L.mf_reinit_min_epsilon.restype = C.c_double
         L.mf_operator_vmult.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_f32.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_add.argtypes = [vp, vp, vp]
@@ -553,6 +567,37 @@ class CurvatureOperator(_LevelSetProjection):
         return float(lib().mf_operator_normal_tolerance(self._h))
 
 
+class ReinitializationOperator(_OperatorBase):
+    """OlssonOperator (source/level_set/olsson_CG_operator.cpp): tangent u v / dt +
+    (eps_t grad u + (eps_n - eps_t)(grad u . n) n) . grad v with n the normalised normal vector;
+    create_rhs(dst, level_set) adds the compressive flux and diffusion terms."""
+
+    def __init__(self, scratch, thickness_type=THICKNESS_PROPORTIONAL, thickness_value=0.5,
+                 tangential_diffusion_factor=0.0, dt=None, normal=None):
+        p = ReinitParams(int(thickness_type), float(thickness_value),
+                         float(tangential_diffusion_factor))
+        h = C.c_void_p()
+        _check(lib().mf_reinit_operator_create(scratch._h, C.byref(p), C.byref(h)))
+        super().__init__(scratch, h)
+        self._normal = None
+        if dt is not None:
+            self.reset_time_increment(dt)
+        if normal is not None:
+            self.set_normal_vector(normal)
+
+    def set_normal_vector(self, normal):
+        self._normal = normal
+        _check(lib().mf_operator_set_normal_vector(self._h, _ptr(normal)))
+
+    @property
+    def min_epsilon(self):
+        return float(lib().mf_reinit_min_epsilon(self._h))
+
+    @property
+    def normal_tolerance(self):
+        return float(lib().mf_operator_normal_tolerance(self._h))
+
+
 def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
                  rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30, precision="fp64",
                  inner_rel_tolerance=0.0, max_outer=50):
@@ -823,3 +868,78 @@ class CurvatureOperation:
 
     def get_normal_vector(self):
         return self.normal_vector_operation.get_solution_normal_vector()
+
+
+class ReinitializationOperation:
+    """Olsson reinitialization on the device: ReinitializationHyperbolicCGOperation
+    (source/level_set/reinitialization_hyperbolic_CG_operation.cpp:80-280) with the pseudo-time
+    loop of LevelSetOperation::do_reinitialization (source/level_set/level_set_operation.cpp:
+    507-590).  Each pseudo step assembles the right-hand side from the current level set, solves
+    (1/dt M + D) delta = rhs with Jacobi-CG (the previous increment as initial guess), adds delta
+    to the level set and stops early when |delta| / |psi| drops below `tolerance`.  The normal
+    vector is computed from the initial level set, and after every step when
+    `update_normal_vector` is set (the reference does so in its first reinitialization)."""
+
+    def __init__(self, scratch, level_set, thickness_type=THICKNESS_PROPORTIONAL,
+                 thickness_value=0.5, tangential_diffusion_factor=0.0, max_n_steps=5,
+                 pseudo_time_step_size=-1.0, pseudo_time_step_factor=1.0, tolerance=2.2250738585072014e-308,
+                 normal_filter_parameter=2.0, update_normal_vector=False, solver="CG",
+                 rel_tolerance=1e-12, abs_tolerance=1e-20, max_iterations=10000,
+                 precision="fp64"):
+        import torch
+        self.scratch = scratch
+        self.level_set = level_set
+        self.normal_vector_operation = NormalVectorOperation(
+            scratch, level_set, normal_filter_parameter, solver=solver,
+            rel_tolerance=rel_tolerance, abs_tolerance=abs_tolerance,
+            max_iterations=max_iterations, precision=precision)
+        self.operator = ReinitializationOperator(
+            scratch, thickness_type, thickness_value, tangential_diffusion_factor,
+            normal=self.normal_vector_operation.get_solution_normal_vector())
+        if pseudo_time_step_size <= 0.0:
+            # time increment from epsilon of the smallest cell (level_set_operation.cpp:528-532)
+            pseudo_time_step_size = pseudo_time_step_factor * self.operator.min_epsilon
+        self.dt = float(pseudo_time_step_size)
+        self.operator.reset_time_increment(self.dt)
+        n = scratch.n_dofs
+        self.delta = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.rhs = torch.zeros(n, dtype=torch.float64, device=level_set.device)
+        self.preconditioner = JacobiPreconditioner(self.operator)
+        self.max_n_steps = max_n_steps
+        self.tolerance = tolerance
+        self.update_normal_vector = update_normal_vector
+        self.solver = dict(solver=solver, rel_tolerance=rel_tolerance,
+                           abs_tolerance=abs_tolerance, max_iterations=max_iterations,
+                           precision=precision)
+        self.max_change_level_set = None
+        self.iterations = []
+
+    def _normal(self):
+        self.normal_vector_operation.reinit()
+        self.normal_vector_operation.solve()
+        self.operator.set_normal_vector(self.normal_vector_operation.get_solution_normal_vector())
+
+    def solve(self):
+        """One pseudo time step (ReinitializationHyperbolicCGOperation::solve)."""
+        self.rhs.zero_()
+        self.operator.create_rhs(self.rhs, self.level_set)
+        self.preconditioner.update()
+        rep = LinearSolver.solve(self.operator, self.delta, self.rhs,
+                                 preconditioner=self.preconditioner, **self.solver)
+        self.iterations.append(rep["iterations"])
+        self.level_set += self.delta
+        self.max_change_level_set = float(self.delta.norm() / self.level_set.norm())
+        return rep["iterations"]
+
+    def reinitialize(self):
+        """The pseudo-time loop; returns the number of steps done."""
+        self._normal()
+        steps = 0
+        for _ in range(self.max_n_steps):
+            self.solve()
+            steps += 1
+            if self.max_change_level_set < self.tolerance:
+                break
+            if self.update_normal_vector:
+                self._normal()
+        return steps

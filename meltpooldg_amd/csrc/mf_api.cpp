@@ -161,6 +161,13 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.theta = Number(op->advec.theta);
     a.diffusivity = Number(op->advec.diffusivity);
     a.mass_factor = Number(op->advec.mass_factor);
+  } else if (op->kind == 4) {
+    double en, et;
+    reinit_epsilon(op, en, et, a.re_scale_h);
+    a.re_eps_n = Number(en);
+    a.re_eps_t = Number(et);
+    a.re_tol = Number(op->ls_tol);
+    a.re_sign = Number(en > et ? 1.0 : (en < et ? -1.0 : 0.0));
   } else {
     // eta = filter_parameter (diameter / sqrt(3))^2 of a level-0 cell
     const double d2 = m.h[0] * m.h[0] + m.h[1] * m.h[1] + m.h[2] * m.h[2];
@@ -205,6 +212,7 @@ static CellArgs<Number> make_args(const mf_operator *op)
 // T: the temperature of the tanh law, or the level set of a narrow band
 static bool needs_T(const mf_operator *op)
 {
+  if (op->kind == 4) return false;
   if (op->kind >= 2) return op->ls.narrow_band != 0;
   return op->kind == 0 && op->heat.law == MF_LAW_TANH;
 }
@@ -216,6 +224,13 @@ static int64_t n_rows(const mf_operator *op) { return n_blocks(op) * op->s->mesh
 static int check_ready(const mf_operator *op)
 {
   MF_REQUIRE(op && op->s, MF_ERR_ARG, "null operator");
+  if (op->kind == 4) {
+    MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
+               "time increment not set (reference AssertThrowZeroTimeIncrement)");
+    MF_REQUIRE(op->vel, MF_ERR_STATE,
+               "the reinitialization operator needs the normal vector (mf_operator_set_normal_vector)");
+    return MF_OK;
+  }
   if (op->kind >= 2) {
     if (op->ls.narrow_band)
       MF_REQUIRE(op->T, MF_ERR_STATE,
@@ -305,7 +320,10 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   a.dot_partial = dot_partial;
   a.stop = stop;
   a.overwrite = (overwrite && !m.ad && !slab) ? 1 : 0;
-  const int code = op->kind == 0 ? HEAT_TANGENT : (op->kind == 1 ? ADVEC_TANGENT : LS_TANGENT);
+  const int code = op->kind == 0 ? HEAT_TANGENT
+                 : op->kind == 1 ? ADVEC_TANGENT
+                 : op->kind == 4 ? REINIT_TANGENT
+                                 : LS_TANGENT;
   if (m.ad) {
     if (!op->gather_plain && m.ad->n_hanging > 0) {
       Number *hv;
@@ -491,6 +509,46 @@ static int levelset_projection_create(mf_scratch *s, const mf_levelset_projectio
   return MF_OK;
 }
 
+int mf_reinit_operator_create(mf_scratch *s, const mf_reinit_params *p, mf_operator **out)
+{
+  MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(p->thickness_type >= MF_THICKNESS_PROPORTIONAL && p->thickness_type <= MF_THICKNESS_CELLS,
+             MF_ERR_ARG, "unknown interface thickness type");
+  MF_REQUIRE(p->thickness_value > 0.0, MF_ERR_ARG,
+             "Reinitialization: the value of epsilon for the reinitialization function must be "
+             "larger than zero!");
+  MF_REQUIRE(p->tangential_diffusion_factor >= 0.0, MF_ERR_ARG,
+             "tangential_diffusion_factor must be >= 0");
+  mf_levelset_projection_params none{};
+  int rc = levelset_projection_create(s, &none, 4, out);
+  if (rc) return rc;
+  (*out)->reinit = *p;
+  return MF_OK;
+}
+
+int mf_operator_set_normal_vector(mf_operator *op, const double *normal)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->kind == 4, MF_ERR_ARG, "only the reinitialization operator takes a normal vector");
+  const MeshInfo &m = op->s->mesh;
+  op->vel = normal;
+  if (normal) {
+    if (!op->vel32) MF_HIP_TRY(hipMalloc(&op->vel32, sizeof(float) * 3 * m.n_dofs));
+    return vec_cast<double, float>(normal, op->vel32, 1.0, 3 * m.n_dofs, op->s->stream);
+  }
+  return MF_OK;
+}
+
+double mf_reinit_min_epsilon(const mf_operator *op)
+{
+  if (!op || op->kind != 4) return 0.0;
+  double en, et;
+  int scale;
+  reinit_epsilon(op, en, et, scale);
+  if (scale && op->s->mesh.ad) en /= double(int64_t(1) << op->s->mesh.ad->levels);
+  return en;
+}
+
 int mf_normal_vector_operator_create(mf_scratch *s, const mf_levelset_projection_params *p,
                                      mf_operator **out)
 {
@@ -506,7 +564,8 @@ int mf_curvature_operator_create(mf_scratch *s, const mf_levelset_projection_par
 int mf_operator_set_level_set(mf_operator *op, const double *level_set)
 {
   MF_REQUIRE(op, MF_ERR_ARG, "null operator");
-  MF_REQUIRE(op->kind >= 2, MF_ERR_ARG, "only the level-set projections take a level set");
+  MF_REQUIRE(op->kind == 2 || op->kind == 3, MF_ERR_ARG,
+             "only the level-set projections take a level set");
   const MeshInfo &m = op->s->mesh;
   op->T = level_set;
   if (level_set) {
@@ -609,7 +668,10 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
   CellArgs<double> a = make_args<double>(op);
-  const int code = op->kind == 0 ? HEAT_DIAG : (op->kind == 1 ? ADVEC_DIAG : LS_DIAG);
+  const int code = op->kind == 0 ? HEAT_DIAG
+                 : op->kind == 1 ? ADVEC_DIAG
+                 : op->kind == 4 ? REINIT_DIAG
+                                 : LS_DIAG;
   if (m.ad) {
     rc = vec_fill<double>(inv_diag, 0.0, m.n_dofs, st);
     if (rc) return rc;
@@ -617,7 +679,10 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
     // diagonal c_D . (A_cell c_D) per target node D
     rc = launch_cell_list<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
     if (rc) return rc;
-    const int tcode = op->kind == 0 ? HEAT_TANGENT : (op->kind == 1 ? ADVEC_TANGENT : LS_TANGENT);
+    const int tcode = op->kind == 0 ? HEAT_TANGENT
+                      : op->kind == 1 ? ADVEC_TANGENT
+                      : op->kind == 4 ? REINIT_TANGENT
+                                      : LS_TANGENT;
     rc = launch_cell_list<double>(tcode, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
   } else {
     a.overwrite = 1;
@@ -649,8 +714,8 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
   MF_REQUIRE(op->kind == 0 || !q_s, MF_ERR_ARG, "only the heat create_rhs takes a source term");
   if (op->kind >= 2) {
     // normal vector: src = level set, dst [3][n_dofs]; curvature: src = normal vector
-    // [3][n_dofs], dst [n_dofs]
-    return launch_levelset_rhs(op, op->kind == 2 ? 0 : 1, src, dst);
+    // [3][n_dofs], dst [n_dofs]; reinitialization: src = level set, dst [n_dofs]
+    return launch_levelset_rhs(op, op->kind == 2 ? 0 : (op->kind == 3 ? 1 : 2), src, dst);
   }
   CellArgs<double> a = make_args<double>(op);
   a.aux1 = q_s;

@@ -682,7 +682,8 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr int NL = K * K * K;
   constexpr int BUF = NL * CPB;
   constexpr bool LS = (OP == LS_TANGENT || OP == LS_DIAG);
-  constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG || OP == LS_DIAG);
+  constexpr bool REINIT = (OP == REINIT_TANGENT || OP == REINIT_DIAG);
+  constexpr bool IS_DIAG = (OP == HEAT_DIAG || OP == ADVEC_DIAG || OP == LS_DIAG || OP == REINIT_DIAG);
   constexpr bool HAS_U = !IS_DIAG;
   constexpr bool PGRAD = (OP == HEAT_PGRAD);
   // T: the linearisation temperature, or for the level-set projections the level set whose
@@ -1023,6 +1024,30 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       val = A.dt_inv * uq[z] - (Number(1) - A.theta) * vg;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] = -(Number(1) - A.theta) * A.diffusivity * g[d];
+    } else if constexpr (REINIT) {
+      // u v / dt + (eps_t grad u + (eps_n - eps_t) (grad u . n) n) . grad v, n = the normal
+      // vector (held in the velocity slot) normalised with the tolerance
+      Number nv[3] = {w3[0], w3[1], w3[2]};
+      const Number nrm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      const Number den = nrm > A.re_tol ? nrm : A.re_tol;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) nv[d] /= den;
+      const Number sc = A.re_scale_h ? hs : Number(1);
+      const Number en = A.re_eps_n * sc, et = A.re_eps_t * sc;
+      if constexpr (OP == REINIT_TANGENT) {
+        val = A.dt_inv * uq[z];
+        const Number gn = g[0] * nv[0] + g[1] * nv[1] + g[2] * nv[2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] = et * g[d] + (en - et) * gn * nv[d];
+      } else {
+        // the anisotropic part enters the diagonal as sign (b . grad phi)^2 with
+        // b = sqrt(|eps_n - eps_t| JxW) n (stored without the JxW factor applied below)
+        cc = A.dt_inv;
+        kk = et;
+        const Number sb = sqrt((en > et ? en - et : et - en) / jxw);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bb[d] = sb * nv[d];
+      }
     } else if constexpr (LS) {
       // m (u v + eta grad u . grad v); m = 0 where |level set| >= threshold (a NaN keeps 1,
       // as the reference's test does)
@@ -1098,8 +1123,13 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
             const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * ih[0];
             const Number phi = sx * sy * sz;
             const Number gx = dx * sy * sz, gy = sx * dy * sz, gz = sx * sy * dz;
-            diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) +
-                        (b0 * gx + b1 * gy + b2 * gz) * phi;
+            if constexpr (OP == REINIT_DIAG) {
+              const Number bg = b0 * gx + b1 * gy + b2 * gz;
+              diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) + A.re_sign * bg * bg;
+            } else {
+              diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) +
+                          (b0 * gx + b1 * gy + b2 * gz) * phi;
+            }
           }
         }
       }
@@ -1292,6 +1322,8 @@ static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArg
   } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
     if (needT) return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
     return launch_list_k<Number, K, OP, false, false>(m, a, src, dst, st, mode);
+  } else if constexpr (OP == REINIT_TANGENT || OP == REINIT_DIAG) {
+    return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
   } else {
     if (needT) {
       if (vel) return launch_list_k<Number, K, OP, true, true>(m, a, src, dst, st, mode);
@@ -1325,6 +1357,12 @@ static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
   case LS_DIAG:
     if constexpr (sizeof(Number) == 8)
       return launch_list_op<Number, K, LS_DIAG>(needT, vel, m, a, src, dst, st, mode);
+    break;
+  case REINIT_TANGENT:
+    return launch_list_op<Number, K, REINIT_TANGENT>(needT, vel, m, a, src, dst, st, mode);
+  case REINIT_DIAG:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, REINIT_DIAG>(needT, vel, m, a, src, dst, st, mode);
     break;
   default:
     break;
@@ -1363,6 +1401,8 @@ static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Num
   } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
     if (needT) return launch_k<Number, K, OP, true, false>(m, a, src, dst, st);
     return launch_k<Number, K, OP, false, false>(m, a, src, dst, st);
+  } else if constexpr (OP == REINIT_TANGENT || OP == REINIT_DIAG) {
+    return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
   } else {
     if (needT) {
       if (vel) return launch_k<Number, K, OP, true, true>(m, a, src, dst, st);
@@ -1388,6 +1428,10 @@ static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const Cel
   case LS_TANGENT: return launch_op<Number, K, LS_TANGENT>(needT, vel, m, a, src, dst, st);
   case LS_DIAG:
     if constexpr (sizeof(Number) == 8) return launch_op<Number, K, LS_DIAG>(needT, vel, m, a, src, dst, st);
+    break;
+  case REINIT_TANGENT: return launch_op<Number, K, REINIT_TANGENT>(needT, vel, m, a, src, dst, st);
+  case REINIT_DIAG:
+    if constexpr (sizeof(Number) == 8) return launch_op<Number, K, REINIT_DIAG>(needT, vel, m, a, src, dst, st);
     break;
   default: break;
   }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -46,10 +47,15 @@ enum CellOp : int {
   ADVEC_DIAG = 5,    // advection_diffusion_operator.cpp:399-446
   HEAT_PGRAD = 6,    // per-cell sum_q JxW grad(src) . grad(T): conductivity gradient (adjoint)
   LS_TANGENT = 7,    // normal_vector_operator.cpp:316-347, curvature_operator.cpp:339-371
-  LS_DIAG = 8        // their diagonal (normal_vector_operator.cpp:256-311)
+  LS_DIAG = 8,       // their diagonal (normal_vector_operator.cpp:256-311)
+  REINIT_TANGENT = 9,  // olsson_CG_operator.cpp:438-478
+  REINIT_DIAG = 10     // olsson_CG_operator.cpp:357-436
 };
 
-constexpr bool is_tangent_op(int op) { return op == HEAT_TANGENT || op == ADVEC_TANGENT || op == LS_TANGENT; }
+constexpr bool is_tangent_op(int op)
+{
+  return op == HEAT_TANGENT || op == ADVEC_TANGENT || op == LS_TANGENT || op == REINIT_TANGENT;
+}
 
 // Everything a cell kernel needs besides src/dst; passed by value (kernel argument segment).
 template <typename Number>
@@ -74,6 +80,11 @@ struct CellArgs {
   // level-set projections: eta of a level-0 cell (scaled by h^2 on refined cells) and the
   // narrow-band threshold on |T| (T holds the level set)
   Number ls_eta, ls_threshold;
+  // Olsson reinitialization (normal vector in vel): eps_n, eps_t of a level-0 cell (scaled by
+  // the cell size on refined cells when re_scale_h), the normalisation tolerance and
+  // sign(eps_n - eps_t) for the diagonal
+  Number re_eps_n, re_eps_t, re_tol, re_sign;
+  int re_scale_h;
   // fields
   const Number *T;         // linearisation temperature (heat tangent / diagonal)
   const Number *vel;       // nodal velocity [3][n_dofs]
@@ -214,7 +225,8 @@ int heat_face_blocks(const mf_operator *op);
 int cell_blocks(const MeshInfo &m, bool fp32);
 
 // level-set right-hand sides (mf_levelset.hip): mode 0 normal vector (src = level set, dst
-// [3][n_dofs]), 1 curvature (src = normal vector [3][n_dofs], dst [n_dofs]); dst += ...
+// [3][n_dofs]), 1 curvature (src = normal vector [3][n_dofs], dst [n_dofs]), 2 Olsson
+// reinitialization (src = level set, normal vector = op->vel, dst [n_dofs]); dst += ...
 int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, double *dst);
 
 // slice-resident tangent kernel (mf_slab.hip)
@@ -251,7 +263,8 @@ struct Arena {
 
 struct mf_operator {
   mf::Arena arena[4];  // 0: refinement vectors, 1: Krylov vectors, 2: inverse diagonal, 3: small work
-  int kind = 0;  // 0 heat, 1 advection, 2 normal vector (3 blocks), 3 curvature
+  int kind = 0;  // 0 heat, 1 advection, 2 normal vector (3 blocks), 3 curvature, 4 reinit
+  mf_reinit_params reinit{};           // kind 4
   mf_levelset_projection_params ls{};  // kinds 2, 3
   double ls_tol = 0.0;                 // curvature: nodal normalisation tolerance
   mf_scratch *s = nullptr;
@@ -267,3 +280,24 @@ struct mf_operator {
   bool bnd_set = false;
   int gather_plain = 0;  // vmult reads src without constraints (source mass products)
 };
+
+namespace mf {
+// eps_n and eps_t of a level-0 cell of the Olsson operator (compute_interface_thickness_parameter_
+// epsilon of the cell size diameter / sqrt(3), reinitialization_data.hpp:181-197); scale_h: eps
+// scales with the cell size on refined cells
+inline void reinit_epsilon(const mf_operator *op, double &eps_n, double &eps_t, int &scale_h)
+{
+  const MeshInfo &m = op->s->mesh;
+  const double cs = std::sqrt((m.h[0] * m.h[0] + m.h[1] * m.h[1] + m.h[2] * m.h[2]) / 3.0);
+  const mf_reinit_params &r = op->reinit;
+  if (r.thickness_type == MF_THICKNESS_ABSOLUTE) {
+    eps_n = r.thickness_value;
+    scale_h = 0;
+  } else {
+    eps_n = r.thickness_type == MF_THICKNESS_CELLS ? r.thickness_value * cs / 6.0
+                                                   : r.thickness_value * cs;
+    scale_h = 1;
+  }
+  eps_t = r.tangential_diffusion_factor * eps_n;
+}
+}  // namespace mf

@@ -6,6 +6,9 @@
 //     zero = 1e-16, include/meltpooldg/utilities/dealii_tensor.hpp:173-191)
 //   CurvatureOperator::create_rhs (source/level_set/curvature_operator.cpp:185-255):
 //     dst += -sum_q v m_q div(n_hat) JxW, n_hat = the nodal normal normalised with tolerance tol
+//   OlssonOperator::create_rhs (source/level_set/olsson_CG_operator.cpp:197-278):
+//     dst += sum_q JxW f . grad v, f = (1 - psi^2)/2 n - eps_n (grad psi . n) n
+//            - eps_t (grad psi - (grad psi . n) n), n = the interpolated normal normalised with tol
 // with m_q the narrow-band indicator of the level set (level_set_tools.hpp:359-368).  Sources
 // are read plain (read_dof_values_plain); the result is added through the transposed
 // hanging-node constraints on adaptive meshes (distribute_local_to_global).
@@ -29,7 +32,10 @@ struct LsRhsArgs {
   int narrow_band;
   double threshold;
   int normalized;
-  double tol;                 // curvature: nodal normalisation tolerance
+  double tol;                 // normalisation tolerance of the normal vector
+  double eps_n, eps_t;        // reinitialization: of a level-0 cell
+  int scale_h;                // reinitialization: eps scales with the cell size
+  const double *normal;       // reinitialization: normal vector [3][n]
   const double *level_set;    // narrow band (nullable when narrow_band == 0)
   const uint32_t *cell_nodes; // adaptive meshes (null on structured meshes)
   const uint8_t *cell_level;
@@ -92,7 +98,8 @@ __device__ __forceinline__ void add_through_constraints(const LsRhsArgs &A, doub
     atomicAdd(dst + A.line_col[e], A.line_val[e] * v);
 }
 
-// MODE 0: normal vector rhs (src = level set, dst [3][n]); 1: curvature (src = [3][n], dst [n])
+// MODE 0: normal vector rhs (src = level set, dst [3][n]); 1: curvature (src = [3][n], dst [n]);
+// 2: reinitialization (src = level set, A.normal = [3][n], dst [n])
 template <int K, int MODE>
 __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
                                                           const double *__restrict__ src,
@@ -156,6 +163,45 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
 #pragma unroll
       for (int d = 0; d < 3; ++d) add_through_constraints(A, dst + d * n, node, hanging, L[d][t]);
     }
+  } else if constexpr (MODE == 2) {
+    L[0][t] = act ? src[node] : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) L[1 + d][t] = act ? A.normal[d * n + node] : 0.0;
+    __syncthreads();
+    sweep3<K, 4, false>(A.S, L, t, act);  // psi and the normal vector at the quadrature points
+    double f[3] = {0, 0, 0};
+    if (act) {
+      const double pq = L[0][t];
+      const double g[3] = {sweep1<K, 0, false>(A.Dq, L[0], t) / h[0],
+                           sweep1<K, 1, false>(A.Dq, L[0], t) / h[1],
+                           sweep1<K, 2, false>(A.Dq, L[0], t) / h[2]};
+      double nv[3] = {L[1][t], L[2][t], L[3][t]};
+      const double nrm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      const double den = nrm > A.tol ? nrm : A.tol;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) nv[d] /= den;
+      const double sc = A.scale_h ? hs : 1.0;
+      const double en = A.eps_n * sc, et = A.eps_t * sc;
+      const double gn = g[0] * nv[0] + g[1] * nv[1] + g[2] * nv[2];
+      const double cf = 0.5 * (1.0 - pq * pq);  // compressive flux
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        f[d] = jxw * (cf * nv[d] - en * gn * nv[d] - et * (g[d] - gn * nv[d])) / h[d];
+    }
+    __syncthreads();  // L read by the gradient sweeps of every thread
+#pragma unroll
+    for (int d = 0; d < 3; ++d) L[d][t] = f[d];
+    __syncthreads();
+    // sum_d Dq_d^T (JxW f_d / h_d), then S^T in every direction
+    double r = 0.0;
+    if (act)
+      r = sweep1<K, 0, true>(A.Dq, L[0], t) + sweep1<K, 1, true>(A.Dq, L[1], t) +
+          sweep1<K, 2, true>(A.Dq, L[2], t);
+    __syncthreads();
+    L[0][t] = r;
+    __syncthreads();
+    sweep3<K, 1, true>(A.S, L, t, act);
+    if (act) add_through_constraints(A, dst, node, hanging, L[0][t]);
   } else {
     // nodal unit normal: n / |n| where |n| > tol, else n / tol
     double v[3] = {0, 0, 0};
@@ -192,6 +238,9 @@ static int launch_k(const LsRhsArgs &a, int mode, int64_t n_cells, const double 
   if (mode == 0)
     hipLaunchKernelGGL((levelset_rhs_kernel<K, 0>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
                        src, dst);
+  else if (mode == 2)
+    hipLaunchKernelGGL((levelset_rhs_kernel<K, 2>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
+                       src, dst);
   else
     hipLaunchKernelGGL((levelset_rhs_kernel<K, 1>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
                        src, dst);
@@ -224,6 +273,10 @@ int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, doub
   a.normalized = op->ls.compute_normalized_vector ? 1 : 0;
   a.tol = op->ls_tol;
   a.level_set = op->T;
+  if (op->kind == 4) {
+    reinit_epsilon(op, a.eps_n, a.eps_t, a.scale_h);
+    a.normal = op->vel;
+  }
   if (const AdaptiveData *ad = s->ad) {
     a.cell_nodes = ad->d.cell_nodes;
     a.cell_level = ad->d.cell_level;

@@ -429,6 +429,29 @@ static double ls_damping(const orc_operator *op, const geo_t *g)
   return op->ls.filter_parameter * d2 / g->dim;
 }
 
+/* interface thickness eps_n of a cell: compute_interface_thickness_parameter_epsilon of the cell
+ * size (diameter / sqrt(dim); n_subdivisions = 1) (reinitialization_data.hpp:181-197,
+ * olsson_CG_operator.cpp:733-745) */
+static double reinit_eps(const orc_operator *op, const geo_t *g)
+{
+  double d2 = 0.0;
+  for (int d = 0; d < g->dim; ++d) d2 += g->h[d] * g->h[d];
+  const double cs = sqrt(d2 / g->dim);
+  const orc_reinit_params *r = &op->reinit;
+  if (r->thickness_type == ORC_THICKNESS_ABSOLUTE) return r->thickness_value;
+  if (r->thickness_type == ORC_THICKNESS_CELLS) return r->thickness_value * cs / 6.0;
+  return r->thickness_value * cs;
+}
+
+/* normalize<dim>(v, tol) (include/meltpooldg/utilities/dealii_tensor.hpp:173-191) */
+static void normalize_tol(double *v, int dim, double tol)
+{
+  double nrm = 0.0;
+  for (int d = 0; d < dim; ++d) nrm += v[d] * v[d];
+  nrm = sqrt(nrm);
+  for (int d = 0; d < dim; ++d) v[d] = nrm > tol ? v[d] / nrm : v[d] / tol;
+}
+
 /* narrow-band indicator (Tools::compute_mask_narrow_band, level_set_tools.hpp:359-368) */
 static double ls_mask(const orc_operator *op, double phi)
 {
@@ -573,12 +596,47 @@ static void ls_tangent_cell(const orc_operator *op, const geo_t *g, long cell, c
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
 }
 
+/* Olsson reinitialization tangent: values u/dt, gradients eps_t grad u + (eps_n - eps_t)
+ * (grad u . n) n with n the normalised normal vector at q (tangent_cell_operation,
+ * olsson_CG_operator.cpp:438-478; the reference caches n per quadrature point in create_rhs,
+ * from the same normal vector field) */
+static void reinit_tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
+{
+  (void)cell;
+  const int nq = g->nloc, dim = g->dim;
+  const long n = op_ndofs(op);
+  const double en = reinit_eps(op, g);
+  const double et = op->reinit.tangential_diffusion_factor * en;
+  eval_vg(g, ws->u, ws->val, ws->grad, &ws->w);
+  for (int e = 0; e < dim; ++e) {
+    for (int l = 0; l < nq; ++l) ws->tmp[l] = op->vel[e * n + ws->idx[l]];
+    eval_v(g, ws->tmp, ws->wv[e], &ws->w);
+  }
+  for (int q = 0; q < nq; ++q) {
+    double nv[3] = {0, 0, 0}, gu[3] = {0, 0, 0};
+    for (int e = 0; e < dim; ++e) {
+      nv[e] = ws->wv[e][q];
+      gu[e] = ws->grad[e * nq + q];
+    }
+    normalize_tol(nv, dim, op->reinit.normal_tolerance);
+    double gn = 0.0;
+    for (int e = 0; e < dim; ++e) gn += gu[e] * nv[e];
+    const double jxw = cell_volume_weight(g, q);
+    ws->fval[q] = jxw * ws->val[q] / op->reinit.dt;
+    for (int e = 0; e < dim; ++e)
+      ws->flux[e * nq + q] = jxw * (et * gu[e] + (en - et) * gn * nv[e]);
+  }
+  integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+}
+
 static void tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
 {
   if (op->kind == ORC_OP_HEAT)
     heat_tangent_cell(op, g, cell, ws);
   else if (op->kind == ORC_OP_ADVEC)
     advec_tangent_cell(op, g, cell, ws);
+  else if (op->kind == ORC_OP_REINIT)
+    reinit_tangent_cell(op, g, cell, ws);
   else
     ls_tangent_cell(op, g, cell, ws);
 }
@@ -808,6 +866,7 @@ static void basis_at(const geo_t *g, int l, int q, double *phi, double *gphi)
  *   a(u,v) = sum_q JxW [ c u v + (bu . grad u) v + kq grad u . grad v + u (bg . grad v) ] */
 typedef struct {
   double c, k, bu[3], bg[3];
+  double dn, nv[3]; /* + dn (nv . grad u)(nv . grad v) */
 } qcoef_t;
 
 static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, const long *idx,
@@ -861,6 +920,17 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
       c->c = ap->mass_factor / ap->dt;
       c->k = ap->theta * ap->diffusivity;
       for (int e = 0; e < 3; ++e) c->bu[e] = ap->theta * w[e];
+    } else if (op->kind == ORC_OP_REINIT) {
+      double nv[3] = {0, 0, 0};
+      for (int e = 0; e < dim; ++e)
+        for (int l = 0; l < nq; ++l) nv[e] += op->vel[e * n + idx[l]] * phi[l];
+      normalize_tol(nv, dim, op->reinit.normal_tolerance);
+      const double en = reinit_eps(op, g);
+      const double et = op->reinit.tangential_diffusion_factor * en;
+      c->c = 1.0 / op->reinit.dt;
+      c->k = et;
+      c->dn = en - et;
+      for (int e = 0; e < 3; ++e) c->nv[e] = nv[e];
     } else {
       double ph = 0.0;
       if (op->ls.narrow_band)
@@ -872,6 +942,7 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
     const double jxw = cell_volume_weight(g, q);
     c->c *= jxw;
     c->k *= jxw;
+    c->dn *= jxw;
     for (int e = 0; e < 3; ++e) {
       c->bu[e] *= jxw;
       c->bg[e] *= jxw;
@@ -898,7 +969,10 @@ void orc_local_matrix(const orc_operator *op, long cell, double *A)
         const double gg = gphi[j][0] * gphi[i][0] + gphi[j][1] * gphi[i][1] + gphi[j][2] * gphi[i][2];
         const double bu = c->bu[0] * gphi[j][0] + c->bu[1] * gphi[j][1] + c->bu[2] * gphi[j][2];
         const double bg = c->bg[0] * gphi[i][0] + c->bg[1] * gphi[i][1] + c->bg[2] * gphi[i][2];
-        A[i * nq + j] += c->c * phi[j] * phi[i] + bu * phi[i] + c->k * gg + phi[j] * bg;
+        const double ni = c->nv[0] * gphi[i][0] + c->nv[1] * gphi[i][1] + c->nv[2] * gphi[i][2];
+        const double nj = c->nv[0] * gphi[j][0] + c->nv[1] * gphi[j][1] + c->nv[2] * gphi[j][2];
+        A[i * nq + j] +=
+            c->c * phi[j] * phi[i] + bu * phi[i] + c->k * gg + phi[j] * bg + c->dn * ni * nj;
       }
   }
   free(qc);
@@ -956,7 +1030,8 @@ static void diag_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws
       const double gg = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
       const double bu = c->bu[0] * gp[0] + c->bu[1] * gp[1] + c->bu[2] * gp[2];
       const double bg = c->bg[0] * gp[0] + c->bg[1] * gp[1] + c->bg[2] * gp[2];
-      ws->out[i] += c->c * phi * phi + bu * phi + c->k * gg + phi * bg;
+      const double ni = c->nv[0] * gp[0] + c->nv[1] * gp[1] + c->nv[2] * gp[2];
+      ws->out[i] += c->c * phi * phi + bu * phi + c->k * gg + phi * bg + c->dn * ni * ni;
     }
   }
   free(qc);
@@ -1877,6 +1952,43 @@ void orc_ls_curvature_rhs(const orc_operator *op, const double *normal, double t
       }
       const double val = -ls_mask(op, pq) * div * cell_volume_weight(&g, q);
       for (int l = 0; l < nq; ++l) out[l] += val * bv[l];
+    }
+    ls_add(op, idx, nq, out, dst);
+  }
+}
+
+void orc_reinit_rhs(const orc_operator *op, const double *psi, double *dst)
+{
+  const long n = op_ndofs(op), ncells = ls_cells(op);
+  const int dim = op_dim(op);
+  for (long cell = 0; cell < ncells; ++cell) {
+    geo_t g;
+    long idx[NLOC_MAX];
+    ls_cell_geo(op, cell, &g, idx);
+    const int nq = g.nloc;
+    const double en = reinit_eps(op, &g);
+    const double et = op->reinit.tangential_diffusion_factor * en;
+    double out[NLOC_MAX];
+    memset(out, 0, sizeof(out));
+    for (int q = 0; q < nq; ++q) {
+      double bv[NLOC_MAX], bg[NLOC_MAX][3];
+      double pq = 0.0, gq[3] = {0, 0, 0}, nv[3] = {0, 0, 0};
+      for (int l = 0; l < nq; ++l) {
+        basis_at(&g, l, q, &bv[l], bg[l]);
+        const double u = psi[idx[l]];
+        pq += u * bv[l];
+        for (int e = 0; e < 3; ++e) gq[e] += u * bg[l][e];
+        for (int e = 0; e < dim; ++e) nv[e] += op->vel[e * n + idx[l]] * bv[l];
+      }
+      normalize_tol(nv, dim, op->reinit.normal_tolerance);
+      double gn = 0.0;
+      for (int e = 0; e < dim; ++e) gn += gq[e] * nv[e];
+      const double cf = 0.5 * (1.0 - pq * pq); /* compressive_flux, olsson_CG_operator.cpp:24-29 */
+      double f[3];
+      for (int e = 0; e < 3; ++e) f[e] = cf * nv[e] - en * gn * nv[e] - et * (gq[e] - gn * nv[e]);
+      const double jxw = cell_volume_weight(&g, q);
+      for (int l = 0; l < nq; ++l)
+        out[l] += jxw * (f[0] * bg[l][0] + f[1] * bg[l][1] + f[2] * bg[l][2]);
     }
     ls_add(op, idx, nq, out, dst);
   }

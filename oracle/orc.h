@@ -19,6 +19,8 @@
  *                                                                   dealii::DoFTools::make_hanging_node_constraints)
  *   - level-set normal vector and curvature operators             (source/level_set/normal_vector_operator.cpp:117-366,
  *                                                                   source/level_set/curvature_operator.cpp:160-371)
+ *   - Olsson (hyperbolic CG) reinitialization operator            (source/level_set/olsson_CG_operator.cpp:26-29,197-278,
+ *                                                                   438-478,717-748)
  * on structured axis-aligned box meshes in 1, 2 or 3 dimensions with lexicographic DoF numbering,
  * and on 2:1-balanced adaptive octree meshes in 3D.
  *
@@ -67,7 +69,23 @@ typedef struct {
 
 /* ORC_OP_NORMAL: the normal vector projection, a block operator on [dim][n] vectors (the same
  * scalar operator on every component); ORC_OP_CURVATURE: the curvature projection (scalar). */
-enum { ORC_OP_HEAT = 0, ORC_OP_ADVEC = 1, ORC_OP_NORMAL = 2, ORC_OP_CURVATURE = 3 };
+enum { ORC_OP_HEAT = 0, ORC_OP_ADVEC = 1, ORC_OP_NORMAL = 2, ORC_OP_CURVATURE = 3,
+       ORC_OP_REINIT = 4 };
+
+/* ReinitializationData (include/meltpooldg/level_set/reinitialization_data.hpp:80-204) for the
+ * Olsson operator ORC_OP_REINIT: tangent a(u,v) = sum_q [u v / dt + (D grad u) . grad v] JxW,
+ * D = eps_t I + (eps_n - eps_t) n n^T with n the interpolated normal vector (op->vel, [3][n])
+ * normalised with normal_tolerance; eps_n from the interface thickness of the cell
+ * (compute_interface_thickness_parameter_epsilon of cell size / n_subdivisions), eps_t =
+ * tangential_diffusion_factor * eps_n. */
+enum { ORC_THICKNESS_PROPORTIONAL = 0, ORC_THICKNESS_ABSOLUTE = 1, ORC_THICKNESS_CELLS = 2 };
+typedef struct {
+  double dt;                          /* pseudo time step */
+  int thickness_type;                 /* ORC_THICKNESS_* */
+  double thickness_value;             /* interface_thickness_parameter.value (0.5) */
+  double tangential_diffusion_factor; /* hyperbolic.cg.tangential_diffusion_factor (0) */
+  double normal_tolerance;            /* compute_numerical_zero_of_norm of the domain */
+} orc_reinit_params;
 
 /* NormalVectorData / CurvatureData (include/meltpooldg/level_set/normal_vector_data.hpp:13-46,
  * curvature_data.hpp:13-46): tangent a(u,v) = sum_q m_q (u v + eta grad u . grad v) JxW with
@@ -136,6 +154,7 @@ typedef struct {
   const orc_heat_boundary *bnd; /* heat: Robin / Neumann sides (NULL: none) */
   orc_ls_params ls;         /* ORC_OP_NORMAL / ORC_OP_CURVATURE */
   const double *level_set;  /* their narrow-band field (read plain) */
+  orc_reinit_params reinit; /* ORC_OP_REINIT (normal vector in vel, vel_kind ORC_VEL_NODAL) */
 } orc_operator;
 
 /* --- basis / mesh ----------------------------------------------------------------------- */
@@ -165,6 +184,10 @@ void orc_ls_normal_rhs(const orc_operator *op, const double *phi, double *dst);
 /* dst [n] += -sum_q v m_q div(n_hat)_q JxW with n_hat the nodal normal vector normalised with
  * tolerance tol (CurvatureOperator::create_rhs, curvature_operator.cpp:185-255) */
 void orc_ls_curvature_rhs(const orc_operator *op, const double *normal, double tol, double *dst);
+/* OlssonOperator::create_rhs (olsson_CG_operator.cpp:197-278): dst += sum_q JxW f . grad v,
+ * f = (1 - psi^2)/2 n - eps_n (grad psi . n) n
+ *     - eps_t (grad psi - (grad psi . n) n), psi = src (read plain) */
+void orc_reinit_rhs(const orc_operator *op, const double *psi, double *dst);
 /* UtilityFunctions::compute_numerical_zero_of_norm (include/meltpooldg/utilities/
  * utility_functions.hpp:104-143) for a domain of the given volume */
 double orc_numerical_zero_of_norm(int dim, double volume);

@@ -17,7 +17,8 @@ _LIB_PATH = os.path.join(_HERE, "liborc.so")
 
 LAW_CONSTANT, LAW_TANH = 0, 1
 VEL_NONE, VEL_NODAL, VEL_VORTEX = 0, 1, 2
-OP_HEAT, OP_ADVEC, OP_NORMAL, OP_CURVATURE = 0, 1, 2, 3
+OP_HEAT, OP_ADVEC, OP_NORMAL, OP_CURVATURE, OP_REINIT = 0, 1, 2, 3, 4
+THICKNESS_PROPORTIONAL, THICKNESS_ABSOLUTE, THICKNESS_CELLS = 0, 1, 2
 CG, GMRES = 0, 1
 SUCCESS, NO_CONVERGENCE = 0, 1
 
@@ -68,6 +69,13 @@ class LsParams(C.Structure):
                 ("narrow_band_threshold", C.c_double), ("compute_normalized_vector", C.c_int)]
 
 
+class ReinitParams(C.Structure):
+    """Olsson reinitialization parameters (orc_reinit_params)."""
+    _fields_ = [("dt", C.c_double), ("thickness_type", C.c_int),
+                ("thickness_value", C.c_double), ("tangential_diffusion_factor", C.c_double),
+                ("normal_tolerance", C.c_double)]
+
+
 class Operator(C.Structure):
     _fields_ = [("kind", C.c_int), ("mesh", Mesh), ("heat", HeatParams), ("advec", AdvecParams),
                 ("vel_kind", C.c_int), ("mask", C.c_void_p), ("use_mask", C.c_int),
@@ -75,7 +83,7 @@ class Operator(C.Structure):
                 ("k_cell", C.c_void_p), ("nthreads", C.c_int),
                 ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC)),
                 ("bnd", C.POINTER(HeatBoundary)), ("ls", LsParams),
-                ("level_set", C.c_void_p)]
+                ("level_set", C.c_void_p), ("reinit", ReinitParams)]
 
 
 def build(force: bool = False) -> str:
@@ -113,6 +121,7 @@ def lib():
         L.orc_ls_curvature_rhs.argtypes = [C.POINTER(Operator), dp, C.c_double, dp]
         L.orc_numerical_zero_of_norm.argtypes = [C.c_int, C.c_double]
         L.orc_numerical_zero_of_norm.restype = C.c_double
+        L.orc_reinit_rhs.argtypes = [C.POINTER(Operator), dp, dp]
         L.orc_n_rows.argtypes = [C.POINTER(Operator)]
         L.orc_n_rows.restype = C.c_long
         _lib = L
@@ -227,7 +236,7 @@ class OracleOperator:
 
     def __init__(self, kind, mesh, heat=None, advec=None, vel_kind=VEL_NONE, mask=None,
                  T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None,
-                 boundary=None, ls=None, level_set=None):
+                 boundary=None, ls=None, level_set=None, reinit=None):
         self._keep = []
         self.amesh = amesh
         if amesh is not None:
@@ -266,6 +275,8 @@ class OracleOperator:
         if ls is not None:
             self.op.ls = ls
         self.op.level_set = ptr(level_set)
+        if reinit is not None:
+            self.op.reinit = reinit
 
     def vmult(self, src):
         dst = np.zeros(self.n)
@@ -313,6 +324,13 @@ class OracleOperator:
         dst = np.zeros(self.n) if dst is None else np.array(dst, dtype=np.float64)
         normal = np.ascontiguousarray(normal, dtype=np.float64)
         lib().orc_ls_curvature_rhs(C.byref(self.op), _p(normal), tol, _p(dst))
+        return dst
+
+    def reinit_rhs(self, psi, dst=None):
+        """OlssonOperator::create_rhs (dst += ...; the normal vector is the operator's vel)."""
+        dst = np.zeros(self.n) if dst is None else np.array(dst, dtype=np.float64)
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        lib().orc_reinit_rhs(C.byref(self.op), _p(psi), _p(dst))
         return dst
 
     def local_matrix(self, cell):

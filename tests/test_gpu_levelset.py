@@ -132,9 +132,10 @@ def test_block_cg_solve(case, band):
     else:
         # With a band the matrix has zero rows (nodes outside it) and rows of cells that the
         # band only grazes, with tiny diagonal entries: CG needs ~1000 steps and roundoff
-        # decides the last ones.  Bar: the step count within 2 %, and the true residual of
-        # every solution no worse than the oracle's own.
-        assert abs(rep["iterations"] - it_ref) <= max(1, 0.02 * it_ref)
+        # decides the last ones (the oracle's own count moves by ~1 % with its OpenMP thread
+        # count).  Bar: the step count within 5 %, and the true residual of every solution no
+        # worse than the oracle's own.
+        assert abs(rep["iterations"] - it_ref) <= max(1, 0.05 * it_ref)
         r_ref = np.linalg.norm(b - P.rn.vmult(x_ref))
         for xs in (x, x2):
             r = np.linalg.norm(b - P.rn.vmult(xs.cpu().numpy()))
@@ -222,3 +223,118 @@ def test_adaptive(name, band):
     rcv = torch.zeros(n, dtype=torch.float64, device="cuda")
     cop.create_rhs(rcv, _dev(dev3(normal)))
     assert rel(rcv, rc.curvature_rhs(normal, cop.normal_tolerance)[perm]) < TOL
+
+
+# ---------------------------------------------------------------- Olsson reinitialization
+def _reinit_pair(s, m, normal_oracle, dt, factor, ttype=0, value=0.5, amesh=None, perm=None):
+    import meltpooldg_amd as mf
+    n_dev = orc.n_dofs(m) if amesh is None else amesh.n_nodes
+    nd = normal_oracle.reshape(3, -1)
+    normal_dev = nd[:, perm].ravel() if perm is not None else nd.ravel()
+    nd_t = _dev(normal_dev)
+    op = mf.ReinitializationOperator(s, thickness_type=ttype, thickness_value=value,
+                                     tangential_diffusion_factor=factor, dt=dt, normal=nd_t)
+    rp = orc.ReinitParams(dt=dt, thickness_type=ttype, thickness_value=value,
+                          tangential_diffusion_factor=factor, normal_tolerance=op.normal_tolerance)
+    ref = orc.OracleOperator(orc.OP_REINIT, m, vel_kind=orc.VEL_NODAL, vel=normal_oracle,
+                             use_mask=0, reinit=rp, amesh=amesh)
+    return op, ref, nd_t
+
+
+@pytest.mark.parametrize("factor", [0.0, 0.4, 1.5])
+@pytest.mark.parametrize("case", STRUCT, ids=sid)
+def test_reinit_tangent_diagonal_rhs(case, factor):
+    import meltpooldg_amd as mf
+    degree, ncell, hi = case
+    s = mf.ScratchData(degree, ncell, hi=hi, stream=torch.cuda.current_stream())
+    m = orc.make_mesh(3, degree, ncell, (0, 0, 0), hi)
+    xyz = orc.node_coords(m)
+    n = orc.n_dofs(m)
+    normal = (xyz - np.array([0.45, 0.5, 0.52])).T.copy()
+    normal[:, ::9] *= 1e-6  # some nodal normals far below the tolerance
+    op, ref, _ = _reinit_pair(s, m, normal.ravel(), 0.02, factor)
+    u = np.random.default_rng(14).uniform(-1, 1, n)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.vmult(y, _dev(u))
+    assert rel(y, ref.vmult(u)) < TOL
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    op.vmult_f32(y32, _dev(u, torch.float32))
+    assert rel(y32.double(), ref.vmult(u)) < TOL32
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.compute_inverse_diagonal_from_matrixfree(d)
+    assert rel(d, ref.inverse_diagonal()) < TOL
+    psi = sphere(xyz)
+    r = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.create_rhs(r, _dev(psi))
+    assert rel(r, ref.reinit_rhs(psi)) < TOL
+
+
+@pytest.mark.parametrize("name", list(ACASES))
+def test_reinit_adaptive(name):
+    import meltpooldg_amd as mf
+    degree, nc, lev, ce, r, hw = ACASES[name]
+    cells = mf.refine_band_cells(nc, lev, ce, r, hw)
+    s = mf.AdaptiveScratchData(degree, nc, cells, stream=torch.cuda.current_stream())
+    am = orc.AdaptiveMesh(degree, nc, cells)
+    xs = s.support_points()
+    pos = {tuple(np.round(x * 2 ** 24).astype(np.int64)): i for i, x in enumerate(am.xyz)}
+    perm = np.array([pos[tuple(np.round(x * 2 ** 24).astype(np.int64))] for x in xs])
+    n = s.n_dofs
+    m = orc.make_mesh(3, degree, (nc,) * 3, (0, 0, 0), (1, 1, 1))
+    normal = np.concatenate([am.distribute(am.xyz[:, k] - ce[k]) for k in range(3)])
+    op, ref, _ = _reinit_pair(s, m, normal, 0.05, 0.3, amesh=am, perm=perm)
+    u = am.distribute(np.random.default_rng(15).uniform(-1, 1, n))
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.vmult(y, _dev(u[perm]))
+    assert rel(y, ref.vmult(u)[perm]) < TOL
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.compute_inverse_diagonal_from_matrixfree(d)
+    assert rel(d, ref.inverse_diagonal()[perm]) < TOL
+    psi = am.distribute(sphere(am.xyz, c=ce, r=r, eps=0.05))
+    rr = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.create_rhs(rr, _dev(psi[perm]))
+    assert rel(rr, ref.reinit_rhs(psi)[perm]) < TOL
+
+
+def test_reinitialization_operation():
+    """Pseudo-time steps on a sharpened tanh profile: every step matches the oracle's step
+    (rhs, Jacobi-CG, update) and the profile relaxes towards the equilibrium tanh(d / (2 eps))
+    with the zero level set fixed."""
+    import meltpooldg_amd as mf
+    degree, ncell, hi = 2, (10, 10, 10), (1.0, 1.0, 1.0)
+    s = mf.ScratchData(degree, ncell, hi=hi, stream=torch.cuda.current_stream())
+    m = orc.make_mesh(3, degree, ncell, (0, 0, 0), hi)
+    xyz = orc.node_coords(m)
+    c, R = np.array([0.45, 0.5, 0.52]), 0.3
+    dist = np.linalg.norm(xyz - c, axis=1) - R
+    eps = 0.5 * 0.1                            # proportional: 0.5 * cell size
+    psi0 = np.tanh(-dist / (2 * eps) * 3.0)    # three times too sharp
+    psi = _dev(psi0)
+    op = mf.ReinitializationOperation(s, psi, max_n_steps=4)
+    steps = op.reinitialize()
+    assert steps == 4 and op.dt == pytest.approx(eps)
+    # the oracle's chain: normal projection once, then the pseudo steps
+    ls = orc.LsParams(filter_parameter=2.0, narrow_band=0, narrow_band_threshold=0.0,
+                      compute_normalized_vector=0)
+    rn = orc.OracleOperator(orc.OP_NORMAL, m, ls=ls, level_set=psi0, use_mask=0)
+    normal, _, _, st = rn.solve(rn.normal_rhs(psi0), inv_diag=rn.inverse_diagonal())
+    assert st == orc.SUCCESS
+    rp = orc.ReinitParams(dt=op.dt, thickness_type=0, thickness_value=0.5,
+                          tangential_diffusion_factor=0.0,
+                          normal_tolerance=orc.numerical_zero_of_norm(m))
+    ref = orc.OracleOperator(orc.OP_REINIT, m, vel_kind=orc.VEL_NODAL, vel=normal, use_mask=0,
+                             reinit=rp)
+    x = psi0.copy()
+    delta = np.zeros_like(x)
+    inv = ref.inverse_diagonal()
+    for _ in range(4):
+        delta, _, _, st = ref.solve(ref.reinit_rhs(x), x0=delta, inv_diag=inv)
+        assert st == orc.SUCCESS
+        x = x + delta
+    assert rel(psi, x) < 1e-9
+    # relaxation: closer to tanh(-d / (2 eps)) near the interface than the start
+    target = np.tanh(-dist / (2 * eps))
+    near = np.abs(dist) < 0.15
+    err0 = np.abs(psi0 - target)[near].max()
+    err1 = np.abs(psi.cpu().numpy() - target)[near].max()
+    assert err1 < 0.8 * err0

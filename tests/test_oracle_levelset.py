@@ -247,3 +247,116 @@ def test_adaptive_tangent_and_normal_rhs(degree):
     for d in range(3):
         assert np.allclose(rhs[d * n:(d + 1) * n], aa[d] * m1, rtol=0, atol=1e-14)
     assert np.all(m1[am.hanging == 1] == 0)
+
+
+# ---------------------------------------------------------------- Olsson reinitialization
+def reinit_op(m, normal, dt=0.01, value=0.5, factor=0.0, ttype=orc.THICKNESS_PROPORTIONAL,
+              tol=None, amesh=None):
+    rp = orc.ReinitParams(dt=dt, thickness_type=ttype, thickness_value=value,
+                          tangential_diffusion_factor=factor,
+                          normal_tolerance=orc.numerical_zero_of_norm(m) if tol is None else tol)
+    return orc.OracleOperator(orc.OP_REINIT, m, vel_kind=orc.VEL_NODAL, vel=normal, use_mask=0,
+                              reinit=rp, amesh=amesh)
+
+
+def cell_size(m):
+    h = [(m.hi[d] - m.lo[d]) / m.ncell[d] for d in range(m.dim)]
+    return np.sqrt(sum(x * x for x in h) / m.dim)
+
+
+RMESHES = [(3, 1, (3, 2, 2)), (3, 2, (2, 2, 3)), (3, 3, (2, 1, 2))]
+
+
+@pytest.mark.parametrize("dim,deg,nc", RMESHES)
+def test_reinit_tangent_closed_forms(dim, deg, nc):
+    m = orc.make_mesh(dim, deg, nc, (0.0,) * dim, (1.0, 0.7, 1.2))
+    n = orc.n_dofs(m)
+    u = np.random.default_rng(8).uniform(-1, 1, n)
+    eps = 0.5 * cell_size(m)
+    dt = 0.02
+    # three axis-aligned normals without tangential diffusion sum to 3/dt M + eps K
+    total = np.zeros(n)
+    for d in range(3):
+        e = np.zeros((3, n))
+        e[d] = 2.5  # normalised by the operator
+        total += reinit_op(m, e.ravel(), dt=dt).vmult(u)
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, t_dependent=0, rho_cp=3.0, k=eps, dt=dt)
+    heat = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, use_mask=0)
+    ref = heat.vmult(u)
+    assert np.max(np.abs(total - ref)) <= 1e-13 * np.max(np.abs(ref))
+    # tangential factor 1: isotropic eps whatever the normal field
+    rnd = np.random.default_rng(9).uniform(-1, 1, 3 * n)
+    hp1 = orc.HeatParams(law=orc.LAW_CONSTANT, t_dependent=0, rho_cp=1.0, k=eps, dt=dt)
+    iso = orc.OracleOperator(orc.OP_HEAT, m, heat=hp1, use_mask=0).vmult(u)
+    a = reinit_op(m, rnd, dt=dt, factor=1.0).vmult(u)
+    assert np.max(np.abs(a - iso)) <= 1e-13 * np.max(np.abs(iso))
+    # the three thickness laws agree where they give the same eps
+    b1 = reinit_op(m, rnd, dt=dt, value=eps, ttype=orc.THICKNESS_ABSOLUTE).vmult(u)
+    b2 = reinit_op(m, rnd, dt=dt, value=3.0, ttype=orc.THICKNESS_CELLS).vmult(u)
+    b0 = reinit_op(m, rnd, dt=dt).vmult(u)
+    assert np.max(np.abs(b1 - b0)) <= 1e-13 * np.max(np.abs(b0))
+    assert np.max(np.abs(b2 - b0)) <= 1e-13 * np.max(np.abs(b0))
+
+
+@pytest.mark.parametrize("dim,deg,nc", RMESHES)
+def test_reinit_sum_factorisation_matches_dense(dim, deg, nc):
+    m = orc.make_mesh(dim, deg, nc, (0.0,) * dim, (1.0, 0.7, 1.2))
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(10)
+    normal = rng.uniform(-1, 1, (3, n))
+    normal[:, ::5] *= 1e-5  # some nodal normals far below the tolerance
+    op = reinit_op(m, normal.ravel(), factor=0.3)
+    u = rng.uniform(-1, 1, n)
+    a, b = op.vmult(u), op.vmult_dense(u)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+    A = np.stack([op.vmult(np.eye(n)[j]) for j in range(n)], axis=1)
+    assert np.allclose(op.diagonal(), np.diag(A), rtol=1e-13, atol=1e-16)
+    assert np.allclose(A, A.T, rtol=1e-12, atol=1e-14)  # D is symmetric
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3])
+def test_reinit_rhs_moments(deg):
+    m = orc.make_mesh(3, deg, (2, 3, 2), (0.1, 0.0, -0.2), (1.0, 1.2, 0.9))
+    xyz = orc.node_coords(m)
+    n = orc.n_dofs(m)
+    L = np.array([0.9, 1.2, 1.1])
+    vol = np.prod(L)
+    ex = np.zeros((3, n))
+    ex[0] = 1.0
+    # constant psi, normal e_x: only the compressive flux; its moments are closed forms
+    c = 0.3
+    r = reinit_op(m, ex.ravel()).reinit_rhs(np.full(n, c))
+    assert abs(r.sum()) < 1e-13
+    assert abs(r @ xyz[:, 0] - 0.5 * (1 - c * c) * vol) < 1e-13
+    assert abs(r @ xyz[:, 1]) < 1e-13
+    # linear psi = a.x + b, isotropic diffusion eps: sum_i x_i r_i = int f_x,
+    # f = (1 - psi^2)/2 e_x - eps a
+    a = np.array([0.4, -0.3, 0.2])
+    b0 = -0.1
+    psi = xyz @ a + b0
+    eps = 0.5 * cell_size(m)
+    r = reinit_op(m, ex.ravel(), factor=1.0).reinit_rhs(psi)
+    center = np.array([0.55, 0.6, 0.35])
+    int_psi2 = vol * ((a @ center + b0) ** 2 + np.sum(a * a * L * L) / 12.0)
+    assert abs(r @ xyz[:, 0] - (0.5 * (vol - int_psi2) - eps * a[0] * vol)) < 1e-12
+    assert abs(r @ xyz[:, 1] - (-eps * a[1] * vol)) < 1e-12
+    assert abs(r @ xyz[:, 2] - (-eps * a[2] * vol)) < 1e-12
+    # without tangential diffusion the gradient part is eps_n (a . n) n = eps a_x e_x
+    r0 = reinit_op(m, ex.ravel(), factor=0.0).reinit_rhs(psi)
+    assert abs(r0 @ xyz[:, 1]) < 1e-12
+    assert abs(r0 @ xyz[:, 0] - (0.5 * (vol - int_psi2) - eps * a[0] * vol)) < 1e-12
+
+
+@pytest.mark.parametrize("degree", [1, 2])
+def test_reinit_adaptive_isotropic_is_heat(degree):
+    am, cells = _amesh(degree)
+    m = orc.make_mesh(3, degree, (2, 2, 2), (0, 0, 0), (1.0, 1.2, 0.9))
+    h0 = np.array([0.5, 0.6, 0.45])
+    eps_cell = np.array([0.5 * np.sqrt(np.sum((h0 / 2 ** l) ** 2) / 3) for l in cells[:, 0]])
+    rnd = np.random.default_rng(12).uniform(-1, 1, 3 * am.n_nodes)
+    op = reinit_op(m, rnd, dt=0.05, factor=1.0, amesh=am)
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, t_dependent=0, rho_cp=1.0, k=1.0, dt=0.05)
+    heat = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, k_cell=eps_cell, use_mask=0, amesh=am)
+    u = am.distribute(np.random.default_rng(13).uniform(-1, 1, am.n_nodes))
+    a, b = op.vmult(u), heat.vmult(u)
+    assert np.max(np.abs(a - b)) <= 1e-14 * np.max(np.abs(b))
