@@ -169,6 +169,7 @@ def main():
         extra["adaptive"] = adaptive_case(mf, torch, stream)
     if not args.no_levelset:
         extra["levelset"] = levelset_case(mf, torch, stream)
+        extra["levelset_geometry"] = levelset_geometry_case(mf, torch, stream)
 
     traffic = None if args.no_traffic else measure_traffic()
     if traffic:
@@ -345,6 +346,77 @@ def levelset_case(mf, torch, stream):
     print(f"# levelset: {s.n_dofs} DoFs, vmult {t_vm*1e6:.0f} us, GMRES fp64 {t64*1e3:.1f} ms "
           f"({r64['iterations']} it), refined {t32*1e3:.1f} ms ({r32['outer_iterations']}/"
           f"{r32['iterations']}), err {err:.2e}", file=sys.stderr, flush=True)
+    return out
+
+
+def levelset_geometry_case(mf, torch, stream):
+    """Normal vector, curvature and Olsson reinitialization on the level-set workload's mesh
+    (FE_Q2, 96^3 cells, phi0 = tanh(d/(2 eps)) of the sphere R=0.25 at (0.35,0.35,0.35),
+    eps = 1.5h), with the reference's defaults: filter parameter 2, no narrow band, Jacobi-CG to
+    rel 1e-12; interface thickness 0.5 h and pseudo time step eps for the reinitialization.
+    Times one application of each tangent (the normal vector one on [3][n] blocks), each
+    right-hand side, and each solve (FP64 and FP32-refined for the normal vector)."""
+    ncell, deg = 96, 2
+    s = mf.ScratchData(deg, ncell, stream=stream)
+    X = s.support_points()
+    h = 1.0 / ncell
+    eps = 1.5 * h
+    phi0 = np.tanh((0.25 - np.linalg.norm(X - 0.35, axis=1)) / (2 * eps))
+    phi = torch.tensor(phi0, dtype=torch.float64, device="cuda")
+    n = s.n_dofs
+
+    def timed(fn, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    nop = mf.NormalVectorOperator(s, level_set=phi)
+    cop = mf.CurvatureOperator(s, level_set=phi)
+    rn = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    t_nrhs = timed(lambda: (rn.zero_(), nop.create_rhs(rn, phi)))
+    u3 = torch.zeros_like(rn)
+    t_nvm = _time_vmult(torch, nop, rn, u3, 10)
+    pcn, _ = _jacobi(mf, torch, nop)
+    normal, rep_n, t_n64 = _timed_solve(mf, torch, nop, rn, solver="CG", preconditioner=pcn)
+    _, rep_n32, t_n32 = _timed_solve(mf, torch, nop, rn, solver="CG", preconditioner=pcn,
+                                     precision="fp32_refined")
+    rk = torch.zeros(n, dtype=torch.float64, device="cuda")
+    t_krhs = timed(lambda: (rk.zero_(), cop.create_rhs(rk, normal)))
+    pck, _ = _jacobi(mf, torch, cop)
+    kappa, rep_k, t_k = _timed_solve(mf, torch, cop, rk, solver="CG", preconditioner=pck)
+    near = np.abs(np.linalg.norm(X - 0.35, axis=1) - 0.25) < 0.5 * h
+    kappa_near = float(np.median(kappa.cpu().numpy()[near]))
+    rop = mf.ReinitializationOperator(s, normal=normal)
+    rop.reset_time_increment(rop.min_epsilon)
+    rr = torch.zeros(n, dtype=torch.float64, device="cuda")
+    t_rrhs = timed(lambda: (rr.zero_(), rop.create_rhs(rr, phi)))
+    y1 = torch.zeros_like(rr)
+    t_rvm = _time_vmult(torch, rop, phi, y1, 10)
+    pcr, _ = _jacobi(mf, torch, rop)
+    _, rep_r, t_r = _timed_solve(mf, torch, rop, rr, solver="CG", preconditioner=pcr)
+    out = {"workload": "FE_Q2 96^3, sphere level set: normal vector (3 blocks), curvature and "
+                       "Olsson reinitialization step, Jacobi-CG rel 1e-12",
+           "n_dofs": int(n),
+           "normal_rhs_us": t_nrhs * 1e6, "normal_vmult_us": t_nvm * 1e6,
+           "normal_vmult_dofs_per_s": 3 * n / t_nvm,
+           "normal_cg_fp64_s": t_n64, "normal_cg_iterations": rep_n["iterations"],
+           "normal_cg_fp32_refined_s": t_n32,
+           "normal_cg_fp32_refined_inner_iterations": rep_n32["iterations"],
+           "curvature_rhs_us": t_krhs * 1e6, "curvature_cg_s": t_k,
+           "curvature_cg_iterations": rep_k["iterations"],
+           "curvature_median_at_interface": kappa_near, "curvature_exact": 2 / 0.25,
+           "reinit_rhs_us": t_rrhs * 1e6, "reinit_vmult_us": t_rvm * 1e6,
+           "reinit_cg_s": t_r, "reinit_cg_iterations": rep_r["iterations"]}
+    print(f"# levelset geometry: normal vmult {t_nvm*1e6:.0f} us (3 blocks), rhs {t_nrhs*1e6:.0f} "
+          f"us, CG {t_n64*1e3:.1f} ms ({rep_n['iterations']} it) / refined {t_n32*1e3:.1f} ms; "
+          f"curvature rhs {t_krhs*1e6:.0f} us, CG {t_k*1e3:.1f} ms ({rep_k['iterations']} it), "
+          f"kappa {kappa_near:.3f} (exact {2/0.25:.3f}); reinit vmult {t_rvm*1e6:.0f} us, rhs "
+          f"{t_rrhs*1e6:.0f} us, CG {t_r*1e3:.1f} ms ({rep_r['iterations']} it)",
+          file=sys.stderr, flush=True)
     return out
 
 

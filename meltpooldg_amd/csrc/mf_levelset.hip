@@ -13,10 +13,11 @@
 // are read plain (read_dof_values_plain); the result is added through the transposed
 // hanging-node constraints on adaptive meshes (distribute_local_to_global).
 //
-// Each right-hand side is assembled once per projection solve, so one 64-thread block per cell
-// does the work: thread t < K^3 owns local node / quadrature point t = i + K j + K^2 k, the 1D
-// sweeps run through LDS, gradients at the Gauss points use the collocation derivative there,
-// and the nodal results are added with atomics (nodes are shared by up to 8 cells).
+// Each right-hand side is assembled once per solve.  A 256-thread block serves 256 / K^3 cells
+// (32 for Q1, 9 for Q2, 4 for Q3); thread t owns local node / quadrature point
+// t % K^3 = i + K j + K^2 k of cell t / K^3, the 1D sweeps run through LDS, gradients at the
+// Gauss points use the collocation derivative there, and the nodal results are added with
+// atomics (nodes are shared by up to 8 cells).
 #include "mf_internal.hpp"
 
 namespace mf {
@@ -29,6 +30,7 @@ struct LsRhsArgs {
   uint32_t NX, NY;
   uint32_t n_dofs;
   double h[3];
+  int n_cells;
   int narrow_band;
   double threshold;
   int normalized;
@@ -45,12 +47,15 @@ struct LsRhsArgs {
   const double *line_val;
 };
 
-// out[t] = sum_m M[a_dir(t)][m] in[t with a_dir = m]  (TR: M^T), for the K^3 active threads
+constexpr int LS_THREADS = 256;
+
+// out[t] = sum_m M[a_dir][m] in[t with a_dir = m]  (TR: M^T), a_dir the DIR index of the
+// thread's local point tl (t = cell slot * K^3 + tl)
 template <int K, int DIR, bool TR>
-__device__ __forceinline__ double sweep1(const double *M, const double *in, int t)
+__device__ __forceinline__ double sweep1(const double *M, const double *in, int t, int tl)
 {
   constexpr int ST = DIR == 0 ? 1 : (DIR == 1 ? K : K * K);
-  const int a = (t / ST) % K;
+  const int a = (tl / ST) % K;
   const int base = t - a * ST;
   double acc = 0.0;
 #pragma unroll
@@ -61,7 +66,8 @@ __device__ __forceinline__ double sweep1(const double *M, const double *in, int 
 // NF fields of L[0..NF) through S (forward: nodes -> points) or S^T (backward) in all three
 // directions, in place
 template <int K, int NF, bool TR>
-__device__ __forceinline__ void sweep3(const double *S, double (*L)[64], int t, bool act)
+__device__ __forceinline__ void sweep3(const double *S, double (*L)[LS_THREADS], int t, int tl,
+                                       bool act)
 {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -72,11 +78,11 @@ __device__ __forceinline__ void sweep3(const double *S, double (*L)[64], int t, 
       if (!act) {
         r[f] = 0.0;
       } else if (d == 0) {
-        r[f] = TR ? sweep1<K, 2, true>(S, in, t) : sweep1<K, 0, false>(S, in, t);
+        r[f] = TR ? sweep1<K, 2, true>(S, in, t, tl) : sweep1<K, 0, false>(S, in, t, tl);
       } else if (d == 1) {
-        r[f] = TR ? sweep1<K, 1, true>(S, in, t) : sweep1<K, 1, false>(S, in, t);
+        r[f] = TR ? sweep1<K, 1, true>(S, in, t, tl) : sweep1<K, 1, false>(S, in, t, tl);
       } else {
-        r[f] = TR ? sweep1<K, 0, true>(S, in, t) : sweep1<K, 2, false>(S, in, t);
+        r[f] = TR ? sweep1<K, 0, true>(S, in, t, tl) : sweep1<K, 2, false>(S, in, t, tl);
       }
     }
     __syncthreads();  // every thread has read L before it is overwritten
@@ -101,16 +107,18 @@ __device__ __forceinline__ void add_through_constraints(const LsRhsArgs &A, doub
 // MODE 0: normal vector rhs (src = level set, dst [3][n]); 1: curvature (src = [3][n], dst [n]);
 // 2: reinitialization (src = level set, A.normal = [3][n], dst [n])
 template <int K, int MODE>
-__global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
-                                                          const double *__restrict__ src,
-                                                          double *__restrict__ dst)
+__global__ void __launch_bounds__(LS_THREADS) levelset_rhs_kernel(const LsRhsArgs A,
+                                                                  const double *__restrict__ src,
+                                                                  double *__restrict__ dst)
 {
   constexpr int NL = K * K * K;
-  __shared__ double L[8][64];
+  constexpr int CPB = LS_THREADS / NL;
+  __shared__ double L[8][LS_THREADS];
   const int t = int(threadIdx.x);
-  const bool act = t < NL;
-  const int cell = int(blockIdx.x);
-  const int i = t % K, j = (t / K) % K, k = t / (K * K);
+  const int slot = t / NL, tl = t - slot * NL;
+  const int cell = int(blockIdx.x) * CPB + slot;
+  const bool act = slot < CPB && cell < A.n_cells;
+  const int i = tl % K, j = (tl / K) % K, k = tl / (K * K);
   const size_t n = A.n_dofs;
 
   // node and geometry of the cell
@@ -119,11 +127,11 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
   double hs = 1.0;
   if (A.cell_nodes) {
     if (act) {
-      node = A.cell_nodes[size_t(cell) * NL + t];
-      hanging = (A.cell_hmask[cell] >> t) & 1u;
+      node = A.cell_nodes[size_t(cell) * NL + tl];
+      hanging = (A.cell_hmask[cell] >> tl) & 1u;
+      hs = ldexp(1.0, -int(A.cell_level[cell]));
     }
-    hs = ldexp(1.0, -int(A.cell_level[cell]));
-  } else {
+  } else if (act) {
     const int cx = cell % A.ncell[0], cy = (cell / A.ncell[0]) % A.ncell[1],
               cz = cell / (A.ncell[0] * A.ncell[1]);
     constexpr int P = K - 1;
@@ -135,13 +143,13 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
   if constexpr (MODE == 0) {
     L[0][t] = act ? src[node] : 0.0;
     __syncthreads();
-    sweep3<K, 1, false>(A.S, L, t, act);  // phi at the quadrature points
+    sweep3<K, 1, false>(A.S, L, t, tl, act);  // phi at the quadrature points
     double g[3] = {0, 0, 0}, pq = 0.0;
     if (act) {
       pq = L[0][t];
-      g[0] = sweep1<K, 0, false>(A.Dq, L[0], t) / h[0];
-      g[1] = sweep1<K, 1, false>(A.Dq, L[0], t) / h[1];
-      g[2] = sweep1<K, 2, false>(A.Dq, L[0], t) / h[2];
+      g[0] = sweep1<K, 0, false>(A.Dq, L[0], t, tl) / h[0];
+      g[1] = sweep1<K, 1, false>(A.Dq, L[0], t, tl) / h[1];
+      g[2] = sweep1<K, 2, false>(A.Dq, L[0], t, tl) / h[2];
     }
     double msk = 1.0;
     if (A.narrow_band) msk = fabs(pq) >= A.threshold ? 0.0 : 1.0;  // phi itself gives the band
@@ -158,7 +166,7 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
 #pragma unroll
     for (int d = 0; d < 3; ++d) L[d][t] = jxw * g[d];
     __syncthreads();
-    sweep3<K, 3, true>(A.S, L, t, act);
+    sweep3<K, 3, true>(A.S, L, t, tl, act);
     if (act) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) add_through_constraints(A, dst + d * n, node, hanging, L[d][t]);
@@ -168,13 +176,13 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
 #pragma unroll
     for (int d = 0; d < 3; ++d) L[1 + d][t] = act ? A.normal[d * n + node] : 0.0;
     __syncthreads();
-    sweep3<K, 4, false>(A.S, L, t, act);  // psi and the normal vector at the quadrature points
+    sweep3<K, 4, false>(A.S, L, t, tl, act);  // psi and the normal vector at the quadrature points
     double f[3] = {0, 0, 0};
     if (act) {
       const double pq = L[0][t];
-      const double g[3] = {sweep1<K, 0, false>(A.Dq, L[0], t) / h[0],
-                           sweep1<K, 1, false>(A.Dq, L[0], t) / h[1],
-                           sweep1<K, 2, false>(A.Dq, L[0], t) / h[2]};
+      const double g[3] = {sweep1<K, 0, false>(A.Dq, L[0], t, tl) / h[0],
+                           sweep1<K, 1, false>(A.Dq, L[0], t, tl) / h[1],
+                           sweep1<K, 2, false>(A.Dq, L[0], t, tl) / h[2]};
       double nv[3] = {L[1][t], L[2][t], L[3][t]};
       const double nrm = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
       const double den = nrm > A.tol ? nrm : A.tol;
@@ -195,12 +203,12 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
     // sum_d Dq_d^T (JxW f_d / h_d), then S^T in every direction
     double r = 0.0;
     if (act)
-      r = sweep1<K, 0, true>(A.Dq, L[0], t) + sweep1<K, 1, true>(A.Dq, L[1], t) +
-          sweep1<K, 2, true>(A.Dq, L[2], t);
+      r = sweep1<K, 0, true>(A.Dq, L[0], t, tl) + sweep1<K, 1, true>(A.Dq, L[1], t, tl) +
+          sweep1<K, 2, true>(A.Dq, L[2], t, tl);
     __syncthreads();
     L[0][t] = r;
     __syncthreads();
-    sweep3<K, 1, true>(A.S, L, t, act);
+    sweep3<K, 1, true>(A.S, L, t, tl, act);
     if (act) add_through_constraints(A, dst, node, hanging, L[0][t]);
   } else {
     // nodal unit normal: n / |n| where |n| > tol, else n / tol
@@ -215,18 +223,18 @@ __global__ void __launch_bounds__(64) levelset_rhs_kernel(const LsRhsArgs A,
     for (int d = 0; d < 3; ++d) L[d][t] = v[d] / den;
     L[3][t] = (act && A.narrow_band) ? A.level_set[node] : 0.0;
     __syncthreads();
-    sweep3<K, 4, false>(A.S, L, t, act);
+    sweep3<K, 4, false>(A.S, L, t, tl, act);
     double div = 0.0, pq = 0.0;
     if (act) {
-      div = sweep1<K, 0, false>(A.Dq, L[0], t) / h[0] + sweep1<K, 1, false>(A.Dq, L[1], t) / h[1] +
-            sweep1<K, 2, false>(A.Dq, L[2], t) / h[2];
+      div = sweep1<K, 0, false>(A.Dq, L[0], t, tl) / h[0] + sweep1<K, 1, false>(A.Dq, L[1], t, tl) / h[1] +
+            sweep1<K, 2, false>(A.Dq, L[2], t, tl) / h[2];
       pq = L[3][t];
     }
     const double msk = (A.narrow_band && fabs(pq) >= A.threshold) ? 0.0 : 1.0;
     __syncthreads();
     L[0][t] = -msk * div * jxw;
     __syncthreads();
-    sweep3<K, 1, true>(A.S, L, t, act);
+    sweep3<K, 1, true>(A.S, L, t, tl, act);
     if (act) add_through_constraints(A, dst, node, hanging, L[0][t]);
   }
 }
@@ -235,15 +243,14 @@ template <int K>
 static int launch_k(const LsRhsArgs &a, int mode, int64_t n_cells, const double *src, double *dst,
                     hipStream_t st)
 {
+  constexpr int CPB = LS_THREADS / (K * K * K);
+  const dim3 grid(unsigned((n_cells + CPB - 1) / CPB));
   if (mode == 0)
-    hipLaunchKernelGGL((levelset_rhs_kernel<K, 0>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
-                       src, dst);
+    hipLaunchKernelGGL((levelset_rhs_kernel<K, 0>), grid, dim3(LS_THREADS), 0, st, a, src, dst);
   else if (mode == 2)
-    hipLaunchKernelGGL((levelset_rhs_kernel<K, 2>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
-                       src, dst);
+    hipLaunchKernelGGL((levelset_rhs_kernel<K, 2>), grid, dim3(LS_THREADS), 0, st, a, src, dst);
   else
-    hipLaunchKernelGGL((levelset_rhs_kernel<K, 1>), dim3(unsigned(n_cells)), dim3(64), 0, st, a,
-                       src, dst);
+    hipLaunchKernelGGL((levelset_rhs_kernel<K, 1>), grid, dim3(LS_THREADS), 0, st, a, src, dst);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
@@ -268,6 +275,7 @@ int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, doub
   a.NX = uint32_t(m.NN[0]);
   a.NY = uint32_t(m.NN[1]);
   a.n_dofs = uint32_t(m.n_dofs);
+  a.n_cells = int(m.n_cells);
   a.narrow_band = op->ls.narrow_band ? 1 : 0;
   a.threshold = op->ls.narrow_band_threshold;
   a.normalized = op->ls.compute_normalized_vector ? 1 : 0;
