@@ -256,6 +256,26 @@ int mf_operator_set_normal_vector(mf_operator *op, const double *normal);
 double mf_reinit_min_epsilon(const mf_operator *op);
 
 /* ------------------------------------------------------------------------------------------ */
+/* Formats around the path.                                                                     */
+/* ------------------------------------------------------------------------------------------ */
+/* VTU output of device-resident fields (replaces GenericDataOut / Postprocessor with DataOut::  */
+/* write_vtu, source/post_processing/generic_data_out.cpp, postprocessor.cpp): one point per     */
+/* DoF (its support point), every cell written as degree^3 trilinear hexahedra on its nodes,    */
+/* appended raw binary.  fields[f]: device pointer, n_components[f] = 1 ([n_dofs]) or 3          */
+/* ([3][n_dofs], written as a vector). */
+int mf_write_vtu(const mf_scratch *s, const char *filename, int n_fields, const char *const *names,
+                 const int *n_components, const double *const *fields);
+/* Restart (replaces Restart::serialize_internal / deserialize_internal, source/utilities/       */
+/* restart.cpp:115-186): one file with the mesh description (box, degree, adaptive cells, the   */
+/* Dirichlet mask) and the given device vectors.  Loading rebuilds the scratch, and with it the  */
+/* same DoF numbering, then reads the vectors back (lengths must match). */
+int mf_restart_save(const mf_scratch *s, const char *path, int n_vectors,
+                    const double *const *vectors, const int64_t *lengths);
+int mf_restart_load_scratch(const char *path, mf_scratch **out);
+int mf_restart_load_vectors(const mf_scratch *s, const char *path, int n_vectors,
+                            double *const *vectors, const int64_t *lengths);
+
+/* ------------------------------------------------------------------------------------------ */
 /* Solvers.  Replaces LinearSolver::solve (include/meltpooldg/linear_algebra/                   */
 /* linear_solver.hpp:22-134) with JacobiPreconditioner (preconditioner_jacobi.hpp:29-89):       */
 /* ReductionControl(max_it, abs_tol, rel_tol), success when |r| <= max(abs_tol, rel_tol |r_0|). */

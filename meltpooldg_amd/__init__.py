@@ -182,6 +182,12 @@ def lib():
         L.mf_operator_n_rows.restype = i64
         L.mf_operator_normal_tolerance.argtypes = [vp]
         L.mf_operator_normal_tolerance.restype = C.c_double
+        L.mf_write_vtu.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(C.c_char_p),
+                                   C.POINTER(C.c_int), C.POINTER(vp)]
+        L.mf_restart_save.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(vp), C.POINTER(i64)]
+        L.mf_restart_load_scratch.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.mf_restart_load_vectors.argtypes = [vp, C.c_char_p, C.c_int, C.POINTER(vp),
+                                              C.POINTER(i64)]
         L.mf_reinit_operator_create.argtypes = [vp, C.POINTER(ReinitParams), C.POINTER(vp)]
         L.mf_operator_set_normal_vector.argtypes = [vp, vp]
         L.mf_reinit_min_epsilon.argtypes = [vp]
@@ -943,3 +949,59 @@ class ReinitializationOperation:
             if self.update_normal_vector:
                 self._normal()
         return steps
+
+
+def write_vtu(scratch, filename, fields):
+    """VTU output of device fields: fields maps a name to a tensor of n_dofs (scalar) or
+    3 n_dofs ([3][n_dofs], a vector) entries (mf_write_vtu)."""
+    names = list(fields)
+    n = scratch.n_dofs
+    ncomp = []
+    for k in names:
+        m = fields[k].numel()
+        if m not in (n, 3 * n):
+            raise ValueError(f"field {k}: {m} entries, expected {n} or {3 * n}")
+        ncomp.append(1 if m == n else 3)
+    arr_n = (C.c_char_p * len(names))(*[k.encode() for k in names])
+    arr_c = (C.c_int * len(names))(*ncomp)
+    arr_f = (C.c_void_p * len(names))(*[_ptr(fields[k]) for k in names])
+    _check(lib().mf_write_vtu(scratch._h, str(filename).encode(), len(names), arr_n, arr_c,
+                              arr_f))
+
+
+def restart_save(scratch, path, vectors):
+    """Restart file with the mesh description and the given device vectors (mf_restart_save)."""
+    arr_v = (C.c_void_p * len(vectors))(*[_ptr(v) for v in vectors])
+    arr_l = (C.c_int64 * len(vectors))(*[v.numel() for v in vectors])
+    _check(lib().mf_restart_save(scratch._h, str(path).encode(), len(vectors), arr_v, arr_l))
+
+
+def restart_load(path, vectors=None, stream=None):
+    """Rebuild the scratch of a restart file (same DoF numbering) and read its vectors into the
+    given device tensors (mf_restart_load_scratch / mf_restart_load_vectors)."""
+    import numpy as np
+    h = C.c_void_p()
+    _check(lib().mf_restart_load_scratch(str(path).encode(), C.byref(h)))
+    raw = np.fromfile(str(path), dtype=np.uint8, count=4096)
+    deg, adaptive = np.frombuffer(raw[8:16].tobytes(), dtype=np.int32)
+    ncell = tuple(int(x) for x in np.frombuffer(raw[16:28].tobytes(), dtype=np.int32))
+    lo = tuple(np.frombuffer(raw[28:52].tobytes(), dtype=np.float64))
+    hi = tuple(np.frombuffer(raw[52:76].tobytes(), dtype=np.float64))
+    n_cells = int(np.frombuffer(raw[76:84].tobytes(), dtype=np.int64)[0])
+    s = object.__new__(AdaptiveScratchData if adaptive else ScratchData)
+    s.degree, s.ncell, s.lo, s.hi = int(deg), ncell, lo, hi
+    if adaptive:
+        s.adaptive = True
+        s.cells = np.fromfile(str(path), dtype=np.int32, count=4 * n_cells,
+                              offset=84).reshape(-1, 4)
+    s._h = h
+    s.n_dofs = int(lib().mf_scratch_n_dofs(h))
+    s.n_cells = int(lib().mf_scratch_n_cells(h))
+    if stream is not None:
+        s.set_stream(stream)
+    if vectors:
+        arr_v = (C.c_void_p * len(vectors))(*[_ptr(v) for v in vectors])
+        arr_l = (C.c_int64 * len(vectors))(*[v.numel() for v in vectors])
+        _check(lib().mf_restart_load_vectors(s._h, str(path).encode(), len(vectors), arr_v,
+                                             arr_l))
+    return s
