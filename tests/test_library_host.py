@@ -116,3 +116,60 @@ def test_levelset_projection_host_logic():
     assert L.mf_operator_create_rhs(nop._h, fake, fake, fake) == mf.MF_ERR_ARG
     heat = mf.HeatDiffuseOperator(s, dt=1.0)
     assert L.mf_operator_set_level_set(heat._h, None) == mf.MF_ERR_ARG
+
+
+def test_reinit_operator_host_logic():
+    """Olsson reinitialization operator: argument validation and the pseudo time step scale
+    (eps of the smallest cell), host side only."""
+    s = mf.ScratchData(2, (4, 2, 2), hi=(1.0, 0.5, 0.5))
+    op = mf.ReinitializationOperator(s)
+    h = np.sqrt((0.25 ** 2 * 3) / 3)   # diameter / sqrt(3) of a 0.25^3 cell
+    assert op.min_epsilon == pytest.approx(0.5 * h)
+    assert mf.ReinitializationOperator(s, thickness_type=mf.THICKNESS_ABSOLUTE,
+                                       thickness_value=0.01).min_epsilon == pytest.approx(0.01)
+    assert mf.ReinitializationOperator(s, thickness_type=mf.THICKNESS_CELLS,
+                                       thickness_value=3.0).min_epsilon == pytest.approx(0.5 * h)
+    with pytest.raises(mf.MFError) as e:
+        mf.ReinitializationOperator(s, thickness_value=0.0)
+    assert "larger than zero" in str(e.value)
+    with pytest.raises(mf.MFError):
+        mf.ReinitializationOperator(s, thickness_type=7)
+    L = mf.lib()
+    fake = C.c_void_p(256)
+    # no time increment / no normal vector: errors before any device work
+    assert L.mf_operator_vmult(op._h, fake, fake) == mf.MF_ERR_STATE
+    op.reset_time_increment(0.1)
+    assert L.mf_operator_vmult(op._h, fake, fake) == mf.MF_ERR_STATE
+    assert "normal vector" in L.mf_last_error().decode()
+    heat = mf.HeatDiffuseOperator(s, dt=1.0)
+    assert L.mf_operator_set_normal_vector(heat._h, None) == mf.MF_ERR_ARG
+
+
+def test_vtu_and_restart_without_fields(tmp_path):
+    """Mesh-only VTU output and a restart file without vectors or mask need no device: the
+    written mesh is parsed back, the restart rebuilds the same scratch."""
+    import re
+    s = mf.ScratchData(2, (3, 2, 1), lo=(0.0, 0.5, 0.0), hi=(1.5, 1.5, 0.25))
+    path = tmp_path / "mesh.vtu"
+    L = mf.lib()
+    assert L.mf_write_vtu(s._h, str(path).encode(), 0, None, None, None) == mf.MF_OK
+    raw = open(path, "rb").read()
+    head = raw[:raw.index(b"<AppendedData")].decode()
+    assert int(re.search(r'NumberOfPoints="(\d+)"', head).group(1)) == s.n_dofs
+    assert int(re.search(r'NumberOfCells="(\d+)"', head).group(1)) == s.n_cells * 8
+    off = int(re.search(r'Name="Points" NumberOfComponents="3" format="appended" '
+                        r'offset="(\d+)"', head).group(1))
+    us = raw.index(b"_", raw.index(b"<AppendedData")) + 1
+    nb = int(np.frombuffer(raw[us + off:us + off + 8], dtype=np.uint64)[0])
+    pts = np.frombuffer(raw[us + off + 8:us + off + 8 + nb], dtype=np.float64).reshape(-1, 3)
+    assert np.array_equal(pts, s.support_points())
+    rp = tmp_path / "mesh.mfr"
+    assert L.mf_restart_save(s._h, str(rp).encode(), 0, None, None) == mf.MF_OK
+    s2 = mf.restart_load(rp)
+    assert (s2.n_dofs, s2.n_cells, s2.degree, s2.ncell) == (s.n_dofs, s.n_cells, 2, (3, 2, 1))
+    assert s2.lo == (0.0, 0.5, 0.0) and s2.hi == (1.5, 1.5, 0.25)
+    assert np.array_equal(s2.support_points(), s.support_points())
+    bad = tmp_path / "bad.mfr"
+    bad.write_bytes(b"not a restart file")
+    h = C.c_void_p()
+    assert L.mf_restart_load_scratch(str(bad).encode(), C.byref(h)) == mf.MF_ERR_ARG
