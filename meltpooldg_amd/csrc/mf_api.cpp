@@ -800,8 +800,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
     for (int k = 0; k < chunk && it < max_it; ++k) {
       ++it;
       if ((rc = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return rc;
-      if ((rc = cg_reduce_cell_dot(pcell, ncb, sd, st))) return rc;
-      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, sd, it, max_it, st))) return rc;
+      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, pcell, ncb, sd, it, max_it, st))) return rc;
     }
     if ((rc = read_to_host(&h, sd, sizeof(SolveState), st))) return rc;
     if (h.stop) break;

@@ -85,6 +85,73 @@ __device__ T reduce_partials(const T *part, int n)
   return result;
 }
 
+// 16-byte chunks of the CG vectors: N entries per load/store (used when every pointer is
+// 16-byte aligned; the entries past the last whole chunk take the scalar loop)
+template <typename T>
+struct Chunk;
+template <>
+struct Chunk<double> {
+  static constexpr int N = 2;
+  using V = double2;
+};
+template <>
+struct Chunk<float> {
+  static constexpr int N = 4;
+  using V = float4;
+};
+template <typename T>
+union ChunkU {
+  typename Chunk<T>::V v;
+  T a[Chunk<T>::N];
+};
+
+__device__ __forceinline__ bool aligned16(const void *p) { return (uintptr_t(p) & 15u) == 0; }
+
+// three reduce_partials in one pass (the same arithmetic, one set of block barriers)
+template <typename T>
+__device__ void reduce_partials3(const T *pa, const T *pb, const T *pc, int n, T &ra, T &rb, T &rc)
+{
+  __shared__ T result[3];
+  __shared__ T rs[3][16], rcmp[3][16];
+  Acc<T> a[3];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a[0].merge(pa[i], pa[n + i]);
+    a[1].merge(pb[i], pb[n + i]);
+    a[2].merge(pc[i], pc[n + i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T os = __shfl_xor(a[k].s, off, 64);
+      const T oc = __shfl_xor(a[k].c, off, 64);
+      a[k].merge(os, oc);
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      rs[k][wid] = a[k].s;
+      rcmp[k][wid] = a[k].c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      Acc<T> r;
+      for (int i = 0; i < nw; ++i) r.merge(rs[k][i], rcmp[k][i]);
+      result[k] = r.value();
+    }
+  }
+  __syncthreads();
+  ra = result[0];
+  rb = result[1];
+  rc = result[2];
+}
+
 template <typename T>
 __device__ void write_partial(T *part, int n, Acc<T> a)
 {
@@ -110,17 +177,47 @@ __global__ void fill_kernel(T *x, T v, int64_t n, const int *stop)
 template <typename Ta, typename Tb>
 __global__ void cast_kernel(const Ta *a, Tb *b, double scale, int64_t n)
 {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    b[i] = Tb(double(a[i]) * scale);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if constexpr (sizeof(Ta) == 8 && sizeof(Tb) == 4) {
+    // 4 entries per step: two 16-byte loads, one 16-byte store
+    if (aligned16(a) && aligned16(b)) {
+      const int64_t nc = n / 4;
+      for (int64_t c = i0; c < nc; c += stride) {
+        const double2 u = reinterpret_cast<const double2 *>(a)[2 * c];
+        const double2 v = reinterpret_cast<const double2 *>(a)[2 * c + 1];
+        reinterpret_cast<float4 *>(b)[c] = make_float4(float(u.x * scale), float(u.y * scale),
+                                                       float(v.x * scale), float(v.y * scale));
+      }
+      tail = nc * 4;
+    }
+  }
+  for (int64_t i = tail + i0; i < n; i += stride) b[i] = Tb(double(a[i]) * scale);
 }
 
 // x += scale * e  (FP64 solution update of the defect correction)
 __global__ void add_scaled_f32_kernel(double *x, const float *e, double scale, int64_t n)
 {
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x)
-    x[i] += scale * double(e[i]);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(x) && aligned16(e)) {
+    const int64_t nc = n / 4;
+    for (int64_t c = i0; c < nc; c += stride) {
+      const float4 d = reinterpret_cast<const float4 *>(e)[c];
+      double2 u = reinterpret_cast<const double2 *>(x)[2 * c];
+      double2 v = reinterpret_cast<const double2 *>(x)[2 * c + 1];
+      u.x += scale * double(d.x);
+      u.y += scale * double(d.y);
+      v.x += scale * double(d.z);
+      v.y += scale * double(d.w);
+      reinterpret_cast<double2 *>(x)[2 * c] = u;
+      reinterpret_cast<double2 *>(x)[2 * c + 1] = v;
+    }
+    tail = nc * 4;
+  }
+  for (int64_t i = tail + i0; i < n; i += stride) x[i] += scale * double(e[i]);
 }
 
 // r = b - q, partials of r.r
@@ -129,9 +226,30 @@ __global__ void residual_kernel(const T *__restrict__ b, const T *__restrict__ q
                                 T *__restrict__ r, int64_t n, T *part)
 {
   Acc<T> a;
+  constexpr int N = Chunk<T>::N;
+  using V = typename Chunk<T>::V;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(b) && aligned16(r) && (!q || aligned16(q))) {
+    const int64_t nc = n / N;
+#pragma unroll 2
+    for (int64_t c = i0; c < nc; c += stride) {
+      ChunkU<T> vb, vq;
+      vb.v = reinterpret_cast<const V *>(b)[c];
+      if (q) vq.v = reinterpret_cast<const V *>(q)[c];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const T v = q ? vb.a[k] - vq.a[k] : vb.a[k];
+        vb.a[k] = v;
+        a.add(v * v);
+      }
+      reinterpret_cast<V *>(r)[c] = vb.v;
+    }
+    tail = nc * N;
+  }
 #pragma unroll 4
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t i = tail + i0; i < n; i += stride) {
     const T v = q ? b[i] - q[i] : b[i];  // q = null: A x = 0 (zero initial guess)
     r[i] = v;
     a.add(v * v);
@@ -201,9 +319,36 @@ __global__ void cg_init_kernel(const T *__restrict__ b, const T *__restrict__ q,
                                T *part_rr, T *part_rz0)
 {
   Acc<T> arr, arz;
+  constexpr int N = Chunk<T>::N;
+  using V = typename Chunk<T>::V;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(b) && aligned16(r) && aligned16(p) && (!q || aligned16(q)) &&
+      (!inv || aligned16(inv))) {
+    const int64_t nc = n / N;
+#pragma unroll 2
+    for (int64_t c = i0; c < nc; c += stride) {
+      ChunkU<T> vb, vq, vd, vz;
+      vb.v = reinterpret_cast<const V *>(b)[c];
+      if (q) vq.v = reinterpret_cast<const V *>(q)[c];
+      if (inv) vd.v = reinterpret_cast<const V *>(inv)[c];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const T v = q ? vb.a[k] - vq.a[k] : vb.a[k];
+        const T z = inv ? vd.a[k] * v : v;
+        vb.a[k] = v;
+        vz.a[k] = z;
+        arr.add(v * v);
+        arz.add(v * z);
+      }
+      reinterpret_cast<V *>(r)[c] = vb.v;
+      reinterpret_cast<V *>(p)[c] = vz.v;
+    }
+    tail = nc * N;
+  }
 #pragma unroll 4
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t i = tail + i0; i < n; i += stride) {
     const T v = q ? b[i] - q[i] : b[i];  // q = null: A x = 0 (zero initial guess)
     const T z = inv ? inv[i] * v : v;
     r[i] = v;
@@ -234,7 +379,9 @@ __global__ void cg_check0_kernel(const T *part_rr, int nparts, SolveState *st, d
   }
 }
 
-// 1-block reduction of the cell kernel's fused p.(Ap) partials into st->pAp
+
+// 1-block reduction of the cell kernel's fused p.(Ap) partials into st->pAp (measured: cheaper
+// than every update block reducing the partials itself)
 __global__ void reduce_cell_dot_kernel(const double *part, int n, SolveState *st)
 {
   if (st->stop) return;
@@ -262,9 +409,32 @@ __global__ void cg_update_kernel(T *__restrict__ r, const T *__restrict__ q,
   const T alpha = T(double(rz) / pAp);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = double(alpha);
   Acc<T> arr, arz;
+  constexpr int N = Chunk<T>::N;
+  using V = typename Chunk<T>::V;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(r) && aligned16(q) && (!inv || aligned16(inv))) {
+    const int64_t nc = n / N;
+#pragma unroll 2
+    for (int64_t c = i0; c < nc; c += stride) {
+      ChunkU<T> vr, vq, vd;
+      vr.v = reinterpret_cast<const V *>(r)[c];
+      vq.v = reinterpret_cast<const V *>(q)[c];
+      if (inv) vd.v = reinterpret_cast<const V *>(inv)[c];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const T v = vr.a[k] - alpha * vq.a[k];
+        vr.a[k] = v;
+        arr.add(v * v);
+        arz.add(v * (inv ? vd.a[k] * v : v));
+      }
+      reinterpret_cast<V *>(r)[c] = vr.v;
+    }
+    tail = nc * N;
+  }
 #pragma unroll 4
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t i = tail + i0; i < n; i += stride) {
     const T v = r[i] - alpha * q[i];
     r[i] = v;
     arr.add(v * v);
@@ -291,7 +461,8 @@ __global__ void cg_direction_kernel(T *__restrict__ x, T *__restrict__ p, const 
     return;
   }
   const T alpha = T(st->alpha);
-  const T rr = reduce_partials(part_rr, nparts);
+  T rr, rz_new, rz_old;
+  reduce_partials3(part_rr, part_rz_new, part_rz_old, nparts, rr, rz_new, rz_old);
   const double res = sqrt(double(rr));
   const double tol = st->tol;
   const bool done = res <= tol;
@@ -309,12 +480,34 @@ __global__ void cg_direction_kernel(T *__restrict__ x, T *__restrict__ p, const 
     }
     return;
   }
-  const T rz_new = reduce_partials(part_rz_new, nparts);
-  const T rz_old = reduce_partials(part_rz_old, nparts);
   const T beta = T(double(rz_new) / double(rz_old));
+  constexpr int N = Chunk<T>::N;
+  using V = typename Chunk<T>::V;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(x) && aligned16(p) && aligned16(r) && (!inv || aligned16(inv))) {
+    const int64_t nc = n / N;
+#pragma unroll 2
+    for (int64_t c = i0; c < nc; c += stride) {
+      ChunkU<T> vx, vp, vr, vd;
+      vx.v = reinterpret_cast<const V *>(x)[c];
+      vp.v = reinterpret_cast<const V *>(p)[c];
+      vr.v = reinterpret_cast<const V *>(r)[c];
+      if (inv) vd.v = reinterpret_cast<const V *>(inv)[c];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        vx.a[k] += alpha * vp.a[k];
+        const T z = inv ? vd.a[k] * vr.a[k] : vr.a[k];
+        vp.a[k] = z + beta * vp.a[k];
+      }
+      reinterpret_cast<V *>(x)[c] = vx.v;
+      reinterpret_cast<V *>(p)[c] = vp.v;
+    }
+    tail = nc * N;
+  }
 #pragma unroll 4
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
-       i += int64_t(gridDim.x) * blockDim.x) {
+  for (int64_t i = tail + i0; i < n; i += stride) {
     const T pi = p[i];
     x[i] += alpha * pi;
     const T z = inv ? inv[i] * r[i] : r[i];
@@ -608,18 +801,15 @@ template int cg_init<double>(const double *, double *, double *, double *, const
 template int cg_init<float>(const float *, float *, float *, float *, const float *, int64_t,
                             float *, float *, SolveState *, double, double, hipStream_t);
 
-int cg_reduce_cell_dot(const double *part, int n, SolveState *sd, hipStream_t st)
-{
-  hipLaunchKernelGGL(reduce_cell_dot_kernel, dim3(1), dim3(1024), 0, st, part, n, sd);
-  MF_HIP_TRY(hipGetLastError());
-  return MF_OK;
-}
 
 template <typename T>
 int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *part_rr,
-            SolveState *sd, int it, int max_it, hipStream_t st)
+            const double *cell_part, int ncell_part, SolveState *sd, int it, int max_it,
+            hipStream_t st)
 {
   T *rz_new = part_rz[it & 1], *rz_old = part_rz[(it - 1) & 1];
+  hipLaunchKernelGGL(reduce_cell_dot_kernel, dim3(1), dim3(1024), 0, st, cell_part, ncell_part, sd);
+  MF_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(cg_update_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, r, q, inv,
                      n, rz_old, rz_new, part_rr, RED_BLOCKS, sd);
   MF_HIP_TRY(hipGetLastError());
@@ -629,9 +819,10 @@ int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *p
   return MF_OK;
 }
 template int cg_step<double>(double *, double *, double *, double *, const double *, int64_t,
-                             double *[2], double *, SolveState *, int, int, hipStream_t);
+                             double *[2], double *, const double *, int, SolveState *, int, int,
+                             hipStream_t);
 template int cg_step<float>(float *, float *, float *, float *, const float *, int64_t, float *[2],
-                            float *, SolveState *, int, int, hipStream_t);
+                            float *, const double *, int, SolveState *, int, int, hipStream_t);
 
 template <typename T>
 int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h,

@@ -36,10 +36,12 @@ template <typename T> int vec_scale_dev(T *dst, const T *src, int64_t n, const d
 template <typename T>
 int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,
             SolveState *sd, double abs_tol, double rel_tol, hipStream_t st);
-int cg_reduce_cell_dot(const double *part, int n, SolveState *sd, hipStream_t st);
+// one CG iteration after q = A p: reduction of the operator's p.(Ap) block partials
+// cell_part[0..ncell_part), update and direction kernels
 template <typename T>
 int cg_step(T *x, T *r, T *p, T *q, const T *inv, int64_t n, T *part_rz[2], T *part_rr,
-            SolveState *sd, int it, int max_it, hipStream_t st);
+            const double *cell_part, int ncell_part, SolveState *sd, int it, int max_it,
+            hipStream_t st);
 
 template <typename T>
 int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h, hipStream_t st);
