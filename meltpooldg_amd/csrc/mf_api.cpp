@@ -757,6 +757,9 @@ namespace mf {
 
 struct SolveOut {
   int first_chunk = 8;  // in: iterations enqueued before the first host check (CG)
+  // in (FP32 CG of the defect correction): right-hand side = float(b64_scale * b64), x = 0
+  const double *b64 = nullptr;
+  double b64_scale = 1.0;
   int iterations = 0;
   double res0 = 0, res = 0;
   int status = MF_OK;
@@ -788,7 +791,14 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   // r0 = b - A x (x known to be zero: r0 = b, no operator application)
   if (!x_zero && (rc = apply_tangent<T>(op, q, x, nullptr, nullptr, true))) return rc;
   // cg_check0 initialises every field of the state block
-  if ((rc = cg_init<T>(b, x_zero ? nullptr : q, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st)))
+  if constexpr (sizeof(T) == 4) {
+    if (out.b64) {
+      rc = cg_init_refine(out.b64, out.b64_scale, x, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st);
+      if (rc) return rc;
+    }
+  }
+  if (!(sizeof(T) == 4 && out.b64) &&
+      (rc = cg_init<T>(b, x_zero ? nullptr : q, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st)))
     return rc;
   T *prz[2] = {prz0, prz1};
   SolveState h{};
@@ -1030,9 +1040,15 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
         break;
       }
       ++outer;
-      if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
-      if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
       SolveOut o;
+      if (desc->solver == MF_SOLVER_CG) {
+        // the inner CG starts from float(r / res) and e = 0 in its first kernel
+        o.b64 = r.p;
+        o.b64_scale = 1.0 / res;
+      } else {
+        if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
+        if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
+      }
       // the passes reduce the residual by about the same factor: enqueue the previous pass's
       // iteration count before the first check
       o.first_chunk = inner_total > 0 ? last_inner + 1 : 8;

@@ -360,6 +360,55 @@ __global__ void cg_init_kernel(const T *__restrict__ b, const T *__restrict__ q,
   write_partial(part_rz0, gridDim.x, arz);
 }
 
+// Start of an FP32 inner CG of the defect correction from the FP64 residual: r = float(s b64),
+// z = D r, p = z, x = 0, partial r.r and r.z (the cast, the fill of x and cg_init in one pass)
+__global__ void cg_init_refine_kernel(const double *__restrict__ b64, double s,
+                                      float *__restrict__ x, float *__restrict__ r,
+                                      float *__restrict__ p, const float *__restrict__ inv,
+                                      int64_t n, float *part_rr, float *part_rz0)
+{
+  Acc<float> arr, arz;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  int64_t tail = 0;
+  if (aligned16(b64) && aligned16(x) && aligned16(r) && aligned16(p) && (!inv || aligned16(inv))) {
+    const int64_t nc = n / 4;
+    for (int64_t c = i0; c < nc; c += stride) {
+      const double2 u = reinterpret_cast<const double2 *>(b64)[2 * c];
+      const double2 w = reinterpret_cast<const double2 *>(b64)[2 * c + 1];
+      ChunkU<float> vr, vz, vd;
+      vr.a[0] = float(u.x * s);
+      vr.a[1] = float(u.y * s);
+      vr.a[2] = float(w.x * s);
+      vr.a[3] = float(w.y * s);
+      if (inv) vd.v = reinterpret_cast<const float4 *>(inv)[c];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float v = vr.a[k];
+        const float z = inv ? vd.a[k] * v : v;
+        vz.a[k] = z;
+        arr.add(v * v);
+        arz.add(v * z);
+      }
+      reinterpret_cast<float4 *>(r)[c] = vr.v;
+      reinterpret_cast<float4 *>(p)[c] = vz.v;
+      reinterpret_cast<float4 *>(x)[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    tail = nc * 4;
+  }
+  for (int64_t i = tail + i0; i < n; i += stride) {
+    const float v = float(b64[i] * s);
+    const float z = inv ? inv[i] * v : v;
+    r[i] = v;
+    p[i] = z;
+    x[i] = 0.f;
+    arr.add(v * v);
+    arz.add(v * z);
+  }
+  write_partial(part_rr, gridDim.x, arr);
+  write_partial(part_rz0, gridDim.x, arz);
+}
+
 // step-0 check: tol = max(abs, rel*|r0|), success if |r0| <= tol
 template <typename T>
 __global__ void cg_check0_kernel(const T *part_rr, int nparts, SolveState *st, double abs_tol,
@@ -798,6 +847,19 @@ int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T
 }
 template int cg_init<double>(const double *, double *, double *, double *, const double *, int64_t,
                              double *, double *, SolveState *, double, double, hipStream_t);
+
+int cg_init_refine(const double *b64, double s, float *x, float *r, float *p, const float *inv,
+                   int64_t n, float *part_rr, float *part_rz0, SolveState *sd, double abs_tol,
+                   double rel_tol, hipStream_t st)
+{
+  hipLaunchKernelGGL(cg_init_refine_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, b64, s, x,
+                     r, p, inv, n, part_rr, part_rz0);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(cg_check0_kernel<float>, dim3(1), dim3(VEC_THREADS), 0, st, part_rr,
+                     RED_BLOCKS, sd, abs_tol, rel_tol);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
 template int cg_init<float>(const float *, float *, float *, float *, const float *, int64_t,
                             float *, float *, SolveState *, double, double, hipStream_t);
 

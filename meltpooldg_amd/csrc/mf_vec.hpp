@@ -36,6 +36,10 @@ template <typename T> int vec_scale_dev(T *dst, const T *src, int64_t n, const d
 template <typename T>
 int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,
             SolveState *sd, double abs_tol, double rel_tol, hipStream_t st);
+// FP32 inner CG of the defect correction started from the FP64 residual b64 scaled by s, x = 0
+int cg_init_refine(const double *b64, double s, float *x, float *r, float *p, const float *inv,
+                   int64_t n, float *part_rr, float *part_rz0, SolveState *sd, double abs_tol,
+                   double rel_tol, hipStream_t st);
 // one CG iteration after q = A p: reduction of the operator's p.(Ap) block partials
 // cell_part[0..ncell_part), update and direction kernels
 template <typename T>
