@@ -257,8 +257,18 @@ def adaptive_case(mf, torch, stream):
     sqrt(1e-10) = 1e-5)."""
     t0 = time.perf_counter()
     cells = mf.refine_band_cells(32, 2, (0.5, 0.5, 0.5), 0.3, 0.05)
-    s = mf.AdaptiveScratchData(3, 32, cells, stream=stream)
-    setup_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    s = mf.AdaptiveScratchData(3, 32, cells, stream=stream)  # node identification on the device
+    t2 = time.perf_counter()
+    setup_s = t2 - t0
+    try:  # the same scratch with every setup step on the host, for comparison
+        mf.set_adaptive_setup(0)
+        t3 = time.perf_counter()
+        s_host = mf.AdaptiveScratchData(3, 32, cells, stream=stream)
+        scratch_host_s = time.perf_counter() - t3
+        del s_host
+    finally:
+        mf.set_adaptive_setup(1)
     X = s.support_points()
     r2 = ((X - np.array([0.5, 0.5, 1.0])) ** 2).sum(axis=1)
     rng = np.random.default_rng(2)
@@ -284,14 +294,17 @@ def adaptive_case(mf, torch, stream):
                        "hanging nodes)",
            "n_cells": int(len(cells)), "n_dofs": int(s.n_dofs),
            "n_hanging": info["n_hanging"], "n_colours": info["n_colours"],
-           "setup_s": setup_s, "vmult_us": t_vm * 1e6, "vmult_dofs_per_s": s.n_dofs / t_vm,
+           "setup_s": setup_s, "refine_s": t1 - t0, "scratch_setup_s": t2 - t1,
+           "scratch_setup_host_s": scratch_host_s,
+           "vmult_us": t_vm * 1e6, "vmult_dofs_per_s": s.n_dofs / t_vm,
            "jacobi_update_s": t_pc,
            "cg_fp64_s": t64, "cg_fp64_iterations": r64["iterations"],
            "cg_fp32_refined_s": t32, "cg_fp32_refined_inner_iterations": r32["iterations"],
            "cg_fp32_refined_outer": r32["outer_iterations"],
            "cg_fp32_refined_rel_residual": r32["final_residual"] / r32["initial_residual"],
            "refined_vs_fp64_rel_error": err, "speedup_refined_over_fp64": t64 / t32}
-    print(f"# adaptive: {len(cells)} cells, {s.n_dofs} DoFs, vmult {t_vm*1e6:.0f} us, CG fp64 "
+    print(f"# adaptive: {len(cells)} cells, {s.n_dofs} DoFs, setup: refine {t1-t0:.2f} s, scratch "
+          f"{t2-t1:.2f} s (host path {scratch_host_s:.2f} s); vmult {t_vm*1e6:.0f} us, CG fp64 "
           f"{t64*1e3:.1f} ms ({r64['iterations']} it), refined {t32*1e3:.1f} ms "
           f"({r32['outer_iterations']}/{r32['iterations']}), err {err:.2e}", file=sys.stderr,
           flush=True)

@@ -82,6 +82,10 @@ int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq
 int mf_mesh_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
                         const double center[3], double radius, double half_width, int32_t *cells,
                         int64_t capacity, int64_t *n_cells);
+/* Where the adaptive-mesh setup runs: 1 (default) the node identification (DoF numbering,
+ * support points, boundary flags) on the device when one is present, with the numbering of the
+ * host path; 0 everything on the host.  No reference counterpart (deal.II numbers on the host). */
+int mf_set_adaptive_setup(int device);
 /* Scratch on an adaptive mesh (cells as above, 2:1 balanced): DoF numbering in order of first
  * appearance over the cells, hanging-node constraints with chains resolved, and the default
  * Dirichlet set = boundary nodes that are not hanging (HNC merged into DBC with the hanging line

@@ -206,6 +206,7 @@ def lib():
         L.mf_solve_preconditioned.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, vp,
                                               C.POINTER(SolveReport)]
         L.mf_set_tangent_kernel.argtypes = [C.c_int]
+        L.mf_set_adaptive_setup.argtypes = [C.c_int]
         L.mf_heat_newton_step.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolverDesc),
                                           C.POINTER(NewtonDesc), C.POINTER(NewtonReport)]
         L.mf_advection_step.argtypes = [vp, vp, vp, vp, C.POINTER(SolverDesc),
@@ -236,6 +237,11 @@ def splitmix64(x):
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
         return z ^ (z >> np.uint64(31))
+
+
+def set_adaptive_setup(device):
+    """1: node identification of adaptive meshes on the device when present (default), 0: host."""
+    _check(lib().mf_set_adaptive_setup(int(bool(device))))
 
 
 def set_tangent_kernel(variant):

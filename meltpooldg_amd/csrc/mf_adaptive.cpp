@@ -163,6 +163,7 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
     }
   }
   // --- node identification ---------------------------------------------------------------
+  // on the device when one is present (mf_adaptive_dev.hip, the same numbering), else here
   auto axis_key = [&](int l, int64_t ix, int i) -> uint64_t {
     const int64_t g = ix * p + i;
     const bool dyadic = (i == 0 || i == p || (p == 2 && i == 1));
@@ -170,12 +171,17 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
     return (1ull << 62) | (uint64_t(l) << 48) | uint64_t(g);
   };
   std::vector<uint64_t> nkeys;  // 3 per node
+  int64_t nn = 0;
+  bool on_device = false;
+  int n_devices = 0;
+  if (g_adaptive_setup_device && hipGetDeviceCount(&n_devices) == hipSuccess && n_devices > 0)
+    on_device = device_node_identification(s, cells, n_cells, L, A, nkeys, nn) == MF_OK;
+  if (!on_device) {
   nkeys.reserve(size_t(n_cells) * (p == 1 ? 3 : 3 * p * p * p) + 64);
   size_t cap = 1;
   while (cap < size_t(2) * size_t(n_cells) * nloc / std::max(1, p) + 64) cap <<= 1;
   std::vector<int32_t> table(cap, -1);
   A->cell_nodes.resize(size_t(n_cells) * nloc);
-  int64_t nn = 0;
   for (int64_t c = 0; c < n_cells; ++c) {
     const int32_t *cl = cells + 4 * c;
     uint64_t ak[3][KMAX];
@@ -220,8 +226,6 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   }
   table.clear();
   table.shrink_to_fit();
-  m.n_dofs = nn;
-  m.n_cells = n_cells;
   // support points and boundary flags
   A->xyz.resize(size_t(nn) * 3);
   A->boundary.assign(size_t(nn), 0);
@@ -239,6 +243,9 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
       }
     }
   }
+  }  // host node identification
+  m.n_dofs = nn;
+  m.n_cells = n_cells;
   // --- hanging-node constraints ----------------------------------------------------------
   std::unordered_map<uint64_t, int64_t> leaf_index;
   leaf_index.reserve(size_t(n_cells) * 2);

@@ -1,0 +1,247 @@
+// Node identification of an adaptive octree mesh on the device.
+//
+// Reference function replaced: FiniteElementUtils::distribute_dofs
+// (include/meltpooldg/utilities/fe_util.hpp:21-81) for the adaptive meshes of the benchmark (the
+// host path in mf_adaptive.cpp restates it; this is its device form, used when a GPU is present).
+//
+// Every (cell, local node) slot s = c K^3 + e gets the three axis keys of its node (the same keys
+// as the host path: dyadic nodes by their coordinate in half-finest-cell units, interior
+// Gauss-Lobatto nodes of Q3 by (level, lattice index)).  The slots are sorted by a 64-bit hash
+// of the keys (stable radix sort, so equal hashes keep increasing slot order); runs of equal
+// hashes are one node, whose full keys are compared to rule out collisions.  A node is numbered
+// by the rank of its first slot in slot order (an exclusive prefix sum over first-slot flags),
+// which is the host's first-appearance numbering.  Support points come from the node's last
+// slot (the host overwrites them cell by cell) and the boundary flag is set by any slot.
+#include <hipcub/hipcub.hpp>
+
+#include <vector>
+
+#include "mf_internal.hpp"
+
+namespace mf {
+
+namespace {
+
+__device__ __forceinline__ uint64_t mix64_d(uint64_t x)
+{
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+struct NodeArgs {
+  int p, K, L;
+  int64_t n_cells;
+  int ncell[3];
+  double lo[3], h[3], gl[KMAX];
+};
+
+__device__ __forceinline__ uint64_t axis_key_d(int p, int L, int l, int64_t ix, int i)
+{
+  const int64_t g = ix * p + i;
+  const bool dyadic = (i == 0 || i == p || (p == 2 && i == 1));
+  if (dyadic) return uint64_t((g << (L - l + 1)) / p);
+  return (1ull << 62) | (uint64_t(l) << 48) | uint64_t(g);
+}
+
+__global__ void slot_keys_kernel(const NodeArgs a, const int32_t *__restrict__ cells,
+                                 uint64_t *__restrict__ keys, uint64_t *__restrict__ hkey,
+                                 uint32_t *__restrict__ sval, int64_t n_slots)
+{
+  const int nloc = a.K * a.K * a.K;
+  for (int64_t s = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; s < n_slots;
+       s += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t c = s / nloc;
+    const int e = int(s - c * nloc);
+    const int32_t *cl = cells + 4 * c;
+    const int ai[3] = {e % a.K, (e / a.K) % a.K, e / (a.K * a.K)};
+    uint64_t k[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) k[d] = axis_key_d(a.p, a.L, cl[0], cl[1 + d], ai[d]);
+    keys[3 * s] = k[0];
+    keys[3 * s + 1] = k[1];
+    keys[3 * s + 2] = k[2];
+    hkey[s] = mix64_d(k[0] ^ mix64_d(k[1] ^ mix64_d(k[2])));
+    sval[s] = uint32_t(s);
+  }
+}
+
+// head flags of the sorted runs (position of the run head, 0 elsewhere, for a max-scan), the
+// first slot of every run, and a collision check of the full keys inside each run
+__global__ void run_heads_kernel(const uint64_t *__restrict__ hs, const uint32_t *__restrict__ ss,
+                                 const uint64_t *__restrict__ keys, int32_t *__restrict__ headpos,
+                                 int32_t *__restrict__ first, int *collision, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const bool head = (i == 0) || hs[i] != hs[i - 1];
+    headpos[i] = head ? int32_t(i) : 0;
+    if (head) {
+      first[ss[i]] = 1;
+    } else {
+      const uint64_t *ka = keys + 3 * size_t(ss[i]), *kb = keys + 3 * size_t(ss[i - 1]);
+      if (ka[0] != kb[0] || ka[1] != kb[1] || ka[2] != kb[2]) atomicOr(collision, 1);
+    }
+  }
+}
+
+__global__ void assign_kernel(const NodeArgs a, const int32_t *__restrict__ cells,
+                              const uint32_t *__restrict__ ss, const int32_t *__restrict__ grp,
+                              const int32_t *__restrict__ nid, const uint64_t *__restrict__ keys,
+                              uint32_t *__restrict__ cell_nodes, uint64_t *__restrict__ node_keys,
+                              double *__restrict__ xyz, uint8_t *__restrict__ boundary, int64_t n)
+{
+  const int nloc = a.K * a.K * a.K;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t g = grp[i];
+    const uint32_t s = ss[i];
+    const uint32_t id = uint32_t(nid[ss[g]]);
+    cell_nodes[s] = id;
+    const int64_t c = int64_t(s) / nloc;
+    const int e = int(int64_t(s) - c * nloc);
+    const int32_t *cl = cells + 4 * c;
+    const int ai[3] = {e % a.K, (e / a.K) % a.K, e / (a.K * a.K)};
+    bool bnd = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int64_t gg = int64_t(cl[1 + d]) * a.p + ai[d];
+      bnd |= (ai[d] == 0 && gg == 0) || (ai[d] == a.p && gg == (int64_t(a.ncell[d]) << cl[0]) * a.p);
+    }
+    if (bnd) boundary[id] = 1;
+    if (i == g) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) node_keys[3 * size_t(id) + d] = keys[3 * size_t(s) + d];
+    }
+    // the last slot of the run (largest slot: the host's last writer) sets the support point
+    const bool tail = (i + 1 == n) || grp[i + 1] != g;
+    if (tail) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double h = a.h[d] / double(int64_t(1) << cl[0]);
+        xyz[3 * size_t(id) + d] = a.lo[d] + h * (double(cl[1 + d]) + a.gl[ai[d]]);
+      }
+    }
+  }
+}
+
+struct Max32 {
+  __device__ __forceinline__ int32_t operator()(int32_t x, int32_t y) const { return x > y ? x : y; }
+};
+
+// device buffers released on every return path
+struct DevBufs {
+  std::vector<void *> p;
+  template <typename T>
+  hipError_t alloc(T **ptr, size_t count)
+  {
+    void *q = nullptr;
+    const hipError_t e = hipMalloc(&q, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) p.push_back(q);
+    *ptr = static_cast<T *>(q);
+    return e;
+  }
+  ~DevBufs()
+  {
+    for (void *q : p) (void)hipFree(q);
+  }
+};
+
+}  // namespace
+
+int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_t n_cells, int L,
+                               AdaptiveData *A, std::vector<uint64_t> &nkeys, int64_t &nn)
+{
+  const MeshInfo &m = s->mesh;
+  const int p = m.degree, K = p + 1, nloc = K * K * K;
+  const int64_t n_slots = n_cells * nloc;
+  MF_REQUIRE(n_slots < (int64_t(1) << 31), MF_ERR_ARG, "adaptive mesh too large for 32-bit slots");
+  hipStream_t st = s->stream;
+  NodeArgs a{};
+  a.p = p;
+  a.K = K;
+  a.L = L;
+  a.n_cells = n_cells;
+  for (int d = 0; d < 3; ++d) {
+    a.ncell[d] = m.ncell[d];
+    a.lo[d] = m.lo[d];
+    a.h[d] = m.h[d];
+  }
+  for (int i = 0; i < K; ++i) a.gl[i] = s->tab.gl[i];
+  DevBufs b;
+  int32_t *d_cells, *headpos, *grp, *first, *nid;
+  uint64_t *keys, *hkey, *hkey_s;
+  uint32_t *sval, *sval_s, *cnodes;
+  int *collision;
+  MF_HIP_TRY(b.alloc(&d_cells, size_t(4 * n_cells)));
+  MF_HIP_TRY(b.alloc(&keys, size_t(3 * n_slots)));
+  MF_HIP_TRY(b.alloc(&hkey, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&hkey_s, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&sval, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&sval_s, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&headpos, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&grp, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&first, size_t(n_slots) + 1));
+  MF_HIP_TRY(b.alloc(&nid, size_t(n_slots) + 1));
+  MF_HIP_TRY(b.alloc(&cnodes, size_t(n_slots)));
+  MF_HIP_TRY(b.alloc(&collision, 1));
+  MF_HIP_TRY(hipMemcpyAsync(d_cells, cells, sizeof(int32_t) * 4 * size_t(n_cells),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemsetAsync(first, 0, sizeof(int32_t) * (size_t(n_slots) + 1), st));
+  MF_HIP_TRY(hipMemsetAsync(collision, 0, sizeof(int), st));
+  const dim3 grid(unsigned(std::min<int64_t>((n_slots + 255) / 256, 65536))), block(256);
+  hipLaunchKernelGGL(slot_keys_kernel, grid, block, 0, st, a, d_cells, keys, hkey, sval, n_slots);
+  MF_HIP_TRY(hipGetLastError());
+  // stable sort of the slots by hash
+  size_t tmp_bytes = 0, tmp2 = 0, tmp3 = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, hkey, hkey_s, sval, sval_s,
+                                                int(n_slots), 0, 64, st));
+  MF_HIP_TRY(hipcub::DeviceScan::InclusiveScan(nullptr, tmp2, headpos, grp, Max32(), int(n_slots), st));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp3, first, nid, int(n_slots) + 1, st));
+  tmp_bytes = std::max(tmp_bytes, std::max(tmp2, tmp3));
+  void *tmp;
+  MF_HIP_TRY(b.alloc(reinterpret_cast<char **>(&tmp), tmp_bytes));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, hkey, hkey_s, sval, sval_s,
+                                                int(n_slots), 0, 64, st));
+  hipLaunchKernelGGL(run_heads_kernel, grid, block, 0, st, hkey_s, sval_s, keys, headpos, first,
+                     collision, n_slots);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipcub::DeviceScan::InclusiveScan(tmp, tmp_bytes, headpos, grp, Max32(), int(n_slots), st));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, first, nid, int(n_slots) + 1, st));
+  int32_t total = 0;
+  int coll = 0;
+  MF_HIP_TRY(hipMemcpyAsync(&total, nid + n_slots, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(&coll, collision, sizeof(int), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  if (coll) {
+    set_error("adaptive mesh: 64-bit key hash collision in the device node identification");
+    return MF_ERR_STATE;  // the caller falls back to the host path
+  }
+  nn = total;
+  uint64_t *node_keys;
+  double *xyz;
+  uint8_t *bnd;
+  MF_HIP_TRY(b.alloc(&node_keys, size_t(3 * nn)));
+  MF_HIP_TRY(b.alloc(&xyz, size_t(3 * nn)));
+  MF_HIP_TRY(b.alloc(&bnd, size_t(nn)));
+  MF_HIP_TRY(hipMemsetAsync(bnd, 0, size_t(nn), st));
+  hipLaunchKernelGGL(assign_kernel, grid, block, 0, st, a, d_cells, sval_s, grp, nid, keys, cnodes,
+                     node_keys, xyz, bnd, n_slots);
+  MF_HIP_TRY(hipGetLastError());
+  A->cell_nodes.resize(size_t(n_slots));
+  nkeys.resize(size_t(3 * nn));
+  A->xyz.resize(size_t(3 * nn));
+  A->boundary.resize(size_t(nn));
+  MF_HIP_TRY(hipMemcpyAsync(A->cell_nodes.data(), cnodes, sizeof(uint32_t) * size_t(n_slots),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(nkeys.data(), node_keys, sizeof(uint64_t) * size_t(3 * nn),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(A->xyz.data(), xyz, sizeof(double) * size_t(3 * nn),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(A->boundary.data(), bnd, size_t(nn), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
+}  // namespace mf

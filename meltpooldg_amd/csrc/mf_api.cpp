@@ -354,6 +354,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
 }
 
 int g_tangent_variant = 0;
+int g_adaptive_setup_device = 1;
 
 }  // namespace mf
 
@@ -367,6 +368,12 @@ extern "C" {
 const char *mf_last_error(void) { return g_last_error.c_str(); }
 
 int mf_version(void) { return 100; }
+
+int mf_set_adaptive_setup(int device)
+{
+  g_adaptive_setup_device = device ? 1 : 0;
+  return MF_OK;
+}
 
 int mf_set_tangent_kernel(int variant)
 {

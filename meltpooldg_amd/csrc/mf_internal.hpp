@@ -183,6 +183,12 @@ struct AdaptiveData {
 };
 
 int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
+// node identification of an adaptive mesh on the device (mf_adaptive_dev.hip): cell_nodes,
+// xyz, boundary and the 3 axis keys per node, numbered as the host path numbers them
+int device_node_identification(const ::mf_scratch *s, const int32_t *cells, int64_t n_cells,
+                               int L, AdaptiveData *A, std::vector<uint64_t> &nkeys, int64_t &nn);
+// 1: adaptive setup steps run on the device when one is present (default), 0: on the host
+extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);
 int ensure_adaptive_on_device(::mf_scratch *s);
 // y[m] += sum_h w_hm y[h] over the hanging nodes h with master m, then y[h] = 0 (the transpose

@@ -1,0 +1,44 @@
+"""Device node identification of adaptive meshes (mf_adaptive_dev.hip) against the host path:
+the same DoF numbering, support points, boundary flags and, built from them, the same
+hanging-node constraints, Dirichlet rows and operator (bit for bit)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "q1": (1, 2, 2, (0.1, 0.1, 0.1), 0.2, 0.03, (1.0, 1.0, 1.0)),
+    "q2": (2, 3, 2, (0.2, 0.3, 0.25), 0.3, 0.04, (1.0, 1.3, 0.8)),
+    "q3": (3, 2, 2, (0.1, 0.1, 0.1), 0.2, 0.03, (1.0, 1.0, 1.0)),
+    "q3b": (3, 4, 2, (0.5, 0.5, 0.5), 0.3, 0.05, (1.0, 1.0, 1.0)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_device_setup_matches_host(name):
+    import meltpooldg_amd as mf
+    degree, nc, lev, ce, r, hw, hi = CASES[name]
+    cells = mf.refine_band_cells(nc, lev, ce, r, hw, hi=hi)
+    st = torch.cuda.current_stream()
+    try:
+        mf.set_adaptive_setup(0)
+        sh = mf.AdaptiveScratchData(degree, nc, cells, hi=hi, stream=st)
+    finally:
+        mf.set_adaptive_setup(1)
+    sd = mf.AdaptiveScratchData(degree, nc, cells, hi=hi, stream=st)
+    assert sd.n_dofs == sh.n_dofs
+    assert np.array_equal(sd.support_points(), sh.support_points())
+    assert np.array_equal(sd.default_dirichlet_mask(), sh.default_dirichlet_mask())
+    assert sd.constraint_info() == sh.constraint_info()
+    for a, b in zip(sd.constraint_lines(), sh.constraint_lines()):
+        assert np.array_equal(a, b)
+    x = torch.tensor(np.random.default_rng(3).uniform(-1, 1, sd.n_dofs), dtype=torch.float64,
+                     device="cuda")
+    ys = []
+    for s in (sd, sh):
+        op = mf.HeatDiffuseOperator(s, rho_cp=1.3, k=0.7, dt=1e-2)
+        y = torch.zeros_like(x)
+        op.vmult(y, x)
+        ys.append(y)
+    assert torch.equal(ys[0], ys[1])
