@@ -401,7 +401,13 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   A->hang_nodes.clear();
   for (int64_t i = 0; i < nn; ++i)
     if (line_of[i] >= 0) A->hang_nodes.push_back(uint32_t(i));
-  {
+  bool transposed = false;
+  if (on_device) {
+    std::vector<uint32_t> hang_of_line(static_cast<size_t>(A->n_hanging));
+    for (uint32_t h : A->hang_nodes) hang_of_line[size_t(line_of[h])] = h;
+    transposed = device_transpose_constraints(A, hang_of_line, s->stream) == MF_OK;
+  }
+  if (!transposed) {
     std::vector<std::pair<uint32_t, std::pair<uint32_t, double>>> tr;  // master, (hanging, w)
     tr.reserve(A->line_col.size());
     for (uint32_t h : A->hang_nodes) {
@@ -446,10 +452,11 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   A->cell_hidx.assign(size_t(n_cells), -1);
   A->ht_ptr.assign(1, 0u);
   A->tgt_ptr.assign(1, 0u);
+  std::vector<std::pair<uint32_t, std::pair<int, double>>> ent;  // node, (local, weight)
   for (int64_t c = 0; c < n_cells; ++c) {
     if (!A->cell_hmask[c]) continue;
     A->cell_hidx[c] = int32_t(A->ht_ptr.size() - 1);
-    std::vector<std::pair<uint32_t, std::pair<int, double>>> ent;  // node, (local, weight)
+    ent.clear();
     for (int e = 0; e < nloc; ++e) {
       const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
       if (line_of[id] >= 0) {

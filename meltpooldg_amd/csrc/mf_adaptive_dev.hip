@@ -244,4 +244,104 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
   return MF_OK;
 }
 
+// Transposed hanging-node constraints: for every master, the hanging nodes it receives from and
+// their weights, masters ascending and, per master, hanging nodes ascending (the order the host
+// path builds).  Entries are keyed by (master << 32 | hanging node) and radix-sorted.
+namespace {
+__global__ void transpose_keys_kernel(const uint32_t *__restrict__ line_ptr,
+                                      const uint32_t *__restrict__ line_col,
+                                      const uint32_t *__restrict__ hang_of_line, int64_t n_lines,
+                                      uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+  for (int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; l < n_lines;
+       l += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t h = hang_of_line[l];
+    for (uint32_t t = line_ptr[l]; t < line_ptr[l + 1]; ++t) {
+      key[t] = (uint64_t(line_col[t]) << 32) | h;
+      val[t] = t;
+    }
+  }
+}
+
+__global__ void transpose_gather_kernel(const uint64_t *__restrict__ key_s,
+                                        const uint32_t *__restrict__ val_s,
+                                        const double *__restrict__ line_val,
+                                        uint32_t *__restrict__ master, uint32_t *__restrict__ hang,
+                                        double *__restrict__ w, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    master[i] = uint32_t(key_s[i] >> 32);
+    hang[i] = uint32_t(key_s[i] & 0xffffffffull);
+    w[i] = line_val[val_s[i]];
+  }
+}
+}  // namespace
+
+int device_transpose_constraints(AdaptiveData *A, const std::vector<uint32_t> &hang_of_line,
+                                 hipStream_t st)
+{
+  const int64_t n_lines = int64_t(hang_of_line.size());
+  const int64_t E = int64_t(A->line_col.size());
+  A->cm_node.clear();
+  A->cm_ptr.assign(1, 0u);
+  A->cm_h.clear();
+  A->cm_w.clear();
+  if (E == 0) return MF_OK;
+  MF_REQUIRE(E < (int64_t(1) << 31), MF_ERR_ARG, "too many constraint entries");
+  DevBufs b;
+  uint32_t *ptr, *col, *hol, *val, *val_s, *master, *hang;
+  uint64_t *key, *key_s;
+  double *lv, *w;
+  MF_HIP_TRY(b.alloc(&ptr, size_t(n_lines + 1)));
+  MF_HIP_TRY(b.alloc(&col, size_t(E)));
+  MF_HIP_TRY(b.alloc(&hol, size_t(n_lines)));
+  MF_HIP_TRY(b.alloc(&lv, size_t(E)));
+  MF_HIP_TRY(b.alloc(&key, size_t(E)));
+  MF_HIP_TRY(b.alloc(&key_s, size_t(E)));
+  MF_HIP_TRY(b.alloc(&val, size_t(E)));
+  MF_HIP_TRY(b.alloc(&val_s, size_t(E)));
+  MF_HIP_TRY(b.alloc(&master, size_t(E)));
+  MF_HIP_TRY(b.alloc(&hang, size_t(E)));
+  MF_HIP_TRY(b.alloc(&w, size_t(E)));
+  MF_HIP_TRY(hipMemcpyAsync(ptr, A->line_ptr.data(), sizeof(uint32_t) * size_t(n_lines + 1),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemcpyAsync(col, A->line_col.data(), sizeof(uint32_t) * size_t(E),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemcpyAsync(hol, hang_of_line.data(), sizeof(uint32_t) * size_t(n_lines),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemcpyAsync(lv, A->line_val.data(), sizeof(double) * size_t(E),
+                            hipMemcpyHostToDevice, st));
+  const dim3 block(256);
+  hipLaunchKernelGGL(transpose_keys_kernel, dim3(unsigned(std::min<int64_t>((n_lines + 255) / 256, 65536))),
+                     block, 0, st, ptr, col, hol, n_lines, key, val);
+  MF_HIP_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key_s, val, val_s, int(E),
+                                                0, 64, st));
+  char *tmp;
+  MF_HIP_TRY(b.alloc(&tmp, tmp_bytes));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_s, val, val_s, int(E), 0,
+                                                64, st));
+  hipLaunchKernelGGL(transpose_gather_kernel, dim3(unsigned(std::min<int64_t>((E + 255) / 256, 65536))),
+                     block, 0, st, key_s, val_s, lv, master, hang, w, E);
+  MF_HIP_TRY(hipGetLastError());
+  std::vector<uint32_t> hm(static_cast<size_t>(E));
+  A->cm_h.resize(size_t(E));
+  A->cm_w.resize(size_t(E));
+  MF_HIP_TRY(hipMemcpyAsync(hm.data(), master, sizeof(uint32_t) * size_t(E), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(A->cm_h.data(), hang, sizeof(uint32_t) * size_t(E),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(A->cm_w.data(), w, sizeof(double) * size_t(E), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  for (int64_t i = 0; i < E; ++i) {
+    if (i == 0 || hm[size_t(i)] != hm[size_t(i - 1)]) {
+      if (i > 0) A->cm_ptr.push_back(uint32_t(i));
+      A->cm_node.push_back(hm[size_t(i)]);
+    }
+  }
+  A->cm_ptr.push_back(uint32_t(E));
+  return MF_OK;
+}
+
 }  // namespace mf

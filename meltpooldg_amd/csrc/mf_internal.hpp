@@ -187,6 +187,10 @@ int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
 // xyz, boundary and the 3 axis keys per node, numbered as the host path numbers them
 int device_node_identification(const ::mf_scratch *s, const int32_t *cells, int64_t n_cells,
                                int L, AdaptiveData *A, std::vector<uint64_t> &nkeys, int64_t &nn);
+// transposed hanging-node constraints (cm_node, cm_ptr, cm_h, cm_w) on the device
+// (mf_adaptive_dev.hip); hang_of_line[l] = the hanging node of constraint line l
+int device_transpose_constraints(AdaptiveData *A, const std::vector<uint32_t> &hang_of_line,
+                                 hipStream_t st);
 // 1: adaptive setup steps run on the device when one is present (default), 0: on the host
 extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);
