@@ -246,12 +246,20 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   }  // host node identification
   m.n_dofs = nn;
   m.n_cells = n_cells;
-  // --- hanging-node constraints ----------------------------------------------------------
+  // --- hanging-node constraints ---------------------------------------------------------
+  // lines before chain resolution as CSR, on the device when node identification ran there
+  std::vector<int32_t> line_of;
+  std::vector<uint32_t> lptr, lcol;
+  std::vector<double> lval;
+  const bool hnc_on_device =
+      on_device && device_hanging_constraints(s, cells, n_cells, L, A->cell_nodes, nn, line_of, lptr,
+                                              lcol, lval) == MF_OK;
+  if (!hnc_on_device) {
   std::unordered_map<uint64_t, int64_t> leaf_index;
   leaf_index.reserve(size_t(n_cells) * 2);
   for (int64_t c = 0; c < n_cells; ++c)
     leaf_index[leaf_key(cells[4 * c], cells[4 * c + 1], cells[4 * c + 2], cells[4 * c + 3])] = c;
-  std::vector<int32_t> line_of(size_t(nn), -1);
+  line_of.assign(size_t(nn), -1);
   std::vector<std::vector<std::pair<uint32_t, double>>> lines;
   for (int64_t c = 0; c < n_cells; ++c) {
     const int32_t *cl = cells + 4 * c;
@@ -311,33 +319,74 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
           }
         }
   }
-  // chains: substitute constrained masters until every master is free
+  lptr.assign(1, 0u);
+  for (const auto &ln : lines) {
+    for (const auto &mw : ln) {
+      lcol.push_back(mw.first);
+      lval.push_back(mw.second);
+    }
+    lptr.push_back(uint32_t(lcol.size()));
+  }
+  }  // host hanging-node constraints
+  // chains: substitute constrained masters until every master is free.  Lines are rewritten in
+  // place in line order, a rewritten line kept in (rcol, rval) and used by the later lines
+  const int64_t n_lines = int64_t(lptr.size()) - 1;
+  std::vector<int32_t> ov(size_t(n_lines), -1);
+  std::vector<std::vector<uint32_t>> rcol;
+  std::vector<std::vector<double>> rval;
+  auto span = [&](int64_t l, const uint32_t *&c, const double *&v) -> size_t {
+    if (ov[size_t(l)] >= 0) {
+      c = rcol[size_t(ov[size_t(l)])].data();
+      v = rval[size_t(ov[size_t(l)])].data();
+      return rcol[size_t(ov[size_t(l)])].size();
+    }
+    c = lcol.data() + lptr[size_t(l)];
+    v = lval.data() + lptr[size_t(l)];
+    return lptr[size_t(l) + 1] - lptr[size_t(l)];
+  };
+  std::vector<std::pair<uint32_t, double>> out;
   for (int pass = 0; pass < 32; ++pass) {
     bool changed = false;
-    for (auto &ln : lines) {
-      std::vector<std::pair<uint32_t, double>> out;
+    for (int64_t l = 0; l < n_lines; ++l) {
+      const uint32_t *c;
+      const double *v;
+      const size_t n = span(l, c, v);
       bool any = false;
-      for (const auto &mw : ln) {
-        if (line_of[mw.first] >= 0) {
-          any = true;
-          for (const auto &m2 : lines[line_of[mw.first]]) out.emplace_back(m2.first, mw.second * m2.second);
-        } else {
-          out.push_back(mw);
-        }
-      }
+      for (size_t i = 0; i < n && !any; ++i) any = line_of[c[i]] >= 0;
       if (!any) continue;
       changed = true;
+      out.clear();
+      for (size_t i = 0; i < n; ++i) {
+        if (line_of[c[i]] >= 0) {
+          const uint32_t *c2;
+          const double *v2;
+          const size_t n2 = span(line_of[c[i]], c2, v2);
+          for (size_t j = 0; j < n2; ++j) out.emplace_back(c2[j], v[i] * v2[j]);
+        } else {
+          out.emplace_back(c[i], v[i]);
+        }
+      }
       std::sort(out.begin(), out.end(),
                 [](const std::pair<uint32_t, double> &a, const std::pair<uint32_t, double> &b) {
                   return a.first < b.first;
                 });
-      ln.clear();
+      std::vector<uint32_t> mc;
+      std::vector<double> mv;
       for (const auto &mw : out) {
-        if (!ln.empty() && ln.back().first == mw.first)
-          ln.back().second += mw.second;
-        else
-          ln.push_back(mw);
+        if (!mc.empty() && mc.back() == mw.first)
+          mv.back() += mw.second;
+        else {
+          mc.push_back(mw.first);
+          mv.push_back(mw.second);
+        }
       }
+      if (ov[size_t(l)] < 0) {
+        ov[size_t(l)] = int32_t(rcol.size());
+        rcol.emplace_back();
+        rval.emplace_back();
+      }
+      rcol[size_t(ov[size_t(l)])].swap(mc);
+      rval[size_t(ov[size_t(l)])].swap(mv);
     }
     if (!changed) break;
     if (pass == 31) {
@@ -345,15 +394,24 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
       return MF_ERR_ARG;
     }
   }
-  A->n_hanging = int64_t(lines.size());
+  A->n_hanging = n_lines;
   A->node_line = line_of;
-  A->line_ptr.assign(1, 0u);
-  for (const auto &ln : lines) {
-    for (const auto &mw : ln) {
-      A->line_col.push_back(mw.first);
-      A->line_val.push_back(mw.second);
+  if (rcol.empty()) {
+    A->line_ptr.swap(lptr);
+    A->line_col.swap(lcol);
+    A->line_val.swap(lval);
+  } else {
+    A->line_ptr.assign(1, 0u);
+    A->line_col.clear();
+    A->line_val.clear();
+    for (int64_t l = 0; l < n_lines; ++l) {
+      const uint32_t *c;
+      const double *v;
+      const size_t n = span(l, c, v);
+      A->line_col.insert(A->line_col.end(), c, c + n);
+      A->line_val.insert(A->line_val.end(), v, v + n);
+      A->line_ptr.push_back(uint32_t(A->line_col.size()));
     }
-    A->line_ptr.push_back(uint32_t(A->line_col.size()));
   }
   // default Dirichlet rows: boundary nodes that are not hanging
   s->h_mask.assign(size_t(nn), 0);

@@ -344,4 +344,318 @@ int device_transpose_constraints(AdaptiveData *A, const std::vector<uint32_t> &h
   return MF_OK;
 }
 
+// Hanging-node constraints on the device, the same lines as the host path: a node of a level-l
+// cell c is constrained when a level-(l-1) leaf touches c at an entity holding the node and the
+// node is not one of that leaf's nodes; its line interpolates the leaf's Lagrange basis at the
+// node.  The host takes, for every node, the first (cell, neighbour direction, local node) in
+// loop order that constrains it, and numbers the lines in that order; here every candidate
+// claims its node with an atomic minimum of that order key, and the lines are numbered by
+// sorting the winning keys.  Chains (masters that are constrained themselves) are resolved by
+// the caller.
+namespace {
+__device__ __forceinline__ uint64_t leaf_key_d(int l, int64_t x, int64_t y, int64_t z)
+{
+  return (uint64_t(l) << 60) | (uint64_t(x) << 40) | (uint64_t(y) << 20) | uint64_t(z);
+}
+
+// index of the leaf with the key, or -1 (binary search in the sorted leaf keys)
+__device__ int64_t find_leaf(const uint64_t *keys, const uint32_t *idx, int64_t n, uint64_t k)
+{
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && keys[lo] == k) ? int64_t(idx[lo]) : -1;
+}
+
+__global__ void leaf_keys_kernel(const int32_t *__restrict__ cells, int64_t n, uint64_t *key,
+                                 uint32_t *val)
+{
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < n;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    const int32_t *cl = cells + 4 * c;
+    key[c] = leaf_key_d(cl[0], cl[1], cl[2], cl[3]);
+    val[c] = uint32_t(c);
+  }
+}
+
+__global__ void fill_u64_kernel(uint64_t *x, uint64_t v, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+struct HncArgs {
+  int p, K, L;
+  int64_t n_cells;
+  int ncell[3];
+  double gl[KMAX];
+};
+
+// the coarse leaf of direction dir of cell c when it is one level coarser (else -1), with its
+// lattice index cc
+__device__ int64_t coarse_neighbour(const HncArgs &a, const int32_t *cl, int dir,
+                                    const uint64_t *lkeys, const uint32_t *lidx, int64_t cc[3])
+{
+  const int l = cl[0];
+  const int del[3] = {dir % 3 - 1, (dir / 3) % 3 - 1, dir / 9 - 1};
+  int64_t nb[3];
+  for (int d = 0; d < 3; ++d) {
+    nb[d] = int64_t(cl[1 + d]) + del[d];
+    if (nb[d] < 0 || nb[d] >= (int64_t(a.ncell[d]) << l)) return -1;
+  }
+  // covering leaf: the finest existing leaf at or above level l containing the cell nb
+  for (int lc = l; lc >= 0; --lc) {
+    const int sh = l - lc;
+    const int64_t f = find_leaf(lkeys, lidx, a.n_cells, leaf_key_d(lc, nb[0] >> sh, nb[1] >> sh, nb[2] >> sh));
+    if (f >= 0) {
+      if (lc != l - 1) return -1;
+      for (int d = 0; d < 3; ++d) cc[d] = nb[d] >> 1;
+      return f;
+    }
+  }
+  return -1;
+}
+
+// 1D weights of the constrained node of local coordinate ai along axis d: one-hot if the node
+// is a node of the coarse cell along d, else the coarse Lagrange basis at its position (the
+// products in the host's order, uncontracted, so the weights are the host's bit for bit); returns
+// whether it is a coarse node along d
+__device__ bool axis_weights(const HncArgs &a, int l, int64_t ix, int ai, int64_t ccd, double *wv)
+{
+#pragma clang fp contract(off)
+  const uint64_t kk = axis_key_d(a.p, a.L, l, ix, ai);
+  int match = -1;
+  for (int i = 0; i < a.K; ++i)
+    if (axis_key_d(a.p, a.L, l - 1, ccd, i) == kk) match = i;
+  if (match >= 0) {
+    for (int i = 0; i < a.K; ++i) wv[i] = (i == match) ? 1.0 : 0.0;
+    return true;
+  }
+  const double t = 0.5 * (double(ix) + a.gl[ai]) - double(ccd);
+  for (int j = 0; j < a.K; ++j) {
+    double val = 1.0;
+    for (int m = 0; m < a.K; ++m) {
+      if (m == j) continue;
+      const double inv = 1.0 / (a.gl[j] - a.gl[m]);
+      val *= (t - a.gl[m]) * inv;
+    }
+    wv[j] = val;
+  }
+  return false;
+}
+
+__device__ __forceinline__ bool on_entity(int dir, const int ai[3], int p)
+{
+  const int del[3] = {dir % 3 - 1, (dir / 3) % 3 - 1, dir / 9 - 1};
+  bool on = true;
+  for (int d = 0; d < 3; ++d) on &= (del[d] == 0) || (del[d] < 0 && ai[d] == 0) || (del[d] > 0 && ai[d] == p);
+  return on;
+}
+
+__global__ void hnc_claim_kernel(const HncArgs a, const int32_t *__restrict__ cells,
+                                 const uint32_t *__restrict__ cell_nodes,
+                                 const uint64_t *__restrict__ lkeys, const uint32_t *__restrict__ lidx,
+                                 unsigned long long *claim)
+{
+  const int nloc = a.K * a.K * a.K;
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < a.n_cells * 27;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t c = t / 27;
+    const int dir = int(t - c * 27);
+    if (dir == 13) continue;  // the cell itself
+    const int32_t *cl = cells + 4 * c;
+    if (cl[0] == 0) continue;
+    int64_t cc[3];
+    if (coarse_neighbour(a, cl, dir, lkeys, lidx, cc) < 0) continue;
+    for (int e = 0; e < nloc; ++e) {
+      const int ai[3] = {e % a.K, (e / a.K) % a.K, e / (a.K * a.K)};
+      if (!on_entity(dir, ai, a.p)) continue;
+      double wv[KMAX];
+      bool coarse_node = true;
+      for (int d = 0; d < 3; ++d) coarse_node &= axis_weights(a, cl[0], cl[1 + d], ai[d], cc[d], wv);
+      if (coarse_node) continue;
+      const unsigned long long key = (unsigned long long)(t) * 64ull + unsigned(e);
+      atomicMin(claim + cell_nodes[size_t(c) * nloc + e], key);
+    }
+  }
+}
+
+// hanging nodes: claim keys and node ids compacted in node order
+__global__ void hnc_flag_kernel(const unsigned long long *claim, int64_t nn, int32_t *flag)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nn;
+       i += int64_t(gridDim.x) * blockDim.x)
+    flag[i] = claim[i] != ~0ull ? 1 : 0;
+}
+
+__global__ void hnc_compact_kernel(const unsigned long long *claim, const int32_t *flag,
+                                   const int32_t *pos, int64_t nn, uint64_t *key, uint32_t *node)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < nn;
+       i += int64_t(gridDim.x) * blockDim.x)
+    if (flag[i]) {
+      key[pos[i]] = claim[i];
+      node[pos[i]] = uint32_t(i);
+    }
+}
+
+// line l of the sorted claims: its node's line index, then the entry count (mode 0) or the
+// entries (mode 1)
+template <int MODE>
+__global__ void hnc_line_kernel(const HncArgs a, const int32_t *__restrict__ cells,
+                                const uint32_t *__restrict__ cell_nodes,
+                                const uint64_t *__restrict__ lkeys, const uint32_t *__restrict__ lidx,
+                                const uint64_t *__restrict__ skey, const uint32_t *__restrict__ snode,
+                                int64_t n_lines, int32_t *line_of, uint32_t *count,
+                                const uint32_t *ptr, uint32_t *col, double *val)
+{
+  const int nloc = a.K * a.K * a.K;
+  for (int64_t l = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; l < n_lines;
+       l += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = skey[l];
+    const int e = int(k % 64);
+    const int64_t t = int64_t(k / 64);
+    const int64_t c = t / 27;
+    const int dir = int(t - c * 27);
+    const int32_t *cl = cells + 4 * c;
+    int64_t cc[3];
+    const int64_t ci = coarse_neighbour(a, cl, dir, lkeys, lidx, cc);
+    const int ai[3] = {e % a.K, (e / a.K) % a.K, e / (a.K * a.K)};
+    double wv[3][KMAX];
+    for (int d = 0; d < 3; ++d) axis_weights(a, cl[0], cl[1 + d], ai[d], cc[d], wv[d]);
+    if (MODE == 0) line_of[snode[l]] = int32_t(l);
+    uint32_t q = MODE == 1 ? ptr[l] : 0u;
+    for (int f = 0; f < nloc; ++f) {
+      const double w = wv[0][f % a.K] * wv[1][(f / a.K) % a.K] * wv[2][f / (a.K * a.K)];
+      if (w == 0.0) continue;
+      if (MODE == 1) {
+        col[q] = cell_nodes[size_t(ci) * nloc + f];
+        val[q] = w;
+      }
+      ++q;
+    }
+    if (MODE == 0) count[l] = q;
+  }
+}
+}  // namespace
+
+int device_hanging_constraints(const mf_scratch *s, const int32_t *cells, int64_t n_cells, int L,
+                               const std::vector<uint32_t> &cell_nodes, int64_t nn,
+                               std::vector<int32_t> &line_of, std::vector<uint32_t> &line_ptr,
+                               std::vector<uint32_t> &line_col, std::vector<double> &line_val)
+{
+  const MeshInfo &m = s->mesh;
+  const int p = m.degree, K = p + 1, nloc = K * K * K;
+  hipStream_t st = s->stream;
+  MF_REQUIRE(n_cells * 27 < (int64_t(1) << 57), MF_ERR_ARG, "adaptive mesh too large");
+  HncArgs a{};
+  a.p = p;
+  a.K = K;
+  a.L = L;
+  a.n_cells = n_cells;
+  for (int d = 0; d < 3; ++d) a.ncell[d] = m.ncell[d];
+  for (int i = 0; i < K; ++i) a.gl[i] = s->tab.gl[i];
+  DevBufs b;
+  int32_t *d_cells, *flag, *pos, *d_line_of;
+  uint32_t *d_nodes, *lval_idx, *lidx, *snode0, *snode, *cnt, *d_ptr, *d_col;
+  uint64_t *lkey0, *lkeys, *skey0, *skey;
+  unsigned long long *claim;
+  double *d_val;
+  MF_HIP_TRY(b.alloc(&d_cells, size_t(4 * n_cells)));
+  MF_HIP_TRY(b.alloc(&d_nodes, size_t(n_cells) * nloc));
+  MF_HIP_TRY(b.alloc(&lkey0, size_t(n_cells)));
+  MF_HIP_TRY(b.alloc(&lkeys, size_t(n_cells)));
+  MF_HIP_TRY(b.alloc(&lval_idx, size_t(n_cells)));
+  MF_HIP_TRY(b.alloc(&lidx, size_t(n_cells)));
+  MF_HIP_TRY(b.alloc(&claim, size_t(nn)));
+  MF_HIP_TRY(b.alloc(&flag, size_t(nn) + 1));
+  MF_HIP_TRY(b.alloc(&pos, size_t(nn) + 1));
+  MF_HIP_TRY(b.alloc(&d_line_of, size_t(nn)));
+  MF_HIP_TRY(hipMemcpyAsync(d_cells, cells, sizeof(int32_t) * 4 * size_t(n_cells),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemcpyAsync(d_nodes, cell_nodes.data(), sizeof(uint32_t) * cell_nodes.size(),
+                            hipMemcpyHostToDevice, st));
+  const dim3 block(256);
+  auto grid = [](int64_t n) { return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)))); };
+  hipLaunchKernelGGL(leaf_keys_kernel, grid(n_cells), block, 0, st, d_cells, n_cells, lkey0, lval_idx);
+  MF_HIP_TRY(hipGetLastError());
+  size_t tb1 = 0, tb2 = 0, tb3 = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, lkey0, lkeys, lval_idx, lidx,
+                                                int(n_cells), 0, 64, st));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, flag, pos, int(nn) + 1, st));
+  char *tmp;
+  const size_t tmax = std::max(tb1, tb2);
+  MF_HIP_TRY(b.alloc(&tmp, tmax));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb1, lkey0, lkeys, lval_idx, lidx,
+                                                int(n_cells), 0, 64, st));
+  hipLaunchKernelGGL(fill_u64_kernel, grid(nn), block, 0, st, reinterpret_cast<uint64_t *>(claim),
+                     ~0ull, nn);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipMemsetAsync(d_line_of, 0xff, sizeof(int32_t) * size_t(nn), st));
+  MF_HIP_TRY(hipMemsetAsync(flag + nn, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(hnc_claim_kernel, grid(n_cells * 27), block, 0, st, a, d_cells, d_nodes,
+                     lkeys, lidx, claim);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(hnc_flag_kernel, grid(nn), block, 0, st, claim, nn, flag);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, flag, pos, int(nn) + 1, st));
+  int32_t n_lines32 = 0;
+  MF_HIP_TRY(hipMemcpyAsync(&n_lines32, pos + nn, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  const int64_t n_lines = n_lines32;
+  line_of.assign(size_t(nn), -1);
+  line_ptr.assign(1, 0u);
+  line_col.clear();
+  line_val.clear();
+  if (n_lines == 0) return MF_OK;
+  MF_HIP_TRY(b.alloc(&skey0, size_t(n_lines)));
+  MF_HIP_TRY(b.alloc(&skey, size_t(n_lines)));
+  MF_HIP_TRY(b.alloc(&snode0, size_t(n_lines)));
+  MF_HIP_TRY(b.alloc(&snode, size_t(n_lines)));
+  MF_HIP_TRY(b.alloc(&cnt, size_t(n_lines) + 1));
+  MF_HIP_TRY(b.alloc(&d_ptr, size_t(n_lines) + 1));
+  hipLaunchKernelGGL(hnc_compact_kernel, grid(nn), block, 0, st, claim, flag, pos, nn, skey0, snode0);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb3, skey0, skey, snode0, snode,
+                                                int(n_lines), 0, 64, st));
+  size_t tb4 = 0;
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb4, cnt, d_ptr, int(n_lines) + 1, st));
+  char *tmp2;
+  MF_HIP_TRY(b.alloc(&tmp2, std::max(tb3, tb4)));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp2, tb3, skey0, skey, snode0, snode,
+                                                int(n_lines), 0, 64, st));
+  MF_HIP_TRY(hipMemsetAsync(cnt + n_lines, 0, sizeof(uint32_t), st));
+  hipLaunchKernelGGL((hnc_line_kernel<0>), grid(n_lines), block, 0, st, a, d_cells, d_nodes, lkeys,
+                     lidx, skey, snode, n_lines, d_line_of, cnt, (const uint32_t *)nullptr,
+                     (uint32_t *)nullptr, (double *)nullptr);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp2, tb4, cnt, d_ptr, int(n_lines) + 1, st));
+  uint32_t n_entries = 0;
+  MF_HIP_TRY(hipMemcpyAsync(&n_entries, d_ptr + n_lines, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  MF_HIP_TRY(b.alloc(&d_col, size_t(n_entries)));
+  MF_HIP_TRY(b.alloc(&d_val, size_t(n_entries)));
+  hipLaunchKernelGGL((hnc_line_kernel<1>), grid(n_lines), block, 0, st, a, d_cells, d_nodes, lkeys,
+                     lidx, skey, snode, n_lines, (int32_t *)nullptr, (uint32_t *)nullptr, d_ptr,
+                     d_col, d_val);
+  MF_HIP_TRY(hipGetLastError());
+  line_ptr.resize(size_t(n_lines) + 1);
+  line_col.resize(n_entries);
+  line_val.resize(n_entries);
+  MF_HIP_TRY(hipMemcpyAsync(line_of.data(), d_line_of, sizeof(int32_t) * size_t(nn),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(line_ptr.data(), d_ptr, sizeof(uint32_t) * (size_t(n_lines) + 1),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(line_col.data(), d_col, sizeof(uint32_t) * n_entries,
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(line_val.data(), d_val, sizeof(double) * n_entries,
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
 }  // namespace mf

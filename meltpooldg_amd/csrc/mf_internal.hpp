@@ -191,6 +191,12 @@ int device_node_identification(const ::mf_scratch *s, const int32_t *cells, int6
 // (mf_adaptive_dev.hip); hang_of_line[l] = the hanging node of constraint line l
 int device_transpose_constraints(AdaptiveData *A, const std::vector<uint32_t> &hang_of_line,
                                  hipStream_t st);
+// hanging-node constraint lines before chain resolution, on the device (mf_adaptive_dev.hip):
+// line_of[node] = its line or -1, lines numbered in the host path's order, CSR (ptr, col, val)
+int device_hanging_constraints(const ::mf_scratch *s, const int32_t *cells, int64_t n_cells, int L,
+                               const std::vector<uint32_t> &cell_nodes, int64_t nn,
+                               std::vector<int32_t> &line_of, std::vector<uint32_t> &line_ptr,
+                               std::vector<uint32_t> &line_col, std::vector<double> &line_val);
 // 1: adaptive setup steps run on the device when one is present (default), 0: on the host
 extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);

@@ -1,6 +1,6 @@
-"""Device node identification of adaptive meshes (mf_adaptive_dev.hip) against the host path:
-the same DoF numbering, support points, boundary flags and, built from them, the same
-hanging-node constraints, Dirichlet rows and operator (bit for bit)."""
+"""Device adaptive setup (mf_adaptive_dev.hip: node identification, hanging-node constraint
+lines, their transposition) against the host path: the same DoF numbering, support points,
+boundary flags, hanging-node constraints, Dirichlet rows and operator (bit for bit)."""
 import numpy as np
 import pytest
 
