@@ -506,50 +506,57 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   A->colour_start[n_col] = int64_t(A->list.size());
   A->n_colours = n_col;
   // hanging-cell scatter targets: each distinct node a hanging cell writes, with the local
-  // entries (local node, weight) that sum into it
-  A->cell_hidx.assign(size_t(n_cells), -1);
-  A->ht_ptr.assign(1, 0u);
-  A->tgt_ptr.assign(1, 0u);
-  std::vector<std::pair<uint32_t, std::pair<int, double>>> ent;  // node, (local, weight)
-  for (int64_t c = 0; c < n_cells; ++c) {
-    if (!A->cell_hmask[c]) continue;
-    A->cell_hidx[c] = int32_t(A->ht_ptr.size() - 1);
-    ent.clear();
-    for (int e = 0; e < nloc; ++e) {
-      const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
-      if (line_of[id] >= 0) {
-        for (uint32_t t = A->line_ptr[line_of[id]]; t < A->line_ptr[line_of[id] + 1]; ++t)
-          ent.push_back({A->line_col[t], {e, A->line_val[t]}});
-      } else {
-        ent.push_back({id, {e, 1.0}});
+  // entries (local node, weight) that sum into it, and the slots of the condensed diagonal,
+  // one per (hanging cell, target), added with atomics.  Built on the device (and kept there)
+  // when node identification ran there, else here
+  if (on_device) {
+    const int rc = device_hanging_targets(A, n_cells, nloc, s->stream);
+    if (rc != MF_OK) return rc;
+  } else {
+    A->cell_hidx.assign(size_t(n_cells), -1);
+    A->ht_ptr.assign(1, 0u);
+    A->tgt_ptr.assign(1, 0u);
+    std::vector<std::pair<uint32_t, std::pair<int, double>>> ent;  // node, (local, weight)
+    for (int64_t c = 0; c < n_cells; ++c) {
+      if (!A->cell_hmask[c]) continue;
+      A->cell_hidx[c] = int32_t(A->ht_ptr.size() - 1);
+      ent.clear();
+      for (int e = 0; e < nloc; ++e) {
+        const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
+        if (line_of[id] >= 0) {
+          for (uint32_t t = A->line_ptr[line_of[id]]; t < A->line_ptr[line_of[id] + 1]; ++t)
+            ent.push_back({A->line_col[t], {e, A->line_val[t]}});
+        } else {
+          ent.push_back({id, {e, 1.0}});
+        }
+      }
+      std::stable_sort(ent.begin(), ent.end(),
+                       [](const auto &a, const auto &b) { return a.first < b.first; });
+      for (size_t a = 0; a < ent.size();) {
+        size_t b = a;
+        A->tgt_node.push_back(ent[a].first);
+        while (b < ent.size() && ent[b].first == ent[a].first) {
+          A->ent_local.push_back(uint8_t(ent[b].second.first));
+          A->ent_w.push_back(ent[b].second.second);
+          ++b;
+        }
+        A->tgt_ptr.push_back(uint32_t(A->ent_w.size()));
+        a = b;
+      }
+      A->ht_ptr.push_back(uint32_t(A->tgt_node.size()));
+    }
+    // slots of the condensed diagonal: (hanging cell, target); added with atomics
+    A->slot_start.assign(2, 0);
+    for (int64_t c = 0; c < n_cells; ++c) {
+      const int32_t hi = A->cell_hidx[c];
+      if (hi < 0) continue;
+      for (uint32_t t = A->ht_ptr[hi]; t < A->ht_ptr[hi + 1]; ++t) {
+        A->slot_cell.push_back(uint32_t(c));
+        A->slot_tgt.push_back(t);
       }
     }
-    std::stable_sort(ent.begin(), ent.end(),
-                     [](const auto &a, const auto &b) { return a.first < b.first; });
-    for (size_t a = 0; a < ent.size();) {
-      size_t b = a;
-      A->tgt_node.push_back(ent[a].first);
-      while (b < ent.size() && ent[b].first == ent[a].first) {
-        A->ent_local.push_back(uint8_t(ent[b].second.first));
-        A->ent_w.push_back(ent[b].second.second);
-        ++b;
-      }
-      A->tgt_ptr.push_back(uint32_t(A->ent_w.size()));
-      a = b;
-    }
-    A->ht_ptr.push_back(uint32_t(A->tgt_node.size()));
+    A->slot_start[1] = int64_t(A->slot_cell.size());
   }
-  // slots of the condensed diagonal: (hanging cell, target); added with atomics
-  A->slot_start.assign(2, 0);
-  for (int64_t c = 0; c < n_cells; ++c) {
-    const int32_t hi = A->cell_hidx[c];
-    if (hi < 0) continue;
-    for (uint32_t t = A->ht_ptr[hi]; t < A->ht_ptr[hi + 1]; ++t) {
-      A->slot_cell.push_back(uint32_t(c));
-      A->slot_tgt.push_back(t);
-    }
-  }
-  A->slot_start[1] = int64_t(A->slot_cell.size());
   m.ad = A;
   return MF_OK;  // device tables are uploaded when the first operator is created
 }
@@ -565,6 +572,7 @@ int ensure_adaptive_on_device(mf_scratch *s)
 template <typename T>
 static int upload(T **d, const std::vector<T> &h)
 {
+  if (*d) return MF_OK;  // built on the device
   if (h.empty()) {
     *d = nullptr;
     return MF_OK;

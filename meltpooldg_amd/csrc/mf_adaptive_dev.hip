@@ -658,4 +658,222 @@ int device_hanging_constraints(const mf_scratch *s, const int32_t *cells, int64_
   return MF_OK;
 }
 
+// Scatter targets of the hanging cells, the tables the cell kernel's hanging pass reads: for the
+// h-th hanging cell (cells in index order) every distinct node it writes through its constraint
+// lines, ascending, with the (local node, weight) entries that sum into it, in the host's order
+// (a stable sort of the cell's entries by node).  Here all entries of all hanging cells are
+// keyed (h << 32 | node) and sorted stably in one radix sort, whose runs are the targets.
+namespace {
+__global__ void tgt_count_kernel(const uint32_t *__restrict__ cell_nodes,
+                                 const int32_t *__restrict__ node_line,
+                                 const uint32_t *__restrict__ line_ptr, int64_t n_cells, int nloc,
+                                 uint32_t *hflag, uint32_t *ecount)
+{
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < n_cells;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    uint32_t cnt = 0, any = 0;
+    for (int e = 0; e < nloc; ++e) {
+      const int32_t l = node_line[cell_nodes[size_t(c) * nloc + e]];
+      if (l >= 0) {
+        any = 1;
+        cnt += line_ptr[l + 1] - line_ptr[l];
+      } else {
+        cnt += 1;
+      }
+    }
+    hflag[c] = any;
+    ecount[c] = any ? cnt : 0u;
+  }
+}
+
+__global__ void tgt_fill_kernel(const uint32_t *__restrict__ cell_nodes,
+                                const int32_t *__restrict__ node_line,
+                                const uint32_t *__restrict__ line_ptr,
+                                const uint32_t *__restrict__ line_col,
+                                const double *__restrict__ line_val, int64_t n_cells, int nloc,
+                                const uint32_t *__restrict__ hflag, const uint32_t *__restrict__ hpos,
+                                const uint32_t *__restrict__ eoff, int32_t *cell_hidx,
+                                uint32_t *hcell, uint64_t *key, uint32_t *idx, uint8_t *loc, double *w)
+{
+  for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < n_cells;
+       c += int64_t(gridDim.x) * blockDim.x) {
+    if (!hflag[c]) {
+      cell_hidx[c] = -1;
+      continue;
+    }
+    const uint32_t h = hpos[c];
+    cell_hidx[c] = int32_t(h);
+    hcell[h] = uint32_t(c);
+    uint32_t q = eoff[c];
+    for (int e = 0; e < nloc; ++e) {
+      const uint32_t id = cell_nodes[size_t(c) * nloc + e];
+      const int32_t l = node_line[id];
+      if (l >= 0) {
+        for (uint32_t t = line_ptr[l]; t < line_ptr[l + 1]; ++t, ++q) {
+          key[q] = (uint64_t(h) << 32) | line_col[t];
+          idx[q] = q;
+          loc[q] = uint8_t(e);
+          w[q] = line_val[t];
+        }
+      } else {
+        key[q] = (uint64_t(h) << 32) | id;
+        idx[q] = q;
+        loc[q] = uint8_t(e);
+        w[q] = 1.0;
+        ++q;
+      }
+    }
+  }
+}
+
+__global__ void tgt_gather_kernel(const uint64_t *__restrict__ key_s, const uint32_t *__restrict__ idx_s,
+                                  const uint8_t *__restrict__ loc, const double *__restrict__ w,
+                                  int64_t n, uint8_t *ent_local, double *ent_w, float *ent_w32,
+                                  uint32_t *head)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t j = idx_s[i];
+    ent_local[i] = loc[j];
+    ent_w[i] = w[j];
+    ent_w32[i] = float(w[j]);
+    head[i] = (i == 0 || key_s[i] != key_s[i - 1]) ? 1u : 0u;
+  }
+}
+
+__global__ void tgt_heads_kernel(const uint64_t *__restrict__ key_s, const uint32_t *__restrict__ head,
+                                 const uint32_t *__restrict__ tpos, const uint32_t *__restrict__ hcell,
+                                 int64_t n, uint32_t *ht_ptr, uint32_t *tgt_node, uint32_t *tgt_ptr,
+                                 uint32_t *slot_cell, uint32_t *slot_tgt)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (!head[i]) continue;
+    const uint32_t t = tpos[i];
+    const uint32_t h = uint32_t(key_s[i] >> 32);
+    tgt_node[t] = uint32_t(key_s[i] & 0xffffffffull);
+    tgt_ptr[t] = uint32_t(i);
+    slot_tgt[t] = t;
+    slot_cell[t] = hcell[h];
+    if (i == 0 || uint32_t(key_s[i - 1] >> 32) != h) ht_ptr[h] = t;
+  }
+}
+
+__global__ void tgt_tail_kernel(uint32_t *ht_ptr, uint32_t *tgt_ptr, int64_t n_hc, uint32_t n_t,
+                                uint32_t n_e)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ht_ptr[n_hc] = n_t;
+    tgt_ptr[n_t] = n_e;
+  }
+}
+
+template <typename T>
+hipError_t keep_on_device(T **d, const std::vector<T> &h, hipStream_t st)
+{
+  if (*d || h.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(d), sizeof(T) * h.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st);
+}
+
+template <typename T>
+hipError_t alloc_kept(T **d, size_t n)
+{
+  return hipMalloc(reinterpret_cast<void **>(d), sizeof(T) * (n ? n : 1));
+}
+}  // namespace
+
+int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream_t st)
+{
+  // the inputs stay on the device as the operator's tables
+  MF_HIP_TRY(keep_on_device(&A->d.cell_nodes, A->cell_nodes, st));
+  MF_HIP_TRY(keep_on_device(&A->d.node_line, A->node_line, st));
+  MF_HIP_TRY(keep_on_device(&A->d.line_ptr, A->line_ptr, st));
+  MF_HIP_TRY(keep_on_device(&A->d.line_col, A->line_col, st));
+  MF_HIP_TRY(keep_on_device(&A->d.line_val, A->line_val, st));
+  MF_HIP_TRY(alloc_kept(&A->d.cell_hidx, size_t(n_cells)));
+  DevBufs b;
+  uint32_t *hflag, *hpos, *ecount, *eoff;
+  MF_HIP_TRY(b.alloc(&hflag, size_t(n_cells) + 1));
+  MF_HIP_TRY(b.alloc(&hpos, size_t(n_cells) + 1));
+  MF_HIP_TRY(b.alloc(&ecount, size_t(n_cells) + 1));
+  MF_HIP_TRY(b.alloc(&eoff, size_t(n_cells) + 1));
+  const dim3 block(256);
+  auto grid = [](int64_t n) { return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)))); };
+  MF_HIP_TRY(hipMemsetAsync(hflag + n_cells, 0, sizeof(uint32_t), st));
+  MF_HIP_TRY(hipMemsetAsync(ecount + n_cells, 0, sizeof(uint32_t), st));
+  // a cell without hanging nodes reads an empty line table: node_line is then all -1
+  hipLaunchKernelGGL(tgt_count_kernel, grid(n_cells), block, 0, st, A->d.cell_nodes, A->d.node_line,
+                     A->d.line_ptr, n_cells, nloc, hflag, ecount);
+  MF_HIP_TRY(hipGetLastError());
+  size_t tb = 0;
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hflag, hpos, int(n_cells) + 1, st));
+  char *tmp;
+  MF_HIP_TRY(b.alloc(&tmp, tb));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, hflag, hpos, int(n_cells) + 1, st));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, ecount, eoff, int(n_cells) + 1, st));
+  uint32_t tot[2] = {0, 0};
+  MF_HIP_TRY(hipMemcpyAsync(&tot[0], hpos + n_cells, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(&tot[1], eoff + n_cells, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  const int64_t n_hc = tot[0], n_e = tot[1];
+  MF_REQUIRE(n_e < (int64_t(1) << 31), MF_ERR_ARG, "too many hanging-cell entries");
+  uint32_t *hcell, *idx, *idx_s, *head, *tpos;
+  uint64_t *key, *key_s;
+  uint8_t *loc;
+  double *w;
+  MF_HIP_TRY(b.alloc(&hcell, size_t(n_hc)));
+  MF_HIP_TRY(b.alloc(&key, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&key_s, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&idx, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&idx_s, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&loc, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&w, size_t(n_e)));
+  MF_HIP_TRY(b.alloc(&head, size_t(n_e) + 1));
+  MF_HIP_TRY(b.alloc(&tpos, size_t(n_e) + 1));
+  hipLaunchKernelGGL(tgt_fill_kernel, grid(n_cells), block, 0, st, A->d.cell_nodes, A->d.node_line,
+                     A->d.line_ptr, A->d.line_col, A->d.line_val, n_cells, nloc, hflag, hpos, eoff,
+                     A->d.cell_hidx, hcell, key, idx, loc, w);
+  MF_HIP_TRY(hipGetLastError());
+  A->slot_start.assign(2, 0);
+  if (n_e == 0) return MF_OK;
+  int hbits = 1;
+  while ((int64_t(1) << hbits) <= n_hc) ++hbits;
+  size_t tb2 = 0, tb3 = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, key, key_s, idx, idx_s, int(n_e), 0,
+                                                32 + hbits, st));
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, head, tpos, int(n_e) + 1, st));
+  char *tmp2;
+  MF_HIP_TRY(b.alloc(&tmp2, std::max(tb2, tb3)));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp2, tb2, key, key_s, idx, idx_s, int(n_e), 0,
+                                                32 + hbits, st));
+  MF_HIP_TRY(alloc_kept(&A->d.ent_local, size_t(n_e)));
+  MF_HIP_TRY(alloc_kept(&A->d.ent_w, size_t(n_e)));
+  MF_HIP_TRY(alloc_kept(&A->d.ent_w32, size_t(n_e)));
+  MF_HIP_TRY(hipMemsetAsync(head + n_e, 0, sizeof(uint32_t), st));
+  hipLaunchKernelGGL(tgt_gather_kernel, grid(n_e), block, 0, st, key_s, idx_s, loc, w, n_e,
+                     A->d.ent_local, A->d.ent_w, A->d.ent_w32, head);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp2, tb3, head, tpos, int(n_e) + 1, st));
+  uint32_t n_t32 = 0;
+  MF_HIP_TRY(hipMemcpyAsync(&n_t32, tpos + n_e, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  const int64_t n_t = n_t32;
+  MF_HIP_TRY(alloc_kept(&A->d.ht_ptr, size_t(n_hc) + 1));
+  MF_HIP_TRY(alloc_kept(&A->d.tgt_node, size_t(n_t)));
+  MF_HIP_TRY(alloc_kept(&A->d.tgt_ptr, size_t(n_t) + 1));
+  MF_HIP_TRY(alloc_kept(&A->d.slot_cell, size_t(n_t)));
+  MF_HIP_TRY(alloc_kept(&A->d.slot_tgt, size_t(n_t)));
+  hipLaunchKernelGGL(tgt_heads_kernel, grid(n_e), block, 0, st, key_s, head, tpos, hcell, n_e,
+                     A->d.ht_ptr, A->d.tgt_node, A->d.tgt_ptr, A->d.slot_cell, A->d.slot_tgt);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(tgt_tail_kernel, dim3(1), dim3(64), 0, st, A->d.ht_ptr, A->d.tgt_ptr, n_hc,
+                     uint32_t(n_t), uint32_t(n_e));
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  A->slot_start[1] = n_t;
+  return MF_OK;
+}
+
 }  // namespace mf

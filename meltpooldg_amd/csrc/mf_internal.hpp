@@ -197,6 +197,10 @@ int device_hanging_constraints(const ::mf_scratch *s, const int32_t *cells, int6
                                const std::vector<uint32_t> &cell_nodes, int64_t nn,
                                std::vector<int32_t> &line_of, std::vector<uint32_t> &line_ptr,
                                std::vector<uint32_t> &line_col, std::vector<double> &line_val);
+// hanging-cell scatter targets and condensed-diagonal slots on the device, written into A->d
+// (cell_hidx, ht_ptr, tgt_node, tgt_ptr, ent_local, ent_w, ent_w32, slot_cell, slot_tgt) with
+// A->slot_start; uploads and keeps cell_nodes, node_line and the constraint lines there too
+int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream_t st);
 // 1: adaptive setup steps run on the device when one is present (default), 0: on the host
 extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);
