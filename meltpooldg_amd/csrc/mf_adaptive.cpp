@@ -277,7 +277,12 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
             inside &= nb[d] >= 0 && nb[d] < (int64_t(m.ncell[d]) << l);
           }
           if (!inside) continue;
-          if (leaves.covering(l, nb[0], nb[1], nb[2]) != l - 1) continue;
+          const int cov = leaves.covering(l, nb[0], nb[1], nb[2]);
+          if (cov >= 0 && cov < l - 1) {
+            set_error("adaptive mesh is not 2:1 balanced");
+            return MF_ERR_ARG;
+          }
+          if (cov != l - 1) continue;
           const int64_t cc[3] = {nb[0] >> 1, nb[1] >> 1, nb[2] >> 1};  // the coarse leaf
           const int64_t ci = leaf_index.at(leaf_key(l - 1, cc[0], cc[1], cc[2]));
           uint64_t ck[3][KMAX];
@@ -419,42 +424,31 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
   // --- per-cell hanging masks and colouring ------------------------------------------------
   // A cell writes its own K^3 nodes (hanging ones included); the hanging rows are moved to
   // their masters afterwards by a separate gather over the masters.  So cells conflict only
-  // when they share a node, as on a conforming mesh.
-  A->cell_hmask.assign(size_t(n_cells), 0);
+  // when they share a node, as on a conforming mesh.  The colour of a cell is its class
+  // (level parity, index parity): cells of one class never share a node (same level: two
+  // apart along some axis; levels two or more apart never touch on a 2:1-balanced mesh, which
+  // the constraint pass checks), so the classes are conflict-free colours.
   A->cell_level.resize(size_t(n_cells));
-  std::vector<uint64_t> used(static_cast<size_t>(nn), 0ull);
-  std::vector<int> colour(static_cast<size_t>(n_cells));
-  int n_col = 0;
-  // greedy colouring visiting the cells class by class, class = (level parity, index parity):
-  // cells of one class never touch (same level: two apart; levels two apart never touch on a
-  // 2:1-balanced mesh), so the first 8 colours come out as balanced parity classes and the
-  // coarse/fine interface cells fill the rest
-  std::vector<int64_t> visit(static_cast<size_t>(n_cells));
-  for (int64_t c = 0; c < n_cells; ++c) visit[c] = c;
+  for (int64_t c = 0; c < n_cells; ++c) A->cell_level[c] = uint8_t(cells[4 * c]);
+  if (on_device) {
+    // the hanging masks with the hanging-cell tables, all kept on the device
+    const int rc = device_hanging_targets(A, n_cells, nloc, s->stream);
+    if (rc != MF_OK) return rc;
+  } else {
+    A->cell_hmask.assign(size_t(n_cells), 0);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int e = 0; e < nloc; ++e)
+        if (line_of[A->cell_nodes[size_t(c) * nloc + e]] >= 0) A->cell_hmask[c] |= 1ull << e;
+  }
   auto cls = [&](int64_t c) {
     const int32_t *cl = cells + 4 * c;
     return (cl[0] & 1) * 8 + (cl[1] & 1) + 2 * (cl[2] & 1) + 4 * (cl[3] & 1);
   };
-  std::stable_sort(visit.begin(), visit.end(),
-                   [&](int64_t a, int64_t b) { return cls(a) < cls(b); });
-  for (int64_t c : visit) {
-    A->cell_level[c] = uint8_t(cells[4 * c]);
-    uint64_t forbidden = 0;
-    for (int e = 0; e < nloc; ++e) {
-      const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
-      if (line_of[id] >= 0) A->cell_hmask[c] |= 1ull << e;
-      forbidden |= used[id];
-    }
-    if (~forbidden == 0) {
-      set_error("adaptive mesh: more than 64 cell colours");
-      return MF_ERR_ARG;
-    }
-    int col = 0;
-    while ((forbidden >> col) & 1ull) ++col;
-    colour[c] = col;
-    n_col = std::max(n_col, col + 1);
-    for (int e = 0; e < nloc; ++e) used[A->cell_nodes[size_t(c) * nloc + e]] |= 1ull << col;
-  }
+  int64_t cls_count[16] = {};
+  for (int64_t c = 0; c < n_cells; ++c) ++cls_count[cls(c)];
+  int cmap[16];
+  int n_col = 0;
+  for (int k = 0; k < 16; ++k) cmap[k] = cls_count[k] ? n_col++ : -1;
   // transposed constraints: for every master, the hanging nodes (and weights) it receives from
   A->hang_nodes.clear();
   for (int64_t i = 0; i < nn; ++i)
@@ -491,28 +485,25 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
       a = b;
     }
   }
-  // cell lists: by colour, non-hanging cells first
+  // cell lists: by colour, non-hanging cells first, each part in cell order
+  std::vector<int64_t> off(size_t(2 * n_col) + 1, 0);
+  for (int64_t c = 0; c < n_cells; ++c) ++off[size_t(2 * cmap[cls(c)] + (A->cell_hmask[c] != 0)) + 1];
+  for (size_t k = 1; k < off.size(); ++k) off[k] += off[k - 1];
   A->colour_start.assign(n_col + 1, 0);
   A->colour_hang.assign(n_col, 0);
-  A->list.clear();
   for (int col = 0; col < n_col; ++col) {
-    A->colour_start[col] = int64_t(A->list.size());
-    for (int64_t c = 0; c < n_cells; ++c)
-      if (colour[c] == col && A->cell_hmask[c] == 0) A->list.push_back(uint32_t(c));
-    A->colour_hang[col] = int64_t(A->list.size());
-    for (int64_t c = 0; c < n_cells; ++c)
-      if (colour[c] == col && A->cell_hmask[c] != 0) A->list.push_back(uint32_t(c));
+    A->colour_start[col] = off[size_t(2 * col)];
+    A->colour_hang[col] = off[size_t(2 * col + 1)];
   }
-  A->colour_start[n_col] = int64_t(A->list.size());
+  A->colour_start[n_col] = n_cells;
+  A->list.resize(size_t(n_cells));
+  for (int64_t c = 0; c < n_cells; ++c)
+    A->list[size_t(off[size_t(2 * cmap[cls(c)] + (A->cell_hmask[c] != 0))]++)] = uint32_t(c);
   A->n_colours = n_col;
   // hanging-cell scatter targets: each distinct node a hanging cell writes, with the local
   // entries (local node, weight) that sum into it, and the slots of the condensed diagonal,
-  // one per (hanging cell, target), added with atomics.  Built on the device (and kept there)
-  // when node identification ran there, else here
-  if (on_device) {
-    const int rc = device_hanging_targets(A, n_cells, nloc, s->stream);
-    if (rc != MF_OK) return rc;
-  } else {
+  // one per (hanging cell, target), added with atomics (built above when on the device)
+  if (!on_device) {
     A->cell_hidx.assign(size_t(n_cells), -1);
     A->ht_ptr.assign(1, 0u);
     A->tgt_ptr.assign(1, 0u);

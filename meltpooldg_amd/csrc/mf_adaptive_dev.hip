@@ -395,8 +395,8 @@ struct HncArgs {
   double gl[KMAX];
 };
 
-// the coarse leaf of direction dir of cell c when it is one level coarser (else -1), with its
-// lattice index cc
+// the coarse leaf of direction dir of cell c when it is one level coarser (else -1; -2 when it
+// is two or more levels coarser), with its lattice index cc
 __device__ int64_t coarse_neighbour(const HncArgs &a, const int32_t *cl, int dir,
                                     const uint64_t *lkeys, const uint32_t *lidx, int64_t cc[3])
 {
@@ -412,6 +412,7 @@ __device__ int64_t coarse_neighbour(const HncArgs &a, const int32_t *cl, int dir
     const int sh = l - lc;
     const int64_t f = find_leaf(lkeys, lidx, a.n_cells, leaf_key_d(lc, nb[0] >> sh, nb[1] >> sh, nb[2] >> sh));
     if (f >= 0) {
+      if (lc < l - 1) return -2;  // not 2:1 balanced
       if (lc != l - 1) return -1;
       for (int d = 0; d < 3; ++d) cc[d] = nb[d] >> 1;
       return f;
@@ -459,7 +460,7 @@ __device__ __forceinline__ bool on_entity(int dir, const int ai[3], int p)
 __global__ void hnc_claim_kernel(const HncArgs a, const int32_t *__restrict__ cells,
                                  const uint32_t *__restrict__ cell_nodes,
                                  const uint64_t *__restrict__ lkeys, const uint32_t *__restrict__ lidx,
-                                 unsigned long long *claim)
+                                 unsigned long long *claim, int *unbalanced)
 {
   const int nloc = a.K * a.K * a.K;
   for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < a.n_cells * 27;
@@ -470,7 +471,9 @@ __global__ void hnc_claim_kernel(const HncArgs a, const int32_t *__restrict__ ce
     const int32_t *cl = cells + 4 * c;
     if (cl[0] == 0) continue;
     int64_t cc[3];
-    if (coarse_neighbour(a, cl, dir, lkeys, lidx, cc) < 0) continue;
+    const int64_t ci = coarse_neighbour(a, cl, dir, lkeys, lidx, cc);
+    if (ci == -2) *unbalanced = 1;
+    if (ci < 0) continue;
     for (int e = 0; e < nloc; ++e) {
       const int ai[3] = {e % a.K, (e / a.K) % a.K, e / (a.K * a.K)};
       if (!on_entity(dir, ai, a.p)) continue;
@@ -575,6 +578,8 @@ int device_hanging_constraints(const mf_scratch *s, const int32_t *cells, int64_
   MF_HIP_TRY(b.alloc(&flag, size_t(nn) + 1));
   MF_HIP_TRY(b.alloc(&pos, size_t(nn) + 1));
   MF_HIP_TRY(b.alloc(&d_line_of, size_t(nn)));
+  int *unbal;
+  MF_HIP_TRY(b.alloc(&unbal, 1));
   MF_HIP_TRY(hipMemcpyAsync(d_cells, cells, sizeof(int32_t) * 4 * size_t(n_cells),
                             hipMemcpyHostToDevice, st));
   MF_HIP_TRY(hipMemcpyAsync(d_nodes, cell_nodes.data(), sizeof(uint32_t) * cell_nodes.size(),
@@ -597,15 +602,19 @@ int device_hanging_constraints(const mf_scratch *s, const int32_t *cells, int64_
   MF_HIP_TRY(hipGetLastError());
   MF_HIP_TRY(hipMemsetAsync(d_line_of, 0xff, sizeof(int32_t) * size_t(nn), st));
   MF_HIP_TRY(hipMemsetAsync(flag + nn, 0, sizeof(int32_t), st));
+  MF_HIP_TRY(hipMemsetAsync(unbal, 0, sizeof(int), st));
   hipLaunchKernelGGL(hnc_claim_kernel, grid(n_cells * 27), block, 0, st, a, d_cells, d_nodes,
-                     lkeys, lidx, claim);
+                     lkeys, lidx, claim, unbal);
   MF_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(hnc_flag_kernel, grid(nn), block, 0, st, claim, nn, flag);
   MF_HIP_TRY(hipGetLastError());
   MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb2, flag, pos, int(nn) + 1, st));
   int32_t n_lines32 = 0;
+  int unbalanced = 0;
   MF_HIP_TRY(hipMemcpyAsync(&n_lines32, pos + nn, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipMemcpyAsync(&unbalanced, unbal, sizeof(int), hipMemcpyDeviceToHost, st));
   MF_HIP_TRY(hipStreamSynchronize(st));
+  MF_REQUIRE(!unbalanced, MF_ERR_ARG, "adaptive mesh is not 2:1 balanced");
   const int64_t n_lines = n_lines32;
   line_of.assign(size_t(nn), -1);
   line_ptr.assign(1, 0u);
@@ -658,7 +667,8 @@ int device_hanging_constraints(const mf_scratch *s, const int32_t *cells, int64_
   return MF_OK;
 }
 
-// Scatter targets of the hanging cells, the tables the cell kernel's hanging pass reads: for the
+// Hanging masks of the cells (bit e: local node e is hanging) and the scatter targets of the
+// hanging cells, the tables the cell kernel's hanging pass reads: for the
 // h-th hanging cell (cells in index order) every distinct node it writes through its constraint
 // lines, ascending, with the (local node, weight) entries that sum into it, in the host's order
 // (a stable sort of the cell's entries by node).  Here all entries of all hanging cells are
@@ -667,22 +677,24 @@ namespace {
 __global__ void tgt_count_kernel(const uint32_t *__restrict__ cell_nodes,
                                  const int32_t *__restrict__ node_line,
                                  const uint32_t *__restrict__ line_ptr, int64_t n_cells, int nloc,
-                                 uint32_t *hflag, uint32_t *ecount)
+                                 uint64_t *hmask, uint32_t *hflag, uint32_t *ecount)
 {
   for (int64_t c = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; c < n_cells;
        c += int64_t(gridDim.x) * blockDim.x) {
-    uint32_t cnt = 0, any = 0;
+    uint32_t cnt = 0;
+    uint64_t mask = 0;
     for (int e = 0; e < nloc; ++e) {
       const int32_t l = node_line[cell_nodes[size_t(c) * nloc + e]];
       if (l >= 0) {
-        any = 1;
+        mask |= 1ull << e;
         cnt += line_ptr[l + 1] - line_ptr[l];
       } else {
         cnt += 1;
       }
     }
-    hflag[c] = any;
-    ecount[c] = any ? cnt : 0u;
+    hmask[c] = mask;
+    hflag[c] = mask ? 1u : 0u;
+    ecount[c] = mask ? cnt : 0u;
   }
 }
 
@@ -793,6 +805,7 @@ int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream
   MF_HIP_TRY(keep_on_device(&A->d.line_col, A->line_col, st));
   MF_HIP_TRY(keep_on_device(&A->d.line_val, A->line_val, st));
   MF_HIP_TRY(alloc_kept(&A->d.cell_hidx, size_t(n_cells)));
+  MF_HIP_TRY(alloc_kept(&A->d.cell_hmask, size_t(n_cells)));
   DevBufs b;
   uint32_t *hflag, *hpos, *ecount, *eoff;
   MF_HIP_TRY(b.alloc(&hflag, size_t(n_cells) + 1));
@@ -805,8 +818,11 @@ int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream
   MF_HIP_TRY(hipMemsetAsync(ecount + n_cells, 0, sizeof(uint32_t), st));
   // a cell without hanging nodes reads an empty line table: node_line is then all -1
   hipLaunchKernelGGL(tgt_count_kernel, grid(n_cells), block, 0, st, A->d.cell_nodes, A->d.node_line,
-                     A->d.line_ptr, n_cells, nloc, hflag, ecount);
+                     A->d.line_ptr, n_cells, nloc, A->d.cell_hmask, hflag, ecount);
   MF_HIP_TRY(hipGetLastError());
+  A->cell_hmask.resize(size_t(n_cells));
+  MF_HIP_TRY(hipMemcpyAsync(A->cell_hmask.data(), A->d.cell_hmask, sizeof(uint64_t) * size_t(n_cells),
+                            hipMemcpyDeviceToHost, st));
   size_t tb = 0;
   MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, hflag, hpos, int(n_cells) + 1, st));
   char *tmp;

@@ -47,3 +47,11 @@ def test_device_setup_matches_host(name):
     assert torch.equal(ys[0], ys[1])
     # the condensed diagonal adds the hanging-cell slots with atomics: equal up to the order
     torch.testing.assert_close(ds[0], ds[1], rtol=1e-13, atol=0)
+
+
+def test_device_setup_rejects_unbalanced_mesh():
+    import meltpooldg_amd as mf
+    fine = [[2, 4 + i, j, k] for i in range(4) for j in range(4) for k in range(4)]
+    with pytest.raises(mf.MFError, match="2:1"):
+        mf.AdaptiveScratchData(2, (2, 1, 1), np.array([[0, 0, 0, 0]] + fine), hi=(2.0, 1.0, 1.0),
+                               stream=torch.cuda.current_stream())

@@ -76,3 +76,12 @@ def test_invalid_adaptive_meshes_are_rejected():
         mf.AdaptiveScratchData(2, 2, np.array([[0, 0, 0, 0], [0, 0, 0, 0]]))   # duplicate
     with pytest.raises(mf.MFError):
         mf.AdaptiveScratchData(2, 2, np.array([[0, 5, 0, 0]]))                  # outside
+    # a level-0 cell touching level-2 cells: hanging nodes would need a chain through a leaf
+    # that does not exist, and the class colouring would not be conflict-free
+    fine = [[2, 4 + i, j, k] for i in range(4) for j in range(4) for k in range(4)]
+    with pytest.raises(mf.MFError, match="2:1"):
+        mf.AdaptiveScratchData(2, (2, 1, 1), np.array([[0, 0, 0, 0]] + fine), hi=(2.0, 1.0, 1.0))
+    # a level-0 cell next to level-1 cells (one level apart) is accepted
+    mid = [[1, 2 + i, j, k] for i in range(4) for j in range(2) for k in range(2)]
+    s = mf.AdaptiveScratchData(2, (3, 1, 1), np.array([[0, 0, 0, 0]] + mid), hi=(3.0, 1.0, 1.0))
+    assert s.constraint_info()["n_hanging"] > 0
