@@ -82,9 +82,12 @@ int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq
 int mf_mesh_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
                         const double center[3], double radius, double half_width, int32_t *cells,
                         int64_t capacity, int64_t *n_cells);
-/* Where the adaptive-mesh setup runs: 1 (default) the node identification (DoF numbering,
- * support points, boundary flags) on the device when one is present, with the numbering of the
- * host path; 0 everything on the host.  No reference counterpart (deal.II numbers on the host). */
+/* Where the adaptive-mesh setup runs: 1 (default) on the device when one is present (band
+ * refinement and 2:1 balance, DoF numbering, support points, boundary flags, hanging-node
+ * constraint lines, their transposition and the hanging-cell tables), with the same cells,
+ * numbering and constraints as the host path; 0 everything on the host.  Chain resolution,
+ * colouring and the cell lists run on the host either way.  No reference counterpart (deal.II
+ * builds all of it on the host). */
 int mf_set_adaptive_setup(int device);
 /* Scratch on an adaptive mesh (cells as above, 2:1 balanced): DoF numbering in order of first
  * appearance over the cells, hanging-node constraints with chains resolved, and the default

@@ -240,7 +240,9 @@ def splitmix64(x):
 
 
 def set_adaptive_setup(device):
-    """1: node identification of adaptive meshes on the device when present (default), 0: host."""
+    """1 (default): the adaptive-mesh setup (band refinement, numbering, hanging-node
+    constraints, hanging-cell tables) on the device when one is present, 0: on the host; both
+    give the same mesh, numbering and constraints."""
     _check(lib().mf_set_adaptive_setup(int(bool(device))))
 
 

@@ -645,6 +645,17 @@ int mf_mesh_refine_band(const int ncell[3], const double lo[3], const double hi[
   }
   b.radius = radius;
   b.half_width = half_width;
+  int n_devices = 0;
+  if (g_adaptive_setup_device && hipGetDeviceCount(&n_devices) == hipSuccess && n_devices > 0) {
+    std::vector<int32_t> dev_cells;
+    const int rc = device_refine_band(ncell, lo, hi, levels, center, radius, half_width, dev_cells);
+    if (rc != MF_OK) return rc;
+    *n_cells = int64_t(dev_cells.size() / 4);
+    if (!cells) return MF_OK;
+    MF_REQUIRE(capacity >= *n_cells, MF_ERR_ARG, "cell buffer too small");
+    std::memcpy(cells, dev_cells.data(), sizeof(int32_t) * dev_cells.size());
+    return MF_OK;
+  }
   LeafSet L;
   for (int64_t z = 0; z < ncell[2]; ++z)
     for (int64_t y = 0; y < ncell[1]; ++y)

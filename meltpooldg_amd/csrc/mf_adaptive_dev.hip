@@ -892,4 +892,302 @@ int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream
   return MF_OK;
 }
 
+// Band refinement of the base mesh with 2:1 balance across faces, edges and vertices, the same
+// leaves in the same (Morton) order as the host path: every level, the leaves of that level
+// that meet the shell are split into their 8 children (a count, a scan and a fill); then, until
+// nothing changes, every leaf of level l >= 2 marks each neighbouring leaf of level < l - 1
+// (found by binary search of the radix-sorted leaf keys), and the marked leaves are split.
+namespace {
+struct BandArgs {
+  int ncell[3];
+  double lo[3], hi[3], center[3], radius, half_width;
+};
+
+__device__ __forceinline__ int key_level(uint64_t k) { return int(k >> 60); }
+__device__ __forceinline__ int64_t key_x(uint64_t k, int d)
+{
+  return int64_t((k >> (40 - 20 * d)) & 0xFFFFFull);
+}
+
+// whether the cell (l, ix) meets the shell |x - center| in (radius - half_width, radius +
+// half_width): the host's formula, uncontracted, so the decision is the host's bit for bit
+__device__ bool meets_band_d(const BandArgs &b, int l, const int64_t *ix)
+{
+#pragma clang fp contract(off)
+  double dmin2 = 0, dmax2 = 0;
+  for (int d = 0; d < 3; ++d) {
+    const double h = (b.hi[d] - b.lo[d]) / b.ncell[d] / double(int64_t(1) << l);
+    const double a = b.lo[d] + h * double(ix[d]), e = a + h;
+    const double nr = fmin(fmax(b.center[d], a), e);
+    dmin2 += (nr - b.center[d]) * (nr - b.center[d]);
+    const double fr = fabs(a - b.center[d]) > fabs(e - b.center[d]) ? a : e;
+    dmax2 += (fr - b.center[d]) * (fr - b.center[d]);
+  }
+  return sqrt(dmax2) > b.radius - b.half_width && sqrt(dmin2) < b.radius + b.half_width;
+}
+
+__global__ void base_leaves_kernel(BandArgs b, uint64_t *keys, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t x = i % b.ncell[0], y = (i / b.ncell[0]) % b.ncell[1], z = i / (int64_t(b.ncell[0]) * b.ncell[1]);
+    keys[i] = leaf_key_d(0, x, y, z);
+  }
+}
+
+__global__ void band_mark_kernel(BandArgs b, const uint64_t *keys, int64_t n, int lev, uint32_t *cnt)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    const int64_t ix[3] = {key_x(k, 0), key_x(k, 1), key_x(k, 2)};
+    cnt[i] = (key_level(k) == lev && meets_band_d(b, lev, ix)) ? 8u : 1u;
+  }
+}
+
+// cnt[i] = 8: split leaf i into its children at pos[i], else copy it
+__global__ void split_kernel(const uint64_t *keys, const uint32_t *cnt, const uint32_t *pos,
+                             int64_t n, uint64_t *out)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    if (cnt[i] == 1u) {
+      out[pos[i]] = k;
+      continue;
+    }
+    const int l = key_level(k);
+    for (int c = 0; c < 8; ++c)
+      out[pos[i] + c] = leaf_key_d(l + 1, 2 * key_x(k, 0) + (c & 1), 2 * key_x(k, 1) + ((c >> 1) & 1),
+                                   2 * key_x(k, 2) + (c >> 2));
+  }
+}
+
+__device__ int64_t find_key(const uint64_t *keys, int64_t n, uint64_t k)
+{
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && keys[lo] == k) ? lo : -1;
+}
+
+// sorted keys: cnt[j] = 8 for every leaf j that is two or more levels coarser than a leaf it
+// touches, else 1; *any = 1 when some leaf is marked
+__global__ void balance_mark_kernel(BandArgs b, const uint64_t *keys, int64_t n, uint32_t *cnt,
+                                    int *any)
+{
+  for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < n * 27;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = t / 27;
+    const int dir = int(t - i * 27);
+    const uint64_t k = keys[i];
+    const int l = key_level(k);
+    if (l < 2 || dir == 13) continue;
+    const int del[3] = {dir % 3 - 1, (dir / 3) % 3 - 1, dir / 9 - 1};
+    int64_t nb[3];
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) {
+      nb[d] = key_x(k, d) + del[d];
+      inside &= nb[d] >= 0 && nb[d] < (int64_t(b.ncell[d]) << l);
+    }
+    if (!inside) continue;
+    for (int lc = l; lc >= 0; --lc) {
+      const int sh = l - lc;
+      const int64_t j = find_key(keys, n, leaf_key_d(lc, nb[0] >> sh, nb[1] >> sh, nb[2] >> sh));
+      if (j >= 0) {
+        if (lc < l - 1) {
+          cnt[j] = 8u;
+          *any = 1;
+        }
+        break;
+      }
+    }
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t *x, uint32_t v, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void morton_kernel(const uint64_t *keys, int64_t n, int L, uint64_t *code)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    const int sh = L - key_level(k);
+    uint64_t c = 0;
+    for (int bit = 0; bit < 21; ++bit)
+      for (int d = 0; d < 3; ++d) c |= ((uint64_t(key_x(k, d) << sh) >> bit) & 1ull) << (3 * bit + d);
+    code[i] = c;
+  }
+}
+
+__global__ void cells_kernel(const uint64_t *keys, int64_t n, int32_t *cells)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const uint64_t k = keys[i];
+    cells[4 * i] = key_level(k);
+    for (int d = 0; d < 3; ++d) cells[4 * i + 1 + d] = int32_t(key_x(k, d));
+  }
+}
+
+__global__ void max_level_kernel(const uint64_t *keys, int64_t n, int *lmax)
+{
+  int m = 0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    m = max(m, key_level(keys[i]));
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(lmax, m);
+}
+}  // namespace
+
+int device_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
+                       const double center[3], double radius, double half_width,
+                       std::vector<int32_t> &cells_out)
+{
+  BandArgs b{};
+  for (int d = 0; d < 3; ++d) {
+    b.ncell[d] = ncell[d];
+    b.lo[d] = lo[d];
+    b.hi[d] = hi[d];
+    b.center[d] = center[d];
+  }
+  b.radius = radius;
+  b.half_width = half_width;
+  hipStream_t st = nullptr;
+  MF_HIP_TRY(hipStreamCreate(&st));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{st};
+  DevBufs b0;
+  const dim3 block(256);
+  auto grid = [](int64_t n) { return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)))); };
+  int64_t n = int64_t(ncell[0]) * ncell[1] * ncell[2];
+  // buffers that grow: reallocated (with the old ones freed) when a step needs more
+  struct Grow {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~Grow()
+    {
+      if (p) (void)hipFree(p);
+    }
+    hipError_t need(size_t nb)
+    {
+      if (nb <= bytes) return hipSuccess;
+      if (p) (void)hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      const hipError_t e = hipMalloc(&p, nb);
+      if (e == hipSuccess) bytes = nb;
+      return e;
+    }
+  } gkeys, gout, gcnt, gpos, gtmp, gcode;
+  int *any;
+  MF_HIP_TRY(b0.alloc(&any, 2));
+  MF_HIP_TRY(gkeys.need(sizeof(uint64_t) * size_t(n)));
+  hipLaunchKernelGGL(base_leaves_kernel, grid(n), block, 0, st, b, static_cast<uint64_t *>(gkeys.p), n);
+  MF_HIP_TRY(hipGetLastError());
+  // split the leaves with cnt = 8: scan, fill into gout, swap with gkeys
+  auto split = [&]() -> int {
+    uint32_t *cnt = static_cast<uint32_t *>(gcnt.p), *pos = static_cast<uint32_t *>(gpos.p);
+    size_t tb = 0;
+    MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, pos, int(n) + 1, st));
+    MF_HIP_TRY(gtmp.need(tb));
+    MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(gtmp.p, tb, cnt, pos, int(n) + 1, st));
+    uint32_t total = 0;
+    MF_HIP_TRY(hipMemcpyAsync(&total, pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    MF_REQUIRE(int64_t(total) < (int64_t(1) << 31), MF_ERR_ARG, "adaptive mesh too large");
+    MF_HIP_TRY(gout.need(sizeof(uint64_t) * size_t(total)));
+    hipLaunchKernelGGL(split_kernel, grid(n), block, 0, st, static_cast<const uint64_t *>(gkeys.p), cnt,
+                       pos, n, static_cast<uint64_t *>(gout.p));
+    MF_HIP_TRY(hipGetLastError());
+    std::swap(gkeys.p, gout.p);
+    std::swap(gkeys.bytes, gout.bytes);
+    n = int64_t(total);
+    return MF_OK;
+  };
+  for (int lev = 0; lev < levels; ++lev) {
+    MF_HIP_TRY(gcnt.need(sizeof(uint32_t) * (size_t(n) + 1)));
+    MF_HIP_TRY(gpos.need(sizeof(uint32_t) * (size_t(n) + 1)));
+    MF_HIP_TRY(hipMemsetAsync(static_cast<uint32_t *>(gcnt.p) + n, 0, sizeof(uint32_t), st));
+    hipLaunchKernelGGL(band_mark_kernel, grid(n), block, 0, st, b, static_cast<const uint64_t *>(gkeys.p),
+                       n, lev, static_cast<uint32_t *>(gcnt.p));
+    MF_HIP_TRY(hipGetLastError());
+    const int rc = split();
+    if (rc != MF_OK) return rc;
+  }
+  // sort the leaf keys (into gout, then swapped back)
+  auto sort_keys = [&]() -> int {
+    MF_HIP_TRY(gout.need(sizeof(uint64_t) * size_t(n)));
+    size_t tb = 0;
+    MF_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, static_cast<const uint64_t *>(gkeys.p),
+                                                 static_cast<uint64_t *>(gout.p), int(n), 0, 64, st));
+    MF_HIP_TRY(gtmp.need(tb));
+    MF_HIP_TRY(hipcub::DeviceRadixSort::SortKeys(gtmp.p, tb, static_cast<const uint64_t *>(gkeys.p),
+                                                 static_cast<uint64_t *>(gout.p), int(n), 0, 64, st));
+    std::swap(gkeys.p, gout.p);
+    std::swap(gkeys.bytes, gout.bytes);
+    return MF_OK;
+  };
+  for (int pass = 0;; ++pass) {
+    MF_REQUIRE(pass < 64, MF_ERR_ARG, "2:1 balance does not terminate");
+    int rc = sort_keys();
+    if (rc != MF_OK) return rc;
+    MF_HIP_TRY(gcnt.need(sizeof(uint32_t) * (size_t(n) + 1)));
+    MF_HIP_TRY(gpos.need(sizeof(uint32_t) * (size_t(n) + 1)));
+    hipLaunchKernelGGL(fill_u32_kernel, grid(n), block, 0, st, static_cast<uint32_t *>(gcnt.p), 1u, n);
+    MF_HIP_TRY(hipGetLastError());
+    MF_HIP_TRY(hipMemsetAsync(static_cast<uint32_t *>(gcnt.p) + n, 0, sizeof(uint32_t), st));
+    MF_HIP_TRY(hipMemsetAsync(any, 0, sizeof(int), st));
+    hipLaunchKernelGGL(balance_mark_kernel, grid(n * 27), block, 0, st, b,
+                       static_cast<const uint64_t *>(gkeys.p), n, static_cast<uint32_t *>(gcnt.p), any);
+    MF_HIP_TRY(hipGetLastError());
+    int h_any = 0;
+    MF_HIP_TRY(hipMemcpyAsync(&h_any, any, sizeof(int), hipMemcpyDeviceToHost, st));
+    MF_HIP_TRY(hipStreamSynchronize(st));
+    if (!h_any) break;
+    rc = split();
+    if (rc != MF_OK) return rc;
+  }
+  // Morton order of the cell origins on the finest lattice
+  MF_HIP_TRY(hipMemsetAsync(any + 1, 0, sizeof(int), st));
+  hipLaunchKernelGGL(max_level_kernel, grid(n), block, 0, st, static_cast<const uint64_t *>(gkeys.p), n,
+                     any + 1);
+  MF_HIP_TRY(hipGetLastError());
+  int L = 0;
+  MF_HIP_TRY(hipMemcpyAsync(&L, any + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  MF_HIP_TRY(gcode.need(sizeof(uint64_t) * 2 * size_t(n)));
+  uint64_t *code = static_cast<uint64_t *>(gcode.p), *code_s = code + n;
+  hipLaunchKernelGGL(morton_kernel, grid(n), block, 0, st, static_cast<const uint64_t *>(gkeys.p), n, L,
+                     code);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(gout.need(sizeof(uint64_t) * size_t(n)));
+  size_t tb = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, code, code_s, static_cast<const uint64_t *>(gkeys.p),
+                                                static_cast<uint64_t *>(gout.p), int(n), 0, 63, st));
+  MF_HIP_TRY(gtmp.need(tb));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(gtmp.p, tb, code, code_s, static_cast<const uint64_t *>(gkeys.p),
+                                                static_cast<uint64_t *>(gout.p), int(n), 0, 63, st));
+  int32_t *dcells;
+  MF_HIP_TRY(b0.alloc(&dcells, 4 * size_t(n)));
+  hipLaunchKernelGGL(cells_kernel, grid(n), block, 0, st, static_cast<const uint64_t *>(gout.p), n, dcells);
+  MF_HIP_TRY(hipGetLastError());
+  cells_out.resize(4 * size_t(n));
+  MF_HIP_TRY(hipMemcpyAsync(cells_out.data(), dcells, sizeof(int32_t) * 4 * size_t(n),
+                            hipMemcpyDeviceToHost, st));
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
 }  // namespace mf

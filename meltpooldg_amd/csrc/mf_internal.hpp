@@ -201,6 +201,11 @@ int device_hanging_constraints(const ::mf_scratch *s, const int32_t *cells, int6
 // (cell_hidx, ht_ptr, tgt_node, tgt_ptr, ent_local, ent_w, ent_w32, slot_cell, slot_tgt) with
 // A->slot_start; uploads and keeps cell_nodes, node_line and the constraint lines there too
 int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream_t st);
+// band-refined, 2:1-balanced leaves in Morton order on the device (mf_adaptive_dev.hip), the
+// host path's cells: [n][4] level, ix, iy, iz
+int device_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
+                       const double center[3], double radius, double half_width,
+                       std::vector<int32_t> &cells_out);
 // 1: adaptive setup steps run on the device when one is present (default), 0: on the host
 extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);

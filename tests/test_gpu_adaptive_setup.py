@@ -1,6 +1,7 @@
-"""Device adaptive setup (mf_adaptive_dev.hip: node identification, hanging-node constraint
-lines, their transposition) against the host path: the same DoF numbering, support points,
-boundary flags, hanging-node constraints, Dirichlet rows and operator (bit for bit)."""
+"""Device adaptive setup (mf_adaptive_dev.hip: band refinement with 2:1 balance, node
+identification, hanging-node constraint lines, their transposition, the hanging-cell tables)
+against the host path: the same cells, DoF numbering, support points, boundary flags,
+hanging-node constraints, Dirichlet rows and operator (bit for bit)."""
 import numpy as np
 import pytest
 
@@ -15,12 +16,39 @@ CASES = {
 }
 
 
+REFINE = [
+    ((32, 32, 32), 2, (0.5, 0.5, 0.5), 0.3, 0.05, (1.0, 1.0, 1.0)),
+    ((5, 7, 6), 3, (0.2, 0.3, 0.25), 0.3, 0.04, (1.0, 1.3, 0.8)),
+    ((4, 4, 4), 4, (0.1, 0.1, 0.1), 0.2, 0.03, (1.0, 1.0, 1.0)),
+    ((3, 3, 3), 0, (0.5, 0.5, 0.5), 0.3, 0.05, (1.0, 1.0, 1.0)),
+    ((6, 6, 6), 2, (5.0, 5.0, 5.0), 0.1, 0.01, (1.0, 1.0, 1.0)),   # band misses the box
+]
+
+
+@pytest.mark.parametrize("case", REFINE)
+def test_device_refinement_matches_host(case):
+    import meltpooldg_amd as mf
+    nc, lev, ce, r, hw, hi = case
+    try:
+        mf.set_adaptive_setup(0)
+        ch = mf.refine_band_cells(nc, lev, ce, r, hw, hi=hi)
+    finally:
+        mf.set_adaptive_setup(1)
+    cd = mf.refine_band_cells(nc, lev, ce, r, hw, hi=hi)
+    assert cd.shape == ch.shape
+    assert np.array_equal(cd, ch)
+
+
 @pytest.mark.parametrize("name", list(CASES))
 def test_device_setup_matches_host(name):
     import meltpooldg_amd as mf
     degree, nc, lev, ce, r, hw, hi = CASES[name]
-    cells = mf.refine_band_cells(nc, lev, ce, r, hw, hi=hi)
     st = torch.cuda.current_stream()
+    try:
+        mf.set_adaptive_setup(0)
+        cells = mf.refine_band_cells(nc, lev, ce, r, hw, hi=hi)
+    finally:
+        mf.set_adaptive_setup(1)
     try:
         mf.set_adaptive_setup(0)
         sh = mf.AdaptiveScratchData(degree, nc, cells, hi=hi, stream=st)
