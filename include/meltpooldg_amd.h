@@ -391,6 +391,55 @@ int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result);
 int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double radius,
                                            const double position[3], double *dq_dr);
 
+/* PredictorType (include/meltpooldg/linear_algebra/predictor_data.hpp:10-19; least-squares
+ * projection not built). */
+enum { MF_PREDICTOR_NONE = 0, MF_PREDICTOR_ZERO = 1, MF_PREDICTOR_LINEAR_EXTRAPOLATION = 2 };
+
+/* One heat time step of the mp-heat-transfer application on the device: the loop body of
+ * HeatTransferApplication::run (applications/mp-heat-transfer/heat_transfer_application.cpp:
+ * 52-120) for the heat-diffuse operator with a Gaussian volumetric laser.  A step advances the
+ * time (TimeIterator::compute_next_time_increment, source/time_integration/time_iterator.cpp:
+ * 44-68), moves the laser by scan_speed * dt along the normalised scan direction
+ * (LaserOperation::move_laser, source/heat/laser_operation.cpp:236-250), sets the heat source
+ * to the volumetric profile at the new position (compute_heat_source, :316-333), resets the
+ * operator's time increment, applies the predictor and commits the history (Predictor::vmult,
+ * include/meltpooldg/linear_algebra/predictor.hpp:119-161) and runs the Newton solve
+ * (HeatDiffuseOperation::solve, source/heat/heat_diffuse_operation.cpp:214-290).  The linear
+ * predictor uses the reference's increments: the old increment is 0 at the first step (the
+ * predictor is then the current solution) and equal to the new one afterwards. */
+typedef struct {
+  int laser;                 /* 1: Gaussian volumetric laser source, 0: no heat source */
+  double power;              /* LaserData::power */
+  double radius;             /* LaserData::radius */
+  double start[3];           /* LaserData::starting_position */
+  double scan_direction[3];  /* LaserData::scan_direction (normalised by the library) */
+  double scan_speed;         /* LaserData::scan_speed */
+  double start_time;         /* TimeSteppingData::start_time */
+  int predictor;             /* MF_PREDICTOR_* (HeatData default: linear extrapolation) */
+  mf_solver_desc solver;     /* linear solver of each Newton iteration */
+  mf_newton_desc newton;
+} mf_heat_stepper_desc;
+
+typedef struct mf_heat_stepper mf_heat_stepper;
+
+/* The stepper keeps the current and recent old temperature and the heat source on the device;
+ * T0 (device, n_dofs) is copied.  The operator's linearisation field points to the stepper's
+ * current temperature after each step. */
+int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const double *T0,
+                           mf_heat_stepper **out);
+int mf_heat_stepper_destroy(mf_heat_stepper *h);
+/* One step of size dt; dirichlet_values: values of the constrained DoFs (NULL = homogeneous).
+ * Failure: MF_NO_CONVERGENCE (reference ExcHeatTransferNoConvergence). */
+int mf_heat_stepper_step(mf_heat_stepper *h, double dt, const double *dirichlet_values,
+                         mf_newton_report *report);
+/* Device pointers to the stepper's vectors (valid until the next step or destroy), the current
+ * time and the laser position; any output may be NULL. */
+int mf_heat_stepper_state(const mf_heat_stepper *h, const double **T, const double **T_old,
+                          const double **heat_source, double *time, double position[3]);
+/* Copies the current temperature (which = 0), the recent old temperature (1) or the heat
+ * source (2) into dst (device, n_dofs) on the scratch's stream. */
+int mf_heat_stepper_copy(const mf_heat_stepper *h, int which, double *dst);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Differentiation of the implicit heat transient (no reference counterpart: new).             */
 /* ------------------------------------------------------------------------------------------ */

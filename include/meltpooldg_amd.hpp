@@ -307,4 +307,44 @@ struct LinearSolver {
   }
 };
 
+// The heat time loop of the mp-heat-transfer application (HeatTransferApplication::run,
+// applications/mp-heat-transfer/heat_transfer_application.cpp:52-120) on the device: each
+// step() moves the laser, recomputes the volumetric source, applies the predictor and runs
+// the Newton solve; the solution history stays in HBM.
+class HeatTimeStepper {
+ public:
+  HeatTimeStepper(HeatDiffuseOperator &op, const mf_heat_stepper_desc &d,
+                  const DeviceVector<double> &T0)
+  {
+    check(mf_heat_stepper_create(op.handle(), &d, T0.data(), &h_));
+  }
+  HeatTimeStepper(const HeatTimeStepper &) = delete;
+  HeatTimeStepper &operator=(const HeatTimeStepper &) = delete;
+  ~HeatTimeStepper() { mf_heat_stepper_destroy(h_); }
+  // throws NoConvergence (ExcHeatTransferNoConvergence)
+  mf_newton_report step(double dt, const DeviceVector<double> *dirichlet_values = nullptr)
+  {
+    mf_newton_report rep{};
+    check(mf_heat_stepper_step(h_, dt, dirichlet_values ? dirichlet_values->data() : nullptr,
+                               &rep));
+    return rep;
+  }
+  // device pointer to the current temperature (valid until the next step)
+  const double *solution() const
+  {
+    const double *T = nullptr;
+    check(mf_heat_stepper_state(h_, &T, nullptr, nullptr, nullptr, nullptr));
+    return T;
+  }
+  double time() const
+  {
+    double t = 0.0;
+    check(mf_heat_stepper_state(h_, nullptr, nullptr, nullptr, &t, nullptr));
+    return t;
+  }
+
+ private:
+  mf_heat_stepper *h_ = nullptr;
+};
+
 }  // namespace mf

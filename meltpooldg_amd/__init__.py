@@ -71,6 +71,16 @@ class NewtonDesc(C.Structure):
                 ("correction_tolerance", C.c_double)]
 
 
+PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR_EXTRAPOLATION = 0, 1, 2
+
+
+class HeatStepperDesc(C.Structure):
+    _fields_ = [("laser", C.c_int), ("power", C.c_double), ("radius", C.c_double),
+                ("start", C.c_double * 3), ("scan_direction", C.c_double * 3),
+                ("scan_speed", C.c_double), ("start_time", C.c_double), ("predictor", C.c_int),
+                ("solver", SolverDesc), ("newton", NewtonDesc)]
+
+
 class NewtonReport(C.Structure):
     _fields_ = [("iterations", C.c_int), ("linear_iterations", C.c_int),
                 ("residual_norm", C.c_double), ("correction_norm", C.c_double),
@@ -221,6 +231,12 @@ def lib():
         L.mf_heat_conductivity_gradient.argtypes = [vp, vp, vp, C.c_double, vp]
         L.mf_laser_heat_source_radius_derivative.argtypes = [vp, C.c_double, C.c_double,
                                                              C.POINTER(C.c_double), vp]
+        L.mf_heat_stepper_create.argtypes = [vp, C.POINTER(HeatStepperDesc), vp, C.POINTER(vp)]
+        L.mf_heat_stepper_destroy.argtypes = [vp]
+        L.mf_heat_stepper_copy.argtypes = [vp, C.c_int, vp]
+        L.mf_heat_stepper_step.argtypes = [vp, C.c_double, vp, C.POINTER(NewtonReport)]
+        L.mf_heat_stepper_state.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.mf_transient_create.argtypes = [vp, C.POINTER(TransientDesc), C.POINTER(vp)]
         L.mf_transient_destroy.argtypes = [vp]
         L.mf_transient_forward.argtypes = [vp, vp, vp]
@@ -685,6 +701,83 @@ def conductivity_gradient(operator, lam, T, grad_k_cell, scale=1.0):
 def laser_heat_source_radius_derivative(scratch, dq, power, radius, position):
     pos = (C.c_double * 3)(*position)
     _check(lib().mf_laser_heat_source_radius_derivative(scratch._h, power, radius, pos, _ptr(dq)))
+
+
+class HeatTimeStepper:
+    """The heat time loop of the mp-heat-transfer application on the device
+    (HeatTransferApplication::run, heat_transfer_application.cpp:52-120, with the heat-diffuse
+    operator and a Gaussian volumetric laser; mf_heat_stepper_*): each step() advances the
+    time, moves the laser, recomputes the heat source, applies the predictor and runs the
+    Newton solve.  T0 is copied; solution(), old_solution() and heat_source() return copies."""
+
+    def __init__(self, operator, T0, laser=None, predictor=PREDICTOR_LINEAR_EXTRAPOLATION,
+                 start_time=0.0, newton=None, **solver):
+        import torch
+        self.operator = operator
+        self.n_dofs = operator.scratch.n_dofs
+        d = HeatStepperDesc()
+        if laser is not None:
+            d.laser = 1
+            d.power = laser["power"]
+            d.radius = laser["radius"]
+            d.start = (C.c_double * 3)(*laser["start"])
+            d.scan_direction = (C.c_double * 3)(*laser.get("scan_direction", (1.0, 0.0, 0.0)))
+            d.scan_speed = laser.get("scan_speed", 0.0)
+        d.start_time = start_time
+        d.predictor = predictor
+        d.solver = _solver_desc(**solver)
+        d.newton = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
+        self._desc = d
+        h = C.c_void_p()
+        _check(lib().mf_heat_stepper_create(operator._h, C.byref(d), _ptr(T0), C.byref(h)))
+        self._h = h
+        self._torch = torch
+
+    def step(self, dt, dirichlet_values=None):
+        rep = NewtonReport()
+        code = lib().mf_heat_stepper_step(self._h, float(dt), _ptr(dirichlet_values),
+                                          C.byref(rep))
+        _check(code)
+        return rep.as_dict()
+
+    def _state(self):
+        T, To, q = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        t = C.c_double()
+        pos = (C.c_double * 3)()
+        _check(lib().mf_heat_stepper_state(self._h, C.byref(T), C.byref(To), C.byref(q),
+                                           C.byref(t), pos))
+        return T.value, To.value, q.value, t.value, tuple(pos)
+
+    def _copy(self, which):
+        torch = self._torch
+        out = torch.empty(self.n_dofs, dtype=torch.float64, device="cuda")
+        _check(lib().mf_heat_stepper_copy(self._h, which, _ptr(out)))
+        return out
+
+    def solution(self):
+        return self._copy(0)
+
+    def old_solution(self):
+        return self._copy(1)
+
+    def heat_source(self):
+        return self._copy(2)
+
+    @property
+    def time(self):
+        return self._state()[3]
+
+    @property
+    def laser_position(self):
+        return self._state()[4]
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().mf_heat_stepper_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
 
 
 class HeatTransient:

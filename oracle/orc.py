@@ -446,3 +446,73 @@ def refine_band_cells(ncell, levels, center, radius, half_width, lo=(0.0, 0.0, 0
                     leaves.add((c[0] + 1, 2 * c[1] + (d & 1), 2 * c[2] + ((d >> 1) & 1),
                                 2 * c[3] + (d >> 2)))
     return np.array(sorted(leaves), dtype=np.int32)
+
+
+def laser_volumetric_source(X, power, radius, position):
+    """Nodal values of the Gaussian volumetric laser source (LaserHeatSourceVolumetric::
+    compute_volumetric_heat_source, source/heat/laser_heat_source_volumetric.cpp:20-58, with the
+    Gaussian profile of include/meltpooldg/heat/laser_intensity_profiles.hpp:175-218):
+    P / (r sqrt(pi/2))^3 exp(-2 |x - x_L|^2 / r^2).  Its integral over R^3 is P."""
+    d2 = ((np.asarray(X) - np.asarray(position, dtype=np.float64)) ** 2).sum(axis=1)
+    return power / (radius * np.sqrt(np.pi / 2)) ** 3 * np.exp(-2.0 * d2 / radius ** 2)
+
+
+def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
+                    dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12):
+    """The heat time loop of the mp-heat-transfer application, restated on the oracle's
+    operator (HeatTransferApplication::run, applications/mp-heat-transfer/
+    heat_transfer_application.cpp:52-120, heat-diffuse operator, volumetric laser):
+      - time increment: the old increment is 0 at the first step and equal to the new one
+        afterwards (TimeIterator::compute_next_time_increment, time_iterator.cpp:44-68);
+      - LaserOperation::move_laser (source/heat/laser_operation.cpp:236-250): position +=
+        normalised scan direction * scan speed * dt, then the volumetric source there;
+      - Predictor::vmult (include/meltpooldg/linear_algebra/predictor.hpp:119-161,
+        source/linear_algebra/predictor_linear.cpp:76-96), then the history commit;
+      - NewtonRaphsonSolver::solve (source/linear_algebra/newton_raphson_solver.cpp:22-80) with
+        the residual of HeatDiffuseOperation (heat_diffuse_operation.cpp:86-116): r = rhs(T),
+        A d = r by Jacobi-GMRES (T-dependent law) or CG, stop when |r| and |d| are below the
+        tolerances, else T += d.
+    heat: HeatParams (dt is set per step); laser: dict(power, radius, start, scan_direction,
+    scan_speed).  Returns the temperature after every step."""
+    T = np.array(T0, dtype=np.float64)
+    T_old = T.copy()
+    X = node_coords(mesh)
+    pos = np.array(laser["start"], dtype=np.float64) if laser else None
+    if laser:
+        sd = np.asarray(laser.get("scan_direction", (1.0, 0.0, 0.0)), dtype=np.float64)
+        sd = sd / np.linalg.norm(sd)
+    out = []
+    for step, dt in enumerate(dts, start=1):
+        dt_old = dt if step > 1 else 0.0
+        q = None
+        if laser:
+            if laser.get("scan_speed", 0.0) != 0.0:
+                pos = pos + sd * laser["scan_speed"] * dt
+            q = laser_volumetric_source(X, laser["power"], laser["radius"], pos)
+        if predictor == "none":
+            P = T.copy()
+        elif predictor == "zero":
+            P = np.zeros_like(T)
+        else:
+            P = T.copy() if abs(dt_old) < 1e-12 else (1 + dt / dt_old) * T - (dt / dt_old) * T_old
+        T_old, T = T, P
+        if dirichlet_values is not None:
+            T[mask == 1] = np.asarray(dirichlet_values)[mask == 1]
+        hp = HeatParams(*(getattr(heat, f) for f, _ in HeatParams._fields_))
+        hp.dt = dt
+        converged = False
+        for _ in range(newton[0]):
+            op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old)
+            r = op.create_rhs(T_old, q)
+            d, _, _, st = op.solve(r, solver=GMRES if hp.t_dependent else CG, rel_tol=rel_tol,
+                                   inv_diag=op.inverse_diagonal())
+            if st != SUCCESS:
+                raise RuntimeError("oracle linear solve failed")
+            if np.linalg.norm(r) < newton[1] and np.linalg.norm(d) < newton[2]:
+                converged = True
+                break
+            T = T + d
+        if not converged:
+            raise RuntimeError("oracle Newton did not converge")
+        out.append(T.copy())
+    return out

@@ -2,6 +2,7 @@
 declares, and its host logic (mesh setup, 1D tables, argument validation, synthetic inputs)
 agrees with the oracle.  No GPU compute is called here."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -173,3 +174,23 @@ def test_vtu_and_restart_without_fields(tmp_path):
     bad.write_bytes(b"not a restart file")
     h = C.c_void_p()
     assert L.mf_restart_load_scratch(str(bad).encode(), C.byref(h)) == mf.MF_ERR_ARG
+
+
+def test_cpp_header_compiles(tmp_path):
+    """include/meltpooldg_amd.hpp (the C++ layer over the C ABI) parses and type-checks:
+    its classes are inline, so including it checks them; the vector template is instantiated
+    explicitly."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "use_hpp.cpp"
+    src.write_text('#include "meltpooldg_amd.hpp"\n'
+                   "template class mf::DeviceVector<double>;\n"
+                   "template class mf::DeviceVector<float>;\n")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    r = subprocess.run([hipcc, "-fsyntax-only", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+                        "-I/opt/rocm/include", "-I" + inc, str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]

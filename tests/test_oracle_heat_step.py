@@ -1,0 +1,72 @@
+"""Pins of the oracle's heat time loop (orc.heat_time_steps) and its laser source: the source
+integrates to the laser power, the loop reproduces the oracle's implicit-Euler heat steps
+(pinned by the golden outputs) for a linear law whatever the predictor, keeps a steady state,
+and its steps with a source satisfy the discrete step equation."""
+import numpy as np
+import pytest
+
+import orc
+
+
+def _mass_integral(mesh, q):
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, rho_cp=1.0, k=0.0, dt=1.0)
+    op = orc.OracleOperator(orc.OP_HEAT, mesh, heat=hp, use_mask=0)
+    return float(op.vmult(q).sum())          # 1^T M q = integral of the interpolant
+
+
+def test_laser_source_integrates_to_power():
+    mesh = orc.make_mesh(3, 3, (12, 12, 12))
+    X = orc.node_coords(mesh)
+    q = orc.laser_volumetric_source(X, 3.0, 0.12, (0.5, 0.45, 0.55))
+    assert abs(_mass_integral(mesh, q) / 3.0 - 1.0) < 2e-3
+    # a beam cut in half by the top face carries half the power
+    q = orc.laser_volumetric_source(X, 3.0, 0.12, (0.5, 0.45, 1.0))
+    assert abs(_mass_integral(mesh, q) / 1.5 - 1.0) < 2e-3
+
+
+@pytest.mark.parametrize("predictor", ["linear", "none", "zero"])
+def test_linear_law_matches_implicit_euler(predictor):
+    mesh = orc.make_mesh(3, 2, (4, 3, 3))
+    mask = orc.boundary_mask(mesh)
+    T0 = 1.0 + orc.random_field(mesh, 5)
+    T0[mask == 1] = 0.0
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, rho_cp=1.3, k=0.7, dt=1e-2)
+    got = orc.heat_time_steps(mesh, hp, mask, T0, [1e-2] * 3, predictor=predictor)
+    ref = orc.OracleOperator(orc.OP_HEAT, mesh, heat=hp, mask=mask)
+    T_ref, _, st = ref.heat_implicit_euler(T0, 3)
+    assert st == 0
+    assert np.max(np.abs(got[-1] - T_ref)) < 1e-9 * np.max(np.abs(T0))
+
+
+def test_steady_state_is_kept():
+    mesh = orc.make_mesh(3, 2, (3, 3, 3))
+    mask = orc.boundary_mask(mesh)
+    T0 = np.full(orc.n_dofs(mesh), 350.0)
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=400.0, tanh_width=50.0, dt=1e-2)
+    out = orc.heat_time_steps(mesh, hp, mask, T0, [1e-2, 2e-2], dirichlet_values=T0)
+    assert np.max(np.abs(out[-1] - 350.0)) < 1e-9
+
+
+def test_step_with_laser_satisfies_step_equation():
+    mesh = orc.make_mesh(3, 2, (4, 4, 3))
+    mask = orc.boundary_mask(mesh)
+    T0 = np.full(orc.n_dofs(mesh), 300.0)
+    hp = orc.HeatParams(law=orc.LAW_CONSTANT, rho_cp=1.0, k=1.0, dt=1e-2)
+    laser = dict(power=50.0, radius=0.3, start=(0.3, 0.5, 0.7), scan_direction=(2.0, 0.0, 0.0),
+                 scan_speed=5.0)
+    out = orc.heat_time_steps(mesh, hp, mask, T0, [1e-2, 1e-2], laser=laser, dirichlet_values=T0)
+    # the second step: the source at start + 2 dt speed e_x (direction normalised)
+    X = orc.node_coords(mesh)
+    q = orc.laser_volumetric_source(X, 50.0, 0.3, (0.3 + 2 * 1e-2 * 5.0, 0.5, 0.7))
+    op = orc.OracleOperator(orc.OP_HEAT, mesh, heat=hp, mask=mask, T=out[1], T_old=out[0])
+    r = op.create_rhs(out[0], q)
+    # against the residual of the step's start (T = T_old, i.e. the source term alone)
+    op0 = orc.OracleOperator(orc.OP_HEAT, mesh, heat=hp, mask=mask, T=out[0], T_old=out[0])
+    r0 = op0.create_rhs(out[0], q)
+    assert np.linalg.norm(r) < 1e-9 * np.linalg.norm(r0)
+    # a source left at the first step's position does not satisfy it
+    q1 = orc.laser_volumetric_source(X, 50.0, 0.3, (0.3 + 1e-2 * 5.0, 0.5, 0.7))
+    assert np.linalg.norm(op.create_rhs(out[0], q1)) > 1e-3 * np.linalg.norm(r0)
+    # the source heats: the interior warms above the boundary value
+    assert out[1][mask == 0].max() > 300.5
