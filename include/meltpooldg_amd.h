@@ -113,6 +113,15 @@ int mf_scratch_constraint_lines(const mf_scratch *s, int32_t *node_line, uint32_
                                 uint32_t *line_col, double *line_val);
 /* AffineConstraints::distribute on a device vector: x_h = sum_m w_hm x_m on hanging nodes */
 int mf_distribute_constraints(mf_scratch *s, double *x);
+/* Field transfer between two adaptive meshes of the same base mesh and degree (the remesh step
+ * of AMR::refine_grid, source/utilities/amr.cpp:59-112, with dealii::SolutionTransfer for
+ * FE_Q): every node of the new mesh takes the value of the old field (its hanging values
+ * first distributed) at its support point, found in the old leaf that contains it; then the
+ * new mesh's hanging constraints are distributed.  Exact for fields of the new space that the
+ * old mesh represents (refinement); interpolation from the finer old cells on coarsening.
+ * old_vec and new_vec are device vectors of the two meshes' sizes. */
+int mf_adaptive_transfer(mf_scratch *old_s, const double *old_vec, mf_scratch *new_s,
+                         double *new_vec);
 
 /* ------------------------------------------------------------------------------------------ */
 /* Operators.  Replaces OperatorMatrixFree<dim,double> (include/meltpooldg/core/              */

@@ -231,6 +231,7 @@ def lib():
         L.mf_heat_conductivity_gradient.argtypes = [vp, vp, vp, C.c_double, vp]
         L.mf_laser_heat_source_radius_derivative.argtypes = [vp, C.c_double, C.c_double,
                                                              C.POINTER(C.c_double), vp]
+        L.mf_adaptive_transfer.argtypes = [vp, vp, vp, vp]
         L.mf_heat_stepper_create.argtypes = [vp, C.POINTER(HeatStepperDesc), vp, C.POINTER(vp)]
         L.mf_heat_stepper_destroy.argtypes = [vp]
         L.mf_heat_stepper_copy.argtypes = [vp, C.c_int, vp]
@@ -382,6 +383,13 @@ class ScratchData:
                 self._h = None
         except Exception:
             pass
+
+
+def adaptive_transfer(old_scratch, old_vec, new_scratch, new_vec):
+    """Field transfer between two adaptive meshes of the same base mesh and degree
+    (mf_adaptive_transfer: the remesh step of AMR::refine_grid with SolutionTransfer)."""
+    _check(lib().mf_adaptive_transfer(old_scratch._h, _ptr(old_vec), new_scratch._h,
+                                      _ptr(new_vec)))
 
 
 def refine_band_cells(ncell, levels, center, radius, half_width, lo=(0.0, 0.0, 0.0),

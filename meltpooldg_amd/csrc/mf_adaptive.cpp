@@ -795,6 +795,38 @@ int mf_distribute_constraints(mf_scratch *s, double *x)
                            s->ad->d.line_val, x, s->mesh.n_dofs, s->stream);
 }
 
+int mf_adaptive_transfer(mf_scratch *old_s, const double *old_vec, mf_scratch *new_s,
+                         double *new_vec)
+{
+  MF_REQUIRE(old_s && old_vec && new_s && new_vec, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(old_s->ad && new_s->ad, MF_ERR_ARG, "the transfer needs two adaptive scratches");
+  const MeshInfo &a = old_s->mesh, &b = new_s->mesh;
+  bool same = a.degree == b.degree;
+  for (int d = 0; d < 3; ++d)
+    same = same && a.ncell[d] == b.ncell[d] && a.lo[d] == b.lo[d] && a.h[d] == b.h[d];
+  MF_REQUIRE(same, MF_ERR_ARG, "the meshes must share the base mesh and the degree");
+  int rc = ensure_adaptive_on_device(old_s);
+  if (rc) return rc;
+  if ((rc = ensure_adaptive_on_device(new_s))) return rc;
+  // the old field made constraint-consistent (hanging values from their masters)
+  double *tmp = nullptr;
+  const size_t bytes = sizeof(double) * size_t(a.n_dofs);
+  MF_HIP_TRY(hipStreamSynchronize(new_s->stream));
+  MF_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&tmp), bytes));
+  rc = hipMemcpyAsync(tmp, old_vec, bytes, hipMemcpyDeviceToDevice, old_s->stream) == hipSuccess
+           ? MF_OK
+           : MF_ERR_HIP;
+  if (rc == MF_OK) rc = mf_distribute_constraints(old_s, tmp);
+  if (rc == MF_OK && hipStreamSynchronize(old_s->stream) != hipSuccess) rc = MF_ERR_HIP;
+  if (rc == MF_OK) rc = device_transfer_field(old_s, tmp, new_s, new_vec);
+  (void)hipFree(tmp);
+  if (rc != MF_OK) {
+    if (rc == MF_ERR_HIP) set_error("device error in the field transfer");
+    return rc;
+  }
+  return mf_distribute_constraints(new_s, new_vec);
+}
+
 int mf_scratch_constraint_lines(const mf_scratch *s, int32_t *node_line, uint32_t *line_ptr,
                                 uint32_t *line_col, double *line_val)
 {

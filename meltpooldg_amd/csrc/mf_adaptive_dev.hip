@@ -1190,4 +1190,110 @@ int device_refine_band(const int ncell[3], const double lo[3], const double hi[3
   return MF_OK;
 }
 
+// Transfer of a finite-element field between two adaptive meshes of the same base mesh and
+// degree: every node of the new mesh takes the value of the old (constraint-consistent) field
+// at its support point, found in the old leaf that contains it.  For FE_Q this is what
+// SolutionTransfer does on refinement (prolongation interpolates the parent's function) and
+// on coarsening (the parent's nodal values are the children's function at its nodes); the
+// caller distributes the new mesh's constraints afterwards, as the reference's post() does.
+namespace {
+struct TransferArgs {
+  int p, K, L;
+  int ncell[3];
+  double lo[3], h[3], gl[KMAX];
+};
+
+__global__ void transfer_eval_kernel(const TransferArgs a, const uint64_t *__restrict__ keys,
+                                     const uint32_t *__restrict__ idx, int64_t n_leaves,
+                                     const uint32_t *__restrict__ cell_nodes,
+                                     const double *__restrict__ val, const double *__restrict__ xyz,
+                                     int64_t n, double *__restrict__ out)
+{
+  const int nloc = a.K * a.K * a.K;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double u[3];
+    for (int d = 0; d < 3; ++d) u[d] = (xyz[3 * i + d] - a.lo[d]) / a.h[d];  // base-cell units
+    int64_t c = -1, ix[3] = {0, 0, 0};
+    int lev = 0;
+    for (int l = a.L; l >= 0 && c < 0; --l) {
+      const double sc = double(int64_t(1) << l);
+      for (int d = 0; d < 3; ++d) {
+        const int64_t top = (int64_t(a.ncell[d]) << l) - 1;
+        ix[d] = min(max(int64_t(floor(u[d] * sc)), int64_t(0)), top);
+      }
+      c = find_leaf(keys, idx, n_leaves, leaf_key_d(l, ix[0], ix[1], ix[2]));
+      lev = l;
+    }
+    if (c < 0) {  // cannot happen for a point of the domain: the old leaves tile it
+      out[i] = 0.0;
+      continue;
+    }
+    double wv[3][KMAX];
+    for (int d = 0; d < 3; ++d) {
+      const double t = fmin(fmax(u[d] * double(int64_t(1) << lev) - double(ix[d]), 0.0), 1.0);
+      for (int j = 0; j < a.K; ++j) {
+        double v = 1.0;
+        for (int m = 0; m < a.K; ++m)
+          if (m != j) v *= (t - a.gl[m]) / (a.gl[j] - a.gl[m]);
+        wv[d][j] = v;
+      }
+    }
+    double s = 0.0;
+    for (int e = 0; e < nloc; ++e)
+      s += val[cell_nodes[size_t(c) * nloc + e]] * (wv[0][e % a.K] * wv[1][(e / a.K) % a.K] *
+                                                    wv[2][e / (a.K * a.K)]);
+    out[i] = s;
+  }
+}
+}  // namespace
+
+int device_transfer_field(const mf_scratch *old_s, const double *old_val, const mf_scratch *new_s,
+                          double *new_vec)
+{
+  const AdaptiveData *A = old_s->ad;
+  const MeshInfo &m = old_s->mesh;
+  hipStream_t st = new_s->stream;
+  const int64_t n_leaves = m.n_cells, n_new = new_s->mesh.n_dofs;
+  TransferArgs a{};
+  a.p = m.degree;
+  a.K = m.degree + 1;
+  a.L = A->levels;
+  for (int d = 0; d < 3; ++d) {
+    a.ncell[d] = m.ncell[d];
+    a.lo[d] = m.lo[d];
+    a.h[d] = m.h[d];
+  }
+  for (int i = 0; i < a.K; ++i) a.gl[i] = old_s->tab.gl[i];
+  DevBufs b;
+  int32_t *d_cells;
+  uint64_t *k0, *k1;
+  uint32_t *v0, *v1;
+  double *xyz;
+  MF_HIP_TRY(b.alloc(&d_cells, 4 * size_t(n_leaves)));
+  MF_HIP_TRY(b.alloc(&k0, size_t(n_leaves)));
+  MF_HIP_TRY(b.alloc(&k1, size_t(n_leaves)));
+  MF_HIP_TRY(b.alloc(&v0, size_t(n_leaves)));
+  MF_HIP_TRY(b.alloc(&v1, size_t(n_leaves)));
+  MF_HIP_TRY(b.alloc(&xyz, 3 * size_t(n_new)));
+  MF_HIP_TRY(hipMemcpyAsync(d_cells, A->cells.data(), sizeof(int32_t) * 4 * size_t(n_leaves),
+                            hipMemcpyHostToDevice, st));
+  MF_HIP_TRY(hipMemcpyAsync(xyz, new_s->ad->xyz.data(), sizeof(double) * 3 * size_t(n_new),
+                            hipMemcpyHostToDevice, st));
+  const dim3 block(256);
+  auto grid = [](int64_t n) { return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)))); };
+  hipLaunchKernelGGL(leaf_keys_kernel, grid(n_leaves), block, 0, st, d_cells, n_leaves, k0, v0);
+  MF_HIP_TRY(hipGetLastError());
+  size_t tb = 0;
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, int(n_leaves), 0, 64, st));
+  char *tmp;
+  MF_HIP_TRY(b.alloc(&tmp, tb));
+  MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, int(n_leaves), 0, 64, st));
+  hipLaunchKernelGGL(transfer_eval_kernel, grid(n_new), block, 0, st, a, k1, v1, n_leaves,
+                     A->d.cell_nodes, old_val, xyz, n_new, new_vec);
+  MF_HIP_TRY(hipGetLastError());
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
 }  // namespace mf

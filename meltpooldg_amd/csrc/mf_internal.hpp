@@ -206,6 +206,10 @@ int device_hanging_targets(AdaptiveData *A, int64_t n_cells, int nloc, hipStream
 int device_refine_band(const int ncell[3], const double lo[3], const double hi[3], int levels,
                        const double center[3], double radius, double half_width,
                        std::vector<int32_t> &cells_out);
+// new_vec[i] = the old field (constraint-consistent values old_val on old_s, whose tables are
+// on the device) at the support point of node i of new_s (mf_adaptive_dev.hip)
+int device_transfer_field(const ::mf_scratch *old_s, const double *old_val,
+                          const ::mf_scratch *new_s, double *new_vec);
 // 1: adaptive setup steps run on the device when one is present (default), 0: on the host
 extern int g_adaptive_setup_device;
 int upload_adaptive(::mf_scratch *s);

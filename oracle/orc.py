@@ -516,3 +516,47 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
             raise RuntimeError("oracle Newton did not converge")
         out.append(T.copy())
     return out
+
+
+def fe_evaluate_adaptive(degree, ncell, cells, node_xyz, values, points, lo=(0.0, 0.0, 0.0),
+                         hi=(1.0, 1.0, 1.0)):
+    """Values of a continuous FE_Q field on an adaptive mesh at arbitrary points (brute force,
+    small meshes only): the leaf containing the point is found among all leaves by its box, its
+    K^3 nodal values are looked up by node coordinate, and the tensor Lagrange basis on the
+    Gauss-Lobatto nodes is evaluated at the point's local coordinates (the nodal interpolation
+    that SolutionTransfer applies for FE_Q, deal.II fe_q_base / solution_transfer)."""
+    if isinstance(ncell, int):
+        ncell = (ncell,) * 3
+    K = degree + 1
+    gl = gauss_lobatto(K)
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    h0 = (hi - lo) / np.asarray(ncell, float)
+    cells = np.asarray(cells)
+    lev = cells[:, 0].astype(float)
+    hc = h0[None, :] / (2.0 ** lev)[:, None]
+    c_lo = lo[None, :] + hc * cells[:, 1:4]
+    key = {tuple(np.round(x * 2 ** 24).astype(np.int64)): i for i, x in enumerate(node_xyz)}
+
+    def lagrange(t):
+        out = np.ones(K)
+        for j in range(K):
+            for m in range(K):
+                if m != j:
+                    out[j] *= (t - gl[m]) / (gl[j] - gl[m])
+        return out
+
+    res = np.zeros(len(points))
+    for i, x in enumerate(points):
+        tol = 1e-12 * hc
+        inside = np.all((x[None, :] >= c_lo - tol) & (x[None, :] <= c_lo + hc + tol), axis=1)
+        c = int(np.nonzero(inside)[0][0])
+        t = np.clip((x - c_lo[c]) / hc[c], 0.0, 1.0)
+        w = [lagrange(t[d]) for d in range(3)]
+        s = 0.0
+        for e in range(K ** 3):
+            a = (e % K, (e // K) % K, e // (K * K))
+            xn = c_lo[c] + hc[c] * np.array([gl[a[0]], gl[a[1]], gl[a[2]]])
+            s += values[key[tuple(np.round(xn * 2 ** 24).astype(np.int64))]] * \
+                w[0][a[0]] * w[1][a[1]] * w[2][a[2]]
+        res[i] = s
+    return res

@@ -166,3 +166,26 @@ def test_cg_on_condensed_system_matches_dense():
     assert st == orc.SUCCESS and it > 3
     xd = np.linalg.solve(A, b[free])
     assert np.abs(x[free] - xd).max() < 1e-9 * np.abs(xd).max()
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_fe_evaluate_adaptive_reproduces_polynomials(degree):
+    """orc.fe_evaluate_adaptive (the transfer check's evaluator): a field of Q_p given by its
+    nodal values on a hanging-node mesh is reproduced at arbitrary points, and at the nodes a
+    field outside Q_p returns its nodal values."""
+    cells = orc.refine_band_cells(2, 1, (0.5, 0.5, 0.5), 0.3, 0.1)
+    am = orc.AdaptiveMesh(degree, 2, cells)
+    p = degree
+
+    def f(X):
+        x, y, z = X[:, 0], X[:, 1], X[:, 2]
+        return 1.0 + x - 2.0 * y ** p + 0.5 * x * y * z + (x * z) ** p
+
+    pts = np.random.default_rng(1).uniform(0, 1, (60, 3))
+    got = orc.fe_evaluate_adaptive(degree, 2, cells, am.xyz, f(am.xyz), pts)
+    assert np.max(np.abs(got - f(pts))) < 1e-12
+    g = np.sin(3 * am.xyz[:, 0]) * np.cos(2 * am.xyz[:, 1]) + am.xyz[:, 2] ** 5
+    g = am.distribute(g)
+    free = am.hanging == 0
+    got = orc.fe_evaluate_adaptive(degree, 2, cells, am.xyz, g, am.xyz[free])
+    assert np.max(np.abs(got - g[free])) < 1e-12
