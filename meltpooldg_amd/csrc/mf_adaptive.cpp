@@ -615,7 +615,7 @@ void free_adaptive(mf_scratch *s)
                   A->d.tgt_ptr,    A->d.ent_local,  A->d.ent_w,      A->d.ent_w32,
                   A->d.slot_cell,  A->d.slot_tgt,   A->d.hang_nodes, A->d.cm_node,
                   A->d.cm_ptr,     A->d.cm_h,       A->d.cm_w,       A->d.cm_w32,
-                  A->d.hang_val,   A->d.hang_val32};
+                  A->d.hang_val,   A->d.hang_val32, A->d.xyz};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete A;

@@ -1352,6 +1352,10 @@ static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
     if constexpr (sizeof(Number) == 8)
       return launch_list_op<Number, K, ADVEC_DIAG>(needT, vel, m, a, src, dst, st, mode);
     break;
+  case HEAT_RHS:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, HEAT_RHS>(needT, vel, m, a, src, dst, st, mode);
+    break;
   case LS_TANGENT:
     return launch_list_op<Number, K, LS_TANGENT>(needT, vel, m, a, src, dst, st, mode);
   case LS_DIAG:

@@ -42,13 +42,13 @@ int mf_laser_heat_source(mf_scratch *s, double power, double radius, const doubl
 int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *x_n,
                                       const double *x_nm1, double tau);
 int mf_operator_reset_time_increment(mf_operator *op, double dt);
+int mf_distribute_constraints(mf_scratch *s, double *x);
 
 int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const double *T0,
                            mf_heat_stepper **out)
 {
   MF_REQUIRE(op && d && T0 && out, MF_ERR_ARG, "null argument");
   MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "the heat stepper needs a heat operator");
-  MF_REQUIRE(!op->s->ad, MF_ERR_ARG, "the heat stepper is available on structured meshes only");
   MF_REQUIRE(d->predictor == MF_PREDICTOR_NONE || d->predictor == MF_PREDICTOR_ZERO ||
                  d->predictor == MF_PREDICTOR_LINEAR_EXTRAPOLATION,
              MF_ERR_ARG, "unknown predictor type");
@@ -82,8 +82,15 @@ int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const
   }
   int rc = MF_OK;
   if (hipMemcpyAsync(h->T, T0, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(h->T_old, T0, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-      hipMemsetAsync(h->q, 0, bytes, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+      hipMemsetAsync(h->q, 0, bytes, st) != hipSuccess) {
+    set_error("device copy of the initial temperature failed");
+    rc = MF_ERR_HIP;
+  }
+  // on an adaptive mesh the history holds constraint-consistent fields
+  if (rc == MF_OK && op->s->ad) rc = mf_distribute_constraints(op->s, h->T);
+  if (rc == MF_OK &&
+      (hipMemcpyAsync(h->T_old, h->T, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+       hipStreamSynchronize(st) != hipSuccess)) {
     set_error("device copy of the initial temperature failed");
     rc = MF_ERR_HIP;
   }

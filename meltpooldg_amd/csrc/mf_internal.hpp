@@ -177,6 +177,7 @@ struct AdaptiveData {
     uint32_t *hang_nodes = nullptr, *cm_node = nullptr, *cm_ptr = nullptr, *cm_h = nullptr;
     double *cm_w = nullptr;
     float *cm_w32 = nullptr;
+    double *xyz = nullptr;        // support points, uploaded by the first call that needs them
     double *hang_val = nullptr;   // work: hanging-node values of the current source (FP64)
     float *hang_val32 = nullptr;  // and FP32
   } d;

@@ -93,6 +93,20 @@ __global__ void laser_kernel(NodeGeo g, double peak, double inv_r2, double x0, d
   }
 }
 
+// the same source at the support points of an adaptive mesh ([n][3])
+__global__ void laser_points_kernel(const double *__restrict__ xyz, double peak, double inv_r2,
+                                    double x0, double y0, double z0, double *q, int64_t n,
+                                    double radius, int d_dr)
+{
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < n;
+       idx += int64_t(gridDim.x) * blockDim.x) {
+    const double dx = xyz[3 * idx] - x0, dy = xyz[3 * idx + 1] - y0, dz = xyz[3 * idx + 2] - z0;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double v = peak * exp(-2.0 * d2 * inv_r2);
+    q[idx] = d_dr ? v * (-3.0 / radius + 4.0 * d2 * inv_r2 / radius) : v;
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // FE L2 norm: per cell sqrt(sum_q u_q^2 JxW) stored in single precision (the reference's
 // integrate_difference writes a Vector<float>, source/utilities/vector_tools.cpp:137-168), then
@@ -325,6 +339,7 @@ extern "C" {
 
 int mf_operator_vmult(mf_operator *op, double *dst, const double *src);
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
+int mf_distribute_constraints(mf_scratch *s, double *x);
 int mf_operator_reset_dof_index(mf_operator *op, int dof_index);
 int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
                            const double *k_cell);
@@ -346,10 +361,27 @@ static int laser_impl(mf_scratch *s, double power, double radius, const double p
                       double *q_s, int d_dr)
 {
   MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
-  MF_REQUIRE(!s->ad, MF_ERR_ARG, "the laser source is available on structured meshes only");
   MF_REQUIRE(s && position && q_s, MF_ERR_ARG, "null argument");
   MF_REQUIRE(radius > 0.0, MF_ERR_ARG, "The laser beam radius must be greater than zero! Abort..");
   const MeshInfo &m = s->mesh;
+  if (s->ad) {
+    // nodal values at the support points, then the hanging values from their masters
+    // (compute_volumetric_heat_source ends with constraints.distribute)
+    AdaptiveData *A = s->ad;
+    int rc = ensure_adaptive_on_device(s);
+    if (rc) return rc;
+    if (!A->d.xyz) {
+      MF_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&A->d.xyz), sizeof(double) * A->xyz.size()));
+      MF_HIP_TRY(hipMemcpy(A->d.xyz, A->xyz.data(), sizeof(double) * A->xyz.size(),
+                           hipMemcpyHostToDevice));
+    }
+    const double a = radius * std::sqrt(0.5 * M_PI);
+    hipLaunchKernelGGL(laser_points_kernel, vgrid(m.n_dofs), dim3(VEC_THREADS), 0, s->stream,
+                       A->d.xyz, power / (a * a * a), 1.0 / (radius * radius), position[0],
+                       position[1], position[2], q_s, m.n_dofs, radius, d_dr);
+    MF_HIP_TRY(hipGetLastError());
+    return mf_distribute_constraints(s, q_s);
+  }
   NodeGeo g{};
   for (int d = 0; d < 3; ++d) {
     g.lo[d] = m.lo[d];
@@ -483,6 +515,9 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
   const double tol_d = newton->correction_tolerance > 0 ? newton->correction_tolerance : 1e-10;
   // constraints of the predictor (AffineConstraints::distribute)
   if (dirichlet_values) rc = vec_masked_assign(T, s->d_mask, dirichlet_values, true, n, st);
+  // on an adaptive mesh the iterate is kept constraint-consistent (hanging values from their
+  // masters): NewtonRaphsonSolver's distribute_constraints after every update
+  if (rc == MF_OK && s->ad) rc = mf_distribute_constraints(s, T);
   const double *vel = op->vel, *kc = op->k_cell;
   bool converged = false;
   for (int it = 0; rc == MF_OK && it < max_it; ++it) {
@@ -506,6 +541,7 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
       break;
     }
     if ((rc = vec_axpby(1.0, d, 1.0, T, n, st))) break;
+    if (s->ad && (rc = mf_distribute_constraints(s, T))) break;
   }
   (void)hipFreeAsync(r, st);
   (void)hipFreeAsync(d, st);

@@ -1082,13 +1082,11 @@ void orc_invert_diagonal(int kind, long n, double *diag)
  * (rhs_cell_loop, heat_diffuse_operator.cpp:393-529).  src = T_old.
  * advection: values phi/dt - (1-theta) w.grad phi, gradients -(1-theta) D grad phi
  * (create_rhs, advection_diffusion_operator.cpp:274-358).  src = phi_old. */
-static void rhs_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+/* local residual of a cell whose DoF indices are in ws->idx: fields read as stored (the
+ * callers pass constraint-consistent vectors), result in ws->out */
+static void rhs_local(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, io_t *io)
 {
-  io_t *io = (io_t *)u;
   const int nq = g->nloc, dim = g->dim;
-  const long n = orc_n_dofs(&op->mesh);
-  (void)n;
-  cell_dofs(g, cell, ws->idx);
   if (op->kind == ORC_OP_HEAT) {
     const orc_heat_params *hp = &op->heat;
     for (int l = 0; l < nq; ++l) ws->tmp[l] = op->T[ws->idx[l]];
@@ -1129,11 +1127,24 @@ static void rhs_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_
     }
   }
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
+}
+
+static void rhs_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws, void *u)
+{
+  io_t *io = (io_t *)u;
+  cell_dofs(g, cell, ws->idx);
+  rhs_local(op, g, cell, ws, io);
   scatter(op, g, io->dst, ws, op->use_mask);
 }
 
+static void arhs(const orc_operator *op, const double *src, const double *q_s, double *dst);
+
 void orc_create_rhs(const orc_operator *op, const double *src, const double *q_s, double *dst)
 {
+  if (op->amesh) {
+    arhs(op, src, q_s, dst);
+    return;
+  }
   geo_t g;
   geo_init(&op->mesh, &g);
   io_t io = {src, dst, q_s};
@@ -1742,6 +1753,26 @@ static void ascatter(const orc_operator *op, const double *out, int nloc, const 
       dst[i] += out[l];
     }
   }
+}
+
+/* heat residual on an adaptive mesh (HeatDiffuseOperator::create_rhs with the HNC of the
+ * constrained DoF index, heat_diffuse_operator.cpp:393-529): T, src (T_old) and q_s read as
+ * stored at the cell's nodes (constraint-consistent vectors), the cell residual distributed to
+ * the masters of its hanging rows and added to dst; constrained rows untouched */
+static void arhs(const orc_operator *op, const double *src, const double *q_s, double *dst)
+{
+  const orc_amesh *am = op->amesh;
+  const int K = am->degree + 1, nloc = K * K * K;
+  cell_ws_t *ws = (cell_ws_t *)malloc(sizeof(cell_ws_t));
+  io_t io = {src, dst, q_s};
+  for (long c = 0; c < am->n_cells; ++c) {
+    geo_t g;
+    acell_geo(am, c, &g);
+    for (int l = 0; l < nloc; ++l) ws->idx[l] = op->adata->cell_nodes[c * nloc + l];
+    rhs_local(op, &g, c, ws, &io);
+    ascatter(op, ws->out, nloc, ws->idx, dst);
+  }
+  free(ws);
 }
 
 static void avmult(const orc_operator *op, const double *src, double *dst)

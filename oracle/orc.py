@@ -458,7 +458,7 @@ def laser_volumetric_source(X, power, radius, position):
 
 
 def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
-                    dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12):
+                    dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12, amesh=None):
     """The heat time loop of the mp-heat-transfer application, restated on the oracle's
     operator (HeatTransferApplication::run, applications/mp-heat-transfer/
     heat_transfer_application.cpp:52-120, heat-diffuse operator, volumetric laser):
@@ -473,10 +473,14 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         A d = r by Jacobi-GMRES (T-dependent law) or CG, stop when |r| and |d| are below the
         tolerances, else T += d.
     heat: HeatParams (dt is set per step); laser: dict(power, radius, start, scan_direction,
-    scan_speed).  Returns the temperature after every step."""
-    T = np.array(T0, dtype=np.float64)
+    scan_speed).  On an adaptive mesh (amesh, mesh ignored) the fields are kept
+    constraint-consistent: the heat source, the iterate after the Dirichlet values and after
+    every update are distributed (compute_volumetric_heat_source and the Newton solver's
+    distribute_constraints).  Returns the temperature after every step."""
+    dist = amesh.distribute if amesh is not None else (lambda v: v)
+    T = dist(np.array(T0, dtype=np.float64))
     T_old = T.copy()
-    X = node_coords(mesh)
+    X = amesh.xyz if amesh is not None else node_coords(mesh)
     pos = np.array(laser["start"], dtype=np.float64) if laser else None
     if laser:
         sd = np.asarray(laser.get("scan_direction", (1.0, 0.0, 0.0)), dtype=np.float64)
@@ -488,7 +492,7 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         if laser:
             if laser.get("scan_speed", 0.0) != 0.0:
                 pos = pos + sd * laser["scan_speed"] * dt
-            q = laser_volumetric_source(X, laser["power"], laser["radius"], pos)
+            q = dist(laser_volumetric_source(X, laser["power"], laser["radius"], pos))
         if predictor == "none":
             P = T.copy()
         elif predictor == "zero":
@@ -498,11 +502,12 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         T_old, T = T, P
         if dirichlet_values is not None:
             T[mask == 1] = np.asarray(dirichlet_values)[mask == 1]
+        T = dist(T)
         hp = HeatParams(*(getattr(heat, f) for f, _ in HeatParams._fields_))
         hp.dt = dt
         converged = False
         for _ in range(newton[0]):
-            op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old)
+            op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old, amesh=amesh)
             r = op.create_rhs(T_old, q)
             d, _, _, st = op.solve(r, solver=GMRES if hp.t_dependent else CG, rel_tol=rel_tol,
                                    inv_diag=op.inverse_diagonal())
@@ -511,7 +516,7 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
             if np.linalg.norm(r) < newton[1] and np.linalg.norm(d) < newton[2]:
                 converged = True
                 break
-            T = T + d
+            T = dist(T + d)
         if not converged:
             raise RuntimeError("oracle Newton did not converge")
         out.append(T.copy())

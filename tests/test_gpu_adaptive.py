@@ -162,3 +162,28 @@ def test_solve_and_refined_solve(name, law, solver):
     assert rep32["status"] == 0 and rep32["outer_iterations"] >= 2
     assert rep32["final_residual"] <= 1e-10 * rep32["initial_residual"]
     assert np.abs(x32.cpu().numpy() - xr)[free].max() <= 1e-8 * np.abs(xr).max()
+
+
+@pytest.mark.parametrize("name", ["q1", "q2", "q3", "q3b"])
+def test_heat_rhs(name):
+    """The heat residual on an adaptive mesh (create_rhs: tanh law, source term, default
+    Dirichlet rows) against the oracle, for constraint-consistent fields."""
+    P = Pair(name)
+    mf = P.mf
+    am = P.am
+    T = am.distribute(P.field(150.0, 1700.0))
+    T_old = am.distribute(P.field(150.0, 1700.0))
+    q = am.distribute(P.field(10.0))
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.3, k=0.7, tanh_amp=0.5,
+                        tanh_Tc=1700.0, tanh_width=50.0, dt=1e-3)
+    ref = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, T=T, mask=am.dirichlet, amesh=am)
+    r_ref = P.to_dev_order(ref.create_rhs(T_old, q))
+    op = mf.HeatDiffuseOperator(P.s, law=mf.LAW_TANH, rho_cp=1.3, k=0.7, tanh_amp=0.5,
+                                tanh_Tc=1700.0, tanh_width=50.0, t_dependent=True, dt=1e-3)
+    Td = _dev(P.to_dev_order(T))
+    op.set_fields(T=Td)
+    r = torch.zeros(P.n, dtype=torch.float64, device="cuda")
+    op.create_rhs(r, _dev(P.to_dev_order(T_old)), q_s=_dev(P.to_dev_order(q)))
+    torch.cuda.synchronize()
+    got = r.cpu().numpy()
+    assert np.max(np.abs(got - r_ref)) <= 1e-12 * np.max(np.abs(r_ref))

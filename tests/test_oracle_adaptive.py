@@ -189,3 +189,52 @@ def test_fe_evaluate_adaptive_reproduces_polynomials(degree):
     free = am.hanging == 0
     got = orc.fe_evaluate_adaptive(degree, 2, cells, am.xyz, g, am.xyz[free])
     assert np.max(np.abs(got - g[free])) < 1e-12
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_heat_rhs_uniform_adaptive_equals_structured(degree):
+    """The adaptive heat residual (orc_create_rhs on an adaptive mesh) on level-1 cells only is
+    the structured residual on the twice finer lattice (tanh law, source term, Dirichlet rows)."""
+    nb = (2, 2, 1)
+    n = [c << 1 for c in nb]
+    cells = np.array([(1, i, j, k) for k in range(n[2]) for j in range(n[1])
+                      for i in range(n[0])], dtype=np.int32)
+    am = orc.AdaptiveMesh(degree, nb, cells)
+    m = orc.make_mesh(3, degree, tuple(n))
+    perm = _coord_map(am.xyz, orc.node_coords(m))
+    T = 1650 + 150 * orc.random_field(m, 3)
+    T_old = 1650 + 150 * orc.random_field(m, 4)
+    q = orc.random_field(m, 5)
+    hp = _heat(law=orc.LAW_TANH, td=1)
+    mask = orc.boundary_mask(m)
+    ref = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, mask=mask, T=T).create_rhs(T_old, q)
+    Ta, Toa, qa, ma = (np.zeros(am.n_nodes) for _ in range(4))
+    Ta[perm], Toa[perm], qa[perm] = T, T_old, q
+    ma[perm] = mask
+    op = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, amesh=am, mask=ma.astype(np.uint8), T=Ta)
+    got = op.create_rhs(Toa, qa)
+    assert np.max(np.abs(got[perm] - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_heat_rhs_consistent_with_condensed_operator(name):
+    """Linear law, constraint-consistent fields: rhs(T, T_old = 0, q = 0) = -A T and
+    rhs(T = 0, T_old, q) = (rho_cp/dt) M T_old + M q with the condensed (pinned) operators."""
+    am, _ = _amesh(name)
+    rng = np.random.default_rng(9)
+    T = am.distribute(rng.uniform(-1, 1, am.n_nodes))
+    T_old = am.distribute(rng.uniform(-1, 1, am.n_nodes))
+    q = am.distribute(rng.uniform(-1, 1, am.n_nodes))
+    hp = _heat()
+    z = np.zeros(am.n_nodes)
+    op = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, amesh=am, use_mask=0, T=T)
+    A = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, amesh=am, use_mask=0)
+    hang = am.hanging == 1
+    r = op.create_rhs(z)
+    assert np.max(np.abs((r + A.vmult(T))[~hang])) < 1e-12 * np.max(np.abs(r))
+    op0 = orc.OracleOperator(orc.OP_HEAT, None, heat=hp, amesh=am, use_mask=0, T=z)
+    r = op0.create_rhs(T_old, q)
+    M = orc.OracleOperator(orc.OP_HEAT, None, heat=_heat(rho_cp=1.0, k=0.0, dt=1.0), amesh=am,
+                           use_mask=0)
+    expect = (1.3 / 1e-2) * M.vmult(T_old) + M.vmult(q)
+    assert np.max(np.abs((r - expect)[~hang])) < 1e-12 * np.max(np.abs(expect))

@@ -70,3 +70,28 @@ def test_step_with_laser_satisfies_step_equation():
     assert np.linalg.norm(op.create_rhs(out[0], q1)) > 1e-3 * np.linalg.norm(r0)
     # the source heats: the interior warms above the boundary value
     assert out[1][mask == 0].max() > 300.5
+
+
+def test_adaptive_loop_on_uniform_mesh_equals_structured():
+    """The loop on an adaptive mesh of level-1 cells only (no hanging nodes) equals the
+    structured loop on the twice finer lattice, node by node."""
+    nb = (2, 2, 2)
+    cells = np.array([(1, i, j, k) for k in range(4) for j in range(4) for i in range(4)],
+                     dtype=np.int32)
+    am = orc.AdaptiveMesh(2, nb, cells)
+    m = orc.make_mesh(3, 2, (4, 4, 4))
+    X = orc.node_coords(m)
+    perm = {tuple(np.round(x * 2 ** 24).astype(np.int64)): i for i, x in enumerate(am.xyz)}
+    perm = np.array([perm[tuple(np.round(x * 2 ** 24).astype(np.int64))] for x in X])
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=400.0, tanh_width=50.0, dt=1e-2)
+    laser = dict(power=1000.0, radius=0.3, start=(0.2, 0.5, 0.8), scan_speed=5.0)
+    mask = orc.boundary_mask(m)
+    T0 = np.full(len(X), 300.0)
+    ref = orc.heat_time_steps(m, hp, mask, T0, [1e-2, 1e-2], laser=laser, dirichlet_values=T0)
+    ma = np.zeros(am.n_nodes, dtype=np.uint8)
+    ma[perm] = mask
+    T0a = np.full(am.n_nodes, 300.0)
+    got = orc.heat_time_steps(None, hp, ma, T0a, [1e-2, 1e-2], laser=laser, dirichlet_values=T0a,
+                              amesh=am)
+    assert np.max(np.abs(got[-1][perm] - ref[-1])) < 1e-9 * np.max(np.abs(ref[-1] - 300.0))
