@@ -202,7 +202,9 @@ int mf_operator_vmult_f32(mf_operator *op, float *dst, const float *src);
 int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
 /* OperatorMatrixFree::create_rhs (operator_base.hpp:208-212; heat_diffuse_operator.cpp:783-806,
  * advection_diffusion_operator.cpp:274-358): ADDS into dst.  Heat: src = T_old, q_s = heat
- * source (may be NULL), result -R(T).  Advection: src = old field, q_s must be NULL. */
+ * source (may be NULL), result -R(T); on an adaptive mesh T, src and q_s are read as stored
+ * (pass constraint-consistent vectors) and the hanging rows of dst are moved to their masters.
+ * Advection: src = old field, q_s must be NULL (structured meshes). */
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
 
 /* ------------------------------------------------------------------------------------------ */
@@ -389,7 +391,8 @@ int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *
 /* LaserHeatSourceVolumetric::compute_volumetric_heat_source with the Gaussian volumetric
  * profile (source/heat/laser_heat_source_volumetric.cpp:20-58,
  * include/meltpooldg/heat/laser_intensity_profiles.hpp:175-218): nodal values of
- * P / (r sqrt(pi/2))^3 exp(-2 |x - x_L|^2 / r^2). */
+ * P / (r sqrt(pi/2))^3 exp(-2 |x - x_L|^2 / r^2); on an adaptive mesh at the support points,
+ * then the hanging values are distributed (as the reference's constraints.distribute). */
 int mf_laser_heat_source(mf_scratch *s, double power, double radius, const double position[3],
                          double *q_s);
 /* VectorTools::compute_norm, FE L2 norm (source/utilities/vector_tools.cpp:137-201) with the
