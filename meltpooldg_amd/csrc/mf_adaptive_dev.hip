@@ -1098,6 +1098,8 @@ int device_refine_band(const int ncell[3], const double lo[3], const double hi[3
   MF_HIP_TRY(hipGetLastError());
   // split the leaves with cnt = 8: scan, fill into gout, swap with gkeys
   auto split = [&]() -> int {
+    // every leaf may become 8: the 32-bit counts and their scan stay exact below 2^28 leaves
+    MF_REQUIRE(n < (int64_t(1) << 28), MF_ERR_ARG, "adaptive mesh too large");
     uint32_t *cnt = static_cast<uint32_t *>(gcnt.p), *pos = static_cast<uint32_t *>(gpos.p);
     size_t tb = 0;
     MF_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, pos, int(n) + 1, st));
