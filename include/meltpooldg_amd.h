@@ -396,7 +396,8 @@ int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *
 int mf_laser_heat_source(mf_scratch *s, double power, double radius, const double position[3],
                          double *q_s);
 /* VectorTools::compute_norm, FE L2 norm (source/utilities/vector_tools.cpp:137-201) with the
- * per-cell values rounded to single precision as in the reference; result on the host. */
+ * per-cell values rounded to single precision as in the reference; result on the host.  On an
+ * adaptive mesh u is read as stored (pass a constraint-consistent vector). */
 int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result);
 
 /* Derivative of the Gaussian volumetric source with respect to the radius (adjoint). */

@@ -122,14 +122,17 @@ __global__ void l2_norm_kernel(CellArgs<double> A, const double *u, double *part
     const int cx = cell % A.ncell[0], cy = (cell / A.ncell[0]) % A.ncell[1],
               cz = cell / (A.ncell[0] * A.ncell[1]);
     double v[K][K][K], w1[K][K][K];
+    // adaptive mesh: the cell's nodes from its table (lexicographic local order)
+    const uint32_t *cn = A.cell_nodes ? A.cell_nodes + size_t(cell) * (K * K * K) : nullptr;
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int j = 0; j < K; ++j)
 #pragma unroll
         for (int i = 0; i < K; ++i)
-          v[k][j][i] = u[uint32_t(cx * P + i) +
-                         A.NX * (uint32_t(cy * P + j) + A.NY * uint32_t(cz * P + k))];
+          v[k][j][i] = cn ? u[cn[i + K * (j + K * k)]]
+                          : u[uint32_t(cx * P + i) +
+                              A.NX * (uint32_t(cy * P + j) + A.NY * uint32_t(cz * P + k))];
     // x, y, z interpolation
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -166,6 +169,7 @@ __global__ void l2_norm_kernel(CellArgs<double> A, const double *u, double *part
           acc += s * s * (A.w[i] * A.w[j] * A.w[a]);
         }
     acc *= A.h[0] * A.h[1] * A.h[2];
+    if (cn) acc = ldexp(acc, -3 * int(A.cell_level[cell]));  // cell size h / 2^level
     const float ef = float(sqrt(acc));
     e2 = double(ef) * double(ef);
   }
@@ -415,10 +419,16 @@ int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double r
 int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result)
 {
   MF_REQUIRE(s && u && result, MF_ERR_ARG, "null argument");
-  MF_REQUIRE(!s->ad, MF_ERR_ARG, "the FE norm is available on structured meshes only");
   const MeshInfo &m = s->mesh;
   CellArgs<double> a;
   std::memset(&a, 0, sizeof(a));
+  if (s->ad) {
+    // adaptive mesh: u read as stored at each cell's nodes (a constraint-consistent vector)
+    int rc = ensure_adaptive_on_device(s);
+    if (rc) return rc;
+    a.cell_nodes = s->ad->d.cell_nodes;
+    a.cell_level = s->ad->d.cell_level;
+  }
   for (int i = 0; i < m.K * m.K; ++i) a.S[i] = s->tab.S[i];
   for (int i = 0; i < m.K; ++i) a.w[i] = s->tab.w[i];
   for (int d = 0; d < 3; ++d) {

@@ -187,3 +187,37 @@ def test_heat_rhs(name):
     torch.cuda.synchronize()
     got = r.cpu().numpy()
     assert np.max(np.abs(got - r_ref)) <= 1e-12 * np.max(np.abs(r_ref))
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_fe_norm_on_adaptive_mesh(degree):
+    """mf_compute_norm_l2 on adaptive meshes: exact (up to the single-precision cell values)
+    for a field of the element space, and on a mesh of level-1 cells only the structured norm
+    of the twice finer lattice."""
+    import meltpooldg_amd as mf
+    st = torch.cuda.current_stream()
+    cells = mf.refine_band_cells(3, 2, (0.5, 0.5, 0.5), 0.3, 0.06)
+    s = mf.AdaptiveScratchData(degree, 3, cells, stream=st)
+    p = degree
+
+    def f(X):
+        x, y, z = X[..., 0], X[..., 1], X[..., 2]
+        return 1.0 + x - 2.0 * y ** p + 0.5 * x * y * z + (x * z) ** p
+
+    got = mf.compute_norm_l2(s, _dev(f(s.support_points())))
+    g, w = np.polynomial.legendre.leggauss(8)
+    g, w = 0.5 * (g + 1), 0.5 * w
+    G = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1)
+    W = w[:, None, None] * w[None, :, None] * w[None, None, :]
+    exact = np.sqrt(np.sum(W * f(G) ** 2))
+    assert abs(got / exact - 1) < 1e-6
+    cu = np.array([(1, i, j, k) for k in range(4) for j in range(4) for i in range(4)],
+                  dtype=np.int32)
+    su = mf.AdaptiveScratchData(degree, 2, cu, stream=st)
+    ss = mf.ScratchData(degree, 4, stream=st)
+    u = ss.random_field(4)
+    Xs, Xu = ss.support_points(), su.support_points()
+    pos = {tuple(np.round(x * 2 ** 24).astype(np.int64)): i for i, x in enumerate(Xs)}
+    uu = u[[pos[tuple(np.round(x * 2 ** 24).astype(np.int64))] for x in Xu]]
+    a, b = mf.compute_norm_l2(su, _dev(uu)), mf.compute_norm_l2(ss, _dev(u))
+    assert abs(a / b - 1) < 1e-12
