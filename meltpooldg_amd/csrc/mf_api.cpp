@@ -778,6 +778,7 @@ struct SolveOut {
   // in (FP32 CG of the defect correction): right-hand side = float(b64_scale * b64), x = 0
   const double *b64 = nullptr;
   double b64_scale = 1.0;
+  const double *b64_den = nullptr;  // when set: the scale is 1 / *b64_den, read on the device
   int iterations = 0;
   double res0 = 0, res = 0;
   int status = MF_OK;
@@ -811,7 +812,8 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   // cg_check0 initialises every field of the state block
   if constexpr (sizeof(T) == 4) {
     if (out.b64) {
-      rc = cg_init_refine(out.b64, out.b64_scale, x, r, p, inv, n, prr, prz0, sd, abs_tol, rel_tol, st);
+      rc = cg_init_refine(out.b64, out.b64_scale, out.b64_den, x, r, p, inv, n, prr, prz0, sd,
+                          abs_tol, rel_tol, st);
       if (rc) return rc;
     }
   }
@@ -823,16 +825,27 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   int it = 0;
   // Iterations are enqueued in chunks and the host reads the state after each chunk; kernels
   // launched after convergence (or after the step-0 check succeeded) exit at once.
-  int chunk = std::max(1, out.first_chunk);
-  for (;;) {
-    for (int k = 0; k < chunk && it < max_it; ++k) {
+  // One iteration is enqueued behind each state read (look-ahead), so the device keeps working
+  // while the host waits for the read; after convergence it exits at once.
+  constexpr int LOOKAHEAD = 1;
+  auto enqueue = [&](int count) -> int {
+    for (int k = 0; k < count && it < max_it; ++k) {
       ++it;
-      if ((rc = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return rc;
-      if ((rc = cg_step<T>(x, r, p, q, inv, n, prz, prr, pcell, ncb, sd, it, max_it, st))) return rc;
+      int c;
+      if ((c = apply_tangent<T>(op, q, p, pcell, &sd->stop, true))) return c;
+      if ((c = cg_step<T>(x, r, p, q, inv, n, prz, prr, pcell, ncb, sd, it, max_it, st))) return c;
     }
-    if ((rc = read_to_host(&h, sd, sizeof(SolveState), st))) return rc;
+    return MF_OK;
+  };
+  int chunk = std::max(1, out.first_chunk);
+  if ((rc = enqueue(chunk))) return rc;
+  for (;;) {
+    const int it_read = it;
+    if ((rc = read_to_host_begin(sd, sizeof(SolveState), st))) return rc;
+    if ((rc = enqueue(LOOKAHEAD))) return rc;
+    if ((rc = read_to_host_end(&h, sizeof(SolveState)))) return rc;
     if (h.stop) break;
-    MF_REQUIRE(it < max_it, MF_ERR_STATE, "CG state not stopped at max_it");
+    MF_REQUIRE(it_read < max_it, MF_ERR_STATE, "CG state not stopped at max_it");
     chunk = 8;
     if (h.iterations > 0 && h.res > 0 && h.res < h.res0 && h.tol > 0) {
       // iterations left at the observed average rate (plus one), so that few launches run
@@ -841,6 +854,7 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
       const double left = std::log(h.tol / h.res) / rate;
       chunk = int(std::min(64.0, std::max(2.0, std::ceil(left) + 1.0)));
     }
+    if ((rc = enqueue(std::max(0, chunk - LOOKAHEAD)))) return rc;
   }
   out.iterations = h.iterations;
   out.res0 = h.res0;
@@ -1015,7 +1029,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
     Carve cv;
     cv.plan<double>(n);
     cv.plan<double>(n);
-    cv.plan<double>(1);
+    cv.plan<double>(2);
     cv.plan<double>(2 * RED_BLOCKS);
     cv.plan<float>(n);
     cv.plan<float>(n);
@@ -1027,7 +1041,7 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
     struct F {
       float *p;
     };
-    D r{cv.take<double>(n)}, q{cv.take<double>(n)}, nrm{cv.take<double>(1)},
+    D r{cv.take<double>(n)}, q{cv.take<double>(n)}, nrm{cv.take<double>(2)},
         part{cv.take<double>(2 * RED_BLOCKS)};
     F r32{cv.take<float>(n)}, e32{cv.take<float>(n)}, inv32{cv.take<float>(n)};
     const float *inv32p = inv32_in;
@@ -1041,28 +1055,41 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
                                  ? desc->inner_rel_tol
                                  : std::min(1e-2, std::max(1e-5, std::sqrt(std::max(desc->rel_tol, 0.0))));
     const int max_outer = desc->max_outer > 0 ? desc->max_outer : 50;
-    auto residual = [&](double &res, bool xz) -> int {
+    // r = b - A x and |r| into the device slot nrm[k]
+    auto residual = [&](bool xz, double *slot) -> int {
       int c;
       if (!xz && (c = apply_tangent<double>(op, q.p, x, nullptr, nullptr, true))) return c;
-      if ((c = vec_residual<double>(b, xz ? nullptr : q.p, r.p, n, part.p, nrm.p, st))) return c;
-      if ((c = read_to_host(&res, nrm.p, sizeof(double), st))) return c;
-      return MF_OK;
+      return vec_residual<double>(b, xz ? nullptr : q.p, r.p, n, part.p, slot, st);
     };
-    double res = 0.0;
-    if ((rc = residual(res, x_zero))) return rc;
-    const double res0 = res, tol = std::max(desc->abs_tol, desc->rel_tol * res0);
+    // The inner CG scales its right-hand side and the correction by |r| read on the device, so
+    // a pass is enqueued without waiting for the norm; the host reads the norms after the pass
+    // (both at once after the first).  GMRES takes the scale from the host.
+    const bool dev_scale = desc->solver == MF_SOLVER_CG;
+    double res = 0.0, res0 = 0.0, tol = 0.0;
+    bool known = false;
+    int slot = 0;
+    if ((rc = residual(x_zero, nrm.p))) return rc;
+    if (!dev_scale) {
+      if ((rc = read_to_host(&res, nrm.p, sizeof(double), st))) return rc;
+      res0 = res;
+      tol = std::max(desc->abs_tol, desc->rel_tol * res0);
+      known = true;
+    }
     int outer = 0, inner_total = 0, last_inner = 0, status = MF_OK;
-    while (!(res <= tol)) {
-      if (outer >= max_outer || std::isnan(res)) {
-        status = MF_NO_CONVERGENCE;
-        break;
+    for (;;) {
+      if (known) {
+        if (res <= tol) break;
+        if (outer >= max_outer || std::isnan(res)) {
+          status = MF_NO_CONVERGENCE;
+          break;
+        }
       }
       ++outer;
       SolveOut o;
-      if (desc->solver == MF_SOLVER_CG) {
-        // the inner CG starts from float(r / res) and e = 0 in its first kernel
+      if (dev_scale) {
+        // the inner CG starts from float(r / |r|) and e = 0 in its first kernel
         o.b64 = r.p;
-        o.b64_scale = 1.0 / res;
+        o.b64_den = nrm.p + slot;
       } else {
         if ((rc = vec_cast<double, float>(r.p, r32.p, 1.0 / res, n, st))) return rc;
         if ((rc = vec_fill<float>(e32.p, 0.0f, n, st))) return rc;
@@ -1078,8 +1105,21 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
         status = MF_BREAKDOWN;
         break;
       }
-      if ((rc = vec_add_scaled_f32(x, e32.p, res, n, st))) return rc;
-      if ((rc = residual(res, false))) return rc;
+      // x += |r| e
+      if ((rc = vec_add_scaled_f32(x, e32.p, res, dev_scale ? nrm.p + slot : nullptr, n, st)))
+        return rc;
+      if ((rc = residual(false, nrm.p + (slot ^ 1)))) return rc;
+      if (!known) {
+        double h2[2];
+        if ((rc = read_to_host(h2, nrm.p, 2 * sizeof(double), st))) return rc;
+        res0 = h2[0];
+        res = h2[1];
+        tol = std::max(desc->abs_tol, desc->rel_tol * res0);
+        known = true;
+      } else if ((rc = read_to_host(&res, nrm.p + (slot ^ 1), sizeof(double), st))) {
+        return rc;
+      }
+      slot ^= 1;
     }
     rep.iterations = inner_total;
     rep.outer_iterations = outer;

@@ -197,8 +197,10 @@ __global__ void cast_kernel(const Ta *a, Tb *b, double scale, int64_t n)
 }
 
 // x += scale * e  (FP64 solution update of the defect correction)
-__global__ void add_scaled_f32_kernel(double *x, const float *e, double scale, int64_t n)
+__global__ void add_scaled_f32_kernel(double *x, const float *e, double scale,
+                                      const double *scale_dev, int64_t n)
 {
+  if (scale_dev) scale = *scale_dev;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   int64_t tail = 0;
@@ -363,10 +365,13 @@ __global__ void cg_init_kernel(const T *__restrict__ b, const T *__restrict__ q,
 // Start of an FP32 inner CG of the defect correction from the FP64 residual: r = float(s b64),
 // z = D r, p = z, x = 0, partial r.r and r.z (the cast, the fill of x and cg_init in one pass)
 __global__ void cg_init_refine_kernel(const double *__restrict__ b64, double s,
-                                      float *__restrict__ x, float *__restrict__ r,
-                                      float *__restrict__ p, const float *__restrict__ inv,
-                                      int64_t n, float *part_rr, float *part_rz0)
+                                      const double *__restrict__ den, float *__restrict__ x,
+                                      float *__restrict__ r, float *__restrict__ p,
+                                      const float *__restrict__ inv, int64_t n, float *part_rr,
+                                      float *part_rz0)
 {
+  // scale 1/|b64| from the device (a zero norm gives a zero right-hand side)
+  if (den) s = *den > 0.0 ? 1.0 / *den : 0.0;
   Acc<float> arr, arz;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -703,23 +708,27 @@ template int vec_cast<double, float>(const double *, float *, double, int64_t, h
 template int vec_cast<float, double>(const float *, double *, double, int64_t, hipStream_t);
 template int vec_cast<double, double>(const double *, double *, double, int64_t, hipStream_t);
 
-int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipStream_t st)
+int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev, int64_t n,
+                       hipStream_t st)
 {
-  hipLaunchKernelGGL(add_scaled_f32_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, x, e, scale, n);
+  hipLaunchKernelGGL(add_scaled_f32_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, x, e, scale,
+                     scale_dev, n);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
 
-int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st)
+namespace {
+// pinned staging buffer and event of the calling thread on the current device
+struct Stage {
+  int dev = -1;
+  void *buf = nullptr;
+  hipEvent_t ev = nullptr;
+};
+thread_local Stage stage;
+}  // namespace
+
+int read_to_host_begin(const void *src, size_t bytes, hipStream_t st)
 {
-  // staged through pinned memory and awaited by polling an event: the host sees the result as
-  // soon as the copy ends instead of waking from a blocking stream synchronisation
-  struct Stage {
-    int dev = -1;
-    void *buf = nullptr;
-    hipEvent_t ev = nullptr;
-  };
-  static thread_local Stage stage;
   MF_REQUIRE(bytes <= 4096, MF_ERR_ARG, "read_to_host: at most 4 KiB");
   int dev = 0;
   MF_HIP_TRY(hipGetDevice(&dev));
@@ -733,12 +742,26 @@ int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st)
   }
   MF_HIP_TRY(hipMemcpyAsync(stage.buf, src, bytes, hipMemcpyDeviceToHost, st));
   MF_HIP_TRY(hipEventRecord(stage.ev, st));
+  return MF_OK;
+}
+
+int read_to_host_end(void *dst, size_t bytes)
+{
   hipError_t e;
   while ((e = hipEventQuery(stage.ev)) == hipErrorNotReady) {
   }
   MF_HIP_TRY(e);
   std::memcpy(dst, stage.buf, bytes);
   return MF_OK;
+}
+
+int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+  // staged through pinned memory and awaited by polling an event: the host sees the result as
+  // soon as the copy ends instead of waking from a blocking stream synchronisation
+  const int rc = read_to_host_begin(src, bytes, st);
+  if (rc) return rc;
+  return read_to_host_end(dst, bytes);
 }
 
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st)
@@ -848,12 +871,12 @@ int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T
 template int cg_init<double>(const double *, double *, double *, double *, const double *, int64_t,
                              double *, double *, SolveState *, double, double, hipStream_t);
 
-int cg_init_refine(const double *b64, double s, float *x, float *r, float *p, const float *inv,
-                   int64_t n, float *part_rr, float *part_rz0, SolveState *sd, double abs_tol,
-                   double rel_tol, hipStream_t st)
+int cg_init_refine(const double *b64, double s, const double *den, float *x, float *r, float *p,
+                   const float *inv, int64_t n, float *part_rr, float *part_rz0, SolveState *sd,
+                   double abs_tol, double rel_tol, hipStream_t st)
 {
-  hipLaunchKernelGGL(cg_init_refine_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, b64, s, x,
-                     r, p, inv, n, part_rr, part_rz0);
+  hipLaunchKernelGGL(cg_init_refine_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, b64, s, den,
+                     x, r, p, inv, n, part_rr, part_rz0);
   MF_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(cg_check0_kernel<float>, dim3(1), dim3(VEC_THREADS), 0, st, part_rr,
                      RED_BLOCKS, sd, abs_tol, rel_tol);

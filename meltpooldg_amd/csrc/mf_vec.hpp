@@ -21,10 +21,16 @@ struct SolveState {
 
 // Device -> host copy of a few bytes that the caller waits for (<= 4 KiB)
 int read_to_host(void *dst, const void *src, size_t bytes, hipStream_t st);
+// the same in two halves: begin enqueues the copy (more work may be enqueued behind it), end
+// waits for it; no other read_to_host of the thread may come in between
+int read_to_host_begin(const void *src, size_t bytes, hipStream_t st);
+int read_to_host_end(void *dst, size_t bytes);
 
 template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop = nullptr);
 template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
-int vec_add_scaled_f32(double *x, const float *e, double scale, int64_t n, hipStream_t st);
+// x += scale * e, the scale read from scale_dev when that is set
+int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev, int64_t n,
+                       hipStream_t st);
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st);
 template <typename T> int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out, hipStream_t st);
 template <typename T> int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st);
@@ -36,10 +42,11 @@ template <typename T> int vec_scale_dev(T *dst, const T *src, int64_t n, const d
 template <typename T>
 int cg_init(const T *b, T *q, T *r, T *p, const T *inv, int64_t n, T *part_rr, T *part_rz0,
             SolveState *sd, double abs_tol, double rel_tol, hipStream_t st);
-// FP32 inner CG of the defect correction started from the FP64 residual b64 scaled by s, x = 0
-int cg_init_refine(const double *b64, double s, float *x, float *r, float *p, const float *inv,
-                   int64_t n, float *part_rr, float *part_rz0, SolveState *sd, double abs_tol,
-                   double rel_tol, hipStream_t st);
+// FP32 inner CG of the defect correction started from the FP64 residual b64 scaled by s (by
+// 1/*den when den is set, 0 for a zero norm), x = 0
+int cg_init_refine(const double *b64, double s, const double *den, float *x, float *r, float *p,
+                   const float *inv, int64_t n, float *part_rr, float *part_rz0, SolveState *sd,
+                   double abs_tol, double rel_tol, hipStream_t st);
 // one CG iteration after q = A p: reduction of the operator's p.(Ap) block partials
 // cell_part[0..ncell_part), update and direction kernels
 template <typename T>
