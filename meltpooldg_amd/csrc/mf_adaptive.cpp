@@ -255,6 +255,17 @@ int build_adaptive(mf_scratch *s, const int32_t *cells, int64_t n_cells)
       on_device && device_hanging_constraints(s, cells, n_cells, L, A->cell_nodes, nn, line_of, lptr,
                                               lcol, lval) == MF_OK;
   if (!hnc_on_device) {
+  if (nkeys.empty()) {  // numbered on the device: the node keys from the cells
+    nkeys.assign(size_t(3 * nn), 0ull);
+    for (int64_t c = 0; c < n_cells; ++c) {
+      const int32_t *cl = cells + 4 * c;
+      for (int e = 0; e < nloc; ++e) {
+        const int a[3] = {e % K, (e / K) % K, e / (K * K)};
+        const uint32_t id = A->cell_nodes[size_t(c) * nloc + e];
+        for (int d = 0; d < 3; ++d) nkeys[3 * size_t(id) + d] = axis_key(cl[0], cl[1 + d], a[d]);
+      }
+    }
+  }
   std::unordered_map<uint64_t, int64_t> leaf_index;
   leaf_index.reserve(size_t(n_cells) * 2);
   for (int64_t c = 0; c < n_cells; ++c)

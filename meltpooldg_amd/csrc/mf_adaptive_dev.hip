@@ -229,13 +229,13 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
   hipLaunchKernelGGL(assign_kernel, grid, block, 0, st, a, d_cells, sval_s, grp, nid, keys, cnodes,
                      node_keys, xyz, bnd, n_slots);
   MF_HIP_TRY(hipGetLastError());
+  // the node keys stay on the device: the constraint pass there recomputes them per slot, and
+  // the host fallback rebuilds them from the cells (nkeys left empty)
   A->cell_nodes.resize(size_t(n_slots));
-  nkeys.resize(size_t(3 * nn));
+  nkeys.clear();
   A->xyz.resize(size_t(3 * nn));
   A->boundary.resize(size_t(nn));
   MF_HIP_TRY(hipMemcpyAsync(A->cell_nodes.data(), cnodes, sizeof(uint32_t) * size_t(n_slots),
-                            hipMemcpyDeviceToHost, st));
-  MF_HIP_TRY(hipMemcpyAsync(nkeys.data(), node_keys, sizeof(uint64_t) * size_t(3 * nn),
                             hipMemcpyDeviceToHost, st));
   MF_HIP_TRY(hipMemcpyAsync(A->xyz.data(), xyz, sizeof(double) * size_t(3 * nn),
                             hipMemcpyDeviceToHost, st));

@@ -185,7 +185,8 @@ struct AdaptiveData {
 
 int build_adaptive(::mf_scratch *s, const int32_t *cells, int64_t n_cells);
 // node identification of an adaptive mesh on the device (mf_adaptive_dev.hip): cell_nodes,
-// xyz, boundary and the 3 axis keys per node, numbered as the host path numbers them
+// xyz and boundary, numbered as the host path numbers them; nkeys is left empty (the node keys
+// stay on the device)
 int device_node_identification(const ::mf_scratch *s, const int32_t *cells, int64_t n_cells,
                                int L, AdaptiveData *A, std::vector<uint64_t> &nkeys, int64_t &nn);
 // transposed hanging-node constraints (cm_node, cm_ptr, cm_h, cm_w) on the device
