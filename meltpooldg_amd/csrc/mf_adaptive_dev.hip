@@ -223,7 +223,16 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
   double *xyz;
   uint8_t *bnd;
   MF_HIP_TRY(b.alloc(&node_keys, size_t(3 * nn)));
-  MF_HIP_TRY(b.alloc(&xyz, size_t(3 * nn)));
+  // the support points stay on the device as the scratch's copy (released on every failure)
+  struct Kept {
+    void *p = nullptr;
+    ~Kept()
+    {
+      if (p) (void)hipFree(p);
+    }
+  } kept;
+  MF_HIP_TRY(hipMalloc(&kept.p, sizeof(double) * 3 * size_t(nn)));
+  xyz = static_cast<double *>(kept.p);
   MF_HIP_TRY(b.alloc(&bnd, size_t(nn)));
   MF_HIP_TRY(hipMemsetAsync(bnd, 0, size_t(nn), st));
   hipLaunchKernelGGL(assign_kernel, grid, block, 0, st, a, d_cells, sval_s, grp, nid, keys, cnodes,
@@ -241,6 +250,9 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
                             hipMemcpyDeviceToHost, st));
   MF_HIP_TRY(hipMemcpyAsync(A->boundary.data(), bnd, size_t(nn), hipMemcpyDeviceToHost, st));
   MF_HIP_TRY(hipStreamSynchronize(st));
+  if (A->d.xyz) (void)hipFree(A->d.xyz);
+  A->d.xyz = xyz;
+  kept.p = nullptr;
   return MF_OK;
 }
 
@@ -1277,11 +1289,16 @@ int device_transfer_field(const mf_scratch *old_s, const double *old_val, const 
   MF_HIP_TRY(b.alloc(&k1, size_t(n_leaves)));
   MF_HIP_TRY(b.alloc(&v0, size_t(n_leaves)));
   MF_HIP_TRY(b.alloc(&v1, size_t(n_leaves)));
-  MF_HIP_TRY(b.alloc(&xyz, 3 * size_t(n_new)));
   MF_HIP_TRY(hipMemcpyAsync(d_cells, A->cells.data(), sizeof(int32_t) * 4 * size_t(n_leaves),
                             hipMemcpyHostToDevice, st));
-  MF_HIP_TRY(hipMemcpyAsync(xyz, new_s->ad->xyz.data(), sizeof(double) * 3 * size_t(n_new),
-                            hipMemcpyHostToDevice, st));
+  // the new mesh's support points: its device copy when node identification ran there
+  const double *xyz_new = new_s->ad->d.xyz;
+  if (!xyz_new) {
+    MF_HIP_TRY(b.alloc(&xyz, 3 * size_t(n_new)));
+    MF_HIP_TRY(hipMemcpyAsync(xyz, new_s->ad->xyz.data(), sizeof(double) * 3 * size_t(n_new),
+                              hipMemcpyHostToDevice, st));
+    xyz_new = xyz;
+  }
   const dim3 block(256);
   auto grid = [](int64_t n) { return dim3(unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)))); };
   hipLaunchKernelGGL(leaf_keys_kernel, grid(n_leaves), block, 0, st, d_cells, n_leaves, k0, v0);
@@ -1292,7 +1309,7 @@ int device_transfer_field(const mf_scratch *old_s, const double *old_val, const 
   MF_HIP_TRY(b.alloc(&tmp, tb));
   MF_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, v0, v1, int(n_leaves), 0, 64, st));
   hipLaunchKernelGGL(transfer_eval_kernel, grid(n_new), block, 0, st, a, k1, v1, n_leaves,
-                     A->d.cell_nodes, old_val, xyz, n_new, new_vec);
+                     A->d.cell_nodes, old_val, xyz_new, n_new, new_vec);
   MF_HIP_TRY(hipGetLastError());
   MF_HIP_TRY(hipStreamSynchronize(st));
   return MF_OK;
