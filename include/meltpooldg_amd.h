@@ -204,7 +204,7 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
  * advection_diffusion_operator.cpp:274-358): ADDS into dst.  Heat: src = T_old, q_s = heat
  * source (may be NULL), result -R(T); on an adaptive mesh T, src and q_s are read as stored
  * (pass constraint-consistent vectors) and the hanging rows of dst are moved to their masters.
- * Advection: src = old field, q_s must be NULL (structured meshes). */
+ * Advection: src = old field, q_s must be NULL. */
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
 
 /* ------------------------------------------------------------------------------------------ */
@@ -379,7 +379,9 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
  * create_rhs_and_apply_dirichlet_matrixfree (include/meltpooldg/linear_algebra/
  * utilities_matrixfree.hpp:33-100).  phi: initial guess in, solution out; phi_old: field the
  * right-hand side is built from (the old solution, or the BDF2 combination 2 phi_n - phi_{n-1}/2
- * with mass_factor 1.5 and theta 1); dirichlet_values: NULL = homogeneous. */
+ * with mass_factor 1.5 and theta 1); dirichlet_values: NULL = homogeneous.  On an adaptive
+ * mesh phi_old must be constraint-consistent; phi comes back with its hanging values
+ * distributed. */
 int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
                       const double *dirichlet_values, const mf_solver_desc *desc,
                       mf_solve_report *report);

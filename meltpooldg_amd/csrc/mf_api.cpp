@@ -727,13 +727,12 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
   CellArgs<double> a = make_args<double>(op);
   a.aux1 = q_s;
   if (op->s->mesh.ad) {
-    // heat residual on an adaptive mesh: T, src and q_s read as stored (constraint-consistent
-    // vectors), each cell adds into its own nodes, then the hanging rows (of the cell sums and
-    // of what dst held) go to their masters
-    MF_REQUIRE(op->kind == 0, MF_ERR_ARG,
-               "the advection create_rhs is available on structured meshes only");
-    int rc3 = launch_cell_list<double>(HEAT_RHS, needs_T(op), op->vel != nullptr, op->s->mesh, a,
-                                       src, dst, op->s->stream, 0);
+    // residual on an adaptive mesh: T, src, q_s and the velocity read as stored
+    // (constraint-consistent vectors), each cell adds into its own nodes, then the hanging rows
+    // (of the cell sums and of what dst held) go to their masters
+    int rc3 = launch_cell_list<double>(op->kind == 0 ? HEAT_RHS : ADVEC_RHS, needs_T(op),
+                                       op->vel != nullptr, op->s->mesh, a, src, dst,
+                                       op->s->stream, 0);
     if (rc3) return rc3;
     return launch_condense<double>(op->s, dst, nullptr, a.use_mask != 0);
   }

@@ -1317,7 +1317,7 @@ template <typename Number, int K, int OP>
 static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                           const Number *src, Number *dst, hipStream_t st, int mode)
 {
-  if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_DIAG) {
+  if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
     return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
   } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
     if (needT) return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
@@ -1355,6 +1355,10 @@ static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
   case HEAT_RHS:
     if constexpr (sizeof(Number) == 8)
       return launch_list_op<Number, K, HEAT_RHS>(needT, vel, m, a, src, dst, st, mode);
+    break;
+  case ADVEC_RHS:
+    if constexpr (sizeof(Number) == 8)
+      return launch_list_op<Number, K, ADVEC_RHS>(needT, vel, m, a, src, dst, st, mode);
     break;
   case LS_TANGENT:
     return launch_list_op<Number, K, LS_TANGENT>(needT, vel, m, a, src, dst, st, mode);

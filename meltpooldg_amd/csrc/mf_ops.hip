@@ -489,6 +489,8 @@ int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
     if (rc != MF_OK && rc != MF_NO_CONVERGENCE && rc != MF_BREAKDOWN) break;
     const int solve_rc = rc;
     if ((rc = vec_masked_assign(phi, s->d_mask, dirichlet_values, true, n, st))) break;
+    // adaptive mesh: the hanging values from their masters (constraints.distribute)
+    if (s->ad && (rc = mf_distribute_constraints(s, phi))) break;
     rc = solve_rc;
   } while (false);
   op->dof_index = saved;

@@ -1407,11 +1407,13 @@ int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *
                           double *phi, double *norms)
 {
   orc_operator op = *op_in;
-  const long n = orc_n_dofs(&op.mesh);
+  const long n = op_ndofs(&op);
   double *old = malloc(sizeof(double) * n), *rhs = malloc(sizeof(double) * n);
   double *xd = malloc(sizeof(double) * n), *inv = malloc(sizeof(double) * n);
   int status = 0;
   for (long i = 0; i < n; ++i) xd[i] = (op.mask && op.mask[i]) ? dirichlet_values[i] : 0.0;
+  /* adaptive mesh: fields kept constraint-consistent (hanging values from their masters) */
+  if (op.amesh) orc_distribute_constraints(op.adata, phi);
   for (int step = 0; step < n_steps; ++step) {
     memcpy(old, phi, sizeof(double) * n);
     op.use_mask = 0;
@@ -1428,7 +1430,8 @@ int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *
     status |= orc_solve(&op, ORC_GMRES, 30, 10000, 1e-20, 1e-12, inv, rhs, phi, &iters, &fres);
     for (long i = 0; i < n; ++i)
       if (op.mask && op.mask[i]) phi[i] = dirichlet_values[i];
-    norms[step] = orc_fe_l2_norm(&op.mesh, phi);
+    if (op.amesh) orc_distribute_constraints(op.adata, phi);
+    norms[step] = op.amesh ? 0.0 : orc_fe_l2_norm(&op.mesh, phi);
   }
   free(old);
   free(rhs);

@@ -221,3 +221,34 @@ def test_fe_norm_on_adaptive_mesh(degree):
     uu = u[[pos[tuple(np.round(x * 2 ** 24).astype(np.int64))] for x in Xu]]
     a, b = mf.compute_norm_l2(su, _dev(uu)), mf.compute_norm_l2(ss, _dev(u))
     assert abs(a / b - 1) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["q1", "q2", "q3"])
+def test_advection_steps(name):
+    """Advection steps on an adaptive mesh (create_rhs with the Dirichlet lifting, GMRES, the
+    hanging values distributed) against the oracle's steps."""
+    P = Pair(name)
+    mf = P.mf
+    am = P.am
+    X = am.xyz
+    vel = np.stack([am.distribute(2 * (X[:, 2] - 0.5)), am.distribute(0.3 + 0 * X[:, 1]),
+                    am.distribute(-2 * (X[:, 0] - 0.5))])
+    phi0 = am.distribute(np.tanh((np.linalg.norm(X - 0.45, axis=1) - 0.25) / 0.1))
+    ap = orc.AdvecParams(dt=0.05, theta=0.5, diffusivity=0.01, mass_factor=1.0, vel_scale=0.0)
+    ref = orc.OracleOperator(orc.OP_ADVEC, None, advec=ap, vel_kind=orc.VEL_NODAL,
+                             vel=vel.ravel(), mask=am.dirichlet, amesh=am)
+    phi_ref, _, st = ref.advec_theta_steps(phi0, 2, phi0)
+    assert st == 0
+    op = mf.AdvectionDiffusionOperator(P.s, theta=0.5, diffusivity=0.01, dt=0.05)
+    vd = _dev(np.concatenate([P.to_dev_order(v) for v in vel]))
+    op.set_fields(velocity=vd)
+    phi = _dev(P.to_dev_order(phi0))
+    dv = _dev(P.to_dev_order(phi0))
+    for _ in range(2):
+        old = phi.clone()
+        rep = mf.advection_step(op, phi, old, dv, solver="GMRES", rel_tolerance=1e-12)
+        assert rep["status"] == 0
+    torch.cuda.synchronize()
+    got = phi.cpu().numpy()
+    want = P.to_dev_order(phi_ref)
+    assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want))

@@ -238,3 +238,31 @@ def test_heat_rhs_consistent_with_condensed_operator(name):
                            use_mask=0)
     expect = (1.3 / 1e-2) * M.vmult(T_old) + M.vmult(q)
     assert np.max(np.abs((r - expect)[~hang])) < 1e-12 * np.max(np.abs(expect))
+
+
+def test_advection_steps_uniform_adaptive_equals_structured():
+    """The oracle's advection steps (Dirichlet lifting, GMRES) on an adaptive mesh of level-1
+    cells only equal the structured steps on the twice finer lattice."""
+    nb = (2, 2, 2)
+    cells = np.array([(1, i, j, k) for k in range(4) for j in range(4) for i in range(4)],
+                     dtype=np.int32)
+    am = orc.AdaptiveMesh(2, nb, cells)
+    m = orc.make_mesh(3, 2, (4, 4, 4))
+    X = orc.node_coords(m)
+    perm = _coord_map(am.xyz, X)            # structured index -> adaptive index
+    vel = np.stack([2 * (X[:, 2] - 0.5), 0.3 * np.ones(len(X)), -2 * (X[:, 0] - 0.5)])
+    phi0 = np.tanh((np.linalg.norm(X - 0.5, axis=1) - 0.25) / 0.1)
+    mask = orc.boundary_mask(m)
+    ap = orc.AdvecParams(dt=0.05, theta=0.5, diffusivity=0.01, mass_factor=1.0, vel_scale=0.0)
+    ref = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, vel=vel.ravel(),
+                             mask=mask)
+    phi_s, _, st = ref.advec_theta_steps(phi0, 2, phi0)
+    assert st == 0
+    na = am.n_nodes
+    va, pa, ma = np.zeros((3, na)), np.zeros(na), np.zeros(na, dtype=np.uint8)
+    va[:, perm], pa[perm], ma[perm] = vel, phi0, mask
+    opa = orc.OracleOperator(orc.OP_ADVEC, None, advec=ap, vel_kind=orc.VEL_NODAL,
+                             vel=va.ravel(), mask=ma, amesh=am)
+    phi_a, _, st = opa.advec_theta_steps(pa, 2, pa)
+    assert st == 0
+    assert np.max(np.abs(phi_a[perm] - phi_s)) < 1e-9
