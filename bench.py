@@ -109,17 +109,6 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
-    # pick the faster of the two tangent kernels (same operator) on a short probe
-    variants = {}
-    for v in (0, 1):
-        mf.set_tangent_kernel(v)
-        for _ in range(args.warmup):
-            op.vmult(dst, src)
-        variants[v] = timed(op.vmult_add, max(3, args.steps // 5))
-    best = min(variants, key=variants.get)
-    mf.set_tangent_kernel(best)
-    print(f"# kernel probe (us): pencil {variants[0]*1e3:.1f}, slab {variants[1]*1e3:.1f}",
-          file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         op.vmult(dst, src)
     # timed region: K full vmults (zero + cell kernel), HIP events on the library's stream
@@ -212,9 +201,7 @@ def main():
                      "kernel_us_per_vmult": kernel_ms * 1e3, "alg_bytes_per_vmult": alg_bytes,
                      "fp64_tflops": achieved_tflops,
                      "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "fp32_vmult_us": f32_ms * 1e3,
-                     "tangent_kernel": ["pencil", "slab"][best],
-                     "kernel_probe_us": {"pencil": variants[0] * 1e3, "slab": variants[1] * 1e3}},
+                     "fp32_vmult_us": f32_ms * 1e3},
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -34,8 +34,8 @@ enum {
 const char *mf_last_error(void);
 int mf_version(void);
 /* Tuning knob without reference counterpart: which device kernel applies the tangent
- * (0 = pencil kernel, one LDS round trip per 1D sweep, default; 1 = slice-resident kernel).
- * Both compute the same operator; process-wide setting. */
+ * (0 = pencil kernel, one LDS round trip per 1D sweep, the only variant at present).
+ * Process-wide setting. */
 int mf_set_tangent_kernel(int variant);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -276,13 +276,29 @@ def _check(code):
         raise MFError(code, msg)
 
 
-def _ptr(t):
+def _current_device():
+    import torch
+    return torch.cuda.current_device() if torch.cuda.is_available() else -1
+
+
+def _ptr(t, n=None, dtype="float64", device=None):
+    """Device pointer of tensor t after the checks the C ABI cannot make itself: a contiguous
+    device tensor of `dtype` with exactly `n` entries (when given) on device index `device`
+    (when given).  Every kernel reads and writes the full length it was set up for, so a
+    shorter or narrower tensor is rejected here with ValueError, before any C call."""
     if t is None:
         return None
-    if not t.is_cuda:
+    if not getattr(t, "is_cuda", False):
         raise ValueError("vectors must be device tensors")
     if not t.is_contiguous():
         raise ValueError("vectors must be contiguous")
+    got = str(t.dtype).replace("torch.", "")
+    if got != dtype:
+        raise ValueError(f"vector of dtype {got}, expected {dtype}")
+    if n is not None and int(t.numel()) != int(n):
+        raise ValueError(f"vector of {int(t.numel())} entries, expected {int(n)}")
+    if device is not None and device >= 0 and t.device.index != device:
+        raise ValueError(f"vector on device {t.device.index}, expected device {device}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -301,10 +317,15 @@ class ScratchData:
         h = C.c_void_p()
         _check(lib().mf_scratch_create(C.byref(d), C.byref(h)))
         self._h = h
+        self.device = _current_device()
         self.n_dofs = int(lib().mf_scratch_n_dofs(h))
         self.n_cells = int(lib().mf_scratch_n_cells(h))
         if stream is not None:
             self.set_stream(stream)
+
+    def _v(self, t, n=None, dtype="float64"):
+        """Checked pointer of a vector of this scratch (n_dofs entries unless n is given)."""
+        return _ptr(t, self.n_dofs if n is None else n, dtype, getattr(self, "device", None))
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream (its HIP handle is passed down) or a raw handle."""
@@ -388,8 +409,8 @@ class ScratchData:
 def adaptive_transfer(old_scratch, old_vec, new_scratch, new_vec):
     """Field transfer between two adaptive meshes of the same base mesh and degree
     (mf_adaptive_transfer: the remesh step of AMR::refine_grid with SolutionTransfer)."""
-    _check(lib().mf_adaptive_transfer(old_scratch._h, _ptr(old_vec), new_scratch._h,
-                                      _ptr(new_vec)))
+    _check(lib().mf_adaptive_transfer(old_scratch._h, old_scratch._v(old_vec), new_scratch._h,
+                                      new_scratch._v(new_vec)))
 
 
 def refine_band_cells(ncell, levels, center, radius, half_width, lo=(0.0, 0.0, 0.0),
@@ -429,6 +450,7 @@ class AdaptiveScratchData(ScratchData):
                                                 self.cells.ctypes.data_as(C.c_void_p),
                                                 len(self.cells), C.byref(h)))
         self._h = h
+        self.device = _current_device()
         self.n_dofs = int(lib().mf_scratch_n_dofs(h))
         self.n_cells = int(lib().mf_scratch_n_cells(h))
         if stream is not None:
@@ -459,7 +481,7 @@ class AdaptiveScratchData(ScratchData):
 
     def distribute_constraints(self, x):
         """x_h = sum_m w x_m on the hanging nodes of the device vector x (in place)."""
-        _check(lib().mf_distribute_constraints(self._h, _ptr(x)))
+        _check(lib().mf_distribute_constraints(self._h, self._v(x)))
 
     def random_field(self, seed):
         raise NotImplementedError("lattice-keyed random fields exist on structured meshes only")
@@ -468,10 +490,18 @@ class AdaptiveScratchData(ScratchData):
 class _OperatorBase:
     """Common part of the OperatorMatrixFree mirror."""
 
+    # entries of the source vector of create_rhs, in units of n_dofs
+    _rhs_src_blocks = 1
+
     def __init__(self, scratch, handle):
         self.scratch = scratch
         self._h = handle
         self._fields = ()
+        self.n_rows = int(lib().mf_operator_n_rows(handle))
+
+    def _r(self, t, dtype="float64"):
+        """Checked pointer of a vector of the operator's rows."""
+        return self.scratch._v(t, self.n_rows, dtype)
 
     def reset_time_increment(self, dt):
         _check(lib().mf_operator_reset_time_increment(self._h, float(dt)))
@@ -482,22 +512,28 @@ class _OperatorBase:
     def set_fields(self, T=None, velocity=None, k_cell=None):
         # keep references alive: the library holds the pointers (as the reference holds const refs)
         self._fields = (T, velocity, k_cell)
-        _check(lib().mf_operator_set_fields(self._h, _ptr(T), _ptr(velocity), _ptr(k_cell)))
+        sc = self.scratch
+        _check(lib().mf_operator_set_fields(self._h, sc._v(T), sc._v(velocity, 3 * sc.n_dofs),
+                                            sc._v(k_cell, sc.n_cells)))
 
     def vmult(self, dst, src):
-        _check(lib().mf_operator_vmult(self._h, _ptr(dst), _ptr(src)))
+        _check(lib().mf_operator_vmult(self._h, self._r(dst), self._r(src)))
 
     def vmult_add(self, dst, src):
-        _check(lib().mf_operator_vmult_add(self._h, _ptr(dst), _ptr(src)))
+        _check(lib().mf_operator_vmult_add(self._h, self._r(dst), self._r(src)))
 
     def vmult_f32(self, dst, src):
-        _check(lib().mf_operator_vmult_f32(self._h, _ptr(dst), _ptr(src)))
+        _check(lib().mf_operator_vmult_f32(self._h, self._r(dst, "float32"),
+                                           self._r(src, "float32")))
 
     def compute_inverse_diagonal_from_matrixfree(self, diag):
-        _check(lib().mf_operator_compute_inverse_diagonal(self._h, _ptr(diag)))
+        _check(lib().mf_operator_compute_inverse_diagonal(self._h, self._r(diag)))
 
     def create_rhs(self, dst, src, q_s=None):
-        _check(lib().mf_operator_create_rhs(self._h, _ptr(dst), _ptr(src), _ptr(q_s)))
+        sc = self.scratch
+        _check(lib().mf_operator_create_rhs(self._h, self._r(dst),
+                                            sc._v(src, self._rhs_src_blocks * sc.n_dofs),
+                                            sc._v(q_s)))
 
     def set_transpose(self, transpose):
         """Apply A^T in vmult / solves (adjoint); the diagonal is unchanged."""
@@ -582,11 +618,7 @@ class _LevelSetProjection(_OperatorBase):
     def set_level_set(self, level_set):
         """The level set of the narrow band (kept by reference; call again after it changes)."""
         self._level_set = level_set
-        _check(lib().mf_operator_set_level_set(self._h, _ptr(level_set)))
-
-    @property
-    def n_rows(self):
-        return int(lib().mf_operator_n_rows(self._h))
+        _check(lib().mf_operator_set_level_set(self._h, self.scratch._v(level_set)))
 
 
 class NormalVectorOperator(_LevelSetProjection):
@@ -599,6 +631,7 @@ class CurvatureOperator(_LevelSetProjection):
     """CurvatureOperator (source/level_set/curvature_operator.cpp); create_rhs(dst, normal
     [3][n_dofs]) adds -div of the nodally normalised normal vector."""
     _create = "mf_curvature_operator_create"
+    _rhs_src_blocks = 3
 
     @property
     def normal_tolerance(self):
@@ -625,7 +658,8 @@ class ReinitializationOperator(_OperatorBase):
 
     def set_normal_vector(self, normal):
         self._normal = normal
-        _check(lib().mf_operator_set_normal_vector(self._h, _ptr(normal)))
+        _check(lib().mf_operator_set_normal_vector(self._h,
+                                                   self.scratch._v(normal, 3 * self.scratch.n_dofs)))
 
     @property
     def min_epsilon(self):
@@ -655,19 +689,19 @@ def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
 def compute_norm_l2(scratch, u):
     """FE L2 norm of a device vector (VectorTools::compute_norm semantics)."""
     r = C.c_double()
-    _check(lib().mf_compute_norm_l2(scratch._h, _ptr(u), C.byref(r)))
+    _check(lib().mf_compute_norm_l2(scratch._h, scratch._v(u), C.byref(r)))
     return r.value
 
 
 def laser_heat_source(scratch, q_s, power, radius, position):
     """Nodal Gaussian volumetric laser source (LaserHeatSourceVolumetric)."""
     pos = (C.c_double * 3)(*position)
-    _check(lib().mf_laser_heat_source(scratch._h, power, radius, pos, _ptr(q_s)))
+    _check(lib().mf_laser_heat_source(scratch._h, power, radius, pos, scratch._v(q_s)))
 
 
 def predictor_linear_extrapolation(scratch, dst, x_n, x_nm1, tau=1.0):
-    _check(lib().mf_predictor_linear_extrapolation(scratch._h, _ptr(dst), _ptr(x_n),
-                                                   _ptr(x_nm1), float(tau)))
+    _check(lib().mf_predictor_linear_extrapolation(scratch._h, scratch._v(dst), scratch._v(x_n),
+                                                   scratch._v(x_nm1), float(tau)))
 
 
 def heat_newton_step(operator, T, T_old, q_s=None, dirichlet_values=None, newton=None,
@@ -675,8 +709,9 @@ def heat_newton_step(operator, T, T_old, q_s=None, dirichlet_values=None, newton
     """One implicit heat step solved by Newton (HeatDiffuseOperation::solve)."""
     nd = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
     rep = NewtonReport()
-    code = lib().mf_heat_newton_step(operator._h, _ptr(T), _ptr(T_old), _ptr(q_s),
-                                     _ptr(dirichlet_values), C.byref(_solver_desc(**solver)),
+    sc = operator.scratch
+    code = lib().mf_heat_newton_step(operator._h, sc._v(T), sc._v(T_old), sc._v(q_s),
+                                     sc._v(dirichlet_values), C.byref(_solver_desc(**solver)),
                                      C.byref(nd), C.byref(rep))
     _check(code)
     return rep.as_dict()
@@ -685,30 +720,33 @@ def heat_newton_step(operator, T, T_old, q_s=None, dirichlet_values=None, newton
 def advection_step(operator, phi, phi_old, dirichlet_values=None, **solver):
     """One advection-diffusion step with Dirichlet lifting (AdvectionDiffusionOperation::solve)."""
     rep = SolveReport()
-    code = lib().mf_advection_step(operator._h, _ptr(phi), _ptr(phi_old), _ptr(dirichlet_values),
+    sc = operator.scratch
+    code = lib().mf_advection_step(operator._h, sc._v(phi), sc._v(phi_old), sc._v(dirichlet_values),
                                    C.byref(_solver_desc(**solver)), C.byref(rep))
     _check(code)
     return rep.as_dict()
 
 
 def mass_vmult(scratch, dst, src):
-    _check(lib().mf_mass_vmult(scratch._h, _ptr(dst), _ptr(src)))
+    _check(lib().mf_mass_vmult(scratch._h, scratch._v(dst), scratch._v(src)))
 
 
 def vector_dot(scratch, a, b):
     r = C.c_double()
-    _check(lib().mf_vector_dot(scratch._h, _ptr(a), _ptr(b), C.byref(r)))
+    _check(lib().mf_vector_dot(scratch._h, scratch._v(a), scratch._v(b), C.byref(r)))
     return r.value
 
 
 def conductivity_gradient(operator, lam, T, grad_k_cell, scale=1.0):
-    _check(lib().mf_heat_conductivity_gradient(operator._h, _ptr(lam), _ptr(T), float(scale),
-                                               _ptr(grad_k_cell)))
+    sc = operator.scratch
+    _check(lib().mf_heat_conductivity_gradient(operator._h, sc._v(lam), sc._v(T), float(scale),
+                                               sc._v(grad_k_cell, sc.n_cells)))
 
 
 def laser_heat_source_radius_derivative(scratch, dq, power, radius, position):
     pos = (C.c_double * 3)(*position)
-    _check(lib().mf_laser_heat_source_radius_derivative(scratch._h, power, radius, pos, _ptr(dq)))
+    _check(lib().mf_laser_heat_source_radius_derivative(scratch._h, power, radius, pos,
+                                                        scratch._v(dq)))
 
 
 class HeatTimeStepper:
@@ -737,13 +775,15 @@ class HeatTimeStepper:
         d.newton = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
         self._desc = d
         h = C.c_void_p()
-        _check(lib().mf_heat_stepper_create(operator._h, C.byref(d), _ptr(T0), C.byref(h)))
+        _check(lib().mf_heat_stepper_create(operator._h, C.byref(d), operator.scratch._v(T0),
+                                            C.byref(h)))
         self._h = h
         self._torch = torch
 
     def step(self, dt, dirichlet_values=None):
         rep = NewtonReport()
-        code = lib().mf_heat_stepper_step(self._h, float(dt), _ptr(dirichlet_values),
+        code = lib().mf_heat_stepper_step(self._h, float(dt),
+                                          self.operator.scratch._v(dirichlet_values),
                                           C.byref(rep))
         _check(code)
         return rep.as_dict()
@@ -759,7 +799,7 @@ class HeatTimeStepper:
     def _copy(self, which):
         torch = self._torch
         out = torch.empty(self.n_dofs, dtype=torch.float64, device="cuda")
-        _check(lib().mf_heat_stepper_copy(self._h, which, _ptr(out)))
+        _check(lib().mf_heat_stepper_copy(self._h, which, self.operator.scratch._v(out)))
         return out
 
     def solution(self):
@@ -810,11 +850,13 @@ class HeatTransient:
         self._h = h
 
     def forward(self, T0, T_final):
-        _check(lib().mf_transient_forward(self._h, _ptr(T0), _ptr(T_final)))
+        sc = self.operator.scratch
+        _check(lib().mf_transient_forward(self._h, sc._v(T0), sc._v(T_final)))
 
     def backward(self, dJ_dTN, grad_k_cell=None):
         gp, gr = C.c_double(), C.c_double()
-        _check(lib().mf_transient_backward(self._h, _ptr(dJ_dTN), _ptr(grad_k_cell),
+        sc = self.operator.scratch
+        _check(lib().mf_transient_backward(self._h, sc._v(dJ_dTN), sc._v(grad_k_cell, sc.n_cells),
                                            C.byref(gp), C.byref(gr)))
         return gp.value, gr.value
 
@@ -841,10 +883,10 @@ class JacobiPreconditioner:
         _check(lib().mf_jacobi_update(self._h))
 
     def vmult(self, dst, src):
-        _check(lib().mf_jacobi_vmult(self._h, _ptr(dst), _ptr(src)))
+        _check(lib().mf_jacobi_vmult(self._h, self.operator._r(dst), self.operator._r(src)))
 
     def inverse_diagonal(self, dst):
-        _check(lib().mf_jacobi_inverse_diagonal(self._h, _ptr(dst)))
+        _check(lib().mf_jacobi_inverse_diagonal(self._h, self.operator._r(dst)))
 
     def __del__(self):
         try:
@@ -877,12 +919,13 @@ class LinearSolver:
         if isinstance(preconditioner, JacobiPreconditioner):
             d.preconditioner = PRECOND_JACOBI
             code = lib().mf_solve_preconditioned(operator._h, C.byref(d), preconditioner._h,
-                                                 _ptr(solution), _ptr(rhs), C.byref(rep))
+                                                 operator._r(solution), operator._r(rhs),
+                                                 C.byref(rep))
         else:
             d.preconditioner = {"Identity": PRECOND_IDENTITY,
                                 "Diagonal": PRECOND_JACOBI}[preconditioner]
-            code = lib().mf_solve(operator._h, C.byref(d), _ptr(solution), _ptr(rhs),
-                                  C.byref(rep))
+            code = lib().mf_solve(operator._h, C.byref(d), operator._r(solution),
+                                  operator._r(rhs), C.byref(rep))
         if code != MF_OK and (raise_on_failure or code not in (MF_NO_CONVERGENCE, MF_BREAKDOWN)):
             _check(code)
         return rep.as_dict()
@@ -1073,14 +1116,14 @@ def write_vtu(scratch, filename, fields):
         ncomp.append(1 if m == n else 3)
     arr_n = (C.c_char_p * len(names))(*[k.encode() for k in names])
     arr_c = (C.c_int * len(names))(*ncomp)
-    arr_f = (C.c_void_p * len(names))(*[_ptr(fields[k]) for k in names])
+    arr_f = (C.c_void_p * len(names))(*[scratch._v(fields[k], fields[k].numel()) for k in names])
     _check(lib().mf_write_vtu(scratch._h, str(filename).encode(), len(names), arr_n, arr_c,
                               arr_f))
 
 
 def restart_save(scratch, path, vectors):
     """Restart file with the mesh description and the given device vectors (mf_restart_save)."""
-    arr_v = (C.c_void_p * len(vectors))(*[_ptr(v) for v in vectors])
+    arr_v = (C.c_void_p * len(vectors))(*[scratch._v(v, v.numel()) for v in vectors])
     arr_l = (C.c_int64 * len(vectors))(*[v.numel() for v in vectors])
     _check(lib().mf_restart_save(scratch._h, str(path).encode(), len(vectors), arr_v, arr_l))
 
@@ -1104,12 +1147,13 @@ def restart_load(path, vectors=None, stream=None):
         s.cells = np.fromfile(str(path), dtype=np.int32, count=4 * n_cells,
                               offset=84).reshape(-1, 4)
     s._h = h
+    s.device = _current_device()
     s.n_dofs = int(lib().mf_scratch_n_dofs(h))
     s.n_cells = int(lib().mf_scratch_n_cells(h))
     if stream is not None:
         s.set_stream(stream)
     if vectors:
-        arr_v = (C.c_void_p * len(vectors))(*[_ptr(v) for v in vectors])
+        arr_v = (C.c_void_p * len(vectors))(*[s._v(v, v.numel()) for v in vectors])
         arr_l = (C.c_int64 * len(vectors))(*[v.numel() for v in vectors])
         _check(lib().mf_restart_load_vectors(s._h, str(path).encode(), len(vectors), arr_v,
                                              arr_l))

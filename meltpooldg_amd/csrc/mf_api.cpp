@@ -298,7 +298,6 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
 {
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
-  const bool slab = g_tangent_variant == 1 && !op->transpose && !m.ad && op->kind < 2;
   if (n_blocks(op) > 1) {
     // the same scalar operator on every block; the fused dot's partials follow block by block
     const int per = op_tangent_blocks(op, sizeof(Number) == 4) / n_blocks(op);
@@ -312,14 +311,14 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     }
     return MF_OK;
   }
-  if (overwrite && (m.ad || slab)) {
+  if (overwrite && m.ad) {
     int rc = vec_fill<Number>(dst, Number(0), m.n_dofs, st, stop);
     if (rc) return rc;
   }
   CellArgs<Number> a = make_args<Number>(op);
   a.dot_partial = dot_partial;
   a.stop = stop;
-  a.overwrite = (overwrite && !m.ad && !slab) ? 1 : 0;
+  a.overwrite = (overwrite && !m.ad) ? 1 : 0;
   const int code = op->kind == 0 ? HEAT_TANGENT
                  : op->kind == 1 ? ADVEC_TANGENT
                  : op->kind == 4 ? REINIT_TANGENT
@@ -341,8 +340,7 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     return launch_condense<Number>(op->s, dst, stop, a.use_mask != 0);
   }
   // the transposed operator (adjoint) and the level-set projections run on the pencil kernel
-  int rc = slab ? launch_slab_tangent<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st)
-                : launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
+  int rc = launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   if (rc) return rc;
   // Robin boundary faces (symmetric: the same for the transposed operator); their part of the
   // fused dot follows the cell kernel's partials
@@ -353,7 +351,6 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
                   : nullptr);
 }
 
-int g_tangent_variant = 0;
 int g_adaptive_setup_device = 1;
 
 }  // namespace mf
@@ -377,8 +374,7 @@ int mf_set_adaptive_setup(int device)
 
 int mf_set_tangent_kernel(int variant)
 {
-  MF_REQUIRE(variant == 0 || variant == 1, MF_ERR_ARG, "tangent kernel variant must be 0 or 1");
-  g_tangent_variant = variant;
+  MF_REQUIRE(variant == 0, MF_ERR_ARG, "tangent kernel variant must be 0");
   return MF_OK;
 }
 

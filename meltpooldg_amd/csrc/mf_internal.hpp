@@ -260,18 +260,8 @@ int cell_blocks(const MeshInfo &m, bool fp32);
 // reinitialization (src = level set, normal vector = op->vel, dst [n_dofs]); dst += ...
 int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, double *dst);
 
-// slice-resident tangent kernel (mf_slab.hip)
-template <typename Number>
-int launch_slab_tangent(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
-                        const Number *src, Number *dst, hipStream_t stream);
-int slab_blocks(const MeshInfo &m);
-
-// tangent kernel selection: 0 = pencil kernel (mf_cell.hip), 1 = slice-resident (mf_slab.hip)
-extern int g_tangent_variant;
-inline int tangent_blocks(const MeshInfo &m, bool transpose, bool fp32)
-{
-  return (g_tangent_variant == 1 && !transpose && !m.ad) ? slab_blocks(m) : cell_blocks(m, fp32);
-}
+// blocks of one tangent application (size of the fused-dot partials)
+inline int tangent_blocks(const MeshInfo &m, bool, bool fp32) { return cell_blocks(m, fp32); }
 
 }  // namespace mf
 

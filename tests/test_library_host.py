@@ -194,3 +194,58 @@ def test_cpp_header_compiles(tmp_path):
                         "-I/opt/rocm/include", "-I" + inc, str(src)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+class _FakeDeviceTensor:
+    """Stands in for a device tensor in the argument checks (nothing is dereferenced)."""
+
+    def __init__(self, n, dtype="float64", index=0, contiguous=True):
+        import torch
+        self._n, self.dtype = n, getattr(torch, dtype)
+        self.device = type("D", (), {"index": index})()
+        self._contiguous = contiguous
+        self.is_cuda = True
+
+    def numel(self):
+        return self._n
+
+    def is_contiguous(self):
+        return self._contiguous
+
+    def data_ptr(self):
+        return 0x1000
+
+
+def test_vector_arguments_are_checked_before_any_call():
+    """Length, dtype, layout and device of every vector are checked on the host: a float32 or
+    short tensor handed to an FP64 entry point raises ValueError instead of reaching a kernel."""
+    import torch
+    ok = _FakeDeviceTensor(10)
+    assert mf._ptr(ok, 10, "float64", 0).value == 0x1000
+    with pytest.raises(ValueError, match="dtype"):
+        mf._ptr(_FakeDeviceTensor(10, "float32"), 10)
+    with pytest.raises(ValueError, match="entries"):
+        mf._ptr(_FakeDeviceTensor(9), 10)
+    with pytest.raises(ValueError, match="entries"):
+        mf._ptr(_FakeDeviceTensor(30), 10)
+    with pytest.raises(ValueError, match="device 1"):
+        mf._ptr(_FakeDeviceTensor(10, index=1), 10, "float64", 0)
+    with pytest.raises(ValueError, match="contiguous"):
+        mf._ptr(_FakeDeviceTensor(10, contiguous=False), 10)
+    with pytest.raises(ValueError, match="device tensors"):
+        mf._ptr(torch.zeros(10, dtype=torch.float64), 10)
+    assert mf._ptr(_FakeDeviceTensor(4, "float32"), 4, "float32").value == 0x1000
+    # through the operator wrappers: the check fires before the C call
+    s = mf.ScratchData(1, 2)
+    op = mf.HeatDiffuseOperator(s, dt=1.0)
+    assert op.n_rows == s.n_dofs == 27
+    with pytest.raises(ValueError, match="entries"):
+        op.vmult(_FakeDeviceTensor(27), _FakeDeviceTensor(26))
+    with pytest.raises(ValueError, match="dtype"):
+        op.vmult_f32(_FakeDeviceTensor(27), _FakeDeviceTensor(27))
+    with pytest.raises(ValueError, match="entries"):
+        op.set_fields(velocity=_FakeDeviceTensor(27))
+    with pytest.raises(ValueError, match="entries"):
+        op.set_fields(k_cell=_FakeDeviceTensor(27))
+    with pytest.raises(ValueError, match="entries"):
+        mf.LinearSolver.solve(op, _FakeDeviceTensor(27), _FakeDeviceTensor(28))
