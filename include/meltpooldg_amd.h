@@ -349,13 +349,23 @@ int mf_solve_preconditioned(mf_operator *op, const mf_solver_desc *desc, mf_jaco
 /* ------------------------------------------------------------------------------------------ */
 /* Callers on either side of the operator                                                      */
 /* ------------------------------------------------------------------------------------------ */
-/* NonlinearSolverData (include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:13-20):
- * absolute tolerances; 0 selects the reference defaults (10, 1e-9, 1e-10). */
+/* NonlinearSolverData (include/meltpooldg/linear_algebra/nonlinear_solver_data.hpp:13-20).
+ * Every field is used as given (a tolerance of 0 cannot be met); mf_newton_desc_init fills the
+ * reference defaults (10, 1e-9, 1e-10, 0, 1e-8, 1e-9, relative off).  Convergence: |r| < tol_r
+ * and |d| < tol_d, checked before d is added; with use_relative_tolerances |r| is divided by
+ * the first iteration's |r| and |d| by the current |x| (each only when that reference norm is
+ * positive).  After max_iterations the *_alt tolerances apply for max_iterations_alt more. */
 typedef struct {
-  int max_iterations;
+  int max_iterations;              /* max_nonlinear_iterations */
   double residual_tolerance;
-  double correction_tolerance;
+  double correction_tolerance;     /* field_correction_tolerance */
+  int max_iterations_alt;          /* max_nonlinear_iterations_alt */
+  double residual_tolerance_alt;
+  double correction_tolerance_alt; /* field_correction_tolerance_alt */
+  int use_relative_tolerances;
 } mf_newton_desc;
+
+int mf_newton_desc_init(mf_newton_desc *d);
 
 typedef struct {
   int iterations;          /* Newton iterations (linear solves) */
@@ -364,6 +374,27 @@ typedef struct {
   double correction_norm;  /* |d| of the last iteration */
   int status;
 } mf_newton_report;
+
+/* The Newton-Raphson driver itself (NewtonRaphsonSolver::solve, source/linear_algebra/
+ * newton_raphson_solver.cpp:22-175), on the host, over the caller's state through callbacks
+ * (the reference's std::function members residual / solve_with_jacobian /
+ * distribute_constraints / norm_of_solution_vector, newton_raphson_solver.hpp).  Each callback
+ * returns an MF_* status; a non-zero status ends the solve with that status.
+ *   residual(user): rhs = -R(x) at the current iterate
+ *   solve_with_jacobian(user, &linear_iterations): d = J^-1 rhs
+ *   norms(user, &|rhs|, &|d|, &|x|): l2 norms (|x| only needed when need_x is non-zero)
+ *   update(user): x += d, then distribute the constraints
+ * The report's residual_norm / correction_norm are the last absolute norms. */
+typedef struct {
+  int (*residual)(void *user);
+  int (*solve_with_jacobian)(void *user, int *linear_iterations);
+  int (*norms)(void *user, int need_x, double *residual_norm, double *update_norm,
+               double *solution_norm);
+  int (*update)(void *user);
+} mf_newton_callbacks;
+
+int mf_newton_solve(const mf_newton_desc *desc, const mf_newton_callbacks *cb, void *user,
+                    mf_newton_report *report);
 
 /* One implicit heat step: NewtonRaphsonSolver::solve (source/linear_algebra/
  * newton_raphson_solver.cpp:22-80) with the residual / Jacobian wiring of HeatDiffuseOperation

@@ -67,8 +67,43 @@ class TransientDesc(C.Structure):
 
 
 class NewtonDesc(C.Structure):
+    """mf_newton_desc = NonlinearSolverData (nonlinear_solver_data.hpp:13-20)."""
     _fields_ = [("max_iterations", C.c_int), ("residual_tolerance", C.c_double),
-                ("correction_tolerance", C.c_double)]
+                ("correction_tolerance", C.c_double), ("max_iterations_alt", C.c_int),
+                ("residual_tolerance_alt", C.c_double),
+                ("correction_tolerance_alt", C.c_double),
+                ("use_relative_tolerances", C.c_int)]
+
+
+def newton_desc(newton=None):
+    """NewtonDesc from None (the reference defaults), a tuple (max_iterations,
+    residual_tolerance, correction_tolerance) or a dict of NewtonDesc fields; unset fields keep
+    the reference defaults."""
+    d = NewtonDesc(10, 1e-9, 1e-10, 0, 1e-8, 1e-9, 0)
+    if newton is None:
+        return d
+    if isinstance(newton, dict):
+        for k, v in newton.items():
+            if k not in dict(NewtonDesc._fields_):
+                raise ValueError(f"unknown Newton parameter {k}")
+            setattr(d, k, type(getattr(d, k))(v))
+        return d
+    d.max_iterations, d.residual_tolerance, d.correction_tolerance = newton
+    return d
+
+
+_NEWTON_RESIDUAL = C.CFUNCTYPE(C.c_int, C.c_void_p)
+_NEWTON_SOLVE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int))
+_NEWTON_NORMS = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                            C.POINTER(C.c_double), C.POINTER(C.c_double))
+_NEWTON_UPDATE = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+
+class NewtonCallbacks(C.Structure):
+    """mf_newton_callbacks: the residual / solve_with_jacobian / norms / update rules of
+    NewtonRaphsonSolver (newton_raphson_solver.hpp)."""
+    _fields_ = [("residual", _NEWTON_RESIDUAL), ("solve_with_jacobian", _NEWTON_SOLVE),
+                ("norms", _NEWTON_NORMS), ("update", _NEWTON_UPDATE)]
 
 
 PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR_EXTRAPOLATION = 0, 1, 2
@@ -217,6 +252,9 @@ def lib():
                                               C.POINTER(SolveReport)]
         L.mf_set_tangent_kernel.argtypes = [C.c_int]
         L.mf_set_adaptive_setup.argtypes = [C.c_int]
+        L.mf_newton_desc_init.argtypes = [C.POINTER(NewtonDesc)]
+        L.mf_newton_solve.argtypes = [C.POINTER(NewtonDesc), C.POINTER(NewtonCallbacks), vp,
+                                      C.POINTER(NewtonReport)]
         L.mf_heat_newton_step.argtypes = [vp, vp, vp, vp, vp, C.POINTER(SolverDesc),
                                           C.POINTER(NewtonDesc), C.POINTER(NewtonReport)]
         L.mf_advection_step.argtypes = [vp, vp, vp, vp, C.POINTER(SolverDesc),
@@ -704,10 +742,47 @@ def predictor_linear_extrapolation(scratch, dst, x_n, x_nm1, tau=1.0):
                                                    scratch._v(x_nm1), float(tau)))
 
 
+def newton_solve(residual, solve_with_jacobian, norms, update, newton=None):
+    """NewtonRaphsonSolver::solve over Python callables (mf_newton_solve; host only):
+    residual() computes rhs = -R(x); solve_with_jacobian() -> linear iterations;
+    norms(need_x) -> (|rhs|, |d|, |x|); update() does x += d.  Returns the report dict; raises
+    SolverNoConvergence like ExcNewtonDidNotConverge."""
+    def wrap(f):
+        def g(*a):
+            try:
+                return f(*a)
+            except Exception:  # an exception may not cross the C frames
+                return MF_ERR_STATE
+        return g
+
+    def res(_):
+        residual()
+        return MF_OK
+
+    def sol(_, its):
+        its[0] = int(solve_with_jacobian())
+        return MF_OK
+
+    def nrm(_, need_x, r, d, x):
+        r[0], d[0], x[0] = (float(v) for v in norms(bool(need_x)))
+        return MF_OK
+
+    def upd(_):
+        update()
+        return MF_OK
+
+    cb = NewtonCallbacks(_NEWTON_RESIDUAL(wrap(res)), _NEWTON_SOLVE(wrap(sol)),
+                         _NEWTON_NORMS(wrap(nrm)), _NEWTON_UPDATE(wrap(upd)))
+    rep = NewtonReport()
+    code = lib().mf_newton_solve(C.byref(newton_desc(newton)), C.byref(cb), None, C.byref(rep))
+    _check(code)
+    return rep.as_dict()
+
+
 def heat_newton_step(operator, T, T_old, q_s=None, dirichlet_values=None, newton=None,
                      **solver):
     """One implicit heat step solved by Newton (HeatDiffuseOperation::solve)."""
-    nd = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
+    nd = newton_desc(newton)
     rep = NewtonReport()
     sc = operator.scratch
     code = lib().mf_heat_newton_step(operator._h, sc._v(T), sc._v(T_old), sc._v(q_s),
@@ -772,7 +847,7 @@ class HeatTimeStepper:
         d.start_time = start_time
         d.predictor = predictor
         d.solver = _solver_desc(**solver)
-        d.newton = NewtonDesc(*(newton or (10, 1e-9, 1e-10)))
+        d.newton = newton_desc(newton)
         self._desc = d
         h = C.c_void_p()
         _check(lib().mf_heat_stepper_create(operator._h, C.byref(d), operator.scratch._v(T0),

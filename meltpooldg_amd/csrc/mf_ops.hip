@@ -500,12 +500,71 @@ int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
 }
 
 // One implicit-Euler heat step solved with NewtonRaphsonSolver::solve
-// (source/linear_algebra/newton_raphson_solver.cpp:22-80, wired as in
-// source/heat/heat_diffuse_operation.cpp:86-116,261-289): the preconditioner is updated once,
-// then r = -R(T) (create_rhs), solve J d = r, converged when |r| < residual_tol and
-// |d| < correction_tol (checked before d is added), else T += d and the constraints are
-// distributed.  T holds the predictor on entry and the converged temperature on exit; the
-// operator's linearisation field is T itself.
+// (source/linear_algebra/newton_raphson_solver.cpp:22-175, wired as in
+// source/heat/heat_diffuse_operation.cpp:86-116,261-289) through the host driver
+// mf_newton_solve: residual r = -R(T) (create_rhs, with T as the linearisation field), J d = r
+// by the linear solver, the l2 norms of r, d (and T for relative tolerances) read back once per
+// iteration, T += d with the constraints distributed.  T holds the predictor on entry and the
+// converged temperature on exit.
+namespace {
+struct HeatNewtonState {
+  mf_operator *op;
+  double *T;
+  const double *T_old, *q_s;
+  const mf_solver_desc *desc;
+  double *r, *d, *nrm, *part;
+  const double *vel, *kc;
+  int64_t n;
+};
+
+int heat_residual(void *u)
+{
+  auto *h = static_cast<HeatNewtonState *>(u);
+  hipStream_t st = h->op->s->stream;
+  int rc;
+  if ((rc = mf_operator_set_fields(h->op, h->T, h->vel, h->kc))) return rc;
+  if ((rc = vec_fill<double>(h->r, 0.0, h->n, st))) return rc;
+  return mf_operator_create_rhs(h->op, h->r, h->T_old, h->q_s);
+}
+
+int heat_jacobian_solve(void *u, int *lin_its)
+{
+  auto *h = static_cast<HeatNewtonState *>(u);
+  int rc;
+  if ((rc = vec_fill<double>(h->d, 0.0, h->n, h->op->s->stream))) return rc;
+  mf_solve_report lin{};
+  rc = mf_solve(h->op, h->desc, h->d, h->r, &lin);
+  *lin_its = lin.iterations;
+  return rc;
+}
+
+int heat_norms(void *u, int need_x, double *rn, double *dn, double *xn)
+{
+  auto *h = static_cast<HeatNewtonState *>(u);
+  hipStream_t st = h->op->s->stream;
+  int rc;
+  if ((rc = vec_norm<double>(h->r, h->n, h->part, h->nrm, st))) return rc;
+  if ((rc = vec_norm<double>(h->d, h->n, h->part, h->nrm + 1, st))) return rc;
+  if (need_x && (rc = vec_norm<double>(h->T, h->n, h->part, h->nrm + 2, st))) return rc;
+  double v[3] = {0.0, 0.0, 0.0};
+  if ((rc = read_to_host(v, h->nrm, sizeof(double) * (need_x ? 3 : 2), st))) return rc;
+  *rn = v[0];
+  *dn = v[1];
+  *xn = v[2];
+  return MF_OK;
+}
+
+int heat_update(void *u)
+{
+  auto *h = static_cast<HeatNewtonState *>(u);
+  int rc;
+  if ((rc = vec_axpby(1.0, h->d, 1.0, h->T, h->n, h->op->s->stream))) return rc;
+  // adaptive mesh: the iterate stays constraint-consistent (distribute_constraints)
+  if (h->op->s->ad) return mf_distribute_constraints(h->op->s, h->T);
+  return MF_OK;
+}
+}  // namespace
+
 int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const double *q_s,
                         const double *dirichlet_values, const mf_solver_desc *desc,
                         const mf_newton_desc *newton, mf_newton_report *report)
@@ -515,58 +574,37 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
   mf_scratch *s = op->s;
   const int64_t n = s->mesh.n_dofs;
   hipStream_t st = s->stream;
-  double *r = nullptr, *d = nullptr, *nrm = nullptr, *part = nullptr;
-  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&r), sizeof(double) * n, st));
-  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&d), sizeof(double) * n, st));
-  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&nrm), sizeof(double) * 2, st));
-  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * 2 * RED_BLOCKS, st));
-  mf_newton_report rep{};
+  HeatNewtonState h{op, T, T_old, q_s, desc, nullptr, nullptr, nullptr, nullptr, op->vel,
+                    op->k_cell, n};
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&h.r), sizeof(double) * n, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&h.d), sizeof(double) * n, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&h.nrm), sizeof(double) * 3, st));
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&h.part), sizeof(double) * 2 * RED_BLOCKS, st));
   int rc = MF_OK;
-  const int max_it = newton->max_iterations > 0 ? newton->max_iterations : 10;
-  const double tol_r = newton->residual_tolerance > 0 ? newton->residual_tolerance : 1e-9;
-  const double tol_d = newton->correction_tolerance > 0 ? newton->correction_tolerance : 1e-10;
   // constraints of the predictor (AffineConstraints::distribute)
   if (dirichlet_values) rc = vec_masked_assign(T, s->d_mask, dirichlet_values, true, n, st);
   // on an adaptive mesh the iterate is kept constraint-consistent (hanging values from their
   // masters): NewtonRaphsonSolver's distribute_constraints after every update
   if (rc == MF_OK && s->ad) rc = mf_distribute_constraints(s, T);
-  const double *vel = op->vel, *kc = op->k_cell;
-  bool converged = false;
-  for (int it = 0; rc == MF_OK && it < max_it; ++it) {
-    if ((rc = mf_operator_set_fields(op, T, vel, kc))) break;
-    if ((rc = vec_fill<double>(r, 0.0, n, st))) break;
-    if ((rc = mf_operator_create_rhs(op, r, T_old, q_s))) break;
-    if ((rc = vec_fill<double>(d, 0.0, n, st))) break;
-    mf_solve_report lin{};
-    rc = mf_solve(op, desc, d, r, &lin);
-    rep.linear_iterations += lin.iterations;
-    if (rc != MF_OK) break;
-    double h[2];
-    if ((rc = vec_norm<double>(r, n, part, nrm, st))) break;
-    if ((rc = vec_norm<double>(d, n, part, nrm + 1, st))) break;
-    if ((rc = read_to_host(h, nrm, sizeof(double) * 2, st))) break;
-    rep.iterations = it + 1;
-    rep.residual_norm = h[0];
-    rep.correction_norm = h[1];
-    if (h[0] < tol_r && h[1] < tol_d) {
-      converged = true;
-      break;
-    }
-    if ((rc = vec_axpby(1.0, d, 1.0, T, n, st))) break;
-    if (s->ad && (rc = mf_distribute_constraints(s, T))) break;
+  mf_newton_report rep{};
+  if (rc == MF_OK) {
+    const mf_newton_callbacks cb{heat_residual, heat_jacobian_solve, heat_norms, heat_update};
+    rc = mf_newton_solve(newton, &cb, &h, &rep);
   }
-  (void)hipFreeAsync(r, st);
-  (void)hipFreeAsync(d, st);
-  (void)hipFreeAsync(nrm, st);
-  (void)hipFreeAsync(part, st);
-  if (rc == MF_OK) rc = mf_operator_set_fields(op, T, vel, kc);
+  (void)hipFreeAsync(h.r, st);
+  (void)hipFreeAsync(h.d, st);
+  (void)hipFreeAsync(h.nrm, st);
+  (void)hipFreeAsync(h.part, st);
+  // the operator's linearisation field is the final temperature
+  if (rc == MF_OK || rc == MF_NO_CONVERGENCE) {
+    const int rc2 = mf_operator_set_fields(op, T, h.vel, h.kc);
+    if (rc == MF_OK) rc = rc2;
+  }
   if (rc == MF_OK) MF_HIP_TRY(hipStreamSynchronize(st));
-  rep.status = rc != MF_OK ? rc : (converged ? MF_OK : MF_NO_CONVERGENCE);
+  rep.status = rc;
   if (report) *report = rep;
-  if (rc == MF_OK && !converged) {
+  if (rc == MF_NO_CONVERGENCE)
     set_error("Newton-Raphson solver did not converge (ExcNewtonDidNotConverge)");
-    return MF_NO_CONVERGENCE;
-  }
   return rc;
 }
 

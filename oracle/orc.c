@@ -372,6 +372,24 @@ static void q_point(const geo_t *g, long cell, int q, double *x)
 /* source/heat/heat_diffuse_operator.cpp:1167-1210)                                            */
 /* ------------------------------------------------------------------------------------------ */
 
+/* Mushy-zone blend of one property (Material::compute_solid_liquid_phases_property and its
+ * temperature derivative, material.templates.hpp:546-600; interpolate_cubic and its derivative,
+ * level_set_tools.hpp:72-89): value l + (s - l)(-2 f^3 + 3 f^2) (l at f = 0, s at f = 1 exactly)
+ * and d/dT = -(s - l)(-6 f^2 + 6 f) / (T_liq - T_sol), zero at f = 0 and f = 1. */
+static void mushy_blend(double f, double inv_mushy, double l, double s, double *v, double *dv)
+{
+  if (f == 0.0) {
+    *v = l;
+    *dv = 0.0;
+  } else if (f == 1.0) {
+    *v = s;
+    *dv = 0.0;
+  } else {
+    *v = l + (s - l) * (-2.0 * f * f * f + 3.0 * f * f);
+    *dv = -1.0 * inv_mushy * ((s - l) * (-6.0 * f * f + 6.0 * f));
+  }
+}
+
 static void material(const orc_heat_params *hp, double T, double *rc, double *k, double *drc,
                      double *dk)
 {
@@ -383,10 +401,29 @@ static void material(const orc_heat_params *hp, double T, double *rc, double *k,
     const double c = cosh(x);
     *k = hp->k + hp->tanh_amp * t;
     *dk = hp->tanh_amp / (hp->tanh_width * c * c);
+  } else if (hp->law == ORC_LAW_MUSHY) {
+    /* compute_temperature_dependent_solid_fraction (material.templates.hpp:748-760) with
+     * limit_to_bounds (utility_functions.hpp:62-82) */
+    const double inv_mushy = 1.0 / (hp->liquidus - hp->solidus);
+    double f = (hp->liquidus - T) * inv_mushy;
+    f = f > 1.0 ? 1.0 : f;
+    f = f < 0.0 ? 0.0 : f;
+    double rho, drho, cp, dcp;
+    mushy_blend(f, inv_mushy, hp->liquid_k, hp->solid_k, k, dk);
+    mushy_blend(f, inv_mushy, hp->liquid_rho, hp->solid_rho, &rho, &drho);
+    mushy_blend(f, inv_mushy, hp->liquid_cp, hp->solid_cp, &cp, &dcp);
+    /* get_material_parameters_and_derivatives (heat_diffuse_operator.cpp:1177-1190) */
+    *rc = rho * cp;
+    *drc = dcp * rho + drho * cp;
   } else {
     *k = hp->k;
     *dk = 0.0;
   }
+}
+
+void orc_material(const orc_heat_params *hp, double T, double *out)
+{
+  material(hp, T, &out[0], &out[1], &out[2], &out[3]);
 }
 
 /* ------------------------------------------------------------------------------------------ */

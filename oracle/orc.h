@@ -44,18 +44,30 @@ typedef struct {
   double hi[3];
 } orc_mesh;
 
-enum { ORC_LAW_CONSTANT = 0, ORC_LAW_TANH = 1 };
+enum { ORC_LAW_CONSTANT = 0, ORC_LAW_TANH = 1, ORC_LAW_MUSHY = 2 };
 
 typedef struct {
-  int law;            /* ORC_LAW_CONSTANT or ORC_LAW_TANH */
+  int law;            /* ORC_LAW_CONSTANT, ORC_LAW_TANH or ORC_LAW_MUSHY */
   int t_dependent;    /* adds the dk/dT and d(rho cp)/dT Jacobian terms (reference: do_solidification) */
-  double rho_cp;      /* volumetric heat capacity (constant for both laws) */
+  double rho_cp;      /* volumetric heat capacity (constant and tanh laws) */
   double k;           /* constant law: conductivity.  tanh law: k(T) = k + amp*tanh((T-Tc)/width) */
   double tanh_amp;
   double tanh_Tc;
   double tanh_width;
   double dt;          /* time increment of the implicit Euler step */
+  /* ORC_LAW_MUSHY: the liquid_solid material with the mushy_zone transition
+   * (include/meltpooldg/core/material.templates.hpp:296-365,546-600,748-780): solid fraction
+   * f = clamp((T_liq - T) / (T_liq - T_sol), 0, 1); conductivity, density and specific heat
+   * each blend as l + (s - l)(3 f^2 - 2 f^3) (level_set_tools.hpp:72-89); rho cp = rho * cp. */
+  double liquid_k, liquid_rho, liquid_cp;
+  double solid_k, solid_rho, solid_cp;
+  double solidus, liquidus;
 } orc_heat_params;
+
+/* The material law at temperature T: out = {rho cp, k, d(rho cp)/dT, dk/dT}
+ * (HeatDiffuseMultiPhaseOperator::get_material_parameters_and_derivatives,
+ * source/heat/heat_diffuse_operator.cpp:1167-1210). */
+void orc_material(const orc_heat_params *hp, double T, double *out);
 
 enum { ORC_VEL_NONE = 0, ORC_VEL_NODAL = 1, ORC_VEL_VORTEX = 2 };
 

@@ -15,7 +15,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liborc.so")
 
-LAW_CONSTANT, LAW_TANH = 0, 1
+LAW_CONSTANT, LAW_TANH, LAW_MUSHY = 0, 1, 2
 VEL_NONE, VEL_NODAL, VEL_VORTEX = 0, 1, 2
 OP_HEAT, OP_ADVEC, OP_NORMAL, OP_CURVATURE, OP_REINIT = 0, 1, 2, 3, 4
 THICKNESS_PROPORTIONAL, THICKNESS_ABSOLUTE, THICKNESS_CELLS = 0, 1, 2
@@ -31,7 +31,17 @@ class Mesh(C.Structure):
 class HeatParams(C.Structure):
     _fields_ = [("law", C.c_int), ("t_dependent", C.c_int), ("rho_cp", C.c_double),
                 ("k", C.c_double), ("tanh_amp", C.c_double), ("tanh_Tc", C.c_double),
-                ("tanh_width", C.c_double), ("dt", C.c_double)]
+                ("tanh_width", C.c_double), ("dt", C.c_double),
+                ("liquid_k", C.c_double), ("liquid_rho", C.c_double), ("liquid_cp", C.c_double),
+                ("solid_k", C.c_double), ("solid_rho", C.c_double), ("solid_cp", C.c_double),
+                ("solidus", C.c_double), ("liquidus", C.c_double)]
+
+
+def material(hp, T):
+    """(rho cp, k, d(rho cp)/dT, dk/dT) of the oracle's law at temperature T."""
+    out = np.zeros(4)
+    lib().orc_material(C.byref(hp), C.c_double(T), _p(out))
+    return out
 
 
 class AdvecParams(C.Structure):
@@ -114,6 +124,7 @@ def lib():
                                 C.c_double, C.c_void_p, dp, dp, C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]
         L.orc_local_matrix.argtypes = [C.POINTER(Operator), C.c_long, dp]
+        L.orc_material.argtypes = [C.POINTER(HeatParams), C.c_double, dp]
         L.orc_amesh_build.argtypes = [C.POINTER(AMeshC), C.POINTER(ADataC)]
         L.orc_adata_free.argtypes = [C.POINTER(ADataC)]
         L.orc_distribute_constraints.argtypes = [C.POINTER(ADataC), dp]
@@ -458,7 +469,8 @@ def laser_volumetric_source(X, power, radius, position):
 
 
 def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
-                    dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12, amesh=None):
+                    dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12, amesh=None,
+                    boundary=None, vel=None):
     """The heat time loop of the mp-heat-transfer application, restated on the oracle's
     operator (HeatTransferApplication::run, applications/mp-heat-transfer/
     heat_transfer_application.cpp:52-120, heat-diffuse operator, volumetric laser):
@@ -473,7 +485,8 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         A d = r by Jacobi-GMRES (T-dependent law) or CG, stop when |r| and |d| are below the
         tolerances, else T += d.
     heat: HeatParams (dt is set per step); laser: dict(power, radius, start, scan_direction,
-    scan_speed).  On an adaptive mesh (amesh, mesh ignored) the fields are kept
+    scan_speed); boundary: HeatBoundary (Robin / Neumann sides); vel: nodal velocity [dim][n].
+    On an adaptive mesh (amesh, mesh ignored) the fields are kept
     constraint-consistent: the heat source, the iterate after the Dirichlet values and after
     every update are distributed (compute_volumetric_heat_source and the Newton solver's
     distribute_constraints).  Returns the temperature after every step."""
@@ -507,9 +520,12 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         hp.dt = dt
         converged = False
         for _ in range(newton[0]):
-            op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old, amesh=amesh)
+            op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old, amesh=amesh,
+                                boundary=boundary, vel=vel,
+                                vel_kind=VEL_NONE if vel is None else VEL_NODAL)
             r = op.create_rhs(T_old, q)
-            d, _, _, st = op.solve(r, solver=GMRES if hp.t_dependent else CG, rel_tol=rel_tol,
+            nonsym = hp.t_dependent or vel is not None
+            d, _, _, st = op.solve(r, solver=GMRES if nonsym else CG, rel_tol=rel_tol,
                                    inv_diag=op.inverse_diagonal())
             if st != SUCCESS:
                 raise RuntimeError("oracle linear solve failed")

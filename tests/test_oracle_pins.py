@@ -381,3 +381,141 @@ def test_face_rhs_terms():
     X = orc.node_coords(m)
     off = np.abs(X[:, 2] - hi[2]) > 1e-12
     assert np.abs(r[off]).max() < 1e-10
+
+
+# ---------------------------------------------------------------- temperature-dependent laws
+def mushy_params(**kw):
+    """Water/ice of the reference's solidification_slab case (solidification_slab.json)."""
+    p = dict(liquid_k=0.556, liquid_rho=1000.0, liquid_cp=4226.0, solid_k=2.22,
+             solid_rho=1000.0, solid_cp=1762.0, solidus=270.0, liquidus=276.0, t_dependent=1,
+             dt=1.0)
+    p.update(kw)
+    return orc.HeatParams(law=orc.LAW_MUSHY, **p)
+
+
+def test_tanh_law_closed_form():
+    """k(T) = k0 + a tanh(x), dk/dT = a (1 - tanh(x)^2) / w with x = (T - Tc)/w, evaluated with
+    numpy's tanh (the oracle uses tanh and cosh from libm: an independent form of sech^2)."""
+    hp = orc.HeatParams(law=orc.LAW_TANH, rho_cp=1.7, k=1.0, tanh_amp=0.5, tanh_Tc=1700.0,
+                        tanh_width=50.0, dt=1.0)
+    for T in np.r_[np.linspace(1000.0, 2400.0, 57), 1700.0, 1700.0 + 1e-9]:
+        rc, k, drc, dk = orc.material(hp, T)
+        t = np.tanh((T - 1700.0) / 50.0)
+        assert rc == 1.7 and drc == 0.0
+        assert abs(k - (1.0 + 0.5 * t)) < 4e-16 * abs(k)
+        assert abs(dk - 0.5 * (1.0 - t * t) / 50.0) <= 1e-15 * 0.5 / 50.0
+    # values where the closed form is exact: k(Tc) = k0, dk/dT(Tc) = a/w
+    rc, k, drc, dk = orc.material(hp, 1700.0)
+    assert k == 1.0 and dk == 0.5 / 50.0
+
+
+def test_mushy_law_closed_form():
+    """Mushy zone: f = clamp((T_l - T)/(T_l - T_s), 0, 1), each property l + (s - l)(3f^2 - 2f^3),
+    rho cp = rho cp, derivatives from the chain rule; the pure phases outside [T_s, T_l]; the
+    blend is C^1 (the derivative vanishes at both ends); derivatives against central differences
+    of the oracle's own values."""
+    hp = mushy_params(liquid_rho=900.0)  # unequal densities exercise d(rho)/dT * cp too
+    Ts, Tl = 270.0, 276.0
+    for T in (250.0, Ts, Tl, 300.0):
+        rc, k, drc, dk = orc.material(hp, T)
+        solid = T <= Ts
+        assert k == (2.22 if solid else 0.556)
+        assert rc == (1000.0 * 1762.0 if solid else 900.0 * 4226.0)
+        assert drc == 0.0 and dk == 0.0
+    for T in np.linspace(270.05, 275.95, 40):
+        f = (Tl - T) / (Tl - Ts)
+        H = 3 * f ** 2 - 2 * f ** 3
+        dHdT = -(6 * f - 6 * f ** 2) / (Tl - Ts)
+        k_ = 0.556 + (2.22 - 0.556) * H
+        rho = 900.0 + 100.0 * H
+        cp = 4226.0 + (1762.0 - 4226.0) * H
+        rc, k, drc, dk = orc.material(hp, T)
+        assert abs(k - k_) < 1e-15 * 2.22 * 4
+        assert abs(rc - rho * cp) < 1e-15 * rc * 4
+        assert abs(dk - (2.22 - 0.556) * dHdT) < 1e-13
+        assert abs(drc - (100.0 * cp + (1762.0 - 4226.0) * rho) * dHdT) < 1e-9 * abs(rc)
+        h = 1e-5
+        kp, km = orc.material(hp, T + h), orc.material(hp, T - h)
+        assert abs((kp[1] - km[1]) / (2 * h) - dk) < 1e-7 * 2.22
+        assert abs((kp[0] - km[0]) / (2 * h) - drc) < 1e-7 * rc
+    # C^1 at both ends: the blend's derivative tends to zero
+    assert abs(orc.material(hp, Ts + 1e-7)[3]) < 1e-6 and abs(orc.material(hp, Tl - 1e-7)[3]) < 1e-6
+
+
+def _fd_case(law, td, vel, kcell, seed=11, degree=2, ncell=(3, 2, 2), dim=3):
+    m = orc.make_mesh(dim, degree, ncell if dim == 3 else ncell[:dim], hi=(1.0, 0.8, 1.2)[:dim] +
+                      (1.0,) * (3 - dim))
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(seed)
+    if law == "tanh":
+        hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=td, rho_cp=1.3, k=1.0, tanh_amp=0.5,
+                            tanh_Tc=1700.0, tanh_width=50.0, dt=1e-2)
+        T = 1700.0 + 120.0 * rng.uniform(-1, 1, n)
+        T_old = T - 5.0 * rng.uniform(-1, 1, n)
+    else:
+        hp = mushy_params(t_dependent=td)
+        hp.dt = 50.0
+        T = 273.0 + 4.0 * rng.uniform(-1, 1, n)   # spans solid, mushy and liquid nodes
+        T_old = T + 2.0 * rng.uniform(-1, 1, n)
+    kw = {}
+    if vel:
+        kw.update(vel_kind=orc.VEL_NODAL, vel=rng.uniform(-1, 1, 3 * n))
+    if kcell:
+        kw.update(k_cell=np.exp(0.3 * rng.standard_normal(orc.n_cells(m))))
+    q = rng.uniform(-1, 1, n)
+    u = rng.uniform(-1, 1, n)
+    return m, hp, T, T_old, q, u, kw
+
+
+def _residual(m, hp, T, T_old, q, kw):
+    """R(T) = -create_rhs(T_old, q) at linearisation point T (no Dirichlet rows)."""
+    op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, T=T, T_old=T_old, use_mask=0, **kw)
+    return -op.create_rhs(T_old, q)
+
+
+@pytest.mark.parametrize("law,vel,kcell", [("tanh", False, False), ("tanh", True, False),
+                                           ("tanh", False, True), ("tanh", True, True),
+                                           ("mushy", False, False), ("mushy", True, True)])
+def test_tangent_is_the_residual_derivative(law, vel, kcell):
+    """The Jacobian J = dR/dT of the Newton step: J u equals the central difference
+    (R(T + eps u) - R(T - eps u)) / (2 eps) of the residual that create_rhs returns (with its
+    sign), at ~1e-8 relative.  This pins every T-dependent term of the tangent (dk/dT grad T u,
+    d(rho cp)/dT (T - T_old)/dt u, d(rho cp)/dT (grad T . w) u) against the residual's k(T) grad T,
+    rho cp(T)(T - T_old)/dt and rho cp(T) grad T . w, which the reference's own golden outputs
+    pin (heat_diffuse_operator.cpp:393-529 against :955-1086)."""
+    m, hp, T, T_old, q, u, kw = _fd_case(law, 1, vel, kcell)
+    op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, T=T, T_old=T_old, use_mask=0, **kw)
+    Ju = op.vmult(u)
+    eps = 1e-4 if law == "tanh" else 2e-5
+    fd = (_residual(m, hp, T + eps * u, T_old, q, kw) -
+          _residual(m, hp, T - eps * u, T_old, q, kw)) / (2 * eps)
+    err = np.abs(Ju - fd).max() / np.abs(Ju).max()
+    assert err < 1e-7, err
+
+
+@pytest.mark.parametrize("law", ["tanh", "mushy"])
+def test_fd_check_detects_a_dropped_tangent_term(law):
+    """Without the T-derivative terms (t_dependent = 0) the same check fails by orders of
+    magnitude: the FD comparison above is sensitive to the dk/dT and d(rho cp)/dT terms."""
+    m, hp, T, T_old, q, u, kw = _fd_case(law, 1, True, False)
+    hp0 = orc.HeatParams(*(getattr(hp, f) for f, _ in orc.HeatParams._fields_))
+    hp0.t_dependent = 0
+    op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp0, T=T, T_old=T_old, use_mask=0, **kw)
+    Ju = op.vmult(u)
+    eps = 1e-4 if law == "tanh" else 2e-5
+    fd = (_residual(m, hp, T + eps * u, T_old, q, kw) -
+          _residual(m, hp, T - eps * u, T_old, q, kw)) / (2 * eps)
+    assert np.abs(Ju - fd).max() / np.abs(Ju).max() > 1e-4
+
+
+@pytest.mark.parametrize("dim", [1, 2])
+def test_tangent_residual_derivative_low_dims(dim):
+    """The same pin in 1D and 2D (the dimensions of the reference's golden heat cases)."""
+    m, hp, T, T_old, q, u, kw = _fd_case("mushy", 1, False, False, degree=1,
+                                         ncell=(7, 5, 1), dim=dim)
+    op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, T=T, T_old=T_old, use_mask=0, **kw)
+    Ju = op.vmult(u)
+    eps = 2e-5
+    fd = (_residual(m, hp, T + eps * u, T_old, q, kw) -
+          _residual(m, hp, T - eps * u, T_old, q, kw)) / (2 * eps)
+    assert np.abs(Ju - fd).max() / np.abs(Ju).max() < 1e-7
