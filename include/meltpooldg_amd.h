@@ -129,16 +129,27 @@ int mf_adaptive_transfer(mf_scratch *old_s, const double *old_vec, mf_scratch *n
 /* heat_diffuse_operator.cpp) and AdvectionDiffusionOperator (source/level_set/                 */
 /* advection_diffusion_operator.cpp).                                                           */
 /* ------------------------------------------------------------------------------------------ */
-enum { MF_LAW_CONSTANT = 0, MF_LAW_TANH = 1 };
+enum { MF_LAW_CONSTANT = 0, MF_LAW_TANH = 1, MF_LAW_MUSHY = 2 };
 
 typedef struct {
-  int law;            /* MF_LAW_CONSTANT or MF_LAW_TANH */
-  int t_dependent;    /* include the dk/dT Jacobian term (reference do_solidification) */
-  double rho_cp;      /* volumetric heat capacity (constant) */
+  int law;            /* MF_LAW_CONSTANT, MF_LAW_TANH or MF_LAW_MUSHY */
+  int t_dependent;    /* include the dk/dT and d(rho cp)/dT Jacobian terms (reference
+                       * do_solidification) */
+  double rho_cp;      /* volumetric heat capacity (constant and tanh laws) */
   double k;           /* constant conductivity, or base value of the tanh law */
   double tanh_amp;    /* k(T) = k + tanh_amp * tanh((T - tanh_Tc) / tanh_width) */
   double tanh_Tc;
   double tanh_width;
+  /* MF_LAW_MUSHY: MaterialTypes::liquid_solid with the mushy_zone transition
+   * (include/meltpooldg/core/material.templates.hpp:296-365,546-600,748-780): solid fraction
+   * f = clamp((liquidus - T) / (liquidus - solidus), 0, 1); conductivity, density and specific
+   * heat each l + (s - l)(3 f^2 - 2 f^3) (level_set_tools.hpp:72-89), rho cp = rho * cp
+   * (get_material_parameters_and_derivatives, heat_diffuse_operator.cpp:1167-1210).  With
+   * t_dependent the tangent adds d(rho cp)/dT ((T - T_old)/dt + grad T . w) u, which reads the
+   * old temperature (mf_operator_set_temperature_old).  No latent heat (apparent capacity). */
+  double liquid_k, liquid_rho, liquid_cp;
+  double solid_k, solid_rho, solid_cp;
+  double solidus, liquidus;
 } mf_heat_params;
 
 typedef struct {
@@ -166,6 +177,13 @@ int mf_operator_reset_dof_index(mf_operator *op, int dof_index);
  * Must be called again whenever the content of T changes (the FP32 copy is refreshed here). */
 int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
                            const double *k_cell);
+
+/* The old temperature of the heat operator (HeatDiffuseMultiPhaseOperator::temperature_old,
+ * wired at heat_diffuse_operation.cpp:55-70; read plain in tangent_local_cell_operation,
+ * heat_diffuse_operator.cpp:1004-1009): needed by the mushy-zone law's d(rho cp)/dT term.
+ * Device pointer of n_dofs doubles kept by reference; call again after its content changes.
+ * mf_heat_newton_step sets it to its T_old argument. */
+int mf_operator_set_temperature_old(mf_operator *op, const double *T_old);
 
 /* Robin / Neumann boundary conditions of the heat operator on the sides of a structured box mesh:
  * side s = 2*direction + (0 lower, 1 upper), the boundary ids of a colourised

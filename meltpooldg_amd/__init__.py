@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libmeltpooldg_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "meltpooldg_amd.h")
 
 MF_OK, MF_ERR_ARG, MF_ERR_HIP, MF_NO_CONVERGENCE, MF_BREAKDOWN, MF_ERR_STATE = range(6)
-LAW_CONSTANT, LAW_TANH = 0, 1
+LAW_CONSTANT, LAW_TANH, LAW_MUSHY = 0, 1, 2
 SOLVER_CG, SOLVER_GMRES = 0, 1
 PRECOND_IDENTITY, PRECOND_JACOBI = 0, 1
 PRECISION_FP64, PRECISION_FP32_REFINED = 0, 1
@@ -47,7 +47,10 @@ class MeshDesc(C.Structure):
 class HeatParams(C.Structure):
     _fields_ = [("law", C.c_int), ("t_dependent", C.c_int), ("rho_cp", C.c_double),
                 ("k", C.c_double), ("tanh_amp", C.c_double), ("tanh_Tc", C.c_double),
-                ("tanh_width", C.c_double)]
+                ("tanh_width", C.c_double),
+                ("liquid_k", C.c_double), ("liquid_rho", C.c_double), ("liquid_cp", C.c_double),
+                ("solid_k", C.c_double), ("solid_rho", C.c_double), ("solid_cp", C.c_double),
+                ("solidus", C.c_double), ("liquidus", C.c_double)]
 
 
 class AdvecParams(C.Structure):
@@ -218,6 +221,7 @@ def lib():
         L.mf_operator_reset_dof_index.argtypes = [vp, C.c_int]
         L.mf_operator_set_fields.argtypes = [vp, vp, vp, vp]
         L.mf_operator_set_boundary.argtypes = [vp, C.POINTER(HeatBoundary)]
+        L.mf_operator_set_temperature_old.argtypes = [vp, vp]
         L.mf_normal_vector_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
                                                        C.POINTER(vp)]
         L.mf_curvature_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
@@ -588,16 +592,28 @@ class _OperatorBase:
 
 class HeatDiffuseOperator(_OperatorBase):
     """Heat Jacobian (reference HeatDiffuseMultiPhaseOperator, source/heat/
-    heat_diffuse_operator.cpp): tangent (rho cp/dt) M + K(k(T)) [+ dk/dT grad T u]."""
+    heat_diffuse_operator.cpp): tangent (rho cp/dt) M + K(k(T)) [+ dk/dT grad T u
+    + d(rho cp)/dT ((T - T_old)/dt + grad T . w) u].  law LAW_MUSHY takes `mushy`, a dict of
+    liquid_k, liquid_rho, liquid_cp, solid_k, solid_rho, solid_cp, solidus, liquidus (the
+    liquid_solid material with the mushy_zone transition)."""
 
     def __init__(self, scratch, law=LAW_CONSTANT, rho_cp=1.0, k=1.0, tanh_amp=0.0, tanh_Tc=0.0,
-                 tanh_width=1.0, t_dependent=False, dt=None):
+                 tanh_width=1.0, t_dependent=False, dt=None, mushy=None):
         p = HeatParams(law, int(bool(t_dependent)), rho_cp, k, tanh_amp, tanh_Tc, tanh_width)
+        for f, v in (mushy or {}).items():
+            if f not in dict(HeatParams._fields_[7:]):
+                raise ValueError(f"unknown mushy-zone parameter {f}")
+            setattr(p, f, float(v))
         h = C.c_void_p()
         _check(lib().mf_heat_operator_create(scratch._h, C.byref(p), C.byref(h)))
         super().__init__(scratch, h)
         if dt is not None:
             self.reset_time_increment(dt)
+
+    def set_temperature_old(self, T_old):
+        """The old temperature (read by the mushy-zone Jacobian; kept by reference)."""
+        self._T_old = T_old
+        _check(lib().mf_operator_set_temperature_old(self._h, self.scratch._v(T_old)))
 
     def set_boundary(self, convection_sides=(), radiation_sides=(), neumann_sides=None,
                      convection_coefficient=0.0, convection_temperature_infinity=0.0,

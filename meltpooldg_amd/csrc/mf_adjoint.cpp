@@ -21,7 +21,7 @@
 namespace mf {
 int vec_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t st);
 template <typename Number>
-int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+int launch_cell(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t stream);
 }  // namespace mf
 

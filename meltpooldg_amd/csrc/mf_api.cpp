@@ -157,6 +157,17 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.tanh_Tc = Number(op->heat.tanh_Tc);
     a.inv_tanh_w = Number(op->heat.tanh_width != 0 ? 1.0 / op->heat.tanh_width : 0.0);
     a.t_dependent = op->heat.t_dependent;
+    if (op->heat.law == MF_LAW_MUSHY) {
+      const mf_heat_params &h = op->heat;
+      a.mu_k[0] = Number(h.liquid_k);
+      a.mu_k[1] = Number(h.solid_k);
+      a.mu_rho[0] = Number(h.liquid_rho);
+      a.mu_rho[1] = Number(h.solid_rho);
+      a.mu_cp[0] = Number(h.liquid_cp);
+      a.mu_cp[1] = Number(h.solid_cp);
+      a.mu_Tliq = Number(h.liquidus);
+      a.mu_inv = Number(1.0 / (h.liquidus - h.solidus));
+    }
   } else if (op->kind == 1) {
     a.theta = Number(op->advec.theta);
     a.diffusivity = Number(op->advec.diffusivity);
@@ -201,20 +212,30 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.T = reinterpret_cast<const Number *>(op->T);
     a.vel = reinterpret_cast<const Number *>(op->vel);
     a.k_cell = reinterpret_cast<const Number *>(op->k_cell);
+    a.T_old = reinterpret_cast<const Number *>(op->T_old);
   } else {
     a.T = reinterpret_cast<const Number *>(op->T ? op->T32 : nullptr);
     a.vel = reinterpret_cast<const Number *>(op->vel ? op->vel32 : nullptr);
     a.k_cell = reinterpret_cast<const Number *>(op->k_cell ? op->kc32 : nullptr);
+    a.T_old = reinterpret_cast<const Number *>(op->T_old ? op->To32 : nullptr);
   }
   return a;
 }
 
-// T: the temperature of the tanh law, or the level set of a narrow band
-static bool needs_T(const mf_operator *op)
+// T: the temperature of a T-dependent law (1: tanh, 2: mushy zone), or the level set of a
+// narrow band (1)
+static int needs_T(const mf_operator *op)
 {
-  if (op->kind == 4) return false;
-  if (op->kind >= 2) return op->ls.narrow_band != 0;
-  return op->kind == 0 && op->heat.law == MF_LAW_TANH;
+  if (op->kind == 4) return 0;
+  if (op->kind >= 2) return op->ls.narrow_band != 0 ? 1 : 0;
+  if (op->kind != 0) return 0;
+  return op->heat.law == MF_LAW_TANH ? 1 : (op->heat.law == MF_LAW_MUSHY ? 2 : 0);
+}
+
+// the mushy law's tangent reads the old temperature
+static bool needs_T_old(const mf_operator *op)
+{
+  return op->kind == 0 && op->heat.law == MF_LAW_MUSHY && op->heat.t_dependent;
 }
 
 // vector blocks of the operator (BlockVectorType of the normal vector operator)
@@ -241,6 +262,9 @@ static int check_ready(const mf_operator *op)
   MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
              "time increment not set (reference AssertThrowZeroTimeIncrement)");
   if (needs_T(op)) MF_REQUIRE(op->T, MF_ERR_STATE, "temperature-dependent law needs T: call mf_operator_set_fields");
+  if (needs_T_old(op))
+    MF_REQUIRE(op->T_old, MF_ERR_STATE,
+               "the mushy-zone Jacobian needs the old temperature: call mf_operator_set_temperature_old");
   if (op->kind == 1) MF_REQUIRE(op->vel, MF_ERR_STATE, "advection operator needs a velocity field");
   return MF_OK;
 }
@@ -464,8 +488,11 @@ int mf_scratch_shape_info(const mf_scratch *s, double *S, double *DS, double *Dq
 int mf_heat_operator_create(mf_scratch *s, const mf_heat_params *p, mf_operator **out)
 {
   MF_REQUIRE(s && p && out, MF_ERR_ARG, "null argument");
-  MF_REQUIRE(p->law == MF_LAW_CONSTANT || p->law == MF_LAW_TANH, MF_ERR_ARG, "unknown law");
+  MF_REQUIRE(p->law == MF_LAW_CONSTANT || p->law == MF_LAW_TANH || p->law == MF_LAW_MUSHY,
+             MF_ERR_ARG, "unknown law");
   MF_REQUIRE(p->law != MF_LAW_TANH || p->tanh_width > 0.0, MF_ERR_ARG, "tanh_width must be > 0");
+  MF_REQUIRE(p->law != MF_LAW_MUSHY || p->liquidus > p->solidus, MF_ERR_ARG,
+             "the liquidus must lie above the solidus");
   if (int rc = ensure_adaptive_on_device(s)) return rc;
   auto *op = new mf_operator();
   op->kind = 0;
@@ -588,6 +615,7 @@ int mf_operator_destroy(mf_operator *op)
   if (op->T32) (void)hipFree(op->T32);
   if (op->vel32) (void)hipFree(op->vel32);
   if (op->kc32) (void)hipFree(op->kc32);
+  if (op->To32) (void)hipFree(op->To32);
   for (auto &ar : op->arena)
     if (ar.p) (void)hipFree(ar.p);
   delete op;
@@ -635,6 +663,19 @@ int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
     if (!op->kc32) MF_HIP_TRY(hipMalloc(&op->kc32, sizeof(float) * m.n_cells));
     int rc = vec_cast<double, float>(k_cell, op->kc32, 1.0, m.n_cells, st);
     if (rc) return rc;
+  }
+  return MF_OK;
+}
+
+int mf_operator_set_temperature_old(mf_operator *op, const double *T_old)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "only the heat operator has an old temperature");
+  op->T_old = T_old;
+  if (T_old) {
+    const MeshInfo &m = op->s->mesh;
+    if (!op->To32) MF_HIP_TRY(hipMalloc(&op->To32, sizeof(float) * m.n_dofs));
+    return vec_cast<double, float>(T_old, op->To32, 1.0, m.n_dofs, op->s->stream);
   }
   return MF_OK;
 }

@@ -278,6 +278,32 @@ __device__ __forceinline__ void tanh_sech2(Number x, Number &t, Number &s2)
   }
 }
 
+// Mushy-zone law (MaterialTypes::liquid_solid with the mushy_zone transition): solid fraction
+// f = clamp((T_liq - T) / (T_liq - T_sol), 0, 1) (compute_temperature_dependent_solid_fraction,
+// material.templates.hpp:748-760); each property l + (s - l) H(f), H = 3 f^2 - 2 f^3, exactly l
+// at f = 0 and s at f = 1, with d/dT = -(s - l)(6 f - 6 f^2) / (T_liq - T_sol) inside the
+// mushy interval and 0 at its ends (material.templates.hpp:546-600); rho cp = rho * cp and
+// d(rho cp)/dT = cp' rho + rho' cp (heat_diffuse_operator.cpp:1177-1190).
+template <typename Number>
+__device__ __forceinline__ void mushy_law(const CellArgs<Number> &A, Number T, Number &k,
+                                          Number &dk, Number &rc, Number &drc)
+{
+  Number f = (A.mu_Tliq - T) * A.mu_inv;
+  f = f > Number(1) ? Number(1) : f;
+  f = f < Number(0) ? Number(0) : f;
+  const bool liquid = f == Number(0), solid = f == Number(1);
+  const Number H = f * f * (Number(3) - Number(2) * f);
+  const Number dH = (liquid || solid) ? Number(0) : -A.mu_inv * (Number(6) * f - Number(6) * f * f);
+  auto blend = [&](const Number (&v)[2]) {
+    return liquid ? v[0] : (solid ? v[1] : v[0] + (v[1] - v[0]) * H);
+  };
+  k = blend(A.mu_k);
+  dk = (A.mu_k[1] - A.mu_k[0]) * dH;
+  const Number rho = blend(A.mu_rho), cp = blend(A.mu_cp);
+  rc = rho * cp;
+  drc = (A.mu_cp[1] - A.mu_cp[0]) * dH * rho + (A.mu_rho[1] - A.mu_rho[0]) * dH * cp;
+}
+
 template <typename Number>
 __device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
 {
@@ -672,7 +698,7 @@ struct SlotAccess {
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT, class Access>
+template <typename Number, int K, int OP, int NEEDT, bool VEL, bool DOT, class Access>
 __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     __attribute__((amdgpu_waves_per_eu(min_waves_per_eu<Number, K>())))
     cell_kernel(const CellArgs<Number> A0, const Number *__restrict__ src, Number *__restrict__ dst,
@@ -692,6 +718,10 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr bool TGRAD = USE_T && !LS;
   constexpr bool RHS_HEAT = (OP == HEAT_RHS);
   constexpr bool TANGENT = is_tangent_op(OP);
+  // mushy-zone law: rho cp varies with T; the tangent and its diagonal read the old temperature
+  // for the d(rho cp)/dT (T - T_old)/dt term
+  constexpr bool MUSHY = NEEDT == 2 && (OP == HEAT_TANGENT || OP == HEAT_DIAG || OP == HEAT_RHS);
+  constexpr bool TOLD = MUSHY && !RHS_HEAT;
 
   // LDS cell tensors [K^3][CPB].  While values are interpolated: u in L0, T in L1, the velocity
   // components and the rhs fields after it.  Then the T gradients use L2, L3 and are reduced to
@@ -700,7 +730,8 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr int I_T = 1;
   constexpr int I_W = 1 + (USE_T ? 1 : 0);
   constexpr int I_A = I_W + (VEL ? 3 : 0);
-  constexpr int N_EVAL = I_A + (RHS_HEAT ? 2 : 0);
+  constexpr int I_O = I_A + (RHS_HEAT ? 2 : 0);
+  constexpr int N_EVAL = I_O + (TOLD ? 1 : 0);
   constexpr int N_GRAD = TGRAD ? 4 : 3;
   // with the fused dot, the gathered source is kept: in registers (FP32 x pencils), or in one
   // more tensor when it was gathered cooperatively or when K more registers would cost the FP64
@@ -774,6 +805,10 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       ac.coop_store(gu, lds + I_A * BUF, nullptr);
       ac.coop_store(gt, lds + (I_A + 1) * BUF, nullptr);
     }
+    if constexpr (TOLD) {
+      ac.coop_load(A, A.T_old, false, gu);
+      ac.coop_store(gu, lds + I_O * BUF, nullptr);
+    }
     __syncthreads();
     if constexpr (HAS_U) sweep_inplace<Number, K, LY, 0, false>(S, L0, p, c);
     if constexpr (USE_T) sweep_inplace<Number, K, LY, 0, false>(S, lds + I_T * BUF, p, c);
@@ -786,6 +821,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       sweep_inplace<Number, K, LY, 0, false>(S, lds + I_A * BUF, p, c);
       sweep_inplace<Number, K, LY, 0, false>(S, lds + (I_A + 1) * BUF, p, c);
     }
+    if constexpr (TOLD) sweep_inplace<Number, K, LY, 0, false>(S, lds + I_O * BUF, p, c);
   } else {
   {
     Number o[K];
@@ -837,6 +873,13 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       matvec<Number, K, false>(S, v, o);
       store_pencil<Number, K, LY, 0>(lds + (I_A + 1) * BUF, p, c, o);
     }
+    if constexpr (TOLD) {
+      Number v[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) v[i] = A.T_old ? ac.plain(A.T_old, i) : Number(0);
+      matvec<Number, K, false>(S, v, o);
+      store_pencil<Number, K, LY, 0>(lds + I_O * BUF, p, c, o);
+    }
   }
   }
   __syncthreads();
@@ -857,10 +900,12 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     sweep_inplace<Number, K, LY, 1, false>(S, lds + I_A * BUF, p, c);
     sweep_inplace<Number, K, LY, 1, false>(S, lds + (I_A + 1) * BUF, p, c);
   }
+  if constexpr (TOLD) sweep_inplace<Number, K, LY, 1, false>(S, lds + I_O * BUF, p, c);
   __syncthreads();
 
   // ---------------- E3: z pencils (values at q and d/dz kept in registers) ----------------
   Number uq[K], guz[K], Tq[K], gTz[K], wq[3][K], a1q[K], a2q[K];
+  [[maybe_unused]] Number Toq[K];
   {
     Number vu[K], vt[K];
     if constexpr (HAS_U) load_pencil<Number, K, LY, 2>(L0, p, c, vu);
@@ -891,6 +936,11 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     load_pencil<Number, K, LY, 2>(lds + (I_A + 1) * BUF, p, c, v);
     matvec<Number, K, false>(S, v, a2q);
   }
+  if constexpr (TOLD) {
+    Number v[K];
+    load_pencil<Number, K, LY, 2>(lds + I_O * BUF, p, c, v);
+    matvec<Number, K, false>(S, v, Toq);
+  }
   __syncthreads();
 
   const Number kc = (A.k_cell && ac.active) ? A.k_cell[ac.cell] : Number(1);
@@ -898,6 +948,8 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   // ---------------- T gradients -> per-quadrature-point coefficients ----------------
   // heat tangent / diagonal: kq = k(T) kc and dkg = dk/dT kc grad T;  PGRAD: dkg = grad T
   Number kq[K], dkg[3][K];
+  // mushy law: rho cp(T) and the extra value coefficient d(rho cp)/dT ((T - T_old)/dt + grad T . w)
+  [[maybe_unused]] Number rcq[K], cxq[K];
   if constexpr (TGRAD) {
     sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, lds + I_T * BUF, L2, Dq, lds + I_T * BUF,
                                                   lds + 3 * BUF, p, c);
@@ -910,6 +962,18 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
         dkg[0][z] = g0;
         dkg[1][z] = g1;
         dkg[2][z] = g2;
+      } else if constexpr (MUSHY) {
+        Number k, dk, rc, drc;
+        mushy_law(A, Tq[z], k, dk, rc, drc);
+        kq[z] = k * kc;
+        dk *= kc;
+        dkg[0][z] = dk * g0;
+        dkg[1][z] = dk * g1;
+        dkg[2][z] = dk * g2;
+        rcq[z] = rc;
+        Number cx = drc * (Tq[z] - Toq[z]) * A.dt_inv;
+        if constexpr (VEL) cx += drc * (g0 * wq[0][z] + g1 * wq[1][z] + g2 * wq[2][z]);
+        cxq[z] = A.t_dependent ? cx : Number(0);
       } else {
         Number t, s2;
         tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
@@ -956,14 +1020,18 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       Number k = A.k0 * kc;
       if constexpr (USE_T) k = kq[z];
       const bool td = USE_T && A.t_dependent;
+      Number rho_cp = A.rho_cp;
+      if constexpr (MUSHY) rho_cp = rcq[z];
       if constexpr (OP == HEAT_TANGENT) {
         const Number u = uq[z];
-        val = A.dt_inv * A.rho_cp * u;
+        val = A.dt_inv * rho_cp * u;
+        // mushy law: d(rho cp)/dT ((T - T_old)/dt + grad T . w) u v (symmetric in u, v)
+        if constexpr (MUSHY) val += cxq[z] * u;
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[d] = k * g[d];
         if (!A.transpose) {
           // (rho cp w . grad u) v + dk/dT u (grad T . grad v)
-          if constexpr (VEL) val += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
+          if constexpr (VEL) val += rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
           if constexpr (USE_T) {
             if (td) {
 #pragma unroll
@@ -974,18 +1042,19 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
           // transposed: (rho cp w u) . grad v + dk/dT (grad T . grad u) v
           if constexpr (VEL) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) f[d] += A.rho_cp * w3[d] * u;
+            for (int d = 0; d < 3; ++d) f[d] += rho_cp * w3[d] * u;
           }
           if constexpr (USE_T) {
             if (td) val += dkg[0][z] * g[0] + dkg[1][z] * g[1] + dkg[2][z] * g[2];
           }
         }
       } else {
-        cc = A.dt_inv * A.rho_cp;
+        cc = A.dt_inv * rho_cp;
+        if constexpr (MUSHY) cc += cxq[z];
         kk = k;
         if constexpr (VEL) {
 #pragma unroll
-          for (int d = 0; d < 3; ++d) bb[d] = A.rho_cp * w3[d];
+          for (int d = 0; d < 3; ++d) bb[d] = rho_cp * w3[d];
         }
         if constexpr (USE_T) {
           if (td) {
@@ -996,15 +1065,18 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
       }
     } else if constexpr (OP == HEAT_RHS) {
       // U holds T: values uq, gradient g
-      Number k = A.k0;
-      if constexpr (NEEDT) {
+      Number k = A.k0, rho_cp = A.rho_cp;
+      if constexpr (MUSHY) {
+        Number dk, drc;
+        mushy_law(A, uq[z], k, dk, rho_cp, drc);
+      } else if constexpr (NEEDT) {
         Number t, s2;
         tanh_sech2((uq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
         k = A.k0 + A.tanh_amp * t;
       }
       k *= kc;
-      Number v = A.dt_inv * A.rho_cp * (uq[z] - a1q[z]) - a2q[z];
-      if constexpr (VEL) v += A.rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
+      Number v = A.dt_inv * rho_cp * (uq[z] - a1q[z]) - a2q[z];
+      if constexpr (VEL) v += rho_cp * (g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2]);
       val = -v;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] = -k * g[d];
@@ -1224,7 +1296,7 @@ int cell_blocks(const MeshInfo &m, bool fp32)
   return fp32 ? cell_blocks_t<float>(m) : cell_blocks_t<double>(m);
 }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT>
+template <typename Number, int K, int OP, int NEEDT, bool VEL, bool DOT>
 static int launch_colours(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
                           Number *dst, hipStream_t st)
 {
@@ -1242,7 +1314,7 @@ static int launch_colours(const MeshInfo &m, const CellArgs<Number> &a, const Nu
   return MF_OK;
 }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+template <typename Number, int K, int OP, int NEEDT, bool VEL>
 static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src, Number *dst,
                     hipStream_t st)
 {
@@ -1252,7 +1324,7 @@ static int launch_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *
   return launch_colours<Number, K, OP, NEEDT, VEL, false>(m, a, src, dst, st);
 }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL, bool DOT, class Access>
+template <typename Number, int K, int OP, int NEEDT, bool VEL, bool DOT, class Access>
 static int launch_parts(const std::vector<int64_t> &start, const std::vector<int64_t> &end,
                         const CellArgs<Number> &a, const Number *src, Number *dst,
                         hipStream_t st)
@@ -1291,7 +1363,7 @@ static void list_ranges(const AdaptiveData &ad, int mode, std::vector<int64_t> &
   }
 }
 
-template <typename Number, int K, int OP, bool NEEDT, bool VEL>
+template <typename Number, int K, int OP, int NEEDT, bool VEL>
 static int launch_list_k(const MeshInfo &m, const CellArgs<Number> &a, const Number *src,
                          Number *dst, hipStream_t st, int mode)
 {
@@ -1314,7 +1386,7 @@ static int launch_list_k(const MeshInfo &m, const CellArgs<Number> &a, const Num
 }
 
 template <typename Number, int K, int OP>
-static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+static int launch_list_op(int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                           const Number *src, Number *dst, hipStream_t st, int mode)
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
@@ -1325,9 +1397,13 @@ static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArg
   } else if constexpr (OP == REINIT_TANGENT || OP == REINIT_DIAG) {
     return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
   } else {
+    if (needT == 2) {
+      if (vel) return launch_list_k<Number, K, OP, 2, true>(m, a, src, dst, st, mode);
+      return launch_list_k<Number, K, OP, 2, false>(m, a, src, dst, st, mode);
+    }
     if (needT) {
-      if (vel) return launch_list_k<Number, K, OP, true, true>(m, a, src, dst, st, mode);
-      return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
+      if (vel) return launch_list_k<Number, K, OP, 1, true>(m, a, src, dst, st, mode);
+      return launch_list_k<Number, K, OP, 1, false>(m, a, src, dst, st, mode);
     }
     if (vel) return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
     return launch_list_k<Number, K, OP, false, false>(m, a, src, dst, st, mode);
@@ -1335,7 +1411,7 @@ static int launch_list_op(bool needT, bool vel, const MeshInfo &m, const CellArg
 }
 
 template <typename Number, int K>
-static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
+static int launch_list_deg(int op, int needT, bool vel, const MeshInfo &m,
                            const CellArgs<Number> &a, const Number *src, Number *dst,
                            hipStream_t st, int mode)
 {
@@ -1380,7 +1456,7 @@ static int launch_list_deg(int op, bool needT, bool vel, const MeshInfo &m,
 }
 
 template <typename Number>
-int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+int launch_cell_list(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                      const Number *src, Number *dst, hipStream_t st, int mode)
 {
   if (!m.ad) {
@@ -1395,13 +1471,13 @@ int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const Cell
   }
 }
 
-template int launch_cell_list<double>(int, bool, bool, const MeshInfo &, const CellArgs<double> &,
+template int launch_cell_list<double>(int, int, bool, const MeshInfo &, const CellArgs<double> &,
                                       const double *, double *, hipStream_t, int);
-template int launch_cell_list<float>(int, bool, bool, const MeshInfo &, const CellArgs<float> &,
+template int launch_cell_list<float>(int, int, bool, const MeshInfo &, const CellArgs<float> &,
                                      const float *, float *, hipStream_t, int);
 
 template <typename Number, int K, int OP>
-static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+static int launch_op(int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                      const Number *src, Number *dst, hipStream_t st)
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
@@ -1412,9 +1488,13 @@ static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Num
   } else if constexpr (OP == REINIT_TANGENT || OP == REINIT_DIAG) {
     return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
   } else {
+    if (needT == 2) {
+      if (vel) return launch_k<Number, K, OP, 2, true>(m, a, src, dst, st);
+      return launch_k<Number, K, OP, 2, false>(m, a, src, dst, st);
+    }
     if (needT) {
-      if (vel) return launch_k<Number, K, OP, true, true>(m, a, src, dst, st);
-      return launch_k<Number, K, OP, true, false>(m, a, src, dst, st);
+      if (vel) return launch_k<Number, K, OP, 1, true>(m, a, src, dst, st);
+      return launch_k<Number, K, OP, 1, false>(m, a, src, dst, st);
     }
     if (vel) return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
     return launch_k<Number, K, OP, false, false>(m, a, src, dst, st);
@@ -1422,7 +1502,7 @@ static int launch_op(bool needT, bool vel, const MeshInfo &m, const CellArgs<Num
 }
 
 template <typename Number, int K>
-static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+static int launch_deg(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                       const Number *src, Number *dst, hipStream_t st)
 {
   switch (op) {
@@ -1448,7 +1528,7 @@ static int launch_deg(int op, bool needT, bool vel, const MeshInfo &m, const Cel
 }
 
 template <typename Number>
-int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+int launch_cell(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t st)
 {
   if (m.ad) {
@@ -1463,9 +1543,9 @@ int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<
   }
 }
 
-template int launch_cell<double>(int, bool, bool, const MeshInfo &, const CellArgs<double> &,
+template int launch_cell<double>(int, int, bool, const MeshInfo &, const CellArgs<double> &,
                                  const double *, double *, hipStream_t);
-template int launch_cell<float>(int, bool, bool, const MeshInfo &, const CellArgs<float> &,
+template int launch_cell<float>(int, int, bool, const MeshInfo &, const CellArgs<float> &,
                                 const float *, float *, hipStream_t);
 
 }  // namespace mf

@@ -74,6 +74,10 @@ struct CellArgs {
   // heat
   Number rho_cp, k0, tanh_amp, tanh_Tc, inv_tanh_w;
   int t_dependent;
+  // mushy-zone law: liquid / solid conductivity, density and heat capacity, the liquidus and
+  // 1 / (liquidus - solidus)
+  Number mu_k[2], mu_rho[2], mu_cp[2], mu_Tliq, mu_inv;
+  const Number *T_old;     // old temperature (mushy law, t_dependent)
   // advection
   Number theta, diffusivity, mass_factor;
   Number dt_inv;
@@ -233,12 +237,12 @@ void free_adaptive(::mf_scratch *s);
 
 // launches (mf_cell.hip)
 template <typename Number>
-int launch_cell(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+int launch_cell(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t stream);
 // adaptive meshes: mode 0 = all cells (tangent ops), 1 = cells without hanging nodes (diagonal
 // ops), 2 = condensed diagonal of the cells with hanging nodes (op = a tangent op)
 template <typename Number>
-int launch_cell_list(int op, bool needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
+int launch_cell_list(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                      const Number *src, Number *dst, hipStream_t stream, int mode);
 
 // Stefan-Boltzmann constant in W/(m^2 K^4) (include/meltpooldg/utilities/physical_constants.hpp:6)
@@ -293,8 +297,8 @@ struct mf_operator {
   mf_advec_params advec{};
   double dt = 0.0;
   int dof_index = 0;
-  const double *T = nullptr, *vel = nullptr, *k_cell = nullptr;
-  float *T32 = nullptr, *vel32 = nullptr, *kc32 = nullptr;  // FP32 copies of the fields
+  const double *T = nullptr, *vel = nullptr, *k_cell = nullptr, *T_old = nullptr;
+  float *T32 = nullptr, *vel32 = nullptr, *kc32 = nullptr, *To32 = nullptr;  // FP32 copies
   bool fields32_valid = false;
   int transpose = 0;  // vmult applies A^T (adjoint solves)
   mf_heat_boundary bnd{};  // Robin / Neumann sides (mf_operator_set_boundary)

@@ -586,6 +586,8 @@ int mf_heat_newton_step(mf_operator *op, double *T, const double *T_old, const d
   // on an adaptive mesh the iterate is kept constraint-consistent (hanging values from their
   // masters): NewtonRaphsonSolver's distribute_constraints after every update
   if (rc == MF_OK && s->ad) rc = mf_distribute_constraints(s, T);
+  // the mushy-zone Jacobian reads the old temperature (the operator's temperature_old)
+  if (rc == MF_OK && op->heat.law == MF_LAW_MUSHY) rc = mf_operator_set_temperature_old(op, T_old);
   mf_newton_report rep{};
   if (rc == MF_OK) {
     const mf_newton_callbacks cb{heat_residual, heat_jacobian_solve, heat_norms, heat_update};

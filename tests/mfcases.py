@@ -16,7 +16,7 @@ class Case:
     lo: tuple = (0.0, 0.0, 0.0)
     hi: tuple = (1.0, 1.2, 0.9)
     kind: str = "heat"          # heat | advec
-    law: str = "const"          # const | tanh
+    law: str = "const"          # const | tanh | mushy
     t_dependent: bool = False
     vel: bool = False
     k_cell: bool = False
@@ -33,14 +33,25 @@ class Case:
         return orc.make_mesh(3, self.degree, self.ncell, self.lo, self.hi)
 
 
+# mushy-zone law of the parity cases: properties of order one so that the mass, stiffness and
+# T-derivative terms all show at the 1e-12 bar; the fields span solid, mushy and liquid nodes
+MUSHY = dict(liquid_k=0.5, liquid_rho=1.0, liquid_cp=1.5, solid_k=2.0, solid_rho=1.2,
+             solid_cp=0.8, solidus=270.0, liquidus=276.0)
+
+
 def fields(case: Case):
     m = case.mesh()
     n = orc.n_dofs(m)
     rng = np.random.default_rng(case.seed)
     out = {}
-    out["T"] = 1700.0 + 50.0 * 3.0 * orc.random_field(m, case.seed + 1) if case.law == "tanh" \
-        else 300.0 + orc.random_field(m, case.seed + 1)
-    out["T_old"] = out["T"] - 5.0 * orc.random_field(m, case.seed + 2)
+    if case.law == "tanh":
+        out["T"] = 1700.0 + 50.0 * 3.0 * orc.random_field(m, case.seed + 1)
+    elif case.law == "mushy":
+        out["T"] = 273.0 + 4.0 * orc.random_field(m, case.seed + 1)
+    else:
+        out["T"] = 300.0 + orc.random_field(m, case.seed + 1)
+    out["T_old"] = out["T"] - (2.0 if case.law == "mushy" else 5.0) * \
+        orc.random_field(m, case.seed + 2)
     out["vel"] = rng.uniform(-1, 1, size=3 * n) if (case.vel or case.kind == "advec") else None
     out["k_cell"] = np.exp(0.3 * rng.standard_normal(orc.n_cells(m))) if case.k_cell else None
     if case.mask == "none":
@@ -70,9 +81,10 @@ def oracle_boundary(spec):
 def oracle_operator(case: Case, f, use_mask=1, boundary=None):
     m = case.mesh()
     if case.kind == "heat":
-        hp = orc.HeatParams(law=orc.LAW_TANH if case.law == "tanh" else orc.LAW_CONSTANT,
-                            t_dependent=int(case.t_dependent), rho_cp=case.rho_cp, k=case.k,
-                            tanh_amp=0.5, tanh_Tc=1700.0, tanh_width=50.0, dt=case.dt)
+        law = {"const": orc.LAW_CONSTANT, "tanh": orc.LAW_TANH, "mushy": orc.LAW_MUSHY}[case.law]
+        hp = orc.HeatParams(law=law, t_dependent=int(case.t_dependent), rho_cp=case.rho_cp,
+                            k=case.k, tanh_amp=0.5, tanh_Tc=1700.0, tanh_width=50.0, dt=case.dt,
+                            **(MUSHY if case.law == "mushy" else {}))
         return orc.OracleOperator(orc.OP_HEAT, m, heat=hp,
                                   vel_kind=orc.VEL_NODAL if case.vel else orc.VEL_NONE,
                                   mask=f["mask"], T=f["T"], T_old=f["T_old"], vel=f["vel"],
@@ -92,9 +104,10 @@ def device_operator(case: Case, f, dev="cuda"):
                        stream=torch.cuda.current_stream())
     s.set_dirichlet_mask(f["mask"])
     if case.kind == "heat":
-        op = mf.HeatDiffuseOperator(s, law=mf.LAW_TANH if case.law == "tanh" else mf.LAW_CONSTANT,
-                                    rho_cp=case.rho_cp, k=case.k, tanh_amp=0.5, tanh_Tc=1700.0,
-                                    tanh_width=50.0, t_dependent=case.t_dependent, dt=case.dt)
+        law = {"const": mf.LAW_CONSTANT, "tanh": mf.LAW_TANH, "mushy": mf.LAW_MUSHY}[case.law]
+        op = mf.HeatDiffuseOperator(s, law=law, rho_cp=case.rho_cp, k=case.k, tanh_amp=0.5,
+                                    tanh_Tc=1700.0, tanh_width=50.0, t_dependent=case.t_dependent,
+                                    dt=case.dt, mushy=MUSHY if case.law == "mushy" else None)
     else:
         op = mf.AdvectionDiffusionOperator(s, theta=case.theta, diffusivity=case.diffusivity,
                                            mass_factor=case.mass_factor, dt=case.dt)
@@ -105,6 +118,8 @@ def device_operator(case: Case, f, dev="cuda"):
     vel = f["vel"] if (case.vel or case.kind == "advec") else None
     op.set_fields(T=t(f["T"]) if case.kind == "heat" else None, velocity=t(vel),
                   k_cell=t(f["k_cell"]))
+    if case.law == "mushy":
+        op.set_temperature_old(t(f["T_old"]))
     return s, op
 
 

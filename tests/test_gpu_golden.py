@@ -117,3 +117,80 @@ def test_laser_source_and_norm():
     mf.predictor_linear_extrapolation(s, out, a, b, tau=0.5)
     torch.cuda.synchronize()
     assert torch.allclose(out, 1.5 * a - 0.5 * b, rtol=0, atol=1e-15)
+
+
+def _steps(mf, s, op, T0, dt_list, newton, dirichlet=None, solver="GMRES"):
+    T = _dev(T0)
+    dv = _dev(dirichlet) if dirichlet is not None else None
+    norms = []
+    for dt in dt_list:
+        op.reset_time_increment(dt)
+        T_old = T.clone()
+        rep = mf.heat_newton_step(op, T, T_old, dirichlet_values=dv, newton=newton,
+                                  solver=solver, rel_tolerance=1e-12)
+        assert rep["status"] == 0
+        norms.append(mf.compute_norm_l2(s, T))
+    return np.array(norms)
+
+
+def test_solidification_slab_golden_bar():
+    """solidification_slab (1D Q1, 128 cells, mushy-zone ice/water, Dirichlet 253 at x = 0,
+    dt 200) as a 3D bar of 128 x 1 x 1 cells of [0,1]^3: the solution does not vary across the
+    bar (natural boundaries), so its 3D L2 norm is the 1D norm.  The Newton tolerances are the
+    1D ones scaled by the bar's cross-section factors (|R| x 1/2, |d| x 2)."""
+    import meltpooldg_amd as mf
+    from test_oracle_golden import slab_case
+    g = json.load(open(GOLDEN))["solidification_slab"]["norms"]
+    hp, T_init, T_hat, dt, newton = slab_case()
+    s = mf.ScratchData(1, (128, 1, 1), stream=torch.cuda.current_stream())
+    X = s.support_points()
+    mask = (np.abs(X[:, 0]) < 1e-12).astype(np.uint8)
+    s.set_dirichlet_mask(mask)
+    mushy = {f: getattr(hp, f) for f, _ in orc.HeatParams._fields_[8:]}
+    op = mf.HeatDiffuseOperator(s, law=mf.LAW_MUSHY, t_dependent=True, dt=dt, mushy=mushy)
+    T0 = np.full(s.n_dofs, T_init)
+    T0[mask == 1] = T_hat
+    norms = _steps(mf, s, op, T0, [dt] * 5, (newton[0], 0.5 * newton[1], 2.0 * newton[2]),
+                   dirichlet=T0)
+    assert [f"{v:.5e}" for v in norms] == g
+
+
+@pytest.mark.parametrize("name", ["unidirectional_heat_transfer_with_convection",
+                                  "unidirectional_heat_transfer_with_radiation",
+                                  "unidirectional_heat_transfer_with_radiation_and_convection",
+                                  "unidirectional_heat_transfer_with_velocity",
+                                  "unidirectional_heat_transfer_with_velocity_and_solidification"])
+def test_unidirectional_robin_velocity_mushy_golden_extruded(name):
+    """The 2D Q1 32^2 unidirectional cases with Robin sides, a velocity and the mushy-zone law,
+    as one-cell-thick 3D slabs (norm / sqrt(thickness) = the 2D norm): the device's boundary
+    faces (convection, radiation), velocity term and mushy law in the Newton loop."""
+    import meltpooldg_amd as mf
+    from test_oracle_golden import unidirectional_case
+    g = json.load(open(GOLDEN))[name]["norms"]
+    hp, b, T0f, v, dt, tol = unidirectional_case(name)
+    Lz = 0.1 / 32
+    s = mf.ScratchData(1, (32, 32, 1), (0, 0, 0), (0.1, 0.1, Lz),
+                       stream=torch.cuda.current_stream())
+    X = s.support_points()
+    n = s.n_dofs
+    if hp.law == orc.LAW_MUSHY:
+        mushy = {f: getattr(hp, f) for f, _ in orc.HeatParams._fields_[8:]}
+        op = mf.HeatDiffuseOperator(s, law=mf.LAW_MUSHY, t_dependent=True, dt=dt, mushy=mushy)
+    else:
+        op = mf.HeatDiffuseOperator(s, rho_cp=hp.rho_cp, k=hp.k, dt=dt)
+    sides = {}
+    if b.convection_sides:
+        sides.update(convection_sides=(1,), convection_coefficient=b.convection_coefficient)
+    if b.radiation_sides:
+        sides.update(radiation_sides=(1,), emissivity=b.emissivity)
+    if sides:
+        op.set_boundary(convection_temperature_infinity=300.0,
+                        radiation_temperature_infinity=300.0, **sides)
+    vel = None
+    if v:
+        vel = np.zeros(3 * n)
+        vel[:n] = -v
+        vel = _dev(vel)
+        op.set_fields(velocity=vel)
+    norms = _steps(mf, s, op, T0f(X[:, 0]), [dt] * 5, (10, tol * Lz, 1e-10)) / np.sqrt(Lz)
+    assert [f"{x:.5e}" for x in norms] == g
