@@ -32,6 +32,9 @@ namespace mf {
 #ifndef MF_CPB_F32_K4
 #define MF_CPB_F32_K4 16
 #endif
+#ifndef MF_CPB_F64_K4
+#define MF_CPB_F64_K4 16
+#endif
 #ifndef MF_PREFETCH_F32
 #define MF_PREFETCH_F32 1
 #endif
@@ -39,7 +42,7 @@ namespace mf {
 #define MF_PREFETCH_F64 0
 #endif
 #ifndef MF_WPE_F64_K4
-#define MF_WPE_F64_K4 1
+#define MF_WPE_F64_K4 4
 #endif
 #ifndef MF_WPE_F32_K4
 #define MF_WPE_F32_K4 1
@@ -50,15 +53,18 @@ template <typename Number, int K>
 constexpr int cells_per_block()
 {
   if constexpr (sizeof(Number) == 4 && K == 4) return MF_CPB_F32_K4;
+  if constexpr (sizeof(Number) == 8 && K == 4) return MF_CPB_F64_K4;
   return K == 2 ? 64 : 16;
 }
 
-// Minimum waves per SIMD the register allocation must allow.
-template <typename Number, int K>
+// Minimum waves per SIMD the register allocation must allow.  The FP64 Q3 tangent kernels
+// without a velocity field fit 4 waves (128 VGPRs) without spilling; with the velocity, and the
+// diagonal kernels, they would spill.
+template <typename Number, int K, bool VEL, int OP>
 constexpr int min_waves_per_eu()
 {
   if constexpr (sizeof(Number) == 4 && K == 4) return MF_WPE_F32_K4;
-  if constexpr (sizeof(Number) == 8 && K == 4) return MF_WPE_F64_K4;
+  if constexpr (sizeof(Number) == 8 && K == 4) return (VEL || !is_tangent_op(OP)) ? 1 : MF_WPE_F64_K4;
   return 1;
 }
 
@@ -354,6 +360,7 @@ struct LaunchPart {
 template <typename Number, int K, int CPB>
 struct GridAccess {
   static constexpr bool COOP = false;
+  static constexpr bool RAW = true;  // gathers through issue / take
   static constexpr bool PREFETCH = prefetch_pencils<Number, K>();
   static constexpr int P = K - 1;
   uint32_t rowbase, gbase;
@@ -401,14 +408,7 @@ struct GridAccess {
     active = cx < ncx;
     cell = cellrow + cx;
     gbase = rowbase + uint32_t(cx * P);
-    if constexpr (!PREFETCH) {
-      // the mask bits of the step (with prefetching they come with the raw pencil)
-      mbits = 0u;
-      if (mask) {
-#pragma unroll
-        for (int i = 0; i < K; ++i) mbits |= uint32_t(mask[(active ? gbase : 0u) + i] != 0) << i;
-      }
-    }
+    // the mask bits of the step come with the raw pencil (take)
   }
   // entry i with the constraints applied (read_dof_values: Dirichlet entries read as zero)
   __device__ Number value(const CellArgs<Number> &, const Number *f, int i) const
@@ -508,6 +508,7 @@ struct GridAccess {
 template <typename Number, int K, int CPB>
 struct ListAccess {
   static constexpr bool COOP = true;
+  static constexpr bool RAW = false;
   static constexpr bool PREFETCH = false;
   struct Raw {};
   __device__ static void map(int t, int &c, int &p)
@@ -631,6 +632,7 @@ struct ListAccess {
 template <typename Number, int K, int CPB>
 struct SlotAccess {
   static constexpr bool COOP = false;
+  static constexpr bool RAW = false;
   static constexpr bool PREFETCH = false;
   struct Raw {};
   __device__ static void map(int t, int &c, int &p)
@@ -700,7 +702,7 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 template <typename Number, int K, int OP, int NEEDT, bool VEL, bool DOT, class Access>
 __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
-    __attribute__((amdgpu_waves_per_eu(min_waves_per_eu<Number, K>())))
+    __attribute__((amdgpu_waves_per_eu(min_waves_per_eu<Number, K, VEL, OP>())))
     cell_kernel(const CellArgs<Number> A0, const Number *__restrict__ src, Number *__restrict__ dst,
                 const LaunchPart lp)
 {
@@ -831,6 +833,11 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     if constexpr (PF) {
       ac.template take<HAS_U, USE_T>(raw, cons, u_loc, t_loc);
       if (it + 1 < ac.iterations()) ac.template issue<HAS_U, USE_T>(it + 1, in, A.T, raw);
+    } else if constexpr (Access::RAW) {
+      // all loads of the step (values and mask bits) go out together, selected afterwards
+      typename Access::Raw rw;
+      ac.template issue<HAS_U, USE_T>(it, in, A.T, rw);
+      ac.template take<HAS_U, USE_T>(rw, cons, u_loc, t_loc);
     } else {
 #pragma unroll
       for (int i = 0; i < K; ++i) {
