@@ -225,6 +225,22 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
  * Advection: src = old field, q_s must be NULL. */
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
 
+/* Inflow/outflow boundary of the advection operator (AdvectionDiffusionOperator::
+ * set_inflow_outflow_bc and do_pre_vmult / do_post_vmult, advection_diffusion_operator.cpp:
+ * 529-578; the DoF set of AdvectionDiffusionOperation::create_inflow_outflow_constraints,
+ * advection_diffusion_operation.cpp:368-499).  mask_host[n_dofs]: 1 on the inflow/outflow DoFs
+ * (NULL removes them).  With the treatment on (mf_operator_set_inflow_outflow_pre_post, the
+ * default), vmult reads their src entries as 0 and returns dst = src on their rows, the inverse
+ * diagonal is 1 there, and mf_advection_step lifts their values from dirichlet_values like
+ * Dirichlet values and sets them after the solve. */
+int mf_advection_set_inflow_outflow(mf_operator *op, const uint8_t *mask_host);
+/* set_apply_inflow_outflow_constraints_pre_post_matrix_free (advection_diffusion_operator.hpp) */
+int mf_operator_set_inflow_outflow_pre_post(mf_operator *op, int enable);
+/* The inflow DoFs of a structured box mesh: boundary nodes on the sides in `sides` (bit
+ * 2*direction + upper, the colourised boundary ids) whose nodal velocity (device, [3][n_dofs])
+ * has n . u <= 0 against the side's outer normal n; mask_host[n_dofs] gets 1 there, 0 elsewhere. */
+int mf_inflow_outflow_mask(mf_scratch *s, const double *vel, unsigned sides, uint8_t *mask_host);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Level-set normal vector and curvature projections.  Replaces NormalVectorOperator           */
 /* (source/level_set/normal_vector_operator.cpp:117-366) and CurvatureOperator                  */

@@ -222,6 +222,9 @@ def lib():
         L.mf_operator_set_fields.argtypes = [vp, vp, vp, vp]
         L.mf_operator_set_boundary.argtypes = [vp, C.POINTER(HeatBoundary)]
         L.mf_operator_set_temperature_old.argtypes = [vp, vp]
+        L.mf_advection_set_inflow_outflow.argtypes = [vp, vp]
+        L.mf_operator_set_inflow_outflow_pre_post.argtypes = [vp, C.c_int]
+        L.mf_inflow_outflow_mask.argtypes = [vp, vp, C.c_uint, vp]
         L.mf_normal_vector_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
                                                        C.POINTER(vp)]
         L.mf_curvature_operator_create.argtypes = [vp, C.POINTER(LevelSetProjectionParams),
@@ -648,6 +651,34 @@ class AdvectionDiffusionOperator(_OperatorBase):
         super().__init__(scratch, h)
         if dt is not None:
             self.reset_time_increment(dt)
+
+    def set_inflow_outflow_bc(self, mask):
+        """Inflow/outflow DoFs (uint8 numpy mask of n_dofs, None removes them); the Dirichlet
+        mask of the scratch must be set before (AdvectionDiffusionOperator::
+        set_inflow_outflow_bc)."""
+        import numpy as np
+        if mask is None:
+            _check(lib().mf_advection_set_inflow_outflow(self._h, None))
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.size != self.scratch.n_dofs:
+            raise ValueError(f"mask of {m.size} entries, expected {self.scratch.n_dofs}")
+        _check(lib().mf_advection_set_inflow_outflow(self._h, m.ctypes.data_as(C.c_void_p)))
+
+    def set_apply_inflow_outflow_constraints_pre_post_matrix_free(self, enable):
+        _check(lib().mf_operator_set_inflow_outflow_pre_post(self._h, int(bool(enable))))
+
+
+def inflow_outflow_mask(scratch, velocity, sides=(0, 1, 2, 3, 4, 5)):
+    """Inflow DoFs of a box (mf_inflow_outflow_mask): boundary nodes of the given sides (0..5 =
+    2*direction + upper) whose nodal velocity points inwards or along the side (n . u <= 0);
+    returns a uint8 numpy array of n_dofs entries."""
+    import numpy as np
+    mask = np.zeros(scratch.n_dofs, dtype=np.uint8)
+    bits = sum(1 << int(x) for x in sides)
+    _check(lib().mf_inflow_outflow_mask(scratch._h, scratch._v(velocity, 3 * scratch.n_dofs),
+                                        bits, mask.ctypes.data_as(C.c_void_p)))
+    return mask
 
 
 class _LevelSetProjection(_OperatorBase):

@@ -238,6 +238,13 @@ static bool needs_T_old(const mf_operator *op)
   return op->kind == 0 && op->heat.law == MF_LAW_MUSHY && op->heat.t_dependent;
 }
 
+// the inflow/outflow treatment applies to this application (advection operator, constrained
+// DoF index, pre/post on, a non-empty set)
+static bool io_active(const mf_operator *op)
+{
+  return op->kind == 1 && op->io_mask && op->io_pre_post && op->dof_index == 0;
+}
+
 // vector blocks of the operator (BlockVectorType of the normal vector operator)
 static int n_blocks(const mf_operator *op) { return op->kind == 2 ? 3 : 1; }
 static int64_t n_rows(const mf_operator *op) { return n_blocks(op) * op->s->mesh.n_dofs; }
@@ -310,7 +317,7 @@ static int op_tangent_blocks(const mf_operator *op, bool fp32)
   const MeshInfo &m = op->s->mesh;
   // the level-set projections use the pencil kernel like the transposed operator
   const int cells = tangent_blocks(m, op->transpose != 0 || op->kind >= 2, fp32);
-  return n_blocks(op) * cells + heat_face_blocks(op);
+  return n_blocks(op) * cells + heat_face_blocks(op) + (io_active(op) ? IO_BLOCKS : 0);
 }
 
 // dst = A src (overwrite = true, the reference's vmult with zeroed dst) or dst += A src.
@@ -340,6 +347,10 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
     if (rc) return rc;
   }
   CellArgs<Number> a = make_args<Number>(op);
+  // inflow/outflow rows (do_pre_vmult): their src entries read as 0, their rows left to the post
+  // pass, like Dirichlet rows
+  const bool io = io_active(op);
+  if (io) a.mask = op->io_union;
   a.dot_partial = dot_partial;
   a.stop = stop;
   a.overwrite = (overwrite && !m.ad) ? 1 : 0;
@@ -366,6 +377,15 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   // the transposed operator (adjoint) and the level-set projections run on the pencil kernel
   int rc = launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
   if (rc) return rc;
+  if (io) {
+    // do_post_vmult: dst = src on the inflow/outflow rows; their part of the fused dot follows
+    // the cell kernel's partials
+    return launch_io_rows<Number>(op->io_mask, src, dst, m.n_dofs,
+                                  dot_partial ? dot_partial + tangent_blocks(m, false,
+                                                                             sizeof(Number) == 4)
+                                              : nullptr,
+                                  stop, st);
+  }
   // Robin boundary faces (symmetric: the same for the transposed operator); their part of the
   // fused dot follows the cell kernel's partials
   return launch_heat_faces<Number>(
@@ -616,6 +636,8 @@ int mf_operator_destroy(mf_operator *op)
   if (op->vel32) (void)hipFree(op->vel32);
   if (op->kc32) (void)hipFree(op->kc32);
   if (op->To32) (void)hipFree(op->To32);
+  if (op->io_mask) (void)hipFree(op->io_mask);
+  if (op->io_union) (void)hipFree(op->io_union);
   for (auto &ar : op->arena)
     if (ar.p) (void)hipFree(ar.p);
   delete op;
@@ -665,6 +687,55 @@ int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
     if (rc) return rc;
   }
   return MF_OK;
+}
+
+int mf_advection_set_inflow_outflow(mf_operator *op, const uint8_t *mask_host)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(op->kind == 1, MF_ERR_ARG, "inflow/outflow boundaries belong to the advection operator");
+  MF_REQUIRE(!op->s->ad, MF_ERR_ARG, "inflow/outflow boundaries need a structured mesh");
+  const int64_t n = op->s->mesh.n_dofs;
+  if (!mask_host) {
+    if (op->io_mask) (void)hipFree(op->io_mask);
+    if (op->io_union) (void)hipFree(op->io_union);
+    op->io_mask = op->io_union = nullptr;
+    return MF_OK;
+  }
+  std::vector<uint8_t> uni(mask_host, mask_host + n);
+  bool any = false;
+  for (int64_t i = 0; i < n; ++i) {
+    any = any || mask_host[i];
+    if (!op->s->h_mask.empty() && op->s->h_mask[i]) uni[i] = 1;
+  }
+  if (!op->io_mask) MF_HIP_TRY(hipMalloc(&op->io_mask, size_t(n)));
+  if (!op->io_union) MF_HIP_TRY(hipMalloc(&op->io_union, size_t(n)));
+  MF_HIP_TRY(hipMemcpy(op->io_mask, mask_host, size_t(n), hipMemcpyHostToDevice));
+  MF_HIP_TRY(hipMemcpy(op->io_union, uni.data(), size_t(n), hipMemcpyHostToDevice));
+  if (!any) return mf_advection_set_inflow_outflow(op, nullptr);
+  return MF_OK;
+}
+
+int mf_operator_set_inflow_outflow_pre_post(mf_operator *op, int enable)
+{
+  MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  op->io_pre_post = enable ? 1 : 0;
+  return MF_OK;
+}
+
+int mf_inflow_outflow_mask(mf_scratch *s, const double *vel, unsigned sides, uint8_t *mask_host)
+{
+  MF_REQUIRE(s && vel && mask_host, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(!s->ad, MF_ERR_ARG, "inflow/outflow boundaries need a structured mesh");
+  const int64_t n = s->mesh.n_dofs;
+  uint8_t *d = nullptr;
+  MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&d), size_t(n), s->stream));
+  int rc = launch_inflow_mask(s, vel, sides, d);
+  if (rc == MF_OK && hipMemcpyAsync(mask_host, d, size_t(n), hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+    rc = MF_ERR_HIP;
+  if (rc == MF_OK && hipStreamSynchronize(s->stream) != hipSuccess) rc = MF_ERR_HIP;
+  (void)hipFreeAsync(d, s->stream);
+  if (rc == MF_ERR_HIP) set_error("mf_inflow_outflow_mask: copy to the host failed");
+  return rc;
 }
 
 int mf_operator_set_temperature_old(mf_operator *op, const double *T_old)
@@ -746,7 +817,10 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
     if ((rc = arena_reserve(op->arena[3], sizeof(double) * RED_BLOCKS))) return rc;
     work = reinterpret_cast<double *>(op->arena[3].p);
   }
-  return vec_invert_diagonal(inv_diag, n_rows(op), relative, work, st);
+  rc = vec_invert_diagonal(inv_diag, n_rows(op), relative, work, st);
+  if (rc || !io_active(op)) return rc;
+  // compute_inverse_diagonal_from_matrixfree: 1 on the inflow/outflow rows
+  return vec_masked_fill(inv_diag, op->io_mask, 1.0, m.n_dofs, st);
 }
 
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)

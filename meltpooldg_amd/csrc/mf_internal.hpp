@@ -264,6 +264,16 @@ int cell_blocks(const MeshInfo &m, bool fp32);
 // reinitialization (src = level set, normal vector = op->vel, dst [n_dofs]); dst += ...
 int launch_levelset_rhs(const mf_operator *op, int mode, const double *src, double *dst);
 
+// advection inflow/outflow rows after a tangent application (mf_ops.hip): dst[i] = src[i] on the
+// rows of io_mask (do_post_vmult); with dot_partial, IO_BLOCKS partials of src . src over them
+constexpr int IO_BLOCKS = 64;
+template <typename Number>
+int launch_io_rows(const uint8_t *io_mask, const Number *src, Number *dst, int64_t n,
+                   double *dot_partial, const int *stop, hipStream_t st);
+// mask[i] = 1 on boundary nodes of the given sides (bit 2*direction + upper) where the outer
+// normal times the nodal velocity is <= 0 (mf_ops.hip, structured meshes)
+int launch_inflow_mask(const ::mf_scratch *s, const double *vel, unsigned sides, uint8_t *mask);
+
 // blocks of one tangent application (size of the fused-dot partials)
 inline int tangent_blocks(const MeshInfo &m, bool, bool fp32) { return cell_blocks(m, fp32); }
 
@@ -304,6 +314,10 @@ struct mf_operator {
   mf_heat_boundary bnd{};  // Robin / Neumann sides (mf_operator_set_boundary)
   bool bnd_set = false;
   int gather_plain = 0;  // vmult reads src without constraints (source mass products)
+  // advection: inflow/outflow DoFs (pre/post-vmult identity rows) and, for the cell kernels,
+  // their union with the Dirichlet rows; io_pre_post switches the treatment in vmult
+  uint8_t *io_mask = nullptr, *io_union = nullptr;
+  int io_pre_post = 1;
 };
 
 namespace mf {

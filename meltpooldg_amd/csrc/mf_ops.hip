@@ -57,6 +57,95 @@ int vec_masked_assign(double *dst, const uint8_t *mask, const double *vals, bool
   return MF_OK;
 }
 
+__global__ void masked_fill_kernel(double *dst, const uint8_t *mask, double value, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    if (mask[i]) dst[i] = value;
+}
+
+int vec_masked_fill(double *dst, const uint8_t *mask, double value, int64_t n, hipStream_t st)
+{
+  hipLaunchKernelGGL(masked_fill_kernel, vgrid(n), dim3(VEC_THREADS), 0, st, dst, mask, value, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+// do_post_vmult (advection_diffusion_operator.cpp:561-578): dst = src on the inflow/outflow
+// rows; with `part`, each of the IO_BLOCKS blocks writes its part of sum src_i^2 over them (the
+// identity rows' share of a fused src . (A src))
+template <typename Number>
+__global__ void io_rows_kernel(const uint8_t *__restrict__ io, const Number *__restrict__ src,
+                               Number *__restrict__ dst, int64_t n, double *part, const int *stop)
+{
+  if (stop && *stop) return;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (io[i]) {
+      const Number v = src[i];
+      dst[i] = v;
+      acc += double(v) * double(v);
+    }
+  }
+  if (!part) return;
+  __shared__ double red[VEC_THREADS / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < VEC_THREADS / 64; ++w) s += red[w];
+    part[blockIdx.x] = s;
+  }
+}
+
+template <typename Number>
+int launch_io_rows(const uint8_t *io_mask, const Number *src, Number *dst, int64_t n,
+                   double *dot_partial, const int *stop, hipStream_t st)
+{
+  hipLaunchKernelGGL(io_rows_kernel<Number>, dim3(IO_BLOCKS), dim3(VEC_THREADS), 0, st, io_mask,
+                     src, dst, n, dot_partial, stop);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+template int launch_io_rows<double>(const uint8_t *, const double *, double *, int64_t, double *,
+                                    const int *, hipStream_t);
+template int launch_io_rows<float>(const uint8_t *, const float *, float *, int64_t, double *,
+                                   const int *, hipStream_t);
+
+// the inflow DoFs of create_inflow_outflow_constraints (advection_diffusion_operation.cpp:
+// 368-499) on a box: a node on side s (direction d = s/2, upper = s%2) with the outer normal
+// n = (2 upper - 1) e_d is an inflow DoF when n . u <= 0 for its nodal velocity u
+__global__ void inflow_mask_kernel(const double *__restrict__ vel, unsigned sides, int64_t NX,
+                                   int64_t NY, int64_t NZ, uint8_t *__restrict__ mask)
+{
+  const int64_t n = NX * NY * NZ;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t c[3] = {i % NX, (i / NX) % NY, i / (NX * NY)};
+    const int64_t NN[3] = {NX, NY, NZ};
+    uint8_t in = 0;
+    for (int d = 0; d < 3; ++d) {
+      const double u = vel[int64_t(d) * n + i];
+      if (((sides >> (2 * d)) & 1u) && c[d] == 0 && -u <= 0.0) in = 1;
+      if (((sides >> (2 * d + 1)) & 1u) && c[d] == NN[d] - 1 && u <= 0.0) in = 1;
+    }
+    mask[i] = in;
+  }
+}
+
+int launch_inflow_mask(const ::mf_scratch *s, const double *vel, unsigned sides, uint8_t *mask)
+{
+  const MeshInfo &m = s->mesh;
+  hipLaunchKernelGGL(inflow_mask_kernel, vgrid(m.n_dofs), dim3(VEC_THREADS), 0, s->stream, vel,
+                     sides, m.NN[0], m.NN[1], m.NN[2], mask);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // laser heat source: nodal interpolation of P/(r sqrt(pi/2))^3 exp(-2 (d/r)^2)
 // (GaussVolumetricIntensityProfile, include/meltpooldg/heat/laser_intensity_profiles.hpp:175-218;
@@ -476,19 +565,25 @@ int mf_advection_step(mf_operator *op, double *phi, const double *phi_old,
   MF_HIP_TRY(hipMallocAsync(reinterpret_cast<void **>(&xd), sizeof(double) * n, st));
   int rc = MF_OK;
   const int saved = op->dof_index;
+  // constrained entries: Dirichlet rows and the inflow/outflow DoFs (their values lifted as the
+  // additional inhomogeneous constraints of create_rhs_and_apply_dirichlet_matrixfree)
+  const bool io = op->io_mask && op->io_pre_post;
+  const uint8_t *cmask = io ? op->io_union : s->d_mask;
+  MF_REQUIRE(!io || dirichlet_values, MF_ERR_ARG,
+             "inflow/outflow DoFs need their values (dirichlet_values)");
   do {
-    // x_d: Dirichlet values on constrained entries, zero elsewhere
-    if ((rc = vec_masked_assign(xd, s->d_mask, dirichlet_values, false, n, st))) break;
+    // x_d: the values on constrained entries, zero elsewhere
+    if ((rc = vec_masked_assign(xd, cmask, dirichlet_values, false, n, st))) break;
     op->dof_index = 1;
     if ((rc = mf_operator_vmult(op, rhs, xd))) break;
     if ((rc = vec_axpby(0.0, rhs, -1.0, rhs, n, st))) break;
     if ((rc = mf_operator_create_rhs(op, rhs, phi_old, nullptr))) break;
     op->dof_index = 0;
-    if ((rc = vec_masked_assign(rhs, s->d_mask, nullptr, true, n, st))) break;
+    if ((rc = vec_masked_assign(rhs, cmask, nullptr, true, n, st))) break;
     rc = mf_solve(op, desc, phi, rhs, report);
     if (rc != MF_OK && rc != MF_NO_CONVERGENCE && rc != MF_BREAKDOWN) break;
     const int solve_rc = rc;
-    if ((rc = vec_masked_assign(phi, s->d_mask, dirichlet_values, true, n, st))) break;
+    if ((rc = vec_masked_assign(phi, cmask, dirichlet_values, true, n, st))) break;
     // adaptive mesh: the hanging values from their masters (constraints.distribute)
     if (s->ad && (rc = mf_distribute_constraints(s, phi))) break;
     rc = solve_rc;

@@ -27,6 +27,11 @@ int read_to_host_begin(const void *src, size_t bytes, hipStream_t st);
 int read_to_host_end(void *dst, size_t bytes);
 
 template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const int *stop = nullptr);
+// dst[i] = mask[i] ? (vals ? vals[i] : 0) : (keep_free ? dst[i] : 0)   (mf_ops.hip)
+int vec_masked_assign(double *dst, const uint8_t *mask, const double *vals, bool keep_free,
+                      int64_t n, hipStream_t st);
+// dst[i] = value where mask[i]   (mf_ops.hip)
+int vec_masked_fill(double *dst, const uint8_t *mask, double value, int64_t n, hipStream_t st);
 template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
 // x += scale * e, the scale read from scale_dev when that is set
 int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev, int64_t n,

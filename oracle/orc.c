@@ -101,6 +101,27 @@ void orc_lagrange(int n, const double *nodes, double x, double *val, double *der
   }
 }
 
+/* second derivatives of the Lagrange polynomials on `nodes` at x, from the product formula
+ * l_j'' = sum_{m != j} sum_{k != j, m} 1/((x_j - x_m)(x_j - x_k)) prod_{l != j, m, k}
+ * (x - x_l)/(x_j - x_l) */
+static void lagrange_d2(int n, const double *nodes, double x, double *d2)
+{
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0;
+    for (int m = 0; m < n; ++m) {
+      if (m == j) continue;
+      for (int k = 0; k < n; ++k) {
+        if (k == j || k == m) continue;
+        double t = 1.0 / ((nodes[j] - nodes[m]) * (nodes[j] - nodes[k]));
+        for (int l = 0; l < n; ++l)
+          if (l != j && l != m && l != k) t *= (x - nodes[l]) / (nodes[j] - nodes[l]);
+        s += t;
+      }
+    }
+    d2[j] = s;
+  }
+}
+
 /* S[q][j] = l_j(x_q), DS[q][j] = l_j'(x_q), Dq[q][r] = L_r'(x_q) with L the Lagrange basis on the
  * Gauss points (collocation derivative), w = Gauss weights.  K = degree + 1. */
 void orc_shape_matrices(int degree, double *S, double *DS, double *Dq, double *w)
@@ -131,7 +152,11 @@ typedef struct {
   long NN[3], stride[3], ncell[3];
   double h[3], lo[3];
   double S[KMAX * KMAX], DS[KMAX * KMAX], wq[KMAX], xq[KMAX], gl[KMAX];
+  double DDS[KMAX * KMAX];  /* DDS[q][j] = l_j''(x_q) */
 } geo_t;
+
+static void tensor_apply(const geo_t *g, const double *const *mats, int transpose,
+                         const double *in, double *out, double *t1, double *t2);
 
 static void geo_init(const orc_mesh *m, geo_t *g)
 {
@@ -160,6 +185,50 @@ static void geo_init(const orc_mesh *m, geo_t *g)
   double w[KMAX];
   orc_gauss(g->K, g->xq, w);
   orc_gauss_lobatto(g->K, g->gl);
+  for (int q = 0; q < g->K; ++q) lagrange_d2(g->K, g->gl, g->xq[q], g->DDS + q * g->K);
+}
+
+/* cell size of the stabilisation parameter: diameter / sqrt(dim) (ScratchData::build,
+ * source/core/scratch_data.cpp:256-278) */
+static double cell_size(const geo_t *g)
+{
+  double d2 = 0.0;
+  for (int d = 0; d < g->dim; ++d) d2 += g->h[d] * g->h[d];
+  return sqrt(d2 / g->dim);
+}
+
+/* SUPG parameter of a cell from the velocity at its quadrature points w[e][q]
+ * (compute_stabilization_parameter, advection_diffusion_operator.cpp:497-530) */
+static double supg_tau(const orc_operator *op, const geo_t *g, const double (*w)[NLOC_MAX])
+{
+  const orc_advec_params *ap = &op->advec;
+  double t;
+  if (ap->supg_coefficient < 0.0) {
+    double mx = 0.0;
+    for (int q = 0; q < g->nloc; ++q) {
+      double n2 = 0.0;
+      for (int e = 0; e < g->dim; ++e) n2 += w[e][q] * w[e][q];
+      mx = fmax(mx, sqrt(n2));
+    }
+    t = mx >= 1e-6 ? 1.0 / mx : 0.0;
+  } else {
+    t = ap->supg_coefficient;
+  }
+  return t * cell_size(g);
+}
+
+/* Laplacian at the quadrature points of the cell function with nodal values u */
+static void eval_laplacian(const geo_t *g, const double *u, double *lap, double *tmp,
+                           double *t1, double *t2)
+{
+  for (int q = 0; q < g->nloc; ++q) lap[q] = 0.0;
+  for (int e = 0; e < g->dim; ++e) {
+    const double *mats[3];
+    for (int d = 0; d < 3; ++d) mats[d] = (d == e) ? g->DDS : g->S;
+    tensor_apply(g, mats, 0, u, tmp, t1, t2);
+    const double s = 1.0 / (g->h[e] * g->h[e]);
+    for (int q = 0; q < g->nloc; ++q) lap[q] += s * tmp[q];
+  }
 }
 
 long orc_n_dofs(const orc_mesh *m)
@@ -436,7 +505,7 @@ typedef struct {
   double Tv[NLOC_MAX], Tg[3 * NLOC_MAX], To[NLOC_MAX];
   double wv[3][NLOC_MAX];
   double fval[NLOC_MAX], flux[3 * NLOC_MAX];
-  double tmp[NLOC_MAX];
+  double tmp[NLOC_MAX], lap[NLOC_MAX];
   long idx[NLOC_MAX];
   work_t w;
 } cell_ws_t;
@@ -592,7 +661,10 @@ static void heat_tangent_cell(const orc_operator *op, const geo_t *g, long cell,
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
 }
 
-/* advection-diffusion tangent (advection_diffusion_operator.cpp:448-495, no SUPG) */
+/* advection-diffusion tangent (tangent_local_cell_operation, advection_diffusion_operator.cpp:
+ * 448-495): values m u/dt + theta w . grad u, gradients theta D grad u, with SUPG
+ * + tau w (m u/dt + theta w . grad u - theta D lap u) (the value is submitted before the SUPG
+ * residual takes the Laplacian) */
 static void advec_tangent_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws_t *ws)
 {
   const orc_advec_params *ap = &op->advec;
@@ -600,13 +672,23 @@ static void advec_tangent_cell(const orc_operator *op, const geo_t *g, long cell
   eval_vg(g, ws->u, ws->val, ws->grad, &ws->w);
   eval_velocity(op, g, cell, ws);
   const double m = ap->mass_factor / ap->dt;
+  double tau = 0.0;
+  if (ap->supg) {
+    tau = supg_tau(op, g, ws->wv);
+    eval_laplacian(g, ws->u, ws->lap, ws->tmp, ws->w.t1, ws->w.t2);
+  }
   for (int q = 0; q < nq; ++q) {
     double vg = 0.0;
     for (int e = 0; e < dim; ++e) vg += ws->wv[e][q] * ws->grad[e * nq + q];
     const double jxw = cell_volume_weight(g, q);
-    ws->fval[q] = jxw * (m * ws->val[q] + ap->theta * vg);
-    for (int e = 0; e < dim; ++e)
-      ws->flux[e * nq + q] = jxw * ap->theta * ap->diffusivity * ws->grad[e * nq + q];
+    double res = m * ws->val[q] + ap->theta * vg;
+    ws->fval[q] = jxw * res;
+    if (ap->supg) res -= ap->theta * ap->diffusivity * ws->lap[q];
+    for (int e = 0; e < dim; ++e) {
+      double f = ap->theta * ap->diffusivity * ws->grad[e * nq + q];
+      if (ap->supg) f += tau * ws->wv[e][q] * res;
+      ws->flux[e * nq + q] = jxw * f;
+    }
   }
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
 }
@@ -856,6 +938,19 @@ void orc_vmult(const orc_operator *op, const double *src, double *dst)
 {
   orc_operator sc;
   long nb;
+  if (op->io_mask && op->use_mask) {
+    /* do_pre_vmult: inflow/outflow entries of src read as 0; do_post_vmult: dst = src there */
+    const long n = op_ndofs(op);
+    double *s0 = malloc(sizeof(double) * n);
+    for (long i = 0; i < n; ++i) s0[i] = op->io_mask[i] ? 0.0 : src[i];
+    orc_operator plain = *op;
+    plain.io_mask = NULL;
+    orc_vmult(&plain, s0, dst);
+    for (long i = 0; i < n; ++i)
+      if (op->io_mask[i]) dst[i] = src[i];
+    free(s0);
+    return;
+  }
   if (for_blocks(op, &sc, &nb)) {
     for (int d = 0; d < op_dim(op); ++d) orc_vmult(&sc, src + d * nb, dst + d * nb);
     return;
@@ -899,11 +994,29 @@ static void basis_at(const geo_t *g, int l, int q, double *phi, double *gphi)
   gphi[2] = v[0] * v[1] * d[2];
 }
 
+/* Laplacian of phi_l at q by direct products of the 1D second derivatives */
+static double basis_lap(const geo_t *g, int l, int q)
+{
+  const int K = g->K;
+  int a[3] = {l % K, (l / K) % K, l / (K * K)};
+  int b[3] = {q % K, (q / K) % K, q / (K * K)};
+  double lap = 0.0;
+  for (int e = 0; e < g->dim; ++e) {
+    double t = g->DDS[b[e] * K + a[e]] / (g->h[e] * g->h[e]);
+    for (int d = 0; d < g->dim; ++d)
+      if (d != e) t *= g->S[b[d] * K + a[d]];
+    lap += t;
+  }
+  return lap;
+}
+
 /* coefficients of the bilinear form at q:
  *   a(u,v) = sum_q JxW [ c u v + (bu . grad u) v + kq grad u . grad v + u (bg . grad v) ] */
 typedef struct {
   double c, k, bu[3], bg[3];
   double dn, nv[3]; /* + dn (nv . grad u)(nv . grad v) */
+  double sw[3];     /* + (sw . grad u)(sw . grad v)   (SUPG: sw = sqrt(tau theta) w) */
+  double lap[3];    /* + lap(u) (lap . grad v)        (SUPG: -tau theta D w) */
 } qcoef_t;
 
 static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, const long *idx,
@@ -911,19 +1024,28 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
 {
   const int nq = g->nloc, dim = g->dim;
   const long n = orc_n_dofs(&op->mesh);
+  /* velocity at every quadrature point first: the SUPG parameter is a maximum over the cell */
+  double wq[3][NLOC_MAX];
   for (int q = 0; q < nq; ++q) {
     double phi[NLOC_MAX], gphi[NLOC_MAX][3];
     for (int l = 0; l < nq; ++l) basis_at(g, l, q, &phi[l], gphi[l]);
-    double w[3] = {0, 0, 0};
+    for (int e = 0; e < 3; ++e) wq[e][q] = 0.0;
     if (op->vel_kind == ORC_VEL_NODAL && op->vel) {
       for (int e = 0; e < dim; ++e)
-        for (int l = 0; l < nq; ++l) w[e] += op->vel[e * n + idx[l]] * phi[l];
+        for (int l = 0; l < nq; ++l) wq[e][q] += op->vel[e * n + idx[l]] * phi[l];
     } else if (op->vel_kind == ORC_VEL_VORTEX) {
       double x[3];
       q_point(g, cell, q, x);
-      w[0] = op->advec.vel_scale * x[dim - 1];
-      w[dim - 1] = -op->advec.vel_scale * x[0];
+      wq[0][q] = op->advec.vel_scale * x[dim - 1];
+      wq[dim - 1][q] = -op->advec.vel_scale * x[0];
     }
+  }
+  const double tau =
+      (op->kind == ORC_OP_ADVEC && op->advec.supg) ? supg_tau(op, g, (const double(*)[NLOC_MAX])wq) : 0.0;
+  for (int q = 0; q < nq; ++q) {
+    double phi[NLOC_MAX], gphi[NLOC_MAX][3];
+    for (int l = 0; l < nq; ++l) basis_at(g, l, q, &phi[l], gphi[l]);
+    const double w[3] = {wq[0][q], wq[1][q], wq[2][q]};
     qcoef_t *c = &qc[q];
     memset(c, 0, sizeof(*c));
     if (op->kind == ORC_OP_HEAT) {
@@ -957,6 +1079,15 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
       c->c = ap->mass_factor / ap->dt;
       c->k = ap->theta * ap->diffusivity;
       for (int e = 0; e < 3; ++e) c->bu[e] = ap->theta * w[e];
+      if (ap->supg) {
+        /* tau (w . grad v)(m u/dt + theta w . grad u - theta D lap u) */
+        const double st = sqrt(tau * ap->theta);
+        for (int e = 0; e < 3; ++e) {
+          c->bg[e] = tau * ap->mass_factor / ap->dt * w[e];
+          c->sw[e] = st * w[e];
+          c->lap[e] = -tau * ap->theta * ap->diffusivity * w[e];
+        }
+      }
     } else if (op->kind == ORC_OP_REINIT) {
       double nv[3] = {0, 0, 0};
       for (int e = 0; e < dim; ++e)
@@ -983,6 +1114,8 @@ static void q_coefficients(const orc_operator *op, const geo_t *g, long cell, co
     for (int e = 0; e < 3; ++e) {
       c->bu[e] *= jxw;
       c->bg[e] *= jxw;
+      c->sw[e] *= sqrt(jxw);
+      c->lap[e] *= jxw;
     }
   }
 }
@@ -998,8 +1131,11 @@ void orc_local_matrix(const orc_operator *op, long cell, double *A)
   q_coefficients(op, &g, cell, idx, qc);
   memset(A, 0, sizeof(double) * nq * nq);
   for (int q = 0; q < nq; ++q) {
-    double phi[NLOC_MAX], gphi[NLOC_MAX][3];
-    for (int l = 0; l < nq; ++l) basis_at(&g, l, q, &phi[l], gphi[l]);
+    double phi[NLOC_MAX], gphi[NLOC_MAX][3], lapj[NLOC_MAX];
+    for (int l = 0; l < nq; ++l) {
+      basis_at(&g, l, q, &phi[l], gphi[l]);
+      lapj[l] = basis_lap(&g, l, q);
+    }
     const qcoef_t *c = &qc[q];
     for (int i = 0; i < nq; ++i)
       for (int j = 0; j < nq; ++j) {
@@ -1008,8 +1144,11 @@ void orc_local_matrix(const orc_operator *op, long cell, double *A)
         const double bg = c->bg[0] * gphi[i][0] + c->bg[1] * gphi[i][1] + c->bg[2] * gphi[i][2];
         const double ni = c->nv[0] * gphi[i][0] + c->nv[1] * gphi[i][1] + c->nv[2] * gphi[i][2];
         const double nj = c->nv[0] * gphi[j][0] + c->nv[1] * gphi[j][1] + c->nv[2] * gphi[j][2];
-        A[i * nq + j] +=
-            c->c * phi[j] * phi[i] + bu * phi[i] + c->k * gg + phi[j] * bg + c->dn * ni * nj;
+        const double si = c->sw[0] * gphi[i][0] + c->sw[1] * gphi[i][1] + c->sw[2] * gphi[i][2];
+        const double sj = c->sw[0] * gphi[j][0] + c->sw[1] * gphi[j][1] + c->sw[2] * gphi[j][2];
+        const double li = c->lap[0] * gphi[i][0] + c->lap[1] * gphi[i][1] + c->lap[2] * gphi[i][2];
+        A[i * nq + j] += c->c * phi[j] * phi[i] + bu * phi[i] + c->k * gg + phi[j] * bg +
+                         c->dn * ni * nj + si * sj + lapj[j] * li;
       }
   }
   free(qc);
@@ -1068,7 +1207,10 @@ static void diag_cell(const orc_operator *op, const geo_t *g, long cell, cell_ws
       const double bu = c->bu[0] * gp[0] + c->bu[1] * gp[1] + c->bu[2] * gp[2];
       const double bg = c->bg[0] * gp[0] + c->bg[1] * gp[1] + c->bg[2] * gp[2];
       const double ni = c->nv[0] * gp[0] + c->nv[1] * gp[1] + c->nv[2] * gp[2];
-      ws->out[i] += c->c * phi * phi + bu * phi + c->k * gg + phi * bg + c->dn * ni * ni;
+      const double si = c->sw[0] * gp[0] + c->sw[1] * gp[1] + c->sw[2] * gp[2];
+      const double li = c->lap[0] * gp[0] + c->lap[1] * gp[1] + c->lap[2] * gp[2];
+      ws->out[i] += c->c * phi * phi + bu * phi + c->k * gg + phi * bg + c->dn * ni * ni +
+                    si * si + basis_lap(g, i, q) * li;
     }
   }
   free(qc);
@@ -1152,15 +1294,23 @@ static void rhs_local(const orc_operator *op, const geo_t *g, long cell, cell_ws
     const orc_advec_params *ap = &op->advec;
     for (int l = 0; l < nq; ++l) ws->tmp[l] = io->src[ws->idx[l]];
     eval_vg(g, ws->tmp, ws->val, ws->grad, &ws->w);
+    /* the Laplacian before eval_velocity reuses ws->tmp */
+    if (ap->supg) eval_laplacian(g, ws->tmp, ws->lap, ws->To, ws->w.t1, ws->w.t2);
     eval_velocity(op, g, cell, ws);
+    const double tau = ap->supg ? supg_tau(op, g, ws->wv) : 0.0;
     for (int q = 0; q < nq; ++q) {
       double vg = 0.0;
       for (int e = 0; e < dim; ++e) vg += ws->wv[e][q] * ws->grad[e * nq + q];
       const double jxw = cell_volume_weight(g, q);
-      ws->fval[q] = jxw * (ws->val[q] / ap->dt - (1.0 - ap->theta) * vg);
-      for (int e = 0; e < dim; ++e)
-        ws->flux[e * nq + q] =
-            -jxw * (1.0 - ap->theta) * ap->diffusivity * ws->grad[e * nq + q];
+      double res = ws->val[q] / ap->dt - (1.0 - ap->theta) * vg;
+      ws->fval[q] = jxw * res;
+      /* SUPG (create_rhs, advection_diffusion_operator.cpp:338-343) */
+      if (ap->supg) res += (1.0 - ap->theta) * ap->diffusivity * ws->lap[q];
+      for (int e = 0; e < dim; ++e) {
+        double f = -(1.0 - ap->theta) * ap->diffusivity * ws->grad[e * nq + q];
+        if (ap->supg) f += tau * ws->wv[e][q] * res;
+        ws->flux[e * nq + q] = jxw * f;
+      }
     }
   }
   integrate(g, ws->fval, ws->flux, ws->out, &ws->w);
@@ -1448,7 +1598,11 @@ int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *
   double *old = malloc(sizeof(double) * n), *rhs = malloc(sizeof(double) * n);
   double *xd = malloc(sizeof(double) * n), *inv = malloc(sizeof(double) * n);
   int status = 0;
-  for (long i = 0; i < n; ++i) xd[i] = (op.mask && op.mask[i]) ? dirichlet_values[i] : 0.0;
+  /* constrained entries: Dirichlet rows and the inflow/outflow DoFs (create_inflow_outflow_
+   * constraints, advection_diffusion_operation.cpp:368-499, lifted through
+   * create_rhs_and_apply_dirichlet_matrixfree's additional inhomogeneous constraints) */
+#define ORC_LIFTED(i) ((op.mask && op.mask[i]) || (op.io_mask && op.io_mask[i]))
+  for (long i = 0; i < n; ++i) xd[i] = ORC_LIFTED(i) ? dirichlet_values[i] : 0.0;
   /* adaptive mesh: fields kept constraint-consistent (hanging values from their masters) */
   if (op.amesh) orc_distribute_constraints(op.adata, phi);
   for (int step = 0; step < n_steps; ++step) {
@@ -1458,15 +1612,19 @@ int orc_advec_theta_steps(const orc_operator *op_in, int n_steps, const double *
     for (long i = 0; i < n; ++i) rhs[i] = -rhs[i];
     orc_create_rhs(&op, old, NULL, rhs);
     for (long i = 0; i < n; ++i)
-      if (op.mask && op.mask[i]) rhs[i] = 0.0;
+      if (ORC_LIFTED(i)) rhs[i] = 0.0;
     op.use_mask = 1;
     orc_diagonal(&op, inv);
     orc_invert_diagonal(ORC_OP_ADVEC, n, inv);
+    /* compute_inverse_diagonal_from_matrixfree: inflow/outflow rows 1 (:437-444) */
+    for (long i = 0; i < n; ++i)
+      if (op.io_mask && op.io_mask[i]) inv[i] = 1.0;
     int iters;
     double fres;
     status |= orc_solve(&op, ORC_GMRES, 30, 10000, 1e-20, 1e-12, inv, rhs, phi, &iters, &fres);
     for (long i = 0; i < n; ++i)
-      if (op.mask && op.mask[i]) phi[i] = dirichlet_values[i];
+      if (ORC_LIFTED(i)) phi[i] = dirichlet_values[i];
+#undef ORC_LIFTED
     if (op.amesh) orc_distribute_constraints(op.adata, phi);
     norms[step] = op.amesh ? 0.0 : orc_fe_l2_norm(&op.mesh, phi);
   }

@@ -77,6 +77,13 @@ typedef struct {
   double diffusivity;
   double mass_factor;  /* multiplies 1/dt in the tangent mass term: 1 for theta schemes, 1.5 for BDF2 */
   double vel_scale;    /* ORC_VEL_VORTEX: w = s*(x_last, 0.., -x_0) */
+  /* SUPG stabilisation (ConvectionStabilizationType::SUPG, advection_diffusion_operator.cpp:
+   * 455-530): flux += tau w (mass_factor/dt u + theta w . grad u - theta D lap u) in the tangent,
+   * tau w (u_old/dt - (1-theta) w . grad u_old + (1-theta) D lap u_old) in create_rhs, with
+   * tau = cell size * (coefficient < 0 ? (max_q |w| >= 1e-6 ? 1/max_q |w| : 0) : coefficient)
+   * per cell (compute_stabilization_parameter, :497-530), cell size = diameter / sqrt(dim). */
+  int supg;
+  double supg_coefficient;
 } orc_advec_params;
 
 /* ORC_OP_NORMAL: the normal vector projection, a block operator on [dim][n] vectors (the same
@@ -167,6 +174,11 @@ typedef struct {
   orc_ls_params ls;         /* ORC_OP_NORMAL / ORC_OP_CURVATURE */
   const double *level_set;  /* their narrow-band field (read plain) */
   orc_reinit_params reinit; /* ORC_OP_REINIT (normal vector in vel, vel_kind ORC_VEL_NODAL) */
+  /* advection: inflow/outflow DoFs treated by pre/post-vmult (AdvectionDiffusionOperator::
+   * do_pre_vmult / do_post_vmult, advection_diffusion_operator.cpp:540-578): with use_mask set,
+   * vmult reads their src entries as 0 and returns dst = src on them (identity rows), and
+   * orc_advec_theta_steps lifts them like Dirichlet values.  NULL = none. */
+  const uint8_t *io_mask;
 } orc_operator;
 
 /* --- basis / mesh ----------------------------------------------------------------------- */

@@ -46,7 +46,8 @@ def material(hp, T):
 
 class AdvecParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("diffusivity", C.c_double),
-                ("mass_factor", C.c_double), ("vel_scale", C.c_double)]
+                ("mass_factor", C.c_double), ("vel_scale", C.c_double),
+                ("supg", C.c_int), ("supg_coefficient", C.c_double)]
 
 
 class AMeshC(C.Structure):
@@ -93,7 +94,7 @@ class Operator(C.Structure):
                 ("k_cell", C.c_void_p), ("nthreads", C.c_int),
                 ("amesh", C.POINTER(AMeshC)), ("adata", C.POINTER(ADataC)),
                 ("bnd", C.POINTER(HeatBoundary)), ("ls", LsParams),
-                ("level_set", C.c_void_p), ("reinit", ReinitParams)]
+                ("level_set", C.c_void_p), ("reinit", ReinitParams), ("io_mask", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -247,7 +248,7 @@ class OracleOperator:
 
     def __init__(self, kind, mesh, heat=None, advec=None, vel_kind=VEL_NONE, mask=None,
                  T=None, T_old=None, vel=None, k_cell=None, nthreads=0, use_mask=1, amesh=None,
-                 boundary=None, ls=None, level_set=None, reinit=None):
+                 boundary=None, ls=None, level_set=None, reinit=None, io_mask=None):
         self._keep = []
         self.amesh = amesh
         if amesh is not None:
@@ -286,6 +287,7 @@ class OracleOperator:
         if ls is not None:
             self.op.ls = ls
         self.op.level_set = ptr(level_set)
+        self.op.io_mask = ptr(io_mask, np.uint8)
         if reinit is not None:
             self.op.reinit = reinit
 

@@ -194,3 +194,34 @@ def test_unidirectional_robin_velocity_mushy_golden_extruded(name):
         op.set_fields(velocity=vel)
     norms = _steps(mf, s, op, T0f(X[:, 0]), [dt] * 5, (10, tol * Lz, 1e-10)) / np.sqrt(Lz)
     assert [f"{x:.5e}" for x in norms] == g
+
+
+def test_advection_vortex_3d_inflow_outflow_golden():
+    """advection_diffusion_vortex_mf_inflow_outflow_3d: as the 3D vortex case, but both boundary
+    ids are inflow/outflow boundaries with value -1: the inflow DoFs (n . u <= 0 at the boundary
+    nodes, the y faces included) come from mf_inflow_outflow_mask, the initial field is not
+    constrained, vmult uses the pre/post identity rows."""
+    import meltpooldg_amd as mf
+    g = json.load(open(GOLDEN))["advection_diffusion_vortex_mf_inflow_outflow_3d"]["norms"]
+    s = mf.ScratchData(1, 8, (-1.0,) * 3, (1.0,) * 3, stream=torch.cuda.current_stream())
+    X = s.support_points()
+    x, z = X[:, 0], X[:, 2]
+    vel = _dev(np.concatenate([4.0 * z, np.zeros_like(z), -4.0 * x]))
+    io = mf.inflow_outflow_mask(s, vel)
+    from test_oracle_golden import vortex_inflow_mask
+    assert np.array_equal(io, vortex_inflow_mask(orc.make_mesh(3, 1, 8, (-1.0,) * 3, (1.0,) * 3), 3))
+    eps = 0.5 * 0.25
+    phi0 = np.tanh(-(np.linalg.norm(X - np.array([0.0, 0.0, 0.5]), axis=1) - 0.25) / (2 * eps))
+    op = mf.AdvectionDiffusionOperator(s, theta=0.5, diffusivity=0.0, dt=0.01)
+    op.set_fields(velocity=vel)
+    op.set_inflow_outflow_bc(io)
+    dv = _dev(-np.ones(s.n_dofs))
+    phi = _dev(phi0)
+    norms = []
+    for _ in range(5):
+        old = phi.clone()
+        rep = mf.advection_step(op, phi, old, dv, solver="GMRES", rel_tolerance=1e-12)
+        assert rep["status"] == 0
+        norms.append(mf.compute_norm_l2(s, phi))
+    rel = np.abs(np.array(norms) / np.array(g, dtype=float) - 1)
+    assert rel.max() < 5e-8, (norms, rel)

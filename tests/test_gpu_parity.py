@@ -130,3 +130,45 @@ def test_no_bc_dof_index(case):
     op.vmult(dst, _dev(src))
     torch.cuda.synchronize()
     assert rel_err(dst.cpu().numpy(), ref.vmult(src)) < TOL64
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_inflow_outflow_rows(degree):
+    """Advection with inflow/outflow DoFs (and Dirichlet rows) against the oracle: vmult (FP64,
+    NaN-filled dst) and FP32 with the identity rows, the inverse diagonal with 1 on them, and
+    the treatment switched off (set_apply_inflow_outflow_constraints_pre_post_matrix_free)."""
+    import meltpooldg_amd as mf
+    case = Case(degree=degree, kind="advec", mask="random")
+    f = fields(case)
+    rng = np.random.default_rng(degree)
+    io = ((rng.uniform(size=len(f["mask"])) < 0.25) & (f["mask"] == 0)).astype(np.uint8)
+    m = case.mesh()
+    ap = orc.AdvecParams(dt=case.dt, theta=case.theta, diffusivity=case.diffusivity,
+                         mass_factor=case.mass_factor, vel_scale=0.0)
+    ref = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=f["mask"],
+                             vel=f["vel"], io_mask=io)
+    s, op = device_operator(case, f)
+    op.set_inflow_outflow_bc(io)
+    src = orc.random_field(m, 21)
+    dst = torch.full((s.n_dofs,), float("nan"), dtype=torch.float64, device="cuda")
+    op.vmult(dst, _dev(src))
+    torch.cuda.synchronize()
+    expect = ref.vmult(src)
+    assert rel_err(dst.cpu().numpy(), expect) < TOL64
+    assert np.array_equal(dst.cpu().numpy()[io == 1], src[io == 1])
+    d32 = torch.full((s.n_dofs,), float("nan"), dtype=torch.float32, device="cuda")
+    op.vmult_f32(d32, torch.tensor(src, dtype=torch.float32, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(d32.double().cpu().numpy(), expect) < TOL32
+    inv = torch.zeros(s.n_dofs, dtype=torch.float64, device="cuda")
+    op.compute_inverse_diagonal_from_matrixfree(inv)
+    torch.cuda.synchronize()
+    ref_inv = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL,
+                                 mask=f["mask"], vel=f["vel"]).inverse_diagonal()
+    ref_inv[io == 1] = 1.0
+    assert rel_err(inv.cpu().numpy(), ref_inv) < TOL64
+    op.set_apply_inflow_outflow_constraints_pre_post_matrix_free(False)
+    op.vmult(dst, _dev(src))
+    torch.cuda.synchronize()
+    plain = oracle_operator(case, f).vmult(src)
+    assert rel_err(dst.cpu().numpy(), plain) < TOL64

@@ -72,6 +72,20 @@ def test_advection_vortex(name, dim, ref, tol):
     assert rel.max() < tol, rel
 
 
+def test_advection_vortex_supg():
+    """advection_diffusion_vortex_mf_supg: the 2D vortex case with SUPG (coefficient -1:
+    tau = h / max_q |w| per cell); without the stabilisation the norms differ by 6e-4."""
+    g = golden()["advection_diffusion_vortex_mf_supg"]
+    m, phi0, mask = vortex_setup(2, 5)
+    ap = orc.AdvecParams(dt=0.01, theta=0.5, diffusivity=0.0, mass_factor=1.0, vel_scale=4.0,
+                         supg=1, supg_coefficient=-1.0)
+    op = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_VORTEX, mask=mask)
+    _, norms, status = op.advec_theta_steps(phi0, 5, -np.ones(len(phi0)))
+    assert status == 0
+    rel = np.abs(norms / np.array(g["norms"], dtype=float) - 1.0)
+    assert rel.max() < 2e-8, rel
+
+
 # ---------------------------------------------------------------- T-dependent law, Robin faces
 def mushy(**kw):
     """HeatParams of the liquid_solid material with the mushy_zone transition."""
@@ -156,3 +170,41 @@ def test_unidirectional_robin_velocity_mushy_2d_q1(name):
     out = orc.heat_time_steps(m, hp, np.zeros(n, np.uint8), T0f(X[:, 0]), [dt] * 5, boundary=b,
                               vel=vel, newton=(10, tol, 1e-10))
     assert [f"{orc.fe_l2_norm(m, T):.5e}" for T in out] == g["norms"]
+
+
+def vortex_inflow_mask(m, dim, scale=4.0):
+    """create_inflow_outflow_constraints (advection_diffusion_operation.cpp:368-499) for the
+    vortex case with both boundary ids inflow/outflow (advection_diffusion_vortex.hpp
+    set_boundary_conditions): boundary support points with n . u <= 0, u = s (y, -x) (2D) or
+    s (z, 0, -x) (3D); in 3D the y faces (n . u = 0) are all inflow."""
+    xyz = orc.node_coords(m)
+    u = np.zeros((len(xyz), 3))
+    u[:, 0] = scale * xyz[:, dim - 1]
+    u[:, dim - 1] = -scale * xyz[:, 0]
+    io = np.zeros(len(xyz), bool)
+    for d in range(dim):
+        for sgn, val in ((-1.0, -1.0), (1.0, 1.0)):
+            io |= (np.abs(xyz[:, d] - val) < 1e-12) & (sgn * u[:, d] <= 0.0)
+    return io.astype(np.uint8)
+
+
+@pytest.mark.parametrize("name,dim,ref,tol", [
+    ("advection_diffusion_vortex_mf_inflow_outflow", 2, 5, 2e-8),
+    ("advection_diffusion_vortex_mf_inflow_outflow_3d", 3, 3, 5e-8)])
+def test_advection_vortex_inflow_outflow(name, dim, ref, tol):
+    """Inflow/outflow boundaries (pre/post-vmult identity rows, value -1 lifted): the initial
+    field is not constrained (no Dirichlet rows), the inflow set comes from the velocity."""
+    g = golden()[name]
+    n = 2 ** ref
+    m = orc.make_mesh(dim, 1, n, (-1.0,) * dim, (1.0,) * dim)
+    xyz = orc.node_coords(m)
+    center = np.array([0.0, 0.5, 0.0]) if dim == 2 else np.array([0.0, 0.0, 0.5])
+    eps = 0.5 * (2.0 / n)
+    phi0 = np.tanh(-(np.linalg.norm(xyz - center, axis=1) - 0.25) / (2 * eps))
+    ap = orc.AdvecParams(dt=0.01, theta=0.5, diffusivity=0.0, mass_factor=1.0, vel_scale=4.0)
+    op = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_VORTEX,
+                            io_mask=vortex_inflow_mask(m, dim))
+    _, norms, status = op.advec_theta_steps(phi0, 5, -np.ones(len(phi0)))
+    assert status == 0
+    rel = np.abs(norms / np.array(g["norms"], dtype=float) - 1.0)
+    assert rel.max() < tol, rel

@@ -69,10 +69,11 @@ def heat_op(m, law="const", td=0, vel=False, kcell=False, mask=None, seed=0, rho
                               use_mask=use_mask)
 
 
-def advec_op(m, mask=None, seed=0, theta=0.5, D=0.1):
+def advec_op(m, mask=None, seed=0, theta=0.5, D=0.1, supg=0, supg_coefficient=-1.0):
     n = orc.n_dofs(m)
     rng = np.random.default_rng(seed)
-    ap = orc.AdvecParams(dt=0.1, theta=theta, diffusivity=D, mass_factor=1.0, vel_scale=0)
+    ap = orc.AdvecParams(dt=0.1, theta=theta, diffusivity=D, mass_factor=1.0, vel_scale=0,
+                         supg=supg, supg_coefficient=supg_coefficient)
     return orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=mask,
                               vel=rng.uniform(-1, 1, 3 * n))
 
@@ -92,8 +93,13 @@ MESHES = [(1, 1, (4,)), (1, 3, (3,)), (2, 1, (3, 2)), (2, 2, (2, 3)), (2, 3, (2,
 
 
 @pytest.mark.parametrize("dim,deg,nc", MESHES)
-@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td", "heat-vel-kc", "advec"])
+@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td", "heat-vel-kc", "advec",
+                                     "advec-supg", "advec-supg-coef"])
 def test_sum_factorisation_matches_dense(dim, deg, nc, variant):
+    """The dense path builds each local matrix from products of 1D basis values, derivatives
+    and second derivatives at each quadrature point; the product path sweeps the 1D
+    matrices.  With SUPG the dense path's Laplacian term is rank-one per quadrature point
+    (lap u (w . grad v)) and its streamline term a square (sw . grad u)(sw . grad v)."""
     hi = (1.0, 0.8, 1.3)[:dim]
     m = orc.make_mesh(dim, deg, nc, (0.0,) * dim, hi)
     mask = orc.boundary_mask(m)
@@ -103,8 +109,12 @@ def test_sum_factorisation_matches_dense(dim, deg, nc, variant):
         op = heat_op(m, law="tanh", td=1, mask=mask)
     elif variant == "heat-vel-kc":
         op = heat_op(m, vel=True, kcell=True, td=1, law="tanh")
-    else:
+    elif variant == "advec":
         op = advec_op(m, mask=mask)
+    elif variant == "advec-supg":
+        op = advec_op(m, mask=mask, supg=1)
+    else:
+        op = advec_op(m, mask=mask, supg=1, supg_coefficient=0.7, theta=1.0)
     u = np.random.default_rng(1).uniform(-1, 1, op.n)
     a, b = op.vmult(u), op.vmult_dense(u)
     assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
@@ -175,7 +185,7 @@ def test_advection_linear_field_and_skew_part():
     assert np.allclose(conv, (0.3 * 1.0 - 0.2 * 2.0 + 0.5 * -1.0) * mass1, atol=1e-13)
 
 
-@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td-vel", "advec"])
+@pytest.mark.parametrize("variant", ["heat", "heat-tanh-td-vel", "advec", "advec-supg"])
 def test_diagonal_matches_assembled(variant):
     m = orc.make_mesh(3, 2, (2, 2, 3), (0, 0, 0), (1, 1.1, 0.9))
     mask = orc.boundary_mask(m)
@@ -184,8 +194,10 @@ def test_diagonal_matches_assembled(variant):
         op = heat_op(m, mask=mask)
     elif variant == "heat-tanh-td-vel":
         op = heat_op(m, law="tanh", td=1, vel=True, kcell=True, mask=mask)
-    else:
+    elif variant == "advec":
         op = advec_op(m, mask=mask)
+    else:
+        op = advec_op(m, mask=mask, supg=1)
     A = assemble(op)
     d = op.diagonal()
     assert np.max(np.abs(d - np.diag(A))) < 1e-13 * np.max(np.abs(d))
@@ -519,3 +531,99 @@ def test_tangent_residual_derivative_low_dims(dim):
     fd = (_residual(m, hp, T + eps * u, T_old, q, kw) -
           _residual(m, hp, T - eps * u, T_old, q, kw)) / (2 * eps)
     assert np.abs(Ju - fd).max() / np.abs(Ju).max() < 1e-7
+
+
+def test_inflow_outflow_rows_are_identity_rows():
+    """do_pre_vmult / do_post_vmult: with inflow/outflow DoFs I, vmult applies A' with
+    A'[i, j] = A[i, j] off I, zero columns j in I on the other rows, and unit rows on I."""
+    m = orc.make_mesh(2, 2, (3, 2), hi=(1.0, 0.7))
+    n = orc.n_dofs(m)
+    rng = np.random.default_rng(3)
+    vel = rng.uniform(-1, 1, 2 * n)
+    io = (rng.uniform(size=n) < 0.3).astype(np.uint8)
+    ap = orc.AdvecParams(dt=0.1, theta=0.5, diffusivity=0.02, mass_factor=1.0)
+    plain = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, vel=vel)
+    iop = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, vel=vel,
+                             io_mask=io)
+    E = np.eye(n)
+    A = np.stack([plain.vmult(E[:, j]) for j in range(n)], axis=1)
+    Ap = np.stack([iop.vmult(E[:, j]) for j in range(n)], axis=1)
+    I = io == 1
+    expect = A.copy()
+    expect[:, I] = 0.0
+    expect[I, :] = 0.0
+    expect[I, I] = 1.0
+    assert np.abs(Ap - expect).max() < 1e-14 * np.abs(A).max()
+    # the no-BC index (use_mask = 0) ignores the inflow/outflow set, as the lifting needs
+    iop0 = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, vel=vel,
+                              io_mask=io, use_mask=0)
+    x = rng.uniform(-1, 1, n)
+    assert np.array_equal(iop0.vmult(x), plain.vmult(x))
+
+
+# ---------------------------------------------------------------- SUPG
+def _supg_ops(m, w, D, coef, theta=0.5, **kw):
+    """advection operators with constant nodal velocity w, diffusivity D or 0, SUPG on or off"""
+    n = orc.n_dofs(m)
+    vel = np.concatenate([np.full(n, wi) for wi in w])
+    out = {}
+    for supg in (0, 1):
+        for d in (D, 0.0):
+            ap = orc.AdvecParams(dt=0.1, theta=theta, diffusivity=d, mass_factor=1.0,
+                                 supg=supg, supg_coefficient=coef)
+            out[supg, d] = orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL,
+                                              vel=vel, **kw)
+    return out
+
+
+# u, its Laplacian integrated against w . grad v for v = x (= w_x int lap u over the unit
+# cube), exact in the FE space of the given degree
+LAP_CASES = [
+    (2, 2, lambda x, y, z: x * x * y * y, 4.0 / 3.0),                     # 2y^2 + 2x^2
+    (2, 3, lambda x, y, z: x ** 3 * y ** 3, 1.5),                        # 6xy^3 + 6x^3y
+    (3, 2, lambda x, y, z: x * x * y * y * z * z, 2.0 / 3.0),            # sum 2 y^2 z^2 ...
+    (3, 3, lambda x, y, z: x ** 3 * y * z + y ** 2, 0.75 + 2.0),         # 6xyz + 2
+]
+
+
+@pytest.mark.parametrize("dim,deg,f,lap_int", LAP_CASES)
+def test_supg_laplacian_term_closed_form(dim, deg, f, lap_int):
+    """The SUPG Laplacian terms alone: (A_supg(D) - A_supg(0) - A(D) + A(0)) u equals
+    -tau theta D (w . grad v, lap u) in the tangent and create_rhs gives +tau (1 - theta) D
+    (w . grad v, lap u_old); weighted with the nodal values of v = x it is the closed form
+    w_x int lap u over the unit cube.  tau = coefficient * h on square cells."""
+    nc = 3
+    m = orc.make_mesh(dim, deg, (nc,) * dim)
+    X = orc.node_coords(m)
+    u = f(X[:, 0], X[:, 1], X[:, 2] if dim == 3 else 0 * X[:, 0])
+    w = (0.6, -0.3, 0.2)[:dim]
+    D, coef, theta = 0.05, 0.8, 0.5
+    ops = _supg_ops(m, w, D, coef, theta)
+    tau = coef / nc
+    r = (ops[1, D].vmult(u) - ops[1, 0.0].vmult(u) - ops[0, D].vmult(u) + ops[0, 0.0].vmult(u))
+    got = X[:, 0] @ r
+    assert abs(got - (-tau * theta * D * w[0] * lap_int)) < 1e-12 * max(1.0, abs(got))
+    rr = (ops[1, D].create_rhs(u) - ops[1, 0.0].create_rhs(u) - ops[0, D].create_rhs(u) +
+          ops[0, 0.0].create_rhs(u))
+    got = X[:, 0] @ rr
+    assert abs(got - tau * (1 - theta) * D * w[0] * lap_int) < 1e-12 * max(1.0, abs(got))
+    # v = 1: w . grad v = 0, no SUPG contribution at all
+    assert abs(r.sum()) < 1e-13 and abs(rr.sum()) < 1e-13
+
+
+def test_supg_streamline_term_closed_form():
+    """The streamline term tau (w . grad v)(m u/dt + theta w . grad u) for D = 0 and u = x,
+    v = x on the unit square: (A_supg - A) u weighted by v's nodal values is
+    tau w_x (m/dt int x + theta w_x) = tau w_x (m/(2 dt) + theta w_x); with the coefficient < 0
+    tau = h / |w|."""
+    m = orc.make_mesh(2, 2, (4, 4))
+    X = orc.node_coords(m)
+    w = (0.6, -0.8)
+    ops = _supg_ops(m, w, 0.0, -1.0, theta=0.5)
+    tau = 0.25 / 1.0
+    r = ops[1, 0.0].vmult(X[:, 0]) - ops[0, 0.0].vmult(X[:, 0])
+    expect = tau * w[0] * (1.0 / (2 * 0.1) + 0.5 * w[0])
+    assert abs(X[:, 0] @ r - expect) < 1e-13
+    # below 1e-6 the velocity switches the stabilisation off
+    ops = _supg_ops(m, (1e-7, 0.0), 0.0, -1.0)
+    assert np.array_equal(ops[1, 0.0].vmult(X[:, 0]), ops[0, 0.0].vmult(X[:, 0]))
