@@ -218,6 +218,13 @@ int mf_operator_vmult_f32(mf_operator *op, float *dst, const float *src);
 /* OperatorMatrixFree::compute_inverse_diagonal_from_matrixfree (operator_base.hpp:268-272;
  * heat_diffuse_operator.cpp:271-284, advection_diffusion_operator.cpp:399-446) */
 int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag);
+/* The same inverse diagonal without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  The reduction partials of the relative threshold live in a static
+ * 8 KiB device array per GPU (1024 doubles, the maximum; the need does not grow with the
+ * vector length), looked up on the first call on each device with that device current.
+ * Calls that overlap on one GPU race on that array. */
+int mf_operator_compute_inverse_diagonal_noalloc(mf_operator *op, double *inv_diag);
 /* OperatorMatrixFree::create_rhs (operator_base.hpp:208-212; heat_diffuse_operator.cpp:783-806,
  * advection_diffusion_operator.cpp:274-358): ADDS into dst.  Heat: src = T_old, q_s = heat
  * source (may be NULL), result -R(T); on an adaptive mesh T, src and q_s are read as stored

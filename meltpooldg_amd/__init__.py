@@ -248,6 +248,7 @@ def lib():
         L.mf_operator_vmult_f32.argtypes = [vp, vp, vp]
         L.mf_operator_vmult_add.argtypes = [vp, vp, vp]
         L.mf_operator_compute_inverse_diagonal.argtypes = [vp, vp]
+        L.mf_operator_compute_inverse_diagonal_noalloc.argtypes = [vp, vp]
         L.mf_operator_create_rhs.argtypes = [vp, vp, vp, vp]
         L.mf_solve.argtypes = [vp, C.POINTER(SolverDesc), vp, vp, C.POINTER(SolveReport)]
         L.mf_jacobi_create.argtypes = [vp, C.POINTER(vp)]
@@ -573,6 +574,11 @@ class _OperatorBase:
 
     def compute_inverse_diagonal_from_matrixfree(self, diag):
         _check(lib().mf_operator_compute_inverse_diagonal(self._h, self._r(diag)))
+
+    def compute_inverse_diagonal_noalloc(self, diag):
+        """The same inverse diagonal with no device allocation (static per-GPU scratch; calls
+        that overlap on one GPU race)."""
+        _check(lib().mf_operator_compute_inverse_diagonal_noalloc(self._h, self._r(diag)))
 
     def create_rhs(self, dst, src, q_s=None):
         sc = self.scratch

@@ -775,7 +775,14 @@ int mf_operator_vmult_f32(mf_operator *op, float *dst, const float *src)
   return apply_tangent<float>(op, dst, src, nullptr, nullptr, true);
 }
 
-int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
+}  // extern "C"
+
+namespace mf {
+
+// The body of mf_operator_compute_inverse_diagonal.  work: RED_BLOCKS doubles for the partial
+// maxima of the relative threshold, or nullptr to take them from the operator's small-work arena
+// (allocated on first use).  With work given, nothing here allocates device memory.
+int compute_inverse_diagonal(mf_operator *op, double *inv_diag, double *work)
 {
   int rc = check_ready(op);
   if (rc) return rc;
@@ -812,15 +819,23 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
                               hipMemcpyDeviceToDevice, st));
   // heat: |d| > 1e-10; advection and the level-set projections: |d| > 1e-14 max(1, |d|_inf)
   const bool relative = op->kind != 0;
-  double *work = nullptr;
-  if (relative) {
+  if (relative && !work) {
     if ((rc = arena_reserve(op->arena[3], sizeof(double) * RED_BLOCKS))) return rc;
     work = reinterpret_cast<double *>(op->arena[3].p);
   }
-  rc = vec_invert_diagonal(inv_diag, n_rows(op), relative, work, st);
+  rc = vec_invert_diagonal(inv_diag, n_rows(op), relative, relative ? work : nullptr, st);
   if (rc || !io_active(op)) return rc;
   // compute_inverse_diagonal_from_matrixfree: 1 on the inflow/outflow rows
   return vec_masked_fill(inv_diag, op->io_mask, 1.0, m.n_dofs, st);
+}
+
+}  // namespace mf
+
+extern "C" {
+
+int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
+{
+  return compute_inverse_diagonal(op, inv_diag, nullptr);
 }
 
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)

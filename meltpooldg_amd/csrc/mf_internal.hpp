@@ -339,4 +339,8 @@ inline void reinit_epsilon(const mf_operator *op, double &eps_n, double &eps_t, 
   }
   eps_t = r.tangential_diffusion_factor * eps_n;
 }
+
+// mf_operator_compute_inverse_diagonal with the RED_BLOCKS doubles of the reduction partials
+// given by the caller (work = nullptr: the operator's own arena)
+int compute_inverse_diagonal(mf_operator *op, double *inv_diag, double *work);
 }  // namespace mf
