@@ -156,6 +156,16 @@ typedef struct {
   double theta;        /* 1 implicit Euler, 0.5 Crank-Nicolson */
   double diffusivity;
   double mass_factor;  /* 1 for theta schemes, 1.5 for BDF2 */
+  /* ConvectionStabilizationData (include/meltpooldg/level_set/advection_diffusion_data.hpp:
+   * 34-38): supg = 1 selects ConvectionStabilizationType::SUPG
+   * (advection_diffusion_operator.cpp:455-530).  The tangent's gradient term gains
+   * tau w (mass_factor/dt u + theta w . grad u - theta D lap u), create_rhs's
+   * tau w (u_old/dt - (1-theta) w . grad u_old + (1-theta) D lap u_old), with per cell
+   * tau = (diameter / sqrt(3)) * (supg_coefficient < 0 ? (max_q |w| >= 1e-6 ? 1 / max_q |w| : 0)
+   * : supg_coefficient).  The reference's default coefficient is -1.  The transposed operator
+   * (mf_operator_set_transpose) is not available with SUPG. */
+  int supg;
+  double supg_coefficient;
 } mf_advec_params;
 
 typedef struct mf_operator mf_operator;

@@ -54,7 +54,8 @@ class HeatParams(C.Structure):
 
 
 class AdvecParams(C.Structure):
-    _fields_ = [("theta", C.c_double), ("diffusivity", C.c_double), ("mass_factor", C.c_double)]
+    _fields_ = [("theta", C.c_double), ("diffusivity", C.c_double), ("mass_factor", C.c_double),
+                ("supg", C.c_int), ("supg_coefficient", C.c_double)]
 
 
 class SolverDesc(C.Structure):
@@ -648,10 +649,14 @@ class HeatDiffuseOperator(_OperatorBase):
 
 class AdvectionDiffusionOperator(_OperatorBase):
     """Level-set advection-diffusion tangent (reference AdvectionDiffusionOperator,
-    source/level_set/advection_diffusion_operator.cpp): (m/dt) M + theta (w . grad) + theta D K."""
+    source/level_set/advection_diffusion_operator.cpp): (m/dt) M + theta (w . grad) + theta D K.
+    supg=True adds the SUPG stabilisation (ConvectionStabilizationType::SUPG) with
+    supg_coefficient (ConvectionStabilizationData::coefficient, default -1: tau = h / max |w|
+    per cell)."""
 
-    def __init__(self, scratch, theta=1.0, diffusivity=0.0, mass_factor=1.0, dt=None):
-        p = AdvecParams(theta, diffusivity, mass_factor)
+    def __init__(self, scratch, theta=1.0, diffusivity=0.0, mass_factor=1.0, dt=None, supg=False,
+                 supg_coefficient=-1.0):
+        p = AdvecParams(theta, diffusivity, mass_factor, int(bool(supg)), supg_coefficient)
         h = C.c_void_p()
         _check(lib().mf_advection_operator_create(scratch._h, C.byref(p), C.byref(h)))
         super().__init__(scratch, h)

@@ -48,6 +48,8 @@ int mf_laser_heat_source(mf_scratch *s, double power, double radius, const doubl
 int mf_operator_set_transpose(mf_operator *op, int transpose)
 {
   MF_REQUIRE(op, MF_ERR_ARG, "null operator");
+  MF_REQUIRE(!transpose || op->kind != 1 || !op->advec.supg, MF_ERR_ARG,
+             "the transposed advection operator is not available with SUPG");
   op->transpose = transpose ? 1 : 0;
   return MF_OK;
 }

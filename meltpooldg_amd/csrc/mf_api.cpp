@@ -172,6 +172,21 @@ static CellArgs<Number> make_args(const mf_operator *op)
     a.theta = Number(op->advec.theta);
     a.diffusivity = Number(op->advec.diffusivity);
     a.mass_factor = Number(op->advec.mass_factor);
+    if (op->advec.supg) {
+      // cell size diameter / sqrt(dim) of a level-0 cell (ScratchData::build,
+      // source/core/scratch_data.cpp:256-278)
+      const double d2 = m.h[0] * m.h[0] + m.h[1] * m.h[1] + m.h[2] * m.h[2];
+      a.supg_cell_size = Number(std::sqrt(d2 / 3.0));
+      a.supg_coef = Number(op->advec.supg_coefficient);
+      // second derivatives of the 1D basis at the Gauss points: the collocation derivative of
+      // the first derivatives (exact: l_j' has degree p - 1)
+      for (int q = 0; q < K; ++q)
+        for (int j = 0; j < K; ++j) {
+          double v = 0.0;
+          for (int r = 0; r < K; ++r) v += s->tab.Dq[q * K + r] * s->tab.DS[r * K + j];
+          a.DDS[q * K + j] = Number(v);
+        }
+    }
   } else if (op->kind == 4) {
     double en, et;
     reinit_epsilon(op, en, et, a.re_scale_h);
@@ -227,9 +242,17 @@ static CellArgs<Number> make_args(const mf_operator *op)
 static int needs_T(const mf_operator *op)
 {
   if (op->kind == 4) return 0;
+  if (op->kind == 1) return 0;
   if (op->kind >= 2) return op->ls.narrow_band != 0 ? 1 : 0;
   if (op->kind != 0) return 0;
   return op->heat.law == MF_LAW_TANH ? 1 : (op->heat.law == MF_LAW_MUSHY ? 2 : 0);
+}
+
+// the cell kernels' field flag: needs_T, or for the advection operator SUPG on
+static int cell_flag(const mf_operator *op)
+{
+  if (op->kind == 1) return op->advec.supg ? 1 : 0;
+  return needs_T(op);
 }
 
 // the mushy law's tangent reads the old temperature
@@ -369,13 +392,13 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
       if (rc) return rc;
       a.hang_val = hv;
     }
-    int rc = launch_cell_list<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st, 0);
+    int rc = launch_cell_list<Number>(code, cell_flag(op), op->vel != nullptr, m, a, src, dst, st, 0);
     if (rc) return rc;
     // hanging rows hold the unconstrained cell sums: move them to their masters
     return launch_condense<Number>(op->s, dst, stop, a.use_mask != 0);
   }
   // the transposed operator (adjoint) and the level-set projections run on the pencil kernel
-  int rc = launch_cell<Number>(code, needs_T(op), op->vel != nullptr, m, a, src, dst, st);
+  int rc = launch_cell<Number>(code, cell_flag(op), op->vel != nullptr, m, a, src, dst, st);
   if (rc) return rc;
   if (io) {
     // do_post_vmult: dst = src on the inflow/outflow rows; their part of the fused dot follows
@@ -799,16 +822,16 @@ int compute_inverse_diagonal(mf_operator *op, double *inv_diag, double *work)
     if (rc) return rc;
     // cells without hanging nodes: local diagonal; cells with hanging nodes: condensed
     // diagonal c_D . (A_cell c_D) per target node D
-    rc = launch_cell_list<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
+    rc = launch_cell_list<double>(code, cell_flag(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 1);
     if (rc) return rc;
     const int tcode = op->kind == 0 ? HEAT_TANGENT
                       : op->kind == 1 ? ADVEC_TANGENT
                       : op->kind == 4 ? REINIT_TANGENT
                                       : LS_TANGENT;
-    rc = launch_cell_list<double>(tcode, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
+    rc = launch_cell_list<double>(tcode, cell_flag(op), op->vel != nullptr, m, a, nullptr, inv_diag, st, 2);
   } else {
     a.overwrite = 1;
-    rc = launch_cell<double>(code, needs_T(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
+    rc = launch_cell<double>(code, cell_flag(op), op->vel != nullptr, m, a, nullptr, inv_diag, st);
     if (rc) return rc;
     rc = launch_heat_faces<double>(op, nullptr, inv_diag, 1, nullptr);
   }
@@ -856,14 +879,14 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
     // residual on an adaptive mesh: T, src, q_s and the velocity read as stored
     // (constraint-consistent vectors), each cell adds into its own nodes, then the hanging rows
     // (of the cell sums and of what dst held) go to their masters
-    int rc3 = launch_cell_list<double>(op->kind == 0 ? HEAT_RHS : ADVEC_RHS, needs_T(op),
+    int rc3 = launch_cell_list<double>(op->kind == 0 ? HEAT_RHS : ADVEC_RHS, cell_flag(op),
                                        op->vel != nullptr, op->s->mesh, a, src, dst,
                                        op->s->stream, 0);
     if (rc3) return rc3;
     return launch_condense<double>(op->s, dst, nullptr, a.use_mask != 0);
   }
   const int code = op->kind == 0 ? HEAT_RHS : ADVEC_RHS;
-  int rc2 = launch_cell<double>(code, needs_T(op), op->vel != nullptr, op->s->mesh, a, src, dst,
+  int rc2 = launch_cell<double>(code, cell_flag(op), op->vel != nullptr, op->s->mesh, a, src, dst,
                                 op->s->stream);
   if (rc2) return rc2;
   return launch_heat_faces<double>(op, nullptr, dst, 2, nullptr);

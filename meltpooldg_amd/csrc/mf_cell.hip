@@ -724,6 +724,9 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   // for the d(rho cp)/dT (T - T_old)/dt term
   constexpr bool MUSHY = NEEDT == 2 && (OP == HEAT_TANGENT || OP == HEAT_DIAG || OP == HEAT_RHS);
   constexpr bool TOLD = MUSHY && !RHS_HEAT;
+  // advection with SUPG (NEEDT set): a per-cell tau from max_q |w|, and the Laplacian of u at
+  // the quadrature points (tangent, rhs) or of the basis functions (diagonal)
+  constexpr bool SUPG = NEEDT && (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG);
 
   // LDS cell tensors [K^3][CPB].  While values are interpolated: u in L0, T in L1, the velocity
   // components and the rhs fields after it.  Then the T gradients use L2, L3 and are reduced to
@@ -950,6 +953,31 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   }
   __syncthreads();
 
+  // SUPG parameter of the cell: the maximum of |w|^2 over its quadrature points, reduced over
+  // the cell's K^2 z pencils through the (free) velocity tensor
+  [[maybe_unused]] Number tau = 0;
+  if constexpr (SUPG) {
+    Number m2 = 0;
+#pragma unroll
+    for (int z = 0; z < K; ++z) {
+      const Number n2 = wq[0][z] * wq[0][z] + wq[1][z] * wq[1][z] + wq[2][z] * wq[2][z];
+      m2 = n2 > m2 ? n2 : m2;
+    }
+    Number *red = lds + I_W * BUF;
+    red[LY::idx(p, c)] = m2;
+    __syncthreads();
+    Number mx = 0;
+    for (int i = 0; i < K * K; ++i) {
+      const Number v = red[LY::idx(i, c)];
+      mx = v > mx ? v : mx;
+    }
+    __syncthreads();
+    mx = sqrt(mx);
+    const Number coef = A.supg_coef < Number(0) ? (mx >= Number(1e-6) ? Number(1) / mx : Number(0))
+                                                : A.supg_coef;
+    tau = A.supg_cell_size * hs * coef;
+  }
+
   const Number kc = (A.k_cell && ac.active) ? A.k_cell[ac.cell] : Number(1);
 
   // ---------------- T gradients -> per-quadrature-point coefficients ----------------
@@ -997,6 +1025,32 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   // ---------------- E4: d/dx and d/dy of u at the quadrature points ----------------
   if constexpr (HAS_U) {
     sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, L0, L1, Dq, L0, L2, p, c);
+    __syncthreads();
+  }
+  // SUPG: d2u/dx2 + d2u/dy2 (scaled) in L3 by the collocation derivative of the first
+  // derivatives (exact for Q_p); d2u/dz2 (unscaled) in registers
+  [[maybe_unused]] Number uzz[K];
+  if constexpr (SUPG && HAS_U) {
+    Number *const L3 = lds + 3 * BUF;
+    {
+      Number v[K], o[K];
+      load_pencil<Number, K, LY, 0>(L1, p, c, v);
+      matvec<Number, K, false>(Dq, v, o);
+#pragma unroll
+      for (int i = 0; i < K; ++i) o[i] *= ih[0] * ih[0];
+      store_pencil<Number, K, LY, 0>(L3, p, c, o);
+    }
+    matvec<Number, K, false>(Dq, guz, uzz);
+    __syncthreads();
+    {
+      Number v[K], o[K], a[K];
+      load_pencil<Number, K, LY, 1>(L2, p, c, v);
+      matvec<Number, K, false>(Dq, v, o);
+      load_pencil<Number, K, LY, 1>(L3, p, c, a);
+#pragma unroll
+      for (int i = 0; i < K; ++i) a[i] += o[i] * (ih[1] * ih[1]);
+      store_pencil<Number, K, LY, 1>(L3, p, c, a);
+    }
     __syncthreads();
   }
 
@@ -1098,11 +1152,25 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[d] += A.theta * w3[d] * uq[z];
       }
+      if constexpr (SUPG) {
+        // tau w (m u/dt + theta w . grad u - theta D lap u) against grad v (not transposed)
+        const Number lap = lds[3 * BUF + e] + uzz[z] * (ih[2] * ih[2]);
+        const Number res = val - A.theta * A.diffusivity * lap;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] += tau * w3[d] * res;
+      }
     } else if constexpr (OP == ADVEC_RHS) {
       const Number vg = g[0] * w3[0] + g[1] * w3[1] + g[2] * w3[2];
       val = A.dt_inv * uq[z] - (Number(1) - A.theta) * vg;
 #pragma unroll
       for (int d = 0; d < 3; ++d) f[d] = -(Number(1) - A.theta) * A.diffusivity * g[d];
+      if constexpr (SUPG) {
+        // tau w (u_old/dt - (1-theta) w . grad u_old + (1-theta) D lap u_old)
+        const Number lap = lds[3 * BUF + e] + uzz[z] * (ih[2] * ih[2]);
+        const Number res = val + (Number(1) - A.theta) * A.diffusivity * lap;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] += tau * w3[d] * res;
+      }
     } else if constexpr (REINIT) {
       // u v / dt + (eps_t grad u + (eps_n - eps_t) (grad u . n) n) . grad v, n = the normal
       // vector (held in the velocity slot) normalised with the tolerance
@@ -1144,11 +1212,11 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
         cc = msk;
         kk = msk * eta;
       }
-    } else {  // ADVEC_DIAG
+    } else {  // ADVEC_DIAG (SUPG: bb = w, the scalar factors applied in the diagonal loop)
       cc = A.mass_factor * A.dt_inv;
       kk = A.theta * A.diffusivity;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) bb[d] = A.theta * w3[d];
+      for (int d = 0; d < 3; ++d) bb[d] = SUPG ? w3[d] : A.theta * w3[d];
     }
     if constexpr (IS_DIAG) {
       L0[e] = jxw * cc;
@@ -1188,6 +1256,15 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     Number diag[K];
 #pragma unroll
     for (int ia = 0; ia < K; ++ia) diag[ia] = 0;
+    // SUPG: (theta + tau m/dt)(w . grad phi) phi + tau theta (w . grad phi)^2
+    // - tau theta D (w . grad phi) lap phi, with b = JxW w stored
+    [[maybe_unused]] Number s_beta = 0, s_gam = 0, s_del = 0, s_vol = 0;
+    if constexpr (SUPG) {
+      s_beta = A.theta + tau * A.mass_factor * A.dt_inv;
+      s_gam = tau * A.theta;
+      s_del = -tau * A.theta * A.diffusivity;
+      s_vol = (A.h[0] * hs) * (A.h[1] * hs) * (A.h[2] * hs);
+    }
     for (int q3 = 0; q3 < K; ++q3)
       for (int q2 = 0; q2 < K; ++q2) {
         const Number sy = A.S[q2 * K + jb], dy = A.DS[q2 * K + jb] * ih[1];
@@ -1202,7 +1279,15 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
             const Number sx = A.S[q1 * K + ia], dx = A.DS[q1 * K + ia] * ih[0];
             const Number phi = sx * sy * sz;
             const Number gx = dx * sy * sz, gy = sx * dy * sz, gz = sx * sy * dz;
-            if constexpr (OP == REINIT_DIAG) {
+            if constexpr (SUPG) {
+              const Number jxw = A.w[q1] * A.w[q2] * A.w[q3] * s_vol;
+              const Number lap = A.DDS[q1 * K + ia] * (ih[0] * ih[0]) * sy * sz +
+                                 sx * A.DDS[q2 * K + jb] * (ih[1] * ih[1]) * sz +
+                                 sx * sy * A.DDS[q3 * K + kb] * (ih[2] * ih[2]);
+              const Number wg = b0 * gx + b1 * gy + b2 * gz;
+              diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) +
+                          wg * (s_beta * phi + s_gam * wg / jxw + s_del * lap);
+            } else if constexpr (OP == REINIT_DIAG) {
               const Number bg = b0 * gx + b1 * gy + b2 * gz;
               diag[ia] += cq * phi * phi + kq2 * (gx * gx + gy * gy + gz * gz) + A.re_sign * bg * bg;
             } else {
@@ -1397,6 +1482,7 @@ static int launch_list_op(int needT, bool vel, const MeshInfo &m, const CellArgs
                           const Number *src, Number *dst, hipStream_t st, int mode)
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
+    if (needT) return launch_list_k<Number, K, OP, 1, true>(m, a, src, dst, st, mode);
     return launch_list_k<Number, K, OP, false, true>(m, a, src, dst, st, mode);
   } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
     if (needT) return launch_list_k<Number, K, OP, true, false>(m, a, src, dst, st, mode);
@@ -1488,6 +1574,7 @@ static int launch_op(int needT, bool vel, const MeshInfo &m, const CellArgs<Numb
                      const Number *src, Number *dst, hipStream_t st)
 {
   if constexpr (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG) {
+    if (needT) return launch_k<Number, K, OP, 1, true>(m, a, src, dst, st);
     return launch_k<Number, K, OP, false, true>(m, a, src, dst, st);
   } else if constexpr (OP == LS_TANGENT || OP == LS_DIAG) {
     if (needT) return launch_k<Number, K, OP, true, false>(m, a, src, dst, st);

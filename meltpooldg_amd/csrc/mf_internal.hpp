@@ -78,9 +78,11 @@ struct CellArgs {
   // 1 / (liquidus - solidus)
   Number mu_k[2], mu_rho[2], mu_cp[2], mu_Tliq, mu_inv;
   const Number *T_old;     // old temperature (mushy law, t_dependent)
-  // advection
+  // advection; SUPG: the coefficient and the cell size diameter / sqrt(3) of a level-0 cell
   Number theta, diffusivity, mass_factor;
   Number dt_inv;
+  Number supg_coef, supg_cell_size;
+  Number DDS[KMAX * KMAX];  // DDS[q][j] = l_j''(x_q) (SUPG diagonal)
   // level-set projections: eta of a level-0 cell (scaled by h^2 on refined cells) and the
   // narrow-band threshold on |T| (T holds the level set)
   Number ls_eta, ls_threshold;

@@ -27,6 +27,8 @@ class Case:
     theta: float = 0.5
     diffusivity: float = 0.05
     mass_factor: float = 1.0
+    supg: bool = False
+    supg_coefficient: float = -1.0
     seed: int = 11
 
     def mesh(self):
@@ -91,7 +93,8 @@ def oracle_operator(case: Case, f, use_mask=1, boundary=None):
                                   k_cell=f["k_cell"], use_mask=use_mask,
                                   boundary=oracle_boundary(boundary) if boundary else None)
     ap = orc.AdvecParams(dt=case.dt, theta=case.theta, diffusivity=case.diffusivity,
-                         mass_factor=case.mass_factor, vel_scale=0.0)
+                         mass_factor=case.mass_factor, vel_scale=0.0, supg=int(case.supg),
+                         supg_coefficient=case.supg_coefficient)
     return orc.OracleOperator(orc.OP_ADVEC, m, advec=ap, vel_kind=orc.VEL_NODAL, mask=f["mask"],
                               vel=f["vel"], use_mask=use_mask)
 
@@ -110,7 +113,8 @@ def device_operator(case: Case, f, dev="cuda"):
                                     dt=case.dt, mushy=MUSHY if case.law == "mushy" else None)
     else:
         op = mf.AdvectionDiffusionOperator(s, theta=case.theta, diffusivity=case.diffusivity,
-                                           mass_factor=case.mass_factor, dt=case.dt)
+                                           mass_factor=case.mass_factor, dt=case.dt,
+                                           supg=case.supg, supg_coefficient=case.supg_coefficient)
 
     def t(a):
         return None if a is None else torch.tensor(a, dtype=torch.float64, device=dev)

@@ -39,12 +39,19 @@ CASES = [
     Case(degree=3, kind="heat", law="mushy", t_dependent=True, vel=True, k_cell=True, dt=1e-2,
          mask="random"),
     Case(degree=1, kind="heat", law="mushy", t_dependent=False, vel=True, dt=1e-2),
+    # SUPG: per-cell tau from max_q |w| (coefficient -1) or a fixed coefficient, the Laplacian of
+    # u in the tangent and the rhs, the basis Laplacians in the diagonal
+    Case(degree=1, kind="advec", supg=True),
+    Case(degree=2, kind="advec", supg=True, mask="random", supg_coefficient=0.3),
+    Case(degree=3, kind="advec", supg=True, theta=1.0, mass_factor=1.5, diffusivity=0.2),
+    Case(degree=2, kind="advec", supg=True, ncell=(35, 3, 2), hi=(3.5, 0.36, 0.18)),
 ]
 
 
 def ids(c):
     return f"{c.kind}-Q{c.degree}-{c.law}{'-td' if c.t_dependent else ''}" \
-           f"{'-vel' if c.vel else ''}{'-kc' if c.k_cell else ''}-{c.mask}"
+           f"{'-vel' if c.vel else ''}{'-kc' if c.k_cell else ''}{'-supg' if c.supg else ''}" \
+           f"-{c.mask}"
 
 
 def _dev(a):
@@ -172,3 +179,11 @@ def test_inflow_outflow_rows(degree):
     torch.cuda.synchronize()
     plain = oracle_operator(case, f).vmult(src)
     assert rel_err(dst.cpu().numpy(), plain) < TOL64
+
+
+def test_supg_rejects_the_transposed_operator():
+    import meltpooldg_amd as mf
+    case = Case(degree=1, kind="advec", supg=True)
+    s, op = device_operator(case, fields(case))
+    with pytest.raises(mf.MFError):
+        op.set_transpose(True)
