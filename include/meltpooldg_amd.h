@@ -355,7 +355,7 @@ enum { MF_PRECISION_FP64 = 0, MF_PRECISION_FP32_REFINED = 1 };
 
 typedef struct {
   int solver;         /* MF_SOLVER_CG or MF_SOLVER_GMRES (right preconditioned) */
-  int restart;        /* GMRES basis size (0 = 30) */
+  int restart;        /* GMRES basis size, 1..30 (0 = 30; larger values: MF_ERR_ARG) */
   int max_iterations; /* 0 = 10000 */
   double abs_tol;
   double rel_tol;

@@ -88,8 +88,7 @@ __global__ void run_heads_kernel(const uint64_t *__restrict__ hs, const uint32_t
 
 __global__ void assign_kernel(const NodeArgs a, const int32_t *__restrict__ cells,
                               const uint32_t *__restrict__ ss, const int32_t *__restrict__ grp,
-                              const int32_t *__restrict__ nid, const uint64_t *__restrict__ keys,
-                              uint32_t *__restrict__ cell_nodes, uint64_t *__restrict__ node_keys,
+                              const int32_t *__restrict__ nid, uint32_t *__restrict__ cell_nodes,
                               double *__restrict__ xyz, uint8_t *__restrict__ boundary, int64_t n)
 {
   const int nloc = a.K * a.K * a.K;
@@ -110,10 +109,6 @@ __global__ void assign_kernel(const NodeArgs a, const int32_t *__restrict__ cell
       bnd |= (ai[d] == 0 && gg == 0) || (ai[d] == a.p && gg == (int64_t(a.ncell[d]) << cl[0]) * a.p);
     }
     if (bnd) boundary[id] = 1;
-    if (i == g) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) node_keys[3 * size_t(id) + d] = keys[3 * size_t(s) + d];
-    }
     // the last slot of the run (largest slot: the host's last writer) sets the support point
     const bool tail = (i + 1 == n) || grp[i + 1] != g;
     if (tail) {
@@ -219,10 +214,8 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
     return MF_ERR_STATE;  // the caller falls back to the host path
   }
   nn = total;
-  uint64_t *node_keys;
   double *xyz;
   uint8_t *bnd;
-  MF_HIP_TRY(b.alloc(&node_keys, size_t(3 * nn)));
   // the support points stay on the device as the scratch's copy (released on every failure)
   struct Kept {
     void *p = nullptr;
@@ -235,11 +228,11 @@ int device_node_identification(const mf_scratch *s, const int32_t *cells, int64_
   xyz = static_cast<double *>(kept.p);
   MF_HIP_TRY(b.alloc(&bnd, size_t(nn)));
   MF_HIP_TRY(hipMemsetAsync(bnd, 0, size_t(nn), st));
-  hipLaunchKernelGGL(assign_kernel, grid, block, 0, st, a, d_cells, sval_s, grp, nid, keys, cnodes,
-                     node_keys, xyz, bnd, n_slots);
+  hipLaunchKernelGGL(assign_kernel, grid, block, 0, st, a, d_cells, sval_s, grp, nid, cnodes, xyz,
+                     bnd, n_slots);
   MF_HIP_TRY(hipGetLastError());
-  // the node keys stay on the device: the constraint pass there recomputes them per slot, and
-  // the host fallback rebuilds them from the cells (nkeys left empty)
+  // node keys are not kept: the device constraint pass recomputes them per slot, and the host
+  // fallback rebuilds them from the cells (nkeys left empty)
   A->cell_nodes.resize(size_t(n_slots));
   nkeys.clear();
   A->xyz.resize(size_t(3 * nn));

@@ -1020,7 +1020,7 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
 {
   const int64_t n = n_rows(op);
   hipStream_t st = op->s->stream;
-  const int mm = std::min(std::max(restart, 1), 30);
+  const int mm = restart;  // 1..30, checked with the solver description
   Carve cv;
   cv.plan<T>(size_t(mm + 1) * n);
   cv.plan<T>(n);
@@ -1249,12 +1249,35 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
         return rc;
       inner_total += o.iterations;
       last_inner = o.iterations;
+      // the first pass of the device-scaled CG was queued before |r0| was known: it changes
+      // nothing when |r0| already meets the tolerance (|r0| <= abs_tol, or rel_tol >= 1) or is
+      // not finite, and the report then shows the state before it (ReductionControl stops after
+      // 0 steps, or fails on a NaN residual)
+      const double skip = (dev_scale && !known)
+                              ? (desc->rel_tol >= 1.0 ? HUGE_VAL : desc->abs_tol)
+                              : -HUGE_VAL;
+      auto undo_first_pass = [&](double r0) {
+        res0 = res = r0;
+        tol = std::max(desc->abs_tol, desc->rel_tol * res0);
+        outer = 0;
+        inner_total = 0;
+        status = std::isfinite(r0) ? MF_OK : MF_NO_CONVERGENCE;
+      };
       if (o.status == MF_BREAKDOWN) {
+        if (!known) {
+          double r0 = 0.0;
+          if ((rc = read_to_host(&r0, nrm.p, sizeof(double), st))) return rc;
+          if (!(r0 > skip) || !std::isfinite(r0)) {
+            undo_first_pass(r0);
+            break;
+          }
+        }
         status = MF_BREAKDOWN;
         break;
       }
-      // x += |r| e
-      if ((rc = vec_add_scaled_f32(x, e32.p, res, dev_scale ? nrm.p + slot : nullptr, n, st)))
+      // x += |r| e (skipped on the device in the case above)
+      if ((rc = vec_add_scaled_f32(x, e32.p, res, dev_scale ? nrm.p + slot : nullptr, skip, n,
+                                   st)))
         return rc;
       if ((rc = residual(false, nrm.p + (slot ^ 1)))) return rc;
       if (!known) {
@@ -1264,6 +1287,10 @@ static int solve_with(mf_operator *op, const mf_solver_desc *desc, const double 
         res = h2[1];
         tol = std::max(desc->abs_tol, desc->rel_tol * res0);
         known = true;
+        if (!(res0 > skip) || !std::isfinite(res0)) {
+          undo_first_pass(res0);  // x unchanged
+          break;
+        }
       } else if ((rc = read_to_host(&res, nrm.p + (slot ^ 1), sizeof(double), st))) {
         return rc;
       }
@@ -1292,6 +1319,8 @@ static int check_solve_args(mf_operator *op, const mf_solver_desc *desc, double 
   MF_REQUIRE(desc && x && b, MF_ERR_ARG, "null argument");
   MF_REQUIRE(desc->solver == MF_SOLVER_CG || desc->solver == MF_SOLVER_GMRES, MF_ERR_ARG,
              "unknown solver type");
+  MF_REQUIRE(desc->restart >= 0 && desc->restart <= 30, MF_ERR_ARG,
+             "GMRES restart (basis size) must lie in 1..30 (0 = 30)");
   return MF_OK;
 }
 

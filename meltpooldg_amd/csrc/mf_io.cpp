@@ -10,6 +10,7 @@
 //     the reference saves the p4est triangulation and the vectors through SolutionTransfer; here
 //     one binary file holds the mesh description (box, degree, cells of an adaptive mesh, the
 //     Dirichlet mask) and the vectors, and loading rebuilds the same DoF numbering from it.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -193,6 +194,22 @@ int mf_restart_save(const mf_scratch *s, const char *path, int n_vectors,
   return MF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// bytes left in the file after the current position (-1 when unknown)
+int64_t bytes_left(std::FILE *f)
+{
+  const long pos = std::ftell(f);
+  if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+  const long end = std::ftell(f);
+  if (end < 0 || std::fseek(f, pos, SEEK_SET) != 0) return -1;
+  return int64_t(end) - int64_t(pos);
+}
+}  // namespace
+
+extern "C" {
+
 int mf_restart_load_scratch(const char *path, mf_scratch **out)
 {
   MF_REQUIRE(path && out, MF_ERR_ARG, "null argument");
@@ -216,6 +233,9 @@ int mf_restart_load_scratch(const char *path, mf_scratch **out)
   get(hi, sizeof(hi));
   get(&ncells, 8);
   MF_REQUIRE(ok && ncells >= 1 && ncells < (int64_t(1) << 31), MF_ERR_ARG, "corrupt restart file");
+  // the declared counts must fit in what the file holds before anything is sized from them
+  MF_REQUIRE(!adaptive || bytes_left(fp.get()) >= int64_t(sizeof(int32_t)) * 4 * ncells,
+             MF_ERR_ARG, "corrupt restart file (shorter than its cell list)");
   mf_scratch *s = nullptr;
   int rc;
   if (adaptive) {
@@ -246,6 +266,11 @@ int mf_restart_load_scratch(const char *path, mf_scratch **out)
     return MF_ERR_ARG;
   }
   if (has_mask) {
+    if (bytes_left(fp.get()) < ndofs) {
+      mf_scratch_destroy(s);
+      set_error("corrupt restart file (shorter than its Dirichlet mask)");
+      return MF_ERR_ARG;
+    }
     std::vector<uint8_t> mask(static_cast<size_t>(ndofs));
     get(mask.data(), mask.size());
     if (!ok || (rc = mf_scratch_set_dirichlet_mask(s, mask.data()))) {
@@ -289,7 +314,19 @@ int mf_restart_load_vectors(const mf_scratch *s, const char *path, int n_vectors
                  ncells == m.n_cells && nc[0] == m.ncell[0] && nc[1] == m.ncell[1] &&
                  nc[2] == m.ncell[2],
              MF_ERR_ARG, "restart file belongs to another mesh");
-  if (adaptive) skip(int64_t(sizeof(int32_t)) * 4 * ncells);
+  for (int d = 0; d < 3; ++d)
+    MF_REQUIRE(lo[d] == m.lo[d] && hi[d] == m.hi[d], MF_ERR_ARG,
+               "restart file belongs to another mesh (box differs)");
+  if (adaptive) {
+    // the same counts on another band mesh would load in another numbering: compare the cells
+    MF_REQUIRE(bytes_left(fp.get()) >= int64_t(sizeof(int32_t)) * 4 * ncells, MF_ERR_ARG,
+               "corrupt restart file (shorter than its cell list)");
+    std::vector<int32_t> cells(static_cast<size_t>(4 * ncells));
+    get(cells.data(), sizeof(int32_t) * cells.size());
+    MF_REQUIRE(ok && cells.size() == s->ad->cells.size() &&
+                   std::equal(cells.begin(), cells.end(), s->ad->cells.begin()),
+               MF_ERR_ARG, "restart file belongs to another mesh (cells differ)");
+  }
   int32_t has_mask = 0, nv = 0;
   get(&ndofs, 8);
   get(&has_mask, 4);
@@ -304,6 +341,8 @@ int mf_restart_load_vectors(const mf_scratch *s, const char *path, int n_vectors
     get(&len, 8);
     MF_REQUIRE(ok && len == lengths[v] && vectors[v], MF_ERR_ARG,
                "vector " + std::to_string(v) + " has length " + std::to_string(len));
+    MF_REQUIRE(len >= 0 && bytes_left(fp.get()) >= int64_t(sizeof(double)) * len, MF_ERR_ARG,
+               "corrupt restart file (shorter than its vectors)");
     std::vector<double> h(static_cast<size_t>(len));
     get(h.data(), sizeof(double) * h.size());
     MF_REQUIRE(ok, MF_ERR_ARG, "corrupt restart file");

@@ -9,6 +9,8 @@
 // (RED_BLOCKS blocks, grid-stride), and consumers re-reduce the partials in a fixed order.
 // FP64 partials are plain sums; FP32 partials carry a Neumaier compensation term, combined with
 // two-sum at every level (wave, block, partials), so FP32 dot products keep ~FP64-like accuracy.
+#include <utility>
+
 #include "mf_vec.hpp"
 
 namespace mf {
@@ -198,9 +200,11 @@ __global__ void cast_kernel(const Ta *a, Tb *b, double scale, int64_t n)
 
 // x += scale * e  (FP64 solution update of the defect correction)
 __global__ void add_scaled_f32_kernel(double *x, const float *e, double scale,
-                                      const double *scale_dev, int64_t n)
+                                      const double *scale_dev, double skip_at_or_below, int64_t n)
 {
   if (scale_dev) scale = *scale_dev;
+  // a scale at or below the threshold, or not finite, leaves x as it is
+  if (!(scale > skip_at_or_below) || !isfinite(scale)) return;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const int64_t i0 = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   int64_t tail = 0;
@@ -576,19 +580,37 @@ __global__ void cg_direction_kernel(T *__restrict__ x, T *__restrict__ p, const 
 // ------------------------------------------------------------------------------------------
 // GMRES helpers (classical Gram-Schmidt, two passes)
 // ------------------------------------------------------------------------------------------
-// partials of h_i = V_i . w for i < m (V row-major [m][n]); part layout [m][2*nparts]
-template <typename T>
-__global__ void multidot_kernel(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part)
+// partials of h_i = V_i . w for i < M (V row-major [M][n]); part layout [M][2*nparts].  M is a
+// template parameter so that the M accumulators stay in registers.
+template <typename T, int M>
+__global__ void multidot_kernel(const T *V, int64_t ld, const T *w, int64_t n, T *part)
 {
-  constexpr int MAXM = 32;
-  Acc<T> acc[MAXM];
+  Acc<T> acc[M];
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x) {
     const T wi = w[i];
-#pragma unroll 4
-    for (int k = 0; k < m; ++k) acc[k].add(V[k * ld + i] * wi);
+#pragma unroll
+    for (int k = 0; k < M; ++k) acc[k].add(V[k * ld + i] * wi);
   }
-  for (int k = 0; k < m; ++k) write_partial(part + size_t(k) * 2 * gridDim.x, gridDim.x, acc[k]);
+#pragma unroll
+  for (int k = 0; k < M; ++k) write_partial(part + size_t(k) * 2 * gridDim.x, gridDim.x, acc[k]);
+}
+
+constexpr int GMRES_MAX_BASIS = 30;
+
+template <typename T, int... Ms>
+static int launch_multidot(int m, const T *V, int64_t ld, const T *w, int64_t n, T *part,
+                           hipStream_t st, std::integer_sequence<int, Ms...>)
+{
+  using Fn = void (*)(const T *, int64_t, const T *, int64_t, T *);
+  static constexpr Fn table[] = {multidot_kernel<T, Ms + 1>...};
+  if (m < 1 || m > int(sizeof...(Ms))) {
+    set_error("GMRES basis beyond the multi-dot kernels (at most 30 vectors)");
+    return MF_ERR_ARG;
+  }
+  hipLaunchKernelGGL(table[m - 1], dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, V, ld, w, n, part);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
 }
 
 // reduce [m][2*nparts] partials into h[m] (1 block)
@@ -708,11 +730,11 @@ template int vec_cast<double, float>(const double *, float *, double, int64_t, h
 template int vec_cast<float, double>(const float *, double *, double, int64_t, hipStream_t);
 template int vec_cast<double, double>(const double *, double *, double, int64_t, hipStream_t);
 
-int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev, int64_t n,
-                       hipStream_t st)
+int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev,
+                       double skip_at_or_below, int64_t n, hipStream_t st)
 {
   hipLaunchKernelGGL(add_scaled_f32_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, x, e, scale,
-                     scale_dev, n);
+                     scale_dev, skip_at_or_below, n);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
@@ -913,9 +935,9 @@ template <typename T>
 int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h,
                    hipStream_t st)
 {
-  hipLaunchKernelGGL(multidot_kernel<T>, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, V, ld, m, w,
-                     n, part);
-  MF_HIP_TRY(hipGetLastError());
+  if (int rc = launch_multidot<T>(m, V, ld, w, n, part, st,
+                                  std::make_integer_sequence<int, GMRES_MAX_BASIS>{}))
+    return rc;
   hipLaunchKernelGGL(reduce_multi_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part, RED_BLOCKS,
                      m, h);
   MF_HIP_TRY(hipGetLastError());

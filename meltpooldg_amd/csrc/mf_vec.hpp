@@ -33,9 +33,10 @@ int vec_masked_assign(double *dst, const uint8_t *mask, const double *vals, bool
 // dst[i] = value where mask[i]   (mf_ops.hip)
 int vec_masked_fill(double *dst, const uint8_t *mask, double value, int64_t n, hipStream_t st);
 template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
-// x += scale * e, the scale read from scale_dev when that is set
-int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev, int64_t n,
-                       hipStream_t st);
+// x += scale * e, the scale read from scale_dev when that is set; nothing is added when the
+// scale is at or below skip_at_or_below or not finite
+int vec_add_scaled_f32(double *x, const float *e, double scale, const double *scale_dev,
+                       double skip_at_or_below, int64_t n, hipStream_t st);
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st);
 template <typename T> int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out, hipStream_t st);
 template <typename T> int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st);

@@ -129,3 +129,48 @@ def test_restart_adaptive(tmp_path):
     assert np.array_equal(s2.support_points(), s.support_points())
     assert torch.equal(x2, x)
     assert torch.equal(_apply(mf, s2, x2), _apply(mf, s, x))
+
+
+def _load_vectors_into(mf, s, path, vectors):
+    arr_v = (mf.C.c_void_p * len(vectors))(*[s._v(v, v.numel()) for v in vectors])
+    arr_l = (mf.C.c_int64 * len(vectors))(*[v.numel() for v in vectors])
+    return mf.lib().mf_restart_load_vectors(s._h, str(path).encode(), len(vectors), arr_v, arr_l)
+
+
+def test_restart_rejects_other_meshes_and_short_files(tmp_path):
+    """Vectors load only into the mesh they were saved on: a shifted box with the same counts,
+    or a band mesh with the same cell and DoF counts but other cells, is refused; a truncated
+    file is refused before anything is sized from its counts."""
+    import meltpooldg_amd as mf
+    st = torch.cuda.current_stream()
+    s = mf.ScratchData(2, (4, 3, 2), lo=(0.1, 0.0, -0.2), hi=(1.0, 0.9, 0.4), stream=st)
+    a = torch.tensor(np.random.default_rng(5).uniform(-1, 1, s.n_dofs), dtype=torch.float64,
+                     device="cuda")
+    path = tmp_path / "s.mfr"
+    mf.restart_save(s, path, [a])
+    same = mf.ScratchData(2, (4, 3, 2), lo=(0.1, 0.0, -0.2), hi=(1.0, 0.9, 0.4), stream=st)
+    a2 = torch.zeros_like(a)
+    assert _load_vectors_into(mf, same, path, [a2]) == mf.MF_OK and torch.equal(a2, a)
+    shifted = mf.ScratchData(2, (4, 3, 2), lo=(0.2, 0.0, -0.2), hi=(1.1, 0.9, 0.4), stream=st)
+    assert _load_vectors_into(mf, shifted, path, [a2]) == mf.MF_ERR_ARG
+    # band meshes mirrored in x: equal counts, other cells
+    c1 = mf.refine_band_cells(2, 2, (0.3, 0.4, 0.35), 0.25, 0.04)
+    c2 = mf.refine_band_cells(2, 2, (0.7, 0.4, 0.35), 0.25, 0.04)
+    b1 = mf.AdaptiveScratchData(2, 2, c1, stream=st)
+    b2 = mf.AdaptiveScratchData(2, 2, c2, stream=st)
+    assert (b1.n_cells, b1.n_dofs) == (b2.n_cells, b2.n_dofs)
+    assert not np.array_equal(np.asarray(c1), np.asarray(c2))
+    x = torch.tensor(np.random.default_rng(6).uniform(-1, 1, b1.n_dofs), dtype=torch.float64,
+                     device="cuda")
+    bpath = tmp_path / "b.mfr"
+    mf.restart_save(b1, bpath, [x])
+    x2 = torch.zeros_like(x)
+    assert _load_vectors_into(mf, b2, bpath, [x2]) == mf.MF_ERR_ARG
+    assert _load_vectors_into(mf, b1, bpath, [x2]) == mf.MF_OK and torch.equal(x2, x)
+    # truncated files: inside the cell list, and inside the vector
+    raw = bpath.read_bytes()
+    for cut in (200, len(raw) - 16):
+        short = tmp_path / f"short{cut}.mfr"
+        short.write_bytes(raw[:cut])
+        with pytest.raises(mf.MFError):
+            mf.restart_load(short, [x2], stream=st)

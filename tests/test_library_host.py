@@ -3,6 +3,7 @@ declares, and its host logic (mesh setup, 1D tables, argument validation, synthe
 agrees with the oracle.  No GPU compute is called here."""
 import ctypes as C
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -249,3 +250,16 @@ def test_vector_arguments_are_checked_before_any_call():
         op.set_fields(k_cell=_FakeDeviceTensor(27))
     with pytest.raises(ValueError, match="entries"):
         mf.LinearSolver.solve(op, _FakeDeviceTensor(27), _FakeDeviceTensor(28))
+
+
+def test_cpp_driver_is_built_against_the_library():
+    """The compiled C++ caller (tests/cpp/hpp_driver, built by build()) resolves
+    libmeltpooldg_amd.so from the tree."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    drv = os.path.join(root, "tests", "cpp", "hpp_driver")
+    assert os.path.exists(drv), "run __graft_entry__.build() first"
+    r = subprocess.run(["ldd", drv], capture_output=True, text=True, timeout=60)
+    line = [ln for ln in r.stdout.splitlines() if "libmeltpooldg_amd.so" in ln]
+    assert line and "not found" not in line[0]
+    assert os.path.realpath(line[0].split("=>")[1].split("(")[0].strip()) == \
+        os.path.realpath(os.path.join(root, "meltpooldg_amd", "libmeltpooldg_amd.so"))
