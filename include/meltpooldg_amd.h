@@ -488,9 +488,20 @@ int mf_compute_norm_l2(mf_scratch *s, const double *u, double *result);
 int mf_laser_heat_source_radius_derivative(mf_scratch *s, double power, double radius,
                                            const double position[3], double *dq_dr);
 
-/* PredictorType (include/meltpooldg/linear_algebra/predictor_data.hpp:10-19; least-squares
- * projection not built). */
-enum { MF_PREDICTOR_NONE = 0, MF_PREDICTOR_ZERO = 1, MF_PREDICTOR_LINEAR_EXTRAPOLATION = 2 };
+/* LeastSquaresProjection::vmult (include/meltpooldg/linear_algebra/predictor.hpp:17-81): the
+ * combination sum_s c_s x_s of the old solutions that minimises |A sum_s c_s x_s - rhs|, with
+ * A = the operator's vmult (constraints as in mf_operator_vmult).  The images A x_s are
+ * orthogonalised by modified Gram-Schmidt (Vector::add_and_dot), stopping at the first s whose
+ * squared residual norm falls below 1e-16 times |A x_0|^2 (that x_s and later ones are not
+ * used); the triangular system is solved by back substitution.  old_solutions: n_old (1..8)
+ * device vectors of mf_operator_n_rows entries; n_used (may be NULL) returns how many entered
+ * the combination. */
+int mf_predictor_least_squares(mf_operator *op, int n_old, const double *const *old_solutions,
+                               const double *rhs, double *solution, int *n_used);
+
+/* PredictorType (include/meltpooldg/linear_algebra/predictor_data.hpp:10-19). */
+enum { MF_PREDICTOR_NONE = 0, MF_PREDICTOR_ZERO = 1, MF_PREDICTOR_LINEAR_EXTRAPOLATION = 2,
+       MF_PREDICTOR_LEAST_SQUARES_PROJECTION = 3 };
 
 /* One heat time step of the mp-heat-transfer application on the device: the loop body of
  * HeatTransferApplication::run (applications/mp-heat-transfer/heat_transfer_application.cpp:

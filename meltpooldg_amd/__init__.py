@@ -111,6 +111,7 @@ class NewtonCallbacks(C.Structure):
 
 
 PREDICTOR_NONE, PREDICTOR_ZERO, PREDICTOR_LINEAR_EXTRAPOLATION = 0, 1, 2
+PREDICTOR_LEAST_SQUARES_PROJECTION = 3
 
 
 class HeatStepperDesc(C.Structure):
@@ -269,6 +270,8 @@ def lib():
         L.mf_advection_step.argtypes = [vp, vp, vp, vp, C.POINTER(SolverDesc),
                                         C.POINTER(SolveReport)]
         L.mf_predictor_linear_extrapolation.argtypes = [vp, vp, vp, vp, C.c_double]
+        L.mf_predictor_least_squares.argtypes = [vp, C.c_int, C.POINTER(vp), vp, vp,
+                                                 C.POINTER(C.c_int)]
         L.mf_laser_heat_source.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double),
                                            vp]
         L.mf_compute_norm_l2.argtypes = [vp, vp, C.POINTER(C.c_double)]
@@ -793,6 +796,18 @@ def laser_heat_source(scratch, q_s, power, radius, position):
     """Nodal Gaussian volumetric laser source (LaserHeatSourceVolumetric)."""
     pos = (C.c_double * 3)(*position)
     _check(lib().mf_laser_heat_source(scratch._h, power, radius, pos, scratch._v(q_s)))
+
+
+def predictor_least_squares(operator, old_solutions, rhs, solution):
+    """LeastSquaresProjection::vmult (include/meltpooldg/linear_algebra/predictor.hpp:17-81):
+    solution = sum_s c_s x_s minimising |A sum_s c_s x_s - rhs| over the old solutions x_s
+    (device vectors of the operator's rows); returns how many of them entered."""
+    ptrs = (C.c_void_p * len(old_solutions))(*[operator._r(x) for x in old_solutions])
+    used = C.c_int(0)
+    _check(lib().mf_predictor_least_squares(operator._h, len(old_solutions), ptrs,
+                                            operator._r(rhs), operator._r(solution),
+                                            C.byref(used)))
+    return used.value
 
 
 def predictor_linear_extrapolation(scratch, dst, x_n, x_nm1, tau=1.0):

@@ -861,6 +861,68 @@ int mf_operator_compute_inverse_diagonal(mf_operator *op, double *inv_diag)
   return compute_inverse_diagonal(op, inv_diag, nullptr);
 }
 
+int mf_predictor_least_squares(mf_operator *op, int n_old, const double *const *old_solutions,
+                               const double *rhs, double *solution, int *n_used)
+{
+  int rc = check_ready(op);
+  if (rc) return rc;
+  MF_REQUIRE(old_solutions && rhs && solution, MF_ERR_ARG, "null argument");
+  MF_REQUIRE(n_old >= 1 && n_old <= LinComb::MAX, MF_ERR_ARG,
+             "least-squares projection over 1..8 old solutions");
+  for (int s = 0; s < n_old; ++s) MF_REQUIRE(old_solutions[s], MF_ERR_ARG, "null old solution");
+  const int64_t n = n_rows(op);
+  hipStream_t st = op->s->stream;
+  // internal_vectors (A x_s, orthogonalised in place), the dot partials and their result
+  Carve cv;
+  for (int s = 0; s < n_old; ++s) cv.plan<double>(n);
+  cv.plan<double>(2 * RED_BLOCKS);
+  cv.plan<double>(1);
+  if ((rc = cv.reserve(op->arena[1]))) return rc;
+  std::vector<double *> v(n_old);
+  for (int s = 0; s < n_old; ++s) v[s] = cv.take<double>(n);
+  double *part = cv.take<double>(2 * RED_BLOCKS), *dres = cv.take<double>(1);
+  auto dot = [&](const double *a, const double *b, double &out) -> int {
+    int c = vec_dot<double>(a, b, n, part, dres, st);
+    return c ? c : read_to_host(&out, dres, sizeof(double), st);
+  };
+  // projection_matrix P (n_old x n_old, row-major), modified Gram-Schmidt with add_and_dot
+  std::vector<double> P(size_t(n_old) * n_old, 0.0);
+  auto Pm = [&](int i, int j) -> double & { return P[size_t(i) * n_old + j]; };
+  int step = 0;
+  for (; step < n_old; ++step) {
+    if ((rc = apply_tangent<double>(op, v[step], old_solutions[step], nullptr, nullptr, true)))
+      return rc;
+    if ((rc = dot(v[step], v[0], Pm(0, step)))) return rc;
+    for (int j = 0; j < step; ++j) {
+      if ((rc = vec_axpy_dot(v[step], -Pm(j, step) / Pm(j, j), v[j], v[j + 1], n, part, dres, st)))
+        return rc;
+      if ((rc = read_to_host(&Pm(j + 1, step), dres, sizeof(double), st))) return rc;
+    }
+    if (Pm(step, step) < 1e-16 * Pm(0, 0)) break;
+  }
+  MF_REQUIRE(step >= 1, MF_ERR_STATE,
+             "least-squares projection: the first old solution has a zero operator image");
+  // the least-squares system by back substitution
+  std::vector<double> c(step);
+  for (int s = 0; s < step; ++s)
+    if ((rc = dot(v[s], rhs, c[s]))) return rc;
+  for (int s = step - 1; s >= 0; --s) {
+    double sum = c[s];
+    for (int j = s + 1; j < step; ++j) sum -= c[j] * Pm(s, j);
+    c[s] = sum / Pm(s, s);
+  }
+  LinComb lc{};
+  lc.m = step;
+  for (int s = 0; s < step; ++s) {
+    lc.x[s] = old_solutions[s];
+    lc.c[s] = c[s];
+  }
+  if ((rc = vec_lincomb(solution, lc, n, st))) return rc;
+  if (n_used) *n_used = step;
+  MF_HIP_TRY(hipStreamSynchronize(st));
+  return MF_OK;
+}
+
 int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s)
 {
   int rc = check_ready(op);

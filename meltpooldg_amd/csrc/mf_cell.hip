@@ -729,9 +729,8 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
   constexpr bool SUPG = NEEDT && (OP == ADVEC_TANGENT || OP == ADVEC_RHS || OP == ADVEC_DIAG);
 
   // LDS cell tensors [K^3][CPB].  While values are interpolated: u in L0, T in L1, the velocity
-  // components and the rhs fields after it.  Then the T gradients use L2, L3 and are reduced to
-  // per-quadrature-point coefficients in registers, after which the u gradients take L1, L2.
-  // The diagonal needs five coefficient tensors.
+  // components and the rhs fields after it.  Then the x, y gradients of T take L2, L3 and those
+  // of u L0, L1 (with T) or L1, L2 (without).  The diagonal needs five coefficient tensors.
   constexpr int I_T = 1;
   constexpr int I_W = 1 + (USE_T ? 1 : 0);
   constexpr int I_A = I_W + (VEL ? 3 : 0);
@@ -980,50 +979,38 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
 
   const Number kc = (A.k_cell && ac.active) ? A.k_cell[ac.cell] : Number(1);
 
-  // ---------------- T gradients -> per-quadrature-point coefficients ----------------
-  // heat tangent / diagonal: kq = k(T) kc and dkg = dk/dT kc grad T;  PGRAD: dkg = grad T
-  Number kq[K], dkg[3][K];
-  // mushy law: rho cp(T) and the extra value coefficient d(rho cp)/dT ((T - T_old)/dt + grad T . w)
-  [[maybe_unused]] Number rcq[K], cxq[K];
-  if constexpr (TGRAD) {
+  // ---------------- x and y gradients of T and u at the quadrature points ----------------
+  // With T and u (FUSED): the x and y pencils of T (L1) and of u (L0) are read, and after a
+  // barrier the T gradients go to L2, L3 and the u gradients to L0 (x, over the u values, which
+  // stay in registers from E3) and L1 (y, over T, whose values also stay in registers): one
+  // barrier less than two separate sweep phases, and the per-point T coefficients are formed in
+  // the quadrature loop from these tensors instead of being held in registers across phases.
+  constexpr bool FUSED = TGRAD && HAS_U;
+  Number *const UGX = FUSED ? L0 : L1;  // tensors of the u gradients
+  Number *const UGY = FUSED ? L1 : L2;
+  if constexpr (FUSED) {
+    Number tx[K], ty[K], ux[K], uy[K], o[K];
+    load_pencil<Number, K, LY, 0>(lds + I_T * BUF, p, c, tx);
+    load_pencil<Number, K, LY, 1>(lds + I_T * BUF, p, c, ty);
+    load_pencil<Number, K, LY, 0>(L0, p, c, ux);
+    load_pencil<Number, K, LY, 1>(L0, p, c, uy);
+    __syncthreads();
+    matvec<Number, K, false>(Dq, tx, o);
+    store_pencil<Number, K, LY, 0>(L2, p, c, o);
+    matvec<Number, K, false>(Dq, ty, o);
+    store_pencil<Number, K, LY, 1>(lds + 3 * BUF, p, c, o);
+    matvec<Number, K, false>(Dq, ux, o);
+    store_pencil<Number, K, LY, 0>(L0, p, c, o);
+    matvec<Number, K, false>(Dq, uy, o);
+    store_pencil<Number, K, LY, 1>(L1, p, c, o);
+    __syncthreads();
+  } else if constexpr (TGRAD) {
+    // the diagonal: T gradients only
     sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, lds + I_T * BUF, L2, Dq, lds + I_T * BUF,
                                                   lds + 3 * BUF, p, c);
     __syncthreads();
-#pragma unroll
-    for (int z = 0; z < K; ++z) {
-      const int e = LY::idx(elem<K>(qi, qj, z), c);
-      const Number g0 = L2[e] * ih[0], g1 = lds[3 * BUF + e] * ih[1], g2 = gTz[z] * ih[2];
-      if constexpr (PGRAD) {
-        dkg[0][z] = g0;
-        dkg[1][z] = g1;
-        dkg[2][z] = g2;
-      } else if constexpr (MUSHY) {
-        Number k, dk, rc, drc;
-        mushy_law(A, Tq[z], k, dk, rc, drc);
-        kq[z] = k * kc;
-        dk *= kc;
-        dkg[0][z] = dk * g0;
-        dkg[1][z] = dk * g1;
-        dkg[2][z] = dk * g2;
-        rcq[z] = rc;
-        Number cx = drc * (Tq[z] - Toq[z]) * A.dt_inv;
-        if constexpr (VEL) cx += drc * (g0 * wq[0][z] + g1 * wq[1][z] + g2 * wq[2][z]);
-        cxq[z] = A.t_dependent ? cx : Number(0);
-      } else {
-        Number t, s2;
-        tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
-        kq[z] = (A.k0 + A.tanh_amp * t) * kc;
-        const Number dk = A.tanh_amp * A.inv_tanh_w * s2 * kc;
-        dkg[0][z] = dk * g0;
-        dkg[1][z] = dk * g1;
-        dkg[2][z] = dk * g2;
-      }
-    }
-    __syncthreads();
-  }
-
-  // ---------------- E4: d/dx and d/dy of u at the quadrature points ----------------
-  if constexpr (HAS_U) {
+  } else if constexpr (HAS_U) {
+    // ---------------- E4: d/dx and d/dy of u at the quadrature points ----------------
     sweep_pair<Number, K, LY, 0, false, 1, false>(Dq, L0, L1, Dq, L0, L2, p, c);
     __syncthreads();
   }
@@ -1063,9 +1050,41 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     const Number jxw = wij * A.w[z];
     Number g[3] = {0, 0, 0};
     if constexpr (HAS_U) {
-      g[0] = L1[e] * ih[0];
-      g[1] = L2[e] * ih[1];
+      g[0] = UGX[e] * ih[0];
+      g[1] = UGY[e] * ih[1];
       g[2] = guz[z] * ih[2];
+    }
+    // T coefficients of the point (read before this iteration overwrites L2, L3 at e):
+    // heat tangent / diagonal kq = k(T) kc and dkg = dk/dT kc grad T; PGRAD dkg = grad T; the
+    // mushy law's rho cp(T) (rcq) and d(rho cp)/dT ((T - T_old)/dt + grad T . w) (cxq)
+    [[maybe_unused]] Number kqz = 0, dkg[3] = {0, 0, 0}, rcq = 0, cxq = 0;
+    if constexpr (TGRAD) {
+      const Number g0 = L2[e] * ih[0], g1 = lds[3 * BUF + e] * ih[1], g2 = gTz[z] * ih[2];
+      if constexpr (PGRAD) {
+        dkg[0] = g0;
+        dkg[1] = g1;
+        dkg[2] = g2;
+      } else if constexpr (MUSHY) {
+        Number k, dk, rc, drc;
+        mushy_law(A, Tq[z], k, dk, rc, drc);
+        kqz = k * kc;
+        dk *= kc;
+        dkg[0] = dk * g0;
+        dkg[1] = dk * g1;
+        dkg[2] = dk * g2;
+        rcq = rc;
+        Number cx = drc * (Tq[z] - Toq[z]) * A.dt_inv;
+        if constexpr (VEL) cx += drc * (g0 * wq[0][z] + g1 * wq[1][z] + g2 * wq[2][z]);
+        cxq = A.t_dependent ? cx : Number(0);
+      } else {
+        Number t, s2;
+        tanh_sech2((Tq[z] - A.tanh_Tc) * A.inv_tanh_w, t, s2);
+        kqz = (A.k0 + A.tanh_amp * t) * kc;
+        const Number dk = A.tanh_amp * A.inv_tanh_w * s2 * kc;
+        dkg[0] = dk * g0;
+        dkg[1] = dk * g1;
+        dkg[2] = dk * g2;
+      }
     }
     Number w3[3] = {0, 0, 0};
     if constexpr (VEL) {
@@ -1076,18 +1095,18 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
     Number val = 0, f[3] = {0, 0, 0};
     Number cc = 0, kk = 0, bb[3] = {0, 0, 0};
     if constexpr (PGRAD) {
-      val = g[0] * dkg[0][z] + g[1] * dkg[1][z] + g[2] * dkg[2][z];  // JxW applied below
+      val = g[0] * dkg[0] + g[1] * dkg[1] + g[2] * dkg[2];  // JxW applied below
     } else if constexpr (OP == HEAT_TANGENT || OP == HEAT_DIAG) {
       Number k = A.k0 * kc;
-      if constexpr (USE_T) k = kq[z];
+      if constexpr (USE_T) k = kqz;
       const bool td = USE_T && A.t_dependent;
       Number rho_cp = A.rho_cp;
-      if constexpr (MUSHY) rho_cp = rcq[z];
+      if constexpr (MUSHY) rho_cp = rcq;
       if constexpr (OP == HEAT_TANGENT) {
         const Number u = uq[z];
         val = A.dt_inv * rho_cp * u;
         // mushy law: d(rho cp)/dT ((T - T_old)/dt + grad T . w) u v (symmetric in u, v)
-        if constexpr (MUSHY) val += cxq[z] * u;
+        if constexpr (MUSHY) val += cxq * u;
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[d] = k * g[d];
         if (!A.transpose) {
@@ -1096,7 +1115,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
           if constexpr (USE_T) {
             if (td) {
 #pragma unroll
-              for (int d = 0; d < 3; ++d) f[d] += dkg[d][z] * u;
+              for (int d = 0; d < 3; ++d) f[d] += dkg[d] * u;
             }
           }
         } else {
@@ -1106,12 +1125,12 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
             for (int d = 0; d < 3; ++d) f[d] += rho_cp * w3[d] * u;
           }
           if constexpr (USE_T) {
-            if (td) val += dkg[0][z] * g[0] + dkg[1][z] * g[1] + dkg[2][z] * g[2];
+            if (td) val += dkg[0] * g[0] + dkg[1] * g[1] + dkg[2] * g[2];
           }
         }
       } else {
         cc = A.dt_inv * rho_cp;
-        if constexpr (MUSHY) cc += cxq[z];
+        if constexpr (MUSHY) cc += cxq;
         kk = k;
         if constexpr (VEL) {
 #pragma unroll
@@ -1120,7 +1139,7 @@ __global__ void __launch_bounds__((cells_per_block<Number, K>() * K * K))
         if constexpr (USE_T) {
           if (td) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) bb[d] += dkg[d][z];
+            for (int d = 0; d < 3; ++d) bb[d] += dkg[d];
           }
         }
       }

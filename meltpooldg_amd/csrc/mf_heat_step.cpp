@@ -30,6 +30,7 @@ struct mf_heat_stepper {
   double *T_old = nullptr;  // recent old solution
   double *buf = nullptr;    // predictor buffer
   double *q = nullptr;      // heat source
+  double *rhs = nullptr;    // least-squares predictor: its right-hand side
   double time = 0.0;
   double position[3] = {0.0, 0.0, 0.0};
   int n_steps = 0;
@@ -43,6 +44,10 @@ int mf_predictor_linear_extrapolation(mf_scratch *s, double *dst, const double *
                                       const double *x_nm1, double tau);
 int mf_operator_reset_time_increment(mf_operator *op, double dt);
 int mf_distribute_constraints(mf_scratch *s, double *x);
+int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
+                           const double *k_cell);
+int mf_operator_set_temperature_old(mf_operator *op, const double *T_old);
+int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, const double *q_s);
 
 int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const double *T0,
                            mf_heat_stepper **out)
@@ -50,7 +55,8 @@ int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const
   MF_REQUIRE(op && d && T0 && out, MF_ERR_ARG, "null argument");
   MF_REQUIRE(op->kind == 0, MF_ERR_ARG, "the heat stepper needs a heat operator");
   MF_REQUIRE(d->predictor == MF_PREDICTOR_NONE || d->predictor == MF_PREDICTOR_ZERO ||
-                 d->predictor == MF_PREDICTOR_LINEAR_EXTRAPOLATION,
+                 d->predictor == MF_PREDICTOR_LINEAR_EXTRAPOLATION ||
+                 d->predictor == MF_PREDICTOR_LEAST_SQUARES_PROJECTION,
              MF_ERR_ARG, "unknown predictor type");
   MF_REQUIRE(!d->laser || d->radius > 0, MF_ERR_ARG, "the laser radius must be positive");
   const size_t bytes = sizeof(double) * size_t(op->s->mesh.n_dofs);
@@ -72,8 +78,10 @@ int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const
     }
     for (int k = 0; k < 3; ++k) h->d.scan_direction[k] = d->scan_direction[k] / nrm;
   }
-  double **bufs[4] = {&h->T, &h->T_old, &h->buf, &h->q};
-  for (double **b : bufs) {
+  double **bufs[5] = {&h->T, &h->T_old, &h->buf, &h->q, &h->rhs};
+  const int n_bufs = d->predictor == MF_PREDICTOR_LEAST_SQUARES_PROJECTION ? 5 : 4;
+  for (int i = 0; i < n_bufs; ++i) {
+    double **b = bufs[i];
     if (hipMalloc(reinterpret_cast<void **>(b), bytes) != hipSuccess) {
       (void)mf_heat_stepper_destroy(h);
       set_error("hipMalloc failed for the heat stepper's vectors");
@@ -105,7 +113,7 @@ int mf_heat_stepper_create(mf_operator *op, const mf_heat_stepper_desc *d, const
 int mf_heat_stepper_destroy(mf_heat_stepper *h)
 {
   if (!h) return MF_OK;
-  double *bufs[4] = {h->T, h->T_old, h->buf, h->q};
+  double *bufs[5] = {h->T, h->T_old, h->buf, h->q, h->rhs};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   delete h;
@@ -142,6 +150,27 @@ int mf_heat_stepper_step(mf_heat_stepper *h, double dt, const double *dirichlet_
       break;
     case MF_PREDICTOR_ZERO:
       MF_HIP_TRY(hipMemsetAsync(h->buf, 0, bytes, st));
+      break;
+    case MF_PREDICTOR_LEAST_SQUARES_PROJECTION:
+      // Predictor::vmult with n_old_solution_vectors = 2 (the stepper keeps two solutions):
+      // the projection once two predictor calls have passed, the current solution before.
+      // (The reference's fallback swaps the current solution with the zeroed predictor buffer
+      // before the history commit, which leaves a zero old solution; the stepper keeps the
+      // current solution as the old one, as for the other predictors.)
+      if (h->n_steps - 1 >= 2) {
+        // HeatDiffuseOperation::init_time_advance: rhs = create_rhs(current solution) with the
+        // operator linearised at the current solution and the new increment and heat source
+        if ((rc = mf_operator_set_fields(op, h->T, op->vel, op->k_cell))) return rc;
+        if (op->heat.law == MF_LAW_MUSHY && (rc = mf_operator_set_temperature_old(op, h->T_old)))
+          return rc;
+        MF_HIP_TRY(hipMemsetAsync(h->rhs, 0, bytes, st));
+        if ((rc = mf_operator_create_rhs(op, h->rhs, h->T, h->d.laser ? h->q : nullptr)))
+          return rc;
+        const double *olds[2] = {h->T, h->T_old};
+        if ((rc = mf_predictor_least_squares(op, 2, olds, h->rhs, h->buf, nullptr))) return rc;
+      } else {
+        MF_HIP_TRY(hipMemcpyAsync(h->buf, h->T, bytes, hipMemcpyDeviceToDevice, st));
+      }
       break;
     default:
       if (std::fabs(dt_old) < 1e-12) {

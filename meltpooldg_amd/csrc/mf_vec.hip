@@ -274,6 +274,31 @@ __global__ void dot_kernel(const T *__restrict__ a, const T *__restrict__ b, int
   write_partial(part, gridDim.x, acc);
 }
 
+// x += a v, then partials of x . w (LA::distributed::Vector::add_and_dot)
+__global__ void axpy_dot_kernel(double *__restrict__ x, double a, const double *__restrict__ v,
+                                const double *w, int64_t n, double *part)
+{
+  Acc<double> acc;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const double xi = x[i] + a * v[i];
+    x[i] = xi;
+    acc.add(xi * w[i]);
+  }
+  write_partial(part, gridDim.x, acc);
+}
+
+// dst = c_0 x_0, then dst += c_s x_s for s = 1..m-1 (Vector::equ followed by Vector::add)
+__global__ void lincomb_kernel(double *__restrict__ dst, LinComb lc, int64_t n)
+{
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    double v = lc.c[0] * lc.x[0][i];
+    for (int s = 1; s < lc.m; ++s) v += lc.c[s] * lc.x[s][i];
+    dst[i] = v;
+  }
+}
+
 // max |d| partials (no compensation needed)
 template <typename T>
 __global__ void absmax_kernel(const T *d, int64_t n, double *part)
@@ -863,6 +888,26 @@ int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st)
 }
 template int vec_norm<double>(const double *, int64_t, double *, double *, hipStream_t);
 template int vec_norm<float>(const float *, int64_t, float *, double *, hipStream_t);
+
+int vec_axpy_dot(double *x, double a, const double *v, const double *w, int64_t n, double *part,
+                 double *out, hipStream_t st)
+{
+  hipLaunchKernelGGL(axpy_dot_kernel, dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, x, a, v, w, n,
+                     part);
+  MF_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(sum_from_partials_kernel<double>, dim3(1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
+int vec_lincomb(double *dst, const LinComb &lc, int64_t n, hipStream_t st)
+{
+  MF_REQUIRE(lc.m >= 1 && lc.m <= LinComb::MAX, MF_ERR_ARG, "linear combination of 1..8 vectors");
+  hipLaunchKernelGGL(lincomb_kernel, grid_for(n), dim3(VEC_THREADS), 0, st, dst, lc, n);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
 
 template <typename T>
 int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st)

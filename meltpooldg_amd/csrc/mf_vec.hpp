@@ -40,6 +40,17 @@ int vec_add_scaled_f32(double *x, const float *e, double scale, const double *sc
 int vec_invert_diagonal(double *d, int64_t n, bool relative, double *work, hipStream_t st);
 template <typename T> int vec_residual(const T *b, const T *q, T *r, int64_t n, T *part, double *norm_out, hipStream_t st);
 template <typename T> int vec_dot(const T *a, const T *b, int64_t n, T *part, double *out, hipStream_t st);
+// x += a v and out = x . w (on the device; partials in part, 2 RED_BLOCKS entries)
+int vec_axpy_dot(double *x, double a, const double *v, const double *w, int64_t n, double *part,
+                 double *out, hipStream_t st);
+// dst = sum_s c_s x_s, s = 0..m-1, accumulated in order of s
+struct LinComb {
+  static constexpr int MAX = 8;
+  const double *x[MAX];
+  double c[MAX];
+  int m;
+};
+int vec_lincomb(double *dst, const LinComb &lc, int64_t n, hipStream_t st);
 template <typename T> int vec_norm(const T *a, int64_t n, T *part, double *norm_out, hipStream_t st);
 template <typename T> int vec_all_zero(const T *x, int64_t n, double *work, bool &zero, hipStream_t st);
 template <typename T> int vec_precond(T *z, const T *v, const T *inv, int64_t n, hipStream_t st);

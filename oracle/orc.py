@@ -470,6 +470,38 @@ def laser_volumetric_source(X, power, radius, position):
     return power / (radius * np.sqrt(np.pi / 2)) ** 3 * np.exp(-2.0 * d2 / radius ** 2)
 
 
+def least_squares_projection(vmult, old_solutions, rhs):
+    """LeastSquaresProjection::vmult (include/meltpooldg/linear_algebra/predictor.hpp:36-80):
+    v_s = A x_s orthogonalised in place by modified Gram-Schmidt written with add_and_dot
+    (v_s += -P[j,s]/P[j,j] v_j, then P[j+1,s] = v_s . v_{j+1}), stopping at the first s with
+    P[s,s] < 1e-16 P[0,0]; c_s = v_s . rhs, back substitution c_s = (c_s - sum_{j>s} c_j P[s,j])
+    / P[s,s]; solution = c_0 x_0 + c_1 x_1 + ...  vmult: the operator's product.  Returns the
+    solution and the number of old solutions used."""
+    n_old = len(old_solutions)
+    v = [None] * n_old
+    P = np.zeros((n_old, n_old))
+    step = 0
+    while step < n_old:
+        v[step] = np.array(vmult(old_solutions[step]), dtype=np.float64)
+        P[0, step] = v[step] @ v[0]
+        for j in range(step):
+            v[step] = v[step] + (-P[j, step] / P[j, j]) * v[j]
+            P[j + 1, step] = v[step] @ v[j + 1]
+        if P[step, step] < 1e-16 * P[0, 0]:
+            break
+        step += 1
+    c = [v[s] @ rhs for s in range(step)]
+    for s in range(step - 1, -1, -1):
+        acc = c[s]
+        for j in range(s + 1, step):
+            acc -= c[j] * P[s, j]
+        c[s] = acc / P[s, s]
+    sol = c[0] * np.asarray(old_solutions[0], dtype=np.float64)
+    for s in range(1, step):
+        sol = sol + c[s] * np.asarray(old_solutions[s], dtype=np.float64)
+    return sol, step
+
+
 def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
                     dirichlet_values=None, newton=(10, 1e-9, 1e-10), rel_tol=1e-12, amesh=None,
                     boundary=None, vel=None):
@@ -482,6 +514,11 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
         normalised scan direction * scan speed * dt, then the volumetric source there;
       - Predictor::vmult (include/meltpooldg/linear_algebra/predictor.hpp:119-161,
         source/linear_algebra/predictor_linear.cpp:76-96), then the history commit;
+        predictor "least_squares": the projection over (current, old) with the Jacobian at
+        the current solution and rhs = create_rhs(current solution) with the new increment
+        and heat source (heat_diffuse_operation.cpp:228-246), once two predictor calls have
+        passed, the current solution before (the reference's fallback leaves a zero old
+        solution through its buffer swap; here the current solution is kept as the old one);
       - NewtonRaphsonSolver::solve (source/linear_algebra/newton_raphson_solver.cpp:22-80) with
         the residual of HeatDiffuseOperation (heat_diffuse_operation.cpp:86-116): r = rhs(T),
         A d = r by Jacobi-GMRES (T-dependent law) or CG, stop when |r| and |d| are below the
@@ -508,18 +545,26 @@ def heat_time_steps(mesh, heat, mask, T0, dts, laser=None, predictor="linear",
             if laser.get("scan_speed", 0.0) != 0.0:
                 pos = pos + sd * laser["scan_speed"] * dt
             q = dist(laser_volumetric_source(X, laser["power"], laser["radius"], pos))
+        hp = HeatParams(*(getattr(heat, f) for f, _ in HeatParams._fields_))
+        hp.dt = dt
         if predictor == "none":
             P = T.copy()
         elif predictor == "zero":
             P = np.zeros_like(T)
+        elif predictor == "least_squares":
+            if step - 1 >= 2:
+                lin = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old,
+                                     amesh=amesh, boundary=boundary, vel=vel,
+                                     vel_kind=VEL_NONE if vel is None else VEL_NODAL)
+                P, _ = least_squares_projection(lin.vmult, [T, T_old], lin.create_rhs(T, q))
+            else:
+                P = T.copy()
         else:
             P = T.copy() if abs(dt_old) < 1e-12 else (1 + dt / dt_old) * T - (dt / dt_old) * T_old
         T_old, T = T, P
         if dirichlet_values is not None:
             T[mask == 1] = np.asarray(dirichlet_values)[mask == 1]
         T = dist(T)
-        hp = HeatParams(*(getattr(heat, f) for f, _ in HeatParams._fields_))
-        hp.dt = dt
         converged = False
         for _ in range(newton[0]):
             op = OracleOperator(OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T_old, amesh=amesh,

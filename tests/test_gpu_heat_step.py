@@ -130,3 +130,47 @@ def test_stepper_on_adaptive_mesh_matches_oracle_loop():
         r = r[perm]
         assert np.max(np.abs(g - r)) / np.max(np.abs(r - 300.0)) < 1e-8
     assert got[-1].max() > 450.0
+
+
+def test_least_squares_projection_matches_oracle():
+    """mf_predictor_least_squares against the oracle's restatement on the tanh Jacobian (three
+    old solutions, general rhs) and on the advection operator (non-symmetric); a dependent old
+    solution stops the projection as in the oracle."""
+    import meltpooldg_amd as mf
+    from mfcases import Case, device_operator, fields, oracle_operator, rel_err
+    for case in (Case(degree=2, kind="heat", law="tanh", t_dependent=True),
+                 Case(degree=3, kind="advec", mask="random")):
+        f = fields(case)
+        ref = oracle_operator(case, f)
+        s, op = device_operator(case, f)
+        m = case.mesh()
+        xs = [orc.random_field(m, 30 + i) for i in range(3)]
+        rhs = orc.random_field(m, 40)
+        exp, used_ref = orc.least_squares_projection(ref.vmult, xs, rhs)
+        sol = torch.full((s.n_dofs,), float("nan"), dtype=torch.float64, device="cuda")
+        used = mf.predictor_least_squares(op, [_dev(x) for x in xs], _dev(rhs), sol)
+        assert used == used_ref == 3
+        assert rel_err(sol.cpu().numpy(), exp) < 1e-12
+        dep = [xs[0], 2.0 * xs[0]]
+        exp, used_ref = orc.least_squares_projection(ref.vmult, dep, rhs)
+        used = mf.predictor_least_squares(op, [_dev(x) for x in dep], _dev(rhs), sol)
+        assert used == used_ref == 1
+        assert rel_err(sol.cpu().numpy(), exp) < 1e-12
+
+
+def test_stepper_least_squares_predictor_matches_oracle_loop():
+    import meltpooldg_amd as mf
+    s, mask, T0, op = _setup(mf)
+    dts = [1e-2, 1e-2, 2e-2, 1e-2, 1e-2]
+    st = mf.HeatTimeStepper(op, _dev(T0), laser=LASER, solver="GMRES", rel_tolerance=1e-12,
+                            predictor=mf.PREDICTOR_LEAST_SQUARES_PROJECTION)
+    got = []
+    for dt in dts:
+        rep = st.step(dt, dirichlet_values=_dev(T0))
+        assert rep["status"] == 0
+        got.append(st.solution().cpu().numpy())
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, dt=1e-2, **TANH)
+    ref = orc.heat_time_steps(orc.make_mesh(3, 2, NC), hp, mask, T0, dts, laser=LASER,
+                              dirichlet_values=T0, predictor="least_squares")
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g - r)) / np.max(np.abs(r - T0)) < 1e-8

@@ -24,7 +24,7 @@ def test_laser_source_integrates_to_power():
     assert abs(_mass_integral(mesh, q) / 1.5 - 1.0) < 2e-3
 
 
-@pytest.mark.parametrize("predictor", ["linear", "none", "zero"])
+@pytest.mark.parametrize("predictor", ["linear", "none", "zero", "least_squares"])
 def test_linear_law_matches_implicit_euler(predictor):
     mesh = orc.make_mesh(3, 2, (4, 3, 3))
     mask = orc.boundary_mask(mesh)
@@ -95,3 +95,69 @@ def test_adaptive_loop_on_uniform_mesh_equals_structured():
     got = orc.heat_time_steps(None, hp, ma, T0a, [1e-2, 1e-2], laser=laser, dirichlet_values=T0a,
                               amesh=am)
     assert np.max(np.abs(got[-1][perm] - ref[-1])) < 1e-9 * np.max(np.abs(ref[-1] - 300.0))
+
+
+# ---------------------------------------------------------------- least-squares projection
+def _lsq_case(seed=3):
+    mesh = orc.make_mesh(3, 2, (3, 3, 2))
+    mask = orc.boundary_mask(mesh)
+    n = orc.n_dofs(mesh)
+    rng = np.random.default_rng(seed)
+    T = 1700.0 + 100.0 * rng.uniform(-1, 1, n)
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=1700.0, tanh_width=50.0, dt=1e-2)
+    op = orc.OracleOperator(orc.OP_HEAT, mesh, heat=hp, mask=mask, T=T, T_old=T)
+    xs = [rng.uniform(-1, 1, n) for _ in range(3)]
+    for x in xs:
+        x[mask == 1] = 0.0
+    return op, xs, rng
+
+
+def test_least_squares_reproduces_a_combination_of_old_solutions():
+    """rhs = A (a x_0 + b x_1 + c x_2): the projection returns that combination."""
+    op, xs, _ = _lsq_case()
+    target = 0.7 * xs[0] - 1.3 * xs[1] + 0.2 * xs[2]
+    sol, used = orc.least_squares_projection(op.vmult, xs, op.vmult(target))
+    assert used == 3
+    assert np.max(np.abs(sol - target)) < 1e-12 * np.max(np.abs(target))
+
+
+def test_least_squares_is_the_least_squares_solution():
+    """For a general rhs the coefficients are the minimiser of |W c - rhs| with W = [A x_s],
+    here from numpy's SVD-based lstsq (an independent method)."""
+    op, xs, rng = _lsq_case(4)
+    rhs = rng.uniform(-1, 1, op.n)
+    sol, used = orc.least_squares_projection(op.vmult, xs, rhs)
+    W = np.stack([op.vmult(x) for x in xs], axis=1)
+    c, *_ = np.linalg.lstsq(W, rhs, rcond=None)
+    ref = sum(ci * x for ci, x in zip(c, xs))
+    assert used == 3
+    assert np.max(np.abs(sol - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
+def test_least_squares_stops_at_a_dependent_old_solution():
+    """x_1 = 2 x_0: its orthogonalised image vanishes (below 1e-16 |A x_0|^2), so only x_0
+    enters, with the coefficient (A x_0 . rhs) / |A x_0|^2."""
+    op, xs, rng = _lsq_case(5)
+    rhs = rng.uniform(-1, 1, op.n)
+    sol, used = orc.least_squares_projection(op.vmult, [xs[0], 2.0 * xs[0], xs[1]], rhs)
+    v0 = op.vmult(xs[0])
+    assert used == 1
+    assert np.max(np.abs(sol - (v0 @ rhs) / (v0 @ v0) * xs[0])) < 1e-13 * np.max(np.abs(xs[0]))
+
+
+def test_least_squares_predictor_loop_reaches_the_same_solutions():
+    """The heat loop with the least-squares predictor converges to the solutions of the loop
+    with linear extrapolation (Newton stops within its tolerances either way)."""
+    mesh = orc.make_mesh(3, 2, (4, 3, 3))
+    mask = orc.boundary_mask(mesh)
+    T0 = np.full(orc.n_dofs(mesh), 300.0)
+    hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
+                        tanh_Tc=400.0, tanh_width=50.0, dt=1e-2)
+    laser = dict(power=1000.0, radius=0.3, start=(0.2, 0.5, 0.8), scan_direction=(1, 0, 0),
+                 scan_speed=5.0)
+    kw = dict(laser=laser, dirichlet_values=T0)
+    a = orc.heat_time_steps(mesh, hp, mask, T0, [1e-2] * 4, predictor="least_squares", **kw)
+    b = orc.heat_time_steps(mesh, hp, mask, T0, [1e-2] * 4, predictor="linear", **kw)
+    for x, y in zip(a, b):
+        assert np.max(np.abs(x - y)) < 1e-8 * np.max(np.abs(y - 300.0))
