@@ -327,23 +327,6 @@ __device__ __forceinline__ Number block_reduce_sum(Number v, Number *red)
   return s;
 }
 
-// Row decomposition of the mesh for colour `colour` (0..3): bit 0 = parity of the cell row y,
-// bit 1 = parity of the cell layer z.  Rows of one colour share no node.
-struct RowColour {
-  int yb, zb;  // parities
-  int hy, hz;  // number of rows / layers of this colour
-};
-
-__host__ __device__ inline RowColour row_colour(const int ncell[3], int colour)
-{
-  RowColour s;
-  s.yb = colour & 1;
-  s.zb = (colour >> 1) & 1;
-  s.hy = (ncell[1] - s.yb + 1) / 2;
-  s.hz = (ncell[2] - s.zb + 1) / 2;
-  return s;
-}
-
 // One colour launch of the cell kernel.
 struct LaunchPart {
   int colour;     // structured meshes: row colour 0..3
@@ -1648,6 +1631,8 @@ int launch_cell(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<N
     set_error("operation not available on adaptive meshes (structured-mesh kernel)");
     return MF_ERR_ARG;
   }
+  if (op == HEAT_TANGENT && plane_applicable<Number>(m, needT, vel, a))
+    return launch_heat_plane<Number>(m, needT, a, src, dst, st);
   switch (m.K) {
   case 2: return launch_deg<Number, 2>(op, needT, vel, m, a, src, dst, st);
   case 3: return launch_deg<Number, 3>(op, needT, vel, m, a, src, dst, st);

@@ -122,6 +122,23 @@ struct CellArgs {
   int overwrite;               // structured meshes: dst = A src (first writer of a node stores)
 };
 
+// Row decomposition of a structured mesh for colour `colour` (0..3): bit 0 = parity of the cell
+// row y, bit 1 = parity of the cell layer z.  Rows of one colour share no node.
+struct RowColour {
+  int yb, zb;  // parities
+  int hy, hz;  // number of rows / layers of this colour
+};
+
+__host__ __device__ inline RowColour row_colour(const int ncell[3], int colour)
+{
+  RowColour s;
+  s.yb = colour & 1;
+  s.zb = (colour >> 1) & 1;
+  s.hy = (ncell[1] - s.yb + 1) / 2;
+  s.hz = (ncell[2] - s.zb + 1) / 2;
+  return s;
+}
+
 struct AdaptiveData;
 
 struct MeshInfo {
@@ -241,6 +258,13 @@ void free_adaptive(::mf_scratch *s);
 template <typename Number>
 int launch_cell(int op, int needT, bool vel, const MeshInfo &m, const CellArgs<Number> &a,
                 const Number *src, Number *dst, hipStream_t stream);
+// structured FE_Q3 heat tangent with the cell tensors as register planes (mf_plane.hip): whether it
+// serves an application, and its 4 colour launches (one workgroup per cell row)
+template <typename Number>
+bool plane_applicable(const MeshInfo &m, int needT, bool vel, const CellArgs<Number> &a);
+template <typename Number>
+int launch_heat_plane(const MeshInfo &m, int needT, const CellArgs<Number> &a, const Number *src,
+                      Number *dst, hipStream_t stream);
 // adaptive meshes: mode 0 = all cells (tangent ops), 1 = cells without hanging nodes (diagonal
 // ops), 2 = condensed diagonal of the cells with hanging nodes (op = a tangent op)
 template <typename Number>
