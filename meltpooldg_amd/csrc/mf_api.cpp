@@ -145,6 +145,7 @@ static CellArgs<Number> make_args(const mf_operator *op)
   a.n_dofs = uint32_t(m.n_dofs);
   a.n_cells = int(m.n_cells);
   a.mask = s->d_mask;
+  a.mask_boundary = s->mask_full_boundary ? 1 : 0;
   // the level-set projections have no Dirichlet rows
   a.use_mask = (op->kind < 2 && op->dof_index == 0) ? 1 : 0;
   a.transpose = op->transpose;
@@ -373,7 +374,10 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
   // inflow/outflow rows (do_pre_vmult): their src entries read as 0, their rows left to the post
   // pass, like Dirichlet rows
   const bool io = io_active(op);
-  if (io) a.mask = op->io_union;
+  if (io) {
+    a.mask = op->io_union;
+    a.mask_boundary = 0;
+  }
   a.dot_partial = dot_partial;
   a.stop = stop;
   a.overwrite = (overwrite && !m.ad) ? 1 : 0;
@@ -502,6 +506,7 @@ int mf_set_stream(mf_scratch *s, void *stream)
 int mf_scratch_set_dirichlet_mask(mf_scratch *s, const uint8_t *mask_host)
 {
   MF_REQUIRE(s, MF_ERR_ARG, "null scratch");
+  s->mask_full_boundary = false;
   if (!mask_host) {
     if (s->d_mask) MF_HIP_TRY(hipFree(s->d_mask));
     s->d_mask = nullptr;
@@ -510,6 +515,23 @@ int mf_scratch_set_dirichlet_mask(mf_scratch *s, const uint8_t *mask_host)
   }
   const size_t n = size_t(s->mesh.n_dofs);
   s->h_mask.assign(mask_host, mask_host + n);
+  if (!s->mesh.ad) {
+    // is the mask exactly the boundary of the box?
+    const int64_t NX = s->mesh.NN[0], NY = s->mesh.NN[1], NZ = s->mesh.NN[2];
+    bool full = true;
+    for (int64_t z = 0, id = 0; z < NZ && full; ++z)
+      for (int64_t y = 0; y < NY && full; ++y) {
+        const bool face = z == 0 || z == NZ - 1 || y == 0 || y == NY - 1;
+        for (int64_t x = 0; x < NX; ++x, ++id) {
+          const bool b = face || x == 0 || x == NX - 1;
+          if (b != (mask_host[id] != 0)) {
+            full = false;
+            break;
+          }
+        }
+      }
+    s->mask_full_boundary = full;
+  }
   if (!s->d_mask) MF_HIP_TRY(hipMalloc(&s->d_mask, n));
   MF_HIP_TRY(hipMemcpy(s->d_mask, mask_host, n, hipMemcpyHostToDevice));
   return MF_OK;

@@ -120,6 +120,7 @@ struct CellArgs {
   const uint32_t *slot_tgt;
   const Number *hang_val;      // [n_hanging] interpolated source values of the hanging nodes
   int overwrite;               // structured meshes: dst = A src (first writer of a node stores)
+  int mask_boundary;           // structured meshes: mask is exactly the boundary of the box
 };
 
 // Row decomposition of a structured mesh for colour `colour` (0..3): bit 0 = parity of the cell
@@ -311,6 +312,9 @@ struct mf_scratch {
   hipStream_t stream = nullptr;
   uint8_t *d_mask = nullptr;
   std::vector<uint8_t> h_mask;
+  // structured meshes: the Dirichlet mask is exactly the boundary of the box (the kernels may
+  // then compute it from the node coordinates instead of reading it)
+  bool mask_full_boundary = false;
   mf::AdaptiveData *ad = nullptr;  // adaptive mesh (null: structured)
 };
 

@@ -46,6 +46,8 @@ struct PlaneArgs {
   int ncell[3];
   uint32_t NX, NY;
   const uint8_t *mask;  // Dirichlet rows (null: none)
+  int mask_boundary;    // the Dirichlet rows are exactly the boundary nodes of the box (computed
+                        // from the node coordinates, mask not read)
   int gather_plain;     // read src without the Dirichlet constraints
   int overwrite;        // dst = A src (first writer stores) instead of dst += A src
   int colour, part_off;
@@ -146,6 +148,39 @@ __device__ __forceinline__ void copy_planes(const Number (&s)[4][4], Number (&d)
     for (int b = 0; b < 4; ++b) d[a][b] = s[a][b];
 }
 
+// two consecutive entries (8-byte aligned) in one 16-byte access
+typedef double plane_d2 __attribute__((ext_vector_type(2)));
+typedef float plane_f2 __attribute__((ext_vector_type(2)));
+typedef plane_d2 plane_d2u __attribute__((aligned(8)));
+typedef plane_f2 plane_f2u __attribute__((aligned(4)));
+template <typename Number>
+struct Pair;
+template <>
+struct Pair<double> {
+  typedef plane_d2 v;
+  typedef plane_d2u u;
+};
+template <>
+struct Pair<float> {
+  typedef plane_f2 v;
+  typedef plane_f2u u;
+};
+template <typename Number>
+__device__ __forceinline__ void load2(const Number *p, Number &a, Number &b)
+{
+  const typename Pair<Number>::v v = *reinterpret_cast<const typename Pair<Number>::u *>(p);
+  a = v.x;
+  b = v.y;
+}
+template <typename Number>
+__device__ __forceinline__ void store2(Number *p, Number a, Number b)
+{
+  typename Pair<Number>::v v;
+  v.x = a;
+  v.y = b;
+  *reinterpret_cast<typename Pair<Number>::u *>(p) = v;
+}
+
 // tanh(x) and sech^2(x) from e = exp(-2|x|) (see the general cell kernel: beyond |x| = 20 in
 // double and 10 in float, tanh is +-1 to the last bit and the sech^2 term is below rounding)
 __device__ __forceinline__ double plane_rcp(double d)
@@ -218,25 +253,36 @@ __device__ __forceinline__ PlaneRow plane_row(const int ncell[3], uint32_t NX, u
   return r;
 }
 
-// Issue the loads of a row: T first, then the source, then the Dirichlet bits.  Nothing is
-// selected here, so that all loads go out together; the T phase of the row then runs while the
-// source values may still be in flight.
+// Issue the loads of a row: T first, then the source, then the Dirichlet bits (read, or from the
+// node coordinates when the mask is the box boundary).  Nothing is selected here, so that all
+// loads go out together; the T phase of the row then runs while the source values may still be in
+// flight.  A pencil's 4 nodes are read as two 16-byte pairs.
 template <typename Number, bool TLAW>
 __device__ __forceinline__ void plane_issue(const PlaneArgs<Number> &A, const Number *src,
-                                            const PlaneRow &r, PlaneRaw<Number, TLAW> &raw)
+                                            const PlaneRow &r, int R, int cx,
+                                            PlaneRaw<Number, TLAW> &raw)
 {
   if constexpr (TLAW) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) raw.t[i + 4 * j] = A.T[r.g[j] + i];
+    for (int j = 0; j < 4; ++j) {
+      load2(A.T + r.g[j], raw.t[4 * j], raw.t[4 * j + 1]);
+      load2(A.T + r.g[j] + 2, raw.t[4 * j + 2], raw.t[4 * j + 3]);
+    }
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) raw.u[i + 4 * j] = src[r.g[j] + i];
+  for (int j = 0; j < 4; ++j) {
+    load2(src + r.g[j], raw.u[4 * j], raw.u[4 * j + 1]);
+    load2(src + r.g[j] + 2, raw.u[4 * j + 2], raw.u[4 * j + 3]);
+  }
   uint32_t mb = 0;
-  if (A.mask) {
+  if (A.mask_boundary) {
+    // node (3 cx + i, 3 cy + j, 3 cz + R) on a face of the box: bit i + 4 j
+    if ((R == 0 && r.cz == 0) || (R == 3 && r.cz == A.ncell[2] - 1)) mb = 0xffffu;
+    if (r.cy == 0) mb |= 0x000fu;
+    if (r.cy == A.ncell[1] - 1) mb |= 0xf000u;
+    if (cx == 0) mb |= 0x1111u;
+    if (cx == A.ncell[0] - 1) mb |= 0x8888u;
+  } else if (A.mask) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -274,7 +320,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
   const int row = int(blockIdx.x);
   const PlaneRow cur = plane_row(A.ncell, A.NX, A.NY, rc, row, R, cx, active);
   PlaneRaw<Number, TLAW> raw;
-  plane_issue<Number, TLAW>(A, src, cur, raw);
+  plane_issue<Number, TLAW>(A, src, cur, R, cx, raw);
   {
     const int cy = cur.cy, cz = cur.cz;
     [[maybe_unused]] Number t[4][4];
@@ -412,8 +458,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
       const bool first = (!shy || rc.yb == 0) && (!shz || rc.zb == 0);
       const bool store = A.overwrite && first;
       store_rows |= uint32_t(store) << j;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) d[i][j] = (active && !store) ? dst[cur_row.g[j] + i] : Number(0);
+      d[0][j] = d[1][j] = d[2][j] = d[3][j] = Number(0);
+      if (active && !store) {
+        load2(dst + cur_row.g[j], d[0][j], d[1][j]);
+        d[2][j] = dst[cur_row.g[j] + 2];
+        if (last) d[3][j] = dst[cur_row.g[j] + 3];
+      }
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -456,19 +506,22 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
       u[0][j] += x > 0 ? prev : (wave > 0 ? xb[wave - 1][R][j] : Number(0));
     }
     if (active) {
+      // every node the cell writes (i < 3, and i = 3 at the row end) gets its final value:
+      // constrained rows are zero where this colour stores them (dst = A src) and keep their
+      // value where it adds (distribute_local_to_global skips them)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const bool store = (store_rows >> j) & 1u;
+        Number v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          if (i == 3 && !last) break;
-          Number *q = dst + cur_row.g[j] + i;
-          if ((mbits >> (i + 4 * j)) & 1u) {
-            if (store) *q = Number(0);  // constrained rows of dst = A src are zero
-            continue;
-          }
-          *q = store ? u[i][j] : d[i][j] + u[i][j];
+          const bool con = (mbits >> (i + 4 * j)) & 1u;
+          v[i] = con ? (store ? Number(0) : d[i][j]) : (store ? u[i][j] : d[i][j] + u[i][j]);
         }
+        Number *q = dst + cur_row.g[j];
+        store2(q, v[0], v[1]);
+        q[2] = v[2];
+        if (last) q[3] = v[3];
       }
     }
   }
@@ -514,6 +567,7 @@ int launch_heat_plane(const MeshInfo &m, int needT, const CellArgs<Number> &a, c
   p.NX = uint32_t(m.NN[0]);
   p.NY = uint32_t(m.NN[1]);
   p.mask = (a.use_mask && a.mask) ? a.mask : nullptr;
+  p.mask_boundary = (p.mask && a.mask_boundary) ? 1 : 0;
   p.gather_plain = a.gather_plain;
   p.overwrite = a.overwrite;
   p.T = a.T;
