@@ -88,7 +88,7 @@ def main():
     print(f"# workload: Q3 64^3, n_dofs={n}, n_cells={s.n_cells}", file=sys.stderr, flush=True)
     if args.pmc_workload:
         for _ in range(5):
-            op.vmult_add(dst, src)
+            op.vmult(dst, src)
         src32, dst32 = src.float(), torch.zeros(n, dtype=torch.float32, device="cuda")
         for _ in range(5):
             op.vmult_f32(dst32, src32)
@@ -114,11 +114,14 @@ def main():
     # timed region: K full vmults (zero + cell kernel), HIP events on the library's stream
     ms_per_step = timed(op.vmult, args.steps)
     value = n / (ms_per_step * 1e-3)
-    # the cell kernel alone (vmult_add: no zeroing) for the roofline.  One application is 4
-    # launches (one per (y, z) row-parity colour, a quarter of the cells each); the achieved
-    # rate is the algorithmic bytes of an application over its 4 launches' time, which equals the
-    # per-launch bytes (a quarter) over the average launch duration.
-    kernel_ms = timed(op.vmult_add, args.steps)
+    # the roofline: one application dst = A src is 4 launches of the cell kernel (one per (y, z)
+    # row-parity colour, a quarter of the cells each; the first colour to write a node stores
+    # it, so there is no zeroing pass); the achieved rate is the algorithmic bytes of an
+    # application over its 4 launches' time (HIP events around args.steps applications on the
+    # library's stream), which equals the per-launch bytes (a quarter) over the average launch
+    # duration.  dst += A src (vmult_add, every colour reads dst) is reported beside it.
+    kernel_ms = timed(op.vmult, args.steps)
+    add_ms = timed(op.vmult_add, args.steps)
     alg_bytes = 3 * n * 8  # src read, dst write, T read: each DoF vector once
     achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
     K = 4
@@ -129,7 +132,8 @@ def main():
     for _ in range(args.warmup):
         op.vmult_f32(dst32, src32)
     f32_ms = timed(lambda d, s_: op.vmult_f32(dst32, src32), args.steps)
-    print(f"# vmult {ms_per_step*1e3:.1f} us/step, cell kernel {kernel_ms*1e3:.1f} us, "
+    print(f"# vmult {ms_per_step*1e3:.1f} us/step, cell kernels {kernel_ms*1e3:.1f} us "
+          f"(vmult_add {add_ms*1e3:.1f} us), "
           f"{achieved_gbs:.0f} GB/s, {achieved_tflops:.2f} TFLOP/s; FP32 vmult {f32_ms*1e3:.1f} us",
           file=sys.stderr, flush=True)
 
@@ -201,7 +205,8 @@ def main():
                      "kernel_us_per_vmult": kernel_ms * 1e3, "alg_bytes_per_vmult": alg_bytes,
                      "fp64_tflops": achieved_tflops,
                      "fp64_frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "fp32_vmult_us": f32_ms * 1e3},
+                     "fp32_vmult_us": f32_ms * 1e3, "vmult_add_us": add_ms * 1e3,
+                     "kernel": TRAFFIC_KERNEL},
         "cpu_baseline": cpu,
     }
     line.update(extra)
@@ -495,7 +500,7 @@ def levelset_geometry_case(mf, torch, stream):
     return out
 
 
-TRAFFIC_KERNEL = "cell_kernel<double, 4, 0, true, false, false"  # FP64 heat tangent, no fused dot
+TRAFFIC_KERNEL = "plane_tangent_kernel<double, true, false>"  # FP64 tanh heat tangent, no fused dot
 COPY_KERNEL = "copyBuffer"
 
 
@@ -551,29 +556,46 @@ def measure_traffic(timeout=180):
                            "57.5 MB written) in the same run"}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(T, mask, b, n, budget):
     """The CPU oracle's sum-factorised vmult on the same 64^3 Q3 workload, OpenMP over the host
-    cores, timed for a bounded number of applications."""
+    cores this process may use (the GPU box allots 16 cores to one GPU: OMP_NUM_THREADS there),
+    5 warm-up applications, then the median of at least 20 timed ones (more if the budget
+    allows)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
     ncores = len(os.sched_getaffinity(0))
-    nthreads = min(ncores, 16)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    nthreads = min(ncores, int(omp)) if omp.isdigit() and int(omp) > 0 else ncores
     m = orc.make_mesh(3, 3, 64)
     hp = orc.HeatParams(law=orc.LAW_TANH, t_dependent=1, rho_cp=1.0, k=1.0, tanh_amp=0.5,
                         tanh_Tc=1700.0, tanh_width=50.0, dt=1e-4)
     op = orc.OracleOperator(orc.OP_HEAT, m, heat=hp, mask=mask, T=T, T_old=T,
                             nthreads=nthreads)
     t0 = time.perf_counter()
-    op.vmult(b)
-    first = time.perf_counter() - t0
-    reps = max(1, min(20, int(budget / max(first, 1e-3))))
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    for _ in range(5):
         op.vmult(b)
-    dt = (time.perf_counter() - t0) / reps
+    first = (time.perf_counter() - t0) / 5
+    reps = max(20, min(200, int(budget / max(first, 1e-3))))
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        op.vmult(b)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
     return {"value": n / dt, "unit": "DoF/s", "cores": nthreads, "kind": "port",
-            "sample": f"{reps} FP64 vmults of the full 64^3 Q3 Jacobian (oracle, OpenMP, "
-                      f"{nthreads} threads), {dt*1e3:.0f} ms each"}
+            "cpu_model": cpu_model(), "host_cores_visible": ncores,
+            "sample": f"median of {reps} FP64 vmults (after 5 warm-ups) of the full 64^3 Q3 "
+                      f"Jacobian (oracle, OpenMP, {nthreads} threads), {dt*1e3:.1f} ms each"}
 
 
 if __name__ == "__main__":
