@@ -233,6 +233,55 @@ int orc_solve(const orc_operator *op, int solver, int restart, int max_it, doubl
 /* --- norms ----------------------------------------------------------------------------------- */
 double orc_fe_l2_norm(const orc_mesh *m, const double *u);
 
+/* --- one-phase CutFEM heat operator (orc_cut.c) -------------------------------------------- */
+/* HeatCutOperator without two_phase (source/heat/heat_cut_operator.cpp), FE_Q1 only (the
+ * reference's AssertThrow at :498), on a structured box mesh.  Cells are classified from the
+ * nodal (Q1) level set phi like dealii::NonMatching::MeshClassifier with the CutUtil mapping
+ * (source/cut/util.cpp:100-128, 146-169): phi >= 0 on the whole cell: liquid ("outside");
+ * phi < 0 on the whole cell: gas ("inside", FE_Nothing, no DoFs); otherwise intersected.  The
+ * liquid part of an intersected cell is {phi > 0}; this oracle (like the device) takes it only
+ * where phi varies along a single axis in that cell, so that it is a sub-box of the cell and
+ * the quadrature of dealii::NonMatching::QuadratureGenerator with QGauss<1>(2) is the tensor
+ * Gauss rule on that sub-box. */
+enum { ORC_CUT_LIQUID = 0, ORC_CUT_INTERSECTED = 1, ORC_CUT_GAS = 2 };
+
+typedef struct {
+  long n_cells, n_dofs;
+  int8_t *category;   /* [n_cells] ORC_CUT_* (CutUtil::CellCategory) */
+  double *box;        /* [n_cells][3][2] liquid part in reference coordinates */
+  uint8_t *active;    /* [n_dofs] 1 = vertex of a liquid or intersected cell */
+  long n_faces;
+  long *faces;        /* [n_faces][2] (lower cell, direction): the ghost-penalty faces */
+} orc_cut_mesh;
+
+typedef struct {
+  /* law ORC_LAW_CONSTANT: the liquid phase with rho_cp and k (do_solidification false);
+   * ORC_LAW_MUSHY: the liquid_solid law (do_solidification true, with the derivative terms) */
+  orc_heat_params mat;
+  double dt, theta;   /* HeatData::cut.theta (0.5): ost factors dt theta and dt (1 - theta) */
+  double gamma_A, gamma_M; /* GhostPenaltyData gamma_A_degree_1, gamma_M_degree_1 */
+  double h;           /* ScratchData::get_min_cell_size */
+} orc_cut_params;
+
+/* 0: success; 1: an intersected cell whose level set varies along more than one axis;
+ * 2: degree != 1 */
+int orc_cut_classify(const orc_mesh *m, const double *phi, orc_cut_mesh *c);
+void orc_cut_free(orc_cut_mesh *c);
+/* dst = J src (tangent_cell_loop + tangent_inner_face_loop, heat_cut_operator.cpp:587-607,
+ * 649-836, 1028-1388); T = linearisation temperature (mushy law).  Rows of dmask (Dirichlet)
+ * and of inactive DoFs are 0, their src entries read as 0. */
+void orc_cut_vmult(const orc_mesh *m, const orc_cut_mesh *c, const orc_cut_params *p,
+                   const uint8_t *dmask, const double *T, const double *src, double *dst);
+/* diagonal of the assembled J (cells and ghost-penalty faces); 1 on dmask / inactive rows */
+void orc_cut_diagonal(const orc_mesh *m, const orc_cut_mesh *c, const orc_cut_params *p,
+                      const uint8_t *dmask, const double *T, double *diag);
+/* dst = -R(T) (create_rhs with residual_cell_loop / residual_inner_face_loop, :609-628,
+ * 1405-1987, 1988-2121), T and T_old read plain; 0 on dmask / inactive rows */
+void orc_cut_residual(const orc_mesh *m, const orc_cut_mesh *c, const orc_cut_params *p,
+                      const uint8_t *dmask, const double *T, const double *T_old, double *dst);
+/* CutUtil::compute_cut_norm, L2 (source/cut/cut_norm.cpp:20-217), one phase */
+double orc_cut_norm_l2(const orc_mesh *m, const orc_cut_mesh *c, const double *u);
+
 /* --- whole-path drivers used by the golden tests ------------------------------------------- */
 int orc_heat_implicit_euler(const orc_operator *op, int n_steps, double *T, double *norms);
 int orc_advec_theta_steps(const orc_operator *op, int n_steps, const double *dirichlet_values,

@@ -628,3 +628,151 @@ def fe_evaluate_adaptive(degree, ncell, cells, node_xyz, values, points, lo=(0.0
                 w[0][a[0]] * w[1][a[1]] * w[2][a[2]]
         res[i] = s
     return res
+
+
+# ------------------------------------------------------------------------------------------
+# One-phase CutFEM heat operator (orc_cut.c)
+# ------------------------------------------------------------------------------------------
+CUT_LIQUID, CUT_INTERSECTED, CUT_GAS = 0, 1, 2
+
+
+class CutParams(C.Structure):
+    _fields_ = [("mat", HeatParams), ("dt", C.c_double), ("theta", C.c_double),
+                ("gamma_A", C.c_double), ("gamma_M", C.c_double), ("h", C.c_double)]
+
+
+class CutMeshC(C.Structure):
+    _fields_ = [("n_cells", C.c_long), ("n_dofs", C.c_long),
+                ("category", C.POINTER(C.c_int8)), ("box", C.POINTER(C.c_double)),
+                ("active", C.POINTER(C.c_uint8)), ("n_faces", C.c_long),
+                ("faces", C.POINTER(C.c_long))]
+
+
+def cut_params(dt, rho_cp=None, k=None, theta=0.5, gamma_A=1.0, gamma_M=1.0, h=None,
+               mushy=None, mesh=None):
+    """CutParams: the constant liquid law (rho_cp, k) or, with mushy = dict(liquid_k, ...,
+    solidus, liquidus), the liquid_solid law.  h defaults to the minimal cell diameter /
+    sqrt(dim) of `mesh` (ScratchData::get_min_cell_size, source/core/scratch_data.cpp:170)."""
+    hp = HeatParams()
+    if mushy is None:
+        hp.law = LAW_CONSTANT
+        hp.rho_cp, hp.k = rho_cp, k
+    else:
+        hp.law = LAW_MUSHY
+        hp.t_dependent = 1
+        for f, v in mushy.items():
+            setattr(hp, f, v)
+    if h is None:
+        hs = [(mesh.hi[d] - mesh.lo[d]) / mesh.ncell[d] for d in range(mesh.dim)]
+        h = np.sqrt(sum(x * x for x in hs)) / np.sqrt(mesh.dim)
+    return CutParams(hp, dt, theta, gamma_A, gamma_M, h)
+
+
+class CutOracle:
+    """The oracle's one-phase CutFEM heat operator on a structured FE_Q1 mesh with the nodal
+    level set phi (liquid where phi > 0).  mask: Dirichlet rows (uint8 [n_dofs]) or None."""
+
+    def __init__(self, mesh, phi, params, mask=None):
+        L = lib()
+        dp = C.POINTER(C.c_double)
+        for name in ("orc_cut_vmult", "orc_cut_diagonal", "orc_cut_residual"):
+            getattr(L, name).restype = None
+        L.orc_cut_norm_l2.restype = C.c_double
+        L.orc_cut_norm_l2.argtypes = [C.POINTER(Mesh), C.POINTER(CutMeshC), dp]
+        self.mesh, self.params = mesh, params
+        self.n = orc_n = n_dofs(mesh)
+        self._phi = np.ascontiguousarray(phi, dtype=np.float64)
+        assert self._phi.shape == (orc_n,)
+        self.cm = CutMeshC()
+        st = L.orc_cut_classify(C.byref(mesh), _p(self._phi), C.byref(self.cm))
+        if st != 0:
+            L.orc_cut_free(C.byref(self.cm))
+            raise ValueError("level set not supported by the cut oracle (status %d)" % st)
+        nc = self.cm.n_cells
+        self.category = np.ctypeslib.as_array(self.cm.category, shape=(nc,)).copy()
+        self.box = np.ctypeslib.as_array(self.cm.box, shape=(nc, 3, 2)).copy()
+        self.active = np.ctypeslib.as_array(self.cm.active, shape=(orc_n,)).copy()
+        nf = self.cm.n_faces
+        self.faces = (np.ctypeslib.as_array(self.cm.faces, shape=(nf, 2)).copy() if nf
+                      else np.zeros((0, 2), dtype=np.int64))
+        self.mask = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+
+    def __del__(self):
+        try:
+            lib().orc_cut_free(C.byref(self.cm))
+        except Exception:
+            pass
+
+    def _mask(self):
+        return None if self.mask is None else self.mask.ctypes.data_as(C.c_void_p)
+
+    def vmult(self, src, T=None):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        Tp = None if T is None else _p(np.ascontiguousarray(T, dtype=np.float64))
+        dst = np.zeros(self.n)
+        lib().orc_cut_vmult(C.byref(self.mesh), C.byref(self.cm), C.byref(self.params),
+                            self._mask(), Tp, _p(src), _p(dst))
+        return dst
+
+    def diagonal(self, T=None):
+        Tp = None if T is None else _p(np.ascontiguousarray(T, dtype=np.float64))
+        d = np.zeros(self.n)
+        lib().orc_cut_diagonal(C.byref(self.mesh), C.byref(self.cm), C.byref(self.params),
+                               self._mask(), Tp, _p(d))
+        return d
+
+    def inverse_diagonal(self, T=None):
+        """compute_inverse_diagonal_from_matrixfree's inversion (heat_cut_operator.cpp:
+        2141-2153): 1/d where |d| > 1e-14 max(1, |d|_inf), else 1."""
+        d = self.diagonal(T)
+        lim = 1e-14 * max(1.0, np.max(np.abs(d)))
+        return np.where(np.abs(d) > lim, 1.0 / np.where(d == 0, 1.0, d), 1.0)
+
+    def residual(self, T, T_old):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        To = np.ascontiguousarray(T_old, dtype=np.float64)
+        dst = np.zeros(self.n)
+        lib().orc_cut_residual(C.byref(self.mesh), C.byref(self.cm), C.byref(self.params),
+                               self._mask(), _p(T), _p(To), _p(dst))
+        return dst
+
+    def norm(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return lib().orc_cut_norm_l2(C.byref(self.mesh), C.byref(self.cm), _p(u))
+
+    def dense(self, T=None):
+        """The assembled operator, column by column (small meshes only)."""
+        A = np.zeros((self.n, self.n))
+        e = np.zeros(self.n)
+        for j in range(self.n):
+            e[j] = 1.0
+            A[:, j] = self.vmult(e, T)
+            e[j] = 0.0
+        return A
+
+
+def cut_time_steps(mesh, phi, params, T0, n_steps, mask=None, newton=(10, 1e-9, 1e-10)):
+    """HeatCutOperation's time loop for a fixed interface (heat_cut_operation.cpp:431-482):
+    the predictor is the old solution, then NewtonRaphsonSolver::solve
+    (source/linear_algebra/newton_raphson_solver.cpp:22-80) with r = create_rhs(T_old) at T,
+    J d = r solved directly on the assembled J (small meshes only), converged when |r| and |d|
+    are below the tolerances, else T += d.  Returns (temperatures, cut L2 norms) per step."""
+    op = CutOracle(mesh, phi, params, mask)
+    T = np.array(T0, dtype=np.float64)
+    temps, norms = [], []
+    for _ in range(n_steps):
+        T_old = T.copy()
+        for _ in range(newton[0]):
+            r = op.residual(T, T_old)
+            A = op.dense(T)
+            fixed = (op.active == 0) if mask is None else ((op.active == 0) | (op.mask != 0))
+            A[fixed, fixed] = 1.0  # the rows of inactive and constrained DoFs: d = 0 there
+            d = np.linalg.solve(A, r)
+            if np.linalg.norm(r) < newton[1] and np.linalg.norm(d) < newton[2]:
+                break
+            T = T + d
+        else:
+            raise RuntimeError("oracle Newton did not converge")
+        temps.append(T.copy())
+        norms.append(op.norm(T))
+    return temps, norms
