@@ -589,6 +589,75 @@ int mf_transient_forward(mf_transient *tr, const double *T0, double *T_final);
 int mf_transient_backward(mf_transient *tr, const double *dJ_dTN, double *grad_k_cell,
                           double *grad_power, double *grad_radius);
 
+/* ------------------------------------------------------------------------------------------ */
+/* One-phase CutFEM heat operator.  Replaces HeatCutOperator without two_phase                 */
+/* (source/heat/heat_cut_operator.cpp) and the matching parts of HeatCutOperation              */
+/* (source/heat/heat_cut_operation.cpp): FE_Q1 only (the reference's AssertThrow at :498) on a */
+/* structured box mesh.  Tangent (tangent_cell_loop + tangent_inner_face_loop, :587-607,      */
+/* 1028-1388): sum over the liquid part of each cell of                                        */
+/*   (cv + dcv/dT T) u v + dt theta (k grad u + dk/dT (grad T) u) . grad v                    */
+/* (tangent_domain_integral, :157-190) plus the ghost penalty on the faces between an          */
+/* intersected cell and an intersected or liquid cell                                          */
+/*   gamma [d_n u][d_n v],  gamma = dt theta gamma_A k_l h / 3 + gamma_M h^3 / 3               */
+/* (ghost_penalty_face_integral, :425-450) over the whole face (QGauss(2)).                    */
+/* Cells are classified from the nodal level set (liquid where phi > 0; MeshClassifier with the */
+/* CutUtil mapping, source/cut/util.cpp:100-128): phi >= 0 on the whole cell: liquid; phi < 0  */
+/* on the whole cell: gas (FE_Nothing: its nodes that touch no other cell carry no DoF);       */
+/* otherwise intersected.  The liquid part of an intersected cell must be a sub-box: phi must  */
+/* vary along one axis only on that cell (axis-aligned planar interfaces, which is what the    */
+/* reference's cut test cases use); the quadrature there is QGauss(2) per direction on the     */
+/* sub-box, the rule dealii::NonMatching::QuadratureGenerator builds for such cells.  Other    */
+/* interfaces give MF_ERR_ARG.  DoFs without a liquid or intersected cell are kept in the      */
+/* library's node numbering as identity rows (vmult 0, diagonal 1, src read as 0).             */
+/* No velocity, evaporative cooling or interface laser flux (HeatCutOperation's optional      */
+/* terms), no FP32 form.                                                                        */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct {
+  /* law MF_LAW_CONSTANT: the liquid phase, rho_cp = rho * cp and k (do_solidification false);
+   * MF_LAW_MUSHY: the liquid_solid law of mf_heat_params with its T-derivatives in the tangent
+   * (do_solidification true).  t_dependent and the tanh fields are ignored. */
+  mf_heat_params material;
+  double theta;              /* HeatData::cut.theta (include/meltpooldg/heat/heat_data.hpp:34),
+                                0.5 in the reference */
+  double gamma_A;            /* GhostPenaltyData::gamma_A_degree_1 (cut_data.hpp:12-32) */
+  double gamma_M;            /* GhostPenaltyData::gamma_M_degree_1 */
+  double cell_side_length;   /* h of the ghost penalty; <= 0: min cell diameter / sqrt(3)
+                                (ScratchData::get_min_cell_size, scratch_data.cpp:170) */
+} mf_heat_cut_params;
+
+/* HeatCutOperator constructor + reinit (:454-559).  The scratch must be a structured FE_Q1
+ * mesh.  The operator takes mf_operator_reset_time_increment (init_time_advance: the ost
+ * factors dt theta and dt (1 - theta)), mf_operator_set_fields (T: the linearisation of the
+ * mushy law and the current temperature of create_rhs), mf_operator_vmult / vmult_add,
+ * mf_operator_compute_inverse_diagonal (compute_inverse_diagonal_from_matrixfree, :2141-2153,
+ * 1/d where |d| > 1e-14 max(1, |d|_inf)), mf_operator_create_rhs (create_rhs, :609-628: src =
+ * T_old, q_s must be NULL; adds -R(T)), mf_solve (FP64) and the Jacobi object. */
+int mf_heat_cut_operator_create(mf_scratch *s, const mf_heat_cut_params *p, mf_operator **out);
+/* The level set (host array of n_dofs nodal values on the scratch's Q1 nodes): classification
+ * of the cells, liquid sub-boxes of the intersected cells, ghost-penalty faces and the DoFs
+ * that exist (HeatCutOperation::classify_cells + distribute_dofs + compute_intersected_
+ * quadrature, heat_cut_operation.cpp:172-330).  MF_ERR_ARG for an intersected cell whose
+ * level set varies along more than one axis. */
+int mf_heat_cut_set_level_set(mf_operator *op, const double *phi_host);
+/* counts of the classification: liquid, intersected and gas cells, ghost-penalty faces and
+ * DoFs that exist (pointers may be NULL) */
+int mf_heat_cut_info(const mf_operator *op, int64_t *n_liquid, int64_t *n_intersected,
+                     int64_t *n_gas, int64_t *n_faces, int64_t *n_active_dofs);
+/* host copies: category[n_cells] (0 liquid, 1 intersected, 2 gas: CutUtil::CellCategory),
+ * box[n_cells][3][2] (liquid part in reference coordinates), active[n_dofs] (any may be NULL) */
+int mf_heat_cut_cells(const mf_operator *op, int8_t *category, double *box, uint8_t *active);
+/* CutUtil::compute_cut_norm, L2 (source/cut/cut_norm.cpp:20-217): sqrt of the integral of u^2
+ * over the liquid part of the domain (u device, read plain); result on the host */
+int mf_heat_cut_norm_l2(mf_operator *op, const double *u, double *result);
+/* One time step of HeatCutOperation::solve for a fixed interface (heat_cut_operation.cpp:
+ * 431-482): NewtonRaphsonSolver::solve with r = create_rhs(T_old) at T, J d = r by the linear
+ * solver, the constraints distributed after each update.  T: the predictor on entry (the
+ * reference copies the old solution), the solution on exit; dirichlet_values: values of the
+ * scratch's constrained DoFs (NULL = homogeneous). */
+int mf_heat_cut_step(mf_operator *op, double *T, const double *T_old,
+                     const double *dirichlet_values, const mf_solver_desc *desc,
+                     const mf_newton_desc *newton, mf_newton_report *report);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,13 @@ class HeatParams(C.Structure):
                 ("solidus", C.c_double), ("liquidus", C.c_double)]
 
 
+class HeatCutParams(C.Structure):
+    # mf_heat_cut_params: the liquid material (constant or mushy law), theta, gamma_A, gamma_M
+    # and the ghost penalty's cell size (<= 0: min cell diameter / sqrt(3))
+    _fields_ = [("material", HeatParams), ("theta", C.c_double), ("gamma_A", C.c_double),
+                ("gamma_M", C.c_double), ("cell_side_length", C.c_double)]
+
+
 class AdvecParams(C.Structure):
     _fields_ = [("theta", C.c_double), ("diffusivity", C.c_double), ("mass_factor", C.c_double),
                 ("supg", C.c_int), ("supg_coefficient", C.c_double)]
@@ -288,6 +295,13 @@ def lib():
         L.mf_heat_stepper_step.argtypes = [vp, C.c_double, vp, C.POINTER(NewtonReport)]
         L.mf_heat_stepper_state.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp),
                                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.mf_heat_cut_operator_create.argtypes = [vp, C.POINTER(HeatCutParams), C.POINTER(vp)]
+        L.mf_heat_cut_set_level_set.argtypes = [vp, vp]
+        L.mf_heat_cut_info.argtypes = [vp] + [C.POINTER(i64)] * 5
+        L.mf_heat_cut_cells.argtypes = [vp, vp, vp, vp]
+        L.mf_heat_cut_norm_l2.argtypes = [vp, vp, C.POINTER(C.c_double)]
+        L.mf_heat_cut_step.argtypes = [vp, vp, vp, vp, C.POINTER(SolverDesc),
+                                       C.POINTER(NewtonDesc), C.POINTER(NewtonReport)]
         L.mf_transient_create.argtypes = [vp, C.POINTER(TransientDesc), C.POINTER(vp)]
         L.mf_transient_destroy.argtypes = [vp]
         L.mf_transient_forward.argtypes = [vp, vp, vp]
@@ -681,6 +695,118 @@ class AdvectionDiffusionOperator(_OperatorBase):
 
     def set_apply_inflow_outflow_constraints_pre_post_matrix_free(self, enable):
         _check(lib().mf_operator_set_inflow_outflow_pre_post(self._h, int(bool(enable))))
+
+
+class HeatCutOperator(_OperatorBase):
+    """One-phase CutFEM heat tangent (reference HeatCutOperator, source/heat/
+    heat_cut_operator.cpp, two_phase false, FE_Q1): over the liquid part {phi > 0} of the mesh
+    (cv + dcv/dT T) u v + dt theta (k grad u + dk/dT grad T u) . grad v, plus the ghost penalty
+    gamma [d_n u][d_n v] on the faces between an intersected cell and an intersected or liquid
+    cell.  law LAW_CONSTANT with rho_cp = rho cp and k of the liquid phase; LAW_MUSHY with
+    `mushy` (the liquid_solid material, do_solidification).  Interfaces must be axis-aligned
+    planes within each intersected cell."""
+
+    def __init__(self, scratch, rho_cp=1.0, k=1.0, law=LAW_CONSTANT, mushy=None, theta=0.5,
+                 gamma_A=1.0, gamma_M=1.0, cell_side_length=0.0, dt=None, level_set=None):
+        mat = HeatParams(law, 0, rho_cp, k, 0.0, 0.0, 1.0)
+        for f, v in (mushy or {}).items():
+            if f not in dict(HeatParams._fields_[7:]):
+                raise ValueError(f"unknown mushy-zone parameter {f}")
+            setattr(mat, f, float(v))
+        p = HeatCutParams(mat, theta, gamma_A, gamma_M, cell_side_length)
+        h = C.c_void_p()
+        _check(lib().mf_heat_cut_operator_create(scratch._h, C.byref(p), C.byref(h)))
+        super().__init__(scratch, h)
+        if dt is not None:
+            self.reset_time_increment(dt)
+        if level_set is not None:
+            self.set_level_set(level_set)
+
+    def init_time_advance(self, dt):
+        """HeatCutOperator::init_time_advance: the ost factors dt theta and dt (1 - theta)."""
+        self.reset_time_increment(dt)
+
+    def set_level_set(self, phi):
+        """Nodal level set (numpy, n_dofs): classification, sub-boxes, faces, DoFs."""
+        import numpy as np
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        if phi.shape != (self.scratch.n_dofs,):
+            raise ValueError("level set must hold one value per node")
+        _check(lib().mf_heat_cut_set_level_set(self._h, phi.ctypes.data_as(C.c_void_p)))
+
+    def info(self):
+        v = [C.c_int64() for _ in range(5)]
+        _check(lib().mf_heat_cut_info(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("n_liquid", "n_intersected", "n_gas", "n_faces", "n_active_dofs"),
+                        (x.value for x in v)))
+
+    def cells(self):
+        """(category[n_cells], box[n_cells][3][2], active[n_dofs]) host copies."""
+        import numpy as np
+        sc = self.scratch
+        cat = np.zeros(sc.n_cells, dtype=np.int8)
+        box = np.zeros((sc.n_cells, 3, 2))
+        act = np.zeros(sc.n_dofs, dtype=np.uint8)
+        _check(lib().mf_heat_cut_cells(self._h, cat.ctypes.data_as(C.c_void_p),
+                                       box.ctypes.data_as(C.c_void_p),
+                                       act.ctypes.data_as(C.c_void_p)))
+        return cat, box, act
+
+    def compute_cut_norm(self, u):
+        """CutUtil::compute_cut_norm (L2) of a device vector over the liquid region."""
+        r = C.c_double()
+        _check(lib().mf_heat_cut_norm_l2(self._h, self.scratch._v(u), C.byref(r)))
+        return r.value
+
+
+def heat_cut_step(operator, T, T_old, dirichlet_values=None, newton=None, **solver):
+    """One HeatCutOperation::solve (Newton over create_rhs(T_old) and the cut tangent).  The
+    reference's heat linear solver defaults are GMRES with the Diagonal preconditioner."""
+    solver.setdefault("solver", "GMRES")
+    rep = NewtonReport()
+    sc = operator.scratch
+    code = lib().mf_heat_cut_step(operator._h, sc._v(T), sc._v(T_old), sc._v(dirichlet_values),
+                                  C.byref(_solver_desc(**solver)), C.byref(newton_desc(newton)),
+                                  C.byref(rep))
+    _check(code)
+    return rep.as_dict()
+
+
+class HeatCutOperation:
+    """HeatCutOperation (source/heat/heat_cut_operation.cpp) for a fixed interface on the
+    device: set_initial_condition copies T0, init_time_advance rotates the history and takes
+    the old solution as the predictor (:431-455), solve() runs the Newton step (:457-482),
+    compute_L2_norm() is the cut L2 norm (:485-496)."""
+
+    def __init__(self, operator, newton=None, **solver):
+        import torch
+        self.operator = operator
+        n = operator.scratch.n_dofs
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.T = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.T_old = torch.zeros_like(self.T)
+        self.newton = newton
+        self.solver = solver
+        self.last_report = None
+
+    def set_initial_condition(self, T0):
+        import torch
+        self.T.copy_(torch.as_tensor(T0, dtype=torch.float64))
+
+    def init_time_advance(self, dt):
+        self.T_old.copy_(self.T)
+        self.operator.init_time_advance(dt)
+
+    def solve(self):
+        self.last_report = heat_cut_step(self.operator, self.T, self.T_old, newton=self.newton,
+                                         **self.solver)
+        return self.last_report
+
+    def compute_L2_norm(self):
+        return self.operator.compute_cut_norm(self.T)
+
+    def get_temperature(self):
+        return self.T
 
 
 def inflow_outflow_mask(scratch, velocity, sides=(0, 1, 2, 3, 4, 5)):

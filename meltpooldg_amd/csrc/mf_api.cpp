@@ -276,6 +276,15 @@ static int64_t n_rows(const mf_operator *op) { return n_blocks(op) * op->s->mesh
 static int check_ready(const mf_operator *op)
 {
   MF_REQUIRE(op && op->s, MF_ERR_ARG, "null operator");
+  if (op->kind == 5) {
+    MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
+               "time increment not set (reference AssertThrowZeroTimeIncrement)");
+    MF_REQUIRE(cut_ready(op), MF_ERR_STATE, "level set not set (mf_heat_cut_set_level_set)");
+    if (op->cut.material.law == MF_LAW_MUSHY)
+      MF_REQUIRE(op->T, MF_ERR_STATE,
+                 "the mushy-zone law needs T: call mf_operator_set_fields");
+    return MF_OK;
+  }
   if (op->kind == 4) {
     MF_REQUIRE(op->dt > 0.0, MF_ERR_STATE,
                "time increment not set (reference AssertThrowZeroTimeIncrement)");
@@ -338,6 +347,7 @@ struct Carve {
 // vector block) and the boundary faces
 static int op_tangent_blocks(const mf_operator *op, bool fp32)
 {
+  if (op->kind == 5) return CUT_DOT_BLOCKS;
   const MeshInfo &m = op->s->mesh;
   // the level-set projections use the pencil kernel like the transposed operator
   const int cells = tangent_blocks(m, op->transpose != 0 || op->kind >= 2, fp32);
@@ -353,6 +363,14 @@ static int apply_tangent(mf_operator *op, Number *dst, const Number *src, double
 {
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
+  if (op->kind == 5) {
+    if constexpr (sizeof(Number) == 8) {
+      return cut_apply(op, dst, src, dot_partial, stop, overwrite);
+    } else {
+      set_error("the cut heat operator has no FP32 form");
+      return MF_ERR_ARG;
+    }
+  }
   if (n_blocks(op) > 1) {
     // the same scalar operator on every block; the fused dot's partials follow block by block
     const int per = op_tangent_blocks(op, sizeof(Number) == 4) / n_blocks(op);
@@ -683,6 +701,7 @@ int mf_operator_destroy(mf_operator *op)
   if (op->To32) (void)hipFree(op->To32);
   if (op->io_mask) (void)hipFree(op->io_mask);
   if (op->io_union) (void)hipFree(op->io_union);
+  cut_free(op);
   for (auto &ar : op->arena)
     if (ar.p) (void)hipFree(ar.p);
   delete op;
@@ -709,10 +728,17 @@ int mf_operator_set_fields(mf_operator *op, const double *T, const double *vel,
                            const double *k_cell)
 {
   MF_REQUIRE(op, MF_ERR_ARG, "null operator");
-  MF_REQUIRE(op->kind < 2, MF_ERR_ARG,
+  MF_REQUIRE(op->kind < 2 || op->kind == 5, MF_ERR_ARG,
              "the level-set projections take their level set from mf_operator_set_level_set");
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
+  if (op->kind == 5) {
+    // the cut operator: T only, no FP32 copies
+    MF_REQUIRE(!vel && !k_cell, MF_ERR_ARG,
+               "the cut heat operator takes no velocity or conductivity field");
+    op->T = T;
+    return MF_OK;
+  }
   op->T = T;
   op->vel = vel;
   op->k_cell = k_cell;
@@ -834,6 +860,16 @@ int compute_inverse_diagonal(mf_operator *op, double *inv_diag, double *work)
   MF_REQUIRE(inv_diag, MF_ERR_ARG, "null vector");
   const MeshInfo &m = op->s->mesh;
   hipStream_t st = op->s->stream;
+  if (op->kind == 5) {
+    // compute_inverse_diagonal_from_matrixfree (heat_cut_operator.cpp:2141-2153): relative
+    // threshold 1e-14 max(1, |d|_inf)
+    if ((rc = cut_diagonal(op, inv_diag))) return rc;
+    if (!work) {
+      if ((rc = arena_reserve(op->arena[3], sizeof(double) * RED_BLOCKS))) return rc;
+      work = reinterpret_cast<double *>(op->arena[3].p);
+    }
+    return vec_invert_diagonal(inv_diag, m.n_dofs, true, work, st);
+  }
   CellArgs<double> a = make_args<double>(op);
   const int code = op->kind == 0 ? HEAT_DIAG
                  : op->kind == 1 ? ADVEC_DIAG
@@ -950,6 +986,11 @@ int mf_operator_create_rhs(mf_operator *op, double *dst, const double *src, cons
   int rc = check_ready(op);
   if (rc) return rc;
   MF_REQUIRE(dst && src, MF_ERR_ARG, "null vector");
+  if (op->kind == 5) {
+    MF_REQUIRE(op->T, MF_ERR_STATE, "cut create_rhs needs T (mf_operator_set_fields)");
+    MF_REQUIRE(!q_s, MF_ERR_ARG, "the cut heat operator's create_rhs takes no source term");
+    return cut_residual(op, dst, src);
+  }
   if (op->kind == 0) MF_REQUIRE(op->T, MF_ERR_STATE, "heat create_rhs needs T (mf_operator_set_fields)");
   MF_REQUIRE(op->kind == 0 || !q_s, MF_ERR_ARG, "only the heat create_rhs takes a source term");
   if (op->kind >= 2) {
@@ -1405,6 +1446,8 @@ static int check_solve_args(mf_operator *op, const mf_solver_desc *desc, double 
              "unknown solver type");
   MF_REQUIRE(desc->restart >= 0 && desc->restart <= 30, MF_ERR_ARG,
              "GMRES restart (basis size) must lie in 1..30 (0 = 30)");
+  MF_REQUIRE(op->kind != 5 || desc->precision == MF_PRECISION_FP64, MF_ERR_ARG,
+             "the cut heat operator has no FP32 form (precision must be MF_PRECISION_FP64)");
   return MF_OK;
 }
 

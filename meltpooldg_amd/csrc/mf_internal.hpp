@@ -326,9 +326,16 @@ struct Arena {
 };
 }  // namespace mf
 
+namespace mf {
+struct CutData;  // one-phase CutFEM heat operator (mf_cut.hip)
+}
+
 struct mf_operator {
   mf::Arena arena[4];  // 0: refinement vectors, 1: Krylov vectors, 2: inverse diagonal, 3: small work
-  int kind = 0;  // 0 heat, 1 advection, 2 normal vector (3 blocks), 3 curvature, 4 reinit
+  // 0 heat, 1 advection, 2 normal vector (3 blocks), 3 curvature, 4 reinit, 5 CutFEM heat
+  int kind = 0;
+  mf_heat_cut_params cut{};            // kind 5
+  mf::CutData *cutd = nullptr;         // kind 5: classification and its device tables
   mf_reinit_params reinit{};           // kind 4
   mf_levelset_projection_params ls{};  // kinds 2, 3
   double ls_tol = 0.0;                 // curvature: nodal normalisation tolerance
@@ -369,6 +376,18 @@ inline void reinit_epsilon(const mf_operator *op, double &eps_n, double &eps_t, 
   }
   eps_t = r.tangential_diffusion_factor * eps_n;
 }
+
+// One-phase CutFEM heat operator (mf_cut.hip).  cut_apply: dst = J src (overwrite) or
+// dst += J src, with CUT_DOT_BLOCKS partials of src . (J src) when dot_partial is given;
+// cut_diagonal: the diagonal of J, 1 on the rows without a free DoF; cut_residual: dst += -R(T)
+// with T = op->T and T_old; cut_ready: the level set is set; cut_free: the device tables.
+constexpr int CUT_DOT_BLOCKS = 256;
+int cut_apply(mf_operator *op, double *dst, const double *src, double *dot_partial,
+              const int *stop, bool overwrite);
+int cut_diagonal(mf_operator *op, double *diag);
+int cut_residual(mf_operator *op, double *dst, const double *T_old);
+bool cut_ready(const mf_operator *op);
+void cut_free(mf_operator *op);
 
 // mf_operator_compute_inverse_diagonal with the RED_BLOCKS doubles of the reduction partials
 // given by the caller (work = nullptr: the operator's own arena)

@@ -30,6 +30,8 @@ template <typename T> int vec_fill(T *x, T v, int64_t n, hipStream_t st, const i
 // dst[i] = mask[i] ? (vals ? vals[i] : 0) : (keep_free ? dst[i] : 0)   (mf_ops.hip)
 int vec_masked_assign(double *dst, const uint8_t *mask, const double *vals, bool keep_free,
                       int64_t n, hipStream_t st);
+// y = a x + b y   (mf_ops.hip)
+int vec_axpby(double a, const double *x, double b, double *y, int64_t n, hipStream_t st);
 // dst[i] = value where mask[i]   (mf_ops.hip)
 int vec_masked_fill(double *dst, const uint8_t *mask, double value, int64_t n, hipStream_t st);
 template <typename Ta, typename Tb> int vec_cast(const Ta *a, Tb *b, double scale, int64_t n, hipStream_t st);
