@@ -64,6 +64,10 @@ def main():
                     help="skip the level-set advection solve section")
     ap.add_argument("--no-heat-step", action="store_true",
                     help="skip the whole-time-step section (device heat stepper)")
+    ap.add_argument("--no-cut", action="store_true",
+                    help="skip the one-phase CutFEM heat operator section")
+    ap.add_argument("--cut-only", action="store_true",
+                    help="run only the CutFEM section (for its kernel-trace profile)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the HBM traffic measurement (two rocprofv3 --pmc child runs)")
     ap.add_argument("--pmc-workload", action="store_true",
@@ -81,6 +85,9 @@ def main():
 
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    if args.cut_only:
+        print(json.dumps(cut_case(mf, torch, args.steps, None)), flush=True)
+        return
     s, op, T, T_dev, mask, b = build_case(mf, torch)
     n = s.n_dofs
     src = torch.tensor(b, dtype=torch.float64, device="cuda")
@@ -167,6 +174,8 @@ def main():
         extra["levelset_geometry"] = levelset_geometry_case(mf, torch, stream)
     if not args.no_heat_step:
         extra["heat_step"] = heat_step_case(mf, torch, op, T_dev)
+    if not args.no_cut:
+        extra["cut"] = cut_case(mf, torch, args.steps, None if args.no_cpu else args.cpu_budget)
 
     traffic = None if args.no_traffic else measure_traffic()
     if traffic:
@@ -243,6 +252,88 @@ def _time_vmult(torch, op, x, y, reps):
         op.vmult(y, x)
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / reps
+
+
+def cut_case(mf, torch, steps, cpu_budget):
+    """The one-phase CutFEM heat operator (FE_Q1) on 192^3 cells of the unit cube (7,077,888
+    cells, 7,189,057 DoFs of the lattice), level set 0.5003 - z (liquid below: the cut layer is
+    cell layer 96), steel-like constants of the reference's one_phase_cut case (rho cp = 7850 *
+    460, k = 55.563), dt = 1e-2, theta 0.5, gamma_A 1.5, gamma_M 0.75.  Reported: the FP64
+    application dst = J src (HIP events on the library's stream) with its algorithmic bytes
+    (src read and dst written once per DoF that exists, the 1-byte node flag of every node),
+    one time step of HeatCutOperation (Newton, Jacobi-GMRES and -CG to 1e-10), and the
+    oracle's serial vmult on a 64^3 sample of the same case."""
+    n = 192
+    s = mf.ScratchData(1, n, stream=torch.cuda.current_stream())
+    X = s.support_points()
+    phi = 0.5003 - X[:, 2]
+    cv, k, dt = 7850.0 * 460.0, 55.563, 1e-2
+    op = mf.HeatCutOperator(s, rho_cp=cv, k=k, theta=0.5, gamma_A=1.5, gamma_M=0.75, dt=dt,
+                            level_set=phi)
+    info = op.info()
+    u = torch.tensor(s.random_field(5), dtype=torch.float64, device="cuda")
+    y = torch.empty_like(u)
+    for _ in range(5):
+        op.vmult(y, u)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        op.vmult(y, u)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    alg = 16 * info["n_active_dofs"] + s.n_dofs
+    gbs = alg / (ms * 1e-3) / 1e9
+    # one time step (init_time_advance + solve) from T0 = x + y, GMRES (the reference default)
+    # and CG, Jacobi, rel 1e-10
+    T0 = torch.tensor(X[:, 0] + X[:, 1], dtype=torch.float64, device="cuda")
+    step = {}
+    for solver in ("GMRES", "CG"):
+        heat = mf.HeatCutOperation(op, newton=dict(residual_tolerance=1e-6), solver=solver,
+                                   rel_tolerance=1e-10, abs_tolerance=1e-30)
+        heat.set_initial_condition(T0)
+        heat.init_time_advance(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = heat.solve()
+        torch.cuda.synchronize()
+        step[solver] = {"s": time.perf_counter() - t0, "newton": rep["iterations"],
+                        "linear_iterations": rep["linear_iterations"]}
+    out = {"workload": "one-phase CutFEM heat tangent, FE_Q1/QGauss(2), 192^3 cells of [0,1]^3, "
+                       "interface z = 0.5003, constant law",
+           "n_dofs": s.n_dofs, "n_active_dofs": info["n_active_dofs"],
+           "n_intersected": info["n_intersected"], "n_faces": info["n_faces"],
+           "vmult_us": ms * 1e3, "dofs_per_s": info["n_active_dofs"] / (ms * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "alg_bytes_per_vmult": alg},
+           "time_step": step}
+    print(f"# cut: {info['n_active_dofs']} active DoFs, vmult {ms*1e3:.1f} us, {gbs:.0f} GB/s; "
+          f"step GMRES {step['GMRES']['s']*1e3:.1f} ms ({step['GMRES']['linear_iterations']} it), "
+          f"CG {step['CG']['s']*1e3:.1f} ms ({step['CG']['linear_iterations']} it)",
+          file=sys.stderr, flush=True)
+    if cpu_budget:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        m = orc.make_mesh(3, 1, 64)
+        Xs = orc.node_coords(m)
+        p = orc.cut_params(dt, rho_cp=cv, k=k, theta=0.5, gamma_A=1.5, gamma_M=0.75, mesh=m)
+        ref = orc.CutOracle(m, 0.5003 - Xs[:, 2], p)
+        us = orc.random_field(m, 5)
+        ref.vmult(us)
+        times = []
+        t_end = time.perf_counter() + min(cpu_budget, 10.0)
+        while len(times) < 5 or (time.perf_counter() < t_end and len(times) < 50):
+            t0 = time.perf_counter()
+            ref.vmult(us)
+            times.append(time.perf_counter() - t0)
+        tc = float(np.median(times))
+        out["cpu_baseline"] = {"value": int(ref.active.sum()) / tc, "unit": "DoF/s", "cores": 1,
+                               "kind": "port", "cpu_model": cpu_model(),
+                               "sample": f"median of {len(times)} serial oracle vmults of the "
+                                         f"same case on 64^3 cells, {tc*1e3:.1f} ms each"}
+    return out
 
 
 def heat_step_case(mf, torch, op, T_dev):

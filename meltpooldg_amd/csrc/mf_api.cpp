@@ -347,7 +347,7 @@ struct Carve {
 // vector block) and the boundary faces
 static int op_tangent_blocks(const mf_operator *op, bool fp32)
 {
-  if (op->kind == 5) return CUT_DOT_BLOCKS;
+  if (op->kind == 5) return cut_dot_blocks(op);
   const MeshInfo &m = op->s->mesh;
   // the level-set projections use the pencil kernel like the transposed operator
   const int cells = tangent_blocks(m, op->transpose != 0 || op->kind >= 2, fp32);

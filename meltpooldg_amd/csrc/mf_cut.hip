@@ -38,9 +38,16 @@ struct CutData {
   std::vector<uint8_t> active;
   int32_t *d_cell = nullptr;      // [n_cells] CELL_LIQUID, CELL_GAS or the box index
   double *d_box = nullptr;        // [n_cut][6] liquid sub-box in reference coordinates
+  double *d_mats = nullptr;       // [n_cut][3][4] their 1D matrices (m00, m01, m11, kk)
+  double *d_faceG = nullptr;      // [n_cut][6][4] work: ghost-penalty face values of one apply
   int64_t *d_face = nullptr;      // [n_faces] (lower cell << 2) | direction
   uint8_t *d_inactive = nullptr;  // [n_dofs] 1 = no DoF (only gas cells around the node)
   double *d_part = nullptr;       // [CUT_NORM_BLOCKS] norm partials
+  // node kernel flags (NODE_*), built with the classification on first use
+  uint8_t *d_nflag = nullptr;
+  int64_t *d_glist = nullptr;     // the nodes with a non-zero flag
+  int64_t n_glist = 0;
+  std::vector<int64_t> faces;     // host copy of the ghost-penalty faces
 };
 
 namespace {
@@ -60,7 +67,25 @@ struct CutArgs {
   const double *T;         // linearisation temperature (tangent, diagonal), T^{n+1} (residual)
   const int *stop;
   int colour;
+  const uint8_t *nflag;    // node kernel: NODE_GENERAL / NODE_INACTIVE per node
+  // per direction: h/6, h/3, 1/h and gamma times the JxW of a face point (0.25 h_d1 h_d2)
+  double h6[3], h3[3], ih[3], wq[3];
+  const double *mats;      // [n_cut][3][4] the 1D matrices of the intersected cells
+  double *faceG;           // [n_cut][6][4] per face of an intersected cell: G_c of the face
+  uint64_t mg_nx, mg_nxy;  // ceil(2^64 / d) of the node counts per x line and per xy plane
 };
+
+// n / d for n < 2^32 by multiply-high with m = ceil(2^64 / d), 2 <= d < 2^32
+__device__ __forceinline__ uint32_t div_magic(uint32_t n, uint64_t m)
+{
+  return uint32_t(__umul64hi(uint64_t(n), m));
+}
+
+// node flags of the constant-law node kernel: 0 = the node, all its 27 neighbours and its 8
+// cells are free and liquid and it is no ghost-penalty node (the uniform stencil applies)
+constexpr uint8_t NODE_GENERAL = 1, NODE_INACTIVE = 2;
+// a node without a DoF that the node kernel treats itself (not in the general list)
+constexpr uint8_t NODE_INACTIVE_ONLY = NODE_INACTIVE;
 
 __device__ __forceinline__ bool fixed_row(const CutArgs &A, int64_t n)
 {
@@ -165,64 +190,142 @@ __device__ __forceinline__ void eval_q1(const Box1D (&B)[3], const double (&u)[2
     for (int qx = 0; qx < 2; ++qx) o.gz[qy][qx] = B[2].D[0] * vy[0][qy][qx] + B[2].D[1] * vy[1][qy][qx];
 }
 
-// out[ez][ey][ex] = sum_q fv phi + fx d_x phi + fy d_y phi + fz d_z phi (the transposed sweeps)
-__device__ __forceinline__ void integrate_q1(const Box1D (&B)[3], const double (&fv)[2][2][2],
-                                             const double (&fx)[2][2][2],
-                                             const double (&fy)[2][2][2],
-                                             const double (&fz)[2][2][2], double (&out)[2][2][2])
-{
-  double Vz[2][2][2], Xz[2][2][2], Yz[2][2][2];
-#pragma unroll
-  for (int ez = 0; ez < 2; ++ez)
-#pragma unroll
-    for (int qy = 0; qy < 2; ++qy)
-#pragma unroll
-      for (int qx = 0; qx < 2; ++qx) {
-        Vz[ez][qy][qx] = B[2].S[0][ez] * fv[0][qy][qx] + B[2].S[1][ez] * fv[1][qy][qx] +
-                         B[2].D[ez] * (fz[0][qy][qx] + fz[1][qy][qx]);
-        Xz[ez][qy][qx] = B[2].S[0][ez] * fx[0][qy][qx] + B[2].S[1][ez] * fx[1][qy][qx];
-        Yz[ez][qy][qx] = B[2].S[0][ez] * fy[0][qy][qx] + B[2].S[1][ez] * fy[1][qy][qx];
-      }
-  double Vy[2][2][2], Xy[2][2][2];
-#pragma unroll
-  for (int ez = 0; ez < 2; ++ez)
-#pragma unroll
-    for (int ey = 0; ey < 2; ++ey)
-#pragma unroll
-      for (int qx = 0; qx < 2; ++qx) {
-        Vy[ez][ey][qx] = B[1].S[0][ey] * Vz[ez][0][qx] + B[1].S[1][ey] * Vz[ez][1][qx] +
-                         B[1].D[ey] * (Yz[ez][0][qx] + Yz[ez][1][qx]);
-        Xy[ez][ey][qx] = B[1].S[0][ey] * Xz[ez][0][qx] + B[1].S[1][ey] * Xz[ez][1][qx];
-      }
-#pragma unroll
-  for (int ez = 0; ez < 2; ++ez)
-#pragma unroll
-    for (int ey = 0; ey < 2; ++ey)
-#pragma unroll
-      for (int ex = 0; ex < 2; ++ex)
-        out[ez][ey][ex] = B[0].S[0][ex] * Vy[ez][ey][0] + B[0].S[1][ex] * Vy[ez][ey][1] +
-                          B[0].D[ex] * (Xy[ez][ey][0] + Xy[ez][ey][1]);
-}
-
 enum { MODE_TANGENT = 0, MODE_DIAG = 1, MODE_RESIDUAL = 2 };
 
 // The local operation of one liquid or intersected cell.  MODE_TANGENT: tangent_domain_integral
 // (heat_cut_operator.cpp:157-190, no velocity) applied to u; MODE_DIAG: the diagonal of that
 // element matrix; MODE_RESIDUAL: residual_domain_integral (:194-223) of T^{n+1} = t and T^n = u.
+// The 4 quadrature points of the plane qz from the z-interpolated nodal values uz[ey][ex] and
+// the z-derivatives dz[ey][ex] (the same on both planes): v[qy][qx], gx[qy], gy[qx], gz[qy][qx].
+struct PlaneVals {
+  double v[2][2], gx[2], gy[2], gz[2][2];
+};
+
+__device__ __forceinline__ void eval_plane(const Box1D (&B)[3], const double (&uz)[2][2],
+                                           const double (&dz)[2][2], PlaneVals &o)
+{
+  double uy[2][2], dzy[2][2], dy[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+      uy[q][ex] = B[1].S[q][0] * uz[0][ex] + B[1].S[q][1] * uz[1][ex];
+      dzy[q][ex] = B[1].S[q][0] * dz[0][ex] + B[1].S[q][1] * dz[1][ex];
+    }
+#pragma unroll
+  for (int ex = 0; ex < 2; ++ex) dy[ex] = B[1].D[0] * uz[0][ex] + B[1].D[1] * uz[1][ex];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int qx = 0; qx < 2; ++qx) {
+      o.v[q][qx] = B[0].S[qx][0] * uy[q][0] + B[0].S[qx][1] * uy[q][1];
+      o.gz[q][qx] = B[0].S[qx][0] * dzy[q][0] + B[0].S[qx][1] * dzy[q][1];
+    }
+    o.gx[q] = B[0].D[0] * uy[q][0] + B[0].D[1] * uy[q][1];
+    o.gy[q] = B[0].S[q][0] * dy[0] + B[0].S[q][1] * dy[1];
+  }
+}
+
+// out[ez][ey][ex] += the plane qz's part of sum_q fv phi + f . grad phi (transposed sweeps)
+__device__ __forceinline__ void integrate_plane(const Box1D (&B)[3], int qz,
+                                                const double (&fv)[2][2],
+                                                const double (&fx)[2][2],
+                                                const double (&fy)[2][2],
+                                                const double (&fz)[2][2], double (&out)[2][2][2])
+{
+  double Vx[2][2], Yx[2][2], Zx[2][2];
+#pragma unroll
+  for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+      Vx[qy][ex] = B[0].S[0][ex] * fv[qy][0] + B[0].S[1][ex] * fv[qy][1] +
+                   B[0].D[ex] * (fx[qy][0] + fx[qy][1]);
+      Yx[qy][ex] = B[0].S[0][ex] * fy[qy][0] + B[0].S[1][ex] * fy[qy][1];
+      Zx[qy][ex] = B[0].S[0][ex] * fz[qy][0] + B[0].S[1][ex] * fz[qy][1];
+    }
+#pragma unroll
+  for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+    for (int ex = 0; ex < 2; ++ex) {
+      const double V = B[1].S[0][ey] * Vx[0][ex] + B[1].S[1][ey] * Vx[1][ex] +
+                       B[1].D[ey] * (Yx[0][ex] + Yx[1][ex]);
+      const double Z = B[1].S[0][ey] * Zx[0][ex] + B[1].S[1][ey] * Zx[1][ex];
+#pragma unroll
+      for (int ez = 0; ez < 2; ++ez) out[ez][ey][ex] += B[2].S[qz][ez] * V + B[2].D[ez] * Z;
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void cut_cell_local(const CutArgs &A, const Box1D (&B)[3],
                                                const double (&u)[2][2][2],
                                                const double (&t)[2][2][2],
                                                double (&out)[2][2][2])
 {
-  QVals U, Tq;
-  if (MODE != MODE_DIAG) eval_q1(B, u, U);
-  if (A.mushy || MODE == MODE_RESIDUAL) eval_q1(B, t, Tq);
-  if (MODE == MODE_DIAG) {
+  const bool needT = A.mushy || MODE == MODE_RESIDUAL;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) out[e >> 2][(e >> 1) & 1][e & 1] = 0.0;
+  for (int e = 0; e < 8; ++e) out[e >> 2][(e >> 1) & 1][e & 1] = 0.0;
+  if (MODE != MODE_DIAG) {
+    // one plane of quadrature points at a time: few values live at once
+    double du[2][2], dt[2][2];
+#pragma unroll
+    for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+      for (int ex = 0; ex < 2; ++ex) {
+        du[ey][ex] = B[2].D[0] * u[0][ey][ex] + B[2].D[1] * u[1][ey][ex];
+        dt[ey][ex] = needT ? B[2].D[0] * t[0][ey][ex] + B[2].D[1] * t[1][ey][ex] : 0.0;
+      }
+#pragma unroll
+    for (int qz = 0; qz < 2; ++qz) {
+      double uz[2][2], tz[2][2];
+#pragma unroll
+      for (int ey = 0; ey < 2; ++ey)
+#pragma unroll
+        for (int ex = 0; ex < 2; ++ex) {
+          uz[ey][ex] = B[2].S[qz][0] * u[0][ey][ex] + B[2].S[qz][1] * u[1][ey][ex];
+          tz[ey][ex] = needT ? B[2].S[qz][0] * t[0][ey][ex] + B[2].S[qz][1] * t[1][ey][ex] : 0.0;
+        }
+      PlaneVals U, Tp;
+      eval_plane(B, uz, du, U);
+      if (needT) eval_plane(B, tz, dt, Tp);
+      double fv[2][2], fx[2][2], fy[2][2], fz[2][2];
+#pragma unroll
+      for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+        for (int qx = 0; qx < 2; ++qx) {
+          const double jxw = B[0].w[qx] * B[1].w[qy] * B[2].w[qz];
+          double cv = A.cv, k = A.k, dcv = 0.0, dk = 0.0;
+          double tq = 0.0, gT[3] = {0.0, 0.0, 0.0};
+          if (needT) {
+            tq = Tp.v[qy][qx];
+            gT[0] = Tp.gx[qy];
+            gT[1] = Tp.gy[qx];
+            gT[2] = Tp.gz[qy][qx];
+            if (A.mushy) cut_law(A, tq, cv, k, dcv, dk);
+          }
+          const double uq = U.v[qy][qx];
+          const double g[3] = {U.gx[qy], U.gy[qx], U.gz[qy][qx]};
+          if (MODE == MODE_TANGENT) {
+            fv[qy][qx] = (cv + dcv * tq) * uq * jxw;
+            const double a = A.ost_i * k * jxw, b = A.ost_i * dk * uq * jxw;
+            fx[qy][qx] = a * g[0] + b * gT[0];
+            fy[qy][qx] = a * g[1] + b * gT[1];
+            fz[qy][qx] = a * g[2] + b * gT[2];
+          } else {
+            // residual: u holds T^n; its material is the constant law or the law at T^n
+            double cvo = A.cv, ko = A.k, d1, d2;
+            if (A.mushy) cut_law(A, uq, cvo, ko, d1, d2);
+            fv[qy][qx] = -(cv * tq - cvo * uq) * jxw;
+            fx[qy][qx] = -(A.ost_i * k * gT[0] + A.ost_e * ko * g[0]) * jxw;
+            fy[qy][qx] = -(A.ost_i * k * gT[1] + A.ost_e * ko * g[1]) * jxw;
+            fz[qy][qx] = -(A.ost_i * k * gT[2] + A.ost_e * ko * g[2]) * jxw;
+          }
+        }
+      integrate_plane(B, qz, fv, fx, fy, fz, out);
+    }
+    return;
   }
-  double fv[2][2][2], fx[2][2][2], fy[2][2][2], fz[2][2][2];
+  QVals Tq;
+  if (needT) eval_q1(B, t, Tq);
 #pragma unroll
   for (int qz = 0; qz < 2; ++qz)
 #pragma unroll
@@ -232,32 +335,14 @@ __device__ __forceinline__ void cut_cell_local(const CutArgs &A, const Box1D (&B
         const double jxw = B[0].w[qx] * B[1].w[qy] * B[2].w[qz];
         double cv = A.cv, k = A.k, dcv = 0.0, dk = 0.0;
         double tq = 0.0, gT[3] = {0.0, 0.0, 0.0};
-        if (A.mushy || MODE == MODE_RESIDUAL) {
+        if (needT) {
           tq = Tq.v[qz][qy][qx];
           gT[0] = Tq.gx[qz][qy];
           gT[1] = Tq.gy[qz][qx];
           gT[2] = Tq.gz[qy][qx];
           if (A.mushy) cut_law(A, tq, cv, k, dcv, dk);
         }
-        if (MODE == MODE_TANGENT) {
-          const double uq = U.v[qz][qy][qx];
-          const double g[3] = {U.gx[qz][qy], U.gy[qz][qx], U.gz[qy][qx]};
-          fv[qz][qy][qx] = (cv + dcv * tq) * uq * jxw;
-          const double a = A.ost_i * k * jxw, b = A.ost_i * dk * uq * jxw;
-          fx[qz][qy][qx] = a * g[0] + b * gT[0];
-          fy[qz][qy][qx] = a * g[1] + b * gT[1];
-          fz[qz][qy][qx] = a * g[2] + b * gT[2];
-        } else if (MODE == MODE_RESIDUAL) {
-          // T^n at the point and its material (the constant law or the law at T^n)
-          const double to = U.v[qz][qy][qx];
-          const double go[3] = {U.gx[qz][qy], U.gy[qz][qx], U.gz[qy][qx]};
-          double cvo = A.cv, ko = A.k, d1, d2;
-          if (A.mushy) cut_law(A, to, cvo, ko, d1, d2);
-          fv[qz][qy][qx] = -(cv * tq - cvo * to) * jxw;
-          fx[qz][qy][qx] = -(A.ost_i * k * gT[0] + A.ost_e * ko * go[0]) * jxw;
-          fy[qz][qy][qx] = -(A.ost_i * k * gT[1] + A.ost_e * ko * go[1]) * jxw;
-          fz[qz][qy][qx] = -(A.ost_i * k * gT[2] + A.ost_e * ko * go[2]) * jxw;
-        } else {
+        {
           // diagonal: (cv + dcv T) phi_e^2 + dt theta (k |grad phi_e|^2 + dk (grad T . grad phi_e) phi_e)
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
@@ -273,7 +358,6 @@ __device__ __forceinline__ void cut_cell_local(const CutArgs &A, const Box1D (&B
           }
         }
       }
-  if (MODE != MODE_DIAG) integrate_q1(B, fv, fx, fy, fz, out);
 }
 
 __device__ __forceinline__ void cell_boxes(const CutArgs &A, int32_t cc, Box1D (&B)[3])
@@ -288,9 +372,125 @@ __device__ __forceinline__ void cell_boxes(const CutArgs &A, int32_t cc, Box1D (
   }
 }
 
+// Constant law: the element matrix of a liquid or intersected cell is exactly
+//   cv Mx(x)My(x)Mz + dt theta k (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz)
+// with the 1D matrices of the cell's liquid interval [a, b] (reference coordinates):
+// M_ij = h int_a^b l_i l_j dt, K_ij = (b - a) / h (-1)^(i+j) (l_0 = 1 - t, l_1 = t).  QGauss(2) on
+// [a, b] integrates these degree-2 products exactly, so this is the quadrature of the reference
+// in closed form.
+struct Mat1D {
+  double m00, m01, m11, kk;
+};
+
+__device__ __forceinline__ void cell_mats(const CutArgs &A, int32_t cc, Mat1D (&M)[3])
+{
+  if (cc >= 0) {
+    const double *t = A.mats + 12 * int64_t(cc);  // computed with the classification
+#pragma unroll
+    for (int d = 0; d < 3; ++d) M[d] = Mat1D{t[4 * d], t[4 * d + 1], t[4 * d + 2], t[4 * d + 3]};
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) M[d] = Mat1D{A.h3[d], A.h6[d], A.h3[d], A.ih[d]};
+  }
+}
+
+// 2x2 applications along one direction of a [2][2][2] array (index [ez][ey][ex])
+template <int DIR>
+__device__ __forceinline__ void apply_m(const Mat1D &m, const double (&in)[2][2][2],
+                                        double (&out)[2][2][2])
+{
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int i0 = p >> 1, i1 = p & 1;  // the two other indices
+    auto at = [&](const double (&x)[2][2][2], int e) -> double {
+      return DIR == 0 ? x[i0][i1][e] : (DIR == 1 ? x[i0][e][i1] : x[e][i0][i1]);
+    };
+    auto put = [&](double (&x)[2][2][2], int e, double v) {
+      if (DIR == 0) x[i0][i1][e] = v; else if (DIR == 1) x[i0][e][i1] = v; else x[e][i0][i1] = v;
+    };
+    const double v0 = at(in, 0), v1 = at(in, 1);
+    put(out, 0, m.m00 * v0 + m.m01 * v1);
+    put(out, 1, m.m01 * v0 + m.m11 * v1);
+  }
+}
+
+template <int DIR>
+__device__ __forceinline__ void apply_k(const Mat1D &m, const double (&in)[2][2][2],
+                                        double (&out)[2][2][2])
+{
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int i0 = p >> 1, i1 = p & 1;
+    auto at = [&](const double (&x)[2][2][2], int e) -> double {
+      return DIR == 0 ? x[i0][i1][e] : (DIR == 1 ? x[i0][e][i1] : x[e][i0][i1]);
+    };
+    auto put = [&](double (&x)[2][2][2], int e, double v) {
+      if (DIR == 0) x[i0][i1][e] = v; else if (DIR == 1) x[i0][e][i1] = v; else x[e][i0][i1] = v;
+    };
+    const double d = m.kk * (at(in, 0) - at(in, 1));
+    put(out, 0, d);
+    put(out, 1, -d);
+  }
+}
+
+// out = Mx(x)My(x)Mz p + (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz) q
+__device__ __forceinline__ void tensor_apply(const Mat1D (&M)[3], const double (&p)[2][2][2],
+                                             const double (&q)[2][2][2], double (&out)[2][2][2])
+{
+  double P[2][2][2], Q[2][2][2], R[2][2][2];
+  apply_m<2>(M[2], p, P);
+  apply_m<2>(M[2], q, Q);
+  apply_k<2>(M[2], q, R);
+  double P2[2][2][2], Q2[2][2][2], S2[2][2][2], T2[2][2][2];
+  apply_m<1>(M[1], P, P2);
+  apply_m<1>(M[1], Q, Q2);
+  apply_k<1>(M[1], Q, S2);
+  apply_m<1>(M[1], R, T2);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int a = e >> 2, b = (e >> 1) & 1, c = e & 1;
+    P2[a][b][c] += S2[a][b][c] + T2[a][b][c];
+  }
+  double X[2][2][2];
+  apply_m<0>(M[0], P2, out);
+  apply_k<0>(M[0], Q2, X);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int a = e >> 2, b = (e >> 1) & 1, c = e & 1;
+    out[a][b][c] += X[a][b][c];
+  }
+}
+
+// out = (cv Mx(x)My(x)Mz + c (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz)) u, few values live
+__device__ __forceinline__ void tensor_apply_1(const Mat1D (&M)[3], double cv, double c,
+                                               const double (&u)[2][2][2], double (&out)[2][2][2])
+{
+  double P[2][2][2], R[2][2][2];
+  apply_m<2>(M[2], u, P);
+  apply_k<2>(M[2], u, R);
+  double P2[2][2][2], S2[2][2][2], T2[2][2][2];
+  apply_m<1>(M[1], P, P2);
+  apply_k<1>(M[1], P, S2);
+  apply_m<1>(M[1], R, T2);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int a = e >> 2, b = (e >> 1) & 1, x = e & 1;
+    S2[a][b][x] = cv * P2[a][b][x] + c * (S2[a][b][x] + T2[a][b][x]);
+  }
+  double X[2][2][2];
+  apply_m<0>(M[0], S2, out);
+  apply_k<0>(M[0], P2, X);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int a = e >> 2, b = (e >> 1) & 1, x = e & 1;
+    out[a][b][x] += c * X[a][b][x];
+  }
+}
+
 // One launch per (y, z) row-parity colour; one workgroup per cell row, CUT_THREADS cells per
 // step.  dst += the cell contributions of the rows of this colour (rows skipped: fixed rows).
-template <int MODE>
+// MUSHY: the T-dependent law on the quadrature points; otherwise the closed-form tensor path.
+template <int MODE, bool MUSHY>
 __global__ __launch_bounds__(CUT_THREADS) void cut_cell_kernel(CutArgs A, const double *src,
                                                                double *dst)
 {
@@ -305,6 +505,7 @@ __global__ __launch_bounds__(CUT_THREADS) void cut_cell_kernel(CutArgs A, const 
 #pragma unroll
   for (int r = 0; r < 4; ++r) base[r] = int64_t(cz + (r >> 1)) * A.NXY + int64_t(cy + (r & 1)) * A.NX;
   const int tid = threadIdx.x;
+  const bool needT = MUSHY || MODE == MODE_RESIDUAL;
   for (int x0 = 0; x0 < ncx; x0 += CUT_THREADS) {
     const int i = x0 + tid;
     double out[2][2][2];
@@ -313,8 +514,6 @@ __global__ __launch_bounds__(CUT_THREADS) void cut_cell_kernel(CutArgs A, const 
     if (i < ncx) {
       const int32_t cc = A.cell[(int64_t(cz) * A.ncell[1] + cy) * ncx + i];
       if (cc != CELL_GAS) {
-        Box1D B[3];
-        cell_boxes(A, cc, B);
         double u[2][2][2], t[2][2][2];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -324,12 +523,42 @@ __global__ __launch_bounds__(CUT_THREADS) void cut_cell_kernel(CutArgs A, const 
             double uv = 0.0, tv = 0.0;
             if (MODE == MODE_TANGENT) uv = fixed_row(A, n + ex) ? 0.0 : src[n + ex];
             if (MODE == MODE_RESIDUAL) uv = src[n + ex];  // T^n, read plain
-            if (A.mushy || MODE == MODE_RESIDUAL) tv = A.T[n + ex];
+            if (needT) tv = A.T[n + ex];
             u[r >> 1][r & 1][ex] = uv;
             t[r >> 1][r & 1][ex] = tv;
           }
         }
-        cut_cell_local<MODE>(A, B, u, t, out);
+        if constexpr (MUSHY) {
+          Box1D B[3];
+          cell_boxes(A, cc, B);
+          cut_cell_local<MODE>(A, B, u, t, out);
+        } else {
+          Mat1D M[3];
+          cell_mats(A, cc, M);
+          const double c = A.ost_i * A.k;
+          if (MODE == MODE_TANGENT) {
+            tensor_apply_1(M, A.cv, c, u, out);
+          } else if (MODE == MODE_RESIDUAL) {
+            // -R = -(cv M (T - T^n) + k K (dt theta T + dt (1 - theta) T^n))
+            double p[2][2][2], q[2][2][2];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const int a = e >> 2, b = (e >> 1) & 1, x = e & 1;
+              p[a][b][x] = -A.cv * (t[a][b][x] - u[a][b][x]);
+              q[a][b][x] = -A.k * (A.ost_i * t[a][b][x] + A.ost_e * u[a][b][x]);
+            }
+            tensor_apply(M, p, q, out);
+          } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const int a = e >> 2, b = (e >> 1) & 1, x = e & 1;
+              const double mx = x ? M[0].m11 : M[0].m00, my = b ? M[1].m11 : M[1].m00,
+                           mz = a ? M[2].m11 : M[2].m00;
+              out[a][b][x] = A.cv * mx * my * mz +
+                             c * (M[0].kk * my * mz + mx * M[1].kk * mz + mx * my * M[2].kk);
+            }
+          }
+        }
       }
     }
     // the right nodes of each cell go to its x-neighbour
@@ -374,7 +603,8 @@ __global__ __launch_bounds__(256) void cut_face_kernel(CutArgs A, const int64_t 
   const int d = int(code & 3);
   const int64_t cell = code >> 2;
   const int64_t ncx = A.ncell[0], ncy = A.ncell[1];
-  const int64_t ci = cell % ncx, cj = (cell / ncx) % ncy, ck = cell / (ncx * ncy);
+  const uint32_t c32 = uint32_t(cell), row = c32 / uint32_t(ncx);
+  const int64_t ci = c32 - row * uint32_t(ncx), cj = row % uint32_t(ncy), ck = row / uint32_t(ncy);
   const int64_t n0 = ck * A.NXY + cj * A.NX + ci;
   const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
   const int64_t stride[3] = {1, A.NX, A.NXY};
@@ -458,7 +688,8 @@ __global__ __launch_bounds__(256) void cut_norm_kernel(CutArgs A, const double *
     if (cc == CELL_GAS) continue;
     Box1D B[3];
     cell_boxes(A, cc, B);
-    const int64_t ci = cell % ncx, cj = (cell / ncx) % ncy, ck = cell / (ncx * ncy);
+    const uint32_t c32 = uint32_t(cell), row = c32 / uint32_t(ncx);
+    const int64_t ci = c32 - row * uint32_t(ncx), cj = row % uint32_t(ncy), ck = row / uint32_t(ncy);
     const int64_t n0 = ck * A.NXY + cj * A.NX + ci;
     double v[2][2][2];
 #pragma unroll
@@ -478,7 +709,7 @@ __global__ __launch_bounds__(256) void cut_norm_kernel(CutArgs A, const double *
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// partials of a . b for the fused p.(Ap) of CG (CUT_DOT_BLOCKS blocks)
+// partials of a . b for the fused p.(Ap) of CG (one per block)
 __global__ __launch_bounds__(256) void cut_dot_kernel(const double *a, const double *b, int64_t n,
                                                       double *part, const int *stop)
 {
@@ -489,6 +720,392 @@ __global__ __launch_bounds__(256) void cut_dot_kernel(const double *a, const dou
     s += a[i] * b[i];
   const double t = block_sum_256(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ------------------------------------------------------------------------------------------
+// Constant law: one thread per node, each node written once (no colours, no atomics).
+// A node's row of J is the sum over its 8 cells of that cell's row, plus the ghost-penalty
+// faces whose 12 nodes contain it.  Nodes whose 8 cells are liquid and whose neighbourhood is
+// free (flag 0) take the separable 27-point stencil of the uniform mesh:
+//   cv Sx Sy Sz + c (Dx Sy Sz + Sx Dy Sz + Sx Sy Dz),  S = (h/6, 2h/3, h/6), D = (-1, 2, -1)/h.
+// The others (flag NODE_GENERAL: near the interface, the box boundary or a Dirichlet node) sum
+// their cells' rows with the cells' 1D matrices and the faces explicitly.
+// ------------------------------------------------------------------------------------------
+
+// row a = (ax, ay, az) of Mx(x)My(x)Mz p + (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz) q
+__device__ __forceinline__ double cell_row(const Mat1D (&M)[3], int ax, int ay, int az,
+                                           const double (&p)[2][2][2],
+                                           const double (&q)[2][2][2])
+{
+  const double mz0 = az ? M[2].m01 : M[2].m00, mz1 = az ? M[2].m11 : M[2].m01;
+  const double my0 = ay ? M[1].m01 : M[1].m00, my1 = ay ? M[1].m11 : M[1].m01;
+  const double mx0 = ax ? M[0].m01 : M[0].m00, mx1 = ax ? M[0].m11 : M[0].m01;
+  const double kz = az ? -M[2].kk : M[2].kk, ky = ay ? -M[1].kk : M[1].kk,
+               kx = ax ? -M[0].kk : M[0].kk;
+  double r_m[2], r_q[2], r_s[2];  // per ex: My Mz p, My Mz q, (Ky Mz + My Kz) q
+#pragma unroll
+  for (int ex = 0; ex < 2; ++ex) {
+    double zp[2], zq[2], zk[2];
+#pragma unroll
+    for (int ey = 0; ey < 2; ++ey) {
+      zp[ey] = mz0 * p[0][ey][ex] + mz1 * p[1][ey][ex];
+      zq[ey] = mz0 * q[0][ey][ex] + mz1 * q[1][ey][ex];
+      zk[ey] = kz * (q[0][ey][ex] - q[1][ey][ex]);
+    }
+    r_m[ex] = my0 * zp[0] + my1 * zp[1];
+    r_q[ex] = my0 * zq[0] + my1 * zq[1];
+    r_s[ex] = ky * (zq[0] - zq[1]) + my0 * zk[0] + my1 * zk[1];
+  }
+  return mx0 * (r_m[0] + r_s[0]) + mx1 * (r_m[1] + r_s[1]) + kx * (r_q[0] - r_q[1]);
+}
+
+// The general path of node (i, j, k), the share of `lane` (0..7): the row of the lane's cell
+// (one of the node's 8) and of the ghost-penalty face candidates lane, lane + 8, ... (36 in all:
+// 3 directions x 3 planes x 4 transversal cells).  The 8 shares sum to the node's row.
+// MODE_TANGENT: (J src)_n; MODE_RESIDUAL: (-R)_n with src = T^n, A.T = T; MODE_DIAG: J_nn.
+template <int MODE>
+__device__ __forceinline__ double node_general(const CutArgs &A, const double *src, int64_t i,
+                                               int64_t j, int64_t k, int lane)
+{
+  const int64_t ncx = A.ncell[0], ncy = A.ncell[1], ncz = A.ncell[2];
+  const int64_t NX = A.NX, NXY = A.NXY;
+  const double c = A.ost_i * A.k;
+  double y = 0.0;
+  {
+    const int64_t cx = i - 1 + (lane & 1), cy = j - 1 + ((lane >> 1) & 1), cz = k - 1 + (lane >> 2);
+    const bool in = cx >= 0 && cx < ncx && cy >= 0 && cy < ncy && cz >= 0 && cz < ncz;
+    const int32_t cc = in ? A.cell[(cz * ncy + cy) * ncx + cx] : CELL_GAS;
+    if (cc != CELL_GAS) {
+      {
+        Mat1D M[3];
+        cell_mats(A, cc, M);
+        const int ax = int(i - cx), ay = int(j - cy), az = int(k - cz);
+        if (MODE == MODE_DIAG) {
+          const double mx = ax ? M[0].m11 : M[0].m00, my = ay ? M[1].m11 : M[1].m00,
+                       mz = az ? M[2].m11 : M[2].m00;
+          y += A.cv * mx * my * mz + c * (M[0].kk * my * mz + mx * M[1].kk * mz + mx * my * M[2].kk);
+        } else {
+        double p[2][2][2], q[2][2][2];
+        const int64_t n0 = cz * NXY + cy * NX + cx;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+          const int64_t m = n0 + ex + ey * NX + ez * NXY;
+          if (MODE == MODE_TANGENT) {
+            const double u = fixed_row(A, m) ? 0.0 : src[m];
+            p[ez][ey][ex] = A.cv * u;
+            q[ez][ey][ex] = c * u;
+          } else {
+            const double t = A.T[m], to = src[m];
+            p[ez][ey][ex] = -A.cv * (t - to);
+            q[ez][ey][ex] = -A.k * (A.ost_i * t + A.ost_e * to);
+          }
+        }
+        y += cell_row(M, ax, ay, az, p, q);
+        }
+      }
+    }
+  }
+  // ghost-penalty faces with this node among their 12 nodes
+  const int64_t nidx[3] = {i, j, k}, lim[3] = {ncx, ncy, ncz};
+  const int64_t sn[3] = {1, NX, NXY}, sc[3] = {1, ncx, ncx * ncy};
+  const double coef[3] = {-1.0, 2.0, -1.0};
+  for (int fc = lane; fc < 36; fc += 8) {
+    const int d = fc / 12, sp = (fc % 12) >> 2, c1 = fc & 1, c2 = (fc >> 1) & 1;
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    const double ih = A.ih[d], wq = A.wq[d];
+    {
+      const int64_t cm = nidx[d] - sp;  // index along d of the lower cell
+      if (cm < 0 || cm + 1 >= lim[d]) continue;
+      {
+        const int64_t t1 = nidx[d1] - c1;
+        if (t1 < 0 || t1 >= lim[d1]) continue;
+        {
+          const int64_t t2 = nidx[d2] - c2;
+          if (t2 < 0 || t2 >= lim[d2]) continue;
+          const int64_t lower = cm * sc[d] + t1 * sc[d1] + t2 * sc[d2];
+          const int32_t a = A.cell[lower], b = A.cell[lower + sc[d]];
+          if (!((a >= 0 && b != CELL_GAS) || (b >= 0 && a != CELL_GAS))) continue;
+          const int cn = c1 + 2 * c2;  // this node's corner on the face
+          if (MODE == MODE_DIAG) {
+            // sum_q wq L_q(corner)^2 for the bilinear L at the 2 x 2 Gauss points
+            const double g0 = 0.5 - GAUSS_OFF, g1 = 0.5 + GAUSS_OFF;
+            const double s2 = (g0 * g0 + g1 * g1) * (g0 * g0 + g1 * g1);
+            y += coef[sp] * coef[sp] * ih * ih * wq * s2;
+            continue;
+          }
+          // G of this corner, computed once per face by cut_face_g_kernel
+          const double G = a >= 0 ? A.faceG[24 * int64_t(a) + 4 * (2 * d + 1) + cn]
+                                  : A.faceG[24 * int64_t(b) + 4 * (2 * d) + cn];
+          y += (MODE == MODE_RESIDUAL ? -1.0 : 1.0) * coef[sp] * ih * G;
+        }
+      }
+    }
+  }
+  return y;
+}
+
+// G_c = sum_q gamma JxW_q L_q(c) [d_n u](q) of every ghost-penalty face (c = its 4 corners),
+// one thread per face; stored with the face's intersected cell(s) (slot 2d + 1: the upper face
+// of the lower cell, slot 2d: the lower face of the upper cell).  MODE_TANGENT: u = src with
+// fixed entries 0; MODE_RESIDUAL: u = T read plain.
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_face_g_kernel(CutArgs A, const int64_t *faces,
+                                                         int64_t n_faces, const double *src)
+{
+  if (A.stop && *A.stop) return;
+  const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= n_faces) return;
+  const int64_t code = faces[f];
+  const int d = int(code & 3);
+  const int64_t cell = code >> 2;
+  const int64_t ncx = A.ncell[0], ncy = A.ncell[1];
+  const uint32_t c32 = uint32_t(cell), row = c32 / uint32_t(ncx);
+  const int64_t ci = c32 - row * uint32_t(ncx), cj = row % uint32_t(ncy), ck = row / uint32_t(ncy);
+  const int64_t n0 = ck * A.NXY + cj * A.NX + ci;
+  const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+  const int64_t stride[3] = {1, A.NX, A.NXY};
+  const int64_t sd = stride[d], s1 = stride[d1], s2 = stride[d2];
+  const double ih = A.ih[d], wq = A.wq[d];
+  const double g[2] = {0.5 - GAUSS_OFF, 0.5 + GAUSS_OFF};
+  double J[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double v[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const int64_t n = n0 + p * sd + (c & 1) * s1 + (c >> 1) * s2;
+      v[p] = MODE == MODE_RESIDUAL ? A.T[n] : (fixed_row(A, n) ? 0.0 : src[n]);
+    }
+    J[c] = (2.0 * v[1] - v[0] - v[2]) * ih;
+  }
+  double G[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double L[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double x = g[q & 1], z = g[q >> 1];
+      L[c] = ((c & 1) ? x : 1.0 - x) * ((c >> 1) ? z : 1.0 - z);
+    }
+    const double jump = L[0] * J[0] + L[1] * J[1] + L[2] * J[2] + L[3] * J[3];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) G[c] += wq * L[c] * jump;
+  }
+  const int64_t sc[3] = {1, ncx, ncx * ncy};
+  const int32_t a = A.cell[cell], b = A.cell[cell + sc[d]];
+  if (a >= 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) A.faceG[24 * int64_t(a) + 4 * (2 * d + 1) + c] = G[c];
+  }
+  if (b >= 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) A.faceG[24 * int64_t(b) + 4 * (2 * d) + c] = G[c];
+  }
+}
+
+// 1D stencils of one direction at a node with lower / upper cell present (l, r in {0, 1}):
+// S = (l h/6, (l + r) h/3, r h/6), D = (-l, l + r, -r) / h; the offsets of the neighbours, a
+// missing one read at the node itself (its weight is 0)
+struct Stencil1D {
+  double S[3], D[3];
+  int64_t off[3];
+};
+
+__device__ __forceinline__ Stencil1D stencil_1d(uint32_t idx, uint32_t last, int64_t stride,
+                                                double h6, double h3, double ih)
+{
+  const double l = idx > 0 ? 1.0 : 0.0, r = idx < last ? 1.0 : 0.0;
+  Stencil1D s;
+  s.S[0] = l * h6;
+  s.S[1] = (l + r) * h3;
+  s.S[2] = r * h6;
+  s.D[0] = -l * ih;
+  s.D[1] = (l + r) * ih;
+  s.D[2] = -r * ih;
+  s.off[0] = idx > 0 ? -stride : 0;
+  s.off[1] = 0;
+  s.off[2] = idx < last ? stride : 0;
+  return s;
+}
+
+// The y-z part of the stencil at node n (row j, layer k), before the x-pass:
+// tangent: m = cv Sy Sz u + c (Dy Sz + Sy Dz) u, s = c Sy Sz u;
+// residual: m = Sy Sz p + (Dy Sz + Sy Dz) q, s = Sy Sz q with p = -cv (T - T^n),
+// q = -k (dt theta T + dt (1 - theta) T^n).  Then row n = Sx . m + Dx . s over (n-1, n, n+1).
+template <int MODE>
+__device__ __forceinline__ void column_yz(const CutArgs &A, const double *src, int64_t n,
+                                          const Stencil1D &sy, const Stencil1D &sz, double c,
+                                          double &m, double &s)
+{
+  if (MODE == MODE_TANGENT) {
+    double a[3], b[3];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      double u[3];
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int64_t mm = n + sy.off[dy] + sz.off[dz];
+        u[dz] = (A.dmask && A.dmask[mm]) ? 0.0 : src[mm];
+      }
+      a[dy] = sz.S[0] * u[0] + sz.S[1] * u[1] + sz.S[2] * u[2];
+      b[dy] = sz.D[0] * u[0] + sz.D[1] * u[1] + sz.D[2] * u[2];
+    }
+    const double sa = sy.S[0] * a[0] + sy.S[1] * a[1] + sy.S[2] * a[2];
+    const double da = sy.D[0] * a[0] + sy.D[1] * a[1] + sy.D[2] * a[2];
+    const double sb = sy.S[0] * b[0] + sy.S[1] * b[1] + sy.S[2] * b[2];
+    m = A.cv * sa + c * (da + sb);
+    s = c * sa;
+  } else {
+    double ap[3], aq[3], bq[3];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      double pz[3], qz[3];
+#pragma unroll
+      for (int dz = 0; dz < 3; ++dz) {
+        const int64_t mm = n + sy.off[dy] + sz.off[dz];
+        const double t = A.T[mm], to = src[mm];
+        pz[dz] = -A.cv * (t - to);
+        qz[dz] = -A.k * (A.ost_i * t + A.ost_e * to);
+      }
+      ap[dy] = sz.S[0] * pz[0] + sz.S[1] * pz[1] + sz.S[2] * pz[2];
+      aq[dy] = sz.S[0] * qz[0] + sz.S[1] * qz[1] + sz.S[2] * qz[2];
+      bq[dy] = sz.D[0] * qz[0] + sz.D[1] * qz[1] + sz.D[2] * qz[2];
+    }
+    const double pa = sy.S[0] * ap[0] + sy.S[1] * ap[1] + sy.S[2] * ap[2];
+    const double qa = sy.S[0] * aq[0] + sy.S[1] * aq[1] + sy.S[2] * aq[2];
+    const double qk = sy.D[0] * aq[0] + sy.D[1] * aq[1] + sy.D[2] * aq[2];
+    const double qb = sy.S[0] * bq[0] + sy.S[1] * bq[1] + sy.S[2] * bq[2];
+    m = pa + qk + qb;
+    s = qa;
+  }
+}
+
+// MODE_TANGENT: dst = J src (overwrite) or dst += J src, with partials of src . (J src) per
+// block when dot_partial is set; MODE_DIAG: dst = diag J (1 on fixed rows); MODE_RESIDUAL:
+// dst += -R(T) with src = T^n.  A block covers NODE_CHUNK = 254 consecutive nodes (chunks in
+// XCD-major order); thread t reduces the y-z column of node base + t - 1 (9 loads) and stages
+// it in LDS, threads 1..254 then do the x-pass of their node from the staged neighbours (the
+// first and last thread only provide the halo).  At the box boundary a direction has one cell
+// only: its stencils are the one-sided ones; across the end of an x line the weight is 0.
+constexpr int NODE_CHUNK = 254;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *src, double *dst,
+                                                       int overwrite, double *dot_partial)
+{
+  if (A.stop && *A.stop) return;
+  __shared__ double red[256];
+  __shared__ double lm[256], ls[256];
+  const int64_t NX = A.NX, NXY = A.NXY;
+  const uint32_t nx = uint32_t(NX), nxy = uint32_t(NXY);
+  const uint32_t lastx = nx - 1, lasty = uint32_t(A.ncell[1]), lastz = uint32_t(A.ncell[2]);
+  const int64_t n_nodes = NXY * int64_t(A.ncell[2] + 1);
+  const double c = A.ost_i * A.k;
+  const double *h6 = A.h6, *h3 = A.h3, *ih = A.ih;
+  const int tid = threadIdx.x;
+  // consecutive chunks on one XCD (the hardware deals blocks to the 8 XCDs round-robin): the
+  // y and z neighbours a block reads were then fetched into the same L2 by its neighbours
+  const int64_t nb8 = gridDim.x / 8;
+  const int64_t chunk = int64_t(blockIdx.x % 8) * nb8 + blockIdx.x / 8;
+  const int64_t base = chunk * NODE_CHUNK;
+  if (base >= n_nodes) {
+    if (MODE == MODE_TANGENT && dot_partial && tid == 0) dot_partial[blockIdx.x] = 0.0;
+    return;
+  }
+  const int64_t n = base + tid - 1;
+  const bool in = n >= 0 && n < n_nodes;
+  const bool own = tid >= 1 && tid <= NODE_CHUNK && in;
+  // 32-bit index arithmetic (the node count is below 2^32: checked with the level set)
+  const uint32_t n32 = uint32_t(in ? n : base);
+  const uint32_t k = div_magic(n32, A.mg_nxy), rem = n32 - k * nxy;
+  const uint32_t j = div_magic(rem, A.mg_nx), i = rem - j * nx;
+  const Stencil1D sy = stencil_1d(j, lasty, NX, h6[1], h3[1], ih[1]);
+  const Stencil1D sz = stencil_1d(k, lastz, NXY, h6[2], h3[2], ih[2]);
+  const uint8_t f = in ? A.nflag[n] : NODE_INACTIVE;
+  if (MODE != MODE_DIAG) {
+    double m = 0.0, s = 0.0;
+    if (f != NODE_INACTIVE) column_yz<MODE>(A, src, n, sy, sz, c, m, s);
+    lm[tid] = m;
+    ls[tid] = s;
+    __syncthreads();
+  }
+  double dot = 0.0;
+  if (own) {
+    const bool fixed = f == NODE_INACTIVE || (A.dmask && A.dmask[n]);
+    if (fixed) {
+      // a node without a free DoF: 0 (tangent), 1 (diagonal), unchanged (residual, add)
+      if (MODE == MODE_DIAG)
+        dst[n] = 1.0;
+      else if (MODE == MODE_TANGENT && overwrite)
+        dst[n] = 0.0;
+    } else if (f == 0) {
+      const Stencil1D sx = stencil_1d(i, lastx, 1, h6[0], h3[0], ih[0]);
+      double val;
+      if (MODE == MODE_DIAG) {
+        val = A.cv * sx.S[1] * sy.S[1] * sz.S[1] +
+              c * (sx.D[1] * sy.S[1] * sz.S[1] + sx.S[1] * sy.D[1] * sz.S[1] +
+                   sx.S[1] * sy.S[1] * sz.D[1]);
+      } else {
+        val = sx.S[0] * lm[tid - 1] + sx.S[1] * lm[tid] + sx.S[2] * lm[tid + 1] +
+              sx.D[0] * ls[tid - 1] + sx.D[1] * ls[tid] + sx.D[2] * ls[tid + 1];
+      }
+      if (overwrite)
+        dst[n] = val;
+      else
+        dst[n] += val;
+      if (MODE == MODE_TANGENT && dot_partial) dot = src[n] * val;
+    }
+  }
+  if (MODE == MODE_TANGENT && dot_partial) {
+    const double t = block_sum_256(dot, red);
+    if (tid == 0) dot_partial[blockIdx.x] = t;
+  }
+}
+
+// The general nodes (flag != 0, a list built on the host): fixed rows (no DoF, Dirichlet) get
+// 0 (tangent), 1 (diagonal) or stay (residual, vmult_add); the others sum their cells' rows
+// and their ghost-penalty faces.  Fused-dot partials follow the node kernel's.
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_general_kernel(CutArgs A, const int64_t *list,
+                                                          int64_t n_list, const double *src,
+                                                          double *dst, int overwrite,
+                                                          double *dot_partial)
+{
+  if (A.stop && *A.stop) return;
+  __shared__ double red[256];
+  const int64_t NX = A.NX, NXY = A.NXY;
+  const int lane = threadIdx.x & 7;  // 8 consecutive lanes share a node
+  double dot = 0.0;
+  for (int64_t idx = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 3; idx < n_list;
+       idx += int64_t(gridDim.x) * 32) {
+    const int64_t n = list[idx];
+    double val;
+    bool write = true;
+    if (A.dmask && A.dmask[n]) {
+      val = MODE == MODE_DIAG ? 1.0 : 0.0;
+      write = MODE == MODE_DIAG || (MODE == MODE_TANGENT && overwrite);
+    } else {
+      const uint32_t n32 = uint32_t(n), nxy = uint32_t(NXY), nx = uint32_t(NX);
+      const uint32_t k32 = div_magic(n32, A.mg_nxy), r32 = n32 - k32 * nxy;
+      const uint32_t j32 = div_magic(r32, A.mg_nx);
+      val = node_general<MODE>(A, src, r32 - j32 * nx, j32, k32, lane);
+      val += __shfl_xor(val, 1);
+      val += __shfl_xor(val, 2);
+      val += __shfl_xor(val, 4);
+      if (MODE == MODE_TANGENT && dot_partial && lane == 0) dot += src[n] * val;
+    }
+    if (write && lane == 0) {
+      if (overwrite)
+        dst[n] = val;
+      else
+        dst[n] += val;
+    }
+  }
+  if (MODE == MODE_TANGENT && dot_partial) {
+    const double t = block_sum_256(dot, red);
+    if (threadIdx.x == 0) dot_partial[blockIdx.x] = t;
+  }
 }
 
 CutArgs make_cut_args(const mf_operator *op)
@@ -522,6 +1139,17 @@ CutArgs make_cut_args(const mf_operator *op)
                        : std::sqrt((m.h[0] * m.h[0] + m.h[1] * m.h[1] + m.h[2] * m.h[2]) / 3.0);
   const double kl = A.mushy ? mp.liquid_k : mp.k;
   A.gamma = A.ost_i * p.gamma_A * kl * h / 3.0 + p.gamma_M * h * h * h / 3.0;
+  for (int d = 0; d < 3; ++d) {
+    A.h6[d] = m.h[d] / 6.0;
+    A.h3[d] = m.h[d] / 3.0;
+    A.ih[d] = 1.0 / m.h[d];
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    A.wq[d] = 0.25 * m.h[d1] * m.h[d2] * A.gamma;
+  }
+  A.mg_nx = ~uint64_t(0) / uint64_t(m.NN[0]) + 1;
+  A.mg_nxy = ~uint64_t(0) / uint64_t(m.NN[0] * m.NN[1]) + 1;
+  A.mats = op->cutd->d_mats;
+  A.faceG = op->cutd->d_faceG;
   A.cell = op->cutd->d_cell;
   A.box = op->cutd->d_box;
   A.dmask = op->dof_index == 0 ? op->s->d_mask : nullptr;
@@ -540,7 +1168,10 @@ int launch_cut(const mf_operator *op, CutArgs A, const double *src, double *dst)
     const int64_t rows = int64_t(rc.hy) * rc.hz;
     if (rows == 0) continue;
     A.colour = colour;
-    cut_cell_kernel<MODE><<<dim3(unsigned(rows)), dim3(CUT_THREADS), 0, st>>>(A, src, dst);
+    if (A.mushy)
+      cut_cell_kernel<MODE, true><<<dim3(unsigned(rows)), dim3(CUT_THREADS), 0, st>>>(A, src, dst);
+    else
+      cut_cell_kernel<MODE, false><<<dim3(unsigned(rows)), dim3(CUT_THREADS), 0, st>>>(A, src, dst);
     MF_HIP_TRY(hipGetLastError());
   }
   const CutData *cd = op->cutd;
@@ -556,9 +1187,18 @@ void free_tables(CutData *cd)
 {
   if (cd->d_cell) (void)hipFree(cd->d_cell);
   if (cd->d_box) (void)hipFree(cd->d_box);
+  if (cd->d_mats) (void)hipFree(cd->d_mats);
+  if (cd->d_faceG) (void)hipFree(cd->d_faceG);
+  cd->d_mats = nullptr;
+  cd->d_faceG = nullptr;
   if (cd->d_face) (void)hipFree(cd->d_face);
   if (cd->d_inactive) (void)hipFree(cd->d_inactive);
   if (cd->d_part) (void)hipFree(cd->d_part);
+  if (cd->d_nflag) (void)hipFree(cd->d_nflag);
+  if (cd->d_glist) (void)hipFree(cd->d_glist);
+  cd->d_nflag = nullptr;
+  cd->d_glist = nullptr;
+  cd->n_glist = 0;
   cd->d_cell = nullptr;
   cd->d_box = nullptr;
   cd->d_face = nullptr;
@@ -567,9 +1207,99 @@ void free_tables(CutData *cd)
   cd->set = false;
 }
 
+// Flags of the node kernel: NODE_INACTIVE for the nodes without a DoF, NODE_GENERAL for the
+// nodes of intersected and gas cells and of ghost-penalty faces; built on the host with the
+// level set.  (The Dirichlet mask is read by the kernels themselves.)
+int ensure_node_flags(mf_operator *op)
+{
+  CutData *cd = op->cutd;
+  const mf_scratch *sc = op->s;
+  if (cd->d_nflag) return MF_OK;
+  const MeshInfo &m = sc->mesh;
+  const int64_t nx = m.ncell[0], ny = m.ncell[1], nz = m.ncell[2];
+  const int64_t NX = m.NN[0], NY = m.NN[1], NZ = m.NN[2], NXY = NX * NY;
+  std::vector<uint8_t> f(size_t(m.n_dofs), 0);
+  auto mark = [&](int64_t n) { f[size_t(n)] |= NODE_GENERAL; };
+  // the nodes of intersected and gas cells
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i)
+        if (cd->category[size_t((k * ny + j) * nx + i)] != 0)
+          for (int e = 0; e < 8; ++e)
+            mark((k + (e >> 2)) * NXY + (j + ((e >> 1) & 1)) * NX + i + (e & 1));
+  // the 12 nodes of every ghost-penalty face
+  for (int64_t code : cd->faces) {
+    const int d = int(code & 3);
+    const int64_t c = code >> 2;
+    const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const int64_t sn[3] = {1, NX, NXY};
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    const int64_t n0 = k * NXY + j * NX + i;
+    for (int pl = 0; pl < 3; ++pl)
+      for (int cr = 0; cr < 4; ++cr) mark(n0 + pl * sn[d] + (cr & 1) * sn[d1] + (cr >> 1) * sn[d2]);
+  }
+  // nodes without a DoF: the node kernel writes their fixed value (flag NODE_INACTIVE alone)
+  for (int64_t n = 0; n < m.n_dofs; ++n)
+    if (!cd->active[size_t(n)]) f[size_t(n)] = NODE_INACTIVE;
+  std::vector<int64_t> list;
+  for (int64_t n = 0; n < m.n_dofs; ++n)
+    if (f[size_t(n)] & NODE_GENERAL) list.push_back(n);
+  MF_HIP_TRY(hipStreamSynchronize(sc->stream));
+  if (!cd->d_nflag) MF_HIP_TRY(hipMalloc(&cd->d_nflag, size_t(m.n_dofs)));
+  MF_HIP_TRY(hipMemcpy(cd->d_nflag, f.data(), size_t(m.n_dofs), hipMemcpyHostToDevice));
+  if (cd->d_glist) MF_HIP_TRY(hipFree(cd->d_glist));
+  cd->d_glist = nullptr;
+  MF_HIP_TRY(hipMalloc(&cd->d_glist, sizeof(int64_t) * std::max<size_t>(list.size(), 1)));
+  if (!list.empty())
+    MF_HIP_TRY(hipMemcpy(cd->d_glist, list.data(), sizeof(int64_t) * list.size(),
+                         hipMemcpyHostToDevice));
+  cd->n_glist = int64_t(list.size());
+  return MF_OK;
+}
+
+// grid of the general-node kernel: 32 nodes per block and step, at most 1024 blocks
+int general_blocks(const CutData *cd)
+{
+  return int(std::max<int64_t>(1, std::min<int64_t>(1024, (cd->n_glist + 31) / 32)));
+}
+
+// blocks of the node kernel: one per NODE_CHUNK nodes, rounded up to a multiple of the 8 XCDs
+int node_blocks(const MeshInfo &m)
+{
+  return int(((m.n_dofs + NODE_CHUNK - 1) / NODE_CHUNK + 7) / 8 * 8);
+}
+
+template <int MODE>
+int launch_node(mf_operator *op, CutArgs A, const double *src, double *dst, int overwrite,
+                double *dot_partial)
+{
+  int rc = ensure_node_flags(op);
+  if (rc) return rc;
+  const CutData *cd = op->cutd;
+  A.nflag = cd->d_nflag;
+  const int nb = node_blocks(op->s->mesh);
+  if (MODE != MODE_DIAG && cd->n_faces > 0) {
+    cut_face_g_kernel<MODE><<<dim3(unsigned((cd->n_faces + 255) / 256)), dim3(256), 0,
+                              op->s->stream>>>(A, cd->d_face, cd->n_faces, src);
+    MF_HIP_TRY(hipGetLastError());
+  }
+  cut_node_kernel<MODE><<<dim3(unsigned(nb)), dim3(256), 0, op->s->stream>>>(A, src, dst, overwrite,
+                                                                             dot_partial);
+  MF_HIP_TRY(hipGetLastError());
+  cut_general_kernel<MODE><<<dim3(unsigned(general_blocks(cd))), dim3(256), 0, op->s->stream>>>(
+      A, cd->d_glist, cd->n_glist, src, dst, overwrite, dot_partial ? dot_partial + nb : nullptr);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+
 }  // namespace
 
 bool cut_ready(const mf_operator *op) { return op->cutd && op->cutd->set; }
+
+int cut_dot_blocks(const mf_operator *op)
+{
+  return node_blocks(op->s->mesh) + general_blocks(op->cutd);
+}
 
 void cut_free(mf_operator *op)
 {
@@ -585,12 +1315,15 @@ int cut_apply(mf_operator *op, double *dst, const double *src, double *dot_parti
   const int64_t n = op->s->mesh.n_dofs;
   hipStream_t st = op->s->stream;
   int rc;
-  if (overwrite && (rc = vec_fill<double>(dst, 0.0, n, st, stop))) return rc;
   CutArgs A = make_cut_args(op);
   A.stop = stop;
+  // constant law: one node-centric launch
+  if (!A.mushy) return launch_node<MODE_TANGENT>(op, A, src, dst, overwrite ? 1 : 0, dot_partial);
+  if (overwrite && (rc = vec_fill<double>(dst, 0.0, n, st, stop))) return rc;
   if ((rc = launch_cut<MODE_TANGENT>(op, A, src, dst))) return rc;
   if (dot_partial) {
-    cut_dot_kernel<<<dim3(CUT_DOT_BLOCKS), dim3(256), 0, st>>>(src, dst, n, dot_partial, stop);
+    cut_dot_kernel<<<dim3(unsigned(cut_dot_blocks(op))), dim3(256), 0, st>>>(src, dst, n,
+                                                                           dot_partial, stop);
     MF_HIP_TRY(hipGetLastError());
   }
   return MF_OK;
@@ -601,8 +1334,9 @@ int cut_diagonal(mf_operator *op, double *diag)
   const int64_t n = op->s->mesh.n_dofs;
   hipStream_t st = op->s->stream;
   int rc;
-  if ((rc = vec_fill<double>(diag, 0.0, n, st))) return rc;
   CutArgs A = make_cut_args(op);
+  if (!A.mushy) return launch_node<MODE_DIAG>(op, A, nullptr, diag, 1, nullptr);
+  if ((rc = vec_fill<double>(diag, 0.0, n, st))) return rc;
   if ((rc = launch_cut<MODE_DIAG>(op, A, nullptr, diag))) return rc;
   // rows without a free DoF: 1 (the identity rows of the solver)
   if ((rc = vec_masked_fill(diag, op->cutd->d_inactive, 1.0, n, st))) return rc;
@@ -613,6 +1347,7 @@ int cut_diagonal(mf_operator *op, double *diag)
 int cut_residual(mf_operator *op, double *dst, const double *T_old)
 {
   CutArgs A = make_cut_args(op);
+  if (!A.mushy) return launch_node<MODE_RESIDUAL>(op, A, T_old, dst, 0, nullptr);
   return launch_cut<MODE_RESIDUAL>(op, A, T_old, dst);
 }
 
@@ -656,7 +1391,7 @@ int mf_heat_cut_set_level_set(mf_operator *op, const double *phi)
   tmp.box.assign(size_t(nc) * 6, 0.0);
   tmp.active.assign(size_t(nd), 0);
   std::vector<int32_t> cell(static_cast<size_t>(nc));
-  std::vector<double> boxes;
+  std::vector<double> boxes, mats;
   for (int64_t k = 0; k < nz; ++k)
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t i = 0; i < nx; ++i) {
@@ -712,9 +1447,19 @@ int mf_heat_cut_set_level_set(mf_operator *op, const double *phi)
         tmp.category[c] = 1;
         cell[c] = int32_t(tmp.n_cut);
         boxes.insert(boxes.end(), b, b + 6);
+        // the 1D matrices of the cell's liquid intervals (M_ij = h int_a^b l_i l_j, K = (b-a)/h)
+        for (int d = 0; d < 3; ++d) {
+          const double a = b[2 * d], e = b[2 * d + 1], h = m.h[d], oa = 1.0 - a, oe = 1.0 - e;
+          mats.push_back(h * (oa * oa * oa - oe * oe * oe) / 3.0);
+          mats.push_back(h * ((e * e - a * a) / 2.0 - (e * e * e - a * a * a) / 3.0));
+          mats.push_back(h * (e * e * e - a * a * a) / 3.0);
+          mats.push_back((e - a) / h);
+        }
         ++tmp.n_cut;
       }
   MF_REQUIRE(tmp.n_cut < (int64_t(1) << 31), MF_ERR_ARG, "too many intersected cells");
+  MF_REQUIRE(nd < (int64_t(1) << 32), MF_ERR_ARG,
+             "the cut heat operator takes meshes of fewer than 2^32 nodes");
   // DoFs: the vertices of liquid and intersected cells; ghost-penalty faces: intersected on one
   // side, intersected or liquid on the other (CutUtil::get_face_type, source/cut/util.cpp:40-98)
   std::vector<int64_t> faces;
@@ -744,12 +1489,16 @@ int mf_heat_cut_set_level_set(mf_operator *op, const double *phi)
   free_tables(cd);
   MF_HIP_TRY(hipMalloc(&cd->d_cell, sizeof(int32_t) * size_t(nc)));
   MF_HIP_TRY(hipMalloc(&cd->d_box, sizeof(double) * std::max<size_t>(boxes.size(), 1)));
+  MF_HIP_TRY(hipMalloc(&cd->d_mats, sizeof(double) * std::max<size_t>(mats.size(), 1)));
+  MF_HIP_TRY(hipMalloc(&cd->d_faceG, sizeof(double) * 24 * std::max<size_t>(size_t(tmp.n_cut), 1)));
   MF_HIP_TRY(hipMalloc(&cd->d_face, sizeof(int64_t) * std::max<size_t>(faces.size(), 1)));
   MF_HIP_TRY(hipMalloc(&cd->d_inactive, size_t(nd)));
   MF_HIP_TRY(hipMalloc(&cd->d_part, sizeof(double) * CUT_NORM_BLOCKS));
   MF_HIP_TRY(hipMemcpy(cd->d_cell, cell.data(), sizeof(int32_t) * size_t(nc), hipMemcpyHostToDevice));
-  if (!boxes.empty())
+  if (!boxes.empty()) {
     MF_HIP_TRY(hipMemcpy(cd->d_box, boxes.data(), sizeof(double) * boxes.size(), hipMemcpyHostToDevice));
+    MF_HIP_TRY(hipMemcpy(cd->d_mats, mats.data(), sizeof(double) * mats.size(), hipMemcpyHostToDevice));
+  }
   if (!faces.empty())
     MF_HIP_TRY(hipMemcpy(cd->d_face, faces.data(), sizeof(int64_t) * faces.size(), hipMemcpyHostToDevice));
   MF_HIP_TRY(hipMemcpy(cd->d_inactive, inactive.data(), size_t(nd), hipMemcpyHostToDevice));
@@ -758,11 +1507,12 @@ int mf_heat_cut_set_level_set(mf_operator *op, const double *phi)
   cd->n_gas = tmp.n_gas;
   cd->n_faces = tmp.n_faces;
   cd->n_active = tmp.n_active;
+  cd->faces = faces;
   cd->category.swap(tmp.category);
   cd->box.swap(tmp.box);
   cd->active.swap(tmp.active);
   cd->set = true;
-  return MF_OK;
+  return ensure_node_flags(op);
 }
 
 int mf_heat_cut_info(const mf_operator *op, int64_t *n_liquid, int64_t *n_intersected,

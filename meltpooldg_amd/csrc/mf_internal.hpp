@@ -378,10 +378,10 @@ inline void reinit_epsilon(const mf_operator *op, double &eps_n, double &eps_t, 
 }
 
 // One-phase CutFEM heat operator (mf_cut.hip).  cut_apply: dst = J src (overwrite) or
-// dst += J src, with CUT_DOT_BLOCKS partials of src . (J src) when dot_partial is given;
+// dst += J src, with cut_dot_blocks(op) partials of src . (J src) when dot_partial is given;
 // cut_diagonal: the diagonal of J, 1 on the rows without a free DoF; cut_residual: dst += -R(T)
 // with T = op->T and T_old; cut_ready: the level set is set; cut_free: the device tables.
-constexpr int CUT_DOT_BLOCKS = 256;
+int cut_dot_blocks(const mf_operator *op);
 int cut_apply(mf_operator *op, double *dst, const double *src, double *dot_partial,
               const int *stop, bool overwrite);
 int cut_diagonal(mf_operator *op, double *diag);
