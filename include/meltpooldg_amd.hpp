@@ -75,6 +75,12 @@ class DeviceVector {
   const Number *data() const { return ptr_; }
   void copy_from_host(const Number *h) { hip_check(hipMemcpy(ptr_, h, sizeof(Number) * n_, hipMemcpyHostToDevice)); }
   void copy_to_host(Number *h) const { hip_check(hipMemcpy(h, ptr_, sizeof(Number) * n_, hipMemcpyDeviceToHost)); }
+  // device-to-device copy of a vector of the same size
+  void copy_from(const DeviceVector &o)
+  {
+    if (o.n_ != n_) throw Exception(MF_ERR_ARG, "DeviceVector::copy_from: sizes differ");
+    hip_check(hipMemcpy(ptr_, o.ptr_, sizeof(Number) * n_, hipMemcpyDeviceToDevice));
+  }
   std::vector<Number> to_host() const
   {
     std::vector<Number> v(n_);
@@ -232,6 +238,42 @@ class ReinitializationOperator : public OperatorMatrixFree {
     check(mf_operator_create_rhs(h_, dst.data(), level_set.data(), nullptr));
   }
   double min_epsilon() const { return mf_reinit_min_epsilon(h_); }
+};
+
+// HeatCutOperator without two_phase (source/heat/heat_cut_operator.cpp), FE_Q1: the tangent
+// over the liquid part {phi > 0} plus the ghost penalty.  set_level_set takes the nodal level
+// set on the host (classification, sub-boxes, faces); init_time_advance sets the ost factors;
+// create_rhs(dst, T_old) adds -R(T) with T from set_temperature.
+class HeatCutOperator : public OperatorMatrixFree {
+ public:
+  HeatCutOperator(ScratchData &s, const mf_heat_cut_params &p)
+  {
+    check(mf_heat_cut_operator_create(s.handle(), &p, &h_));
+  }
+  void set_level_set(const double *phi_host) { check(mf_heat_cut_set_level_set(h_, phi_host)); }
+  void init_time_advance(double dt) { reset_time_increment(dt); }
+  void set_temperature(const VectorType *T)
+  {
+    check(mf_operator_set_fields(h_, T ? T->data() : nullptr, nullptr, nullptr));
+  }
+  // CutUtil::compute_cut_norm (L2)
+  double compute_cut_norm(const VectorType &u)
+  {
+    double r = 0.0;
+    check(mf_heat_cut_norm_l2(h_, u.data(), &r));
+    return r;
+  }
+  // HeatCutOperation::solve for a fixed interface: T holds the predictor, then the solution;
+  // throws NoConvergence (ExcNewtonDidNotConverge)
+  mf_newton_report solve(VectorType &T, const VectorType &T_old, const mf_solver_desc &solver,
+                         const mf_newton_desc &newton, const VectorType *dirichlet_values = nullptr)
+  {
+    mf_newton_report rep{};
+    check(mf_heat_cut_step(h_, T.data(), T_old.data(),
+                           dirichlet_values ? dirichlet_values->data() : nullptr, &solver,
+                           &newton, &rep));
+    return rep;
+  }
 };
 
 // JacobiPreconditioner (include/meltpooldg/linear_algebra/preconditioner_jacobi.hpp:29-89)

@@ -50,3 +50,28 @@ def test_cpp_caller_vmult_and_solve(tmp_path):
     assert st == 0 and abs(int(its) - it_ref) <= 1, (its, it_ref)
     assert rel_err(x, x_ref) < 1e-10
     assert threw == 1
+
+
+CUT_DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "cut_driver")
+GOLDEN_ONE_PHASE_CUT = [4.04890e-03, 4.01890e-03, 3.99116e-03, 3.96522e-03, 3.94086e-03]
+
+
+def test_cpp_caller_cut_heat_steps(tmp_path):
+    """tests/cpp/cut_driver runs the one_phase_cut golden case (extruded to 3D) through
+    mf::HeatCutOperator: its norms match the reference's output to the printed digits (+-1 in
+    the sixth, see tests/test_oracle_cut.py) and the oracle's loop to 1e-9."""
+    assert os.path.exists(CUT_DRIVER), "tests/cpp/cut_driver is built by __graft_entry__.build()"
+    out = tmp_path / "cut.bin"
+    print("cpp cut driver: one_phase_cut golden, 17 x 17 x 1", flush=True)
+    r = subprocess.run([CUT_DRIVER, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    ns = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    norms = np.frombuffer(raw[4:4 + 8 * ns], dtype=np.float64)
+    np.testing.assert_allclose(norms, GOLDEN_ONE_PHASE_CUT, rtol=3e-6)
+    m = orc.make_mesh(3, 1, (17, 17, 1), (0.0, 0.0, 0.0), (0.1, 0.1, 1.0))
+    X = orc.node_coords(m)
+    p = orc.cut_params(2.0, rho_cp=7850.0 * 460.0, k=55.563, theta=0.5, gamma_A=1.5,
+                       gamma_M=0.75, h=0.1 / 17)
+    _, ref = orc.cut_time_steps(m, 0.05 - X[:, 1], p, X[:, 0].copy(), ns)
+    np.testing.assert_allclose(norms, ref, rtol=1e-9)
