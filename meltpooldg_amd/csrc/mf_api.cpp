@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -1136,9 +1137,13 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
   return MF_OK;
 }
 
-// Right-preconditioned GMRES(m) with two-pass classical Gram-Schmidt.  Convergence measure:
-// the Givens residual estimate |gamma_{j+1}| inside a cycle and the true residual at the start
-// of each cycle (deal.II SolverGMRES with right_preconditioning = true, linear_solver.hpp:112-119).
+// Right-preconditioned GMRES(m) with classical Gram-Schmidt and deal.II's re-orthogonalisation
+// rule: one pass, and a second one for this and every later basis vector of the solve once a
+// pass has cancelled the vector below 10 sqrt(eps) of its norm (SolverGMRES's default, without
+// force_re_orthogonalization).  The pass is fused with the dots of the second pass, so that the
+// second pass costs one more read of the basis when it is needed.  Convergence measure: the
+// Givens residual estimate |gamma_{j+1}| inside a cycle and the true residual at the start of
+// each cycle (deal.II SolverGMRES with right_preconditioning = true, linear_solver.hpp:112-119).
 template <typename T>
 static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int restart, int max_it,
                        double abs_tol, double rel_tol, SolveOut &out, bool x_zero)
@@ -1170,6 +1175,8 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
     return 0;
   };
   bool first = true;
+  bool reorth = false;  // second Gram-Schmidt pass (sticky within the solve)
+  const double cancel = 10.0 * std::sqrt(double(std::numeric_limits<T>::epsilon()));
   for (;;) {
     T *V0 = V.p;
     const bool r_is_b = first && x_zero;  // zero initial guess: r0 = b
@@ -1194,10 +1201,26 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
       if ((rc = vec_precond<T>(z.p, vj, inv, n, st))) return rc;
       if ((rc = apply_tangent<T>(op, w, z.p, nullptr, nullptr, true))) return rc;
       if ((rc = gmres_multidot<T>(V.p, n, j + 1, w, n, part.p, dh.p, st))) return rc;
-      if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p, w, n, part.p, dh.p + 64, st))) return rc;
-      if ((rc = gmres_multidot<T>(V.p, n, j + 1, w, n, part.p, dh.p + 32, st))) return rc;
-      if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p + 32, w, n, part.p, dh.p + 64, st))) return rc;
+      if ((rc = gmres_multiaxpy_dot<T>(V.p, n, j + 1, dh.p, w, n, part.p, dh.p + 32, dh.p + 64,
+                                       st)))
+        return rc;
+      if (reorth &&
+          (rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p + 32, w, n, part.p, dh.p + 64, st)))
+        return rc;
       if ((rc = read_to_host(hbuf, dh.p, sizeof(double) * 65, st))) return rc;
+      if (!reorth) {
+        // |w| before the pass from its parts: |h|^2 + |w - V h|^2
+        double start = hbuf[64] * hbuf[64];
+        for (int i = 0; i <= j; ++i) start += hbuf[i] * hbuf[i];
+        if (hbuf[64] < cancel * std::sqrt(start)) {
+          reorth = true;
+          if ((rc = gmres_multiaxpy<T>(V.p, n, j + 1, dh.p + 32, w, n, part.p, dh.p + 64, st)))
+            return rc;
+          if ((rc = read_to_host(hbuf + 64, dh.p + 64, sizeof(double), st))) return rc;
+        } else {
+          for (int i = 0; i <= j; ++i) hbuf[32 + i] = 0.0;
+        }
+      }
       std::vector<double> hc(j + 2);
       for (int i = 0; i <= j; ++i) hc[i] = hbuf[i] + hbuf[32 + i];
       const double s = hbuf[64];

@@ -638,18 +638,16 @@ static int launch_multidot(int m, const T *V, int64_t ld, const T *w, int64_t n,
   return MF_OK;
 }
 
-// reduce [m][2*nparts] partials into h[m] (1 block)
+// reduce [m][2*nparts] partials into h[m] (block k reduces h_k)
 template <typename T>
-__global__ void reduce_multi_kernel(const T *part, int nparts, int m, double *h)
+__global__ void reduce_multi_kernel(const T *part, int nparts, double *h)
 {
-  for (int k = 0; k < m; ++k) {
-    Acc<T> a;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
-      a.merge(part[size_t(k) * 2 * nparts + i], part[size_t(k) * 2 * nparts + nparts + i]);
-    a = block_reduce(a);
-    if (threadIdx.x == 0) h[k] = double(a.value());
-    __syncthreads();
-  }
+  const int k = int(blockIdx.x);
+  Acc<T> a;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    a.merge(part[size_t(k) * 2 * nparts + i], part[size_t(k) * 2 * nparts + nparts + i]);
+  a = block_reduce(a);
+  if (threadIdx.x == 0) h[k] = double(a.value());
 }
 
 // w -= sum_k h_k V_k, then partial of w.w
@@ -669,6 +667,71 @@ __global__ void multiaxpy_kernel(const T *V, int64_t ld, int m, const double *h,
     a.add(v * v);
   }
   write_partial(part_ww, gridDim.x, a);
+}
+
+// A Gram-Schmidt pass fused with the dots of a second one: w -= sum_k h_k V_k, then partials
+// of V_k . w for k < M (rows 0..M-1 of part) and of w . w (row M).  Each V_k is read once for
+// both.  The update is the one of multiaxpy_kernel and the dots run in multidot_kernel's
+// grid-stride order and partial layout, so the dots are bit for bit those of a separate
+// multi-dot.
+template <typename T, int M>
+__global__ void __launch_bounds__(VEC_THREADS)
+    multiaxpy_dot_kernel(const T *V, int64_t ld, const double *h, T *w, int64_t n, T *part)
+{
+  __shared__ T hs[32];
+  if (threadIdx.x < M) hs[threadIdx.x] = T(h[threadIdx.x]);
+  __syncthreads();
+  Acc<T> acc[M], aww;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    T vk[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) vk[k] = V[k * ld + i];
+    T v = w[i];
+#pragma unroll
+    for (int k = 0; k < M; ++k) v -= hs[k] * vk[k];
+    w[i] = v;
+#pragma unroll
+    for (int k = 0; k < M; ++k) acc[k].add(vk[k] * v);
+    aww.add(v * v);
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) write_partial(part + size_t(k) * 2 * gridDim.x, gridDim.x, acc[k]);
+  write_partial(part + size_t(M) * 2 * gridDim.x, gridDim.x, aww);
+}
+
+// reduce [m + 1][2*nparts] partials: h[k] for k < m (block k), and the square root of row m into
+// *norm (block m)
+template <typename T>
+__global__ void reduce_multi_norm_kernel(const T *part, int nparts, int m, double *h, double *norm)
+{
+  const int k = int(blockIdx.x);
+  Acc<T> a;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+    a.merge(part[size_t(k) * 2 * nparts + i], part[size_t(k) * 2 * nparts + nparts + i]);
+  a = block_reduce(a);
+  if (threadIdx.x == 0) {
+    if (k < m)
+      h[k] = double(a.value());
+    else
+      *norm = sqrt(double(a.value()));
+  }
+}
+
+template <typename T, int... Ms>
+static int launch_multiaxpy_dot(int m, const T *V, int64_t ld, const double *h, T *w, int64_t n,
+                                T *part, hipStream_t st, std::integer_sequence<int, Ms...>)
+{
+  using Fn = void (*)(const T *, int64_t, const double *, T *, int64_t, T *);
+  static constexpr Fn table[] = {multiaxpy_dot_kernel<T, Ms + 1>...};
+  if (m < 1 || m > int(sizeof...(Ms))) {
+    set_error("GMRES basis beyond the multi-dot kernels (at most 30 vectors)");
+    return MF_ERR_ARG;
+  }
+  hipLaunchKernelGGL(table[m - 1], dim3(RED_BLOCKS), dim3(VEC_THREADS), 0, st, V, ld, h, w, n,
+                     part);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
 }
 
 // dst = scale * src, scale read on the device (1/s for normalisation)
@@ -983,8 +1046,8 @@ int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part
   if (int rc = launch_multidot<T>(m, V, ld, w, n, part, st,
                                   std::make_integer_sequence<int, GMRES_MAX_BASIS>{}))
     return rc;
-  hipLaunchKernelGGL(reduce_multi_kernel<T>, dim3(1), dim3(VEC_THREADS), 0, st, part, RED_BLOCKS,
-                     m, h);
+  hipLaunchKernelGGL(reduce_multi_kernel<T>, dim3(m), dim3(VEC_THREADS), 0, st, part, RED_BLOCKS,
+                     h);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
@@ -1005,6 +1068,23 @@ int gmres_multiaxpy(const T *V, int64_t ld, int m, const double *h, T *w, int64_
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
+template <typename T>
+int gmres_multiaxpy_dot(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n, T *part,
+                        double *h2, double *norm_out, hipStream_t st)
+{
+  if (int rc = launch_multiaxpy_dot<T>(m, V, ld, h, w, n, part, st,
+                                       std::make_integer_sequence<int, GMRES_MAX_BASIS>{}))
+    return rc;
+  hipLaunchKernelGGL(reduce_multi_norm_kernel<T>, dim3(m + 1), dim3(VEC_THREADS), 0, st, part,
+                     RED_BLOCKS, m, h2, norm_out);
+  MF_HIP_TRY(hipGetLastError());
+  return MF_OK;
+}
+template int gmres_multiaxpy_dot<double>(const double *, int64_t, int, const double *, double *,
+                                         int64_t, double *, double *, double *, hipStream_t);
+template int gmres_multiaxpy_dot<float>(const float *, int64_t, int, const double *, float *,
+                                        int64_t, float *, double *, double *, hipStream_t);
+
 template int gmres_multiaxpy<double>(const double *, int64_t, int, const double *, double *,
                                      int64_t, double *, double *, hipStream_t);
 template int gmres_multiaxpy<float>(const float *, int64_t, int, const double *, float *, int64_t,

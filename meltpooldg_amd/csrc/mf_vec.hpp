@@ -77,6 +77,10 @@ template <typename T>
 int gmres_multidot(const T *V, int64_t ld, int m, const T *w, int64_t n, T *part, double *h, hipStream_t st);
 template <typename T>
 int gmres_multiaxpy(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n, T *part, double *norm_out, hipStream_t st);
+// w -= V h (a Gram-Schmidt pass), then h2 = V^T w (the dots of a second pass) and |w| into
+// *norm_out, in one read of V (part: (m + 1) x 2 RED_BLOCKS partials)
+template <typename T>
+int gmres_multiaxpy_dot(const T *V, int64_t ld, int m, const double *h, T *w, int64_t n, T *part, double *h2, double *norm_out, hipStream_t st);
 template <typename T>
 int gmres_update(T *x, const T *V, int64_t ld, int m, const double *y, const T *inv, int64_t n, hipStream_t st);
 
