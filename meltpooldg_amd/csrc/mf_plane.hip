@@ -29,6 +29,8 @@
 // and the integration uses Dq^T and S^T.
 #include "mf_internal.hpp"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 namespace mf {
@@ -43,6 +45,10 @@ struct PlaneArgs {
   Number mass;        // rho cp / dt
   Number k0, amp, Tc, inv_w, dk0;  // k(T) = k0 + amp tanh((T - Tc) inv_w), dk0 = amp inv_w
   int tdep;           // the dk/dT u grad T . grad v term
+  // saturation test of a wave: every quadrature-point T of the wave lies within
+  // leb * (max - min) / 2 of the midpoint of its nodal T range (leb = (max_q sum_j |S_qj|)^3),
+  // and tanh is +-1 to the last bit and sech^2 is 0 beyond |T - Tc| inv_w >= sat
+  Number leb, sat;
   int ncell[3];
   uint32_t NX, NY;
   const uint8_t *mask;  // Dirichlet rows (null: none)
@@ -337,35 +343,69 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
                                            : Number(1);
 
     // ---------------- T: values and gradients at the quadrature points (P_y) ----------------
-    // Aq = JxW k(T) kc, Bd = JxW dk/dT(T) kc dT/dx_d / h_d^2
+    // Aq = JxW k(T) kc, Bd = JxW dk/dT(T) kc dT/dx_d / h_d^2.  A saturated wave (tanh = +-1 at
+    // every quadrature point of its 16 cells, decided from the nodal T range) needs no T at the
+    // quadrature points: k = k0 +- amp and dk/dT = 0, so the T sweeps and the B terms vanish
+    // exactly; without the dk/dT term (tdep = 0) the T gradients are not formed either.
     Number Aq[4][4];
     [[maybe_unused]] Number Bx[4][4], By[4][4], Bz[4][4];
+    bool with_b = false;
     if constexpr (TLAW) {
-      sweep_a<Number, false>(A.S, t);
-      sweep_b<Number, false>(A.S, t);
-      transpose_planes(t);
-      sweep_b<Number, false>(A.S, t);  // T at the quadrature points
-      copy_planes(t, Bx);
-      sweep_a<Number, false>(A.DqS[0], Bx);
-      copy_planes(t, Bz);
-      sweep_b<Number, false>(A.DqS[2], Bz);
-      copy_planes(t, By);
-      transpose_planes(By);
-      sweep_b<Number, false>(A.DqS[1], By);
-      transpose_planes(By);
+      // nodal T range of the wave (a NaN or infinite T takes the general path, as before)
+      Number tmin = raw.t[0], tmax = raw.t[0], probe = raw.t[0] * Number(0);
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int e = 1; e < 16; ++e) {
+        tmin = fmin(tmin, raw.t[e]);
+        tmax = fmax(tmax, raw.t[e]);
+        probe += raw.t[e] * Number(0);
+      }
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const Number jxw = A.ww[a * 4 + b] * wrR;
-          Number th, s2;
-          plane_tanh_sech2((t[a][b] - A.Tc) * A.inv_w, th, s2);
-          Aq[a][b] = jxw * ((A.k0 + A.amp * th) * kc);
-          const Number dk = A.tdep ? jxw * (A.dk0 * kc) * s2 : Number(0);
-          Bx[a][b] *= dk;
-          By[a][b] *= dk;
-          Bz[a][b] *= dk;
+      for (int off = 32; off > 0; off >>= 1) {
+        tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
+        tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
+      }
+      const Number mid = Number(0.5) * (tmin + tmax), rad = Number(0.5) * (tmax - tmin);
+      const bool saturated = __all(probe == Number(0)) &&
+                             (fabs(mid - A.Tc) - A.leb * rad) * A.inv_w >= A.sat;
+      if (saturated) {
+        const Number th = mid < A.Tc ? Number(-1) : Number(1);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            Aq[a][b] = (A.ww[a * 4 + b] * wrR) * ((A.k0 + A.amp * th) * kc);
+      } else {
+        with_b = A.tdep != 0;
+        sweep_a<Number, false>(A.S, t);
+        sweep_b<Number, false>(A.S, t);
+        transpose_planes(t);
+        sweep_b<Number, false>(A.S, t);  // T at the quadrature points
+        if (with_b) {
+          copy_planes(t, Bx);
+          sweep_a<Number, false>(A.DqS[0], Bx);
+          copy_planes(t, Bz);
+          sweep_b<Number, false>(A.DqS[2], Bz);
+          copy_planes(t, By);
+          transpose_planes(By);
+          sweep_b<Number, false>(A.DqS[1], By);
+          transpose_planes(By);
         }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            const Number jxw = A.ww[a * 4 + b] * wrR;
+            Number th, s2;
+            plane_tanh_sech2((t[a][b] - A.Tc) * A.inv_w, th, s2);
+            Aq[a][b] = jxw * ((A.k0 + A.amp * th) * kc);
+            if (with_b) {
+              const Number dk = jxw * (A.dk0 * kc) * s2;
+              Bx[a][b] *= dk;
+              By[a][b] *= dk;
+              Bz[a][b] *= dk;
+            }
+          }
+      }
     } else {
 #pragma unroll
       for (int a = 0; a < 4; ++a)
@@ -402,7 +442,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if constexpr (TLAW)
+        if (with_b)
           acc[a][b] = fma(Aq[a][b], acc[a][b], Bx[a][b] * u[a][b]);
         else
           acc[a][b] *= Aq[a][b];
@@ -414,7 +454,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if constexpr (TLAW)
+        if (with_b)
           f[a][b] = fma(Aq[a][b], f[a][b], Bz[a][b] * u[a][b]);
         else
           f[a][b] *= Aq[a][b];
@@ -432,7 +472,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if constexpr (TLAW)
+        if (with_b)
           f[a][b] = fma(Aq[a][b], f[a][b], By[a][b] * u[a][b]);
         else
           f[a][b] *= Aq[a][b];
@@ -563,6 +603,17 @@ int launch_heat_plane(const MeshInfo &m, int needT, const CellArgs<Number> &a, c
   p.inv_w = a.inv_tanh_w;
   p.dk0 = Number(double(a.tanh_amp) * double(a.inv_tanh_w));
   p.tdep = a.t_dependent;
+  {
+    double m = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      double r = 0.0;
+      for (int j = 0; j < 4; ++j) r += std::fabs(double(a.S[q * 4 + j]));
+      m = std::max(m, r);
+    }
+    // a small margin over the cut of plane_tanh_sech2 covers the rounding of the sweeps
+    p.leb = Number(m * m * m * (1.0 + 1e-6));
+    p.sat = sizeof(Number) == 8 ? Number(20.0 * (1.0 + 1e-6)) : Number(10.0 * (1.0 + 1e-3));
+  }
   for (int d = 0; d < 3; ++d) p.ncell[d] = m.ncell[d];
   p.NX = uint32_t(m.NN[0]);
   p.NY = uint32_t(m.NN[1]);

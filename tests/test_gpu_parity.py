@@ -187,3 +187,36 @@ def test_supg_rejects_the_transposed_operator():
     s, op = device_operator(case, fields(case))
     with pytest.raises(mf.MFError):
         op.set_transpose(True)
+
+
+@pytest.mark.parametrize("t_dependent", [True, False], ids=["td", "k_only"])
+@pytest.mark.parametrize("mask", ["boundary", "random"])
+def test_tanh_saturated_rows(t_dependent, mask):
+    """T rising along z from far below to far above Tc (300 + 2800 z): the rows of the bottom
+    cell layer have tanh = -1 at every quadrature point, those of the top layer +1, the others
+    lie in the transition.  The structured Q3 kernel evaluates a saturated wave without T at the
+    quadrature points (k = k0 -+ amp, dk/dT = 0); both kinds and the mixed rows must match the
+    oracle: FP64 within 1e-12, FP32 within 1e-5."""
+    case = Case(degree=3, kind="heat", law="tanh", t_dependent=t_dependent, mask=mask,
+                ncell=(12, 4, 12))
+    f = fields(case)
+    m = case.mesh()
+    f["T"] = 300.0 + 2800.0 * orc.node_coords(m)[:, 2] + 5.0 * orc.random_field(m, 3)
+    ref = oracle_operator(case, f)
+    s, op = device_operator(case, f)
+    src = orc.random_field(m, 9)
+    expect = ref.vmult(src)
+    dst = torch.full((s.n_dofs,), float("nan"), dtype=torch.float64, device="cuda")
+    op.vmult(dst, _dev(src))
+    torch.cuda.synchronize()
+    assert rel_err(dst.cpu().numpy(), expect) < TOL64
+    d32 = torch.full((s.n_dofs,), float("nan"), dtype=torch.float32, device="cuda")
+    op.vmult_f32(d32, torch.tensor(src, dtype=torch.float32, device="cuda"))
+    torch.cuda.synchronize()
+    assert rel_err(d32.double().cpu().numpy(), expect) < TOL32
+    # the saturated layers alone: a source supported on the bottom and top cell layers
+    z = orc.node_coords(m)[:, 2]
+    src2 = np.where((z <= 1.0 / 12) | (z >= 11.0 / 12), src, 0.0)
+    op.vmult(dst, _dev(src2))
+    torch.cuda.synchronize()
+    assert rel_err(dst.cpu().numpy(), ref.vmult(src2)) < TOL64
