@@ -349,7 +349,6 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     // exactly; without the dk/dT term (tdep = 0) the T gradients are not formed either.
     Number Aq[4][4];
     [[maybe_unused]] Number Bx[4][4], By[4][4], Bz[4][4];
-    bool with_b = false;
     if constexpr (TLAW) {
       // nodal T range of the wave (a NaN or infinite T takes the general path, as before)
       Number tmin = raw.t[0], tmax = raw.t[0], probe = raw.t[0] * Number(0);
@@ -372,10 +371,12 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-          for (int b = 0; b < 4; ++b)
+          for (int b = 0; b < 4; ++b) {
             Aq[a][b] = (A.ww[a * 4 + b] * wrR) * ((A.k0 + A.amp * th) * kc);
+            Bx[a][b] = By[a][b] = Bz[a][b] = Number(0);
+          }
       } else {
-        with_b = A.tdep != 0;
+        const bool with_b = A.tdep != 0;
         sweep_a<Number, false>(A.S, t);
         sweep_b<Number, false>(A.S, t);
         transpose_planes(t);
@@ -403,6 +404,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
               Bx[a][b] *= dk;
               By[a][b] *= dk;
               Bz[a][b] *= dk;
+            } else {
+              Bx[a][b] = By[a][b] = Bz[a][b] = Number(0);
             }
           }
       }
@@ -442,7 +445,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if (with_b)
+        if constexpr (TLAW)
           acc[a][b] = fma(Aq[a][b], acc[a][b], Bx[a][b] * u[a][b]);
         else
           acc[a][b] *= Aq[a][b];
@@ -454,7 +457,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if (with_b)
+        if constexpr (TLAW)
           f[a][b] = fma(Aq[a][b], f[a][b], Bz[a][b] * u[a][b]);
         else
           f[a][b] *= Aq[a][b];
@@ -472,7 +475,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
-        if (with_b)
+        if constexpr (TLAW)
           f[a][b] = fma(Aq[a][b], f[a][b], By[a][b] * u[a][b]);
         else
           f[a][b] *= Aq[a][b];
