@@ -364,6 +364,11 @@ typedef struct {
   double inner_rel_tol; /* FP32-refined: relative tolerance of each inner FP32 solve (0 = auto:
                            sqrt(rel_tol) clamped to [1e-5, 1e-2], 2 passes for 1e-10) */
   int max_outer;      /* FP32-refined: maximum defect-correction passes (0 = 50) */
+  int force_re_orthogonalization; /* GMRES: second Gram-Schmidt pass for every basis vector
+                           (deal.II SolverGMRES::AdditionalData::force_re_orthogonalization; the
+                           reference keeps the default, 0: a second pass only once a pass has
+                           cancelled the vector below 10 sqrt(eps) of its norm,
+                           linear_solver.hpp:112-114) */
 } mf_solver_desc;
 
 typedef struct {

@@ -68,7 +68,8 @@ class AdvecParams(C.Structure):
 class SolverDesc(C.Structure):
     _fields_ = [("solver", C.c_int), ("restart", C.c_int), ("max_iterations", C.c_int),
                 ("abs_tol", C.c_double), ("rel_tol", C.c_double), ("preconditioner", C.c_int),
-                ("precision", C.c_int), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int)]
+                ("precision", C.c_int), ("inner_rel_tol", C.c_double), ("max_outer", C.c_int),
+                ("force_re_orthogonalization", C.c_int)]
 
 
 class TransientDesc(C.Structure):
@@ -897,8 +898,9 @@ class ReinitializationOperator(_OperatorBase):
 
 def _solver_desc(solver="CG", preconditioner="Diagonal", max_iterations=10000,
                  rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30, precision="fp64",
-                 inner_rel_tolerance=0.0, max_outer=50):
+                 inner_rel_tolerance=0.0, max_outer=50, force_re_orthogonalization=False):
     d = SolverDesc()
+    d.force_re_orthogonalization = int(bool(force_re_orthogonalization))
     d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
     d.restart = restart
     d.max_iterations = max_iterations
@@ -1179,8 +1181,10 @@ class LinearSolver:
     @staticmethod
     def solve(operator, solution, rhs, solver="CG", preconditioner="Diagonal",
               max_iterations=10000, rel_tolerance=1e-12, abs_tolerance=1e-20, restart=30,
-              precision="fp64", inner_rel_tolerance=0.0, max_outer=50, raise_on_failure=True):
+              precision="fp64", inner_rel_tolerance=0.0, max_outer=50, raise_on_failure=True,
+              force_re_orthogonalization=False):
         d = SolverDesc()
+        d.force_re_orthogonalization = int(bool(force_re_orthogonalization))
         d.solver = {"CG": SOLVER_CG, "GMRES": SOLVER_GMRES}[solver]
         d.restart = restart
         d.max_iterations = max_iterations

@@ -1146,7 +1146,8 @@ static int solve_cg(mf_operator *op, T *x, const T *b, const T *inv, int max_it,
 // each cycle (deal.II SolverGMRES with right_preconditioning = true, linear_solver.hpp:112-119).
 template <typename T>
 static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int restart, int max_it,
-                       double abs_tol, double rel_tol, SolveOut &out, bool x_zero)
+                       double abs_tol, double rel_tol, SolveOut &out, bool x_zero,
+                       bool force_reorth)
 {
   const int64_t n = n_rows(op);
   hipStream_t st = op->s->stream;
@@ -1175,7 +1176,7 @@ static int solve_gmres(mf_operator *op, T *x, const T *b, const T *inv, int rest
     return 0;
   };
   bool first = true;
-  bool reorth = false;  // second Gram-Schmidt pass (sticky within the solve)
+  bool reorth = force_reorth;  // second Gram-Schmidt pass (sticky within the solve)
   const double cancel = 10.0 * std::sqrt(double(std::numeric_limits<T>::epsilon()));
   for (;;) {
     T *V0 = V.p;
@@ -1273,7 +1274,7 @@ static int krylov(mf_operator *op, const mf_solver_desc &d, T *x, const T *b, co
   if (d.solver == MF_SOLVER_CG)
     return solve_cg<T>(op, x, b, inv, max_it, abs_tol, rel_tol, out, x_zero);
   return solve_gmres<T>(op, x, b, inv, d.restart > 0 ? d.restart : 30, max_it, abs_tol, rel_tol,
-                        out, x_zero);
+                        out, x_zero, d.force_re_orthogonalization != 0);
 }
 
 }  // namespace mf

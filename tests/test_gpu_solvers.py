@@ -60,6 +60,34 @@ def test_gmres_matches_oracle(case):
     assert rel_err(x.cpu().numpy(), x_ref) < 1e-9
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_gmres_forced_re_orthogonalization(precision):
+    """force_re_orthogonalization runs the second Gram-Schmidt pass for every basis vector (the
+    path that the default takes only after a cancellation): the same iterates up to rounding, so
+    the solution matches the oracle's (two passes always) and the iteration count stays within
+    +-1 of the oracle's and of the default's."""
+    import meltpooldg_amd as mf
+    case = Case(degree=2, kind="advec", ncell=(5, 5, 4), dt=0.05, diffusivity=0.01)
+    f = fields(case)
+    ref = oracle_operator(case, f)
+    s, op = device_operator(case, f)
+    b = _rhs(case, f)
+    x_ref, it_ref, _, st = ref.solve(b, solver=orc.GMRES, rel_tol=1e-10,
+                                     inv_diag=ref.inverse_diagonal())
+    assert st == 0
+    out = {}
+    for force in (False, True):
+        x = torch.zeros(s.n_dofs, dtype=torch.float64, device="cuda")
+        rep = mf.LinearSolver.solve(op, x, _dev(b), solver="GMRES", rel_tolerance=1e-10,
+                                    precision=precision, force_re_orthogonalization=force)
+        assert rep["status"] == 0
+        out[force] = (rep["iterations"], x.cpu().numpy())
+        if precision == "fp64":
+            assert abs(rep["iterations"] - it_ref) <= 1
+        assert rel_err(out[force][1], x_ref) < 1e-9
+    assert abs(out[True][0] - out[False][0]) <= 1
+
+
 @pytest.mark.parametrize("solver", ["CG", "GMRES"])
 def test_fp32_refined_reaches_fp64_tolerance(solver):
     import meltpooldg_amd as mf
