@@ -929,66 +929,57 @@ __device__ __forceinline__ Stencil1D stencil_1d(uint32_t idx, uint32_t last, int
   return s;
 }
 
-// The y-z part of the stencil at node n (row j, layer k), before the x-pass:
-// tangent: m = cv Sy Sz u + c (Dy Sz + Sy Dz) u, s = c Sy Sz u;
-// residual: m = Sy Sz p + (Dy Sz + Sy Dz) q, s = Sy Sz q with p = -cv (T - T^n),
-// q = -k (dt theta T + dt (1 - theta) T^n).  Then row n = Sx . m + Dx . s over (n-1, n, n+1).
+// The y part of the stencil in one layer at node n (row j): with Sy = (l h/6, (l + r) h/3, r h/6)
+// and Dy = (-l, l + r, -r)/h, tangent: ys = Sy u, yd = Dy u (u = src with Dirichlet entries 0);
+// residual: ys = Sy p, yd = Dy q, yq = Sy q with p = -cv (T - T^n), q = -k (dt theta T +
+// dt (1 - theta) T^n), src = T^n.  The centre value u (src[n]) is returned for the fused dot.
+struct LayerY {
+  double ys, yd, yq, uc;
+};
+
 template <int MODE>
-__device__ __forceinline__ void column_yz(const CutArgs &A, const double *src, int64_t n,
-                                          const Stencil1D &sy, const Stencil1D &sz, double c,
-                                          double &m, double &s)
+__device__ __forceinline__ LayerY layer_y(const CutArgs &A, const double *src, int64_t n,
+                                          const Stencil1D &sy)
 {
+  LayerY o{0.0, 0.0, 0.0, 0.0};
   if (MODE == MODE_TANGENT) {
-    double a[3], b[3];
+    double u[3];
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy) {
-      double u[3];
-#pragma unroll
-      for (int dz = 0; dz < 3; ++dz) {
-        const int64_t mm = n + sy.off[dy] + sz.off[dz];
-        u[dz] = (A.dmask && A.dmask[mm]) ? 0.0 : src[mm];
-      }
-      a[dy] = sz.S[0] * u[0] + sz.S[1] * u[1] + sz.S[2] * u[2];
-      b[dy] = sz.D[0] * u[0] + sz.D[1] * u[1] + sz.D[2] * u[2];
+      const int64_t mm = n + sy.off[dy];
+      u[dy] = (A.dmask && A.dmask[mm]) ? 0.0 : src[mm];
     }
-    const double sa = sy.S[0] * a[0] + sy.S[1] * a[1] + sy.S[2] * a[2];
-    const double da = sy.D[0] * a[0] + sy.D[1] * a[1] + sy.D[2] * a[2];
-    const double sb = sy.S[0] * b[0] + sy.S[1] * b[1] + sy.S[2] * b[2];
-    m = A.cv * sa + c * (da + sb);
-    s = c * sa;
-  } else {
-    double ap[3], aq[3], bq[3];
+    o.ys = sy.S[0] * u[0] + sy.S[1] * u[1] + sy.S[2] * u[2];
+    o.yd = sy.D[0] * u[0] + sy.D[1] * u[1] + sy.D[2] * u[2];
+    o.uc = src[n];
+  } else if (MODE == MODE_RESIDUAL) {
+    double pv[3], qv[3];
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy) {
-      double pz[3], qz[3];
-#pragma unroll
-      for (int dz = 0; dz < 3; ++dz) {
-        const int64_t mm = n + sy.off[dy] + sz.off[dz];
-        const double t = A.T[mm], to = src[mm];
-        pz[dz] = -A.cv * (t - to);
-        qz[dz] = -A.k * (A.ost_i * t + A.ost_e * to);
-      }
-      ap[dy] = sz.S[0] * pz[0] + sz.S[1] * pz[1] + sz.S[2] * pz[2];
-      aq[dy] = sz.S[0] * qz[0] + sz.S[1] * qz[1] + sz.S[2] * qz[2];
-      bq[dy] = sz.D[0] * qz[0] + sz.D[1] * qz[1] + sz.D[2] * qz[2];
+      const int64_t mm = n + sy.off[dy];
+      const double t = A.T[mm], to = src[mm];
+      pv[dy] = -A.cv * (t - to);
+      qv[dy] = -A.k * (A.ost_i * t + A.ost_e * to);
     }
-    const double pa = sy.S[0] * ap[0] + sy.S[1] * ap[1] + sy.S[2] * ap[2];
-    const double qa = sy.S[0] * aq[0] + sy.S[1] * aq[1] + sy.S[2] * aq[2];
-    const double qk = sy.D[0] * aq[0] + sy.D[1] * aq[1] + sy.D[2] * aq[2];
-    const double qb = sy.S[0] * bq[0] + sy.S[1] * bq[1] + sy.S[2] * bq[2];
-    m = pa + qk + qb;
-    s = qa;
+    o.ys = sy.S[0] * pv[0] + sy.S[1] * pv[1] + sy.S[2] * pv[2];
+    o.yd = sy.D[0] * qv[0] + sy.D[1] * qv[1] + sy.D[2] * qv[2];
+    o.yq = sy.S[0] * qv[0] + sy.S[1] * qv[1] + sy.S[2] * qv[2];
   }
+  return o;
 }
 
 // MODE_TANGENT: dst = J src (overwrite) or dst += J src, with partials of src . (J src) per
 // block when dot_partial is set; MODE_DIAG: dst = diag J (1 on fixed rows); MODE_RESIDUAL:
-// dst += -R(T) with src = T^n.  A block covers NODE_CHUNK = 254 consecutive nodes (chunks in
-// XCD-major order); thread t reduces the y-z column of node base + t - 1 (9 loads) and stages
-// it in LDS, threads 1..254 then do the x-pass of their node from the staged neighbours (the
-// first and last thread only provide the halo).  At the box boundary a direction has one cell
-// only: its stencils are the one-sided ones; across the end of an x line the weight is 0.
+// dst += -R(T) with src = T^n.  A block covers NODE_CHUNK = 254 consecutive nodes of a layer
+// (chunks in XCD-major order) and NODE_LAYERS layers, marching along z: thread t keeps the y
+// parts (layer_y) of its in-layer node base + t - 1 for the layers z - 1, z, z + 1, combines
+// them with the z stencil, stages the result in LDS, and threads 1..254 do the x-pass of their
+// node from the staged neighbours (the first and last thread only provide the halo).  Each
+// layer is read once per block, plus the two layers before the block's first.  At the box
+// boundary a direction has one cell only: its stencils are the one-sided ones; across the end
+// of an x line the weight is 0.
 constexpr int NODE_CHUNK = 254;
+constexpr int NODE_LAYERS = 16;
 
 template <int MODE>
 __global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *src, double *dst,
@@ -998,63 +989,95 @@ __global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *
   __shared__ double red[256];
   __shared__ double lm[256], ls[256];
   const int64_t NX = A.NX, NXY = A.NXY;
-  const uint32_t nx = uint32_t(NX), nxy = uint32_t(NXY);
+  const uint32_t nx = uint32_t(NX);
   const uint32_t lastx = nx - 1, lasty = uint32_t(A.ncell[1]), lastz = uint32_t(A.ncell[2]);
-  const int64_t n_nodes = NXY * int64_t(A.ncell[2] + 1);
+  const int64_t n_layer_chunks = (NXY + NODE_CHUNK - 1) / NODE_CHUNK;
+  const int64_t n_z_blocks = (int64_t(lastz) + NODE_LAYERS) / NODE_LAYERS;
   const double c = A.ost_i * A.k;
   const double *h6 = A.h6, *h3 = A.h3, *ih = A.ih;
   const int tid = threadIdx.x;
   // consecutive chunks on one XCD (the hardware deals blocks to the 8 XCDs round-robin): the
-  // y and z neighbours a block reads were then fetched into the same L2 by its neighbours
+  // y neighbours a block reads were then fetched into the same L2 by its neighbours
   const int64_t nb8 = gridDim.x / 8;
   const int64_t chunk = int64_t(blockIdx.x % 8) * nb8 + blockIdx.x / 8;
-  const int64_t base = chunk * NODE_CHUNK;
-  if (base >= n_nodes) {
-    if (MODE == MODE_TANGENT && dot_partial && tid == 0) dot_partial[blockIdx.x] = 0.0;
-    return;
-  }
-  const int64_t n = base + tid - 1;
-  const bool in = n >= 0 && n < n_nodes;
-  const bool own = tid >= 1 && tid <= NODE_CHUNK && in;
-  // 32-bit index arithmetic (the node count is below 2^32: checked with the level set)
-  const uint32_t n32 = uint32_t(in ? n : base);
-  const uint32_t k = div_magic(n32, A.mg_nxy), rem = n32 - k * nxy;
-  const uint32_t j = div_magic(rem, A.mg_nx), i = rem - j * nx;
-  const Stencil1D sy = stencil_1d(j, lasty, NX, h6[1], h3[1], ih[1]);
-  const Stencil1D sz = stencil_1d(k, lastz, NXY, h6[2], h3[2], ih[2]);
-  const uint8_t f = in ? A.nflag[n] : NODE_INACTIVE;
-  if (MODE != MODE_DIAG) {
-    double m = 0.0, s = 0.0;
-    if (f != NODE_INACTIVE) column_yz<MODE>(A, src, n, sy, sz, c, m, s);
-    lm[tid] = m;
-    ls[tid] = s;
-    __syncthreads();
-  }
   double dot = 0.0;
-  if (own) {
-    const bool fixed = f == NODE_INACTIVE || (A.dmask && A.dmask[n]);
-    if (fixed) {
-      // a node without a free DoF: 0 (tangent), 1 (diagonal), unchanged (residual, add)
-      if (MODE == MODE_DIAG)
-        dst[n] = 1.0;
-      else if (MODE == MODE_TANGENT && overwrite)
-        dst[n] = 0.0;
-    } else if (f == 0) {
-      const Stencil1D sx = stencil_1d(i, lastx, 1, h6[0], h3[0], ih[0]);
-      double val;
-      if (MODE == MODE_DIAG) {
-        val = A.cv * sx.S[1] * sy.S[1] * sz.S[1] +
-              c * (sx.D[1] * sy.S[1] * sz.S[1] + sx.S[1] * sy.D[1] * sz.S[1] +
-                   sx.S[1] * sy.S[1] * sz.D[1]);
-      } else {
-        val = sx.S[0] * lm[tid - 1] + sx.S[1] * lm[tid] + sx.S[2] * lm[tid + 1] +
-              sx.D[0] * ls[tid - 1] + sx.D[1] * ls[tid] + sx.D[2] * ls[tid + 1];
+  if (chunk < n_layer_chunks * n_z_blocks) {
+    const int64_t xy_chunk = chunk % n_layer_chunks;
+    const uint32_t z0 = uint32_t(chunk / n_layer_chunks) * NODE_LAYERS;
+    const uint32_t z1 = min(z0 + NODE_LAYERS, lastz + 1);
+    const int64_t q = xy_chunk * NODE_CHUNK + tid - 1;  // node in the layer
+    const bool in = q >= 0 && q < NXY;
+    const bool own = tid >= 1 && tid <= NODE_CHUNK && in;
+    // 32-bit index arithmetic (the node count is below 2^32: checked with the level set)
+    const uint32_t q32 = uint32_t(in ? q : xy_chunk * NODE_CHUNK);
+    const uint32_t j = div_magic(q32, A.mg_nx), i = q32 - j * nx;
+    const Stencil1D sy = stencil_1d(j, lasty, NX, h6[1], h3[1], ih[1]);
+    const Stencil1D sx = stencil_1d(i, lastx, 1, h6[0], h3[0], ih[0]);
+    // y parts of the layers z - 1 (lo), z (mid), z + 1 (hi)
+    LayerY lo{0.0, 0.0, 0.0, 0.0}, mid{0.0, 0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0, 0.0};
+    if (MODE != MODE_DIAG) {
+      if (z0 > 0) lo = layer_y<MODE>(A, src, int64_t(q32) + int64_t(z0 - 1) * NXY, sy);
+      mid = layer_y<MODE>(A, src, int64_t(q32) + int64_t(z0) * NXY, sy);
+    }
+    for (uint32_t k = z0; k < z1; ++k) {
+      const int64_t n = int64_t(q32) + int64_t(k) * NXY;
+      const uint8_t f = in ? A.nflag[n] : NODE_INACTIVE;
+      if (MODE != MODE_DIAG) {
+        if (k < lastz) hi = layer_y<MODE>(A, src, n + NXY, sy);
+        const Stencil1D sz = stencil_1d(k, lastz, NXY, h6[2], h3[2], ih[2]);
+        double m = 0.0, s = 0.0;
+        if (f != NODE_INACTIVE) {
+          if (MODE == MODE_TANGENT) {
+            // sa = Sz Sy u, da = Sz Dy u, sb = Dz Sy u
+            const double sa = sz.S[0] * lo.ys + sz.S[1] * mid.ys + sz.S[2] * hi.ys;
+            const double da = sz.S[0] * lo.yd + sz.S[1] * mid.yd + sz.S[2] * hi.yd;
+            const double sb = sz.D[0] * lo.ys + sz.D[1] * mid.ys + sz.D[2] * hi.ys;
+            m = A.cv * sa + c * (da + sb);
+            s = c * sa;
+          } else {
+            const double pa = sz.S[0] * lo.ys + sz.S[1] * mid.ys + sz.S[2] * hi.ys;
+            const double qa = sz.S[0] * lo.yq + sz.S[1] * mid.yq + sz.S[2] * hi.yq;
+            const double qk = sz.S[0] * lo.yd + sz.S[1] * mid.yd + sz.S[2] * hi.yd;
+            const double qb = sz.D[0] * lo.yq + sz.D[1] * mid.yq + sz.D[2] * hi.yq;
+            m = pa + qk + qb;
+            s = qa;
+          }
+        }
+        lm[tid] = m;
+        ls[tid] = s;
+        __syncthreads();
       }
-      if (overwrite)
-        dst[n] = val;
-      else
-        dst[n] += val;
-      if (MODE == MODE_TANGENT && dot_partial) dot = src[n] * val;
+      if (own) {
+        const bool fixed = f == NODE_INACTIVE || (A.dmask && A.dmask[n]);
+        if (fixed) {
+          // a node without a free DoF: 0 (tangent), 1 (diagonal), unchanged (residual, add)
+          if (MODE == MODE_DIAG)
+            dst[n] = 1.0;
+          else if (MODE == MODE_TANGENT && overwrite)
+            dst[n] = 0.0;
+        } else if (f == 0) {
+          double val;
+          if (MODE == MODE_DIAG) {
+            const Stencil1D sz = stencil_1d(k, lastz, NXY, h6[2], h3[2], ih[2]);
+            val = A.cv * sx.S[1] * sy.S[1] * sz.S[1] +
+                  c * (sx.D[1] * sy.S[1] * sz.S[1] + sx.S[1] * sy.D[1] * sz.S[1] +
+                       sx.S[1] * sy.S[1] * sz.D[1]);
+          } else {
+            val = sx.S[0] * lm[tid - 1] + sx.S[1] * lm[tid] + sx.S[2] * lm[tid + 1] +
+                  sx.D[0] * ls[tid - 1] + sx.D[1] * ls[tid] + sx.D[2] * ls[tid + 1];
+          }
+          if (overwrite)
+            dst[n] = val;
+          else
+            dst[n] += val;
+          if (MODE == MODE_TANGENT && dot_partial) dot += mid.uc * val;
+        }
+      }
+      if (MODE != MODE_DIAG) {
+        __syncthreads();  // lm, ls are rewritten by the next layer
+        lo = mid;
+        mid = hi;
+      }
     }
   }
   if (MODE == MODE_TANGENT && dot_partial) {
@@ -1263,10 +1286,14 @@ int general_blocks(const CutData *cd)
   return int(std::max<int64_t>(1, std::min<int64_t>(1024, (cd->n_glist + 31) / 32)));
 }
 
-// blocks of the node kernel: one per NODE_CHUNK nodes, rounded up to a multiple of the 8 XCDs
+// blocks of the node kernel: one per NODE_CHUNK nodes of a layer and NODE_LAYERS layers,
+// rounded up to a multiple of the 8 XCDs
 int node_blocks(const MeshInfo &m)
 {
-  return int(((m.n_dofs + NODE_CHUNK - 1) / NODE_CHUNK + 7) / 8 * 8);
+  const int64_t nxy = m.NN[0] * m.NN[1];
+  const int64_t chunks = (nxy + NODE_CHUNK - 1) / NODE_CHUNK;
+  const int64_t zb = (m.NN[2] + NODE_LAYERS - 1) / NODE_LAYERS;
+  return int((chunks * zb + 7) / 8 * 8);
 }
 
 template <int MODE>
