@@ -47,6 +47,16 @@ struct CutData {
   uint8_t *d_nflag = nullptr;
   int64_t *d_glist = nullptr;     // the nodes with a non-zero flag
   int64_t n_glist = 0;
+  // the non-gas cells with a general node among their corners: their element rows, formed once
+  // per application by cut_gcell_kernel and gathered by the general nodes
+  int32_t *d_gslot = nullptr;     // [n_cells] slot of a general cell, -1 otherwise
+  int64_t *d_gcell = nullptr;     // [n_gcell] the general cells
+  double *d_gout = nullptr;       // [n_gcell][8] work: their element rows of one application
+  int64_t n_gcell = 0;
+  // ghost-penalty faces of each general node (CSR over the general list): entry = index into
+  // faceG of the node's corner value | direction << 28 | plane << 30
+  int32_t *d_gfptr = nullptr;
+  uint32_t *d_gfent = nullptr;
   std::vector<int64_t> faces;     // host copy of the ghost-penalty faces
 };
 
@@ -73,6 +83,10 @@ struct CutArgs {
   const double *mats;      // [n_cut][3][4] the 1D matrices of the intersected cells
   double *faceG;           // [n_cut][6][4] per face of an intersected cell: G_c of the face
   uint64_t mg_nx, mg_nxy;  // ceil(2^64 / d) of the node counts per x line and per xy plane
+  const int32_t *gslot;    // general cells: slot per cell (-1: none)
+  double *gout;            // [n_gcell][8] element rows of the general cells
+  const int32_t *gfptr;    // ghost-penalty faces per general node (CSR)
+  const uint32_t *gfent;
 };
 
 // n / d for n < 2^32 by multiply-high with m = ceil(2^64 / d), 2 <= d < 2^32
@@ -733,39 +747,13 @@ __global__ __launch_bounds__(256) void cut_dot_kernel(const double *a, const dou
 // ------------------------------------------------------------------------------------------
 
 // row a = (ax, ay, az) of Mx(x)My(x)Mz p + (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz) q
-__device__ __forceinline__ double cell_row(const Mat1D (&M)[3], int ax, int ay, int az,
-                                           const double (&p)[2][2][2],
-                                           const double (&q)[2][2][2])
-{
-  const double mz0 = az ? M[2].m01 : M[2].m00, mz1 = az ? M[2].m11 : M[2].m01;
-  const double my0 = ay ? M[1].m01 : M[1].m00, my1 = ay ? M[1].m11 : M[1].m01;
-  const double mx0 = ax ? M[0].m01 : M[0].m00, mx1 = ax ? M[0].m11 : M[0].m01;
-  const double kz = az ? -M[2].kk : M[2].kk, ky = ay ? -M[1].kk : M[1].kk,
-               kx = ax ? -M[0].kk : M[0].kk;
-  double r_m[2], r_q[2], r_s[2];  // per ex: My Mz p, My Mz q, (Ky Mz + My Kz) q
-#pragma unroll
-  for (int ex = 0; ex < 2; ++ex) {
-    double zp[2], zq[2], zk[2];
-#pragma unroll
-    for (int ey = 0; ey < 2; ++ey) {
-      zp[ey] = mz0 * p[0][ey][ex] + mz1 * p[1][ey][ex];
-      zq[ey] = mz0 * q[0][ey][ex] + mz1 * q[1][ey][ex];
-      zk[ey] = kz * (q[0][ey][ex] - q[1][ey][ex]);
-    }
-    r_m[ex] = my0 * zp[0] + my1 * zp[1];
-    r_q[ex] = my0 * zq[0] + my1 * zq[1];
-    r_s[ex] = ky * (zq[0] - zq[1]) + my0 * zk[0] + my1 * zk[1];
-  }
-  return mx0 * (r_m[0] + r_s[0]) + mx1 * (r_m[1] + r_s[1]) + kx * (r_q[0] - r_q[1]);
-}
-
-// The general path of node (i, j, k), the share of `lane` (0..7): the row of the lane's cell
-// (one of the node's 8) and of the ghost-penalty face candidates lane, lane + 8, ... (36 in all:
-// 3 directions x 3 planes x 4 transversal cells).  The 8 shares sum to the node's row.
+// The general path of node (i, j, k), entry idx of the general list, the share of `lane` (0..7):
+// the row of the lane's cell (one of the node's 8) and of the node's ghost-penalty faces lane,
+// lane + 8, ...  The 8 shares sum to the node's row.
 // MODE_TANGENT: (J src)_n; MODE_RESIDUAL: (-R)_n with src = T^n, A.T = T; MODE_DIAG: J_nn.
 template <int MODE>
 __device__ __forceinline__ double node_general(const CutArgs &A, const double *src, int64_t i,
-                                               int64_t j, int64_t k, int lane)
+                                               int64_t j, int64_t k, int64_t idx, int lane)
 {
   const int64_t ncx = A.ncell[0], ncy = A.ncell[1], ncz = A.ncell[2];
   const int64_t NX = A.NX, NXY = A.NXY;
@@ -774,75 +762,84 @@ __device__ __forceinline__ double node_general(const CutArgs &A, const double *s
   {
     const int64_t cx = i - 1 + (lane & 1), cy = j - 1 + ((lane >> 1) & 1), cz = k - 1 + (lane >> 2);
     const bool in = cx >= 0 && cx < ncx && cy >= 0 && cy < ncy && cz >= 0 && cz < ncz;
-    const int32_t cc = in ? A.cell[(cz * ncy + cy) * ncx + cx] : CELL_GAS;
-    if (cc != CELL_GAS) {
-      {
+    const int64_t cell = (cz * ncy + cy) * ncx + cx;
+    const int ax = int(i - cx), ay = int(j - cy), az = int(k - cz);
+    if (MODE == MODE_DIAG) {
+      const int32_t cc = in ? A.cell[cell] : CELL_GAS;
+      if (cc != CELL_GAS) {
         Mat1D M[3];
         cell_mats(A, cc, M);
-        const int ax = int(i - cx), ay = int(j - cy), az = int(k - cz);
-        if (MODE == MODE_DIAG) {
-          const double mx = ax ? M[0].m11 : M[0].m00, my = ay ? M[1].m11 : M[1].m00,
-                       mz = az ? M[2].m11 : M[2].m00;
-          y += A.cv * mx * my * mz + c * (M[0].kk * my * mz + mx * M[1].kk * mz + mx * my * M[2].kk);
-        } else {
-        double p[2][2][2], q[2][2][2];
-        const int64_t n0 = cz * NXY + cy * NX + cx;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
-          const int64_t m = n0 + ex + ey * NX + ez * NXY;
-          if (MODE == MODE_TANGENT) {
-            const double u = fixed_row(A, m) ? 0.0 : src[m];
-            p[ez][ey][ex] = A.cv * u;
-            q[ez][ey][ex] = c * u;
-          } else {
-            const double t = A.T[m], to = src[m];
-            p[ez][ey][ex] = -A.cv * (t - to);
-            q[ez][ey][ex] = -A.k * (A.ost_i * t + A.ost_e * to);
-          }
-        }
-        y += cell_row(M, ax, ay, az, p, q);
-        }
+        const double mx = ax ? M[0].m11 : M[0].m00, my = ay ? M[1].m11 : M[1].m00,
+                     mz = az ? M[2].m11 : M[2].m00;
+        y += A.cv * mx * my * mz + c * (M[0].kk * my * mz + mx * M[1].kk * mz + mx * my * M[2].kk);
       }
+    } else {
+      // the row of this corner, formed once per cell by cut_gcell_kernel (every non-gas cell
+      // of a general node is a general cell; gas cells have no slot)
+      const int32_t slot = in ? A.gslot[cell] : -1;
+      if (slot >= 0) y += A.gout[8 * int64_t(slot) + ax + 2 * ay + 4 * az];
     }
   }
-  // ghost-penalty faces with this node among their 12 nodes
-  const int64_t nidx[3] = {i, j, k}, lim[3] = {ncx, ncy, ncz};
-  const int64_t sn[3] = {1, NX, NXY}, sc[3] = {1, ncx, ncx * ncy};
+  // the ghost-penalty faces with this node among their 12 nodes (listed with the level set):
+  // the lane's share of them
   const double coef[3] = {-1.0, 2.0, -1.0};
-  for (int fc = lane; fc < 36; fc += 8) {
-    const int d = fc / 12, sp = (fc % 12) >> 2, c1 = fc & 1, c2 = (fc >> 1) & 1;
-    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
-    const double ih = A.ih[d], wq = A.wq[d];
-    {
-      const int64_t cm = nidx[d] - sp;  // index along d of the lower cell
-      if (cm < 0 || cm + 1 >= lim[d]) continue;
-      {
-        const int64_t t1 = nidx[d1] - c1;
-        if (t1 < 0 || t1 >= lim[d1]) continue;
-        {
-          const int64_t t2 = nidx[d2] - c2;
-          if (t2 < 0 || t2 >= lim[d2]) continue;
-          const int64_t lower = cm * sc[d] + t1 * sc[d1] + t2 * sc[d2];
-          const int32_t a = A.cell[lower], b = A.cell[lower + sc[d]];
-          if (!((a >= 0 && b != CELL_GAS) || (b >= 0 && a != CELL_GAS))) continue;
-          const int cn = c1 + 2 * c2;  // this node's corner on the face
-          if (MODE == MODE_DIAG) {
-            // sum_q wq L_q(corner)^2 for the bilinear L at the 2 x 2 Gauss points
-            const double g0 = 0.5 - GAUSS_OFF, g1 = 0.5 + GAUSS_OFF;
-            const double s2 = (g0 * g0 + g1 * g1) * (g0 * g0 + g1 * g1);
-            y += coef[sp] * coef[sp] * ih * ih * wq * s2;
-            continue;
-          }
-          // G of this corner, computed once per face by cut_face_g_kernel
-          const double G = a >= 0 ? A.faceG[24 * int64_t(a) + 4 * (2 * d + 1) + cn]
-                                  : A.faceG[24 * int64_t(b) + 4 * (2 * d) + cn];
-          y += (MODE == MODE_RESIDUAL ? -1.0 : 1.0) * coef[sp] * ih * G;
-        }
-      }
+  for (int32_t e = A.gfptr[idx] + lane; e < A.gfptr[idx + 1]; e += 8) {
+    const uint32_t code = A.gfent[e];
+    const int d = int((code >> 28) & 3u), sp = int(code >> 30);
+    const double ih = A.ih[d];
+    if (MODE == MODE_DIAG) {
+      // sum_q wq L_q(corner)^2 for the bilinear L at the 2 x 2 Gauss points
+      const double g0 = 0.5 - GAUSS_OFF, g1 = 0.5 + GAUSS_OFF;
+      const double s2 = (g0 * g0 + g1 * g1) * (g0 * g0 + g1 * g1);
+      y += coef[sp] * coef[sp] * ih * ih * A.wq[d] * s2;
+    } else {
+      // G of this corner, computed once per face by cut_face_g_kernel
+      const double G = A.faceG[code & 0x0fffffffu];
+      y += (MODE == MODE_RESIDUAL ? -1.0 : 1.0) * coef[sp] * ih * G;
     }
   }
   return y;
+}
+
+// Element rows of the general cells, one thread per cell: out[e] (corner e = ex + 2 ey + 4 ez)
+// = (Mx(x)My(x)Mz p + (Kx(x)My(x)Mz + Mx(x)Ky(x)Mz + Mx(x)My(x)Kz) q)_e with the cell's 1D
+// matrices (sub-box of an intersected cell, the full cell otherwise).  MODE_TANGENT: p = cv u,
+// q = dt theta k u (u = src, fixed entries 0); MODE_RESIDUAL: p = -cv (T - T^n),
+// q = -k (dt theta T + dt (1 - theta) T^n) with src = T^n.
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_gcell_kernel(CutArgs A, const int64_t *gcell,
+                                                        int64_t n_gcell, const double *src)
+{
+  if (A.stop && *A.stop) return;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n_gcell) return;
+  const int64_t cell = gcell[g];
+  const int64_t ncx = A.ncell[0], ncy = A.ncell[1];
+  const int64_t cx = cell % ncx, cy = (cell / ncx) % ncy, cz = cell / (ncx * ncy);
+  const double c = A.ost_i * A.k;
+  Mat1D M[3];
+  cell_mats(A, A.cell[cell], M);
+  double p[2][2][2], q[2][2][2];
+  const int64_t n0 = cz * A.NXY + cy * A.NX + cx;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int ex = e & 1, ey = (e >> 1) & 1, ez = e >> 2;
+    const int64_t m = n0 + ex + ey * A.NX + ez * A.NXY;
+    if (MODE == MODE_TANGENT) {
+      const double u = fixed_row(A, m) ? 0.0 : src[m];
+      p[ez][ey][ex] = A.cv * u;
+      q[ez][ey][ex] = c * u;
+    } else {
+      const double t = A.T[m], to = src[m];
+      p[ez][ey][ex] = -A.cv * (t - to);
+      q[ez][ey][ex] = -A.k * (A.ost_i * t + A.ost_e * to);
+    }
+  }
+  double out[2][2][2];
+  tensor_apply(M, p, q, out);
+  double *o = A.gout + 8 * g;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) o[e] = out[e >> 2][(e >> 1) & 1][e & 1];
 }
 
 // G_c = sum_q gamma JxW_q L_q(c) [d_n u](q) of every ghost-penalty face (c = its 4 corners),
@@ -1112,7 +1109,7 @@ __global__ __launch_bounds__(256) void cut_general_kernel(CutArgs A, const int64
       const uint32_t n32 = uint32_t(n), nxy = uint32_t(NXY), nx = uint32_t(NX);
       const uint32_t k32 = div_magic(n32, A.mg_nxy), r32 = n32 - k32 * nxy;
       const uint32_t j32 = div_magic(r32, A.mg_nx);
-      val = node_general<MODE>(A, src, r32 - j32 * nx, j32, k32, lane);
+      val = node_general<MODE>(A, src, r32 - j32 * nx, j32, k32, idx, lane);
       val += __shfl_xor(val, 1);
       val += __shfl_xor(val, 2);
       val += __shfl_xor(val, 4);
@@ -1219,6 +1216,17 @@ void free_tables(CutData *cd)
   if (cd->d_part) (void)hipFree(cd->d_part);
   if (cd->d_nflag) (void)hipFree(cd->d_nflag);
   if (cd->d_glist) (void)hipFree(cd->d_glist);
+  if (cd->d_gslot) (void)hipFree(cd->d_gslot);
+  if (cd->d_gcell) (void)hipFree(cd->d_gcell);
+  if (cd->d_gout) (void)hipFree(cd->d_gout);
+  if (cd->d_gfptr) (void)hipFree(cd->d_gfptr);
+  if (cd->d_gfent) (void)hipFree(cd->d_gfent);
+  cd->d_gfptr = nullptr;
+  cd->d_gfent = nullptr;
+  cd->d_gslot = nullptr;
+  cd->d_gcell = nullptr;
+  cd->d_gout = nullptr;
+  cd->n_gcell = 0;
   cd->d_nflag = nullptr;
   cd->d_glist = nullptr;
   cd->n_glist = 0;
@@ -1267,6 +1275,58 @@ int ensure_node_flags(mf_operator *op)
   std::vector<int64_t> list;
   for (int64_t n = 0; n < m.n_dofs; ++n)
     if (f[size_t(n)] & NODE_GENERAL) list.push_back(n);
+  // the non-gas cells with a general node among their corners, and their slots
+  std::vector<int32_t> gslot(size_t(nx * ny * nz), -1);
+  std::vector<int64_t> gcell;
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t cl = (k * ny + j) * nx + i;
+        if (cd->category[size_t(cl)] == 2) continue;
+        bool g = false;
+        for (int e = 0; e < 8 && !g; ++e)
+          g = (f[size_t((k + (e >> 2)) * NXY + (j + ((e >> 1) & 1)) * NX + i + (e & 1))] &
+               NODE_GENERAL) != 0;
+        if (g) {
+          gslot[size_t(cl)] = int32_t(gcell.size());
+          gcell.push_back(cl);
+        }
+      }
+  // the ghost-penalty faces of each general node: the 36 candidates (3 directions x 3 planes of
+  // the lower cell x 4 transversal positions), kept where the face joins two non-gas cells one
+  // of which is intersected; the corner value sits with the intersected cell (upper face of the
+  // lower cell, slot 2 d + 1, or lower face of the upper cell, slot 2 d)
+  std::vector<int32_t> cutidx(size_t(nx * ny * nz), -1);
+  {
+    int32_t nc = 0;
+    for (size_t cl = 0; cl < cutidx.size(); ++cl)
+      if (cd->category[cl] == 1) cutidx[cl] = nc++;
+  }
+  MF_REQUIRE(24 * cd->n_cut < (int64_t(1) << 28), MF_ERR_ARG,
+             "too many intersected cells for the face tables (24 n_cut >= 2^28)");
+  std::vector<int32_t> gfptr(1, 0);
+  std::vector<uint32_t> gfent;
+  {
+    const int64_t lim[3] = {nx, ny, nz}, scl[3] = {1, nx, nx * ny};
+    for (int64_t n : list) {
+      const int64_t nidx[3] = {n % NX, (n / NX) % NY, n / NXY};
+      for (int fc = 0; fc < 36; ++fc) {
+        const int d = fc / 12, sp = (fc % 12) >> 2, c1 = fc & 1, c2 = (fc >> 1) & 1;
+        const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+        const int64_t cm = nidx[d] - sp, t1 = nidx[d1] - c1, t2 = nidx[d2] - c2;
+        if (cm < 0 || cm + 1 >= lim[d] || t1 < 0 || t1 >= lim[d1] || t2 < 0 || t2 >= lim[d2])
+          continue;
+        const int64_t lower = cm * scl[d] + t1 * scl[d1] + t2 * scl[d2], upper = lower + scl[d];
+        const int8_t ca = cd->category[size_t(lower)], cb = cd->category[size_t(upper)];
+        if (!((ca == 1 && cb != 2) || (cb == 1 && ca != 2))) continue;
+        const int cn = c1 + 2 * c2;
+        const int64_t gi = ca == 1 ? 24 * int64_t(cutidx[size_t(lower)]) + 4 * (2 * d + 1) + cn
+                                   : 24 * int64_t(cutidx[size_t(upper)]) + 4 * (2 * d) + cn;
+        gfent.push_back(uint32_t(gi) | (uint32_t(d) << 28) | (uint32_t(sp) << 30));
+      }
+      gfptr.push_back(int32_t(gfent.size()));
+    }
+  }
   MF_HIP_TRY(hipStreamSynchronize(sc->stream));
   if (!cd->d_nflag) MF_HIP_TRY(hipMalloc(&cd->d_nflag, size_t(m.n_dofs)));
   MF_HIP_TRY(hipMemcpy(cd->d_nflag, f.data(), size_t(m.n_dofs), hipMemcpyHostToDevice));
@@ -1277,6 +1337,30 @@ int ensure_node_flags(mf_operator *op)
     MF_HIP_TRY(hipMemcpy(cd->d_glist, list.data(), sizeof(int64_t) * list.size(),
                          hipMemcpyHostToDevice));
   cd->n_glist = int64_t(list.size());
+  if (!cd->d_gslot) MF_HIP_TRY(hipMalloc(&cd->d_gslot, sizeof(int32_t) * gslot.size()));
+  MF_HIP_TRY(hipMemcpy(cd->d_gslot, gslot.data(), sizeof(int32_t) * gslot.size(),
+                       hipMemcpyHostToDevice));
+  if (cd->d_gcell) MF_HIP_TRY(hipFree(cd->d_gcell));
+  if (cd->d_gout) MF_HIP_TRY(hipFree(cd->d_gout));
+  cd->d_gcell = nullptr;
+  cd->d_gout = nullptr;
+  MF_HIP_TRY(hipMalloc(&cd->d_gcell, sizeof(int64_t) * std::max<size_t>(gcell.size(), 1)));
+  MF_HIP_TRY(hipMalloc(&cd->d_gout, sizeof(double) * 8 * std::max<size_t>(gcell.size(), 1)));
+  if (!gcell.empty())
+    MF_HIP_TRY(hipMemcpy(cd->d_gcell, gcell.data(), sizeof(int64_t) * gcell.size(),
+                         hipMemcpyHostToDevice));
+  cd->n_gcell = int64_t(gcell.size());
+  if (cd->d_gfptr) MF_HIP_TRY(hipFree(cd->d_gfptr));
+  if (cd->d_gfent) MF_HIP_TRY(hipFree(cd->d_gfent));
+  cd->d_gfptr = nullptr;
+  cd->d_gfent = nullptr;
+  MF_HIP_TRY(hipMalloc(&cd->d_gfptr, sizeof(int32_t) * gfptr.size()));
+  MF_HIP_TRY(hipMalloc(&cd->d_gfent, sizeof(uint32_t) * std::max<size_t>(gfent.size(), 1)));
+  MF_HIP_TRY(hipMemcpy(cd->d_gfptr, gfptr.data(), sizeof(int32_t) * gfptr.size(),
+                       hipMemcpyHostToDevice));
+  if (!gfent.empty())
+    MF_HIP_TRY(hipMemcpy(cd->d_gfent, gfent.data(), sizeof(uint32_t) * gfent.size(),
+                         hipMemcpyHostToDevice));
   return MF_OK;
 }
 
@@ -1305,9 +1389,18 @@ int launch_node(mf_operator *op, CutArgs A, const double *src, double *dst, int 
   const CutData *cd = op->cutd;
   A.nflag = cd->d_nflag;
   const int nb = node_blocks(op->s->mesh);
+  A.gslot = cd->d_gslot;
+  A.gout = cd->d_gout;
+  A.gfptr = cd->d_gfptr;
+  A.gfent = cd->d_gfent;
   if (MODE != MODE_DIAG && cd->n_faces > 0) {
     cut_face_g_kernel<MODE><<<dim3(unsigned((cd->n_faces + 255) / 256)), dim3(256), 0,
                               op->s->stream>>>(A, cd->d_face, cd->n_faces, src);
+    MF_HIP_TRY(hipGetLastError());
+  }
+  if (MODE != MODE_DIAG && cd->n_gcell > 0) {
+    cut_gcell_kernel<MODE><<<dim3(unsigned((cd->n_gcell + 255) / 256)), dim3(256), 0,
+                             op->s->stream>>>(A, cd->d_gcell, cd->n_gcell, src);
     MF_HIP_TRY(hipGetLastError());
   }
   cut_node_kernel<MODE><<<dim3(unsigned(nb)), dim3(256), 0, op->s->stream>>>(A, src, dst, overwrite,
