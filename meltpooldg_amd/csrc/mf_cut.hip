@@ -807,11 +807,10 @@ __device__ __forceinline__ double node_general(const CutArgs &A, const double *s
 // q = dt theta k u (u = src, fixed entries 0); MODE_RESIDUAL: p = -cv (T - T^n),
 // q = -k (dt theta T + dt (1 - theta) T^n) with src = T^n.
 template <int MODE>
-__global__ __launch_bounds__(256) void cut_gcell_kernel(CutArgs A, const int64_t *gcell,
-                                                        int64_t n_gcell, const double *src)
+__device__ __forceinline__ void gcell_block(const CutArgs &A, const int64_t *gcell,
+                                            int64_t n_gcell, const double *src, int64_t blk)
 {
-  if (A.stop && *A.stop) return;
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t g = blk * 256 + threadIdx.x;
   if (g >= n_gcell) return;
   const int64_t cell = gcell[g];
   const int64_t ncx = A.ncell[0], ncy = A.ncell[1];
@@ -847,11 +846,10 @@ __global__ __launch_bounds__(256) void cut_gcell_kernel(CutArgs A, const int64_t
 // of the lower cell, slot 2d: the lower face of the upper cell).  MODE_TANGENT: u = src with
 // fixed entries 0; MODE_RESIDUAL: u = T read plain.
 template <int MODE>
-__global__ __launch_bounds__(256) void cut_face_g_kernel(CutArgs A, const int64_t *faces,
-                                                         int64_t n_faces, const double *src)
+__device__ __forceinline__ void face_g_block(const CutArgs &A, const int64_t *faces,
+                                             int64_t n_faces, const double *src, int64_t blk)
 {
-  if (A.stop && *A.stop) return;
-  const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t f = blk * 256 + threadIdx.x;
   if (f >= n_faces) return;
   const int64_t code = faces[f];
   const int d = int(code & 3);
@@ -979,10 +977,10 @@ constexpr int NODE_CHUNK = 254;
 constexpr int NODE_LAYERS = 16;
 
 template <int MODE>
-__global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *src, double *dst,
-                                                       int overwrite, double *dot_partial)
+__device__ __forceinline__ void node_block(const CutArgs &A, const double *src, double *dst,
+                                           int overwrite, double *dot_partial, int64_t blk,
+                                           int64_t nblk)
 {
-  if (A.stop && *A.stop) return;
   __shared__ double red[256];
   __shared__ double lm[256], ls[256];
   const int64_t NX = A.NX, NXY = A.NXY;
@@ -995,8 +993,8 @@ __global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *
   const int tid = threadIdx.x;
   // consecutive chunks on one XCD (the hardware deals blocks to the 8 XCDs round-robin): the
   // y neighbours a block reads were then fetched into the same L2 by its neighbours
-  const int64_t nb8 = gridDim.x / 8;
-  const int64_t chunk = int64_t(blockIdx.x % 8) * nb8 + blockIdx.x / 8;
+  const int64_t nb8 = nblk / 8;
+  const int64_t chunk = (blk % 8) * nb8 + blk / 8;
   double dot = 0.0;
   if (chunk < n_layer_chunks * n_z_blocks) {
     const int64_t xy_chunk = chunk % n_layer_chunks;
@@ -1079,8 +1077,34 @@ __global__ __launch_bounds__(256) void cut_node_kernel(CutArgs A, const double *
   }
   if (MODE == MODE_TANGENT && dot_partial) {
     const double t = block_sum_256(dot, red);
-    if (tid == 0) dot_partial[blockIdx.x] = t;
+    if (tid == 0) dot_partial[blk] = t;
   }
+}
+
+// One application's first launch: the ghost-penalty face values (blocks [0, fb)), the element
+// rows of the general cells ([fb, fb + gb)) and the node stencil (the rest, nb blocks); fb and
+// gb are multiples of 8, so that the node blocks keep their XCD-major chunk order.  The general
+// nodes, which read the first two, follow in a launch of their own.
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_apply_kernel(CutArgs A, const double *src,
+                                                        double *dst, int overwrite,
+                                                        double *dot_partial,
+                                                        const int64_t *faces, int64_t n_faces,
+                                                        int64_t fb, const int64_t *gcell,
+                                                        int64_t n_gcell, int64_t gb, int64_t nb)
+{
+  if (A.stop && *A.stop) return;  // uniform over the grid
+  int64_t blk = blockIdx.x;
+  if (blk < fb) {
+    face_g_block<MODE>(A, faces, n_faces, src, blk);
+    return;
+  }
+  blk -= fb;
+  if (blk < gb) {
+    gcell_block<MODE>(A, gcell, n_gcell, src, blk);
+    return;
+  }
+  node_block<MODE>(A, src, dst, overwrite, dot_partial, blk - gb, nb);
 }
 
 // The general nodes (flag != 0, a list built on the host): fixed rows (no DoF, Dirichlet) get
@@ -1393,18 +1417,13 @@ int launch_node(mf_operator *op, CutArgs A, const double *src, double *dst, int 
   A.gout = cd->d_gout;
   A.gfptr = cd->d_gfptr;
   A.gfent = cd->d_gfent;
-  if (MODE != MODE_DIAG && cd->n_faces > 0) {
-    cut_face_g_kernel<MODE><<<dim3(unsigned((cd->n_faces + 255) / 256)), dim3(256), 0,
-                              op->s->stream>>>(A, cd->d_face, cd->n_faces, src);
-    MF_HIP_TRY(hipGetLastError());
-  }
-  if (MODE != MODE_DIAG && cd->n_gcell > 0) {
-    cut_gcell_kernel<MODE><<<dim3(unsigned((cd->n_gcell + 255) / 256)), dim3(256), 0,
-                             op->s->stream>>>(A, cd->d_gcell, cd->n_gcell, src);
-    MF_HIP_TRY(hipGetLastError());
-  }
-  cut_node_kernel<MODE><<<dim3(unsigned(nb)), dim3(256), 0, op->s->stream>>>(A, src, dst, overwrite,
-                                                                             dot_partial);
+  // blocks of the face values and of the general cells, rounded up to multiples of 8
+  auto blocks8 = [](int64_t n) { return n > 0 ? ((n + 255) / 256 + 7) / 8 * 8 : int64_t(0); };
+  const int64_t fb = MODE != MODE_DIAG ? blocks8(cd->n_faces) : 0;
+  const int64_t gb = MODE != MODE_DIAG ? blocks8(cd->n_gcell) : 0;
+  cut_apply_kernel<MODE><<<dim3(unsigned(fb + gb + nb)), dim3(256), 0, op->s->stream>>>(
+      A, src, dst, overwrite, dot_partial, cd->d_face, cd->n_faces, fb, cd->d_gcell, cd->n_gcell,
+      gb, nb);
   MF_HIP_TRY(hipGetLastError());
   cut_general_kernel<MODE><<<dim3(unsigned(general_blocks(cd))), dim3(256), 0, op->s->stream>>>(
       A, cd->d_glist, cd->n_glist, src, dst, overwrite, dot_partial ? dot_partial + nb : nullptr);
