@@ -48,7 +48,7 @@ struct CutData {
   int64_t *d_glist = nullptr;     // the nodes with a non-zero flag
   int64_t n_glist = 0;
   // the non-gas cells with a general node among their corners: their element rows, formed once
-  // per application by cut_gcell_kernel and gathered by the general nodes
+  // per application by gcell_block and gathered by the general nodes
   int32_t *d_gslot = nullptr;     // [n_cells] slot of a general cell, -1 otherwise
   int64_t *d_gcell = nullptr;     // [n_gcell] the general cells
   double *d_gout = nullptr;       // [n_gcell][8] work: their element rows of one application
@@ -774,7 +774,7 @@ __device__ __forceinline__ double node_general(const CutArgs &A, const double *s
         y += A.cv * mx * my * mz + c * (M[0].kk * my * mz + mx * M[1].kk * mz + mx * my * M[2].kk);
       }
     } else {
-      // the row of this corner, formed once per cell by cut_gcell_kernel (every non-gas cell
+      // the row of this corner, formed once per cell by gcell_block (every non-gas cell
       // of a general node is a general cell; gas cells have no slot)
       const int32_t slot = in ? A.gslot[cell] : -1;
       if (slot >= 0) y += A.gout[8 * int64_t(slot) + ax + 2 * ay + 4 * az];
@@ -793,7 +793,7 @@ __device__ __forceinline__ double node_general(const CutArgs &A, const double *s
       const double s2 = (g0 * g0 + g1 * g1) * (g0 * g0 + g1 * g1);
       y += coef[sp] * coef[sp] * ih * ih * A.wq[d] * s2;
     } else {
-      // G of this corner, computed once per face by cut_face_g_kernel
+      // G of this corner, computed once per face by face_g_block
       const double G = A.faceG[code & 0x0fffffffu];
       y += (MODE == MODE_RESIDUAL ? -1.0 : 1.0) * coef[sp] * ih * G;
     }
