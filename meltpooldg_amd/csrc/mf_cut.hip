@@ -1081,48 +1081,21 @@ __device__ __forceinline__ void node_block(const CutArgs &A, const double *src, 
   }
 }
 
-// One application's first launch: the ghost-penalty face values (blocks [0, fb)), the element
-// rows of the general cells ([fb, fb + gb)) and the node stencil (the rest, nb blocks); fb and
-// gb are multiples of 8, so that the node blocks keep their XCD-major chunk order.  The general
-// nodes, which read the first two, follow in a launch of their own.
-template <int MODE>
-__global__ __launch_bounds__(256) void cut_apply_kernel(CutArgs A, const double *src,
-                                                        double *dst, int overwrite,
-                                                        double *dot_partial,
-                                                        const int64_t *faces, int64_t n_faces,
-                                                        int64_t fb, const int64_t *gcell,
-                                                        int64_t n_gcell, int64_t gb, int64_t nb)
-{
-  if (A.stop && *A.stop) return;  // uniform over the grid
-  int64_t blk = blockIdx.x;
-  if (blk < fb) {
-    face_g_block<MODE>(A, faces, n_faces, src, blk);
-    return;
-  }
-  blk -= fb;
-  if (blk < gb) {
-    gcell_block<MODE>(A, gcell, n_gcell, src, blk);
-    return;
-  }
-  node_block<MODE>(A, src, dst, overwrite, dot_partial, blk - gb, nb);
-}
 
 // The general nodes (flag != 0, a list built on the host): fixed rows (no DoF, Dirichlet) get
 // 0 (tangent), 1 (diagonal) or stay (residual, vmult_add); the others sum their cells' rows
 // and their ghost-penalty faces.  Fused-dot partials follow the node kernel's.
 template <int MODE>
-__global__ __launch_bounds__(256) void cut_general_kernel(CutArgs A, const int64_t *list,
-                                                          int64_t n_list, const double *src,
-                                                          double *dst, int overwrite,
-                                                          double *dot_partial)
+__device__ __forceinline__ void general_block(const CutArgs &A, const int64_t *list,
+                                              int64_t n_list, const double *src, double *dst,
+                                              int overwrite, double *dot_partial, int64_t blk,
+                                              int64_t nblk)
 {
-  if (A.stop && *A.stop) return;
   __shared__ double red[256];
   const int64_t NX = A.NX, NXY = A.NXY;
   const int lane = threadIdx.x & 7;  // 8 consecutive lanes share a node
   double dot = 0.0;
-  for (int64_t idx = (int64_t(blockIdx.x) * 256 + threadIdx.x) >> 3; idx < n_list;
-       idx += int64_t(gridDim.x) * 32) {
+  for (int64_t idx = (blk * 256 + threadIdx.x) >> 3; idx < n_list; idx += nblk * 32) {
     const int64_t n = list[idx];
     double val;
     bool write = true;
@@ -1148,8 +1121,42 @@ __global__ __launch_bounds__(256) void cut_general_kernel(CutArgs A, const int64
   }
   if (MODE == MODE_TANGENT && dot_partial) {
     const double t = block_sum_256(dot, red);
-    if (threadIdx.x == 0) dot_partial[blockIdx.x] = t;
+    if (threadIdx.x == 0) dot_partial[blk] = t;
   }
+}
+
+// One application in two launches.  The first forms the ghost-penalty face values (blocks
+// [0, fb)) and the element rows of the general cells ([fb, fb + gb)); the second runs the
+// general nodes, which read both (blocks [0, genb)), beside the node stencil (the remaining nb
+// blocks).  genb is a multiple of 8, so that the node blocks keep their XCD-major chunk order.
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_pre_kernel(CutArgs A, const double *src,
+                                                      const int64_t *faces, int64_t n_faces,
+                                                      int64_t fb, const int64_t *gcell,
+                                                      int64_t n_gcell)
+{
+  if (A.stop && *A.stop) return;  // uniform over the grid
+  const int64_t blk = blockIdx.x;
+  if (blk < fb)
+    face_g_block<MODE>(A, faces, n_faces, src, blk);
+  else
+    gcell_block<MODE>(A, gcell, n_gcell, src, blk - fb);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void cut_apply_kernel(CutArgs A, const double *src,
+                                                        double *dst, int overwrite,
+                                                        double *dot_partial,
+                                                        const int64_t *list, int64_t n_list,
+                                                        int64_t genb, int64_t nb)
+{
+  if (A.stop && *A.stop) return;  // uniform over the grid
+  const int64_t blk = blockIdx.x;
+  if (blk < genb)
+    general_block<MODE>(A, list, n_list, src, dst, overwrite,
+                        dot_partial ? dot_partial + nb : nullptr, blk, genb);
+  else
+    node_block<MODE>(A, src, dst, overwrite, dot_partial, blk - genb, nb);
 }
 
 CutArgs make_cut_args(const mf_operator *op)
@@ -1388,10 +1395,10 @@ int ensure_node_flags(mf_operator *op)
   return MF_OK;
 }
 
-// grid of the general-node kernel: 32 nodes per block and step, at most 1024 blocks
+// blocks of the general nodes: 32 nodes per block and step, at most 1024 blocks, a multiple of 8
 int general_blocks(const CutData *cd)
 {
-  return int(std::max<int64_t>(1, std::min<int64_t>(1024, (cd->n_glist + 31) / 32)));
+  return int((std::max<int64_t>(1, std::min<int64_t>(1024, (cd->n_glist + 31) / 32)) + 7) / 8 * 8);
 }
 
 // blocks of the node kernel: one per NODE_CHUNK nodes of a layer and NODE_LAYERS layers,
@@ -1421,12 +1428,14 @@ int launch_node(mf_operator *op, CutArgs A, const double *src, double *dst, int 
   auto blocks8 = [](int64_t n) { return n > 0 ? ((n + 255) / 256 + 7) / 8 * 8 : int64_t(0); };
   const int64_t fb = MODE != MODE_DIAG ? blocks8(cd->n_faces) : 0;
   const int64_t gb = MODE != MODE_DIAG ? blocks8(cd->n_gcell) : 0;
-  cut_apply_kernel<MODE><<<dim3(unsigned(fb + gb + nb)), dim3(256), 0, op->s->stream>>>(
-      A, src, dst, overwrite, dot_partial, cd->d_face, cd->n_faces, fb, cd->d_gcell, cd->n_gcell,
-      gb, nb);
-  MF_HIP_TRY(hipGetLastError());
-  cut_general_kernel<MODE><<<dim3(unsigned(general_blocks(cd))), dim3(256), 0, op->s->stream>>>(
-      A, cd->d_glist, cd->n_glist, src, dst, overwrite, dot_partial ? dot_partial + nb : nullptr);
+  if (fb + gb > 0) {
+    cut_pre_kernel<MODE><<<dim3(unsigned(fb + gb)), dim3(256), 0, op->s->stream>>>(
+        A, src, cd->d_face, cd->n_faces, fb, cd->d_gcell, cd->n_gcell);
+    MF_HIP_TRY(hipGetLastError());
+  }
+  const int64_t genb = general_blocks(cd);
+  cut_apply_kernel<MODE><<<dim3(unsigned(genb + nb)), dim3(256), 0, op->s->stream>>>(
+      A, src, dst, overwrite, dot_partial, cd->d_glist, cd->n_glist, genb, nb);
   MF_HIP_TRY(hipGetLastError());
   return MF_OK;
 }
