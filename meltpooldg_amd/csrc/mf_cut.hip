@@ -1319,6 +1319,8 @@ int ensure_node_flags(mf_operator *op)
           g = (f[size_t((k + (e >> 2)) * NXY + (j + ((e >> 1) & 1)) * NX + i + (e & 1))] &
                NODE_GENERAL) != 0;
         if (g) {
+          MF_REQUIRE(gcell.size() < size_t(INT32_MAX), MF_ERR_ARG,
+                     "too many general cells for the 32-bit slot table");
           gslot[size_t(cl)] = int32_t(gcell.size());
           gcell.push_back(cl);
         }
@@ -1355,6 +1357,8 @@ int ensure_node_flags(mf_operator *op)
                                    : 24 * int64_t(cutidx[size_t(upper)]) + 4 * (2 * d) + cn;
         gfent.push_back(uint32_t(gi) | (uint32_t(d) << 28) | (uint32_t(sp) << 30));
       }
+      MF_REQUIRE(gfent.size() < size_t(INT32_MAX), MF_ERR_ARG,
+                 "too many ghost-penalty face entries for the 32-bit node face lists");
       gfptr.push_back(int32_t(gfent.size()));
     }
   }
