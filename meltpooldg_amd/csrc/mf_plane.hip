@@ -350,7 +350,9 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
     Number Aq[4][4];
     [[maybe_unused]] Number Bx[4][4], By[4][4], Bz[4][4];
     if constexpr (TLAW) {
-      // nodal T range of the wave (a NaN or infinite T takes the general path, as before)
+      // nodal T range of the lane's cell (its 4 planes sit in the lanes x + 16 R); a NaN or
+      // infinite T gives an infinite range and the general path.  The wave takes the shortcut
+      // when all its active cells are saturated (each with its own sign).
       Number tmin = raw.t[0], tmax = raw.t[0], probe = raw.t[0] * Number(0);
 #pragma unroll
       for (int e = 1; e < 16; ++e) {
@@ -358,14 +360,18 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
         tmax = fmax(tmax, raw.t[e]);
         probe += raw.t[e] * Number(0);
       }
+      if (!(probe == Number(0))) {
+        tmin = -Number(INFINITY);
+        tmax = Number(INFINITY);
+      }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
+      for (int off = 16; off < 64; off <<= 1) {
         tmin = fmin(tmin, __shfl_xor(tmin, off, 64));
         tmax = fmax(tmax, __shfl_xor(tmax, off, 64));
       }
       const Number mid = Number(0.5) * (tmin + tmax), rad = Number(0.5) * (tmax - tmin);
-      const bool saturated = __all(probe == Number(0)) &&
-                             (fabs(mid - A.Tc) - A.leb * rad) * A.inv_w >= A.sat;
+      const bool cell_saturated = (fabs(mid - A.Tc) - A.leb * rad) * A.inv_w >= A.sat;
+      const bool saturated = __all(cell_saturated || !active);
       if (saturated) {
         const Number th = mid < A.Tc ? Number(-1) : Number(1);
 #pragma unroll
